@@ -1,0 +1,15 @@
+"""cuminlpsearch_amd -- MI355X-native interval branch and bound.
+
+The engine (HIP kernels for gfx950 + C++ search host + C-ABI) is
+``lib/libcmi.so``; this package binds it with ctypes. There is no CPU
+fallback: ``capi.lib()`` raises if the library has not been built.
+"""
+from . import capi
+from .engine import (Candidate, Context, EngineProblem, Role, Tape, qcqp_spec, serve_plain, serve_stream, solve,
+                     solve_batch, solve_options)
+from .model import Expr, Problem, ProblemArrays, evaluate_point, satisfies, sqr
+
+__all__ = [
+    "capi", "Candidate", "Context", "EngineProblem", "Role", "Tape", "qcqp_spec", "serve_plain", "serve_stream",
+    "solve", "solve_batch", "solve_options", "Expr", "Problem", "ProblemArrays", "evaluate_point", "satisfies", "sqr",
+]

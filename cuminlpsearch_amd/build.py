@@ -1,0 +1,99 @@
+"""Build the gfx950 engine library and the CPU oracle, in-tree.
+
+    python -m cuminlpsearch_amd.build            # both
+    python -m cuminlpsearch_amd.build --engine   # libcmi.so only
+
+Outputs: cuminlpsearch_amd/lib/libcmi.so (HIP kernels + C++ host + C-ABI) and
+oracle/liboracle.so (the CPU restatement used by tests and the CPU baseline).
+"""
+from __future__ import annotations
+
+import argparse
+import concurrent.futures as cf
+import os
+import pathlib
+import shutil
+import subprocess
+import sys
+
+ROOT = pathlib.Path(__file__).resolve().parent.parent
+PKG = ROOT / "cuminlpsearch_amd"
+CSRC = PKG / "csrc"
+LIB_DIR = PKG / "lib"
+OBJ_DIR = ROOT / "_build" / "engine"
+ENGINE_SO = LIB_DIR / "libcmi.so"
+ORACLE_DIR = ROOT / "oracle"
+ORACLE_SO = ORACLE_DIR / "liboracle.so"
+
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+ARCH = "gfx950"
+
+# -ffp-contract=off: no a*b+c fusion anywhere (the decode and the point
+# evaluator are specified unfused); packed-fp32-ops disabled for correctness.
+HIP_FLAGS = [
+    f"--offload-arch={ARCH}",
+    "-O3",
+    "-std=c++20",
+    "-fPIC",
+    "-ffp-contract=off",
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    f"-I{CSRC}",
+    f"-I{ROOT / 'include'}",
+]
+
+ENGINE_SOURCES = ["kernels.hip", "hip_backend.cpp", "batch.cpp", "capi.cpp", "server.cpp"]
+
+
+def _run(cmd: list[str]) -> None:
+    proc = subprocess.run(cmd, capture_output=True, text=True)
+    if proc.returncode != 0:
+        sys.stderr.write(proc.stdout)
+        sys.stderr.write(proc.stderr)
+        raise RuntimeError(f"build step failed: {' '.join(cmd)}")
+
+
+def build_engine(jobs: int = 8) -> pathlib.Path:
+    OBJ_DIR.mkdir(parents=True, exist_ok=True)
+    LIB_DIR.mkdir(parents=True, exist_ok=True)
+    objs = []
+    cmds = []
+    for src in ENGINE_SOURCES:
+        path = CSRC / src
+        if not path.exists():
+            continue
+        obj = OBJ_DIR / (src + ".o")
+        lang = [] if src.endswith(".hip") else ["-x", "hip"]
+        cmds.append([HIPCC, *HIP_FLAGS, *lang, "-c", str(path), "-o", str(obj)])
+        objs.append(obj)
+    with cf.ThreadPoolExecutor(max_workers=max(1, jobs)) as ex:
+        list(ex.map(_run, cmds))
+    tmp = ENGINE_SO.with_suffix(".so.tmp")
+    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", str(tmp), *map(str, objs)])
+    os.replace(tmp, ENGINE_SO)
+    return ENGINE_SO
+
+
+def build_oracle() -> pathlib.Path:
+    make = shutil.which("make")
+    if make is None:
+        raise RuntimeError("make not found")
+    _run([make, "-s", "-C", str(ORACLE_DIR), "liboracle.so"])
+    return ORACLE_SO
+
+
+def main(argv: list[str] | None = None) -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--engine", action="store_true")
+    ap.add_argument("--oracle", action="store_true")
+    ap.add_argument("-j", type=int, default=8)
+    a = ap.parse_args(argv)
+    both = not (a.engine or a.oracle)
+    if a.engine or both:
+        print(build_engine(a.j))
+    if a.oracle or both:
+        print(build_oracle())
+    return 0
+
+
+if __name__ == "__main__":
+    raise SystemExit(main())

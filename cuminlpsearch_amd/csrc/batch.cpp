@@ -1,0 +1,348 @@
+#include "batch.hpp"
+
+#include <algorithm>
+#include <chrono>
+#include <cstring>
+#include <limits>
+
+#include "cmi/policy.hpp"
+
+namespace cmi::capi
+{
+std::shared_ptr<cmi::policy::GreedyEnumCompositionPolicy> make_policy(const cmi_solve_options& o);
+}
+
+namespace cmi::batch
+{
+
+namespace
+{
+std::size_t align_up(std::size_t x, std::size_t a) { return (x + a - 1) / a * a; }
+
+struct Candidate
+{
+  double value;
+  std::int64_t index;
+};
+}  // namespace
+
+Runner::Runner(std::shared_ptr<hip::Context> ctx)
+    : ctx_(std::move(ctx))
+{
+}
+
+void Runner::ensure(hip::DeviceBuffer& d, std::size_t bytes)
+{
+  if (d.bytes() < bytes) {
+    d.reset(std::max(bytes, d.bytes() * 2));
+  }
+}
+
+template<typename T>
+void Runner::ensure_host(hip::PinnedBuffer<T>& h, std::size_t n)
+{
+  if (h.size() < n) {
+    h.reset(std::max(n, h.size() * 2));
+  }
+}
+
+void Runner::run(std::span<const Request> reqs)
+{
+  using hip::check;
+  ctx_->activate();
+  hipStream_t const s = ctx_->stream();
+  std::size_t const n = reqs.size();
+  slots_.assign(n, Placement {});
+  if (n == 0) {
+    return;
+  }
+
+  // Order items by launch: samplers, enumerators, bounders.
+  std::vector<std::size_t> order;
+  order.reserve(n);
+  std::size_t counts[3] = {0, 0, 0};
+  for (int pass = 0; pass < 3; ++pass) {
+    dev::Mode const m = pass == 0 ? dev::Mode::Sample : pass == 1 ? dev::Mode::Enumerate : dev::Mode::Bound;
+    for (std::size_t i = 0; i < n; ++i) {
+      if (reqs[i].mode == m) {
+        order.push_back(i);
+        ++counts[pass];
+      }
+    }
+  }
+
+  // Sizes.
+  std::size_t tiles[3] = {0, 0, 0};
+  std::size_t total_regions = 0, n_cand = 0, witness_doubles = 0, data_bytes = 0;
+  std::uint32_t lds[3] = {1, 1, 1};
+  for (std::size_t k = 0; k < n; ++k) {
+    std::size_t const i = order[k];
+    Request const& r = reqs[i];
+    int const pass = k < counts[0] ? 0 : k < counts[0] + counts[1] ? 1 : 2;
+    std::size_t const slots = r.assignment->size();
+    std::size_t const n_regions = region::composition_fan_out(r.assignment->widths);
+    std::size_t const samples = r.mode == dev::Mode::Sample ? r.samples : 1;
+    if (n_regions == 0 || samples == 0 || n_regions > (std::size_t(1) << 31) / samples) {
+      throw InvalidConfiguration("batched request: n_regions * samples must be in [1, 2^31)");
+    }
+    if (slots > dev::kMaxSlotsPerItem) {
+      throw InvalidConfiguration("batched request: too many slots");
+    }
+    if (r.box.size() != r.tape->n_vars()) {
+      throw ShapeMismatch("batched request: box size does not match the problem's variable count");
+    }
+    std::size_t const n_elems = n_regions * samples;
+    Placement& p = slots_[i];
+    p.item = k;
+    p.n_regions = n_regions;
+    p.n_elems = n_elems;
+    p.n_vars = r.tape->n_vars();
+    tiles[pass] += (n_elems + dev::kTile - 1) / dev::kTile;
+    lds[pass] = std::max(lds[pass], r.tape->n_slots());
+    if (r.mode == dev::Mode::Bound) {
+      p.region_off = total_regions;
+      total_regions += n_regions;
+    } else {
+      p.cand_off = n_cand++;
+      p.witness_off = witness_doubles;
+      witness_doubles += p.n_vars;
+    }
+    data_bytes += align_up(p.n_vars * 16, 16) + align_up(slots * 4, 16) * 2 + align_up(slots * 8, 16)
+        + align_up(slots, 16);
+  }
+  if (lds[2] > ctx_->lds_interval_slots() || std::max(lds[0], lds[1]) > ctx_->lds_point_slots()) {
+    throw ResourceExhausted("batched request needs more LDS slots than a workgroup holds");
+  }
+  std::size_t const total_tiles = tiles[0] + tiles[1] + tiles[2];
+  std::size_t const cand_tiles = tiles[0] + tiles[1];
+
+  std::size_t const off_items = 0;
+  std::size_t const off_tile_item = align_up(n * sizeof(dev::Item), 256);
+  std::size_t const off_data = align_up(off_tile_item + total_tiles * 4, 256);
+  std::size_t const staging_bytes = off_data + data_bytes;
+
+  off_obj_lb_ = align_up(total_regions, 16);
+  off_cand_ = off_obj_lb_ + total_regions * 8;
+  off_witness_ = off_cand_ + n_cand * sizeof(Candidate);
+  results_bytes_ = std::max<std::size_t>(off_witness_ + witness_doubles * 8, 16);
+  std::size_t const scratch_bytes = std::max<std::size_t>(cand_tiles * 16, 16);
+
+  ensure(staging_d_, staging_bytes);
+  ensure(results_d_, results_bytes_);
+  ensure(scratch_d_, scratch_bytes);
+  if (done_capacity_ < n_cand) {
+    std::size_t const cap = std::max<std::size_t>(n_cand, done_capacity_ * 2);
+    done_d_.reset(cap * 4);
+    check(hipMemsetAsync(done_d_.get(), 0, cap * 4, s), "hipMemsetAsync(done)");
+    done_capacity_ = cap;
+  }
+  ensure_host(staging_h_, staging_bytes);
+  ensure_host(results_h_, results_bytes_);
+
+  unsigned char* const h = staging_h_.data();
+  auto* const d = staging_d_.as<unsigned char>();
+  auto* const items = reinterpret_cast<dev::Item*>(h + off_items);
+  auto* const tile_item = reinterpret_cast<std::uint32_t*>(h + off_tile_item);
+  auto* const res = results_d_.as<unsigned char>();
+  auto* const scratch = scratch_d_.as<unsigned char>();
+
+  std::size_t data_off = off_data;
+  std::size_t tile_cursor = 0;       // global tile index
+  std::size_t cand_tile_cursor = 0;  // tile index inside the candidate scratch
+  std::size_t launch_first_tile = 0;
+  std::size_t launch_first_item = 0;
+  int current_pass = -1;
+  for (std::size_t k = 0; k < n; ++k) {
+    std::size_t const i = order[k];
+    Request const& r = reqs[i];
+    Placement const& p = slots_[i];
+    int const pass = k < counts[0] ? 0 : k < counts[0] + counts[1] ? 1 : 2;
+    if (pass != current_pass) {
+      current_pass = pass;
+      launch_first_tile = tile_cursor;
+      launch_first_item = k;
+    }
+    std::size_t const slots = r.assignment->size();
+    std::size_t const off_parent = data_off;
+    std::size_t const off_var = off_parent + align_up(p.n_vars * 16, 16);
+    std::size_t const off_width = off_var + align_up(slots * 4, 16);
+    std::size_t const off_prefix = off_width + align_up(slots * 4, 16);
+    std::size_t const off_kind = off_prefix + align_up(slots * 8, 16);
+    data_off = off_kind + align_up(slots, 16);
+
+    std::memcpy(h + off_parent, r.box.data(), p.n_vars * 16);
+    auto* var = reinterpret_cast<std::uint32_t*>(h + off_var);
+    auto* width = reinterpret_cast<std::uint32_t*>(h + off_width);
+    auto* prefix = reinterpret_cast<std::uint64_t*>(h + off_prefix);
+    auto* kind = reinterpret_cast<std::uint8_t*>(h + off_kind);
+    std::uint64_t running = 1;
+    for (std::size_t j = 0; j < slots; ++j) {
+      if (r.assignment->var_ids[j] >= p.n_vars) {
+        throw ShapeMismatch("batched request: slot var_id out of range");
+      }
+      var[j] = static_cast<std::uint32_t>(r.assignment->var_ids[j]);
+      width[j] = static_cast<std::uint32_t>(r.assignment->widths[j]);
+      prefix[j] = running;
+      running *= r.assignment->widths[j];
+      kind[j] = static_cast<std::uint8_t>(r.assignment->composition[j]);
+    }
+
+    std::uint32_t const n_tiles = static_cast<std::uint32_t>((p.n_elems + dev::kTile - 1) / dev::kTile);
+    dev::Item& it = items[k];
+    std::memset(&it, 0, sizeof(it));
+    it.code = r.tape->code();
+    it.n_instr = r.tape->n_instr();
+    it.n_vars = static_cast<std::uint32_t>(p.n_vars);
+    it.parent = reinterpret_cast<const double*>(d + off_parent);
+    it.var_kinds = r.tape->var_kinds();
+    it.slot_var = reinterpret_cast<const std::uint32_t*>(d + off_var);
+    it.slot_width = reinterpret_cast<const std::uint32_t*>(d + off_width);
+    it.slot_prefix = reinterpret_cast<const std::uint64_t*>(d + off_prefix);
+    it.slot_kind = reinterpret_cast<const std::uint8_t*>(d + off_kind);
+    it.slot_count = static_cast<std::uint32_t>(slots);
+    it.samples = static_cast<std::uint32_t>(r.mode == dev::Mode::Sample ? r.samples : 1);
+    it.n_elems = p.n_elems;
+    it.salt = r.salt;
+    it.tile_begin = static_cast<std::uint32_t>(tile_cursor - launch_first_tile);
+    it.n_tiles = n_tiles;
+    if (r.mode == dev::Mode::Bound) {
+      it.feasible = res + p.region_off;
+      it.obj_lb = reinterpret_cast<double*>(res + off_obj_lb_) + p.region_off;
+    } else {
+      it.tile_min = reinterpret_cast<double*>(scratch) + cand_tile_cursor;
+      it.tile_arg = reinterpret_cast<std::int64_t*>(scratch + cand_tiles * 8) + cand_tile_cursor;
+      it.done = done_d_.as<std::uint32_t>() + p.cand_off;
+      it.cand_value = reinterpret_cast<double*>(res + off_cand_ + p.cand_off * sizeof(Candidate));
+      it.cand_index = reinterpret_cast<std::int64_t*>(res + off_cand_ + p.cand_off * sizeof(Candidate) + 8);
+      it.witness = reinterpret_cast<double*>(res + off_witness_) + p.witness_off;
+      cand_tile_cursor += n_tiles;
+    }
+    for (std::uint32_t t = 0; t < n_tiles; ++t) {
+      tile_item[tile_cursor + t] = static_cast<std::uint32_t>(k - launch_first_item);
+    }
+    tile_cursor += n_tiles;
+  }
+
+  check(hipMemcpyAsync(d, h, staging_bytes, hipMemcpyHostToDevice, s), "hipMemcpyAsync(batch staging)");
+  std::size_t first_item = 0, first_tile = 0;
+  for (int pass = 0; pass < 3; ++pass) {
+    dev::Mode const m = pass == 0 ? dev::Mode::Sample : pass == 1 ? dev::Mode::Enumerate : dev::Mode::Bound;
+    if (counts[pass] > 0) {
+      dev::launch_items(m, reinterpret_cast<const dev::Item*>(d + off_items) + first_item,
+                        reinterpret_cast<const std::uint32_t*>(d + off_tile_item) + first_tile,
+                        static_cast<std::uint32_t>(tiles[pass]), lds[pass], s);
+      check(hipGetLastError(), "launch(k_items batch)");
+      ++launches_;
+    }
+    first_item += counts[pass];
+    first_tile += tiles[pass];
+  }
+  check(hipMemcpyAsync(results_h_.data(), results_d_.get(), results_bytes_, hipMemcpyDeviceToHost, s),
+        "hipMemcpyAsync(batch results)");
+  check(hipStreamSynchronize(s), "hipStreamSynchronize");
+}
+
+std::span<const unsigned char> Runner::feasible(std::size_t i) const
+{
+  Placement const& p = slots_[i];
+  return {results_h_.data() + p.region_off, p.n_regions};
+}
+
+std::span<const double> Runner::obj_lb(std::size_t i) const
+{
+  Placement const& p = slots_[i];
+  return {reinterpret_cast<const double*>(results_h_.data() + off_obj_lb_) + p.region_off, p.n_regions};
+}
+
+backend::CandidateResult Runner::candidate(std::size_t i) const
+{
+  Placement const& p = slots_[i];
+  Candidate c;
+  std::memcpy(&c, results_h_.data() + off_cand_ + p.cand_off * sizeof(Candidate), sizeof(c));
+  backend::CandidateResult out;
+  out.found = c.value < std::numeric_limits<double>::max();
+  out.value = c.value;
+  out.index = c.index;
+  out.point = {reinterpret_cast<const double*>(results_h_.data() + off_witness_) + p.witness_off, p.n_vars};
+  return out;
+}
+
+std::vector<Result> solve_batch(std::shared_ptr<hip::Context> ctx,
+                                std::span<const model::Problem<double>* const> problems,
+                                const cmi_solve_options& o)
+{
+  auto const t0 = std::chrono::steady_clock::now();
+  std::size_t const n = problems.size();
+  auto pol = capi::make_policy(o);
+  search::SolveOptions so;
+  so.iter_limit = o.iter_limit;
+  so.tolerance = o.tolerance;
+  so.sample_points = o.sample_points;
+  so.host_budget_bytes = o.host_budget_bytes;
+  so.frontier_policy = o.compact_frontier ? search::FrontierPolicy::Compact : search::FrontierPolicy::StopAtBudget;
+  if (o.has_known_primal) {
+    so.known_primal_bound = o.known_primal;
+  }
+  std::vector<std::unique_ptr<hip::DeviceTape>> tapes;
+  std::vector<std::unique_ptr<search::SolveState>> states;
+  tapes.reserve(n);
+  states.reserve(n);
+  for (std::size_t i = 0; i < n; ++i) {
+    tapes.push_back(std::make_unique<hip::DeviceTape>(ctx, *problems[i]));
+    states.push_back(std::make_unique<search::SolveState>(*problems[i], pol, so));
+  }
+  std::vector<Result> results(n);
+  std::vector<bool> done(n, false);
+  Runner runner(ctx);
+  std::vector<Request> reqs;
+  struct Owner
+  {
+    std::size_t solve;
+    std::size_t first_req;
+    bool enumerate;
+  };
+  std::vector<Owner> owners;
+  std::size_t remaining = n;
+  while (remaining > 0) {
+    reqs.clear();
+    owners.clear();
+    for (std::size_t i = 0; i < n; ++i) {
+      if (done[i]) {
+        continue;
+      }
+      const search::SolveState::Request* q = states[i]->next();
+      if (q == nullptr) {
+        results[i].outcome = states[i]->finish();
+        results[i].seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        done[i] = true;
+        --remaining;
+        continue;
+      }
+      owners.push_back({i, reqs.size(), q->kind == search::SolveState::Kind::Enumerate});
+      if (q->kind == search::SolveState::Kind::Enumerate) {
+        reqs.push_back({tapes[i].get(), dev::Mode::Enumerate, q->box, q->assignment, q->salt, 1});
+      } else {
+        reqs.push_back({tapes[i].get(), dev::Mode::Sample, q->box, q->assignment, q->salt,
+                        static_cast<std::uint32_t>(o.sample_points)});
+        reqs.push_back({tapes[i].get(), dev::Mode::Bound, q->box, q->assignment, q->salt, 1});
+      }
+    }
+    if (reqs.empty()) {
+      continue;
+    }
+    runner.run(reqs);
+    for (Owner const& w : owners) {
+      search::SolveState& st = *states[w.solve];
+      if (w.enumerate) {
+        st.on_enumerate(runner.candidate(w.first_req), runner.n_elems(w.first_req));
+      } else {
+        backend::BoundResult const b {runner.feasible(w.first_req + 1), runner.obj_lb(w.first_req + 1),
+                                      runner.n_elems(w.first_req + 1)};
+        st.on_subdivide(runner.candidate(w.first_req), runner.n_elems(w.first_req), b);
+      }
+    }
+  }
+  return results;
+}
+
+}  // namespace cmi::batch

@@ -1,0 +1,689 @@
+// C-ABI over the C++ engine: status codes instead of exceptions, opaque
+// handles instead of objects. Declarations and the reference interfaces they
+// replace are in include/cmi.h.
+
+#include "cmi.h"
+
+#include <chrono>
+#include <cstring>
+#include <memory>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "batch.hpp"
+#include "cmi/model.hpp"
+#include "cmi/policy.hpp"
+#include "cmi/qcqp.hpp"
+#include "cmi/search.hpp"
+#include "handles.hpp"
+#include "hip_backend.hpp"
+
+namespace
+{
+
+thread_local std::string g_last_error;
+
+cmi_status fail(cmi_status s, const char* what)
+{
+  g_last_error = what;
+  return s;
+}
+
+template<class F>
+cmi_status guarded(F&& f)
+{
+  try {
+    f();
+    g_last_error.clear();
+    return CMI_OK;
+  } catch (const cmi::backend::OverBudgetError& e) {
+    return fail(CMI_OVER_BUDGET, e.what());
+  } catch (const cmi::ResourceExhausted& e) {
+    return fail(CMI_OVER_BUDGET, e.what());
+  } catch (const cmi::ShapeMismatch& e) {
+    return fail(CMI_SHAPE_MISMATCH, e.what());
+  } catch (const cmi::InvalidConfiguration& e) {
+    return fail(CMI_INVALID_CONFIG, e.what());
+  } catch (const cmi::InvalidDAG& e) {
+    return fail(CMI_INVALID_DAG, e.what());
+  } catch (const cmi::InvalidProblem& e) {
+    return fail(CMI_INVALID_PROBLEM, e.what());
+  } catch (const cmi::DeviceError& e) {
+    return fail(CMI_DEVICE_ERROR, e.what());
+  } catch (const std::bad_alloc&) {
+    return fail(CMI_ERROR, "host allocation failed");
+  } catch (const std::exception& e) {
+    return fail(CMI_ERROR, e.what());
+  } catch (...) {
+    return fail(CMI_ERROR, "unknown error");
+  }
+}
+
+cmi::model::Problem<double> problem_from_desc(const cmi_problem_desc& d)
+{
+  using namespace cmi::model;
+  Problem<double> p;
+  if (d.n_nodes > 0 && d.op == nullptr) {
+    throw cmi::InvalidDAG("problem description has nodes but no op array");
+  }
+  p.graph.nodes.reserve(d.n_nodes);
+  for (std::size_t i = 0; i < d.n_nodes; ++i) {
+    if (d.op[i] > static_cast<std::uint8_t>(Op::Max)) {
+      throw cmi::InvalidDAG("unknown op code in problem description");
+    }
+    Op const op = static_cast<Op>(d.op[i]);
+    DAGNode<double> node {};
+    node.id = i;
+    node.op = op;
+    std::size_t const arity = op_arity(op);
+    if (arity >= 1) {
+      if (d.in0 == nullptr) {
+        throw cmi::InvalidDAG("problem description lacks in0 for a node with operands");
+      }
+      node.in.push_back(d.in0[i]);
+    }
+    if (arity >= 2) {
+      if (d.in1 == nullptr) {
+        throw cmi::InvalidDAG("problem description lacks in1 for a binary node");
+      }
+      node.in.push_back(d.in1[i]);
+    }
+    for (std::size_t in : node.in) {
+      if (in >= i) {
+        throw cmi::InvalidDAG(
+            "DAG node operand id is not strictly less than the node's own id; topological-order invariant "
+            "violated");
+      }
+    }
+    node.op_count = 1;
+    for (std::size_t in : node.in) {
+      node.op_count += p.graph.nodes[in].op_count;
+    }
+    if (op == Op::Const) {
+      if (d.constant == nullptr) {
+        throw cmi::InvalidDAG("problem description lacks constant payloads");
+      }
+      node.payload.constant = d.constant[i];
+    } else if (op == Op::Var) {
+      if (d.var_index == nullptr) {
+        throw cmi::InvalidDAG("problem description lacks var_index payloads");
+      }
+      node.payload.var_index = d.var_index[i];
+    } else if (op == Op::PowN) {
+      if (d.int_exp == nullptr) {
+        throw cmi::InvalidDAG("problem description lacks int_exp payloads");
+      }
+      node.payload.int_exp = d.int_exp[i];
+    } else {
+      node.payload.constant = 0.0;
+    }
+    p.graph.nodes.push_back(std::move(node));
+  }
+  if (d.n_vars > 0 && (d.box == nullptr || d.var_kind == nullptr)) {
+    throw cmi::InvalidProblem("problem description lacks the box or the variable kinds");
+  }
+  for (std::size_t v = 0; v < d.n_vars; ++v) {
+    p.box_bounds.push_back({d.box[v].lb, d.box[v].ub});
+    if (d.var_kind[v] > static_cast<std::uint8_t>(VarKind::Binary)) {
+      throw cmi::InvalidProblem("unknown variable kind");
+    }
+    p.var_kinds.push_back(static_cast<VarKind>(d.var_kind[v]));
+  }
+  p.objective_root = d.objective_root;
+  if (d.n_constraints > 0 && (d.con_root == nullptr || d.con_cmp == nullptr || d.con_rhs == nullptr)) {
+    throw cmi::InvalidProblem("problem description lacks constraint arrays");
+  }
+  for (std::size_t k = 0; k < d.n_constraints; ++k) {
+    if (d.con_cmp[k] > static_cast<std::uint8_t>(Cmp::EQ)) {
+      throw cmi::InvalidProblem("unknown constraint comparison");
+    }
+    p.constraints.push_back({d.con_root[k], static_cast<Cmp>(d.con_cmp[k]), d.con_rhs[k]});
+  }
+  p.validate();
+  return p;
+}
+
+cmi::region::Composition composition_of(const std::uint8_t* kinds, std::size_t n)
+{
+  cmi::region::Composition c;
+  for (std::size_t j = 0; j < n; ++j) {
+    if (kinds[j] > static_cast<std::uint8_t>(cmi::region::SlotKind::BinaryEnumerate)) {
+      throw cmi::InvalidConfiguration("unknown slot kind");
+    }
+    c.kinds.push_back(static_cast<cmi::region::SlotKind>(kinds[j]));
+  }
+  return c;
+}
+
+void fill_outcome(const cmi::search::SolveOutcome& o, double seconds, cmi_solve_outcome* out, double* best,
+                  std::size_t n_vars)
+{
+  out->lower_bound = o.lower_bound;
+  out->upper_bound = o.upper_bound;
+  out->stop_reason = static_cast<int>(o.stop_reason);
+  out->proven_optimal = o.proven_optimal ? 1 : 0;
+  out->infeasible = o.infeasible ? 1 : 0;
+  out->found_incumbent = o.found_incumbent() ? 1 : 0;
+  out->iterations = o.counters.iterations;
+  out->fathomed = o.counters.fathomed;
+  out->bounded = o.counters.bounded;
+  out->sampled = o.counters.sampled;
+  out->enqueued = o.counters.enqueued;
+  out->pruned_infeasible = o.counters.pruned_infeasible;
+  out->pruned_dominated = o.counters.pruned_dominated;
+  out->pending = o.counters.pending;
+  out->wall_seconds = seconds;
+  if (best != nullptr) {
+    for (std::size_t v = 0; v < n_vars; ++v) {
+      best[v] = v < o.best_point.size() ? o.best_point[v] : __builtin_nan("");
+    }
+  }
+}
+
+}  // namespace
+
+namespace cmi::capi
+{
+
+std::shared_ptr<cmi::policy::GreedyEnumCompositionPolicy> make_policy(const cmi_solve_options& o)
+{
+  cmi::policy::SearchCalibration cal;
+  cal.max_cycle_size = o.max_cycle_size;
+  return std::make_shared<cmi::policy::GreedyEnumCompositionPolicy>(
+      cmi::region::FanOutSpec(o.partition_num, o.enumerate_cap), cal);
+}
+
+cmi_status run_guarded(void (*fn)(void*), void* arg)
+{
+  return guarded([&] { fn(arg); });
+}
+
+void fill_outcome_public(const cmi::search::SolveOutcome& o, double seconds, cmi_solve_outcome* out, double* best,
+                         std::size_t n_vars)
+{
+  fill_outcome(o, seconds, out, best, n_vars);
+}
+
+}  // namespace cmi::capi
+
+extern "C" {
+
+const char* cmi_last_error(void) { return g_last_error.c_str(); }
+
+const char* cmi_version(void) { return "cuminlpsearch_amd 0.1 (gfx950)"; }
+
+cmi_status cmi_problem_create(const cmi_problem_desc* desc, cmi_problem** out)
+{
+  if (desc == nullptr || out == nullptr) {
+    return fail(CMI_INVALID_HANDLE, "null argument");
+  }
+  return guarded([&] { *out = new cmi_problem {problem_from_desc(*desc)}; });
+}
+
+cmi_status cmi_problem_qcqp(const cmi_qcqp_spec* spec, uint64_t seed, cmi_problem** out)
+{
+  if (spec == nullptr || out == nullptr) {
+    return fail(CMI_INVALID_HANDLE, "null argument");
+  }
+  return guarded([&] {
+    cmi::qcqp::Spec s;
+    s.n = spec->n;
+    s.m = spec->m;
+    s.n_int = spec->n_int;
+    s.cont_lb = spec->cont_lb;
+    s.cont_ub = spec->cont_ub;
+    s.int_lb = spec->int_lb;
+    s.int_ub = spec->int_ub;
+    s.pair_density = spec->pair_density;
+    s.term_prob = spec->term_prob;
+    s.lin_prob = spec->lin_prob;
+    s.symmetric_pairs = spec->symmetric_pairs != 0;
+    s.rhs_lo = spec->rhs_lo;
+    s.rhs_hi = spec->rhs_hi;
+    if (s.n == 0 || s.n_int > s.n) {
+      throw cmi::InvalidConfiguration("qcqp spec needs n >= 1 and n_int <= n");
+    }
+    if (spec->target_nodes > 0) {
+      s = cmi::qcqp::tune_term_prob(s, seed, spec->target_nodes);
+    }
+    auto p = cmi::qcqp::generate(s, seed);
+    p.validate();
+    *out = new cmi_problem {std::move(p)};
+  });
+}
+
+cmi_status cmi_problem_info_get(const cmi_problem* p, cmi_problem_info* out)
+{
+  if (p == nullptr || out == nullptr) {
+    return fail(CMI_INVALID_HANDLE, "null argument");
+  }
+  return guarded([&] {
+    out->n_nodes = p->problem.graph.nodes.size();
+    out->n_vars = p->problem.box_bounds.size();
+    out->n_constraints = p->problem.constraints.size();
+    out->reachable_nodes = cmi::qcqp::reachable_nodes(p->problem);
+  });
+}
+
+cmi_status cmi_problem_export(const cmi_problem* p, uint8_t* op, uint64_t* in0, uint64_t* in1, double* constant,
+                              uint64_t* var_index, int32_t* int_exp, cmi_ivl* box, uint8_t* var_kind,
+                              uint64_t* objective_root, uint64_t* con_root, uint8_t* con_cmp, double* con_rhs)
+{
+  if (p == nullptr) {
+    return fail(CMI_INVALID_HANDLE, "null problem");
+  }
+  return guarded([&] {
+    auto const& pr = p->problem;
+    for (std::size_t i = 0; i < pr.graph.nodes.size(); ++i) {
+      auto const& nd = pr.graph.nodes[i];
+      if (op != nullptr) {
+        op[i] = static_cast<uint8_t>(nd.op);
+      }
+      if (in0 != nullptr) {
+        in0[i] = nd.in.size() > 0 ? nd.in[0] : 0;
+      }
+      if (in1 != nullptr) {
+        in1[i] = nd.in.size() > 1 ? nd.in[1] : 0;
+      }
+      if (constant != nullptr) {
+        constant[i] = nd.op == cmi::model::Op::Const ? nd.payload.constant : 0.0;
+      }
+      if (var_index != nullptr) {
+        var_index[i] = nd.op == cmi::model::Op::Var ? nd.payload.var_index : 0;
+      }
+      if (int_exp != nullptr) {
+        int_exp[i] = nd.op == cmi::model::Op::PowN ? nd.payload.int_exp : 0;
+      }
+    }
+    for (std::size_t v = 0; v < pr.box_bounds.size(); ++v) {
+      if (box != nullptr) {
+        box[v] = {pr.box_bounds[v].lb, pr.box_bounds[v].ub};
+      }
+      if (var_kind != nullptr) {
+        var_kind[v] = static_cast<uint8_t>(pr.var_kinds[v]);
+      }
+    }
+    if (objective_root != nullptr) {
+      *objective_root = pr.objective_root;
+    }
+    for (std::size_t k = 0; k < pr.constraints.size(); ++k) {
+      if (con_root != nullptr) {
+        con_root[k] = pr.constraints[k].root_id;
+      }
+      if (con_cmp != nullptr) {
+        con_cmp[k] = static_cast<uint8_t>(pr.constraints[k].cmp);
+      }
+      if (con_rhs != nullptr) {
+        con_rhs[k] = pr.constraints[k].rhs;
+      }
+    }
+  });
+}
+
+void cmi_problem_destroy(cmi_problem* p) { delete p; }
+
+cmi_status cmi_ctx_create(int device, cmi_ctx** out)
+{
+  if (out == nullptr) {
+    return fail(CMI_INVALID_HANDLE, "null argument");
+  }
+  return guarded([&] { *out = new cmi_ctx {std::make_shared<cmi::hip::Context>(device)}; });
+}
+
+void cmi_ctx_destroy(cmi_ctx* ctx) { delete ctx; }
+
+cmi_status cmi_tape_create(cmi_ctx* ctx, const cmi_problem* p, cmi_tape** out)
+{
+  if (ctx == nullptr || p == nullptr || out == nullptr) {
+    return fail(CMI_INVALID_HANDLE, "null argument");
+  }
+  return guarded([&] { *out = new cmi_tape {std::make_shared<cmi::hip::DeviceTape>(ctx->ctx, p->problem)}; });
+}
+
+cmi_status cmi_tape_info_get(const cmi_tape* t, cmi_tape_info* out)
+{
+  if (t == nullptr || out == nullptr) {
+    return fail(CMI_INVALID_HANDLE, "null argument");
+  }
+  return guarded([&] {
+    auto const& tp = t->tape->tape();
+    out->n_instr = tp.code.size();
+    out->n_slots = tp.n_slots;
+    out->n_vars = tp.n_vars;
+    out->n_outputs = tp.n_outputs;
+    out->reachable_nodes = tp.reachable_nodes;
+    out->flops_per_box = tp.flops;
+    out->bytes_per_box = 16 * static_cast<std::size_t>(tp.n_vars) + 16 + 1;
+  });
+}
+
+void cmi_tape_destroy(cmi_tape* t) { delete t; }
+
+cmi_status cmi_tape_compile_info(const cmi_problem* p, cmi_tape_info* out, char* listing, size_t cap)
+{
+  if (p == nullptr || out == nullptr) {
+    return fail(CMI_INVALID_HANDLE, "null argument");
+  }
+  return guarded([&] {
+    p->problem.validate();
+    auto const tp = cmi::tape::compile(p->problem);
+    out->n_instr = tp.code.size();
+    out->n_slots = tp.n_slots;
+    out->n_vars = tp.n_vars;
+    out->n_outputs = tp.n_outputs;
+    out->reachable_nodes = tp.reachable_nodes;
+    out->flops_per_box = tp.flops;
+    out->bytes_per_box = 16 * static_cast<std::size_t>(tp.n_vars) + 16 + 1;
+    if (listing != nullptr && cap > 0) {
+      std::string const text = cmi::tape::disassemble(tp);
+      std::size_t const k = std::min(cap - 1, text.size());
+      std::memcpy(listing, text.data(), k);
+      listing[k] = '\0';
+    }
+  });
+}
+
+cmi_status cmi_eval_boxes_device(cmi_tape* t, const double* lb, const double* ub, size_t ld, size_t n_boxes,
+                                 uint8_t* feasible, double* obj_lb, double* obj_ub, cmi_ivl* roots, void* stream)
+{
+  if (t == nullptr) {
+    return fail(CMI_INVALID_HANDLE, "null tape");
+  }
+  return guarded([&] {
+    if (n_boxes == 0) {
+      return;
+    }
+    if (ld < n_boxes) {
+      throw cmi::ShapeMismatch("eval_boxes: leading dimension smaller than n_boxes");
+    }
+    if (feasible == nullptr || obj_lb == nullptr || lb == nullptr || ub == nullptr) {
+      throw cmi::ShapeMismatch("eval_boxes: lb, ub, feasible and obj_lb are required");
+    }
+    if (n_boxes > (std::size_t(1) << 31) * 64) {
+      throw cmi::InvalidConfiguration("eval_boxes: too many boxes for one launch");
+    }
+    auto const& ctx = *t->tape->context();
+    ctx.activate();
+    hipStream_t const s = stream != nullptr ? static_cast<hipStream_t>(stream) : ctx.stream();
+    t->tape->eval_boxes_device(lb, ub, ld, n_boxes, feasible, obj_lb, obj_ub, reinterpret_cast<double*>(roots), s);
+  });
+}
+
+cmi_status cmi_eval_boxes(cmi_tape* t, const double* lb, const double* ub, size_t ld, size_t n_boxes,
+                          uint8_t* feasible, double* obj_lb, double* obj_ub, cmi_ivl* roots)
+{
+  if (t == nullptr) {
+    return fail(CMI_INVALID_HANDLE, "null tape");
+  }
+  return guarded([&] {
+    using cmi::hip::check;
+    using cmi::hip::DeviceBuffer;
+    if (n_boxes == 0) {
+      return;
+    }
+    if (ld < n_boxes) {
+      throw cmi::ShapeMismatch("eval_boxes: leading dimension smaller than n_boxes");
+    }
+    if (feasible == nullptr || obj_lb == nullptr || lb == nullptr || ub == nullptr) {
+      throw cmi::ShapeMismatch("eval_boxes: lb, ub, feasible and obj_lb are required");
+    }
+    auto const& dt = *t->tape;
+    auto const& ctx = *dt.context();
+    ctx.activate();
+    hipStream_t const s = ctx.stream();
+    std::size_t const nv = dt.n_vars();
+    std::size_t const n_out = dt.tape().n_outputs;
+    DeviceBuffer dlb(std::max<std::size_t>(nv, 1) * n_boxes * 8), dub(std::max<std::size_t>(nv, 1) * n_boxes * 8);
+    for (std::size_t v = 0; v < nv; ++v) {
+      check(hipMemcpyAsync(dlb.as<double>() + v * n_boxes, lb + v * ld, n_boxes * 8, hipMemcpyHostToDevice, s),
+            "hipMemcpyAsync(lb)");
+      check(hipMemcpyAsync(dub.as<double>() + v * n_boxes, ub + v * ld, n_boxes * 8, hipMemcpyHostToDevice, s),
+            "hipMemcpyAsync(ub)");
+    }
+    DeviceBuffer dfeas(n_boxes), dlo(n_boxes * 8), dhi(obj_ub != nullptr ? n_boxes * 8 : 0),
+        droots(roots != nullptr ? std::max<std::size_t>(n_out, 1) * n_boxes * 16 : 0);
+    dt.eval_boxes_device(dlb.as<double>(), dub.as<double>(), n_boxes, n_boxes, dfeas.as<uint8_t>(), dlo.as<double>(),
+                         dhi.as<double>(), droots.as<double>(), s);
+    check(hipMemcpyAsync(feasible, dfeas.get(), n_boxes, hipMemcpyDeviceToHost, s), "hipMemcpyAsync(feasible)");
+    check(hipMemcpyAsync(obj_lb, dlo.get(), n_boxes * 8, hipMemcpyDeviceToHost, s), "hipMemcpyAsync(obj_lb)");
+    if (obj_ub != nullptr) {
+      check(hipMemcpyAsync(obj_ub, dhi.get(), n_boxes * 8, hipMemcpyDeviceToHost, s), "hipMemcpyAsync(obj_ub)");
+    }
+    if (roots != nullptr && n_out > 0) {
+      check(hipMemcpyAsync(roots, droots.get(), n_out * n_boxes * 16, hipMemcpyDeviceToHost, s),
+            "hipMemcpyAsync(roots)");
+    }
+    check(hipStreamSynchronize(s), "hipStreamSynchronize");
+  });
+}
+
+cmi_status cmi_eval_points(cmi_tape* t, const double* x, size_t ld, size_t n_points, uint8_t* feasible, double* obj,
+                           double* roots)
+{
+  if (t == nullptr) {
+    return fail(CMI_INVALID_HANDLE, "null tape");
+  }
+  return guarded([&] {
+    using cmi::hip::check;
+    using cmi::hip::DeviceBuffer;
+    if (n_points == 0) {
+      return;
+    }
+    if (ld < n_points || x == nullptr || feasible == nullptr || obj == nullptr) {
+      throw cmi::ShapeMismatch("eval_points: bad shape or missing output");
+    }
+    auto const& dt = *t->tape;
+    auto const& ctx = *dt.context();
+    ctx.activate();
+    hipStream_t const s = ctx.stream();
+    std::size_t const nv = dt.n_vars();
+    std::size_t const n_out = dt.tape().n_outputs;
+    DeviceBuffer dx(std::max<std::size_t>(nv, 1) * n_points * 8);
+    for (std::size_t v = 0; v < nv; ++v) {
+      check(hipMemcpyAsync(dx.as<double>() + v * n_points, x + v * ld, n_points * 8, hipMemcpyHostToDevice, s),
+            "hipMemcpyAsync(x)");
+    }
+    DeviceBuffer dfeas(n_points), dobj(n_points * 8),
+        droots(roots != nullptr ? std::max<std::size_t>(n_out, 1) * n_points * 8 : 0);
+    dt.eval_points_device(dx.as<double>(), n_points, n_points, dfeas.as<uint8_t>(), dobj.as<double>(),
+                          droots.as<double>(), s);
+    check(hipMemcpyAsync(feasible, dfeas.get(), n_points, hipMemcpyDeviceToHost, s), "hipMemcpyAsync(feasible)");
+    check(hipMemcpyAsync(obj, dobj.get(), n_points * 8, hipMemcpyDeviceToHost, s), "hipMemcpyAsync(obj)");
+    if (roots != nullptr && n_out > 0) {
+      check(hipMemcpyAsync(roots, droots.get(), n_out * n_points * 8, hipMemcpyDeviceToHost, s),
+            "hipMemcpyAsync(roots)");
+    }
+    check(hipStreamSynchronize(s), "hipStreamSynchronize");
+  });
+}
+
+cmi_status cmi_cost_model_get(const cmi_tape* t, size_t samples_per_region, cmi_cost_model* out)
+{
+  if (t == nullptr || out == nullptr) {
+    return fail(CMI_INVALID_HANDLE, "null argument");
+  }
+  return guarded([&] {
+    auto const m = cmi::hip::cost_model_for(t->tape->tape(), samples_per_region);
+    out->sampler_fixed = m.sampler_fixed;
+    out->sampler_per_region = m.sampler_per_region;
+    out->bounder_fixed = m.bounder_fixed;
+    out->bounder_per_region = m.bounder_per_region;
+    out->enumerator_fixed = m.enumerator_fixed;
+    out->enumerator_per_region = m.enumerator_per_region;
+  });
+}
+
+cmi_status cmi_role_build(cmi_tape* t, int role_kind, const uint8_t* slot_kinds, size_t slot_count, size_t n_regions,
+                          size_t samples_per_region, size_t budget_bytes, cmi_role** out, cmi_over_budget* facts)
+{
+  if (t == nullptr || out == nullptr || (slot_count > 0 && slot_kinds == nullptr)) {
+    return fail(CMI_INVALID_HANDLE, "null argument");
+  }
+  try {
+    auto comp = composition_of(slot_kinds, slot_count);
+    cmi::dev::Mode mode;
+    switch (role_kind) {
+      case CMI_ROLE_SAMPLER: mode = cmi::dev::Mode::Sample; break;
+      case CMI_ROLE_BOUNDER: mode = cmi::dev::Mode::Bound; break;
+      case CMI_ROLE_ENUMERATOR:
+        if (!cmi::region::is_fully_enumerable(comp)) {
+          throw cmi::InvalidConfiguration("an enumerator needs a fully enumerable composition");
+        }
+        mode = cmi::dev::Mode::Enumerate;
+        break;
+      default: throw cmi::InvalidConfiguration("unknown role kind");
+    }
+    auto impl = std::make_unique<cmi::hip::RoleImpl>(t->tape, mode, comp, n_regions, samples_per_region,
+                                                     budget_bytes);
+    *out = new cmi_role {std::move(impl), comp};
+    g_last_error.clear();
+    return CMI_OK;
+  } catch (const cmi::backend::OverBudgetError& e) {
+    if (facts != nullptr) {
+      std::memset(facts, 0, sizeof(*facts));
+      std::strncpy(facts->role, e.facts().role.c_str(), sizeof(facts->role) - 1);
+      facts->needed_bytes = e.facts().needed_bytes;
+      facts->budget_bytes = e.facts().budget_bytes;
+      facts->n_regions = e.facts().n_regions;
+      facts->bytes_per_element = e.facts().bytes_per_element;
+    }
+    return fail(CMI_OVER_BUDGET, e.what());
+  } catch (...) {
+    return guarded([] { throw; });
+  }
+}
+
+static cmi_status run_role(cmi_role* r, const cmi_ivl* box, size_t n_vars, const size_t* var_ids,
+                           const size_t* widths, size_t slot_count, uint64_t salt)
+{
+  if (r == nullptr || r->impl == nullptr) {
+    return fail(CMI_INVALID_HANDLE, "null or destroyed role");
+  }
+  if (box == nullptr || (slot_count > 0 && (var_ids == nullptr || widths == nullptr))) {
+    return fail(CMI_SHAPE_MISMATCH, "null box or slot arrays");
+  }
+  return guarded([&] {
+    std::span<const cmi::interval> b(reinterpret_cast<const cmi::interval*>(box), n_vars);
+    r->impl->run(b, {var_ids, slot_count}, {widths, slot_count}, salt);
+  });
+}
+
+cmi_status cmi_bound(cmi_role* r, const cmi_ivl* box, size_t n_vars, const size_t* var_ids, const size_t* widths,
+                     size_t slot_count, cmi_bound_view* out)
+{
+  if (r != nullptr && r->impl != nullptr && r->impl->mode() != cmi::dev::Mode::Bound) {
+    return fail(CMI_INVALID_HANDLE, "role is not a bounder");
+  }
+  cmi_status const s = run_role(r, box, n_vars, var_ids, widths, slot_count, 0);
+  if (s == CMI_OK && out != nullptr) {
+    out->feasible = r->impl->feasible().data();
+    out->obj_lb = r->impl->obj_lb().data();
+    out->n_regions = r->impl->n_regions();
+  }
+  return s;
+}
+
+static void fill_candidate(const cmi::hip::RoleImpl& impl, cmi_candidate_view* out)
+{
+  if (out == nullptr) {
+    return;
+  }
+  out->found = impl.has_candidate() ? 1 : 0;
+  out->value = impl.cand_value();
+  out->point = impl.witness().data();
+  out->index = impl.cand_index();
+}
+
+cmi_status cmi_sample(cmi_role* r, const cmi_ivl* box, size_t n_vars, const size_t* var_ids, const size_t* widths,
+                      size_t slot_count, uint64_t salt, cmi_candidate_view* out)
+{
+  if (r != nullptr && r->impl != nullptr && r->impl->mode() != cmi::dev::Mode::Sample) {
+    return fail(CMI_INVALID_HANDLE, "role is not a sampler");
+  }
+  cmi_status const s = run_role(r, box, n_vars, var_ids, widths, slot_count, salt);
+  if (s == CMI_OK) {
+    fill_candidate(*r->impl, out);
+  }
+  return s;
+}
+
+cmi_status cmi_enumerate(cmi_role* r, const cmi_ivl* box, size_t n_vars, const size_t* var_ids,
+                         const size_t* widths, size_t slot_count, cmi_candidate_view* out)
+{
+  if (r != nullptr && r->impl != nullptr && r->impl->mode() != cmi::dev::Mode::Enumerate) {
+    return fail(CMI_INVALID_HANDLE, "role is not an enumerator");
+  }
+  cmi_status const s = run_role(r, box, n_vars, var_ids, widths, slot_count, 0);
+  if (s == CMI_OK) {
+    fill_candidate(*r->impl, out);
+  }
+  return s;
+}
+
+size_t cmi_role_size(const cmi_role* r) { return r != nullptr && r->impl != nullptr ? r->impl->n_elems() : 0; }
+
+void cmi_role_destroy(cmi_role* r) { delete r; }
+
+void cmi_solve_options_default(cmi_solve_options* o)
+{
+  if (o == nullptr) {
+    return;
+  }
+  std::memset(o, 0, sizeof(*o));
+  o->partition_num = 2;
+  o->enumerate_cap = 2;
+  o->max_cycle_size = 0;
+  o->sample_points = 1;
+  o->iter_limit = 1000000;
+  o->tolerance = 1e-6;
+}
+
+cmi_status cmi_solve(cmi_ctx* ctx, const cmi_problem* p, const cmi_solve_options* o, cmi_solve_outcome* out,
+                     double* best_point)
+{
+  if (ctx == nullptr || p == nullptr || o == nullptr || out == nullptr) {
+    return fail(CMI_INVALID_HANDLE, "null argument");
+  }
+  return guarded([&] {
+    auto const t0 = std::chrono::steady_clock::now();
+    auto pol = cmi::capi::make_policy(*o);
+    auto factory = std::make_shared<cmi::hip::HipBackendFactory>(ctx->ctx);
+    std::optional<double> known;
+    if (o->has_known_primal) {
+      known = o->known_primal;
+    }
+    cmi::search::SearchDriver drv(pol, factory, o->iter_limit, o->tolerance, o->sample_points, o->budget_bytes,
+                                  o->host_budget_bytes,
+                                  o->compact_frontier ? cmi::search::FrontierPolicy::Compact
+                                                      : cmi::search::FrontierPolicy::StopAtBudget,
+                                  known);
+    auto const res = drv.solve(p->problem);
+    double const secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    fill_outcome(res, secs, out, best_point, p->problem.box_bounds.size());
+  });
+}
+
+cmi_status cmi_solve_batch(cmi_ctx* ctx, size_t n, const cmi_problem* const* problems, const cmi_solve_options* o,
+                           cmi_solve_outcome* outs, double* const* best_points)
+{
+  if (ctx == nullptr || (n > 0 && (problems == nullptr || outs == nullptr)) || o == nullptr) {
+    return fail(CMI_INVALID_HANDLE, "null argument");
+  }
+  return guarded([&] {
+    std::vector<const cmi::model::Problem<double>*> ps(n);
+    for (std::size_t i = 0; i < n; ++i) {
+      if (problems[i] == nullptr) {
+        throw cmi::InvalidConfiguration("null problem in batch");
+      }
+      ps[i] = &problems[i]->problem;
+    }
+    auto results = cmi::batch::solve_batch(ctx->ctx, ps, *o);
+    for (std::size_t i = 0; i < n; ++i) {
+      fill_outcome(results[i].outcome, results[i].seconds, &outs[i],
+                   best_points != nullptr ? best_points[i] : nullptr, ps[i]->box_bounds.size());
+    }
+  });
+}
+
+}  // extern "C"

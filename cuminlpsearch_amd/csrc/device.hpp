@@ -1,0 +1,78 @@
+#pragma once
+
+#include <cstddef>
+#include <cstdint>
+
+#include <hip/hip_runtime.h>
+
+#include "cmi/tape.hpp"
+
+// Device-side launch descriptors shared by the kernels (kernels.hip) and the
+// host code that fills them (hip_backend.cpp, batch.cpp, capi.cpp).
+namespace cmi::dev
+{
+
+inline constexpr unsigned kTile = 64;          // elements per workgroup: one wave64
+inline constexpr unsigned kMaxSlotsPerItem = 64;
+
+/// One unit of decode-mode work: a parent box split by a slot assignment,
+/// evaluated as a bounder (intervals), sampler (points drawn per region) or
+/// enumerator (lattice points). A launch covers any number of items, each a
+/// run of consecutive 64-element tiles, possibly from different problems.
+struct Item
+{
+  const tape::Instr* code;
+  std::uint32_t n_instr;
+  std::uint32_t n_vars;
+  const double* parent;  // n_vars (lb, ub) pairs
+  const std::uint8_t* var_kinds;  // n_vars, VarKind
+  const std::uint32_t* slot_var;  // slot_count entries each
+  const std::uint32_t* slot_width;
+  const std::uint64_t* slot_prefix;
+  const std::uint8_t* slot_kind;
+  std::uint32_t slot_count;
+  std::uint32_t samples;  // points per region (sampler); 1 otherwise
+  std::uint64_t n_elems;  // regions * samples
+  std::uint64_t salt;
+  std::uint32_t tile_begin;
+  std::uint32_t n_tiles;
+  // bounder outputs, per region
+  std::uint8_t* feasible;
+  double* obj_lb;
+  // sampler / enumerator outputs
+  double* tile_min;  // n_tiles partial minima
+  std::int64_t* tile_arg;
+  std::uint32_t* done;  // zero between launches; the last tile resets it
+  double* cand_value;  // 1
+  std::int64_t* cand_index;  // 1
+  double* witness;  // n_vars
+};
+
+enum class Mode : int
+{
+  Bound = 0,
+  Sample = 1,
+  Enumerate = 2,
+};
+
+/// Launch `n_tiles` workgroups over `items`; `tile_item[t]` names the item of
+/// global tile t. `lds_slots` must be >= every item's tape slot count.
+void launch_items(Mode mode, const Item* items, const std::uint32_t* tile_item, std::uint32_t n_tiles,
+                  std::uint32_t lds_slots, hipStream_t stream);
+
+/// Explicit-box evaluation: box e's variable v is [lb[v*ld+e], ub[v*ld+e]].
+/// Writes feasible[e] and obj_lb[e]; obj_ub and roots (n_outputs intervals,
+/// output-major: roots[k*n_boxes+e]) when non-null.
+void launch_boxes(const tape::Instr* code, std::uint32_t n_instr, std::uint32_t n_slots, std::uint32_t n_vars,
+                  const double* lb, const double* ub, std::uint64_t ld, std::uint64_t n_boxes,
+                  std::uint8_t* feasible, double* obj_lb, double* obj_ub, double* roots, hipStream_t stream);
+
+/// Same as launch_boxes but evaluates points (round to nearest): x[v*ld+e].
+void launch_points(const tape::Instr* code, std::uint32_t n_instr, std::uint32_t n_slots, std::uint32_t n_vars,
+                   const double* x, std::uint64_t ld, std::uint64_t n_points, std::uint8_t* feasible,
+                   double* obj, double* roots, hipStream_t stream);
+
+/// Largest per-lane slot count a 64-lane workgroup can hold in LDS.
+std::uint32_t max_lds_slots(bool interval);
+
+}  // namespace cmi::dev

@@ -1,0 +1,408 @@
+#include "hip_backend.hpp"
+
+#include <algorithm>
+#include <cstring>
+#include <limits>
+#include <stdexcept>
+
+namespace cmi::hip
+{
+
+void check(hipError_t e, const char* what)
+{
+  if (e != hipSuccess) {
+    throw DeviceError(static_cast<int>(e), std::string(what) + ": " + hipGetErrorString(e));
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Context
+// ---------------------------------------------------------------------------
+
+Context::Context(int device)
+    : device_(device)
+{
+  int count = 0;
+  check(hipGetDeviceCount(&count), "hipGetDeviceCount");
+  if (device < 0 || device >= count) {
+    throw InvalidConfiguration("device index " + std::to_string(device) + " out of range (" + std::to_string(count)
+                               + " devices)");
+  }
+  check(hipSetDevice(device), "hipSetDevice");
+  check(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking), "hipStreamCreateWithFlags");
+  lds_interval_slots_ = dev::max_lds_slots(true);
+  lds_point_slots_ = dev::max_lds_slots(false);
+}
+
+Context::~Context()
+{
+  if (stream_ != nullptr) {
+    (void)hipSetDevice(device_);
+    (void)hipStreamDestroy(stream_);
+  }
+}
+
+void Context::activate() const { check(hipSetDevice(device_), "hipSetDevice"); }
+
+std::size_t Context::free_bytes() const
+{
+  activate();
+  std::size_t free_b = 0, total_b = 0;
+  check(hipMemGetInfo(&free_b, &total_b), "hipMemGetInfo");
+  return free_b;
+}
+
+// ---------------------------------------------------------------------------
+// DeviceTape
+// ---------------------------------------------------------------------------
+
+DeviceTape::DeviceTape(std::shared_ptr<Context> ctx, const model::Problem<double>& problem,
+                       const tape::CompileOptions& opt)
+    : ctx_(std::move(ctx))
+{
+  problem.validate();
+  tape_ = tape::compile(problem, opt);
+  if (tape_.n_slots > ctx_->lds_interval_slots()) {
+    throw ResourceExhausted("tape needs " + std::to_string(tape_.n_slots) + " slots per lane; LDS holds "
+                            + std::to_string(ctx_->lds_interval_slots()) + " interval slots per 64-lane workgroup");
+  }
+  ctx_->activate();
+  code_.reset(std::max<std::size_t>(tape_.code.size(), 1) * sizeof(tape::Instr));
+  if (!tape_.code.empty()) {
+    check(hipMemcpy(code_.get(), tape_.code.data(), tape_.code.size() * sizeof(tape::Instr), hipMemcpyHostToDevice),
+          "hipMemcpy(tape)");
+  }
+  std::vector<std::uint8_t> kinds(problem.var_kinds.size());
+  for (std::size_t i = 0; i < kinds.size(); ++i) {
+    kinds[i] = static_cast<std::uint8_t>(problem.var_kinds[i]);
+  }
+  kinds_.reset(std::max<std::size_t>(kinds.size(), 1));
+  check(hipMemcpy(kinds_.get(), kinds.data(), kinds.size(), hipMemcpyHostToDevice), "hipMemcpy(var kinds)");
+}
+
+void DeviceTape::eval_boxes_device(const double* lb, const double* ub, std::size_t ld, std::size_t n,
+                                   std::uint8_t* feasible, double* obj_lb, double* obj_ub, double* roots,
+                                   hipStream_t stream) const
+{
+  dev::launch_boxes(code(), n_instr(), n_slots(), n_vars(), lb, ub, ld, n, feasible, obj_lb, obj_ub, roots, stream);
+  check(hipGetLastError(), "launch(k_boxes)");
+}
+
+void DeviceTape::eval_points_device(const double* x, std::size_t ld, std::size_t n, std::uint8_t* feasible,
+                                    double* obj, double* roots, hipStream_t stream) const
+{
+  if (tape_.n_slots > ctx_->lds_point_slots()) {
+    throw ResourceExhausted("tape needs more point slots than LDS holds");
+  }
+  dev::launch_points(code(), n_instr(), n_slots(), n_vars(), x, ld, n, feasible, obj, roots, stream);
+  check(hipGetLastError(), "launch(k_points)");
+}
+
+// ---------------------------------------------------------------------------
+// Roles
+// ---------------------------------------------------------------------------
+
+backend::RegionCostModel cost_model_for(const tape::Tape& t, std::size_t samples)
+{
+  backend::RegionCostModel m;
+  std::size_t const fixed = 4096 + t.n_vars * 24 + 64 * 32;
+  m.bounder_fixed = fixed;
+  m.bounder_per_region = 1 + sizeof(double) + 4 / 64 + 1;
+  m.sampler_fixed = fixed + t.n_vars * 8;
+  m.sampler_per_region = (samples * 16 + 63) / 64 + 1;
+  m.enumerator_fixed = fixed + t.n_vars * 8;
+  m.enumerator_per_region = 1;
+  return m;
+}
+
+namespace
+{
+std::size_t align_up(std::size_t x, std::size_t a) { return (x + a - 1) / a * a; }
+}  // namespace
+
+RoleImpl::RoleImpl(std::shared_ptr<const DeviceTape> tape, dev::Mode mode, const region::Composition& composition,
+                   std::size_t n_regions, std::size_t samples_per_region, std::size_t budget_bytes)
+    : tape_(std::move(tape))
+    , mode_(mode)
+    , composition_(composition)
+    , n_regions_(n_regions)
+    , samples_(mode == dev::Mode::Sample ? samples_per_region : 1)
+    , slot_count_(composition.size())
+    , n_vars_(tape_->n_vars())
+{
+  if (samples_ < 1) {
+    throw InvalidConfiguration("sample_points must be at least 1");
+  }
+  if (n_regions_ == 0 || n_regions_ > (std::size_t(1) << 31) / samples_) {
+    throw InvalidConfiguration("n_regions * samples must be in [1, 2^31)");
+  }
+  if (slot_count_ > dev::kMaxSlotsPerItem) {
+    throw InvalidConfiguration("composition has more slots than the device decode supports");
+  }
+  Context const& ctx = *tape_->context();
+  if (mode_ != dev::Mode::Bound && tape_->n_slots() > ctx.lds_point_slots()) {
+    throw ResourceExhausted("tape needs more point slots than LDS holds");
+  }
+  n_elems_ = n_regions_ * samples_;
+  n_tiles_ = static_cast<std::uint32_t>((n_elems_ + dev::kTile - 1) / dev::kTile);
+
+  // Budget check before any allocation (reference replay.cuh:285-304).
+  backend::RegionCostModel const cm = cost_model_for(tape_->tape(), samples_);
+  std::size_t needed = 0;
+  std::string role;
+  switch (mode_) {
+    case dev::Mode::Bound:
+      needed = cm.bounder_fixed + cm.bounder_per_region * n_regions_;
+      role = "interval bounder";
+      break;
+    case dev::Mode::Sample:
+      needed = cm.sampler_fixed + cm.sampler_per_region * n_regions_;
+      role = "point sampler";
+      break;
+    case dev::Mode::Enumerate:
+      needed = cm.enumerator_fixed + cm.enumerator_per_region * n_regions_;
+      role = "exact enumerator";
+      break;
+  }
+  std::size_t const budget = budget_bytes != 0 ? budget_bytes : ctx.free_bytes();
+  if (needed > budget) {
+    backend::OverBudget f;
+    f.role = role;
+    f.needed_bytes = needed;
+    f.budget_bytes = budget;
+    f.n_regions = n_regions_;
+    f.bytes_per_element = mode_ == dev::Mode::Bound ? cm.bounder_per_region : cm.sampler_per_region;
+    f.composition = region::spell(composition_);
+    throw backend::OverBudgetError(f);
+  }
+
+  ctx.activate();
+  off_parent_ = align_up(sizeof(dev::Item), 256);
+  off_var_ = align_up(off_parent_ + n_vars_ * 16, 16);
+  off_width_ = align_up(off_var_ + slot_count_ * 4, 16);
+  off_prefix_ = align_up(off_width_ + slot_count_ * 4, 16);
+  off_kind_ = align_up(off_prefix_ + slot_count_ * 8, 16);
+  staging_bytes_ = align_up(off_kind_ + slot_count_, 16);
+  staging_d_.reset(staging_bytes_);
+  staging_h_.reset(staging_bytes_);
+  std::memset(staging_h_.data(), 0, staging_bytes_);
+
+  std::vector<std::uint32_t> tile_item(n_tiles_, 0);
+  tile_item_d_.reset(n_tiles_ * sizeof(std::uint32_t));
+  check(hipMemcpy(tile_item_d_.get(), tile_item.data(), n_tiles_ * sizeof(std::uint32_t), hipMemcpyHostToDevice),
+        "hipMemcpy(tile_item)");
+
+  if (mode_ == dev::Mode::Bound) {
+    feasible_d_.reset(n_regions_);
+    obj_lb_d_.reset(n_regions_ * sizeof(double));
+    feasible_h_.reset(n_regions_);
+    obj_lb_h_.reset(n_regions_);
+  } else {
+    tile_min_d_.reset(n_tiles_ * sizeof(double));
+    tile_arg_d_.reset(n_tiles_ * sizeof(std::int64_t));
+    done_d_.reset(sizeof(std::uint32_t));
+    check(hipMemset(done_d_.get(), 0, sizeof(std::uint32_t)), "hipMemset(done)");
+    out_d_.reset(sizeof(Candidate));
+    witness_d_.reset(std::max<std::size_t>(n_vars_, 1) * sizeof(double));
+    out_h_.reset(1);
+    out_h_[0] = {std::numeric_limits<double>::max(), -1};
+    witness_h_.reset(std::max<std::size_t>(n_vars_, 1));
+  }
+
+  // The Item's pointers never change; only the parent box, slot data and salt
+  // are rewritten per call.
+  auto* item = reinterpret_cast<dev::Item*>(staging_h_.data());
+  auto* base = staging_d_.as<unsigned char>();
+  item->code = tape_->code();
+  item->n_instr = tape_->n_instr();
+  item->n_vars = static_cast<std::uint32_t>(n_vars_);
+  item->parent = reinterpret_cast<const double*>(base + off_parent_);
+  item->var_kinds = tape_->var_kinds();
+  item->slot_var = reinterpret_cast<const std::uint32_t*>(base + off_var_);
+  item->slot_width = reinterpret_cast<const std::uint32_t*>(base + off_width_);
+  item->slot_prefix = reinterpret_cast<const std::uint64_t*>(base + off_prefix_);
+  item->slot_kind = reinterpret_cast<const std::uint8_t*>(base + off_kind_);
+  item->slot_count = static_cast<std::uint32_t>(slot_count_);
+  item->samples = static_cast<std::uint32_t>(samples_);
+  item->n_elems = n_elems_;
+  item->salt = 0;
+  item->tile_begin = 0;
+  item->n_tiles = n_tiles_;
+  item->feasible = feasible_d_.as<std::uint8_t>();
+  item->obj_lb = obj_lb_d_.as<double>();
+  item->tile_min = tile_min_d_.as<double>();
+  item->tile_arg = tile_arg_d_.as<std::int64_t>();
+  item->done = done_d_.as<std::uint32_t>();
+  item->cand_value = out_d_.as<double>();
+  item->cand_index = out_d_.get() != nullptr ? reinterpret_cast<std::int64_t*>(out_d_.as<unsigned char>() + 8) : nullptr;
+  item->witness = witness_d_.as<double>();
+  auto* kinds = reinterpret_cast<std::uint8_t*>(staging_h_.data() + off_kind_);
+  for (std::size_t j = 0; j < slot_count_; ++j) {
+    kinds[j] = static_cast<std::uint8_t>(composition_[j]);
+  }
+}
+
+void RoleImpl::run(std::span<const interval> box, std::span<const std::size_t> var_ids,
+                   std::span<const std::size_t> widths, std::uint64_t salt)
+{
+  if (box.size() != n_vars_) {
+    throw ShapeMismatch("set_domain: domain size does not match the problem's variable count");
+  }
+  if (var_ids.size() != slot_count_ || widths.size() != slot_count_) {
+    throw ShapeMismatch("set_domain: var_ids/widths size does not match this role's slot count");
+  }
+  if (region::composition_fan_out(widths) != n_regions_) {
+    throw ShapeMismatch("set_domain: product of widths does not match this role's n_regions");
+  }
+  for (std::size_t j = 0; j < slot_count_; ++j) {
+    if (var_ids[j] >= n_vars_) {
+      throw ShapeMismatch("set_domain: slot var_id out of range");
+    }
+  }
+  Context const& ctx = *tape_->context();
+  ctx.activate();
+  hipStream_t const s = ctx.stream();
+
+  unsigned char* h = staging_h_.data();
+  auto* item = reinterpret_cast<dev::Item*>(h);
+  item->salt = salt;
+  std::memcpy(h + off_parent_, box.data(), n_vars_ * sizeof(interval));
+  auto* var = reinterpret_cast<std::uint32_t*>(h + off_var_);
+  auto* width = reinterpret_cast<std::uint32_t*>(h + off_width_);
+  auto* prefix = reinterpret_cast<std::uint64_t*>(h + off_prefix_);
+  std::uint64_t running = 1;
+  for (std::size_t j = 0; j < slot_count_; ++j) {
+    var[j] = static_cast<std::uint32_t>(var_ids[j]);
+    width[j] = static_cast<std::uint32_t>(widths[j]);
+    prefix[j] = running;
+    running *= widths[j];
+  }
+  check(hipMemcpyAsync(staging_d_.get(), h, staging_bytes_, hipMemcpyHostToDevice, s), "hipMemcpyAsync(staging)");
+  dev::launch_items(mode_, staging_d_.as<dev::Item>(), tile_item_d_.as<std::uint32_t>(), n_tiles_, tape_->n_slots(),
+                    s);
+  check(hipGetLastError(), "launch(k_items)");
+  if (mode_ == dev::Mode::Bound) {
+    check(hipMemcpyAsync(feasible_h_.data(), feasible_d_.get(), n_regions_, hipMemcpyDeviceToHost, s),
+          "hipMemcpyAsync(feasible)");
+    check(hipMemcpyAsync(obj_lb_h_.data(), obj_lb_d_.get(), n_regions_ * sizeof(double), hipMemcpyDeviceToHost, s),
+          "hipMemcpyAsync(obj_lb)");
+  } else {
+    check(hipMemcpyAsync(out_h_.data(), out_d_.get(), sizeof(Candidate), hipMemcpyDeviceToHost, s),
+          "hipMemcpyAsync(candidate)");
+    check(hipMemcpyAsync(witness_h_.data(), witness_d_.get(), n_vars_ * sizeof(double), hipMemcpyDeviceToHost, s),
+          "hipMemcpyAsync(witness)");
+  }
+  check(hipStreamSynchronize(s), "hipStreamSynchronize");
+}
+
+namespace
+{
+
+class HipBounder final : public backend::RegionBounder
+{
+public:
+  explicit HipBounder(std::unique_ptr<RoleImpl> impl)
+      : impl_(std::move(impl))
+  {
+  }
+  backend::BoundResult bound(const backend::Region& r) override
+  {
+    impl_->run(r.box, r.assignment.var_ids, r.assignment.widths, 0);
+    return {impl_->feasible(), impl_->obj_lb(), impl_->n_regions()};
+  }
+  std::size_t n_regions() const override { return impl_->n_regions(); }
+
+private:
+  std::unique_ptr<RoleImpl> impl_;
+};
+
+backend::CandidateResult candidate_of(const RoleImpl& impl)
+{
+  return {impl.has_candidate(), impl.cand_value(), impl.witness(), impl.cand_index()};
+}
+
+class HipSampler final : public backend::RegionSampler
+{
+public:
+  explicit HipSampler(std::unique_ptr<RoleImpl> impl)
+      : impl_(std::move(impl))
+  {
+  }
+  backend::CandidateResult sample(const backend::Region& r, std::uint64_t salt) override
+  {
+    impl_->run(r.box, r.assignment.var_ids, r.assignment.widths, salt);
+    return candidate_of(*impl_);
+  }
+  std::size_t n_samples() const override { return impl_->n_elems(); }
+
+private:
+  std::unique_ptr<RoleImpl> impl_;
+};
+
+class HipEnumerator final : public backend::RegionEnumerator
+{
+public:
+  explicit HipEnumerator(std::unique_ptr<RoleImpl> impl)
+      : impl_(std::move(impl))
+  {
+  }
+  backend::CandidateResult enumerate(const backend::Region& r) override
+  {
+    impl_->run(r.box, r.assignment.var_ids, r.assignment.widths, 0);
+    return candidate_of(*impl_);
+  }
+  std::size_t n_points() const override { return impl_->n_elems(); }
+
+private:
+  std::unique_ptr<RoleImpl> impl_;
+};
+
+}  // namespace
+
+HipBackendFactory::HipBackendFactory(std::shared_ptr<Context> ctx, tape::CompileOptions opt)
+    : ctx_(std::move(ctx))
+    , opt_(opt)
+{
+  if (ctx_ == nullptr) {
+    throw InvalidConfiguration("HipBackendFactory requires a device context");
+  }
+}
+
+std::shared_ptr<const DeviceTape> HipBackendFactory::tape_for(const model::Problem<double>& problem) const
+{
+  if (cached_problem_ != &problem || cached_tape_ == nullptr) {
+    cached_tape_ = std::make_shared<DeviceTape>(ctx_, problem, opt_);
+    cached_problem_ = &problem;
+  }
+  return cached_tape_;
+}
+
+backend::RegionCostModel HipBackendFactory::cost_model(const model::Problem<double>& problem) const
+{
+  return cost_model_for(tape::compile(problem, opt_), 1);
+}
+
+backend::SubdivisionBundle HipBackendFactory::build_subdivision(const model::Problem<double>& problem,
+                                                                const region::Composition& composition,
+                                                                std::size_t n_regions,
+                                                                const backend::BuildBudget& budget,
+                                                                backend::RoleRequest roles) const
+{
+  auto t = tape_for(problem);
+  backend::SubdivisionBundle out;
+  if (roles.sampler) {
+    out.sampler = std::make_unique<HipSampler>(std::make_unique<RoleImpl>(
+        t, dev::Mode::Sample, composition, n_regions, budget.samples_per_region, budget.bytes));
+  }
+  if (roles.bounder) {
+    out.bounder = std::make_unique<HipBounder>(
+        std::make_unique<RoleImpl>(t, dev::Mode::Bound, composition, n_regions, 1, budget.bytes));
+  }
+  if (roles.enumerator && region::is_fully_enumerable(composition)) {
+    out.enumerator = std::make_unique<HipEnumerator>(
+        std::make_unique<RoleImpl>(t, dev::Mode::Enumerate, composition, n_regions, 1, budget.bytes));
+  }
+  return out;
+}
+
+}  // namespace cmi::hip

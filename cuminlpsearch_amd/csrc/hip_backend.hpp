@@ -1,0 +1,254 @@
+#pragma once
+
+#include <cstddef>
+#include <cstdint>
+#include <memory>
+#include <span>
+#include <string>
+#include <vector>
+
+#include <hip/hip_runtime.h>
+
+#include "cmi/backend.hpp"
+#include "cmi/model.hpp"
+#include "cmi/tape.hpp"
+#include "device.hpp"
+
+// The HIP backend: device-resident tapes and the three roles implemented over
+// the tape kernels. HipBackendFactory is a drop-in RegionBackendFactory
+// (reference backend/backend.hpp:193-219, its CUDA-graph implementation
+// backend/graph/factory.cuh:58-117).
+namespace cmi::hip
+{
+
+void check(hipError_t e, const char* what);
+
+/// A pinned host buffer (hipHostMalloc) of trivially copyable T.
+template<typename T>
+class PinnedBuffer
+{
+public:
+  PinnedBuffer() = default;
+  explicit PinnedBuffer(std::size_t n) { reset(n); }
+  PinnedBuffer(const PinnedBuffer&) = delete;
+  PinnedBuffer& operator=(const PinnedBuffer&) = delete;
+  PinnedBuffer(PinnedBuffer&& o) noexcept { swap(o); }
+  PinnedBuffer& operator=(PinnedBuffer&& o) noexcept
+  {
+    swap(o);
+    return *this;
+  }
+  ~PinnedBuffer()
+  {
+    if (p_ != nullptr) {
+      (void)hipHostFree(p_);
+    }
+  }
+
+  void reset(std::size_t n)
+  {
+    if (p_ != nullptr) {
+      (void)hipHostFree(p_);
+      p_ = nullptr;
+    }
+    n_ = n;
+    if (n > 0) {
+      check(hipHostMalloc(reinterpret_cast<void**>(&p_), n * sizeof(T), hipHostMallocDefault), "hipHostMalloc");
+    }
+  }
+
+  T* data() { return p_; }
+  const T* data() const { return p_; }
+  std::size_t size() const { return n_; }
+  T& operator[](std::size_t i) { return p_[i]; }
+  const T& operator[](std::size_t i) const { return p_[i]; }
+
+private:
+  void swap(PinnedBuffer& o) noexcept
+  {
+    std::swap(p_, o.p_);
+    std::swap(n_, o.n_);
+  }
+  T* p_ = nullptr;
+  std::size_t n_ = 0;
+};
+
+/// A device allocation (hipMalloc) owned by value.
+class DeviceBuffer
+{
+public:
+  DeviceBuffer() = default;
+  explicit DeviceBuffer(std::size_t bytes) { reset(bytes); }
+  DeviceBuffer(const DeviceBuffer&) = delete;
+  DeviceBuffer& operator=(const DeviceBuffer&) = delete;
+  DeviceBuffer(DeviceBuffer&& o) noexcept
+  {
+    std::swap(p_, o.p_);
+    std::swap(bytes_, o.bytes_);
+  }
+  DeviceBuffer& operator=(DeviceBuffer&& o) noexcept
+  {
+    std::swap(p_, o.p_);
+    std::swap(bytes_, o.bytes_);
+    return *this;
+  }
+  ~DeviceBuffer()
+  {
+    if (p_ != nullptr) {
+      (void)hipFree(p_);
+    }
+  }
+  void reset(std::size_t bytes)
+  {
+    if (p_ != nullptr) {
+      (void)hipFree(p_);
+      p_ = nullptr;
+    }
+    bytes_ = bytes;
+    if (bytes > 0) {
+      check(hipMalloc(&p_, bytes), "hipMalloc");
+    }
+  }
+  template<typename T>
+  T* as() const
+  {
+    return static_cast<T*>(p_);
+  }
+  void* get() const { return p_; }
+  std::size_t bytes() const { return bytes_; }
+
+private:
+  void* p_ = nullptr;
+  std::size_t bytes_ = 0;
+};
+
+/// One device and the stream all of this engine's work for it is ordered on.
+class Context
+{
+public:
+  explicit Context(int device);
+  ~Context();
+  Context(const Context&) = delete;
+  Context& operator=(const Context&) = delete;
+
+  int device() const { return device_; }
+  hipStream_t stream() const { return stream_; }
+  void activate() const;
+  std::uint32_t lds_interval_slots() const { return lds_interval_slots_; }
+  std::uint32_t lds_point_slots() const { return lds_point_slots_; }
+  std::size_t free_bytes() const;
+
+private:
+  int device_;
+  hipStream_t stream_ = nullptr;
+  std::uint32_t lds_interval_slots_ = 0;
+  std::uint32_t lds_point_slots_ = 0;
+};
+
+/// A problem compiled to a tape and uploaded once.
+class DeviceTape
+{
+public:
+  DeviceTape(std::shared_ptr<Context> ctx, const model::Problem<double>& problem,
+             const tape::CompileOptions& opt = {});
+
+  const tape::Tape& tape() const { return tape_; }
+  const tape::Instr* code() const { return code_.as<tape::Instr>(); }
+  const std::uint8_t* var_kinds() const { return kinds_.as<std::uint8_t>(); }
+  std::uint32_t n_vars() const { return tape_.n_vars; }
+  std::uint32_t n_instr() const { return static_cast<std::uint32_t>(tape_.code.size()); }
+  std::uint32_t n_slots() const { return tape_.n_slots; }
+  const std::shared_ptr<Context>& context() const { return ctx_; }
+
+  /// C2-style evaluation of explicit boxes held in device memory.
+  void eval_boxes_device(const double* lb, const double* ub, std::size_t ld, std::size_t n, std::uint8_t* feasible,
+                         double* obj_lb, double* obj_ub, double* roots, hipStream_t stream) const;
+  void eval_points_device(const double* x, std::size_t ld, std::size_t n, std::uint8_t* feasible, double* obj,
+                          double* roots, hipStream_t stream) const;
+
+private:
+  std::shared_ptr<Context> ctx_;
+  tape::Tape tape_;
+  DeviceBuffer code_;
+  DeviceBuffer kinds_;
+};
+
+/// Bytes a role of each kind needs for n_regions (affine in n_regions).
+backend::RegionCostModel cost_model_for(const tape::Tape& t, std::size_t samples_per_region);
+
+/// The sampler, bounder and enumerator over one DeviceTape and a fixed
+/// composition shape (slot count, n_regions), reusable across calls; results
+/// are views into this object's pinned staging, valid until its next call.
+class RoleImpl
+{
+public:
+  RoleImpl(std::shared_ptr<const DeviceTape> tape, dev::Mode mode, const region::Composition& composition,
+           std::size_t n_regions, std::size_t samples_per_region, std::size_t budget_bytes);
+
+  dev::Mode mode() const { return mode_; }
+  std::size_t n_regions() const { return n_regions_; }
+  std::size_t n_elems() const { return n_elems_; }
+  std::size_t slot_count() const { return slot_count_; }
+
+  /// Upload the parent box and slot data, launch, and copy results back.
+  void run(std::span<const interval> box, std::span<const std::size_t> var_ids, std::span<const std::size_t> widths,
+           std::uint64_t salt);
+
+  std::span<const unsigned char> feasible() const { return {feasible_h_.data(), n_regions_}; }
+  std::span<const double> obj_lb() const { return {obj_lb_h_.data(), n_regions_}; }
+  double cand_value() const { return out_h_[0].value; }
+  std::int64_t cand_index() const { return out_h_[0].index; }
+  std::span<const double> witness() const { return {witness_h_.data(), n_vars_}; }
+  bool has_candidate() const { return out_h_[0].value < 1.7976931348623157e308; }
+
+private:
+  struct Candidate
+  {
+    double value;
+    std::int64_t index;
+  };
+
+  std::shared_ptr<const DeviceTape> tape_;
+  dev::Mode mode_;
+  region::Composition composition_;
+  std::size_t n_regions_;
+  std::size_t samples_;
+  std::size_t n_elems_;
+  std::size_t slot_count_;
+  std::size_t n_vars_;
+  std::uint32_t n_tiles_;
+  DeviceBuffer staging_d_;   // Item + parent box + slot arrays
+  DeviceBuffer tile_item_d_;
+  DeviceBuffer feasible_d_, obj_lb_d_;
+  DeviceBuffer tile_min_d_, tile_arg_d_, done_d_, out_d_, witness_d_;
+  PinnedBuffer<unsigned char> staging_h_;
+  PinnedBuffer<unsigned char> feasible_h_;
+  PinnedBuffer<double> obj_lb_h_;
+  PinnedBuffer<Candidate> out_h_;
+  PinnedBuffer<double> witness_h_;
+  std::size_t off_parent_ = 0, off_var_ = 0, off_width_ = 0, off_prefix_ = 0, off_kind_ = 0, staging_bytes_ = 0;
+};
+
+class HipBackendFactory : public backend::RegionBackendFactory
+{
+public:
+  explicit HipBackendFactory(std::shared_ptr<Context> ctx, tape::CompileOptions opt = {});
+
+  backend::BackendCapabilities capabilities() const override { return {}; }
+  backend::RegionCostModel cost_model(const model::Problem<double>& problem) const override;
+  backend::SubdivisionBundle build_subdivision(const model::Problem<double>& problem,
+                                               const region::Composition& composition, std::size_t n_regions,
+                                               const backend::BuildBudget& budget,
+                                               backend::RoleRequest roles) const override;
+
+  /// The tape this factory compiled for `problem` (compiled on first use).
+  std::shared_ptr<const DeviceTape> tape_for(const model::Problem<double>& problem) const;
+
+private:
+  std::shared_ptr<Context> ctx_;
+  tape::CompileOptions opt_;
+  mutable const model::Problem<double>* cached_problem_ = nullptr;
+  mutable std::shared_ptr<const DeviceTape> cached_tape_;
+};
+
+}  // namespace cmi::hip

@@ -1,0 +1,688 @@
+// Device evaluation of DAG tapes on gfx950 (MI355X, CDNA4, wave64).
+//
+// One workgroup is one wave64; each lane evaluates the tape for one element
+// (a box, a region, or a sample point). The tape word stream is uniform across
+// the wave and is read with scalar loads; per-lane slot values live in LDS,
+// slot-major ([slot][lane]) so every operand read or result write is one
+// conflict-free ds_read_b128/ds_write_b128 (16 B per lane, 1 KiB per wave).
+//
+// Outward rounding. gfx950 has no per-instruction rounding modes for FP64, so
+// the interval interpreter runs inside a region where the MODE register's FP64
+// rounding field is set to round-toward-+infinity, and every lower bound is
+// computed as RD(x) = -RU(-x) through VALU source negation, which is exact.
+// Every rounding FP64 instruction of the interval path is written as inline
+// asm that names the region's token, so the compiler can neither fold the
+// negations away (-(-a*b) == a*b only holds in round-to-nearest) nor move an
+// operation out of the region; the region is entered and left by volatile asm
+// with a memory clobber, so the round-to-nearest variable set-up before it and
+// the result stores after it stay outside. The point (sampler/enumerator)
+// interpreter never enters the region and is ordinary round-to-nearest code.
+//
+// The op semantics follow the reference's interval and point functors
+// (backend/graph/ops.cuh:24-355, kernels.cuh:177-340) for the QCQP op set; the
+// multiplication lower/upper bounds are the min/max of the four directed
+// endpoint products with signed zeros canonicalised to +0 (equal in value to
+// the IEEE-1788 case table for finite operands).
+
+#include <hip/hip_runtime.h>
+
+#include <cfloat>
+#include <cmath>
+#include <cstdint>
+
+#include "cmi/tape.hpp"
+#include "device.hpp"
+
+namespace cmi::dev
+{
+
+using tape::Instr;
+using tape::TOp;
+
+namespace
+{
+
+// ---------------------------------------------------------------------------
+// Round-toward-+infinity region and directed FP64 primitives
+// ---------------------------------------------------------------------------
+
+__device__ __forceinline__ int enter_round_up()
+{
+  int tok;
+  asm volatile(
+      "s_nop 1\n\t"
+      "s_setreg_imm32_b32 hwreg(HW_REG_MODE, 2, 2), 1\n\t"
+      "s_mov_b32 %0, 0"
+      : "=s"(tok)
+      :
+      : "memory");
+  return tok;
+}
+
+__device__ __forceinline__ void leave_round_up(int tok)
+{
+  asm volatile(
+      "s_nop 1\n\t"
+      "s_setreg_imm32_b32 hwreg(HW_REG_MODE, 2, 2), 0 ; %0"
+      :
+      : "s"(tok)
+      : "memory");
+}
+
+struct Iv
+{
+  double lo, hi;
+};
+
+// RU(a + b)
+__device__ __forceinline__ double ru_add(double a, double b, int t)
+{
+  double r;
+  asm("v_add_f64 %0, %1, %2 ; %3" : "=v"(r) : "v"(a), "v"(b), "s"(t));
+  return r;
+}
+
+// RU(a - b)
+__device__ __forceinline__ double ru_sub(double a, double b, int t)
+{
+  double r;
+  asm("v_add_f64 %0, %1, -%2 ; %3" : "=v"(r) : "v"(a), "v"(b), "s"(t));
+  return r;
+}
+
+// RD(a + b) = -RU(-a + -b)
+__device__ __forceinline__ double rd_add(double a, double b, int t)
+{
+  double r;
+  asm("v_add_f64 %0, -%1, -%2 ; %3" : "=v"(r) : "v"(a), "v"(b), "s"(t));
+  return -r;
+}
+
+// RD(a - b) = -RU(-a + b)
+__device__ __forceinline__ double rd_sub(double a, double b, int t)
+{
+  double r;
+  asm("v_add_f64 %0, -%1, %2 ; %3" : "=v"(r) : "v"(a), "v"(b), "s"(t));
+  return -r;
+}
+
+__device__ __forceinline__ Iv iv_add(Iv a, Iv b, int t) { return {rd_add(a.lo, b.lo, t), ru_add(a.hi, b.hi, t)}; }
+
+__device__ __forceinline__ Iv iv_sub(Iv a, Iv b, int t) { return {rd_sub(a.lo, b.hi, t), ru_sub(a.hi, b.lo, t)}; }
+
+// [a] + c
+__device__ __forceinline__ Iv iv_add_imm(Iv a, double c, int t) { return {rd_add(a.lo, c, t), ru_add(a.hi, c, t)}; }
+
+// [a] - c
+__device__ __forceinline__ Iv iv_sub_imm(Iv a, double c, int t) { return {rd_sub(a.lo, c, t), ru_sub(a.hi, c, t)}; }
+
+// c - [a]
+__device__ __forceinline__ Iv iv_imm_sub(double c, Iv a, int t) { return {rd_sub(c, a.hi, t), ru_sub(c, a.lo, t)}; }
+
+__device__ __forceinline__ Iv iv_neg(Iv a) { return {-a.hi, -a.lo}; }
+
+// [xl,xu]*[yl,yu]: lo = -max RU(-xi*yj) + 0, hi = max RU(xi*yj) + 0. The "+ 0"
+// (exact for nonzero values) maps -0 to +0 in round-up mode.
+__device__ __forceinline__ Iv iv_mul(Iv x, Iv y, int t)
+{
+  double lo, hi, t0, t1, t2, t3;
+  asm("v_mul_f64 %2, -%6, %8\n\t"
+      "v_mul_f64 %3, -%6, %9\n\t"
+      "v_mul_f64 %4, -%7, %8\n\t"
+      "v_mul_f64 %5, -%7, %9\n\t"
+      "v_max_f64 %2, %2, %3\n\t"
+      "v_max_f64 %4, %4, %5\n\t"
+      "v_max_f64 %2, %2, %4\n\t"
+      "v_add_f64 %0, -%2, 0\n\t"
+      "v_mul_f64 %2, %6, %8\n\t"
+      "v_mul_f64 %3, %6, %9\n\t"
+      "v_mul_f64 %4, %7, %8\n\t"
+      "v_mul_f64 %5, %7, %9\n\t"
+      "v_max_f64 %2, %2, %3\n\t"
+      "v_max_f64 %4, %4, %5\n\t"
+      "v_max_f64 %2, %2, %4\n\t"
+      "v_add_f64 %1, %2, 0 ; %10"
+      : "=&v"(lo), "=&v"(hi), "=&v"(t0), "=&v"(t1), "=&v"(t2), "=&v"(t3)
+      : "v"(x.lo), "v"(x.hi), "v"(y.lo), "v"(y.hi), "s"(t));
+  return {lo, hi};
+}
+
+// [a] * c for a scalar c: c > 0 -> [RD(lo*c), RU(hi*c)], c < 0 ->
+// [RD(hi*c), RU(lo*c)], otherwise [0, 0]; zeros canonicalised to +0.
+__device__ __forceinline__ Iv iv_mul_imm(Iv a, double c, int t)
+{
+  if (c > 0.0 || c < 0.0) {
+    double const pl = c > 0.0 ? a.lo : a.hi;
+    double const ph = c > 0.0 ? a.hi : a.lo;
+    double lo, hi, nl;
+    asm("v_mul_f64 %2, -%3, %5\n\t"
+        "v_add_f64 %0, -%2, 0\n\t"
+        "v_mul_f64 %1, %4, %5\n\t"
+        "v_add_f64 %1, %1, 0 ; %6"
+        : "=&v"(lo), "=&v"(hi), "=&v"(nl)
+        : "v"(pl), "v"(ph), "v"(c), "s"(t));
+    return {lo, hi};
+  }
+  return {0.0, 0.0};
+}
+
+// [a]^2: lo = RD(s*s) with s the endpoint nearest zero (0 if a straddles it),
+// hi = RU(m*m) with m = max(|lo|, |hi|).
+__device__ __forceinline__ Iv iv_sqr(Iv a, int t)
+{
+  double const s = a.lo >= 0.0 ? a.lo : (a.hi <= 0.0 ? a.hi : 0.0);
+  double lo, hi, nl, m;
+  asm("v_mul_f64 %2, -%4, %4\n\t"
+      "v_add_f64 %0, -%2, 0\n\t"
+      "v_max_f64 %3, |%5|, |%6|\n\t"
+      "v_mul_f64 %1, %3, %3 ; %7"
+      : "=&v"(lo), "=&v"(hi), "=&v"(nl), "=&v"(m)
+      : "v"(s), "v"(a.lo), "v"(a.hi), "s"(t));
+  return {lo, hi};
+}
+
+// ---------------------------------------------------------------------------
+// Region decode and sampling (same arithmetic as region::decode::slot_bounds
+// and the reference's subregion_sampler.cuh:53-120)
+// ---------------------------------------------------------------------------
+
+__device__ __forceinline__ Iv slot_bounds(std::uint8_t kind, Iv parent, std::uint32_t part, std::uint32_t fan_out)
+{
+  // SlotKind: 0 Continuous, 1 IntegerPartition, 2 IntegerEnumerate, 3 BinaryEnumerate
+  if (kind >= 2) {
+    double const lo = ceil(parent.lo);
+    double v = lo + static_cast<double>(part);
+    if (v > parent.hi) {
+      v = parent.hi;
+    }
+    return {v, v};
+  }
+  double const width = (parent.hi - parent.lo) / static_cast<double>(fan_out);
+  double const step_lo = width * static_cast<double>(part);
+  Iv out;
+  out.lo = parent.lo + step_lo;
+  if (part + 1 == fan_out) {
+    out.hi = parent.hi;
+  } else {
+    double const step_hi = width * static_cast<double>(part + 1);
+    out.hi = parent.lo + step_hi;
+  }
+  return out;
+}
+
+__device__ __forceinline__ std::uint64_t sample_hash(std::uint64_t vid, std::uint64_t i, std::uint64_t region,
+                                                     std::uint64_t salt)
+{
+  std::uint64_t x = vid * 0x9e3779b97f4a7c15ULL;
+  x ^= i * 0xbf58476d1ce4e5b9ULL;
+  x ^= region * 0x94d049bb133111ebULL;
+  x ^= salt * 0xd6e8feb86659fd93ULL;
+  x += 0x9e3779b97f4a7c15ULL;
+  x = (x ^ (x >> 30)) * 0xbf58476d1ce4e5b9ULL;
+  x = (x ^ (x >> 27)) * 0x94d049bb133111ebULL;
+  x ^= x >> 31;
+  return x;
+}
+
+__device__ __forceinline__ double draw(bool discrete, double lb, double ub, std::uint64_t vid, std::uint64_t i,
+                                       std::uint64_t region, std::uint64_t salt)
+{
+  std::uint64_t const x = sample_hash(vid, i, region, salt);
+  if (!discrete) {
+    double const u = static_cast<double>(x >> 11) * (1.0 / 9007199254740992.0);
+    return __builtin_fma(u, ub - lb, lb);
+  }
+  double const lo = ceil(lb);
+  double const hi = floor(ub);
+  if (hi < lo) {
+    return lo;
+  }
+  std::uint64_t const count = static_cast<std::uint64_t>(hi - lo) + 1;
+  return lo + static_cast<double>(x % count);
+}
+
+// ---------------------------------------------------------------------------
+// Tape interpreters
+// ---------------------------------------------------------------------------
+
+// Tape words are read through the constant address space so the compiler
+// issues scalar loads (the pc is wave-uniform), as four whole dwords.
+typedef std::uint32_t u32x4 __attribute__((ext_vector_type(4)));
+using ConstInstrPtr = const __attribute__((address_space(4))) u32x4*;
+
+__device__ __forceinline__ ConstInstrPtr as_constant(const Instr* p)
+{
+  return (ConstInstrPtr)(p);
+}
+
+struct Word
+{
+  std::uint32_t op, dst, a, b;
+  double imm;
+};
+
+__device__ __forceinline__ Word fetch(ConstInstrPtr code, std::uint32_t pc)
+{
+  u32x4 const w = code[pc];
+  Word out;
+  out.op = w.x & 0xffu;
+  out.dst = w.x >> 16;
+  out.a = w.y & 0xffffu;
+  out.b = w.y >> 16;
+  out.imm = __builtin_bit_cast(double, (static_cast<std::uint64_t>(w.w) << 32) | w.z);
+  return out;
+}
+
+struct IntervalResult
+{
+  Iv obj;
+  bool feasible;
+};
+
+// Interval tape over LDS slots S[slot * kTile + lane]. Must run inside the
+// round-up region (token t). Output instructions store into
+// roots[(b * roots_stride + e)] when roots is non-null.
+__device__ __forceinline__ IntervalResult run_interval(ConstInstrPtr code, std::uint32_t n_instr, double2* S,
+                                                       unsigned lane, int t, double2* roots,
+                                                       std::uint64_t roots_stride, std::uint64_t e)
+{
+  IntervalResult res {{0.0, 0.0}, true};
+  for (std::uint32_t pc = 0; pc < n_instr; ++pc) {
+    Word const ins = fetch(code, pc);
+    TOp const op = static_cast<TOp>(ins.op);
+    double2* const dst = &S[ins.dst * kTile + lane];
+    double2 const av = S[ins.a * kTile + lane];
+    Iv const a {av.x, av.y};
+    Iv r;
+    switch (op) {
+      case TOp::LoadConst:
+        r = {ins.imm, ins.imm};
+        break;
+      case TOp::Add: {
+        double2 const bv = S[ins.b * kTile + lane];
+        r = iv_add(a, {bv.x, bv.y}, t);
+        break;
+      }
+      case TOp::AddImm:
+        r = iv_add_imm(a, ins.imm, t);
+        break;
+      case TOp::Sub: {
+        double2 const bv = S[ins.b * kTile + lane];
+        r = iv_sub(a, {bv.x, bv.y}, t);
+        break;
+      }
+      case TOp::SubImm:
+        r = iv_sub_imm(a, ins.imm, t);
+        break;
+      case TOp::ImmSub:
+        r = iv_imm_sub(ins.imm, a, t);
+        break;
+      case TOp::Mul: {
+        double2 const bv = S[ins.b * kTile + lane];
+        r = iv_mul(a, {bv.x, bv.y}, t);
+        break;
+      }
+      case TOp::MulImm:
+        r = iv_mul_imm(a, ins.imm, t);
+        break;
+      case TOp::Sqr:
+        r = iv_sqr(a, t);
+        break;
+      case TOp::Neg:
+        r = iv_neg(a);
+        break;
+      case TOp::CheckLE:
+        res.feasible = res.feasible && (a.lo <= ins.imm);
+        continue;
+      case TOp::CheckEQ:
+        res.feasible = res.feasible && (a.lo <= ins.imm) && (ins.imm <= a.hi);
+        continue;
+      case TOp::Objective:
+        res.obj = a;
+        continue;
+      case TOp::Output:
+        if (roots != nullptr) {
+          roots[static_cast<std::uint64_t>(ins.b) * roots_stride + e] = av;
+        }
+        continue;
+      default:
+        continue;
+    }
+    *dst = make_double2(r.lo, r.hi);
+  }
+  return res;
+}
+
+__device__ __forceinline__ bool almost_equal(double a, double b)
+{
+  double const diff = fabs(a - b);
+  if (diff <= 1e-8) {
+    return true;
+  }
+  return diff <= 1e-6 * fmax(fabs(a), fabs(b));
+}
+
+struct PointResult
+{
+  double obj;
+  bool feasible;
+};
+
+// Round-to-nearest point tape over LDS slots P[slot * kTile + lane].
+__device__ __forceinline__ PointResult run_point(ConstInstrPtr code, std::uint32_t n_instr, double* P, unsigned lane,
+                                                 double* roots, std::uint64_t roots_stride, std::uint64_t e)
+{
+  PointResult res {0.0, true};
+  for (std::uint32_t pc = 0; pc < n_instr; ++pc) {
+    Word const ins = fetch(code, pc);
+    TOp const op = static_cast<TOp>(ins.op);
+    double* const dst = &P[ins.dst * kTile + lane];
+    double const a = P[ins.a * kTile + lane];
+    double r;
+    switch (op) {
+      case TOp::LoadConst: r = ins.imm; break;
+      case TOp::Add: r = a + P[ins.b * kTile + lane]; break;
+      case TOp::AddImm: r = a + ins.imm; break;
+      case TOp::Sub: r = a - P[ins.b * kTile + lane]; break;
+      case TOp::SubImm: r = a - ins.imm; break;
+      case TOp::ImmSub: r = ins.imm - a; break;
+      case TOp::Mul: r = a * P[ins.b * kTile + lane]; break;
+      case TOp::MulImm: r = a * ins.imm; break;
+      case TOp::Sqr: r = a * a; break;
+      case TOp::Neg: r = -a; break;
+      case TOp::CheckLE:
+        res.feasible = res.feasible && (a <= ins.imm);
+        continue;
+      case TOp::CheckEQ:
+        res.feasible = res.feasible && almost_equal(a, ins.imm);
+        continue;
+      case TOp::Objective:
+        res.obj = a;
+        continue;
+      case TOp::Output:
+        if (roots != nullptr) {
+          roots[static_cast<std::uint64_t>(ins.b) * roots_stride + e] = a;
+        }
+        continue;
+      default:
+        continue;
+    }
+    *dst = r;
+  }
+  return res;
+}
+
+// Wave64 (value, index) min with the lowest index winning ties.
+__device__ __forceinline__ void wave_argmin(double& v, std::int64_t& i)
+{
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    double const ov = __shfl_xor(v, off, 64);
+    std::int64_t const oi = __shfl_xor(i, off, 64);
+    if (ov < v || (ov == v && oi < i)) {
+      v = ov;
+      i = oi;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Kernels
+// ---------------------------------------------------------------------------
+
+__global__ __launch_bounds__(kTile) void k_boxes(const Instr* __restrict__ code_g, std::uint32_t n_instr,
+                                                  std::uint32_t n_vars, const double* __restrict__ lb,
+                                                  const double* __restrict__ ub, std::uint64_t ld,
+                                                  std::uint64_t n_boxes, std::uint8_t* __restrict__ feasible,
+                                                  double* __restrict__ obj_lb, double* __restrict__ obj_ub,
+                                                  double2* __restrict__ roots)
+{
+  extern __shared__ double2 S[];
+  unsigned const lane = threadIdx.x;
+  std::uint64_t const e = static_cast<std::uint64_t>(blockIdx.x) * kTile + lane;
+  bool const active = e < n_boxes;
+  std::uint64_t const ee = active ? e : n_boxes - 1;
+  for (std::uint32_t v = 0; v < n_vars; ++v) {
+    S[v * kTile + lane] = make_double2(lb[v * ld + ee], ub[v * ld + ee]);
+  }
+  int const t = enter_round_up();
+  IntervalResult const res =
+      run_interval(as_constant(code_g), n_instr, S, lane, t, active ? roots : nullptr, n_boxes, e);
+  leave_round_up(t);
+  if (active) {
+    feasible[e] = res.feasible ? 1 : 0;
+    obj_lb[e] = res.obj.lo;
+    if (obj_ub != nullptr) {
+      obj_ub[e] = res.obj.hi;
+    }
+  }
+}
+
+__global__ __launch_bounds__(kTile) void k_points(const Instr* __restrict__ code_g, std::uint32_t n_instr,
+                                                   std::uint32_t n_vars, const double* __restrict__ x,
+                                                   std::uint64_t ld, std::uint64_t n_points,
+                                                   std::uint8_t* __restrict__ feasible, double* __restrict__ obj,
+                                                   double* __restrict__ roots)
+{
+  extern __shared__ double P[];
+  unsigned const lane = threadIdx.x;
+  std::uint64_t const e = static_cast<std::uint64_t>(blockIdx.x) * kTile + lane;
+  bool const active = e < n_points;
+  std::uint64_t const ee = active ? e : n_points - 1;
+  for (std::uint32_t v = 0; v < n_vars; ++v) {
+    P[v * kTile + lane] = x[v * ld + ee];
+  }
+  PointResult const res = run_point(as_constant(code_g), n_instr, P, lane, active ? roots : nullptr, n_points, e);
+  if (active) {
+    feasible[e] = res.feasible ? 1 : 0;
+    obj[e] = res.obj;
+  }
+}
+
+// Decode-mode item kernels. Each workgroup is one 64-element tile of one item.
+template<int M>
+__global__ __launch_bounds__(kTile) void k_items(const Item* __restrict__ items,
+                                                  const std::uint32_t* __restrict__ tile_item)
+{
+  extern __shared__ double2 lds[];
+  unsigned const lane = threadIdx.x;
+  std::uint32_t const tile = blockIdx.x;
+  Item const& it = items[tile_item[tile]];
+  std::uint32_t const lt = tile - it.tile_begin;
+  std::uint64_t const e = static_cast<std::uint64_t>(lt) * kTile + lane;
+  bool const active = e < it.n_elems;
+  std::uint32_t const ee = static_cast<std::uint32_t>(active ? e : it.n_elems - 1);
+  std::uint32_t const S = it.samples;
+  std::uint32_t const region = (M == static_cast<int>(Mode::Sample)) ? ee / S : ee;
+  std::uint32_t const sample = (M == static_cast<int>(Mode::Sample)) ? ee - region * S : 0;
+  ConstInstrPtr const code = as_constant(it.code);
+  std::uint32_t const n_vars = it.n_vars;
+
+  if constexpr (M == static_cast<int>(Mode::Bound)) {
+    double2* const Sl = lds;
+    for (std::uint32_t v = 0; v < n_vars; ++v) {
+      Sl[v * kTile + lane] = make_double2(it.parent[2 * v], it.parent[2 * v + 1]);
+    }
+    std::uint32_t idx = region;
+    for (std::uint32_t j = 0; j < it.slot_count; ++j) {
+      std::uint32_t const w = it.slot_width[j];
+      std::uint32_t const part = idx % w;
+      idx /= w;
+      std::uint32_t const vid = it.slot_var[j];
+      Iv const b = slot_bounds(it.slot_kind[j], {it.parent[2 * vid], it.parent[2 * vid + 1]}, part, w);
+      Sl[vid * kTile + lane] = make_double2(b.lo, b.hi);
+    }
+    int const t = enter_round_up();
+    IntervalResult const res = run_interval(code, it.n_instr, Sl, lane, t, nullptr, 0, 0);
+    leave_round_up(t);
+    if (active) {
+      it.feasible[region] = res.feasible ? 1 : 0;
+      it.obj_lb[region] = res.obj.lo;
+    }
+    return;
+  } else {
+    double* const P = reinterpret_cast<double*>(lds);
+    bool const sampling = M == static_cast<int>(Mode::Sample);
+    for (std::uint32_t v = 0; v < n_vars; ++v) {
+      double const lo = it.parent[2 * v];
+      double const hi = it.parent[2 * v + 1];
+      P[v * kTile + lane] = sampling ? draw(it.var_kinds[v] != 0, lo, hi, v, sample, region, it.salt) : lo;
+    }
+    std::uint32_t idx = region;
+    for (std::uint32_t j = 0; j < it.slot_count; ++j) {
+      std::uint32_t const w = it.slot_width[j];
+      std::uint32_t const part = idx % w;
+      idx /= w;
+      std::uint32_t const vid = it.slot_var[j];
+      Iv const b = slot_bounds(it.slot_kind[j], {it.parent[2 * vid], it.parent[2 * vid + 1]}, part, w);
+      P[vid * kTile + lane] = sampling ? draw(it.var_kinds[vid] != 0, b.lo, b.hi, vid, sample, region, it.salt) : b.lo;
+    }
+    PointResult const res = run_point(code, it.n_instr, P, lane, nullptr, 0, 0);
+    double v = (active && res.feasible) ? res.obj : INFINITY;
+    if (v != v) {
+      v = INFINITY;
+    }
+    std::int64_t i = active ? static_cast<std::int64_t>(e) : INT64_MAX;
+    wave_argmin(v, i);
+    if (lane == 0) {
+      it.tile_min[lt] = v;
+      it.tile_arg[lt] = i;
+    }
+    __threadfence();
+    std::uint32_t ticket = 0;
+    if (lane == 0) {
+      ticket = atomicAdd(it.done, 1u);
+    }
+    ticket = __shfl(ticket, 0, 64);
+    if (ticket != it.n_tiles - 1) {
+      return;
+    }
+    __threadfence();
+    double bv = INFINITY;
+    std::int64_t bi = INT64_MAX;
+    for (std::uint32_t k = lane; k < it.n_tiles; k += kTile) {
+      std::uint64_t const vb = __hip_atomic_load(reinterpret_cast<std::uint64_t*>(&it.tile_min[k]),
+                                                 __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      std::int64_t const ib = __hip_atomic_load(&it.tile_arg[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      double const vv = __builtin_bit_cast(double, vb);
+      if (vv < bv || (vv == bv && ib < bi)) {
+        bv = vv;
+        bi = ib;
+      }
+    }
+    wave_argmin(bv, bi);
+    bool const found = bv < DBL_MAX;
+    if (lane == 0) {
+      *it.cand_value = found ? bv : DBL_MAX;
+      *it.cand_index = found ? bi : -1;
+      *it.done = 0;
+    }
+    std::uint32_t const best = found ? static_cast<std::uint32_t>(bi) : 0;
+    std::uint32_t const b_region = sampling ? best / S : best;
+    std::uint32_t const b_sample = sampling ? best - b_region * S : 0;
+    for (std::uint32_t v = lane; v < n_vars; v += kTile) {
+      double lo = it.parent[2 * v];
+      double hi = it.parent[2 * v + 1];
+      std::uint32_t bidx = b_region;
+      for (std::uint32_t j = 0; j < it.slot_count; ++j) {
+        std::uint32_t const w = it.slot_width[j];
+        std::uint32_t const part = bidx % w;
+        bidx /= w;
+        if (it.slot_var[j] == v) {
+          Iv const b = slot_bounds(it.slot_kind[j], {lo, hi}, part, w);
+          lo = b.lo;
+          hi = b.hi;
+        }
+      }
+      double const x = sampling ? draw(it.var_kinds[v] != 0, lo, hi, v, b_sample, b_region, it.salt) : lo;
+      it.witness[v] = found ? x : NAN;
+    }
+  }
+}
+
+template<typename K>
+void allow_large_lds(K kernel)
+{
+  int dev = 0;
+  (void)hipGetDevice(&dev);
+  int optin = 0;
+  if (hipDeviceGetAttribute(&optin, hipDeviceAttributeSharedMemPerBlockOptin, dev) != hipSuccess || optin <= 0) {
+    (void)hipDeviceGetAttribute(&optin, hipDeviceAttributeMaxSharedMemoryPerBlock, dev);
+  }
+  (void)hipFuncSetAttribute(reinterpret_cast<const void*>(kernel), hipFuncAttributeMaxDynamicSharedMemorySize,
+                            optin);
+}
+
+}  // namespace
+
+std::uint32_t max_lds_slots(bool interval)
+{
+  int dev = 0;
+  (void)hipGetDevice(&dev);
+  int optin = 0;
+  if (hipDeviceGetAttribute(&optin, hipDeviceAttributeSharedMemPerBlockOptin, dev) != hipSuccess || optin <= 0) {
+    (void)hipDeviceGetAttribute(&optin, hipDeviceAttributeMaxSharedMemoryPerBlock, dev);
+  }
+  return static_cast<std::uint32_t>(optin) / (kTile * (interval ? 16u : 8u));
+}
+
+void launch_items(Mode mode, const Item* items, const std::uint32_t* tile_item, std::uint32_t n_tiles,
+                  std::uint32_t lds_slots, hipStream_t stream)
+{
+  if (n_tiles == 0) {
+    return;
+  }
+  std::size_t const bytes = static_cast<std::size_t>(lds_slots) * kTile * (mode == Mode::Bound ? 16 : 8);
+  switch (mode) {
+    case Mode::Bound: {
+      static bool once = (allow_large_lds(k_items<0>), true);
+      (void)once;
+      hipLaunchKernelGGL(k_items<0>, dim3(n_tiles), dim3(kTile), bytes, stream, items, tile_item);
+      break;
+    }
+    case Mode::Sample: {
+      static bool once = (allow_large_lds(k_items<1>), true);
+      (void)once;
+      hipLaunchKernelGGL(k_items<1>, dim3(n_tiles), dim3(kTile), bytes, stream, items, tile_item);
+      break;
+    }
+    case Mode::Enumerate: {
+      static bool once = (allow_large_lds(k_items<2>), true);
+      (void)once;
+      hipLaunchKernelGGL(k_items<2>, dim3(n_tiles), dim3(kTile), bytes, stream, items, tile_item);
+      break;
+    }
+  }
+}
+
+void launch_boxes(const Instr* code, std::uint32_t n_instr, std::uint32_t n_slots, std::uint32_t n_vars,
+                  const double* lb, const double* ub, std::uint64_t ld, std::uint64_t n_boxes, std::uint8_t* feasible,
+                  double* obj_lb, double* obj_ub, double* roots, hipStream_t stream)
+{
+  if (n_boxes == 0) {
+    return;
+  }
+  static bool once = (allow_large_lds(k_boxes), true);
+  (void)once;
+  std::uint64_t const blocks = (n_boxes + kTile - 1) / kTile;
+  std::size_t const bytes = static_cast<std::size_t>(n_slots) * kTile * 16;
+  hipLaunchKernelGGL(k_boxes, dim3(static_cast<unsigned>(blocks)), dim3(kTile), bytes, stream, code, n_instr,
+                     n_vars, lb, ub, ld, n_boxes, feasible, obj_lb, obj_ub, reinterpret_cast<double2*>(roots));
+}
+
+void launch_points(const Instr* code, std::uint32_t n_instr, std::uint32_t n_slots, std::uint32_t n_vars,
+                   const double* x, std::uint64_t ld, std::uint64_t n_points, std::uint8_t* feasible, double* obj,
+                   double* roots, hipStream_t stream)
+{
+  if (n_points == 0) {
+    return;
+  }
+  static bool once = (allow_large_lds(k_points), true);
+  (void)once;
+  std::uint64_t const blocks = (n_points + kTile - 1) / kTile;
+  std::size_t const bytes = static_cast<std::size_t>(n_slots) * kTile * 8;
+  hipLaunchKernelGGL(k_points, dim3(static_cast<unsigned>(blocks)), dim3(kTile), bytes, stream, code, n_instr,
+                     n_vars, x, ld, n_points, feasible, obj, roots);
+}
+
+}  // namespace cmi::dev

@@ -1,0 +1,280 @@
+// Request-stream serving: continuous batching of searches on one device, and
+// the one-at-a-time baseline it is measured against.
+
+#include <algorithm>
+#include <chrono>
+#include <cstring>
+#include <memory>
+#include <numeric>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "batch.hpp"
+#include "cmi.h"
+#include "cmi/policy.hpp"
+#include "cmi/search.hpp"
+#include "handles.hpp"
+#include "hip_backend.hpp"
+
+namespace cmi::capi
+{
+std::shared_ptr<cmi::policy::GreedyEnumCompositionPolicy> make_policy(const cmi_solve_options& o);
+}  // namespace cmi::capi
+
+namespace
+{
+
+using Clock = std::chrono::steady_clock;
+
+cmi::search::SolveOptions solve_options(const cmi_solve_options& o)
+{
+  cmi::search::SolveOptions so;
+  so.iter_limit = o.iter_limit;
+  so.tolerance = o.tolerance;
+  so.sample_points = o.sample_points;
+  so.host_budget_bytes = o.host_budget_bytes;
+  so.frontier_policy =
+      o.compact_frontier ? cmi::search::FrontierPolicy::Compact : cmi::search::FrontierPolicy::StopAtBudget;
+  if (o.has_known_primal) {
+    so.known_primal_bound = o.known_primal;
+  }
+  return so;
+}
+
+/// Device bytes one resident solve holds between steps plus what one step of
+/// it writes: the tape, its variable kinds, and its share of the step's
+/// staging/result/partial buffers for the widest composition the policy can
+/// produce.
+std::size_t resident_bytes(const cmi::hip::DeviceTape& t, const cmi_solve_options& o, std::size_t max_regions)
+{
+  std::size_t const n_vars = t.n_vars();
+  std::size_t const samples = std::max<std::size_t>(o.sample_points, 1);
+  return t.n_instr() * sizeof(cmi::tape::Instr) + n_vars + 3 * (512 + n_vars * 16) + max_regions * 9
+      + (max_regions * samples / 64 + 1) * 16 + n_vars * 8 + 64;
+}
+
+std::size_t max_regions_of(const cmi_solve_options& o, std::size_t n_vars)
+{
+  std::size_t const slots = std::min<std::size_t>(o.max_cycle_size == 0 ? 64 : o.max_cycle_size, n_vars);
+  std::size_t const w = std::max(o.partition_num, std::max<std::size_t>(o.enumerate_cap, 2));
+  std::size_t r = 1;
+  for (std::size_t j = 0; j < slots && r < (std::size_t(1) << 31); ++j) {
+    r *= w;
+  }
+  return r;
+}
+
+struct Active
+{
+  std::size_t idx;
+  std::unique_ptr<cmi::hip::DeviceTape> tape;
+  std::unique_ptr<cmi::search::SolveState> state;
+  std::size_t bytes;
+};
+
+void wait_until(Clock::time_point t0, double at)
+{
+  auto const target = t0 + std::chrono::duration_cast<Clock::duration>(std::chrono::duration<double>(at));
+  std::this_thread::sleep_until(target);
+}
+
+}  // namespace
+
+extern "C" cmi_status cmi_serve_stream(cmi_ctx* ctx, size_t n, const cmi_problem* const* problems,
+                                       const double* arrival, const cmi_solve_options* o,
+                                       const cmi_server_options* so, cmi_solve_outcome* outs, double* finish,
+                                       double* latency, cmi_serve_stats* stats)
+{
+  struct Args
+  {
+    cmi_ctx* ctx;
+    size_t n;
+    const cmi_problem* const* problems;
+    const double* arrival;
+    const cmi_solve_options* o;
+    const cmi_server_options* so;
+    cmi_solve_outcome* outs;
+    double* finish;
+    double* latency;
+    cmi_serve_stats* stats;
+  } a {ctx, n, problems, arrival, o, so, outs, finish, latency, stats};
+  if (ctx == nullptr || o == nullptr || so == nullptr || (n > 0 && (problems == nullptr || outs == nullptr))) {
+    return CMI_INVALID_HANDLE;
+  }
+  return cmi::capi::run_guarded(
+      [](void* p) {
+        Args& a = *static_cast<Args*>(p);
+        if (a.so->max_active < 1) {
+          throw cmi::InvalidConfiguration("server max_active must be at least 1");
+        }
+        auto const pol = cmi::capi::make_policy(*a.o);
+        cmi::search::SolveOptions const sopt = solve_options(*a.o);
+        std::vector<std::size_t> order(a.n);
+        std::iota(order.begin(), order.end(), 0);
+        auto arrive = [&](std::size_t i) { return (a.so->realtime && a.arrival != nullptr) ? a.arrival[i] : 0.0; };
+        std::stable_sort(order.begin(), order.end(), [&](std::size_t x, std::size_t y) { return arrive(x) < arrive(y); });
+
+        cmi::batch::Runner runner(a.ctx->ctx);
+        std::vector<Active> active;
+        std::size_t next = 0, completed = 0, used_bytes = 0, peak = 0;
+        std::uint64_t steps = 0;
+        std::vector<cmi::batch::Request> reqs;
+        struct Owner
+        {
+          std::size_t slot, first_req;
+          bool enumerate;
+        };
+        std::vector<Owner> owners;
+        auto const t0 = Clock::now();
+        auto elapsed = [&] { return std::chrono::duration<double>(Clock::now() - t0).count(); };
+
+        while (completed < a.n) {
+          double const now = elapsed();
+          // Admission: arrived, a slot free, and the device budget fits.
+          while (next < a.n && active.size() < a.so->max_active && arrive(order[next]) <= now) {
+            std::size_t const i = order[next];
+            auto tape = std::make_unique<cmi::hip::DeviceTape>(a.ctx->ctx, a.problems[i]->problem);
+            std::size_t const bytes =
+                resident_bytes(*tape, *a.o, max_regions_of(*a.o, a.problems[i]->problem.box_bounds.size()));
+            if (a.so->hbm_budget_bytes != 0 && !active.empty() && used_bytes + bytes > a.so->hbm_budget_bytes) {
+              break;  // wait for a resident solve to finish
+            }
+            used_bytes += bytes;
+            active.push_back(Active {i, std::move(tape),
+                                     std::make_unique<cmi::search::SolveState>(a.problems[i]->problem, pol, sopt),
+                                     bytes});
+            ++next;
+          }
+          peak = std::max(peak, active.size());
+          if (active.empty()) {
+            if (next < a.n) {
+              wait_until(t0, arrive(order[next]));
+            }
+            continue;
+          }
+          reqs.clear();
+          owners.clear();
+          for (std::size_t s = 0; s < active.size(); ++s) {
+            const cmi::search::SolveState::Request* q = active[s].state->next();
+            if (q == nullptr) {
+              continue;
+            }
+            owners.push_back({s, reqs.size(), q->kind == cmi::search::SolveState::Kind::Enumerate});
+            if (q->kind == cmi::search::SolveState::Kind::Enumerate) {
+              reqs.push_back({active[s].tape.get(), cmi::dev::Mode::Enumerate, q->box, q->assignment, q->salt, 1});
+            } else {
+              reqs.push_back({active[s].tape.get(), cmi::dev::Mode::Sample, q->box, q->assignment, q->salt,
+                              static_cast<std::uint32_t>(a.o->sample_points)});
+              reqs.push_back({active[s].tape.get(), cmi::dev::Mode::Bound, q->box, q->assignment, q->salt, 1});
+            }
+          }
+          if (!reqs.empty()) {
+            runner.run(reqs);
+            ++steps;
+            for (Owner const& w : owners) {
+              cmi::search::SolveState& st = *active[w.slot].state;
+              if (w.enumerate) {
+                st.on_enumerate(runner.candidate(w.first_req), runner.n_elems(w.first_req));
+              } else {
+                cmi::backend::BoundResult const b {runner.feasible(w.first_req + 1), runner.obj_lb(w.first_req + 1),
+                                                   runner.n_elems(w.first_req + 1)};
+                st.on_subdivide(runner.candidate(w.first_req), runner.n_elems(w.first_req), b);
+              }
+            }
+          }
+          // Retire finished solves; their slots are free for the next admission.
+          double const t_done = elapsed();
+          for (std::size_t s = active.size(); s-- > 0;) {
+            if (!active[s].state->finished()) {
+              continue;
+            }
+            std::size_t const i = active[s].idx;
+            cmi::capi::fill_outcome_public(active[s].state->finish(), t_done - arrive(i), &a.outs[i], nullptr,
+                                           a.problems[i]->problem.box_bounds.size());
+            if (a.finish != nullptr) {
+              a.finish[i] = t_done;
+            }
+            if (a.latency != nullptr) {
+              a.latency[i] = t_done - arrive(i);
+            }
+            used_bytes -= active[s].bytes;
+            active.erase(active.begin() + static_cast<std::ptrdiff_t>(s));
+            ++completed;
+          }
+        }
+        if (a.stats != nullptr) {
+          a.stats->steps = steps;
+          a.stats->launches = runner.launches();
+          a.stats->peak_active = peak;
+          a.stats->wall_seconds = elapsed();
+        }
+      },
+      &a);
+}
+
+extern "C" cmi_status cmi_serve_plain(cmi_ctx* ctx, size_t n, const cmi_problem* const* problems,
+                                      const double* arrival, const cmi_solve_options* o, cmi_solve_outcome* outs,
+                                      double* finish, double* latency, cmi_serve_stats* stats)
+{
+  struct Args
+  {
+    cmi_ctx* ctx;
+    size_t n;
+    const cmi_problem* const* problems;
+    const double* arrival;
+    const cmi_solve_options* o;
+    cmi_solve_outcome* outs;
+    double* finish;
+    double* latency;
+    cmi_serve_stats* stats;
+  } a {ctx, n, problems, arrival, o, outs, finish, latency, stats};
+  if (ctx == nullptr || o == nullptr || (n > 0 && (problems == nullptr || outs == nullptr))) {
+    return CMI_INVALID_HANDLE;
+  }
+  return cmi::capi::run_guarded(
+      [](void* p) {
+        Args& a = *static_cast<Args*>(p);
+        auto const pol = cmi::capi::make_policy(*a.o);
+        std::vector<std::size_t> order(a.n);
+        std::iota(order.begin(), order.end(), 0);
+        auto arrive = [&](std::size_t i) { return a.arrival != nullptr ? a.arrival[i] : 0.0; };
+        std::stable_sort(order.begin(), order.end(), [&](std::size_t x, std::size_t y) { return arrive(x) < arrive(y); });
+        auto const t0 = Clock::now();
+        auto elapsed = [&] { return std::chrono::duration<double>(Clock::now() - t0).count(); };
+        std::uint64_t iters = 0;
+        for (std::size_t i : order) {
+          if (elapsed() < arrive(i)) {
+            wait_until(t0, arrive(i));
+          }
+          auto factory = std::make_shared<cmi::hip::HipBackendFactory>(a.ctx->ctx);
+          std::optional<double> known;
+          if (a.o->has_known_primal) {
+            known = a.o->known_primal;
+          }
+          cmi::search::SearchDriver drv(pol, factory, a.o->iter_limit, a.o->tolerance, a.o->sample_points,
+                                        a.o->budget_bytes, a.o->host_budget_bytes,
+                                        a.o->compact_frontier ? cmi::search::FrontierPolicy::Compact
+                                                              : cmi::search::FrontierPolicy::StopAtBudget,
+                                        known);
+          auto const res = drv.solve(a.problems[i]->problem);
+          double const t_done = elapsed();
+          iters += res.counters.iterations;
+          cmi::capi::fill_outcome_public(res, t_done - arrive(i), &a.outs[i], nullptr,
+                                         a.problems[i]->problem.box_bounds.size());
+          if (a.finish != nullptr) {
+            a.finish[i] = t_done;
+          }
+          if (a.latency != nullptr) {
+            a.latency[i] = t_done - arrive(i);
+          }
+        }
+        if (a.stats != nullptr) {
+          a.stats->steps = iters;
+          a.stats->launches = 0;
+          a.stats->peak_active = a.n > 0 ? 1 : 0;
+          a.stats->wall_seconds = elapsed();
+        }
+      },
+      &a);
+}

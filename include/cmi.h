@@ -1,0 +1,365 @@
+/*
+ * cmi.h -- C-ABI of the MI355X interval branch-and-bound engine.
+ *
+ * Plain C: opaque handles, plain pointers and sizes, status codes instead of
+ * exceptions. Every entry point is the flat form of one reference interface
+ * on the hot path; each declaration cites the reference file:line it replaces
+ * (paths relative to the reference repository root).
+ *
+ * Ownership: handles are created by *_create and released by the matching
+ * *_destroy. View structs (cmi_bound_view, cmi_candidate_view) point into
+ * storage owned by the role and stay valid until the next call on that role,
+ * as the reference's result spans do (include/cuminlp/backend/backend.hpp:49-50).
+ * Threading: a handle is used by one thread at a time; distinct handles on the
+ * same context may be used from different threads only if serialised by the
+ * caller (all device work of a context is ordered on its one stream).
+ * Errors: a non-CMI_OK return leaves the reason in cmi_last_error() (per thread).
+ */
+#ifndef CMI_H
+#define CMI_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* == cu::interval<double> layout (16 B), reference kernels.cuh:426-427 */
+typedef struct
+{
+  double lb;
+  double ub;
+} cmi_ivl;
+
+/* Status codes: one per reference exception class (errors.hpp:11-57,
+ * backend.hpp:236-287, cuda_utils.cuh:22-35). */
+typedef enum
+{
+  CMI_OK = 0,
+  CMI_SHAPE_MISMATCH = 1,   /* cuminlp::ShapeMismatch */
+  CMI_INVALID_CONFIG = 2,   /* cuminlp::InvalidConfiguration */
+  CMI_OVER_BUDGET = 3,      /* backend::OverBudgetError / ResourceExhausted */
+  CMI_DEVICE_ERROR = 4,     /* CUDAError -> a HIP runtime failure */
+  CMI_INVALID_HANDLE = 5,   /* null or destroyed handle (moved-from role) */
+  CMI_INVALID_DAG = 6,      /* cuminlp::InvalidDAG */
+  CMI_INVALID_PROBLEM = 7,  /* cuminlp::InvalidProblem */
+  CMI_ERROR = 8             /* any other failure */
+} cmi_status;
+
+/* model::Op, reference model/dag.hpp:12-33 (same order) */
+enum
+{
+  CMI_OP_VAR = 0,
+  CMI_OP_CONST,
+  CMI_OP_ADD,
+  CMI_OP_SUB,
+  CMI_OP_MUL,
+  CMI_OP_DIV,
+  CMI_OP_SQR,
+  CMI_OP_NEG,
+  CMI_OP_EXP,
+  CMI_OP_LOG,
+  CMI_OP_SQRT,
+  CMI_OP_SIN,
+  CMI_OP_COS,
+  CMI_OP_TANH,
+  CMI_OP_POWN,
+  CMI_OP_POW,
+  CMI_OP_ABS,
+  CMI_OP_MIN,
+  CMI_OP_MAX
+};
+
+/* model::VarKind and model::Cmp, reference model/problem.hpp:15-26 */
+enum
+{
+  CMI_CONTINUOUS = 0,
+  CMI_INTEGER = 1,
+  CMI_BINARY = 2
+};
+enum
+{
+  CMI_LE = 0,
+  CMI_EQ = 1
+};
+
+/* region::SlotKind, reference region/slot.hpp:12-18 */
+enum
+{
+  CMI_SLOT_CONTINUOUS = 0,
+  CMI_SLOT_INTEGER_PARTITION = 1,
+  CMI_SLOT_INTEGER_ENUMERATE = 2,
+  CMI_SLOT_BINARY_ENUMERATE = 3
+};
+
+/* the three backend roles, reference backend/backend.hpp:81-116 */
+enum
+{
+  CMI_ROLE_SAMPLER = 0,
+  CMI_ROLE_BOUNDER = 1,
+  CMI_ROLE_ENUMERATOR = 2
+};
+
+/* search::StopReason, reference search/budget.hpp:21-30 */
+enum
+{
+  CMI_STOP_CONVERGED = 0,
+  CMI_STOP_EXHAUSTED,
+  CMI_STOP_ITERATION_LIMIT,
+  CMI_STOP_HOST_MEMORY,
+  CMI_STOP_ALLOCATION_FAILURE,
+  CMI_STOP_DEVICE_MEMORY
+};
+
+typedef struct cmi_problem cmi_problem;
+typedef struct cmi_ctx cmi_ctx;
+typedef struct cmi_tape cmi_tape;
+typedef struct cmi_role cmi_role;
+typedef struct cmi_server cmi_server;
+
+/* A Problem as flat arrays: the reference's Problem<double> (model/problem.hpp:
+ * 195-358) with its ExprDAG (model/dag.hpp:58-93) spelled node by node. in0/in1
+ * are read only for ops of arity >= 1 / 2; constant/var_index/int_exp only for
+ * Const/Var/PowN nodes (any of the three may be NULL if no node needs it). */
+typedef struct
+{
+  size_t n_nodes;
+  const uint8_t* op;
+  const uint64_t* in0;
+  const uint64_t* in1;
+  const double* constant;
+  const uint64_t* var_index;
+  const int32_t* int_exp;
+  size_t n_vars;
+  const cmi_ivl* box;
+  const uint8_t* var_kind;
+  uint64_t objective_root;
+  size_t n_constraints;
+  const uint64_t* con_root;
+  const uint8_t* con_cmp;
+  const double* con_rhs;
+} cmi_problem_desc;
+
+/* Seeded synthetic QCQP generator (no reference counterpart; it builds the
+ * benchmark instances). See qcqp.hpp for the construction. */
+typedef struct
+{
+  size_t n, m, n_int;
+  double cont_lb, cont_ub, int_lb, int_ub;
+  double pair_density, term_prob, lin_prob;
+  int symmetric_pairs;
+  double rhs_lo, rhs_hi;
+  size_t target_nodes; /* > 0: choose term_prob so the reachable node count is closest to this */
+} cmi_qcqp_spec;
+
+typedef struct
+{
+  size_t n_nodes;
+  size_t n_vars;
+  size_t n_constraints;
+  size_t reachable_nodes; /* reference backend/graph/cost.hpp:38-68 buffer_node_count */
+} cmi_problem_info;
+
+typedef struct
+{
+  size_t n_instr;
+  size_t n_slots;
+  size_t n_vars;
+  size_t n_outputs;
+  size_t reachable_nodes;
+  uint64_t flops_per_box; /* algorithmic FP64 ops per box of the interval tape */
+  size_t bytes_per_box;   /* algorithmic HBM bytes per explicit box: 16 n_vars + 16 + 1 */
+} cmi_tape_info;
+
+/* Affine device cost per role, reference backend/cost_model.hpp:26-76 */
+typedef struct
+{
+  size_t sampler_fixed, sampler_per_region;
+  size_t bounder_fixed, bounder_per_region;
+  size_t enumerator_fixed, enumerator_per_region;
+} cmi_cost_model;
+
+/* OverBudget facts, reference backend/backend.hpp:236-262 */
+typedef struct
+{
+  char role[32];
+  size_t needed_bytes;
+  size_t budget_bytes;
+  size_t n_regions;
+  size_t bytes_per_element;
+} cmi_over_budget;
+
+/* BoundResult, reference backend/backend.hpp:68-74 */
+typedef struct
+{
+  const uint8_t* feasible;
+  const double* obj_lb;
+  size_t n_regions;
+} cmi_bound_view;
+
+/* CandidateResult, reference backend/backend.hpp:57-64 */
+typedef struct
+{
+  int found;
+  double value;
+  const double* point; /* n_vars values; all NaN when !found */
+  int64_t index;
+} cmi_candidate_view;
+
+/* SearchDriver arguments, reference search/driver.hpp:90-117 plus the
+ * GreedyEnumerate policy's FanOutSpec/SearchCalibration (policy/greedy_enum.hpp:
+ * 38-44, region/fan_out.hpp:22-57). */
+typedef struct
+{
+  size_t partition_num;   /* FanOutSpec partition width (>= 2) */
+  size_t enumerate_cap;   /* FanOutSpec enumerate cap (>= 1) */
+  size_t max_cycle_size;  /* slots per iteration, 0 = 64 */
+  size_t sample_points;   /* points per region for the sampler */
+  uint32_t iter_limit;
+  double tolerance;
+  size_t budget_bytes;      /* device bytes per role build, 0 = free memory */
+  size_t host_budget_bytes; /* 0 = half of MemAvailable */
+  int compact_frontier;     /* FrontierPolicy::Compact instead of StopAtBudget */
+  int has_known_primal;
+  double known_primal;
+} cmi_solve_options;
+
+/* SolveOutcome + RunTelemetry, reference search/driver.hpp:36-54 */
+typedef struct
+{
+  double lower_bound;
+  double upper_bound;
+  int stop_reason;
+  int proven_optimal;
+  int infeasible;
+  int found_incumbent;
+  uint64_t iterations;
+  uint64_t fathomed;
+  uint64_t bounded;
+  uint64_t sampled;
+  uint64_t enqueued;
+  uint64_t pruned_infeasible;
+  uint64_t pruned_dominated;
+  uint64_t pending;
+  double wall_seconds;
+} cmi_solve_outcome;
+
+const char* cmi_last_error(void);
+const char* cmi_version(void);
+
+/* -- problems (host) --------------------------------------------------------- */
+
+/* Build and validate a Problem; replaces Problem construction + validate()
+ * (model/problem.hpp:209-351). */
+cmi_status cmi_problem_create(const cmi_problem_desc* desc, cmi_problem** out);
+cmi_status cmi_problem_qcqp(const cmi_qcqp_spec* spec, uint64_t seed, cmi_problem** out);
+cmi_status cmi_problem_info_get(const cmi_problem* p, cmi_problem_info* out);
+/* Copy the problem back out as flat arrays; every array argument may be NULL
+ * and must otherwise hold n_nodes / n_vars / n_constraints entries. */
+cmi_status cmi_problem_export(const cmi_problem* p, uint8_t* op, uint64_t* in0, uint64_t* in1, double* constant,
+                              uint64_t* var_index, int32_t* int_exp, cmi_ivl* box, uint8_t* var_kind,
+                              uint64_t* objective_root, uint64_t* con_root, uint8_t* con_cmp, double* con_rhs);
+void cmi_problem_destroy(cmi_problem* p);
+
+/* -- device context and tapes -------------------------------------------------- */
+
+cmi_status cmi_ctx_create(int device, cmi_ctx** out);
+void cmi_ctx_destroy(cmi_ctx* ctx);
+
+/* Compile the problem's DAG to a tape and upload it; replaces GraphBuilder's
+ * DAG walk (backend/graph/builder.cuh:261-514). */
+cmi_status cmi_tape_create(cmi_ctx* ctx, const cmi_problem* p, cmi_tape** out);
+cmi_status cmi_tape_info_get(const cmi_tape* t, cmi_tape_info* out);
+void cmi_tape_destroy(cmi_tape* t);
+/* Compile the tape on the host only (no device) and report its shape; also
+ * writes the listing into `listing` (up to `cap` bytes, NUL-terminated) when
+ * non-NULL. Same checks and errors as cmi_tape_create. */
+cmi_status cmi_tape_compile_info(const cmi_problem* p, cmi_tape_info* out, char* listing, size_t cap);
+
+/* Evaluate explicit boxes (box e, variable v: [lb[v*ld+e], ub[v*ld+e]]):
+ * feasible[e] and obj_lb[e] as the bounder defines them (kernels.cuh:325-372),
+ * obj_ub[e] and roots[k*n_boxes+e] (k = 0 objective, 1.. constraints) when
+ * non-NULL. Host pointers; synchronous. */
+cmi_status cmi_eval_boxes(cmi_tape* t, const double* lb, const double* ub, size_t ld, size_t n_boxes,
+                          uint8_t* feasible, double* obj_lb, double* obj_ub, cmi_ivl* roots);
+/* Same with device pointers, enqueued on `stream` (a hipStream_t, NULL = the
+ * context's stream); returns without synchronising. */
+cmi_status cmi_eval_boxes_device(cmi_tape* t, const double* lb, const double* ub, size_t ld, size_t n_boxes,
+                                 uint8_t* feasible, double* obj_lb, double* obj_ub, cmi_ivl* roots, void* stream);
+/* Round-to-nearest point evaluation (the point graph's op path, ops.cuh
+ * apply(T,T) overloads): x[v*ld+e]; host pointers. */
+cmi_status cmi_eval_points(cmi_tape* t, const double* x, size_t ld, size_t n_points, uint8_t* feasible, double* obj,
+                           double* roots);
+
+/* -- backend roles ------------------------------------------------------------ */
+
+/* RegionBackendFactory::cost_model, backend/backend.hpp:200-201 */
+cmi_status cmi_cost_model_get(const cmi_tape* t, size_t samples_per_region, cmi_cost_model* out);
+/* RegionBackendFactory::build_subdivision for one role, backend/backend.hpp:213-218.
+ * On CMI_OVER_BUDGET `facts` (if non-NULL) is filled and nothing was allocated. */
+cmi_status cmi_role_build(cmi_tape* t, int role_kind, const uint8_t* slot_kinds, size_t slot_count, size_t n_regions,
+                          size_t samples_per_region, size_t budget_bytes, cmi_role** out, cmi_over_budget* facts);
+/* RegionBounder::bound, backend/backend.hpp:99 */
+cmi_status cmi_bound(cmi_role* r, const cmi_ivl* box, size_t n_vars, const size_t* var_ids, const size_t* widths,
+                     size_t slot_count, cmi_bound_view* out);
+/* RegionSampler::sample, backend/backend.hpp:87 */
+cmi_status cmi_sample(cmi_role* r, const cmi_ivl* box, size_t n_vars, const size_t* var_ids, const size_t* widths,
+                      size_t slot_count, uint64_t salt, cmi_candidate_view* out);
+/* RegionEnumerator::enumerate, backend/backend.hpp:114 */
+cmi_status cmi_enumerate(cmi_role* r, const cmi_ivl* box, size_t n_vars, const size_t* var_ids,
+                         const size_t* widths, size_t slot_count, cmi_candidate_view* out);
+size_t cmi_role_size(const cmi_role* r); /* n_regions / n_samples / n_points */
+void cmi_role_destroy(cmi_role* r);
+
+/* -- search ------------------------------------------------------------------- */
+
+void cmi_solve_options_default(cmi_solve_options* o);
+/* SearchDriver<double>::solve with GreedyEnumerate over the device roles,
+ * search/driver.hpp:137-537. best_point (n_vars) may be NULL. */
+cmi_status cmi_solve(cmi_ctx* ctx, const cmi_problem* p, const cmi_solve_options* o, cmi_solve_outcome* out,
+                     double* best_point);
+/* The same solves for many problems, interleaved: every step runs one search
+ * iteration of every unfinished solve in one sampler launch and one bounder
+ * launch (the batched form of AggregateBounder::bound_children's span API,
+ * aggregate/bound.hpp:85-100). Per-solve results equal cmi_solve's. */
+cmi_status cmi_solve_batch(cmi_ctx* ctx, size_t n, const cmi_problem* const* problems, const cmi_solve_options* o,
+                           cmi_solve_outcome* outs, double* const* best_points);
+
+/* -- serving (no reference counterpart: the reference solves one problem per
+ * process; these run a request stream through the same searches) ----------- */
+
+typedef struct
+{
+  size_t max_active;       /* solves on the device at once (continuous-batching slots), >= 1 */
+  size_t hbm_budget_bytes; /* admission budget for the admitted solves' device state, 0 = unlimited */
+  int realtime;            /* 1: request i arrives arrival[i] s after the call starts; 0: all at t = 0 */
+} cmi_server_options;
+
+typedef struct
+{
+  uint64_t steps;       /* lock-step iterations run */
+  uint64_t launches;    /* kernel launches issued */
+  size_t peak_active;   /* most solves resident at once */
+  double wall_seconds;  /* first admission to last completion */
+} cmi_serve_stats;
+
+/* Continuous batching: requests are admitted as they arrive while a slot and
+ * the HBM budget are free; every step advances every resident solve by one
+ * search iteration in one launch per role kind; a finished solve frees its
+ * slot at once. finish[i] (seconds since the call started) and latency[i]
+ * (finish - arrival) may be NULL. Per-request results equal cmi_solve's. */
+cmi_status cmi_serve_stream(cmi_ctx* ctx, size_t n, const cmi_problem* const* problems, const double* arrival,
+                            const cmi_solve_options* o, const cmi_server_options* so, cmi_solve_outcome* outs,
+                            double* finish, double* latency, cmi_serve_stats* stats);
+/* The plain baseline of the same stream: one request at a time, in arrival
+ * order, each through cmi_solve's one-launch-per-role driver. */
+cmi_status cmi_serve_plain(cmi_ctx* ctx, size_t n, const cmi_problem* const* problems, const double* arrival,
+                           const cmi_solve_options* o, cmi_solve_outcome* outs, double* finish, double* latency,
+                           cmi_serve_stats* stats);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* CMI_H */

@@ -1,0 +1,135 @@
+"""ctypes binding of the CPU oracle (oracle/liboracle.so) -- test infrastructure.
+
+Only tests/, the smoke check and the benchmark's CPU-baseline leg import this.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import pathlib
+
+import numpy as np
+
+SO = pathlib.Path(__file__).resolve().parent / "liboracle.so"
+
+_lib = None
+
+
+def lib() -> C.CDLL:
+    global _lib
+    if _lib is None:
+        if not SO.exists():
+            raise RuntimeError(f"{SO} missing: run `make -C oracle`")
+        h = C.CDLL(str(SO))
+        d = C.c_double
+        P = C.c_void_p
+        for name in ("orc_add_rd", "orc_add_ru", "orc_sub_rd", "orc_sub_ru", "orc_mul_rd", "orc_mul_ru"):
+            getattr(h, name).restype = d
+            getattr(h, name).argtypes = [d, d]
+        for name in ("orc_iv_mul", "orc_iv_mul_case"):
+            getattr(h, name).restype = None
+            getattr(h, name).argtypes = [P, P, P]
+        h.orc_iv_mul_s.restype = None
+        h.orc_iv_mul_s.argtypes = [P, d, P]
+        h.orc_iv_sqr.restype = None
+        h.orc_iv_sqr.argtypes = [P, P]
+        h.orc_slot_bounds.restype = None
+        h.orc_slot_bounds.argtypes = [C.c_int, P, C.c_size_t, C.c_size_t, P]
+        h.orc_sample_hash.restype = C.c_uint64
+        h.orc_sample_hash.argtypes = [C.c_uint64] * 4
+        h.orc_draw.restype = d
+        h.orc_draw.argtypes = [C.c_int, d, d, C.c_uint64, C.c_uint64, C.c_uint64, C.c_uint64]
+        h.orc_last_error.restype = C.c_char_p
+        h.orc_eval_boxes.restype = C.c_int
+        h.orc_eval_boxes.argtypes = [P, P, P, C.c_size_t, C.c_size_t, P, P, P, P]
+        h.orc_eval_points.restype = C.c_int
+        h.orc_eval_points.argtypes = [P, P, C.c_size_t, C.c_size_t, P, P, P]
+        h.orc_bound.restype = C.c_int
+        h.orc_bound.argtypes = [P, P, P, P, P, C.c_size_t, P, P]
+        h.orc_candidate.restype = C.c_int
+        h.orc_candidate.argtypes = [P, P, P, P, P, C.c_size_t, C.c_size_t, C.c_uint64, P, P, P, P]
+        h.orc_solve.restype = C.c_int
+        h.orc_solve.argtypes = [P, C.c_size_t, C.c_size_t, C.c_size_t, C.c_size_t, C.c_uint32, d, P, P]
+        _lib = h
+    return _lib
+
+
+def _ok(rc: int) -> None:
+    if rc != 0:
+        raise RuntimeError("oracle: " + lib().orc_last_error().decode())
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+def iv(fn: str, *args):
+    out = np.zeros(2)
+    arrs = [np.ascontiguousarray(a, np.float64) if isinstance(a, (list, tuple, np.ndarray)) else a for a in args]
+    cargs = [_p(a) if isinstance(a, np.ndarray) else a for a in arrs]
+    getattr(lib(), fn)(*cargs, _p(out))
+    return out
+
+
+def eval_boxes(arrays, lb, ub, roots=False):
+    d = arrays.desc()
+    lb = np.ascontiguousarray(lb, np.float64)
+    ub = np.ascontiguousarray(ub, np.float64)
+    nv, n = lb.shape
+    feas = np.zeros(n, np.uint8)
+    olb = np.zeros(n)
+    oub = np.zeros(n)
+    r = np.zeros((1 + arrays.n_constraints, n, 2)) if roots else None
+    _ok(lib().orc_eval_boxes(C.byref(d), _p(lb), _p(ub), n, n, _p(feas), _p(olb), _p(oub), _p(r)))
+    return feas, olb, oub, r
+
+
+def eval_points(arrays, x, roots=False):
+    d = arrays.desc()
+    x = np.ascontiguousarray(x, np.float64)
+    nv, n = x.shape
+    feas = np.zeros(n, np.uint8)
+    obj = np.zeros(n)
+    r = np.zeros((1 + arrays.n_constraints, n)) if roots else None
+    _ok(lib().orc_eval_points(C.byref(d), _p(x), n, n, _p(feas), _p(obj), _p(r)))
+    return feas, obj, r
+
+
+def bound(arrays, box, var_ids, widths, kinds):
+    d = arrays.desc()
+    box = np.ascontiguousarray(box, np.float64).reshape(-1, 2)
+    vid = np.ascontiguousarray(var_ids, np.uint64)
+    wid = np.ascontiguousarray(widths, np.uint64)
+    kd = np.ascontiguousarray(kinds, np.uint8)
+    n = int(np.prod(wid)) if len(wid) else 1
+    feas = np.zeros(n, np.uint8)
+    olb = np.zeros(n)
+    _ok(lib().orc_bound(C.byref(d), _p(box), _p(vid), _p(wid), _p(kd), len(vid), _p(feas), _p(olb)))
+    return feas, olb
+
+
+def candidate(arrays, box, var_ids, widths, kinds, samples, salt):
+    """samples == 0: the enumerator; otherwise the sampler with that many points per region."""
+    d = arrays.desc()
+    box = np.ascontiguousarray(box, np.float64).reshape(-1, 2)
+    vid = np.ascontiguousarray(var_ids, np.uint64)
+    wid = np.ascontiguousarray(widths, np.uint64)
+    kd = np.ascontiguousarray(kinds, np.uint8)
+    found = C.c_int()
+    value = C.c_double()
+    index = C.c_int64()
+    point = np.zeros(arrays.n_vars)
+    _ok(lib().orc_candidate(C.byref(d), _p(box), _p(vid), _p(wid), _p(kd), len(vid), samples, salt,
+                            C.byref(found), C.byref(value), C.byref(index), _p(point)))
+    return bool(found.value), value.value, index.value, point
+
+
+def solve(arrays, partition_num=2, enumerate_cap=2, max_cycle_size=0, sample_points=1, iter_limit=1000000,
+          tolerance=1e-6):
+    from cuminlpsearch_amd import capi as _c
+
+    d = arrays.desc()
+    out = _c.SolveOutcome()
+    best = np.zeros(arrays.n_vars)
+    _ok(lib().orc_solve(C.byref(d), partition_num, enumerate_cap, max_cycle_size, sample_points, iter_limit,
+                        tolerance, C.byref(out), _p(best)))
+    return out, best

@@ -1,0 +1,1030 @@
+// CPU oracle -- TEST INFRASTRUCTURE ONLY.
+//
+// A plain-C++ restatement of the reference's hot path used as the checker for
+// the gfx950 engine and as the CPU baseline of the benchmark. Nothing in the
+// product links, loads or calls this library; only tests/, the smoke check and
+// the benchmark's cpu_baseline leg do.
+//
+// Directed rounding uses the x86 MXCSR rounding field per operation (the
+// pattern of the reference's test/source/cpu_rounding.hpp:14-24), one mode
+// switch per bound, in a translation unit built with -frounding-math. The DAG
+// is evaluated node by node in id order over the nodes the roots reach, with
+// Const operands consumed by value -- the reference's one-kernel-per-node
+// evaluation (backend/graph/builder.cuh:261-514) restated sequentially; no
+// tape, scheduling or slot allocation is shared with the engine.
+//
+// What pins each part (tests/test_oracle_*.py):
+//   directed add/sub/mul            mpmath golden vectors (prec 53, rnd f/c)
+//   interval Mul                    against the IEEE-1788 case table
+//                                   (orc_iv_mul_case) and brute-force
+//                                   containment of products of sample points
+//   decode                          decode_test.cpp:48-188 known answers
+//   sampler                         closed-form hash values; lattice draws
+//   solves                          published optimum of ex2_1_2 (-213,
+//                                   qps.cu:22-43), exhaustive enumeration of
+//                                   small all-integer problems
+//
+// Reference files restated: backend/graph/ops.cuh:24-355 (op semantics),
+// kernels.cuh:283-461 (feasibility, extract, mask, witness), subregion_sampler.cuh:
+// 53-120 (hash and draws), region/decode.hpp:43-71 and materialise.hpp:22-40
+// (decode), search/{driver,frontier,history,budget,epilogue}.hpp and
+// policy/greedy_enum.hpp (the search).
+
+#include <algorithm>
+#include <cfloat>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <limits>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include <xmmintrin.h>
+
+#include "cmi.h"
+
+namespace
+{
+
+// ---------------------------------------------------------------------------
+// Directed rounding
+// ---------------------------------------------------------------------------
+
+// Each primitive reads its operands through volatile locals after the MXCSR
+// switch and writes its result to a volatile before switching back, and is
+// never inlined: the optimizer can neither share one product between the two
+// rounding directions nor move it across a switch (-frounding-math alone does
+// not stop GCC from doing either).
+
+struct RoundGuard
+{
+  unsigned saved;
+  explicit RoundGuard(unsigned mode)
+      : saved(_MM_GET_ROUNDING_MODE())
+  {
+    _MM_SET_ROUNDING_MODE(mode);
+    asm volatile("" ::: "memory");
+  }
+  ~RoundGuard()
+  {
+    asm volatile("" ::: "memory");
+    _MM_SET_ROUNDING_MODE(saved);
+  }
+};
+
+__attribute__((noinline)) double add_rd(double a, double b)
+{
+  RoundGuard g(_MM_ROUND_DOWN);
+  volatile double va = a, vb = b;
+  volatile double r = va + vb;
+  return r;
+}
+__attribute__((noinline)) double add_ru(double a, double b)
+{
+  RoundGuard g(_MM_ROUND_UP);
+  volatile double va = a, vb = b;
+  volatile double r = va + vb;
+  return r;
+}
+__attribute__((noinline)) double sub_rd(double a, double b)
+{
+  RoundGuard g(_MM_ROUND_DOWN);
+  volatile double va = a, vb = b;
+  volatile double r = va - vb;
+  return r;
+}
+__attribute__((noinline)) double sub_ru(double a, double b)
+{
+  RoundGuard g(_MM_ROUND_UP);
+  volatile double va = a, vb = b;
+  volatile double r = va - vb;
+  return r;
+}
+__attribute__((noinline)) double mul_rd(double a, double b)
+{
+  RoundGuard g(_MM_ROUND_DOWN);
+  volatile double va = a, vb = b;
+  volatile double r = va * vb;
+  return r;
+}
+__attribute__((noinline)) double mul_ru(double a, double b)
+{
+  RoundGuard g(_MM_ROUND_UP);
+  volatile double va = a, vb = b;
+  volatile double r = va * vb;
+  return r;
+}
+
+double plus_zero(double x) { return x == 0.0 ? 0.0 : x; }
+
+int arity(std::uint8_t op)
+{
+  switch (op) {
+    case CMI_OP_VAR:
+    case CMI_OP_CONST:
+      return 0;
+    case CMI_OP_ADD:
+    case CMI_OP_SUB:
+    case CMI_OP_MUL:
+    case CMI_OP_DIV:
+    case CMI_OP_POW:
+    case CMI_OP_MIN:
+    case CMI_OP_MAX:
+      return 2;
+    default:
+      return 1;
+  }
+}
+
+struct Iv
+{
+  double lb, ub;
+};
+
+// ---------------------------------------------------------------------------
+// Interval ops (ops.cuh semantics; Mul/Sqr/scalar Mul report a zero bound as +0)
+// ---------------------------------------------------------------------------
+
+Iv iv_add(Iv a, Iv b) { return {add_rd(a.lb, b.lb), add_ru(a.ub, b.ub)}; }
+Iv iv_sub(Iv a, Iv b) { return {sub_rd(a.lb, b.ub), sub_ru(a.ub, b.lb)}; }
+Iv iv_add_s(Iv a, double c) { return {add_rd(a.lb, c), add_ru(a.ub, c)}; }
+Iv iv_s_add(double c, Iv a) { return {add_rd(c, a.lb), add_ru(c, a.ub)}; }
+Iv iv_sub_s(Iv a, double c) { return {sub_rd(a.lb, c), sub_ru(a.ub, c)}; }
+Iv iv_s_sub(double c, Iv a) { return {sub_rd(c, a.ub), sub_ru(c, a.lb)}; }
+Iv iv_neg(Iv a) { return {-a.ub, -a.lb}; }
+
+// Min/max over the four directed endpoint products.
+Iv iv_mul(Iv x, Iv y)
+{
+  double const l[4] = {mul_rd(x.lb, y.lb), mul_rd(x.lb, y.ub), mul_rd(x.ub, y.lb), mul_rd(x.ub, y.ub)};
+  double const u[4] = {mul_ru(x.lb, y.lb), mul_ru(x.lb, y.ub), mul_ru(x.ub, y.lb), mul_ru(x.ub, y.ub)};
+  double lo = l[0], hi = u[0];
+  for (int k = 1; k < 4; ++k) {
+    lo = l[k] < lo ? l[k] : lo;
+    hi = u[k] > hi ? u[k] : hi;
+  }
+  return {plus_zero(lo), plus_zero(hi)};
+}
+
+// The IEEE-1788 sign-case table (Hickey, Ju & van Emden 2001), used only to
+// pin iv_mul's min/max formulation in the tests.
+Iv iv_mul_case(Iv x, Iv y)
+{
+  bool const xz = x.lb == 0.0 && x.ub == 0.0, yz = y.lb == 0.0 && y.ub == 0.0;
+  if (xz || yz) {
+    return {0.0, 0.0};
+  }
+  bool const xp = x.lb >= 0.0, xn = x.ub <= 0.0, yp = y.lb >= 0.0, yn = y.ub <= 0.0;
+  Iv r;
+  if (xp) {
+    if (yp) {
+      r = {mul_rd(x.lb, y.lb), mul_ru(x.ub, y.ub)};
+    } else if (yn) {
+      r = {mul_rd(x.ub, y.lb), mul_ru(x.lb, y.ub)};
+    } else {
+      r = {mul_rd(x.ub, y.lb), mul_ru(x.ub, y.ub)};
+    }
+  } else if (xn) {
+    if (yp) {
+      r = {mul_rd(x.lb, y.ub), mul_ru(x.ub, y.lb)};
+    } else if (yn) {
+      r = {mul_rd(x.ub, y.ub), mul_ru(x.lb, y.lb)};
+    } else {
+      r = {mul_rd(x.lb, y.ub), mul_ru(x.lb, y.lb)};
+    }
+  } else {
+    if (yp) {
+      r = {mul_rd(x.lb, y.ub), mul_ru(x.ub, y.ub)};
+    } else if (yn) {
+      r = {mul_rd(x.ub, y.lb), mul_ru(x.lb, y.lb)};
+    } else {
+      r = {std::min(mul_rd(x.lb, y.ub), mul_rd(x.ub, y.lb)), std::max(mul_ru(x.lb, y.lb), mul_ru(x.ub, y.ub))};
+    }
+  }
+  return r;
+}
+
+Iv iv_mul_s(Iv a, double c)
+{
+  if (c > 0.0) {
+    return {plus_zero(mul_rd(a.lb, c)), plus_zero(mul_ru(a.ub, c))};
+  }
+  if (c < 0.0) {
+    return {plus_zero(mul_rd(a.ub, c)), plus_zero(mul_ru(a.lb, c))};
+  }
+  return {0.0, 0.0};
+}
+
+Iv iv_sqr(Iv a)
+{
+  if (a.lb >= 0.0) {
+    return {plus_zero(mul_rd(a.lb, a.lb)), mul_ru(a.ub, a.ub)};
+  }
+  if (a.ub <= 0.0) {
+    return {plus_zero(mul_rd(a.ub, a.ub)), mul_ru(a.lb, a.lb)};
+  }
+  return {0.0, std::max(mul_ru(a.lb, a.lb), mul_ru(a.ub, a.ub))};
+}
+
+// ---------------------------------------------------------------------------
+// Problem view
+// ---------------------------------------------------------------------------
+
+struct Prob
+{
+  std::size_t n_nodes, n_vars, n_con;
+  std::vector<std::uint8_t> op;
+  std::vector<std::uint64_t> in0, in1;
+  std::vector<double> cst;
+  std::vector<std::uint64_t> vidx;
+  std::vector<Iv> box;
+  std::vector<std::uint8_t> kind;
+  std::uint64_t obj;
+  std::vector<std::uint64_t> con_root;
+  std::vector<std::uint8_t> con_cmp;
+  std::vector<double> con_rhs;
+  std::vector<char> reach;
+};
+
+Prob load(const cmi_problem_desc* d)
+{
+  Prob p;
+  p.n_nodes = d->n_nodes;
+  p.n_vars = d->n_vars;
+  p.n_con = d->n_constraints;
+  p.op.assign(d->op, d->op + d->n_nodes);
+  p.in0.assign(d->n_nodes, 0);
+  p.in1.assign(d->n_nodes, 0);
+  p.cst.assign(d->n_nodes, 0.0);
+  p.vidx.assign(d->n_nodes, 0);
+  for (std::size_t i = 0; i < d->n_nodes; ++i) {
+    if (d->in0 != nullptr) {
+      p.in0[i] = d->in0[i];
+    }
+    if (d->in1 != nullptr) {
+      p.in1[i] = d->in1[i];
+    }
+    if (d->constant != nullptr) {
+      p.cst[i] = d->constant[i];
+    }
+    if (d->var_index != nullptr) {
+      p.vidx[i] = d->var_index[i];
+    }
+  }
+  for (std::size_t v = 0; v < d->n_vars; ++v) {
+    p.box.push_back({d->box[v].lb, d->box[v].ub});
+    p.kind.push_back(d->var_kind[v]);
+  }
+  p.obj = d->objective_root;
+  for (std::size_t k = 0; k < d->n_constraints; ++k) {
+    p.con_root.push_back(d->con_root[k]);
+    p.con_cmp.push_back(d->con_cmp[k]);
+    p.con_rhs.push_back(d->con_rhs[k]);
+  }
+  p.reach.assign(p.n_nodes, 0);
+  p.reach[p.obj] = 1;
+  for (std::uint64_t r : p.con_root) {
+    p.reach[r] = 1;
+  }
+  for (std::size_t i = p.n_nodes; i-- > 0;) {
+    if (!p.reach[i]) {
+      continue;
+    }
+    int const ar = arity(p.op[i]);
+    if (ar >= 1) {
+      p.reach[p.in0[i]] = 1;
+    }
+    if (ar >= 2) {
+      p.reach[p.in1[i]] = 1;
+    }
+  }
+  return p;
+}
+
+// ---------------------------------------------------------------------------
+// DAG evaluation, node by node (one "kernel" per reachable node)
+// ---------------------------------------------------------------------------
+
+void eval_interval(const Prob& p, const Iv* x, std::vector<Iv>& val)
+{
+  val.resize(p.n_nodes);
+  for (std::size_t i = 0; i < p.n_nodes; ++i) {
+    if (!p.reach[i]) {
+      continue;
+    }
+    std::uint8_t const op = p.op[i];
+    if (op == CMI_OP_VAR) {
+      val[i] = x[p.vidx[i]];
+      continue;
+    }
+    if (op == CMI_OP_CONST) {
+      continue;
+    }
+    std::uint64_t const a = p.in0[i], b = p.in1[i];
+    bool const ca = p.op[a] == CMI_OP_CONST;
+    bool const cb = arity(op) > 1 && p.op[b] == CMI_OP_CONST;
+    double const ka = p.cst[a], kb = arity(op) > 1 ? p.cst[b] : 0.0;
+    switch (op) {
+      case CMI_OP_ADD:
+        val[i] = (ca && cb) ? iv_add({ka, ka}, {kb, kb})
+            : ca            ? iv_s_add(ka, val[b])
+            : cb            ? iv_add_s(val[a], kb)
+                            : iv_add(val[a], val[b]);
+        break;
+      case CMI_OP_SUB:
+        val[i] = (ca && cb) ? iv_sub({ka, ka}, {kb, kb})
+            : ca            ? iv_s_sub(ka, val[b])
+            : cb            ? iv_sub_s(val[a], kb)
+                            : iv_sub(val[a], val[b]);
+        break;
+      case CMI_OP_MUL:
+        val[i] = (ca && cb) ? iv_mul({ka, ka}, {kb, kb})
+            : ca            ? iv_mul_s(val[b], ka)
+            : cb            ? iv_mul_s(val[a], kb)
+                            : iv_mul(val[a], val[b]);
+        break;
+      case CMI_OP_SQR:
+        if (ca) {
+          throw std::runtime_error("unary op on a Const");
+        }
+        val[i] = iv_sqr(val[a]);
+        break;
+      case CMI_OP_NEG:
+        if (ca) {
+          throw std::runtime_error("unary op on a Const");
+        }
+        val[i] = iv_neg(val[a]);
+        break;
+      default:
+        throw std::runtime_error("oracle: op outside the QCQP set");
+    }
+  }
+}
+
+void eval_point(const Prob& p, const double* x, std::vector<double>& val)
+{
+  val.resize(p.n_nodes);
+  for (std::size_t i = 0; i < p.n_nodes; ++i) {
+    if (!p.reach[i]) {
+      continue;
+    }
+    std::uint8_t const op = p.op[i];
+    if (op == CMI_OP_VAR) {
+      val[i] = x[p.vidx[i]];
+      continue;
+    }
+    if (op == CMI_OP_CONST) {
+      val[i] = p.cst[i];
+      continue;
+    }
+    double const a = val[p.in0[i]];
+    double const b = arity(op) > 1 ? val[p.in1[i]] : 0.0;
+    switch (op) {
+      case CMI_OP_ADD: val[i] = a + b; break;
+      case CMI_OP_SUB: val[i] = a - b; break;
+      case CMI_OP_MUL: val[i] = a * b; break;
+      case CMI_OP_SQR: val[i] = a * a; break;
+      case CMI_OP_NEG: val[i] = -a; break;
+      default: throw std::runtime_error("oracle: op outside the QCQP set");
+    }
+  }
+}
+
+bool almost_equal(double a, double b)
+{
+  double const diff = std::fabs(a - b);
+  if (diff <= 1e-8) {
+    return true;
+  }
+  return diff <= 1e-6 * std::max(std::fabs(a), std::fabs(b));
+}
+
+bool feasible_interval(const Prob& p, const std::vector<Iv>& val)
+{
+  bool f = true;
+  for (std::size_t k = 0; k < p.n_con; ++k) {
+    Iv const v = val[p.con_root[k]];
+    double const rhs = p.con_rhs[k];
+    bool const ok = p.con_cmp[k] == CMI_LE ? v.lb <= rhs : (v.lb <= rhs && rhs <= v.ub);
+    f = f && ok;
+  }
+  return f;
+}
+
+bool feasible_point(const Prob& p, const std::vector<double>& val)
+{
+  bool f = true;
+  for (std::size_t k = 0; k < p.n_con; ++k) {
+    double const v = val[p.con_root[k]];
+    double const rhs = p.con_rhs[k];
+    bool const ok = p.con_cmp[k] == CMI_LE ? v <= rhs : almost_equal(v, rhs);
+    f = f && ok;
+  }
+  return f;
+}
+
+// ---------------------------------------------------------------------------
+// Region decode and sampling
+// ---------------------------------------------------------------------------
+
+Iv slot_bounds(std::uint8_t kind, Iv parent, std::size_t part, std::size_t fan_out)
+{
+  if (kind == CMI_SLOT_INTEGER_ENUMERATE || kind == CMI_SLOT_BINARY_ENUMERATE) {
+    double const lo = std::ceil(parent.lb);
+    double v = lo + static_cast<double>(part);
+    if (v > parent.ub) {
+      v = parent.ub;
+    }
+    return {v, v};
+  }
+  double const width = (parent.ub - parent.lb) / static_cast<double>(fan_out);
+  Iv b;
+  b.lb = parent.lb + width * static_cast<double>(part);
+  b.ub = (part + 1 == fan_out) ? parent.ub : parent.lb + width * static_cast<double>(part + 1);
+  return b;
+}
+
+void materialise(const std::vector<Iv>& parent, std::size_t r, const std::size_t* var_ids, const std::size_t* widths,
+                 const std::uint8_t* kinds, std::size_t slots, std::vector<Iv>& out)
+{
+  out = parent;
+  std::size_t idx = r;
+  for (std::size_t j = 0; j < slots; ++j) {
+    std::size_t const part = idx % widths[j];
+    idx /= widths[j];
+    out[var_ids[j]] = slot_bounds(kinds[j], parent[var_ids[j]], part, widths[j]);
+  }
+}
+
+std::uint64_t sample_hash(std::uint64_t vid, std::uint64_t i, std::uint64_t region, std::uint64_t salt)
+{
+  std::uint64_t x = vid * 0x9e3779b97f4a7c15ULL;
+  x ^= i * 0xbf58476d1ce4e5b9ULL;
+  x ^= region * 0x94d049bb133111ebULL;
+  x ^= salt * 0xd6e8feb86659fd93ULL;
+  x += 0x9e3779b97f4a7c15ULL;
+  x = (x ^ (x >> 30)) * 0xbf58476d1ce4e5b9ULL;
+  x = (x ^ (x >> 27)) * 0x94d049bb133111ebULL;
+  x ^= x >> 31;
+  return x;
+}
+
+double draw(bool discrete, double lb, double ub, std::uint64_t vid, std::uint64_t i, std::uint64_t region,
+            std::uint64_t salt)
+{
+  std::uint64_t const x = sample_hash(vid, i, region, salt);
+  if (!discrete) {
+    double const u = static_cast<double>(x >> 11) * (1.0 / 9007199254740992.0);
+    return std::fma(u, ub - lb, lb);
+  }
+  double const lo = std::ceil(lb);
+  double const hi = std::floor(ub);
+  if (hi < lo) {
+    return lo;
+  }
+  std::uint64_t const count = static_cast<std::uint64_t>(hi - lo) + 1;
+  return lo + static_cast<double>(x % count);
+}
+
+struct Cand
+{
+  bool found;
+  double value;
+  std::int64_t index;
+  std::vector<double> point;
+};
+
+// The sampler (S > 0, salted draws) or the enumerator (S == 0: lattice points,
+// one per region): min over feasible elements, lowest index on ties
+// (CUB ArgMin), witness regenerated at the winner.
+Cand points_argmin(const Prob& p, const std::vector<Iv>& box, const std::size_t* var_ids, const std::size_t* widths,
+                   const std::uint8_t* kinds, std::size_t slots, std::size_t n_regions, std::size_t S,
+                   std::uint64_t salt)
+{
+  bool const sampling = S > 0;
+  std::size_t const per = sampling ? S : 1;
+  std::vector<Iv> sub;
+  std::vector<double> x(p.n_vars), val;
+  double best = INFINITY;
+  std::int64_t best_i = -1;
+  std::vector<double> best_x;
+  for (std::size_t r = 0; r < n_regions; ++r) {
+    materialise(box, r, var_ids, widths, kinds, slots, sub);
+    for (std::size_t i = 0; i < per; ++i) {
+      for (std::size_t v = 0; v < p.n_vars; ++v) {
+        x[v] = sampling ? draw(p.kind[v] != CMI_CONTINUOUS, sub[v].lb, sub[v].ub, v, i, r, salt) : sub[v].lb;
+      }
+      eval_point(p, x.data(), val);
+      double m = feasible_point(p, val) ? val[p.obj] : INFINITY;
+      if (m != m) {
+        m = INFINITY;
+      }
+      if (m < best) {
+        best = m;
+        best_i = static_cast<std::int64_t>(r * per + i);
+        best_x = x;
+      }
+    }
+  }
+  Cand c;
+  c.found = best < DBL_MAX;
+  c.value = c.found ? best : DBL_MAX;
+  c.index = c.found ? best_i : -1;
+  c.point = c.found ? best_x : std::vector<double>(p.n_vars, NAN);
+  return c;
+}
+
+// ---------------------------------------------------------------------------
+// Search (GreedyEnumerate + best-first driver)
+// ---------------------------------------------------------------------------
+
+struct Assign
+{
+  std::vector<std::uint8_t> kinds;
+  std::vector<std::size_t> vars, widths;
+};
+
+std::size_t int_domain(Iv b)
+{
+  double const lo = std::ceil(b.lb), hi = std::floor(b.ub);
+  return hi < lo ? 0 : static_cast<std::size_t>(hi - lo) + 1;
+}
+
+Assign choose(const std::vector<Iv>& box, const std::vector<std::uint8_t>& vk, std::size_t part, std::size_t cap,
+              std::size_t max_slots)
+{
+  Assign a;
+  auto full = [&] { return a.vars.size() >= max_slots; };
+  auto push = [&](std::uint8_t k, std::size_t v, std::size_t w) {
+    a.kinds.push_back(k);
+    a.vars.push_back(v);
+    a.widths.push_back(w);
+  };
+  for (std::size_t v = 0; v < vk.size() && !full(); ++v) {
+    if (vk[v] == CMI_BINARY && box[v].ub > box[v].lb) {
+      push(CMI_SLOT_BINARY_ENUMERATE, v, 2);
+    }
+  }
+  std::vector<std::size_t> ints;
+  for (std::size_t v = 0; v < vk.size(); ++v) {
+    if (vk[v] == CMI_INTEGER && box[v].ub > box[v].lb) {
+      ints.push_back(v);
+    }
+  }
+  std::sort(ints.begin(), ints.end(), [&](std::size_t x, std::size_t y) { return int_domain(box[x]) < int_domain(box[y]); });
+  for (std::size_t v : ints) {
+    if (full()) {
+      break;
+    }
+    if (int_domain(box[v]) <= cap) {
+      push(CMI_SLOT_INTEGER_ENUMERATE, v, cap);
+    } else {
+      push(CMI_SLOT_INTEGER_PARTITION, v, part);
+    }
+  }
+  std::vector<std::size_t> cont;
+  for (std::size_t v = 0; v < vk.size(); ++v) {
+    if (vk[v] == CMI_CONTINUOUS && box[v].ub > box[v].lb) {
+      cont.push_back(v);
+    }
+  }
+  std::sort(cont.begin(), cont.end(),
+            [&](std::size_t x, std::size_t y) { return (box[x].ub - box[x].lb) > (box[y].ub - box[y].lb); });
+  for (std::size_t v : cont) {
+    if (full()) {
+      break;
+    }
+    push(CMI_SLOT_CONTINUOUS, v, part);
+  }
+  return a;
+}
+
+struct Node
+{
+  std::size_t sidx, pidx, depth;
+  double lb;
+};
+
+bool before(const Node& a, const Node& b)
+{
+  if (a.lb < b.lb) {
+    return true;
+  }
+  if (b.lb < a.lb) {
+    return false;
+  }
+  if (a.depth != b.depth) {
+    return a.depth > b.depth;
+  }
+  return a.pidx > b.pidx;
+}
+
+struct Heap
+{
+  std::vector<Node> e;
+  std::size_t n = 0;
+  void push(Node x)
+  {
+    if (n == e.size()) {
+      e.resize(e.empty() ? 1 : 2 * e.size());
+    }
+    std::size_t i = n++;
+    while (i > 0 && before(x, e[(i - 1) / 2])) {
+      e[i] = e[(i - 1) / 2];
+      i = (i - 1) / 2;
+    }
+    e[i] = x;
+  }
+  Node pop()
+  {
+    Node top = e[0];
+    --n;
+    if (n > 0) {
+      Node x = e[n];
+      std::size_t i = 0;
+      while (2 * i + 1 < n) {
+        std::size_t c = 2 * i + 1;
+        if (c + 1 < n && before(e[c + 1], e[c])) {
+          ++c;
+        }
+        if (!before(e[c], x)) {
+          break;
+        }
+        e[i] = e[c];
+        i = c;
+      }
+      e[i] = x;
+    }
+    return top;
+  }
+  std::size_t viable(double gub) const
+  {
+    std::size_t c = 0;
+    std::vector<std::size_t> st {0};
+    while (!st.empty()) {
+      std::size_t i = st.back();
+      st.pop_back();
+      if (i >= n || e[i].lb > gub) {
+        continue;
+      }
+      ++c;
+      st.push_back(2 * i + 1);
+      st.push_back(2 * i + 2);
+    }
+    return c;
+  }
+};
+
+struct History
+{
+  std::vector<std::vector<Iv>> box;
+  std::vector<std::size_t> refs, free_list;
+  std::size_t put(std::vector<Iv> b)
+  {
+    if (!free_list.empty()) {
+      std::size_t i = free_list.back();
+      free_list.pop_back();
+      box[i] = std::move(b);
+      refs[i] = 1;
+      return i;
+    }
+    box.push_back(std::move(b));
+    refs.push_back(1);
+    return box.size() - 1;
+  }
+  void ref(std::size_t i)
+  {
+    if (i != 0) {
+      ++refs[i];
+    }
+  }
+  void unref(std::size_t i)
+  {
+    if (i != 0 && --refs[i] == 0) {
+      std::vector<Iv>().swap(box[i]);
+      free_list.push_back(i);
+    }
+  }
+};
+
+bool admissible(const std::vector<double>& x, const Prob& p)
+{
+  if (x.size() != p.n_vars) {
+    return false;
+  }
+  for (std::size_t i = 0; i < x.size(); ++i) {
+    if (!(x[i] >= p.box[i].lb) || !(x[i] <= p.box[i].ub)) {
+      return false;
+    }
+    if (p.kind[i] != CMI_CONTINUOUS && std::fabs(x[i] - std::round(x[i])) > 1e-6) {
+      return false;
+    }
+  }
+  return true;
+}
+
+bool closed(double gub, double bound, double tol) { return gub - bound <= tol * std::max(1.0, std::fabs(gub)); }
+
+}  // namespace
+
+extern "C" {
+
+double orc_add_rd(double a, double b) { return add_rd(a, b); }
+double orc_add_ru(double a, double b) { return add_ru(a, b); }
+double orc_sub_rd(double a, double b) { return sub_rd(a, b); }
+double orc_sub_ru(double a, double b) { return sub_ru(a, b); }
+double orc_mul_rd(double a, double b) { return mul_rd(a, b); }
+double orc_mul_ru(double a, double b) { return mul_ru(a, b); }
+
+void orc_iv_mul(const double* x, const double* y, double* out)
+{
+  Iv const r = iv_mul({x[0], x[1]}, {y[0], y[1]});
+  out[0] = r.lb;
+  out[1] = r.ub;
+}
+void orc_iv_mul_case(const double* x, const double* y, double* out)
+{
+  Iv const r = iv_mul_case({x[0], x[1]}, {y[0], y[1]});
+  out[0] = r.lb;
+  out[1] = r.ub;
+}
+void orc_iv_mul_s(const double* x, double c, double* out)
+{
+  Iv const r = iv_mul_s({x[0], x[1]}, c);
+  out[0] = r.lb;
+  out[1] = r.ub;
+}
+void orc_iv_sqr(const double* x, double* out)
+{
+  Iv const r = iv_sqr({x[0], x[1]});
+  out[0] = r.lb;
+  out[1] = r.ub;
+}
+void orc_slot_bounds(int kind, const double* parent, size_t part, size_t fan_out, double* out)
+{
+  Iv const r = slot_bounds(static_cast<std::uint8_t>(kind), {parent[0], parent[1]}, part, fan_out);
+  out[0] = r.lb;
+  out[1] = r.ub;
+}
+uint64_t orc_sample_hash(uint64_t vid, uint64_t i, uint64_t region, uint64_t salt)
+{
+  return sample_hash(vid, i, region, salt);
+}
+double orc_draw(int discrete, double lb, double ub, uint64_t vid, uint64_t i, uint64_t region, uint64_t salt)
+{
+  return draw(discrete != 0, lb, ub, vid, i, region, salt);
+}
+
+static thread_local std::string g_err;
+const char* orc_last_error(void) { return g_err.c_str(); }
+
+int orc_eval_boxes(const cmi_problem_desc* d, const double* lb, const double* ub, size_t ld, size_t n,
+                   uint8_t* feasible, double* obj_lb, double* obj_ub, double* roots)
+{
+  try {
+    Prob const p = load(d);
+    std::vector<Iv> x(p.n_vars), val;
+    for (std::size_t e = 0; e < n; ++e) {
+      for (std::size_t v = 0; v < p.n_vars; ++v) {
+        x[v] = {lb[v * ld + e], ub[v * ld + e]};
+      }
+      eval_interval(p, x.data(), val);
+      feasible[e] = feasible_interval(p, val) ? 1 : 0;
+      obj_lb[e] = val[p.obj].lb;
+      if (obj_ub != nullptr) {
+        obj_ub[e] = val[p.obj].ub;
+      }
+      if (roots != nullptr) {
+        for (std::size_t k = 0; k <= p.n_con; ++k) {
+          Iv const r = val[k == 0 ? p.obj : p.con_root[k - 1]];
+          roots[2 * (k * n + e)] = r.lb;
+          roots[2 * (k * n + e) + 1] = r.ub;
+        }
+      }
+    }
+    return 0;
+  } catch (const std::exception& ex) {
+    g_err = ex.what();
+    return 1;
+  }
+}
+
+int orc_eval_points(const cmi_problem_desc* d, const double* xs, size_t ld, size_t n, uint8_t* feasible, double* obj,
+                    double* roots)
+{
+  try {
+    Prob const p = load(d);
+    std::vector<double> x(p.n_vars), val;
+    for (std::size_t e = 0; e < n; ++e) {
+      for (std::size_t v = 0; v < p.n_vars; ++v) {
+        x[v] = xs[v * ld + e];
+      }
+      eval_point(p, x.data(), val);
+      feasible[e] = feasible_point(p, val) ? 1 : 0;
+      obj[e] = val[p.obj];
+      if (roots != nullptr) {
+        for (std::size_t k = 0; k <= p.n_con; ++k) {
+          roots[k * n + e] = val[k == 0 ? p.obj : p.con_root[k - 1]];
+        }
+      }
+    }
+    return 0;
+  } catch (const std::exception& ex) {
+    g_err = ex.what();
+    return 1;
+  }
+}
+
+int orc_bound(const cmi_problem_desc* d, const cmi_ivl* box, const size_t* var_ids, const size_t* widths,
+              const uint8_t* kinds, size_t slots, uint8_t* feasible, double* obj_lb)
+{
+  try {
+    Prob const p = load(d);
+    std::vector<Iv> parent(p.n_vars), sub, val;
+    for (std::size_t v = 0; v < p.n_vars; ++v) {
+      parent[v] = {box[v].lb, box[v].ub};
+    }
+    std::size_t n_regions = 1;
+    for (std::size_t j = 0; j < slots; ++j) {
+      n_regions *= widths[j];
+    }
+    for (std::size_t r = 0; r < n_regions; ++r) {
+      materialise(parent, r, var_ids, widths, kinds, slots, sub);
+      eval_interval(p, sub.data(), val);
+      feasible[r] = feasible_interval(p, val) ? 1 : 0;
+      obj_lb[r] = val[p.obj].lb;
+    }
+    return 0;
+  } catch (const std::exception& ex) {
+    g_err = ex.what();
+    return 1;
+  }
+}
+
+// samples == 0: the enumerator (lattice points); otherwise the sampler.
+int orc_candidate(const cmi_problem_desc* d, const cmi_ivl* box, const size_t* var_ids, const size_t* widths,
+                  const uint8_t* kinds, size_t slots, size_t samples, uint64_t salt, int* found, double* value,
+                  int64_t* index, double* point)
+{
+  try {
+    Prob const p = load(d);
+    std::vector<Iv> parent(p.n_vars);
+    for (std::size_t v = 0; v < p.n_vars; ++v) {
+      parent[v] = {box[v].lb, box[v].ub};
+    }
+    std::size_t n_regions = 1;
+    for (std::size_t j = 0; j < slots; ++j) {
+      n_regions *= widths[j];
+    }
+    Cand const c = points_argmin(p, parent, var_ids, widths, kinds, slots, n_regions, samples, salt);
+    *found = c.found ? 1 : 0;
+    *value = c.value;
+    *index = c.index;
+    for (std::size_t v = 0; v < p.n_vars; ++v) {
+      point[v] = c.point[v];
+    }
+    return 0;
+  } catch (const std::exception& ex) {
+    g_err = ex.what();
+    return 1;
+  }
+}
+
+// The best-first search, restated. Host budget: unbounded (the engine's
+// default budget is half of MemAvailable; the small parity solves never reach
+// it).
+int orc_solve(const cmi_problem_desc* d, size_t partition_num, size_t enumerate_cap, size_t max_cycle_size,
+              size_t sample_points, uint32_t iter_limit, double tol, cmi_solve_outcome* out, double* best_point)
+{
+  try {
+    Prob const p = load(d);
+    std::size_t const max_slots = max_cycle_size == 0 ? 64 : max_cycle_size;
+    double gub = DBL_MAX, glb = -DBL_MAX;
+    std::uint32_t iter = 0;
+    std::vector<double> best;
+    Heap pending;
+    pending.e.resize(10000);
+    History hist;
+    hist.put({});
+    pending.push({0, 0, 0, glb});
+    bool converged = false;
+    int stop = CMI_STOP_ITERATION_LIMIT;
+    bool early = false;
+    std::uint64_t fathomed = 0, bounded = 0, sampled = 0, enqueued = 0, p_inf = 0, p_dom = 0;
+    std::vector<Iv> box;
+    while (iter < iter_limit && pending.n > 0 && !closed(gub, glb, tol)) {
+      ++iter;
+      Node const cur = pending.pop();
+      if (closed(gub, cur.lb, tol)) {
+        converged = true;
+        stop = CMI_STOP_CONVERGED;
+        early = true;
+        break;
+      }
+      glb = cur.lb;
+      if (cur.pidx == 0) {
+        box = p.box;
+      } else {
+        std::vector<Iv> const& parent = hist.box[cur.pidx];
+        Assign const pa = choose(parent, p.kind, partition_num, enumerate_cap, max_slots);
+        materialise(parent, cur.sidx, pa.vars.data(), pa.widths.data(), pa.kinds.data(), pa.vars.size(), box);
+      }
+      hist.unref(cur.pidx);
+      Assign const a = choose(box, p.kind, partition_num, enumerate_cap, max_slots);
+      std::size_t live = 0;
+      for (Iv const& b : box) {
+        live += b.ub > b.lb ? 1 : 0;
+      }
+      bool fully = true;
+      for (std::uint8_t k : a.kinds) {
+        fully = fully && (k == CMI_SLOT_INTEGER_ENUMERATE || k == CMI_SLOT_BINARY_ENUMERATE);
+      }
+      std::size_t n_regions = 1;
+      for (std::size_t w : a.widths) {
+        n_regions *= w;
+      }
+      if (live <= a.kinds.size() && fully) {
+        Cand const c = points_argmin(p, box, a.vars.data(), a.widths.data(), a.kinds.data(), a.vars.size(),
+                                     n_regions, 0, 0);
+        if (c.found && c.value < gub && admissible(c.point, p)) {
+          gub = c.value;
+          best = c.point;
+        }
+        ++fathomed;
+        continue;
+      }
+      std::size_t const box_idx = hist.put(box);
+      Cand const c = points_argmin(p, box, a.vars.data(), a.widths.data(), a.kinds.data(), a.vars.size(), n_regions,
+                                   sample_points, iter);
+      sampled += n_regions * sample_points;
+      if (c.value < gub && admissible(c.point, p)) {
+        gub = c.value;
+        best = c.point;
+      }
+      std::vector<Iv> sub, val;
+      for (std::size_t r = 0; r < n_regions; ++r) {
+        materialise(box, r, a.vars.data(), a.widths.data(), a.kinds.data(), a.vars.size(), sub);
+        eval_interval(p, sub.data(), val);
+        ++bounded;
+        if (!feasible_interval(p, val)) {
+          ++p_inf;
+          continue;
+        }
+        double const lb = val[p.obj].lb;
+        if (lb > gub) {
+          ++p_dom;
+          continue;
+        }
+        pending.push({r, box_idx, cur.depth + 1, lb});
+        hist.ref(box_idx);
+        ++enqueued;
+      }
+      hist.unref(box_idx);
+    }
+    if (!early) {
+      stop = pending.n == 0 ? CMI_STOP_EXHAUSTED : closed(gub, glb, tol) ? CMI_STOP_CONVERGED : CMI_STOP_ITERATION_LIMIT;
+    }
+    bool const found = gub < DBL_MAX;
+    std::size_t const viable = pending.viable(gub);
+    bool const proven = found && (converged || pending.n == 0 || viable == 0);
+    bool infeasible = false;
+    double out_glb = glb;
+    if (proven) {
+      out_glb = gub;
+    } else if (pending.n == 0) {
+      infeasible = true;
+    } else {
+      out_glb = pending.e[0].lb;
+      if (found) {
+        out_glb = std::min(out_glb, gub);
+      }
+    }
+    std::memset(out, 0, sizeof(*out));
+    out->lower_bound = out_glb;
+    out->upper_bound = gub;
+    out->stop_reason = stop;
+    out->proven_optimal = proven ? 1 : 0;
+    out->infeasible = infeasible ? 1 : 0;
+    out->found_incumbent = found ? 1 : 0;
+    out->iterations = iter;
+    out->fathomed = fathomed;
+    out->bounded = bounded;
+    out->sampled = sampled;
+    out->enqueued = enqueued;
+    out->pruned_infeasible = p_inf;
+    out->pruned_dominated = p_dom;
+    out->pending = pending.n;
+    if (best_point != nullptr) {
+      for (std::size_t v = 0; v < p.n_vars; ++v) {
+        best_point[v] = v < best.size() ? best[v] : NAN;
+      }
+    }
+    return 0;
+  } catch (const std::exception& ex) {
+    g_err = ex.what();
+    return 1;
+  }
+}
+
+}  // extern "C"

@@ -1,0 +1,163 @@
+"""The three backend roles (bounder, sampler, enumerator) on the device against
+the CPU oracle, plus the reference's role-contract checks: determinism per
+salt, NaN witness when nothing is feasible, lattice draws, over-budget facts
+before allocation, shape errors."""
+import numpy as np
+import pytest
+
+import cuminlpsearch_amd as cmi
+from cuminlpsearch_amd import capi as C
+from cuminlpsearch_amd.model import Problem
+from oracle import binding as orc
+
+import problems as P
+
+pytestmark = pytest.mark.gpu
+
+
+def same_bits(a, b):
+    return np.array_equal(np.asarray(a, np.float64).view(np.uint64), np.asarray(b, np.float64).view(np.uint64))
+
+
+def mixed_problem(seed):
+    """Random DAG with two integer variables, so every slot kind has a target."""
+    return cmi.EngineProblem.from_problem(P.random_dag(seed, n_vars=5, n_ops=50, n_con=3, int_vars=2))
+
+
+CASES = [
+    # (slot kinds, var ids, widths)
+    ([C.SLOT_CONTINUOUS], [2], [4]),
+    ([C.SLOT_CONTINUOUS, C.SLOT_CONTINUOUS], [3, 2], [5, 3]),
+    ([C.SLOT_INTEGER_ENUMERATE, C.SLOT_CONTINUOUS], [0, 4], [7, 6]),
+    ([C.SLOT_INTEGER_PARTITION, C.SLOT_INTEGER_ENUMERATE, C.SLOT_CONTINUOUS], [1, 0, 2], [3, 7, 20]),
+]
+
+
+@pytest.mark.parametrize("case", range(len(CASES)))
+@pytest.mark.parametrize("seed", [1, 2, 3])
+def test_bounder_matches_oracle(ctx, case, seed):
+    kinds, vids, widths = CASES[case]
+    eng = mixed_problem(seed)
+    a = eng.arrays()
+    tape = cmi.Tape(ctx, eng)
+    n = int(np.prod(widths))
+    role = cmi.Role(tape, C.ROLE_BOUNDER, kinds, n)
+    box = a.box
+    f_d, lb_d = role.bound(box, vids, widths)
+    f_o, lb_o = orc.bound(a, box, vids, widths, kinds)
+    assert np.array_equal(f_d, f_o)
+    assert same_bits(lb_d, lb_o)
+
+
+@pytest.mark.parametrize("case", range(len(CASES)))
+@pytest.mark.parametrize("samples", [1, 7, 64])
+def test_sampler_matches_oracle(ctx, case, samples):
+    kinds, vids, widths = CASES[case]
+    eng = mixed_problem(10 + case)
+    a = eng.arrays()
+    tape = cmi.Tape(ctx, eng)
+    n = int(np.prod(widths))
+    role = cmi.Role(tape, C.ROLE_SAMPLER, kinds, n, samples=samples)
+    for salt in (1, 2, 99):
+        d = role.sample(a.box, vids, widths, salt)
+        found, value, index, point = orc.candidate(a, a.box, vids, widths, kinds, samples, salt)
+        assert d.found == found
+        assert same_bits([d.value], [value])
+        assert d.index == index
+        assert same_bits(d.point, point)
+
+
+def test_sampler_determinism_and_salt(ctx):
+    p = Problem()
+    x = p.var(0.0, 10.0)
+    p.set_objective(x * x)
+    p.add_constraint(x, C.LE, 5.0)
+    eng = cmi.EngineProblem.from_problem(p)
+    tape = cmi.Tape(ctx, eng)
+    role = cmi.Role(tape, C.ROLE_SAMPLER, [C.SLOT_CONTINUOUS], 4, samples=32)
+    box = [[0.0, 5.0]]
+    a1 = role.sample(box, [0], [4], 7)
+    a2 = role.sample(box, [0], [4], 7)
+    b = role.sample(box, [0], [4], 8)
+    assert a1.found and same_bits([a1.value], [a2.value]) and same_bits(a1.point, a2.point)
+    assert a1.value != b.value
+    # interval bounds do not depend on the salt
+    bnd = cmi.Role(tape, C.ROLE_BOUNDER, [C.SLOT_CONTINUOUS], 4)
+    f1, l1 = bnd.bound([[0.0, 10.0]], [0], [4])
+    f2, l2 = bnd.bound([[0.0, 10.0]], [0], [4])
+    assert same_bits(l1, l2) and np.array_equal(f1, [1, 1, 0, 0])
+
+
+def test_infeasible_domain_has_nan_witness(ctx):
+    p = Problem()
+    x = p.var(0.0, 10.0)
+    p.set_objective(x * x)
+    p.add_constraint(x, C.LE, -100.0)
+    eng = cmi.EngineProblem.from_problem(p)
+    role = cmi.Role(cmi.Tape(ctx, eng), C.ROLE_SAMPLER, [C.SLOT_CONTINUOUS], 4, samples=16)
+    r = role.sample([[0.0, 10.0]], [0], [4], 1)
+    assert not r.found
+    assert r.value == np.finfo(np.float64).max
+    assert np.all(np.isnan(r.point))
+
+
+def test_lattice_draws(ctx):
+    """Integer/binary samples sit on the lattice (reference sample_discrete_test.cu:173-202)."""
+    p = Problem()
+    x = p.var(0.0, 10.0)
+    k = p.int_var(3.0, 9.0)
+    b = p.bin_var()
+    p.set_objective(x + k + b)
+    eng = cmi.EngineProblem.from_problem(p)
+    tape = cmi.Tape(ctx, eng)
+    role = cmi.Role(tape, C.ROLE_SAMPLER, [C.SLOT_CONTINUOUS], 4, samples=50)
+    for salt in range(20):
+        r = role.sample(eng.arrays().box, [0], [4], salt)
+        assert r.found
+        assert r.point[1] == np.floor(r.point[1]) and 3 <= r.point[1] <= 9
+        assert r.point[2] in (0.0, 1.0)
+        assert 0.0 <= r.point[0] <= 10.0
+
+
+def test_enumerator_matches_oracle_and_brute_force(ctx):
+    """All-integer problem: the enumerator's value is the exact minimum over the lattice."""
+    p = Problem()
+    xs = [p.int_var(0, 4), p.int_var(-2, 3), p.int_var(1, 5)]
+    obj = 3.0 * xs[0] * xs[1] - 2.0 * (xs[2] * xs[2]) + xs[0] * xs[2] - 0.5 * xs[1]
+    p.set_objective(obj)
+    p.add_constraint(xs[0] + xs[1] + xs[2], C.LE, 6.0)
+    p.add_constraint(xs[0] * xs[1], C.EQ, 2.0)
+    eng = cmi.EngineProblem.from_problem(p)
+    a = eng.arrays()
+    kinds = [C.SLOT_INTEGER_ENUMERATE] * 3
+    widths = [5, 6, 5]
+    role = cmi.Role(cmi.Tape(ctx, eng), C.ROLE_ENUMERATOR, kinds, 150)
+    d = role.enumerate(a.box, [0, 1, 2], widths)
+    found, value, index, point = orc.candidate(a, a.box, [0, 1, 2], widths, kinds, 0, 0)
+    assert d.found == found and same_bits([d.value], [value]) and d.index == index and same_bits(d.point, point)
+    best = min(3.0 * i * j - 2.0 * (k * k) + i * k - 0.5 * j
+               for i in range(0, 5) for j in range(-2, 4) for k in range(1, 6) if i + j + k <= 6 and i * j == 2)
+    assert d.value == best
+
+
+def test_over_budget_is_reported_before_allocation(ctx):
+    eng = P.c1(0)
+    tape = cmi.Tape(ctx, eng)
+    with pytest.raises(C.CmiError) as ei:
+        cmi.Role(tape, C.ROLE_BOUNDER, [C.SLOT_CONTINUOUS] * 2, 400, budget_bytes=16)
+    assert ei.value.status == C.OVER_BUDGET
+
+
+def test_shape_errors(ctx):
+    eng = P.c1(0)
+    tape = cmi.Tape(ctx, eng)
+    role = cmi.Role(tape, C.ROLE_BOUNDER, [C.SLOT_CONTINUOUS], 4)
+    with pytest.raises(C.CmiError) as ei:
+        role.bound(eng.arrays().box[:3], [0], [4])
+    assert ei.value.status == C.SHAPE_MISMATCH
+    with pytest.raises(C.CmiError) as ei:
+        role.bound(eng.arrays().box, [0], [5])
+    assert ei.value.status == C.SHAPE_MISMATCH
+    with pytest.raises(C.CmiError) as ei:
+        cmi.Role(tape, C.ROLE_ENUMERATOR, [C.SLOT_CONTINUOUS], 4)
+    assert ei.value.status == C.INVALID_CONFIG

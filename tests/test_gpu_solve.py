@@ -1,0 +1,75 @@
+"""Whole searches on the device against the oracle's restated search: the same
+iterates (counts, bracket, incumbent) bit for bit, the published optimum of
+ex2_1_2 inside the bracket, and batched / served solves equal to plain ones."""
+import numpy as np
+import pytest
+
+import cuminlpsearch_amd as cmi
+from oracle import binding as orc
+
+import problems as P
+
+pytestmark = pytest.mark.gpu
+
+FIELDS = ("iterations", "fathomed", "bounded", "sampled", "enqueued", "pruned_infeasible", "pruned_dominated",
+          "pending", "stop_reason", "proven_optimal", "infeasible", "found_incumbent")
+
+
+def same_outcome(d, o):
+    for f in FIELDS:
+        assert getattr(d, f) == getattr(o, f), f
+    assert np.float64(d.lower_bound).tobytes() == np.float64(o.lower_bound).tobytes()
+    assert np.float64(d.upper_bound).tobytes() == np.float64(o.upper_bound).tobytes()
+
+
+C1_OPTS = dict(partition_num=4, enumerate_cap=4, max_cycle_size=2, sample_points=32, iter_limit=400)
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_c1_solve_matches_oracle(ctx, seed):
+    eng = P.c1(seed)
+    d, best_d = cmi.solve(ctx, eng, **C1_OPTS)
+    o, best_o = orc.solve(eng.arrays(), **C1_OPTS)
+    same_outcome(d, o)
+    assert np.array_equal(best_d.view(np.uint64), best_o.view(np.uint64))
+
+
+def test_ex2_1_2_brackets_published_optimum(ctx):
+    """qps.cu:98-162 settings: FanOut 20, max_cycle_size 2, 100 samples, tol 1e-6."""
+    eng = cmi.EngineProblem.from_problem(P.ex2_1_2())
+    opts = dict(partition_num=20, enumerate_cap=20, max_cycle_size=2, sample_points=100, iter_limit=600)
+    d, best = cmi.solve(ctx, eng, **opts)
+    o, _ = orc.solve(eng.arrays(), **opts)
+    same_outcome(d, o)
+    assert d.lower_bound <= -213.0 <= d.upper_bound
+    assert d.upper_bound - (-213.0) < 1.0
+
+
+def test_batch_equals_plain(ctx):
+    probs = [P.c1(s) for s in range(8)]
+    outs, bests = cmi.solve_batch(ctx, probs, **C1_OPTS)
+    for p, d, b in zip(probs, outs, bests):
+        s, sb = cmi.solve(ctx, p, **C1_OPTS)
+        same_outcome(d, s)
+        assert np.array_equal(b.view(np.uint64), sb.view(np.uint64))
+
+
+def test_mixed_size_batch_equals_plain(ctx):
+    probs = [cmi.EngineProblem.qcqp(s, n=n, m=n // 2, cont_lb=-5.0, cont_ub=5.0, pair_density=0.15)
+             for s, n in enumerate([4, 9, 5, 13, 6, 4, 17])]
+    opts = dict(partition_num=2, enumerate_cap=2, max_cycle_size=3, sample_points=8, iter_limit=60)
+    outs, _ = cmi.solve_batch(ctx, probs, **opts)
+    for p, d in zip(probs, outs):
+        s, _ = cmi.solve(ctx, p, **opts)
+        same_outcome(d, s)
+
+
+def test_served_equals_plain(ctx):
+    probs = [P.c1(s) for s in range(10)]
+    arrival = np.linspace(0.0, 0.02, len(probs))
+    outs, fin, lat, stats = cmi.serve_stream(ctx, probs, arrival, max_active=4, **C1_OPTS)
+    for p, d in zip(probs, outs):
+        s, _ = cmi.solve(ctx, p, **C1_OPTS)
+        same_outcome(d, s)
+    assert stats.peak_active <= 4
+    assert np.all(lat >= 0)
