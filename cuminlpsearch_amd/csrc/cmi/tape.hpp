@@ -2,9 +2,9 @@
 
 #include <algorithm>
 #include <cstddef>
+#include <cstdint>
 #include <cstdio>
 #include <functional>
-#include <cstdint>
 #include <string>
 #include <vector>
 
@@ -15,20 +15,28 @@
 // that one device lane interprets per box (or per sample point). It replaces
 // the reference's one-kernel-per-DAG-node CUDA graph (backend/graph/
 // builder.cuh:261-514) while evaluating exactly the same node operations:
-// every reachable non-Const node becomes one instruction applying that node's
-// Op to its operands, a Const operand is an immediate consumed by value (as
+// every reachable non-Const node is one interval operation applied to that
+// node's operands, a Const operand is an immediate consumed by value (as
 // wire_binary passes it, builder.cuh:412-455), and Const (+) Const is
 // evaluated on point intervals (binary_op_scalar_scalar_kernel,
 // kernels.cuh:240-253).
 //
 // Evaluation order among independent nodes does not change any node's value,
-// so the compiler is free to schedule: roots are walked depth-first in
-// objective-then-constraint order, which turns each left-deep sum into an
-// accumulator chain, and "leaf" nodes (all operands Var/Const) are
-// re-evaluated at each use instead of being held live, which bounds the live
-// set by the variables plus a few temporaries. Slots are recycled on last use.
-// Constraint checks and the objective read-out are tape instructions too, so a
-// root slot dies as soon as it has been checked.
+// so the compiler schedules for the interpreter:
+//
+//  * Left-deep sums and differences (the shape every polynomial takes) run in
+//    an accumulator register: the chain's running value never touches LDS,
+//    and each term -- a variable or shared value, its square, a product of two,
+//    optionally times a constant -- is evaluated inside the accumulate
+//    instruction that consumes it. Each term instruction performs exactly the
+//    node operations of the DAG nodes it covers, in DAG order.
+//  * "Leaf" nodes (all operands Var/Const) are re-evaluated at each use
+//    instead of being held live, which bounds the live set by the variables
+//    plus a few temporaries.
+//  * Values are emitted into virtual registers and mapped onto LDS slots by a
+//    linear scan afterwards (variables are pinned to slots [0, n_vars)).
+//  * Constraint checks and the objective read-out are tape instructions, so a
+//    root dies as soon as it has been checked.
 namespace cmi::tape
 {
 
@@ -49,28 +57,63 @@ enum class TOp : std::uint8_t
   CheckEQ,    // feasible &= a == imm (interval: lb <= imm <= ub; point: almost_equal)
   Objective,  // objective <- a
   Output,     // roots_out[b] <- a (verification mode only)
+  // Accumulator register (ACC) instructions; the term instructions follow.
+  AccStore = 16,  // dst <- ACC
+  AccAddImm,      // ACC <- ACC + imm
+  AccSubImm,      // ACC <- ACC - imm
+  AccImmSub,      // ACC <- imm - ACC
+  AccScale,       // ACC <- ACC * imm
+  AccNeg,         // ACC <- -ACC
+  AccCheckLE,     // feasible &= ACC <= imm
+  AccCheckEQ,     // feasible &= ACC == imm
+  AccObjective,   // objective <- ACC
+  AccOutput,      // roots_out[b] <- ACC
   Count
 };
 
-inline const char* top_name(TOp op)
+// Term instructions: op = kTermBase + mode * 8 + term.
+inline constexpr std::uint8_t kTermBase = 64;
+enum class Term : std::uint8_t
 {
-  switch (op) {
-    case TOp::Nop: return "nop";
-    case TOp::LoadConst: return "ldc";
-    case TOp::Add: return "add";
-    case TOp::AddImm: return "addi";
-    case TOp::Sub: return "sub";
-    case TOp::SubImm: return "subi";
-    case TOp::ImmSub: return "isub";
-    case TOp::Mul: return "mul";
-    case TOp::MulImm: return "muli";
-    case TOp::Sqr: return "sqr";
-    case TOp::Neg: return "neg";
-    case TOp::CheckLE: return "chkle";
-    case TOp::CheckEQ: return "chkeq";
-    case TOp::Objective: return "obj";
-    case TOp::Output: return "out";
-    case TOp::Count: break;
+  S = 0,    // a
+  SC = 1,   // a * imm
+  SS = 2,   // a * b
+  Q = 3,    // a * a (Sqr)
+  SSC = 4,  // (a * b) * imm
+  QC = 5,   // (a * a) * imm
+};
+inline constexpr std::uint8_t kTerms = 6;
+enum class AccMode : std::uint8_t
+{
+  Set = 0,  // ACC <- term
+  Add = 1,  // ACC <- ACC + term
+  Sub = 2,  // ACC <- ACC - term
+};
+
+inline std::uint8_t term_op(AccMode m, Term t)
+{
+  return static_cast<std::uint8_t>(kTermBase + static_cast<std::uint8_t>(m) * 8 + static_cast<std::uint8_t>(t));
+}
+
+inline bool is_term_op(std::uint8_t op) { return op >= kTermBase && op < kTermBase + 24 && (op & 7) < kTerms; }
+
+inline std::string op_name(std::uint8_t op)
+{
+  static const char* base[] = {"nop", "ldc", "add", "addi", "sub", "subi", "isub", "mul", "muli", "sqr",
+                               "neg", "chkle", "chkeq", "obj", "out"};
+  static const char* acc[] = {"acc.st", "acc.addi", "acc.subi", "acc.isub", "acc.scale",
+                              "acc.neg", "acc.chkle", "acc.chkeq", "acc.obj", "acc.out"};
+  static const char* modes[] = {"set", "add", "sub"};
+  static const char* terms[] = {"s", "sc", "ss", "q", "ssc", "qc"};
+  if (op < 15) {
+    return base[op];
+  }
+  if (op >= 16 && op < 26) {
+    return acc[op - 16];
+  }
+  if (is_term_op(op)) {
+    unsigned const k = op - kTermBase;
+    return std::string("acc.") + modes[k >> 3] + "." + terms[k & 7];
   }
   return "?";
 }
@@ -87,12 +130,28 @@ struct Instr
 };
 static_assert(sizeof(Instr) == 16, "tape words are 16 bytes");
 
-/// FP64 operations one instruction costs as algorithmic work: each
-/// directed-rounded product or sum counts once; selects, compares used for
-/// case analysis and the rounding-mode emulation are not counted.
-inline std::uint32_t interval_flops(TOp op)
+/// Algorithmic FP64 work of one instruction: each directed-rounded product or
+/// sum counts once (an interval + is 2, an interval x interval product is 8, a
+/// scalar product or a square is 2, a feasibility compare 1); selects, case
+/// compares and the rounding-mode handling are not counted.
+inline std::uint32_t interval_flops(std::uint8_t op)
 {
-  switch (op) {
+  if (is_term_op(op)) {
+    unsigned const k = op - kTermBase;
+    unsigned const mode = k >> 3;
+    Term const t = static_cast<Term>(k & 7);
+    std::uint32_t f = mode == 0 ? 0 : 2;
+    switch (t) {
+      case Term::S: break;
+      case Term::SC: f += 2; break;
+      case Term::SS: f += 8; break;
+      case Term::Q: f += 2; break;
+      case Term::SSC: f += 10; break;
+      case Term::QC: f += 4; break;
+    }
+    return f;
+  }
+  switch (static_cast<TOp>(op)) {
     case TOp::Add:
     case TOp::AddImm:
     case TOp::Sub:
@@ -100,12 +159,18 @@ inline std::uint32_t interval_flops(TOp op)
     case TOp::ImmSub:
     case TOp::MulImm:
     case TOp::Sqr:
+    case TOp::AccAddImm:
+    case TOp::AccSubImm:
+    case TOp::AccImmSub:
+    case TOp::AccScale:
       return 2;
     case TOp::Mul:
       return 8;
     case TOp::CheckLE:
+    case TOp::AccCheckLE:
       return 1;
     case TOp::CheckEQ:
+    case TOp::AccCheckEQ:
       return 2;
     default:
       return 0;
@@ -120,360 +185,531 @@ struct Tape
   std::uint32_t n_outputs = 0;   // Output instructions: 1 + n_constraints
   std::uint32_t n_constraints = 0;
   std::uint32_t reachable_nodes = 0;  // non-Const DAG nodes the roots reach
-  std::uint32_t remat_extra = 0;      // instructions added by re-evaluating leaves
+  std::uint32_t node_ops = 0;         // DAG node operations the tape performs (leaves counted per use)
   std::uint64_t flops = 0;            // interval_flops summed over the tape
 };
 
 struct CompileOptions
 {
   bool rematerialise_leaves = true;
+  bool accumulate = true;    // chains in the accumulator register, fused terms
   bool emit_outputs = true;  // Output instructions for the objective and each constraint root
 };
 
-/// Compile a validated Problem into a Tape. Throws InvalidDAG for the shapes the
-/// reference's builder rejects (a unary op on a Const, a Const root) and
-/// InvalidConfiguration for ops the device evaluator does not implement.
-inline Tape compile(const model::Problem<double>& p, const CompileOptions& opt = {})
+namespace detail
 {
-  using model::Op;
-  auto const& nodes = p.graph.nodes;
-  std::size_t const n = nodes.size();
-  std::size_t const n_vars = p.box_bounds.size();
-  if (n_vars > 30000 || n > (std::size_t(1) << 31)) {
-    throw InvalidConfiguration("problem too large for the 16-bit tape slot encoding");
-  }
 
-  std::vector<std::size_t> roots;
-  roots.push_back(p.objective_root);
-  for (auto const& c : p.constraints) {
-    roots.push_back(c.root_id);
-  }
+/// Instruction with virtual register operands (-1: unused).
+struct VInstr
+{
+  std::uint8_t op;
+  std::int64_t dst = -1, a = -1, b = -1;
+  double imm = 0.0;
+};
 
-  // Reachability and the op set the device implements.
-  std::vector<bool> reach(n, false);
-  for (std::size_t r : roots) {
-    reach[r] = true;
-  }
-  std::uint32_t reachable = 0;
-  for (std::size_t i = n; i-- > 0;) {
-    if (!reach[i]) {
-      continue;
-    }
-    auto const& nd = nodes[i];
-    for (std::size_t in : nd.in) {
-      reach[in] = true;
-    }
-    if (nd.op == Op::Const) {
-      continue;
-    }
-    ++reachable;
-    switch (nd.op) {
-      case Op::Var:
-      case Op::Add:
-      case Op::Sub:
-      case Op::Mul:
-        break;
-      case Op::Sqr:
-      case Op::Neg:
-        if (nodes[nd.in[0]].op == Op::Const) {
-          throw InvalidDAG("unary op applied directly to an Op::Const");
-        }
-        break;
-      default:
-        throw InvalidConfiguration(std::string("Op::") + model::op_name(nd.op)
-                                   + " is not implemented by the device tape evaluator");
-    }
-  }
-
-  auto is_leafish = [&](std::size_t id) {
-    Op const op = nodes[id].op;
-    return op == Op::Var || op == Op::Const;
-  };
-  // A leaf: an op node all of whose operands are Var or Const.
-  std::vector<bool> leaf(n, false);
-  for (std::size_t i = 0; i < n; ++i) {
-    if (!reach[i] || is_leafish(i)) {
-      continue;
-    }
-    bool all = true;
-    for (std::size_t in : nodes[i].in) {
-      all = all && is_leafish(in);
-    }
-    leaf[i] = all && opt.rematerialise_leaves;
-  }
-
-  // Remaining uses of each shared (non-leaf, non-Var) node, counted over the
-  // instruction stream the walk below will produce.
-  std::vector<std::uint32_t> uses(n, 0);
+class Compiler
+{
+public:
+  Compiler(const model::Problem<double>& p, const CompileOptions& opt)
+      : p_(p)
+      , nodes_(p.graph.nodes)
+      , opt_(opt)
   {
-    std::vector<std::uint8_t> seen(n, 0);
-    std::vector<std::size_t> stack;
-    auto count_walk = [&](std::size_t root) {
-      stack.push_back(root);
-      while (!stack.empty()) {
-        std::size_t const id = stack.back();
-        stack.pop_back();
-        auto const& nd = nodes[id];
-        if (nd.op == Op::Var || nd.op == Op::Const) {
-          continue;
-        }
-        if (!leaf[id]) {
-          if (seen[id]) {
-            continue;
-          }
-          seen[id] = 1;
-        }
-        for (std::size_t in : nd.in) {
-          if (nodes[in].op != Op::Const && nodes[in].op != Op::Var && !leaf[in]) {
-            ++uses[in];
-          }
-          stack.push_back(in);
-        }
-      }
-    };
-    for (std::size_t r : roots) {
-      count_walk(r);
-      if (nodes[r].op != Op::Var && !leaf[r]) {
-        ++uses[r];  // the root's own check/objective read
-      }
-    }
-    // A root referenced as the objective and as (several) constraints is read
-    // once per reference, and Output reads it once more.
-    if (opt.emit_outputs) {
-      for (std::size_t r : roots) {
-        if (nodes[r].op != Op::Var && !leaf[r]) {
-          ++uses[r];
-        }
-      }
-    }
   }
 
-  Tape t;
-  t.n_vars = static_cast<std::uint32_t>(n_vars);
-  t.n_constraints = static_cast<std::uint32_t>(p.constraints.size());
-  t.reachable_nodes = reachable;
-  std::vector<std::uint32_t> free_slots;  // kept sorted descending: back() is the lowest free slot
-  std::uint32_t next_slot = static_cast<std::uint32_t>(n_vars);
-  auto alloc = [&]() -> std::uint32_t {
-    if (!free_slots.empty()) {
-      std::uint32_t const s = free_slots.back();
-      free_slots.pop_back();
-      return s;
-    }
-    return next_slot++;
-  };
-  auto release = [&](std::uint32_t s) {
-    if (s < n_vars) {
-      return;
-    }
-    free_slots.insert(std::upper_bound(free_slots.begin(), free_slots.end(), s, std::greater<>()), s);
-  };
-
-  std::vector<std::int64_t> slot_of(n, -1);  // computed shared nodes
-  std::vector<std::uint32_t> left(uses);     // remaining reads
-
-  auto emit = [&](TOp op, std::uint32_t dst, std::uint32_t a, std::uint32_t b, double imm) {
-    Instr ins {};
-    ins.op = static_cast<std::uint8_t>(op);
-    ins.dst = static_cast<std::uint16_t>(dst);
-    ins.a = static_cast<std::uint16_t>(a);
-    ins.b = static_cast<std::uint16_t>(b);
-    ins.imm = imm;
-    t.code.push_back(ins);
-    t.flops += interval_flops(op);
-  };
-
-  // Slot holding the value of operand `id` for one read; `temp` reports whether
-  // that slot must be released after this read (a re-evaluated leaf copy).
-  struct Frame
+  Tape run()
   {
-    std::size_t id;
-    bool expanded;
-  };
-  std::vector<Frame> stack;
-  std::vector<std::uint32_t> value_stack;  // slots of evaluated operands, in walk order
-  std::vector<bool> value_temp;
+    using model::Op;
+    std::size_t const n = nodes_.size();
+    std::size_t const n_vars = p_.box_bounds.size();
+    if (n_vars > 30000) {
+      throw InvalidConfiguration("problem too large for the 16-bit tape slot encoding");
+    }
+    roots_.push_back(p_.objective_root);
+    for (auto const& c : p_.constraints) {
+      roots_.push_back(c.root_id);
+    }
+    analyse();
 
-  auto read_done = [&](std::size_t id, std::uint32_t slot, bool temp) {
-    if (temp) {
-      release(slot);
-      return;
-    }
-    if (nodes[id].op == Op::Var) {
-      return;
-    }
-    if (--left[id] == 0) {
-      release(slot);
-    }
-  };
-
-  std::uint32_t leaf_instrs = 0;
-  auto evaluate = [&](std::size_t root) {
-    stack.push_back({root, false});
-    while (!stack.empty()) {
-      Frame f = stack.back();
-      stack.pop_back();
-      std::size_t const id = f.id;
-      auto const& nd = nodes[id];
-      if (nd.op == Op::Var) {
-        value_stack.push_back(static_cast<std::uint32_t>(nd.payload.var_index));
-        value_temp.push_back(false);
+    next_vreg_ = static_cast<std::int64_t>(n_vars);
+    cache_.assign(n, -1);
+    for (std::size_t k = 0; k < roots_.size(); ++k) {
+      std::size_t const r = roots_[k];
+      bool const is_obj = k == 0;
+      double const rhs = is_obj ? 0.0 : p_.constraints[k - 1].rhs;
+      bool const le = is_obj || p_.constraints[k - 1].cmp == model::Cmp::LE;
+      if (opt_.accumulate && acc_capable(r) && parents_[r] == 1 && cache_[r] < 0) {
+        emit_acc(r);
+        if (opt_.emit_outputs) {
+          emit({static_cast<std::uint8_t>(TOp::AccOutput), -1, -1, -1, 0.0}, static_cast<std::int64_t>(k));
+        }
+        if (is_obj) {
+          emit({static_cast<std::uint8_t>(TOp::AccObjective), -1, -1, -1, 0.0});
+        } else {
+          emit({static_cast<std::uint8_t>(le ? TOp::AccCheckLE : TOp::AccCheckEQ), -1, -1, -1, rhs});
+        }
         continue;
+      }
+      std::int64_t const v = emit_slot(r);
+      if (opt_.emit_outputs) {
+        emit({static_cast<std::uint8_t>(TOp::Output), -1, v, -1, 0.0}, static_cast<std::int64_t>(k));
+      }
+      if (is_obj) {
+        emit({static_cast<std::uint8_t>(TOp::Objective), -1, v, -1, 0.0});
+      } else {
+        emit({static_cast<std::uint8_t>(le ? TOp::CheckLE : TOp::CheckEQ), -1, v, -1, rhs});
+      }
+    }
+    return allocate(n_vars);
+  }
+
+private:
+  using Op = model::Op;
+
+  bool is_var(std::size_t id) const { return nodes_[id].op == Op::Var; }
+  bool is_const(std::size_t id) const { return nodes_[id].op == Op::Const; }
+  double cval(std::size_t id) const { return nodes_[id].payload.constant; }
+
+  void analyse()
+  {
+    std::size_t const n = nodes_.size();
+    reach_.assign(n, false);
+    parents_.assign(n, 0);
+    for (std::size_t r : roots_) {
+      reach_[r] = true;
+      ++parents_[r];
+    }
+    for (std::size_t i = n; i-- > 0;) {
+      if (!reach_[i]) {
+        continue;
+      }
+      auto const& nd = nodes_[i];
+      for (std::size_t in : nd.in) {
+        reach_[in] = true;
+        ++parents_[in];
       }
       if (nd.op == Op::Const) {
-        value_stack.push_back(UINT32_MAX);  // placeholder, consumed as an immediate
-        value_temp.push_back(false);
         continue;
       }
-      if (!leaf[id] && slot_of[id] >= 0) {
-        value_stack.push_back(static_cast<std::uint32_t>(slot_of[id]));
-        value_temp.push_back(false);
-        continue;
-      }
-      if (!f.expanded) {
-        stack.push_back({id, true});
-        for (std::size_t k = nd.in.size(); k-- > 0;) {
-          stack.push_back({nd.in[k], false});
-        }
-        continue;
-      }
-      // Operands are on value_stack in order.
-      std::size_t const arity = nd.in.size();
-      std::uint32_t sl[2] = {0, 0};
-      bool tmp[2] = {false, false};
-      for (std::size_t k = arity; k-- > 0;) {
-        sl[k] = value_stack.back();
-        tmp[k] = value_temp.back();
-        value_stack.pop_back();
-        value_temp.pop_back();
-      }
-      bool const c0 = nodes[nd.in[0]].op == Op::Const;
-      bool const c1 = arity > 1 && nodes[nd.in[1]].op == Op::Const;
-      double const k0 = c0 ? nodes[nd.in[0]].payload.constant : 0.0;
-      double const k1 = c1 ? nodes[nd.in[1]].payload.constant : 0.0;
-      std::uint32_t const before = static_cast<std::uint32_t>(t.code.size());
-
-      // Both operands Const: materialise the left one as a point interval and
-      // apply the op with the right one as an immediate.
-      if (c0 && c1) {
-        std::uint32_t const s = alloc();
-        emit(TOp::LoadConst, s, 0, 0, k0);
-        sl[0] = s;
-        tmp[0] = true;
-      }
-      bool const a_const = c0 && !c1;
-      bool const b_const = c1;
-      // Release operand slots before allocating the destination so an
-      // operand's slot can be reused in place (operands are read first).
-      if (arity > 0 && !(a_const)) {
-        read_done(nd.in[0], sl[0], tmp[0]);
-      }
-      if (arity > 1 && !b_const) {
-        read_done(nd.in[1], sl[1], tmp[1]);
-      }
-      std::uint32_t const dst = alloc();
+      ++reachable_;
       switch (nd.op) {
+        case Op::Var:
         case Op::Add:
-          if (b_const) {
-            emit(TOp::AddImm, dst, sl[0], 0, k1);
-          } else if (a_const) {
-            emit(TOp::AddImm, dst, sl[1], 0, k0);
-          } else {
-            emit(TOp::Add, dst, sl[0], sl[1], 0.0);
-          }
-          break;
         case Op::Sub:
-          if (b_const) {
-            emit(TOp::SubImm, dst, sl[0], 0, k1);
-          } else if (a_const) {
-            emit(TOp::ImmSub, dst, sl[1], 0, k0);
-          } else {
-            emit(TOp::Sub, dst, sl[0], sl[1], 0.0);
-          }
-          break;
         case Op::Mul:
-          if (b_const) {
-            emit(TOp::MulImm, dst, sl[0], 0, k1);
-          } else if (a_const) {
-            emit(TOp::MulImm, dst, sl[1], 0, k0);
-          } else {
-            emit(TOp::Mul, dst, sl[0], sl[1], 0.0);
-          }
           break;
         case Op::Sqr:
-          emit(TOp::Sqr, dst, sl[0], 0, 0.0);
-          break;
         case Op::Neg:
-          emit(TOp::Neg, dst, sl[0], 0, 0.0);
+          if (is_const(nd.in[0])) {
+            throw InvalidDAG("unary op applied directly to an Op::Const");
+          }
           break;
         default:
-          throw InvalidConfiguration("unsupported op reached the tape emitter");
-      }
-      if (leaf[id]) {
-        leaf_instrs += static_cast<std::uint32_t>(t.code.size()) - before;
-        value_stack.push_back(dst);
-        value_temp.push_back(true);
-      } else {
-        slot_of[id] = dst;
-        value_stack.push_back(dst);
-        value_temp.push_back(false);
+          throw InvalidConfiguration(std::string("Op::") + model::op_name(nd.op)
+                                     + " is not implemented by the device tape evaluator");
       }
     }
-  };
-
-  auto root_value = [&](std::size_t r, std::uint32_t& slot, bool& temp) {
-    evaluate(r);
-    slot = value_stack.back();
-    temp = value_temp.back();
-    value_stack.pop_back();
-    value_temp.pop_back();
-  };
-
-  std::uint32_t output_index = 0;
-  for (std::size_t k = 0; k < roots.size(); ++k) {
-    std::size_t const r = roots[k];
-    std::uint32_t slot = 0;
-    bool temp = false;
-    root_value(r, slot, temp);
-    if (opt.emit_outputs) {
-      emit(TOp::Output, 0, slot, output_index, 0.0);
-      if (!temp) {
-        read_done(r, slot, false);  // a re-evaluated leaf copy stays alive for the check below
+    leaf_.assign(n, false);
+    for (std::size_t i = 0; i < n; ++i) {
+      if (!reach_[i] || is_var(i) || is_const(i)) {
+        continue;
       }
+      bool all = true;
+      for (std::size_t in : nodes_[i].in) {
+        all = all && (is_var(in) || is_const(in));
+      }
+      leaf_[i] = all && opt_.rematerialise_leaves;
     }
-    ++output_index;
-    if (k == 0) {
-      emit(TOp::Objective, 0, slot, 0, 0.0);
+  }
+
+  void emit(VInstr v, std::int64_t output_index = -1)
+  {
+    if (output_index >= 0) {
+      v.b = -2 - output_index;  // encoded literal, not a register
+    }
+    code_.push_back(v);
+  }
+
+  std::int64_t fresh() { return next_vreg_++; }
+
+  // -- generic slot evaluation -------------------------------------------
+
+  /// Virtual register holding node `id`'s value (emitting it if needed).
+  std::int64_t emit_slot(std::size_t id)
+  {
+    auto const& nd = nodes_[id];
+    if (nd.op == Op::Var) {
+      return static_cast<std::int64_t>(nd.payload.var_index);
+    }
+    if (cache_[id] >= 0) {
+      return cache_[id];
+    }
+    std::int64_t v;
+    if (opt_.accumulate && !leaf_[id] && acc_capable(id)) {
+      emit_acc(id);
+      v = fresh();
+      emit({static_cast<std::uint8_t>(TOp::AccStore), v, -1, -1, 0.0});
     } else {
-      auto const& c = p.constraints[k - 1];
-      emit(c.cmp == model::Cmp::LE ? TOp::CheckLE : TOp::CheckEQ, 0, slot, 0, c.rhs);
+      v = emit_generic(id);
     }
-    read_done(r, slot, temp);
+    ++node_ops_;
+    if (!leaf_[id]) {
+      cache_[id] = v;
+    }
+    return v;
   }
-  t.n_outputs = opt.emit_outputs ? output_index : 0;
-  t.n_slots = std::max<std::uint32_t>(next_slot, 1);
-  std::uint32_t distinct_leaves = 0;
-  for (std::size_t i = 0; i < n; ++i) {
-    if (leaf[i]) {
-      distinct_leaves += (nodes[i].in.size() == 2 && nodes[nodes[i].in[0]].op == Op::Const
-                          && nodes[nodes[i].in[1]].op == Op::Const)
-          ? 2
-          : 1;
+
+  std::int64_t emit_generic(std::size_t id)
+  {
+    auto const& nd = nodes_[id];
+    std::size_t const arity = nd.in.size();
+    bool const c0 = is_const(nd.in[0]);
+    bool const c1 = arity > 1 && is_const(nd.in[1]);
+    std::int64_t a = -1, b = -1;
+    if (c0 && c1) {
+      a = fresh();
+      emit({static_cast<std::uint8_t>(TOp::LoadConst), a, -1, -1, cval(nd.in[0])});
+    } else if (!c0) {
+      a = emit_slot(nd.in[0]);
+    }
+    if (arity > 1 && !c1) {
+      b = emit_slot(nd.in[1]);
+    }
+    std::int64_t const d = fresh();
+    auto out = [&](TOp op, std::int64_t x, std::int64_t y, double imm) {
+      emit({static_cast<std::uint8_t>(op), d, x, y, imm});
+    };
+    switch (nd.op) {
+      case Op::Add:
+        if (c1) {
+          out(TOp::AddImm, a, -1, cval(nd.in[1]));
+        } else if (c0) {
+          out(TOp::AddImm, b, -1, cval(nd.in[0]));
+        } else {
+          out(TOp::Add, a, b, 0.0);
+        }
+        break;
+      case Op::Sub:
+        if (c1) {
+          out(TOp::SubImm, a, -1, cval(nd.in[1]));
+        } else if (c0) {
+          out(TOp::ImmSub, b, -1, cval(nd.in[0]));
+        } else {
+          out(TOp::Sub, a, b, 0.0);
+        }
+        break;
+      case Op::Mul:
+        if (c1) {
+          out(TOp::MulImm, a, -1, cval(nd.in[1]));
+        } else if (c0) {
+          out(TOp::MulImm, b, -1, cval(nd.in[0]));
+        } else {
+          out(TOp::Mul, a, b, 0.0);
+        }
+        break;
+      case Op::Sqr: out(TOp::Sqr, a, -1, 0.0); break;
+      case Op::Neg: out(TOp::Neg, a, -1, 0.0); break;
+      default: throw InvalidConfiguration("unsupported op reached the tape emitter");
+    }
+    return d;
+  }
+
+  // -- accumulator chains -----------------------------------------------
+
+  /// A node whose value can be produced in ACC by accumulator instructions.
+  bool acc_capable(std::size_t id) const
+  {
+    auto const& nd = nodes_[id];
+    switch (nd.op) {
+      case Op::Add:
+      case Op::Sub:
+        return !(is_const(nd.in[0]) && is_const(nd.in[1]));
+      case Op::Mul:
+        return is_const(nd.in[0]) != is_const(nd.in[1]);
+      case Op::Neg:
+        return true;
+      default:
+        return false;
     }
   }
-  t.remat_extra = leaf_instrs > distinct_leaves ? leaf_instrs - distinct_leaves : 0;
-  return t;
+
+  /// Consumed only here and not yet materialised: may live in ACC or be folded
+  /// into a term.
+  bool private_value(std::size_t id) const { return parents_[id] == 1 && cache_[id] < 0 && !is_var(id); }
+
+  struct TermRef
+  {
+    Term kind;
+    std::int64_t a = -1, b = -1;
+    double imm = 0.0;
+    std::uint32_t ops = 0;  // DAG node operations the term performs
+  };
+
+  /// Operand of a term: a variable or an already materialised value. Emits
+  /// nothing unless `allow_emit` (then the value is materialised first).
+  bool term_operand(std::size_t id, bool allow_emit, std::int64_t& v)
+  {
+    if (is_const(id)) {
+      return false;
+    }
+    if (is_var(id)) {
+      v = static_cast<std::int64_t>(nodes_[id].payload.var_index);
+      return true;
+    }
+    if (cache_[id] >= 0) {
+      v = cache_[id];
+      return true;
+    }
+    if (allow_emit) {
+      v = emit_slot(id);
+      return true;
+    }
+    return false;
+  }
+
+  /// Product pattern (a*b or a*a) on term operands; emits operand values first
+  /// when they are neither variables nor materialised (ACC must be free).
+  bool product_term(std::size_t id, TermRef& t)
+  {
+    auto const& nd = nodes_[id];
+    if (!(leaf_[id] || private_value(id))) {
+      return false;
+    }
+    if (nd.op == Op::Sqr) {
+      std::int64_t a;
+      if (!term_operand(nd.in[0], true, a)) {
+        return false;
+      }
+      t = {Term::Q, a, -1, 0.0, 1};
+      return true;
+    }
+    if (nd.op == Op::Mul && !is_const(nd.in[0]) && !is_const(nd.in[1])) {
+      std::int64_t a, b;
+      if (!term_operand(nd.in[0], true, a) || !term_operand(nd.in[1], true, b)) {
+        return false;
+      }
+      t = {Term::SS, a, b, 0.0, 1};
+      return true;
+    }
+    return false;
+  }
+
+  /// The value of node `id` as a term of an accumulate instruction. Must be
+  /// called while ACC is free: it may emit (materialise) operands.
+  TermRef prepare_term(std::size_t id)
+  {
+    auto const& nd = nodes_[id];
+    TermRef t;
+    if (is_var(id) || cache_[id] >= 0) {
+      std::int64_t v;
+      term_operand(id, false, v);
+      return {Term::S, v, -1, 0.0, 0};
+    }
+    if (product_term(id, t)) {
+      return t;
+    }
+    if ((leaf_[id] || private_value(id)) && nd.op == Op::Mul && is_const(nd.in[0]) != is_const(nd.in[1])) {
+      std::size_t const x = is_const(nd.in[0]) ? nd.in[1] : nd.in[0];
+      double const c = is_const(nd.in[0]) ? cval(nd.in[0]) : cval(nd.in[1]);
+      TermRef inner;
+      if (is_var(x) || cache_[x] >= 0) {
+        std::int64_t v;
+        term_operand(x, false, v);
+        return {Term::SC, v, -1, c, 1};
+      }
+      if (product_term(x, inner)) {
+        return {inner.kind == Term::Q ? Term::QC : Term::SSC, inner.a, inner.b, c, 2};
+      }
+    }
+    std::int64_t const v = emit_slot(id);
+    return {Term::S, v, -1, 0.0, 0};
+  }
+
+  void emit_term(AccMode m, const TermRef& t)
+  {
+    emit({term_op(m, t.kind), -1, t.a, t.b, t.imm});
+    node_ops_ += t.ops;
+  }
+
+  /// ACC <- value of `x` (ACC is free on entry).
+  void acc_from(std::size_t x)
+  {
+    if (acc_capable(x) && private_value(x)) {
+      emit_acc(x);
+      return;
+    }
+    emit_term(AccMode::Set, prepare_term(x));
+  }
+
+  /// ACC <- value of acc-capable node `id` (ACC is free on entry).
+  void emit_acc(std::size_t id)
+  {
+    auto const& nd = nodes_[id];
+    auto simple = [&](TOp op, double imm) {
+      emit({static_cast<std::uint8_t>(op), -1, -1, -1, imm});
+      ++node_ops_;
+    };
+    switch (nd.op) {
+      case Op::Add:
+      case Op::Sub: {
+        std::size_t const l = nd.in[0], r = nd.in[1];
+        if (is_const(r)) {
+          acc_from(l);
+          simple(nd.op == Op::Add ? TOp::AccAddImm : TOp::AccSubImm, cval(r));
+          return;
+        }
+        if (is_const(l)) {
+          acc_from(r);
+          simple(nd.op == Op::Add ? TOp::AccAddImm : TOp::AccImmSub, cval(l));
+          return;
+        }
+        // The chain runs through the left operand; for a sum whose left
+        // operand is a term and right operand a chain, run the right one
+        // (IEEE addition commutes bit for bit).
+        std::size_t chain = l, other = r;
+        if (nd.op == Op::Add && !(acc_capable(l) && private_value(l)) && acc_capable(r) && private_value(r)) {
+          chain = r;
+          other = l;
+        }
+        TermRef const t = prepare_term(other);  // may materialise; ACC still free
+        acc_from(chain);
+        emit_term(nd.op == Op::Add ? AccMode::Add : AccMode::Sub, t);
+        ++node_ops_;
+        return;
+      }
+      case Op::Mul: {
+        std::size_t const x = is_const(nd.in[0]) ? nd.in[1] : nd.in[0];
+        double const c = is_const(nd.in[0]) ? cval(nd.in[0]) : cval(nd.in[1]);
+        TermRef inner;
+        if (is_var(x) || cache_[x] >= 0) {
+          std::int64_t v;
+          term_operand(x, false, v);
+          emit_term(AccMode::Set, {Term::SC, v, -1, c, 1});
+          return;
+        }
+        if (!(acc_capable(x) && private_value(x)) && product_term(x, inner)) {
+          emit_term(AccMode::Set, {inner.kind == Term::Q ? Term::QC : Term::SSC, inner.a, inner.b, c, 2});
+          return;
+        }
+        acc_from(x);
+        simple(TOp::AccScale, c);
+        return;
+      }
+      case Op::Neg:
+        acc_from(nd.in[0]);
+        simple(TOp::AccNeg, 0.0);
+        return;
+      default:
+        throw InvalidConfiguration("emit_acc on a node that cannot run in the accumulator");
+    }
+  }
+
+  // -- register allocation ------------------------------------------------
+
+  Tape allocate(std::size_t n_vars)
+  {
+    std::size_t const n_v = static_cast<std::size_t>(next_vreg_);
+    std::vector<std::int64_t> last(n_v, -1);
+    for (std::size_t i = 0; i < code_.size(); ++i) {
+      for (std::int64_t r : {code_[i].a, code_[i].b}) {
+        if (r >= 0) {
+          last[static_cast<std::size_t>(r)] = static_cast<std::int64_t>(i);
+        }
+      }
+    }
+    std::vector<std::int64_t> phys(n_v, -1);
+    for (std::size_t v = 0; v < n_vars; ++v) {
+      phys[v] = static_cast<std::int64_t>(v);
+    }
+    std::vector<std::uint32_t> free_slots;  // sorted descending: back() is the lowest
+    std::uint32_t next_slot = static_cast<std::uint32_t>(n_vars);
+    Tape t;
+    t.n_vars = static_cast<std::uint32_t>(n_vars);
+    t.n_constraints = static_cast<std::uint32_t>(p_.constraints.size());
+    t.reachable_nodes = reachable_;
+    t.node_ops = node_ops_;
+    for (std::size_t i = 0; i < code_.size(); ++i) {
+      VInstr const& v = code_[i];
+      Instr ins {};
+      ins.op = v.op;
+      ins.imm = v.imm;
+      auto map = [&](std::int64_t r) -> std::uint16_t {
+        if (r < 0) {
+          return 0;
+        }
+        return static_cast<std::uint16_t>(phys[static_cast<std::size_t>(r)]);
+      };
+      ins.a = map(v.a);
+      ins.b = v.b <= -2 ? static_cast<std::uint16_t>(-2 - v.b) : map(v.b);
+      // operands whose last use is here are free before the destination is
+      // chosen (the interpreter reads every operand before writing)
+      for (std::int64_t r : {v.a, v.b}) {
+        if (r >= static_cast<std::int64_t>(n_vars) && last[static_cast<std::size_t>(r)] == static_cast<std::int64_t>(i)) {
+          std::uint32_t const s = static_cast<std::uint32_t>(phys[static_cast<std::size_t>(r)]);
+          if (std::find(free_slots.begin(), free_slots.end(), s) == free_slots.end()) {
+            free_slots.insert(std::upper_bound(free_slots.begin(), free_slots.end(), s, std::greater<>()), s);
+          }
+        }
+      }
+      if (v.dst >= 0) {
+        std::uint32_t s;
+        if (!free_slots.empty()) {
+          s = free_slots.back();
+          free_slots.pop_back();
+        } else {
+          s = next_slot++;
+        }
+        phys[static_cast<std::size_t>(v.dst)] = s;
+        ins.dst = static_cast<std::uint16_t>(s);
+        if (last[static_cast<std::size_t>(v.dst)] < 0) {
+          // never read: the slot is free again after this instruction
+          free_slots.insert(std::upper_bound(free_slots.begin(), free_slots.end(), s, std::greater<>()), s);
+        }
+      }
+      t.code.push_back(ins);
+      t.flops += interval_flops(ins.op);
+    }
+    if (next_slot > 65535) {
+      throw InvalidConfiguration("tape needs more slots than the 16-bit encoding holds");
+    }
+    t.n_slots = std::max<std::uint32_t>(next_slot, 1);
+    t.n_outputs = opt_.emit_outputs ? static_cast<std::uint32_t>(roots_.size()) : 0;
+    return t;
+  }
+
+  const model::Problem<double>& p_;
+  const std::vector<model::DAGNode<double>>& nodes_;
+  CompileOptions opt_;
+  std::vector<std::size_t> roots_;
+  std::vector<bool> reach_;
+  std::vector<bool> leaf_;
+  std::vector<std::uint32_t> parents_;
+  std::vector<std::int64_t> cache_;
+  std::vector<VInstr> code_;
+  std::int64_t next_vreg_ = 0;
+  std::uint32_t reachable_ = 0;
+  std::uint32_t node_ops_ = 0;
+};
+
+}  // namespace detail
+
+/// Compile a validated Problem into a Tape. Throws InvalidDAG for the shapes the
+/// reference's builder rejects (a unary op on a Const) and InvalidConfiguration
+/// for ops the device evaluator does not implement.
+inline Tape compile(const model::Problem<double>& p, const CompileOptions& opt = {})
+{
+  return detail::Compiler(p, opt).run();
 }
 
 /// Human-readable listing, one instruction per line.
 inline std::string disassemble(const Tape& t)
 {
   std::string out;
-  char buf[160];
+  char buf[192];
   for (std::size_t i = 0; i < t.code.size(); ++i) {
     Instr const& in = t.code[i];
-    std::snprintf(buf, sizeof(buf), "%5zu %-6s d=%u a=%u b=%u imm=%.17g\n", i, top_name(static_cast<TOp>(in.op)),
-                  in.dst, in.a, in.b, in.imm);
+    std::snprintf(buf, sizeof(buf), "%5zu %-12s d=%u a=%u b=%u imm=%.17g\n", i, op_name(in.op).c_str(), in.dst, in.a,
+                  in.b, in.imm);
     out += buf;
   }
   return out;

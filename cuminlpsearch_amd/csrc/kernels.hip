@@ -279,49 +279,114 @@ struct IntervalResult
   bool feasible;
 };
 
-// Interval tape over LDS slots S[slot * kTile + lane]. Must run inside the
-// round-up region (token t). Output instructions store into
-// roots[(b * roots_stride + e)] when roots is non-null.
+__device__ __forceinline__ Iv load_iv(const double2* S, std::uint32_t slot, unsigned lane)
+{
+  double2 const v = S[slot * kTile + lane];
+  return {v.x, v.y};
+}
+
+// Interval tape over LDS slots S[slot * kTile + lane] and the accumulator
+// register. Must run inside the round-up region (token t). Output
+// instructions store into roots[b * roots_stride + e] when roots is non-null.
 __device__ __forceinline__ IntervalResult run_interval(ConstInstrPtr code, std::uint32_t n_instr, double2* S,
                                                        unsigned lane, int t, double2* roots,
                                                        std::uint64_t roots_stride, std::uint64_t e)
 {
   IntervalResult res {{0.0, 0.0}, true};
+  Iv acc {0.0, 0.0};
   for (std::uint32_t pc = 0; pc < n_instr; ++pc) {
     Word const ins = fetch(code, pc);
-    TOp const op = static_cast<TOp>(ins.op);
-    double2* const dst = &S[ins.dst * kTile + lane];
-    double2 const av = S[ins.a * kTile + lane];
-    Iv const a {av.x, av.y};
+    std::uint32_t const op = ins.op;
+    if (op >= tape::kTermBase) {
+      // acc <- acc (set|+|-) term(a, b, imm)
+      std::uint32_t const term = op & 7u;
+      std::uint32_t const mode = (op - tape::kTermBase) >> 3;
+      Iv const a = load_iv(S, ins.a, lane);
+      Iv x;
+      if (term == static_cast<std::uint32_t>(tape::Term::S)) {
+        x = a;
+      } else if (term == static_cast<std::uint32_t>(tape::Term::SC)) {
+        x = iv_mul_imm(a, ins.imm, t);
+      } else if (term == static_cast<std::uint32_t>(tape::Term::Q)) {
+        x = iv_sqr(a, t);
+      } else if (term == static_cast<std::uint32_t>(tape::Term::QC)) {
+        x = iv_mul_imm(iv_sqr(a, t), ins.imm, t);
+      } else {
+        Iv const b = load_iv(S, ins.b, lane);
+        x = iv_mul(a, b, t);
+        if (term == static_cast<std::uint32_t>(tape::Term::SSC)) {
+          x = iv_mul_imm(x, ins.imm, t);
+        }
+      }
+      if (mode == static_cast<std::uint32_t>(tape::AccMode::Set)) {
+        acc = x;
+      } else if (mode == static_cast<std::uint32_t>(tape::AccMode::Add)) {
+        acc = iv_add(acc, x, t);
+      } else {
+        acc = iv_sub(acc, x, t);
+      }
+      continue;
+    }
+    switch (static_cast<TOp>(op)) {
+      case TOp::AccStore:
+        S[ins.dst * kTile + lane] = make_double2(acc.lo, acc.hi);
+        continue;
+      case TOp::AccAddImm:
+        acc = iv_add_imm(acc, ins.imm, t);
+        continue;
+      case TOp::AccSubImm:
+        acc = iv_sub_imm(acc, ins.imm, t);
+        continue;
+      case TOp::AccImmSub:
+        acc = iv_imm_sub(ins.imm, acc, t);
+        continue;
+      case TOp::AccScale:
+        acc = iv_mul_imm(acc, ins.imm, t);
+        continue;
+      case TOp::AccNeg:
+        acc = iv_neg(acc);
+        continue;
+      case TOp::AccCheckLE:
+        res.feasible = res.feasible && (acc.lo <= ins.imm);
+        continue;
+      case TOp::AccCheckEQ:
+        res.feasible = res.feasible && (acc.lo <= ins.imm) && (ins.imm <= acc.hi);
+        continue;
+      case TOp::AccObjective:
+        res.obj = acc;
+        continue;
+      case TOp::AccOutput:
+        if (roots != nullptr) {
+          roots[static_cast<std::uint64_t>(ins.b) * roots_stride + e] = make_double2(acc.lo, acc.hi);
+        }
+        continue;
+      default:
+        break;
+    }
+    Iv const a = load_iv(S, ins.a, lane);
     Iv r;
-    switch (op) {
+    switch (static_cast<TOp>(op)) {
       case TOp::LoadConst:
         r = {ins.imm, ins.imm};
         break;
-      case TOp::Add: {
-        double2 const bv = S[ins.b * kTile + lane];
-        r = iv_add(a, {bv.x, bv.y}, t);
+      case TOp::Add:
+        r = iv_add(a, load_iv(S, ins.b, lane), t);
         break;
-      }
       case TOp::AddImm:
         r = iv_add_imm(a, ins.imm, t);
         break;
-      case TOp::Sub: {
-        double2 const bv = S[ins.b * kTile + lane];
-        r = iv_sub(a, {bv.x, bv.y}, t);
+      case TOp::Sub:
+        r = iv_sub(a, load_iv(S, ins.b, lane), t);
         break;
-      }
       case TOp::SubImm:
         r = iv_sub_imm(a, ins.imm, t);
         break;
       case TOp::ImmSub:
         r = iv_imm_sub(ins.imm, a, t);
         break;
-      case TOp::Mul: {
-        double2 const bv = S[ins.b * kTile + lane];
-        r = iv_mul(a, {bv.x, bv.y}, t);
+      case TOp::Mul:
+        r = iv_mul(a, load_iv(S, ins.b, lane), t);
         break;
-      }
       case TOp::MulImm:
         r = iv_mul_imm(a, ins.imm, t);
         break;
@@ -342,13 +407,13 @@ __device__ __forceinline__ IntervalResult run_interval(ConstInstrPtr code, std::
         continue;
       case TOp::Output:
         if (roots != nullptr) {
-          roots[static_cast<std::uint64_t>(ins.b) * roots_stride + e] = av;
+          roots[static_cast<std::uint64_t>(ins.b) * roots_stride + e] = make_double2(a.lo, a.hi);
         }
         continue;
       default:
         continue;
     }
-    *dst = make_double2(r.lo, r.hi);
+    S[ins.dst * kTile + lane] = make_double2(r.lo, r.hi);
   }
   return res;
 }
@@ -368,18 +433,65 @@ struct PointResult
   bool feasible;
 };
 
-// Round-to-nearest point tape over LDS slots P[slot * kTile + lane].
+// Round-to-nearest point tape over LDS slots P[slot * kTile + lane] and the
+// accumulator register.
 __device__ __forceinline__ PointResult run_point(ConstInstrPtr code, std::uint32_t n_instr, double* P, unsigned lane,
                                                  double* roots, std::uint64_t roots_stride, std::uint64_t e)
 {
   PointResult res {0.0, true};
+  double acc = 0.0;
   for (std::uint32_t pc = 0; pc < n_instr; ++pc) {
     Word const ins = fetch(code, pc);
-    TOp const op = static_cast<TOp>(ins.op);
-    double* const dst = &P[ins.dst * kTile + lane];
+    std::uint32_t const op = ins.op;
+    if (op >= tape::kTermBase) {
+      std::uint32_t const term = op & 7u;
+      std::uint32_t const mode = (op - tape::kTermBase) >> 3;
+      double const a = P[ins.a * kTile + lane];
+      double x;
+      if (term == static_cast<std::uint32_t>(tape::Term::S)) {
+        x = a;
+      } else if (term == static_cast<std::uint32_t>(tape::Term::SC)) {
+        x = a * ins.imm;
+      } else if (term == static_cast<std::uint32_t>(tape::Term::Q)) {
+        x = a * a;
+      } else if (term == static_cast<std::uint32_t>(tape::Term::QC)) {
+        x = (a * a) * ins.imm;
+      } else {
+        x = a * P[ins.b * kTile + lane];
+        if (term == static_cast<std::uint32_t>(tape::Term::SSC)) {
+          x = x * ins.imm;
+        }
+      }
+      if (mode == static_cast<std::uint32_t>(tape::AccMode::Set)) {
+        acc = x;
+      } else if (mode == static_cast<std::uint32_t>(tape::AccMode::Add)) {
+        acc = acc + x;
+      } else {
+        acc = acc - x;
+      }
+      continue;
+    }
+    switch (static_cast<TOp>(op)) {
+      case TOp::AccStore: P[ins.dst * kTile + lane] = acc; continue;
+      case TOp::AccAddImm: acc = acc + ins.imm; continue;
+      case TOp::AccSubImm: acc = acc - ins.imm; continue;
+      case TOp::AccImmSub: acc = ins.imm - acc; continue;
+      case TOp::AccScale: acc = acc * ins.imm; continue;
+      case TOp::AccNeg: acc = -acc; continue;
+      case TOp::AccCheckLE: res.feasible = res.feasible && (acc <= ins.imm); continue;
+      case TOp::AccCheckEQ: res.feasible = res.feasible && almost_equal(acc, ins.imm); continue;
+      case TOp::AccObjective: res.obj = acc; continue;
+      case TOp::AccOutput:
+        if (roots != nullptr) {
+          roots[static_cast<std::uint64_t>(ins.b) * roots_stride + e] = acc;
+        }
+        continue;
+      default:
+        break;
+    }
     double const a = P[ins.a * kTile + lane];
     double r;
-    switch (op) {
+    switch (static_cast<TOp>(op)) {
       case TOp::LoadConst: r = ins.imm; break;
       case TOp::Add: r = a + P[ins.b * kTile + lane]; break;
       case TOp::AddImm: r = a + ins.imm; break;
@@ -407,7 +519,7 @@ __device__ __forceinline__ PointResult run_point(ConstInstrPtr code, std::uint32
       default:
         continue;
     }
-    *dst = r;
+    P[ins.dst * kTile + lane] = r;
   }
   return res;
 }

@@ -120,8 +120,8 @@ def test_tape_shapes():
     assert ti.n_outputs == 17
     lines = listing.strip().splitlines()
     assert len(lines) == ti.n_instr
-    assert sum(1 for l in lines if " chkle " in l) == 16
-    assert sum(1 for l in lines if " obj " in l) == 1
+    assert sum(1 for l in lines if re.search(r" (acc\.)?chkle ", l)) == 16
+    assert sum(1 for l in lines if re.search(r" (acc\.)?obj ", l)) == 1
     # every slot a tape instruction names is below n_slots
     for l in lines:
         d = int(re.search(r"d=(\d+)", l).group(1))
