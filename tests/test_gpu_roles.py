@@ -85,7 +85,8 @@ def test_sampler_determinism_and_salt(ctx):
     bnd = cmi.Role(tape, C.ROLE_BOUNDER, [C.SLOT_CONTINUOUS], 4)
     f1, l1 = bnd.bound([[0.0, 10.0]], [0], [4])
     f2, l2 = bnd.bound([[0.0, 10.0]], [0], [4])
-    assert same_bits(l1, l2) and np.array_equal(f1, [1, 1, 0, 0])
+    # x <= 5 excludes only (7.5, 10]; [5, 7.5] touches the bound and stays
+    assert same_bits(l1, l2) and np.array_equal(f1, [1, 1, 1, 0]) and np.array_equal(f1, f2)
 
 
 def test_infeasible_domain_has_nan_witness(ctx):
