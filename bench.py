@@ -116,6 +116,10 @@ def main(argv=None) -> int:
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--serve", action="store_true")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU-baseline leg")
+    ap.add_argument("--box-groups", type=int, default=0,
+                    help="waves (function groups) per 64-box workgroup; 0 = chosen from the tape")
+    ap.add_argument("--sweep-groups", action="store_true",
+                    help="print the kernel time for each group count to stderr before the timed run")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run one 512 MiB device copy (read 512 MiB, write 512 MiB) after the timed region, as "
                          "the reference kernel that scales hardware byte counters")
@@ -139,7 +143,6 @@ def main(argv=None) -> int:
     ctx = cmi.Context(0)
     eng, lb, ub = make_workload()
     tape = cmi.Tape(ctx, eng)
-    info = tape.info()
 
     # The boxes are replicated into enough distinct buffers (> the 256 MB
     # Infinity Cache in total) that every step reads boxes that are not
@@ -160,20 +163,42 @@ def main(argv=None) -> int:
                                              C.c_void_p(d_feas.data_ptr()), C.c_void_p(d_olb.data_ptr()),
                                              C.c_void_p(d_oub.data_ptr()), None, C.c_void_p(stream.cuda_stream)))
 
-    stream = torch.cuda.current_stream(dev)
+    # A stream of our own: its handle is non-null, so the engine launches on it
+    # (a null handle would select the engine context's internal stream) and the
+    # HIP events below time exactly these launches.
+    stream = torch.cuda.Stream(dev)
+    torch.cuda.synchronize()
     for k in range(a.warmup):
         launch(k, stream)
     torch.cuda.synchronize()
 
-    # Kernel duration: HIP events around single launches on the stream they run on.
-    starts = [torch.cuda.Event(enable_timing=True) for _ in range(a.steps)]
-    ends = [torch.cuda.Event(enable_timing=True) for _ in range(a.steps)]
-    for k in range(a.steps):
-        starts[k].record(stream)
+    def kernel_ms(steps):
+        """Mean kernel duration: HIP events around single launches on the stream they run on."""
+        starts = [torch.cuda.Event(enable_timing=True) for _ in range(steps)]
+        ends = [torch.cuda.Event(enable_timing=True) for _ in range(steps)]
+        for k in range(steps):
+            starts[k].record(stream)
+            launch(k, stream)
+            ends[k].record(stream)
+        torch.cuda.synchronize()
+        return float(np.mean([s.elapsed_time(e) for s, e in zip(starts, ends)]))
+
+    if a.sweep_groups:
+        for g in range(1, 9):
+            got = tape.set_box_groups(g)
+            if got != g:
+                continue
+            for k in range(3):
+                launch(k, stream)
+            i = tape.info()
+            print(f"box_groups={g} instr={i.box_instr} slots={i.box_slots} kernel_ms={kernel_ms(20):.4f}",
+                  file=sys.stderr, flush=True)
+    tape.set_box_groups(a.box_groups)
+    info = tape.info()
+    for k in range(3):
         launch(k, stream)
-        ends[k].record(stream)
     torch.cuda.synchronize()
-    kern_ms = float(np.mean([s.elapsed_time(e) for s, e in zip(starts, ends)]))
+    kern_ms = kernel_ms(a.steps)
 
     # Throughput: the K steps are captured once into a HIP graph and replayed,
     # so the timed region is device work, not Python launch overhead.
@@ -233,6 +258,9 @@ def main(argv=None) -> int:
             "timed_region": "one HIP-graph replay of K launches",
             "tape_instructions": info.n_instr,
             "tape_slots": info.n_slots,
+            "box_groups": info.box_groups,
+            "box_instructions": info.box_instr,
+            "box_slots": info.box_slots,
             "reachable_nodes": info.reachable_nodes,
         },
         "roofline": {

@@ -78,7 +78,8 @@ class ProblemInfo(C.Structure):
 
 class TapeInfo(C.Structure):
     _fields_ = [("n_instr", C.c_size_t), ("n_slots", C.c_size_t), ("n_vars", C.c_size_t), ("n_outputs", C.c_size_t),
-                ("reachable_nodes", C.c_size_t), ("flops_per_box", C.c_uint64), ("bytes_per_box", C.c_size_t)]
+                ("reachable_nodes", C.c_size_t), ("flops_per_box", C.c_uint64), ("bytes_per_box", C.c_size_t),
+                ("box_groups", C.c_size_t), ("box_instr", C.c_size_t), ("box_slots", C.c_size_t)]
 
 
 class CostModel(C.Structure):
@@ -153,6 +154,8 @@ _SIGNATURES = {
     "cmi_tape_create": (C.c_int, [_P, _P, C.POINTER(_P)]),
     "cmi_tape_info_get": (C.c_int, [_P, C.POINTER(TapeInfo)]),
     "cmi_tape_destroy": (None, [_P]),
+    "cmi_tape_set_box_groups": (C.c_int, [_P, _P, C.c_size_t]),
+    "cmi_tape_compile_words": (C.c_int, [_P, C.c_size_t, _P, C.c_size_t, C.POINTER(C.c_uint32), C.POINTER(TapeInfo)]),
     "cmi_tape_compile_info": (C.c_int, [_P, C.POINTER(TapeInfo), C.c_char_p, C.c_size_t]),
     "cmi_eval_boxes": (C.c_int, [_P, _P, _P, C.c_size_t, C.c_size_t, _P, _P, _P, _P]),
     "cmi_eval_boxes_device": (C.c_int, [_P, _P, _P, C.c_size_t, C.c_size_t, _P, _P, _P, _P, _P]),

@@ -355,10 +355,64 @@ cmi_status cmi_tape_info_get(const cmi_tape* t, cmi_tape_info* out)
     out->reachable_nodes = tp.reachable_nodes;
     out->flops_per_box = tp.flops;
     out->bytes_per_box = 16 * static_cast<std::size_t>(tp.n_vars) + 16 + 1;
+    auto const& gt = t->tape->grouped_tape();
+    out->box_groups = gt.n_groups;
+    out->box_instr = gt.code.size();
+    out->box_slots = gt.n_slots;
   });
 }
 
 void cmi_tape_destroy(cmi_tape* t) { delete t; }
+
+cmi_status cmi_tape_set_box_groups(cmi_tape* t, const cmi_problem* p, size_t k)
+{
+  if (t == nullptr || p == nullptr) {
+    return fail(CMI_INVALID_HANDLE, "null argument");
+  }
+  return guarded([&] {
+    if (p->problem.n_vars() != t->tape->n_vars()) {
+      throw cmi::ShapeMismatch("problem does not match the tape");
+    }
+    t->tape->set_box_groups(p->problem, static_cast<std::uint32_t>(std::min<size_t>(k, 1u << 16)));
+  });
+}
+
+cmi_status cmi_tape_compile_words(const cmi_problem* p, size_t k, void* words, size_t cap_words,
+                                  uint32_t* group_begin, cmi_tape_info* out)
+{
+  if (p == nullptr || out == nullptr) {
+    return fail(CMI_INVALID_HANDLE, "null argument");
+  }
+  return guarded([&] {
+    p->problem.validate();
+    auto const whole = cmi::tape::compile(p->problem);
+    auto const tp = k <= 1 ? whole
+                           : cmi::tape::compile_groups(p->problem,
+                                                       static_cast<std::uint32_t>(std::min<size_t>(k, 1u << 16)));
+    out->n_instr = whole.code.size();
+    out->n_slots = whole.n_slots;
+    out->n_vars = whole.n_vars;
+    out->n_outputs = whole.n_outputs;
+    out->reachable_nodes = whole.reachable_nodes;
+    out->flops_per_box = whole.flops;
+    out->bytes_per_box = 16 * static_cast<std::size_t>(whole.n_vars) + 16 + 1;
+    out->box_groups = tp.n_groups;
+    out->box_instr = tp.code.size();
+    out->box_slots = tp.n_slots;
+    if (words != nullptr) {
+      if (cap_words < tp.code.size()) {
+        throw cmi::ShapeMismatch("word buffer holds " + std::to_string(cap_words) + " words; the tape has "
+                                 + std::to_string(tp.code.size()));
+      }
+      std::memcpy(words, tp.code.data(), tp.code.size() * sizeof(cmi::tape::Instr));
+    }
+    if (group_begin != nullptr) {
+      for (std::uint32_t g = 0; g <= tp.n_groups; ++g) {
+        group_begin[g] = tp.group_begin[g];
+      }
+    }
+  });
+}
 
 cmi_status cmi_tape_compile_info(const cmi_problem* p, cmi_tape_info* out, char* listing, size_t cap)
 {
@@ -375,6 +429,9 @@ cmi_status cmi_tape_compile_info(const cmi_problem* p, cmi_tape_info* out, char*
     out->reachable_nodes = tp.reachable_nodes;
     out->flops_per_box = tp.flops;
     out->bytes_per_box = 16 * static_cast<std::size_t>(tp.n_vars) + 16 + 1;
+    out->box_groups = 1;
+    out->box_instr = tp.code.size();
+    out->box_slots = tp.n_slots;
     if (listing != nullptr && cap > 0) {
       std::string const text = cmi::tape::disassemble(tp);
       std::size_t const k = std::min(cap - 1, text.size());

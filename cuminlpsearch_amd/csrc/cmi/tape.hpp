@@ -187,6 +187,12 @@ struct Tape
   std::uint32_t reachable_nodes = 0;  // non-Const DAG nodes the roots reach
   std::uint32_t node_ops = 0;         // DAG node operations the tape performs (leaves counted per use)
   std::uint64_t flops = 0;            // interval_flops summed over the tape
+  // Function groups: group g is code[group_begin[g], group_begin[g+1]) and
+  // uses the temporary slots [n_vars + g*temps_per_group, ...+temps_per_group).
+  std::uint32_t n_groups = 1;
+  std::vector<std::uint32_t> group_begin;
+  std::uint32_t temps_per_group = 0;
+  std::uint32_t objective_group = 0;  // the group holding the Objective instruction
 };
 
 struct CompileOptions
@@ -217,7 +223,10 @@ public:
   {
   }
 
-  Tape run()
+  /// Compile the functions whose root indices (0 = objective, 1 + j =
+  /// constraint j) are listed in `selected`, in that order; all of them when
+  /// `selected` is empty.
+  Tape run(std::vector<std::size_t> selected = {})
   {
     using model::Op;
     std::size_t const n = nodes_.size();
@@ -229,11 +238,17 @@ public:
     for (auto const& c : p_.constraints) {
       roots_.push_back(c.root_id);
     }
+    if (selected.empty()) {
+      for (std::size_t k = 0; k < roots_.size(); ++k) {
+        selected.push_back(k);
+      }
+    }
+    selected_ = selected;
     analyse();
 
     next_vreg_ = static_cast<std::int64_t>(n_vars);
     cache_.assign(n, -1);
-    for (std::size_t k = 0; k < roots_.size(); ++k) {
+    for (std::size_t k : selected_) {
       std::size_t const r = roots_[k];
       bool const is_obj = k == 0;
       double const rhs = is_obj ? 0.0 : p_.constraints[k - 1].rhs;
@@ -275,9 +290,9 @@ private:
     std::size_t const n = nodes_.size();
     reach_.assign(n, false);
     parents_.assign(n, 0);
-    for (std::size_t r : roots_) {
-      reach_[r] = true;
-      ++parents_[r];
+    for (std::size_t k : selected_) {
+      reach_[roots_[k]] = true;
+      ++parents_[roots_[k]];
     }
     for (std::size_t i = n; i-- > 0;) {
       if (!reach_[i]) {
@@ -681,6 +696,7 @@ private:
   const std::vector<model::DAGNode<double>>& nodes_;
   CompileOptions opt_;
   std::vector<std::size_t> roots_;
+  std::vector<std::size_t> selected_;
   std::vector<bool> reach_;
   std::vector<bool> leaf_;
   std::vector<std::uint32_t> parents_;
@@ -698,7 +714,84 @@ private:
 /// for ops the device evaluator does not implement.
 inline Tape compile(const model::Problem<double>& p, const CompileOptions& opt = {})
 {
-  return detail::Compiler(p, opt).run();
+  Tape t = detail::Compiler(p, opt).run();
+  t.group_begin = {0, static_cast<std::uint32_t>(t.code.size())};
+  t.temps_per_group = t.n_slots - t.n_vars;
+  return t;
+}
+
+/// Compile the objective and constraints into `n_groups` independent groups
+/// (each one wave's program over the same box), balancing instruction counts
+/// by longest-processing-time assignment. A node that several groups need is
+/// evaluated by each of them. The groups' temporaries get disjoint slot ranges
+/// so the waves of one workgroup share the variable slots and nothing else.
+inline Tape compile_groups(const model::Problem<double>& p, std::uint32_t n_groups, const CompileOptions& opt = {})
+{
+  std::size_t const n_roots = 1 + p.constraints.size();
+  n_groups = std::max<std::uint32_t>(1, std::min<std::uint32_t>(n_groups, static_cast<std::uint32_t>(n_roots)));
+  if (n_groups == 1) {
+    return compile(p, opt);
+  }
+  std::vector<std::pair<std::size_t, std::size_t>> cost;  // (instructions, root index)
+  for (std::size_t k = 0; k < n_roots; ++k) {
+    cost.push_back({detail::Compiler(p, opt).run({k}).code.size(), k});
+  }
+  std::stable_sort(cost.begin(), cost.end(), [](auto const& a, auto const& b) { return a.first > b.first; });
+  std::vector<std::vector<std::size_t>> members(n_groups);
+  std::vector<std::size_t> load(n_groups, 0);
+  for (auto const& [c, k] : cost) {
+    std::size_t const g = static_cast<std::size_t>(std::min_element(load.begin(), load.end()) - load.begin());
+    members[g].push_back(k);
+    load[g] += c;
+  }
+  std::vector<Tape> parts;
+  Tape t;
+  for (std::uint32_t g = 0; g < n_groups; ++g) {
+    auto& m = members[g];
+    std::sort(m.begin(), m.end());
+    if (!m.empty() && m.front() == 0) {
+      t.objective_group = g;
+    }
+    parts.push_back(detail::Compiler(p, opt).run(m));
+  }
+  t.n_vars = parts[0].n_vars;
+  t.n_constraints = static_cast<std::uint32_t>(p.constraints.size());
+  t.n_outputs = parts[0].n_outputs;
+  t.n_groups = n_groups;
+  for (auto const& q : parts) {
+    t.temps_per_group = std::max(t.temps_per_group, q.n_slots - q.n_vars);
+  }
+  Tape const whole = detail::Compiler(p, opt).run();
+  t.reachable_nodes = whole.reachable_nodes;
+  for (std::uint32_t g = 0; g < n_groups; ++g) {
+    t.group_begin.push_back(static_cast<std::uint32_t>(t.code.size()));
+    std::uint32_t const shift = g * t.temps_per_group;
+    auto remap = [&](std::uint16_t s) -> std::uint16_t {
+      return s >= t.n_vars ? static_cast<std::uint16_t>(s + shift) : s;
+    };
+    for (Instr ins : parts[g].code) {
+      bool const out = ins.op == static_cast<std::uint8_t>(TOp::Output)
+          || ins.op == static_cast<std::uint8_t>(TOp::AccOutput);
+      bool const writes = ins.op >= static_cast<std::uint8_t>(TOp::LoadConst) && ins.op <= static_cast<std::uint8_t>(TOp::Neg);
+      if (writes || ins.op == static_cast<std::uint8_t>(TOp::AccStore)) {
+        ins.dst = remap(ins.dst);
+      }
+      ins.a = remap(ins.a);
+      if (!out) {
+        ins.b = remap(ins.b);
+      }
+      t.code.push_back(ins);
+    }
+    t.node_ops += parts[g].node_ops;
+  }
+  t.flops = whole.flops;  // algorithmic work: every DAG operation once
+  t.group_begin.push_back(static_cast<std::uint32_t>(t.code.size()));
+  std::uint64_t const slots = std::uint64_t(t.n_vars) + std::uint64_t(n_groups) * t.temps_per_group;
+  if (slots > 65535) {
+    throw InvalidConfiguration("grouped tape needs more slots than the 16-bit encoding holds");
+  }
+  t.n_slots = static_cast<std::uint32_t>(std::max<std::uint64_t>(slots, 1));
+  return t;
 }
 
 /// Human-readable listing, one instruction per line.

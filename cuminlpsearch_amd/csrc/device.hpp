@@ -14,6 +14,16 @@ namespace cmi::dev
 
 inline constexpr unsigned kTile = 64;          // elements per workgroup: one wave64
 inline constexpr unsigned kMaxSlotsPerItem = 64;
+inline constexpr unsigned kMaxGroups = 8;      // waves per explicit-box workgroup
+
+/// Function groups of a grouped tape (tape::compile_groups): wave g of an
+/// explicit-box workgroup runs code[begin[g], begin[g+1]).
+struct GroupTable
+{
+  std::uint32_t n_groups;
+  std::uint32_t objective_group;
+  std::uint32_t begin[kMaxGroups + 1];
+};
 
 /// One unit of decode-mode work: a parent box split by a slot assignment,
 /// evaluated as a bounder (intervals), sampler (points drawn per region) or
@@ -62,12 +72,21 @@ void launch_items(Mode mode, const Item* items, const std::uint32_t* tile_item, 
 
 /// Explicit-box evaluation: box e's variable v is [lb[v*ld+e], ub[v*ld+e]].
 /// Writes feasible[e] and obj_lb[e]; obj_ub and roots (n_outputs intervals,
-/// output-major: roots[k*n_boxes+e]) when non-null.
-void launch_boxes(const tape::Instr* code, std::uint32_t n_instr, std::uint32_t n_slots, std::uint32_t n_vars,
+/// output-major: roots[k*n_boxes+e]) when non-null. One workgroup of
+/// groups.n_groups waves per 64 boxes; `code` holds every group's code
+/// followed by one Nop word.
+void launch_boxes(const tape::Instr* code, const GroupTable& groups, std::uint32_t n_slots, std::uint32_t n_vars,
                   const double* lb, const double* ub, std::uint64_t ld, std::uint64_t n_boxes,
                   std::uint8_t* feasible, double* obj_lb, double* obj_ub, double* roots, hipStream_t stream);
 
-/// Same as launch_boxes but evaluates points (round to nearest): x[v*ld+e].
+/// LDS bytes of one explicit-box workgroup over a tape with n_slots slots.
+std::size_t boxes_lds_bytes(std::uint32_t n_slots);
+
+/// LDS bytes one workgroup may use on the current device.
+std::size_t lds_limit_bytes();
+
+/// Points (round to nearest): x[v*ld+e], one wave per 64 points over an
+/// ungrouped tape of n_instr words followed by one Nop word.
 void launch_points(const tape::Instr* code, std::uint32_t n_instr, std::uint32_t n_slots, std::uint32_t n_vars,
                    const double* x, std::uint64_t ld, std::uint64_t n_points, std::uint8_t* feasible,
                    double* obj, double* roots, hipStream_t stream);

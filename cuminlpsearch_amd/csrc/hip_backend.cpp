@@ -56,35 +56,85 @@ std::size_t Context::free_bytes() const
 // DeviceTape
 // ---------------------------------------------------------------------------
 
+namespace
+{
+
+/// Upload a tape's words followed by one Nop word (the interpreters read one
+/// word ahead).
+void upload_code(const std::vector<tape::Instr>& code, DeviceBuffer& buf)
+{
+  std::vector<tape::Instr> padded(code);
+  padded.push_back(tape::Instr {});
+  buf.reset(padded.size() * sizeof(tape::Instr));
+  check(hipMemcpy(buf.get(), padded.data(), padded.size() * sizeof(tape::Instr), hipMemcpyHostToDevice),
+        "hipMemcpy(tape)");
+}
+
+/// Groups for explicit boxes: about one wave per 32 tape words, at most 8.
+std::uint32_t auto_box_groups(const tape::Tape& t)
+{
+  std::size_t const by_size = (t.code.size() + 31) / 32;
+  return static_cast<std::uint32_t>(std::clamp<std::size_t>(by_size, 1, dev::kMaxGroups));
+}
+
+}  // namespace
+
 DeviceTape::DeviceTape(std::shared_ptr<Context> ctx, const model::Problem<double>& problem,
-                       const tape::CompileOptions& opt)
+                       const tape::CompileOptions& opt, std::uint32_t box_groups)
     : ctx_(std::move(ctx))
+    , opt_(opt)
 {
   problem.validate();
   tape_ = tape::compile(problem, opt);
-  if (tape_.n_slots > ctx_->lds_interval_slots()) {
+  if (tape_.n_slots > ctx_->lds_interval_slots() || dev::boxes_lds_bytes(tape_.n_slots) > dev::lds_limit_bytes()) {
     throw ResourceExhausted("tape needs " + std::to_string(tape_.n_slots) + " slots per lane; LDS holds "
                             + std::to_string(ctx_->lds_interval_slots()) + " interval slots per 64-lane workgroup");
   }
   ctx_->activate();
-  code_.reset(std::max<std::size_t>(tape_.code.size(), 1) * sizeof(tape::Instr));
-  if (!tape_.code.empty()) {
-    check(hipMemcpy(code_.get(), tape_.code.data(), tape_.code.size() * sizeof(tape::Instr), hipMemcpyHostToDevice),
-          "hipMemcpy(tape)");
-  }
+  upload_code(tape_.code, code_);
   std::vector<std::uint8_t> kinds(problem.var_kinds.size());
   for (std::size_t i = 0; i < kinds.size(); ++i) {
     kinds[i] = static_cast<std::uint8_t>(problem.var_kinds[i]);
   }
   kinds_.reset(std::max<std::size_t>(kinds.size(), 1));
   check(hipMemcpy(kinds_.get(), kinds.data(), kinds.size(), hipMemcpyHostToDevice), "hipMemcpy(var kinds)");
+  set_box_groups(problem, box_groups);
+}
+
+void DeviceTape::set_box_groups(const model::Problem<double>& problem, std::uint32_t k)
+{
+  if (k == 0) {
+    k = auto_box_groups(tape_);
+  }
+  std::uint32_t const n_roots = 1 + tape_.n_constraints;
+  k = std::clamp<std::uint32_t>(k, 1, std::min<std::uint32_t>(dev::kMaxGroups, n_roots));
+  tape::Tape g = tape_;
+  for (; k > 1; --k) {
+    g = tape::compile_groups(problem, k, opt_);
+    if (dev::boxes_lds_bytes(g.n_slots) <= dev::lds_limit_bytes()) {
+      break;
+    }
+  }
+  if (k == 1) {
+    g = tape_;
+  }
+  ctx_->activate();
+  upload_code(g.code, grouped_code_);
+  grouped_ = std::move(g);
+  groups_ = {};
+  groups_.n_groups = grouped_.n_groups;
+  groups_.objective_group = grouped_.objective_group;
+  for (std::uint32_t i = 0; i <= grouped_.n_groups; ++i) {
+    groups_.begin[i] = grouped_.group_begin[i];
+  }
 }
 
 void DeviceTape::eval_boxes_device(const double* lb, const double* ub, std::size_t ld, std::size_t n,
                                    std::uint8_t* feasible, double* obj_lb, double* obj_ub, double* roots,
                                    hipStream_t stream) const
 {
-  dev::launch_boxes(code(), n_instr(), n_slots(), n_vars(), lb, ub, ld, n, feasible, obj_lb, obj_ub, roots, stream);
+  dev::launch_boxes(grouped_code_.as<tape::Instr>(), groups_, grouped_.n_slots, n_vars(), lb, ub, ld, n, feasible,
+                    obj_lb, obj_ub, roots, stream);
   check(hipGetLastError(), "launch(k_boxes)");
 }
 

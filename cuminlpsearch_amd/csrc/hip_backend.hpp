@@ -149,8 +149,17 @@ private:
 class DeviceTape
 {
 public:
+  /// box_groups: waves per explicit-box workgroup (0 = chosen from the tape).
   DeviceTape(std::shared_ptr<Context> ctx, const model::Problem<double>& problem,
-             const tape::CompileOptions& opt = {});
+             const tape::CompileOptions& opt = {}, std::uint32_t box_groups = 0);
+
+  /// Recompile the explicit-box code of `problem` (the one this tape was
+  /// built from) into k function groups (0 = automatic); k is clamped to
+  /// [1, min(8, 1 + n_constraints)] and lowered until the workgroup's slots
+  /// fit in LDS. Not safe while a launch reads the code.
+  void set_box_groups(const model::Problem<double>& problem, std::uint32_t k);
+  std::uint32_t box_groups() const { return grouped_.n_groups; }
+  const tape::Tape& grouped_tape() const { return grouped_; }
 
   const tape::Tape& tape() const { return tape_; }
   const tape::Instr* code() const { return code_.as<tape::Instr>(); }
@@ -168,8 +177,12 @@ public:
 
 private:
   std::shared_ptr<Context> ctx_;
+  tape::CompileOptions opt_;
   tape::Tape tape_;
+  tape::Tape grouped_;
+  dev::GroupTable groups_ {};
   DeviceBuffer code_;
+  DeviceBuffer grouped_code_;
   DeviceBuffer kinds_;
 };
 

@@ -1,28 +1,32 @@
 // Device evaluation of DAG tapes on gfx950 (MI355X, CDNA4, wave64).
 //
-// One workgroup is one wave64; each lane evaluates the tape for one element
-// (a box, a region, or a sample point). The tape word stream is uniform across
-// the wave and is read with scalar loads; per-lane slot values live in LDS,
-// slot-major ([slot][lane]) so every operand read or result write is one
-// conflict-free ds_read_b128/ds_write_b128 (16 B per lane, 1 KiB per wave).
+// Each lane evaluates the tape for one element (a box, a region, or a sample
+// point); a wave covers 64 elements. The tape word stream is uniform across the
+// wave and is read with scalar loads, one word ahead. Per-lane slot values live
+// in LDS, slot-major ([slot][lane]), so every operand read or result write is
+// one conflict-free ds_read_b128/ds_write_b128 (16 B per lane, 1 KiB per wave);
+// the accumulator that carries sums lives in VGPRs. For explicit boxes the
+// functions of one problem are split into groups, one wave per group, so the
+// waves of a workgroup evaluate the same 64 boxes together and share the
+// variable slots.
 //
 // Outward rounding. gfx950 has no per-instruction rounding modes for FP64, so
 // the interval interpreter runs inside a region where the MODE register's FP64
-// rounding field is set to round-toward-+infinity, and every lower bound is
-// computed as RD(x) = -RU(-x) through VALU source negation, which is exact.
-// Every rounding FP64 instruction of the interval path is written as inline
-// asm that names the region's token, so the compiler can neither fold the
-// negations away (-(-a*b) == a*b only holds in round-to-nearest) nor move an
-// operation out of the region; the region is entered and left by volatile asm
-// with a memory clobber, so the round-to-nearest variable set-up before it and
-// the result stores after it stay outside. The point (sampler/enumerator)
-// interpreter never enters the region and is ordinary round-to-nearest code.
+// rounding field is set to round-toward-+infinity, and intervals are held as
+// (-lb, ub): lower bounds are then computed as -RD(x) = RU(-x) through VALU
+// source negation, which is exact, and every interval operation is one
+// instruction per bound. Every rounding FP64 instruction of the interval path
+// is written as inline asm that names the region's token, so the compiler can
+// neither fold the negations away nor move an operation out of the region; the
+// region is entered and left by volatile asm with a memory clobber. The point
+// (sampler/enumerator) interpreter never enters the region and is ordinary
+// round-to-nearest code.
 //
 // The op semantics follow the reference's interval and point functors
-// (backend/graph/ops.cuh:24-355, kernels.cuh:177-340) for the QCQP op set; the
-// multiplication lower/upper bounds are the min/max of the four directed
-// endpoint products with signed zeros canonicalised to +0 (equal in value to
-// the IEEE-1788 case table for finite operands).
+// (backend/graph/ops.cuh:24-355, kernels.cuh:177-340) for the QCQP op set.
+// Multiplication takes the min/max of the four directed endpoint products
+// (equal in value to the IEEE-1788 case table for finite operands). A zero
+// bound in any output is reported as +0.
 
 #include <hip/hip_runtime.h>
 
@@ -43,7 +47,7 @@ namespace
 {
 
 // ---------------------------------------------------------------------------
-// Round-toward-+infinity region and directed FP64 primitives
+// Round-toward-+infinity region and the interval operations on (-lb, ub)
 // ---------------------------------------------------------------------------
 
 __device__ __forceinline__ int enter_round_up()
@@ -69,124 +73,165 @@ __device__ __forceinline__ void leave_round_up(int tok)
       : "memory");
 }
 
+/// An interval [lb, ub] held as nl = -lb and ub.
 struct Iv
 {
-  double lo, hi;
+  double nl, ub;
 };
 
-// RU(a + b)
-__device__ __forceinline__ double ru_add(double a, double b, int t)
+// [a] + [b]: -lb = RU(-al + -bl), ub = RU(au + bu)
+__device__ __forceinline__ Iv iv_add(Iv a, Iv b, int t)
 {
-  double r;
-  asm("v_add_f64 %0, %1, %2 ; %3" : "=v"(r) : "v"(a), "v"(b), "s"(t));
+  Iv r;
+  asm("v_add_f64 %0, %2, %3\n\t"
+      "v_add_f64 %1, %4, %5 ; %6"
+      : "=&v"(r.nl), "=&v"(r.ub)
+      : "v"(a.nl), "v"(b.nl), "v"(a.ub), "v"(b.ub), "s"(t));
   return r;
 }
 
-// RU(a - b)
-__device__ __forceinline__ double ru_sub(double a, double b, int t)
+// [a] - [b]: -lb = RU(-al + bu), ub = RU(au + -bl)
+__device__ __forceinline__ Iv iv_sub(Iv a, Iv b, int t)
 {
-  double r;
-  asm("v_add_f64 %0, %1, -%2 ; %3" : "=v"(r) : "v"(a), "v"(b), "s"(t));
+  Iv r;
+  asm("v_add_f64 %0, %2, %3\n\t"
+      "v_add_f64 %1, %4, %5 ; %6"
+      : "=&v"(r.nl), "=&v"(r.ub)
+      : "v"(a.nl), "v"(b.ub), "v"(a.ub), "v"(b.nl), "s"(t));
   return r;
 }
 
-// RD(a + b) = -RU(-a + -b)
-__device__ __forceinline__ double rd_add(double a, double b, int t)
+// [a] + c: -lb = RU(-al - c), ub = RU(au + c)
+__device__ __forceinline__ Iv iv_add_imm(Iv a, double c, int t)
 {
-  double r;
-  asm("v_add_f64 %0, -%1, -%2 ; %3" : "=v"(r) : "v"(a), "v"(b), "s"(t));
-  return -r;
+  Iv r;
+  asm("v_add_f64 %0, %2, -%4\n\t"
+      "v_add_f64 %1, %3, %4 ; %5"
+      : "=&v"(r.nl), "=&v"(r.ub)
+      : "v"(a.nl), "v"(a.ub), "v"(c), "s"(t));
+  return r;
 }
 
-// RD(a - b) = -RU(-a + b)
-__device__ __forceinline__ double rd_sub(double a, double b, int t)
+// [a] - c: -lb = RU(-al + c), ub = RU(au - c)
+__device__ __forceinline__ Iv iv_sub_imm(Iv a, double c, int t)
 {
-  double r;
-  asm("v_add_f64 %0, -%1, %2 ; %3" : "=v"(r) : "v"(a), "v"(b), "s"(t));
-  return -r;
+  Iv r;
+  asm("v_add_f64 %0, %2, %4\n\t"
+      "v_add_f64 %1, %3, -%4 ; %5"
+      : "=&v"(r.nl), "=&v"(r.ub)
+      : "v"(a.nl), "v"(a.ub), "v"(c), "s"(t));
+  return r;
 }
 
-__device__ __forceinline__ Iv iv_add(Iv a, Iv b, int t) { return {rd_add(a.lo, b.lo, t), ru_add(a.hi, b.hi, t)}; }
+// c - [a]: -lb = RU(au - c), ub = RU(c + -al)
+__device__ __forceinline__ Iv iv_imm_sub(double c, Iv a, int t)
+{
+  Iv r;
+  asm("v_add_f64 %0, %3, -%4\n\t"
+      "v_add_f64 %1, %4, %2 ; %5"
+      : "=&v"(r.nl), "=&v"(r.ub)
+      : "v"(a.nl), "v"(a.ub), "v"(c), "s"(t));
+  return r;
+}
 
-__device__ __forceinline__ Iv iv_sub(Iv a, Iv b, int t) { return {rd_sub(a.lo, b.hi, t), ru_sub(a.hi, b.lo, t)}; }
+__device__ __forceinline__ Iv iv_neg(Iv a) { return {a.ub, a.nl}; }
 
-// [a] + c
-__device__ __forceinline__ Iv iv_add_imm(Iv a, double c, int t) { return {rd_add(a.lo, c, t), ru_add(a.hi, c, t)}; }
+// The point interval [c, c] (exact: c + 0 and -c + 0 are c and -c, or +0).
+__device__ __forceinline__ Iv iv_const(double c, int t)
+{
+  Iv r;
+  asm("v_add_f64 %0, -%2, 0\n\t"
+      "v_add_f64 %1, %2, 0 ; %3"
+      : "=&v"(r.nl), "=&v"(r.ub)
+      : "v"(c), "s"(t));
+  return r;
+}
 
-// [a] - c
-__device__ __forceinline__ Iv iv_sub_imm(Iv a, double c, int t) { return {rd_sub(a.lo, c, t), ru_sub(a.hi, c, t)}; }
-
-// c - [a]
-__device__ __forceinline__ Iv iv_imm_sub(double c, Iv a, int t) { return {rd_sub(c, a.hi, t), ru_sub(c, a.lo, t)}; }
-
-__device__ __forceinline__ Iv iv_neg(Iv a) { return {-a.hi, -a.lo}; }
-
-// [xl,xu]*[yl,yu]: lo = -max RU(-xi*yj) + 0, hi = max RU(xi*yj) + 0. The "+ 0"
-// (exact for nonzero values) maps -0 to +0 in round-up mode.
+// [x] * [y] from the four endpoint products (xl = -xnl, yl = -ynl):
+// -lb = max RU(-xi*yj), ub = max RU(xi*yj).
 __device__ __forceinline__ Iv iv_mul(Iv x, Iv y, int t)
 {
-  double lo, hi, t0, t1, t2, t3;
-  asm("v_mul_f64 %2, -%6, %8\n\t"
-      "v_mul_f64 %3, -%6, %9\n\t"
-      "v_mul_f64 %4, -%7, %8\n\t"
-      "v_mul_f64 %5, -%7, %9\n\t"
+  Iv r;
+  double t0, t1, t2, t3;
+  asm("v_mul_f64 %2, -%6, %8\n\t"  // -(xl*yl) = -(xnl*ynl)
+      "v_mul_f64 %3, %6, %9\n\t"   // -(xl*yu) =   xnl*yu
+      "v_mul_f64 %4, %7, %8\n\t"   // -(xu*yl) =   xu*ynl
+      "v_mul_f64 %5, -%7, %9\n\t"  // -(xu*yu)
       "v_max_f64 %2, %2, %3\n\t"
       "v_max_f64 %4, %4, %5\n\t"
-      "v_max_f64 %2, %2, %4\n\t"
-      "v_add_f64 %0, -%2, 0\n\t"
-      "v_mul_f64 %2, %6, %8\n\t"
-      "v_mul_f64 %3, %6, %9\n\t"
-      "v_mul_f64 %4, %7, %8\n\t"
-      "v_mul_f64 %5, %7, %9\n\t"
+      "v_max_f64 %0, %2, %4\n\t"
+      "v_mul_f64 %2, %6, %8\n\t"   // xl*yl = xnl*ynl
+      "v_mul_f64 %3, -%6, %9\n\t"  // xl*yu = -xnl*yu
+      "v_mul_f64 %4, -%7, %8\n\t"  // xu*yl = -xu*ynl
+      "v_mul_f64 %5, %7, %9\n\t"   // xu*yu
       "v_max_f64 %2, %2, %3\n\t"
       "v_max_f64 %4, %4, %5\n\t"
-      "v_max_f64 %2, %2, %4\n\t"
-      "v_add_f64 %1, %2, 0 ; %10"
-      : "=&v"(lo), "=&v"(hi), "=&v"(t0), "=&v"(t1), "=&v"(t2), "=&v"(t3)
-      : "v"(x.lo), "v"(x.hi), "v"(y.lo), "v"(y.hi), "s"(t));
-  return {lo, hi};
+      "v_max_f64 %1, %2, %4 ; %10"
+      : "=&v"(r.nl), "=&v"(r.ub), "=&v"(t0), "=&v"(t1), "=&v"(t2), "=&v"(t3)
+      : "v"(x.nl), "v"(x.ub), "v"(y.nl), "v"(y.ub), "s"(t));
+  return r;
 }
 
-// [a] * c for a scalar c: c > 0 -> [RD(lo*c), RU(hi*c)], c < 0 ->
-// [RD(hi*c), RU(lo*c)], otherwise [0, 0]; zeros canonicalised to +0.
+// [a] * c for a (wave-uniform) scalar c: c > 0 -> [RD(al*c), RU(au*c)],
+// c < 0 -> [RD(au*c), RU(al*c)], otherwise [0, 0].
 __device__ __forceinline__ Iv iv_mul_imm(Iv a, double c, int t)
 {
-  if (c > 0.0 || c < 0.0) {
-    double const pl = c > 0.0 ? a.lo : a.hi;
-    double const ph = c > 0.0 ? a.hi : a.lo;
-    double lo, hi, nl;
-    asm("v_mul_f64 %2, -%3, %5\n\t"
-        "v_add_f64 %0, -%2, 0\n\t"
-        "v_mul_f64 %1, %4, %5\n\t"
-        "v_add_f64 %1, %1, 0 ; %6"
-        : "=&v"(lo), "=&v"(hi), "=&v"(nl)
-        : "v"(pl), "v"(ph), "v"(c), "s"(t));
-    return {lo, hi};
+  Iv r;
+  if (c > 0.0) {
+    asm("v_mul_f64 %0, %2, %4\n\t"  // -(al*c) = anl*c
+        "v_mul_f64 %1, %3, %4 ; %5"
+        : "=&v"(r.nl), "=&v"(r.ub)
+        : "v"(a.nl), "v"(a.ub), "v"(c), "s"(t));
+  } else if (c < 0.0) {
+    asm("v_mul_f64 %0, -%3, %4\n\t"  // -(au*c)
+        "v_mul_f64 %1, -%2, %4 ; %5"  // al*c = -anl*c
+        : "=&v"(r.nl), "=&v"(r.ub)
+        : "v"(a.nl), "v"(a.ub), "v"(c), "s"(t));
+  } else {
+    r = {0.0, 0.0};
   }
-  return {0.0, 0.0};
+  return r;
 }
 
-// [a]^2: lo = RD(s*s) with s the endpoint nearest zero (0 if a straddles it),
-// hi = RU(m*m) with m = max(|lo|, |hi|).
+// [a]^2: s = max(lb, min(ub, 0)) is the endpoint nearest zero, or 0 when the
+// interval straddles zero; -lb = RU(-s*s); ub = RU(m*m), m = max(|lb|, |ub|).
 __device__ __forceinline__ Iv iv_sqr(Iv a, int t)
 {
-  double const s = a.lo >= 0.0 ? a.lo : (a.hi <= 0.0 ? a.hi : 0.0);
-  double lo, hi, nl, m;
-  asm("v_mul_f64 %2, -%4, %4\n\t"
-      "v_add_f64 %0, -%2, 0\n\t"
-      "v_max_f64 %3, |%5|, |%6|\n\t"
-      "v_mul_f64 %1, %3, %3 ; %7"
-      : "=&v"(lo), "=&v"(hi), "=&v"(nl), "=&v"(m)
-      : "v"(s), "v"(a.lo), "v"(a.hi), "s"(t));
-  return {lo, hi};
+  Iv r;
+  double s, m;
+  asm("v_min_f64 %2, %5, 0\n\t"
+      "v_max_f64 %2, -%4, %2\n\t"
+      "v_mul_f64 %0, -%2, %2\n\t"
+      "v_max_f64 %3, |%4|, |%5|\n\t"
+      "v_mul_f64 %1, %3, %3 ; %6"
+      : "=&v"(r.nl), "=&v"(r.ub), "=&v"(s), "=&v"(m)
+      : "v"(a.nl), "v"(a.ub), "s"(t));
+  return r;
+}
+
+/// [lb, ub] with a zero bound reported as +0 (x + 0 is exact otherwise).
+__device__ __forceinline__ double2 to_output(Iv a, int t)
+{
+  double2 o;
+  asm("v_add_f64 %0, -%2, 0\n\t"
+      "v_add_f64 %1, %3, 0 ; %4"
+      : "=&v"(o.x), "=&v"(o.y)
+      : "v"(a.nl), "v"(a.ub), "s"(t));
+  return o;
 }
 
 // ---------------------------------------------------------------------------
 // Region decode and sampling (same arithmetic as region::decode::slot_bounds
-// and the reference's subregion_sampler.cuh:53-120)
+// and the reference's subregion_sampler.cuh:53-120), round to nearest
 // ---------------------------------------------------------------------------
 
-__device__ __forceinline__ Iv slot_bounds(std::uint8_t kind, Iv parent, std::uint32_t part, std::uint32_t fan_out)
+struct Bounds
+{
+  double lo, hi;
+};
+
+__device__ __forceinline__ Bounds slot_bounds(std::uint8_t kind, Bounds parent, std::uint32_t part,
+                                              std::uint32_t fan_out)
 {
   // SlotKind: 0 Continuous, 1 IntegerPartition, 2 IntegerEnumerate, 3 BinaryEnumerate
   if (kind >= 2) {
@@ -199,7 +244,7 @@ __device__ __forceinline__ Iv slot_bounds(std::uint8_t kind, Iv parent, std::uin
   }
   double const width = (parent.hi - parent.lo) / static_cast<double>(fan_out);
   double const step_lo = width * static_cast<double>(part);
-  Iv out;
+  Bounds out;
   out.lo = parent.lo + step_lo;
   if (part + 1 == fan_out) {
     out.hi = parent.hi;
@@ -246,7 +291,9 @@ __device__ __forceinline__ double draw(bool discrete, double lb, double ub, std:
 // ---------------------------------------------------------------------------
 
 // Tape words are read through the constant address space so the compiler
-// issues scalar loads (the pc is wave-uniform), as four whole dwords.
+// issues scalar loads (the pc is wave-uniform), as four whole dwords. Every
+// device tape carries one trailing Nop word, so reading one word past the last
+// instruction (the look-ahead fetch) stays in bounds.
 typedef std::uint32_t u32x4 __attribute__((ext_vector_type(4)));
 using ConstInstrPtr = const __attribute__((address_space(4))) u32x4*;
 
@@ -261,9 +308,8 @@ struct Word
   double imm;
 };
 
-__device__ __forceinline__ Word fetch(ConstInstrPtr code, std::uint32_t pc)
+__device__ __forceinline__ Word decode_word(u32x4 w)
 {
-  u32x4 const w = code[pc];
   Word out;
   out.op = w.x & 0xffu;
   out.dst = w.x >> 16;
@@ -285,17 +331,25 @@ __device__ __forceinline__ Iv load_iv(const double2* S, std::uint32_t slot, unsi
   return {v.x, v.y};
 }
 
-// Interval tape over LDS slots S[slot * kTile + lane] and the accumulator
-// register. Must run inside the round-up region (token t). Output
-// instructions store into roots[b * roots_stride + e] when roots is non-null.
+__device__ __forceinline__ void store_iv(double2* S, std::uint32_t slot, unsigned lane, Iv v)
+{
+  S[slot * kTile + lane] = make_double2(v.nl, v.ub);
+}
+
+// Interval tape code[0, n_instr) over LDS slots S[slot * kTile + lane] holding
+// (-lb, ub), and the accumulator register. Must run inside the round-up region
+// (token t). Output instructions store [lb, ub] into roots[b*stride + e] when
+// roots is non-null.
 __device__ __forceinline__ IntervalResult run_interval(ConstInstrPtr code, std::uint32_t n_instr, double2* S,
                                                        unsigned lane, int t, double2* roots,
                                                        std::uint64_t roots_stride, std::uint64_t e)
 {
   IntervalResult res {{0.0, 0.0}, true};
   Iv acc {0.0, 0.0};
+  u32x4 next = code[0];
   for (std::uint32_t pc = 0; pc < n_instr; ++pc) {
-    Word const ins = fetch(code, pc);
+    Word const ins = decode_word(next);
+    next = code[pc + 1];
     std::uint32_t const op = ins.op;
     if (op >= tape::kTermBase) {
       // acc <- acc (set|+|-) term(a, b, imm)
@@ -303,117 +357,79 @@ __device__ __forceinline__ IntervalResult run_interval(ConstInstrPtr code, std::
       std::uint32_t const mode = (op - tape::kTermBase) >> 3;
       Iv const a = load_iv(S, ins.a, lane);
       Iv x;
-      if (term == static_cast<std::uint32_t>(tape::Term::S)) {
-        x = a;
-      } else if (term == static_cast<std::uint32_t>(tape::Term::SC)) {
-        x = iv_mul_imm(a, ins.imm, t);
-      } else if (term == static_cast<std::uint32_t>(tape::Term::Q)) {
-        x = iv_sqr(a, t);
+      if (term == static_cast<std::uint32_t>(tape::Term::SSC)) {
+        x = iv_mul_imm(iv_mul(a, load_iv(S, ins.b, lane), t), ins.imm, t);
       } else if (term == static_cast<std::uint32_t>(tape::Term::QC)) {
         x = iv_mul_imm(iv_sqr(a, t), ins.imm, t);
+      } else if (term == static_cast<std::uint32_t>(tape::Term::SC)) {
+        x = iv_mul_imm(a, ins.imm, t);
+      } else if (term == static_cast<std::uint32_t>(tape::Term::SS)) {
+        x = iv_mul(a, load_iv(S, ins.b, lane), t);
+      } else if (term == static_cast<std::uint32_t>(tape::Term::Q)) {
+        x = iv_sqr(a, t);
       } else {
-        Iv const b = load_iv(S, ins.b, lane);
-        x = iv_mul(a, b, t);
-        if (term == static_cast<std::uint32_t>(tape::Term::SSC)) {
-          x = iv_mul_imm(x, ins.imm, t);
-        }
+        x = a;
       }
-      if (mode == static_cast<std::uint32_t>(tape::AccMode::Set)) {
-        acc = x;
-      } else if (mode == static_cast<std::uint32_t>(tape::AccMode::Add)) {
+      if (mode == static_cast<std::uint32_t>(tape::AccMode::Add)) {
         acc = iv_add(acc, x, t);
+      } else if (mode == static_cast<std::uint32_t>(tape::AccMode::Set)) {
+        acc = x;
       } else {
         acc = iv_sub(acc, x, t);
       }
       continue;
     }
-    switch (static_cast<TOp>(op)) {
-      case TOp::AccStore:
-        S[ins.dst * kTile + lane] = make_double2(acc.lo, acc.hi);
-        continue;
-      case TOp::AccAddImm:
-        acc = iv_add_imm(acc, ins.imm, t);
-        continue;
-      case TOp::AccSubImm:
-        acc = iv_sub_imm(acc, ins.imm, t);
-        continue;
-      case TOp::AccImmSub:
-        acc = iv_imm_sub(ins.imm, acc, t);
-        continue;
-      case TOp::AccScale:
-        acc = iv_mul_imm(acc, ins.imm, t);
-        continue;
-      case TOp::AccNeg:
-        acc = iv_neg(acc);
-        continue;
-      case TOp::AccCheckLE:
-        res.feasible = res.feasible && (acc.lo <= ins.imm);
-        continue;
-      case TOp::AccCheckEQ:
-        res.feasible = res.feasible && (acc.lo <= ins.imm) && (ins.imm <= acc.hi);
-        continue;
-      case TOp::AccObjective:
-        res.obj = acc;
-        continue;
-      case TOp::AccOutput:
-        if (roots != nullptr) {
-          roots[static_cast<std::uint64_t>(ins.b) * roots_stride + e] = make_double2(acc.lo, acc.hi);
-        }
-        continue;
-      default:
-        break;
+    if (op >= static_cast<std::uint32_t>(TOp::AccStore)) {
+      switch (static_cast<TOp>(op)) {
+        case TOp::AccStore: store_iv(S, ins.dst, lane, acc); break;
+        case TOp::AccAddImm: acc = iv_add_imm(acc, ins.imm, t); break;
+        case TOp::AccSubImm: acc = iv_sub_imm(acc, ins.imm, t); break;
+        case TOp::AccImmSub: acc = iv_imm_sub(ins.imm, acc, t); break;
+        case TOp::AccScale: acc = iv_mul_imm(acc, ins.imm, t); break;
+        case TOp::AccNeg: acc = iv_neg(acc); break;
+        case TOp::AccCheckLE: res.feasible = res.feasible && (-acc.nl <= ins.imm); break;
+        case TOp::AccCheckEQ: res.feasible = res.feasible && (-acc.nl <= ins.imm) && (ins.imm <= acc.ub); break;
+        case TOp::AccObjective: res.obj = acc; break;
+        case TOp::AccOutput:
+          if (roots != nullptr) {
+            roots[static_cast<std::uint64_t>(ins.b) * roots_stride + e] = to_output(acc, t);
+          }
+          break;
+        default: break;
+      }
+      continue;
     }
     Iv const a = load_iv(S, ins.a, lane);
     Iv r;
     switch (static_cast<TOp>(op)) {
-      case TOp::LoadConst:
-        r = {ins.imm, ins.imm};
-        break;
-      case TOp::Add:
-        r = iv_add(a, load_iv(S, ins.b, lane), t);
-        break;
-      case TOp::AddImm:
-        r = iv_add_imm(a, ins.imm, t);
-        break;
-      case TOp::Sub:
-        r = iv_sub(a, load_iv(S, ins.b, lane), t);
-        break;
-      case TOp::SubImm:
-        r = iv_sub_imm(a, ins.imm, t);
-        break;
-      case TOp::ImmSub:
-        r = iv_imm_sub(ins.imm, a, t);
-        break;
-      case TOp::Mul:
-        r = iv_mul(a, load_iv(S, ins.b, lane), t);
-        break;
-      case TOp::MulImm:
-        r = iv_mul_imm(a, ins.imm, t);
-        break;
-      case TOp::Sqr:
-        r = iv_sqr(a, t);
-        break;
-      case TOp::Neg:
-        r = iv_neg(a);
-        break;
+      case TOp::LoadConst: r = iv_const(ins.imm, t); break;
+      case TOp::Add: r = iv_add(a, load_iv(S, ins.b, lane), t); break;
+      case TOp::AddImm: r = iv_add_imm(a, ins.imm, t); break;
+      case TOp::Sub: r = iv_sub(a, load_iv(S, ins.b, lane), t); break;
+      case TOp::SubImm: r = iv_sub_imm(a, ins.imm, t); break;
+      case TOp::ImmSub: r = iv_imm_sub(ins.imm, a, t); break;
+      case TOp::Mul: r = iv_mul(a, load_iv(S, ins.b, lane), t); break;
+      case TOp::MulImm: r = iv_mul_imm(a, ins.imm, t); break;
+      case TOp::Sqr: r = iv_sqr(a, t); break;
+      case TOp::Neg: r = iv_neg(a); break;
       case TOp::CheckLE:
-        res.feasible = res.feasible && (a.lo <= ins.imm);
+        res.feasible = res.feasible && (-a.nl <= ins.imm);
         continue;
       case TOp::CheckEQ:
-        res.feasible = res.feasible && (a.lo <= ins.imm) && (ins.imm <= a.hi);
+        res.feasible = res.feasible && (-a.nl <= ins.imm) && (ins.imm <= a.ub);
         continue;
       case TOp::Objective:
         res.obj = a;
         continue;
       case TOp::Output:
         if (roots != nullptr) {
-          roots[static_cast<std::uint64_t>(ins.b) * roots_stride + e] = make_double2(a.lo, a.hi);
+          roots[static_cast<std::uint64_t>(ins.b) * roots_stride + e] = to_output(a, t);
         }
         continue;
       default:
         continue;
     }
-    S[ins.dst * kTile + lane] = make_double2(r.lo, r.hi);
+    store_iv(S, ins.dst, lane, r);
   }
   return res;
 }
@@ -440,54 +456,57 @@ __device__ __forceinline__ PointResult run_point(ConstInstrPtr code, std::uint32
 {
   PointResult res {0.0, true};
   double acc = 0.0;
+  u32x4 next = code[0];
   for (std::uint32_t pc = 0; pc < n_instr; ++pc) {
-    Word const ins = fetch(code, pc);
+    Word const ins = decode_word(next);
+    next = code[pc + 1];
     std::uint32_t const op = ins.op;
     if (op >= tape::kTermBase) {
       std::uint32_t const term = op & 7u;
       std::uint32_t const mode = (op - tape::kTermBase) >> 3;
       double const a = P[ins.a * kTile + lane];
       double x;
-      if (term == static_cast<std::uint32_t>(tape::Term::S)) {
-        x = a;
-      } else if (term == static_cast<std::uint32_t>(tape::Term::SC)) {
-        x = a * ins.imm;
-      } else if (term == static_cast<std::uint32_t>(tape::Term::Q)) {
-        x = a * a;
+      if (term == static_cast<std::uint32_t>(tape::Term::SSC)) {
+        x = (a * P[ins.b * kTile + lane]) * ins.imm;
       } else if (term == static_cast<std::uint32_t>(tape::Term::QC)) {
         x = (a * a) * ins.imm;
-      } else {
+      } else if (term == static_cast<std::uint32_t>(tape::Term::SC)) {
+        x = a * ins.imm;
+      } else if (term == static_cast<std::uint32_t>(tape::Term::SS)) {
         x = a * P[ins.b * kTile + lane];
-        if (term == static_cast<std::uint32_t>(tape::Term::SSC)) {
-          x = x * ins.imm;
-        }
+      } else if (term == static_cast<std::uint32_t>(tape::Term::Q)) {
+        x = a * a;
+      } else {
+        x = a;
       }
-      if (mode == static_cast<std::uint32_t>(tape::AccMode::Set)) {
-        acc = x;
-      } else if (mode == static_cast<std::uint32_t>(tape::AccMode::Add)) {
+      if (mode == static_cast<std::uint32_t>(tape::AccMode::Add)) {
         acc = acc + x;
+      } else if (mode == static_cast<std::uint32_t>(tape::AccMode::Set)) {
+        acc = x;
       } else {
         acc = acc - x;
       }
       continue;
     }
-    switch (static_cast<TOp>(op)) {
-      case TOp::AccStore: P[ins.dst * kTile + lane] = acc; continue;
-      case TOp::AccAddImm: acc = acc + ins.imm; continue;
-      case TOp::AccSubImm: acc = acc - ins.imm; continue;
-      case TOp::AccImmSub: acc = ins.imm - acc; continue;
-      case TOp::AccScale: acc = acc * ins.imm; continue;
-      case TOp::AccNeg: acc = -acc; continue;
-      case TOp::AccCheckLE: res.feasible = res.feasible && (acc <= ins.imm); continue;
-      case TOp::AccCheckEQ: res.feasible = res.feasible && almost_equal(acc, ins.imm); continue;
-      case TOp::AccObjective: res.obj = acc; continue;
-      case TOp::AccOutput:
-        if (roots != nullptr) {
-          roots[static_cast<std::uint64_t>(ins.b) * roots_stride + e] = acc;
-        }
-        continue;
-      default:
-        break;
+    if (op >= static_cast<std::uint32_t>(TOp::AccStore)) {
+      switch (static_cast<TOp>(op)) {
+        case TOp::AccStore: P[ins.dst * kTile + lane] = acc; break;
+        case TOp::AccAddImm: acc = acc + ins.imm; break;
+        case TOp::AccSubImm: acc = acc - ins.imm; break;
+        case TOp::AccImmSub: acc = ins.imm - acc; break;
+        case TOp::AccScale: acc = acc * ins.imm; break;
+        case TOp::AccNeg: acc = -acc; break;
+        case TOp::AccCheckLE: res.feasible = res.feasible && (acc <= ins.imm); break;
+        case TOp::AccCheckEQ: res.feasible = res.feasible && almost_equal(acc, ins.imm); break;
+        case TOp::AccObjective: res.obj = acc; break;
+        case TOp::AccOutput:
+          if (roots != nullptr) {
+            roots[static_cast<std::uint64_t>(ins.b) * roots_stride + e] = acc;
+          }
+          break;
+        default: break;
+      }
+      continue;
     }
     double const a = P[ins.a * kTile + lane];
     double r;
@@ -542,30 +561,56 @@ __device__ __forceinline__ void wave_argmin(double& v, std::int64_t& i)
 // Kernels
 // ---------------------------------------------------------------------------
 
-__global__ __launch_bounds__(kTile) void k_boxes(const Instr* __restrict__ code_g, std::uint32_t n_instr,
-                                                  std::uint32_t n_vars, const double* __restrict__ lb,
-                                                  const double* __restrict__ ub, std::uint64_t ld,
-                                                  std::uint64_t n_boxes, std::uint8_t* __restrict__ feasible,
-                                                  double* __restrict__ obj_lb, double* __restrict__ obj_ub,
-                                                  double2* __restrict__ roots)
+// Explicit boxes, one wave per function group: workgroup = n_groups waves over
+// the same 64 boxes. Variables are loaded once per workgroup into the shared
+// slots; each wave runs its group's code; wave 0 combines the groups'
+// feasibility and writes the outputs.
+__global__ __launch_bounds__(kTile * kMaxGroups) void k_boxes(const Instr* __restrict__ code_g, GroupTable groups,
+                                                               std::uint32_t n_vars, std::uint32_t n_slots,
+                                                               const double* __restrict__ lb,
+                                                               const double* __restrict__ ub, std::uint64_t ld,
+                                                               std::uint64_t n_boxes,
+                                                               std::uint8_t* __restrict__ feasible,
+                                                               double* __restrict__ obj_lb,
+                                                               double* __restrict__ obj_ub, double2* __restrict__ roots)
 {
   extern __shared__ double2 S[];
-  unsigned const lane = threadIdx.x;
+  unsigned const lane = threadIdx.x % kTile;
+  // the wave index is uniform across the wave; say so, so the tape pointer,
+  // the word fetches and the dispatch stay scalar
+  unsigned const wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kTile);
+  unsigned const n_groups = groups.n_groups;
   std::uint64_t const e = static_cast<std::uint64_t>(blockIdx.x) * kTile + lane;
   bool const active = e < n_boxes;
   std::uint64_t const ee = active ? e : n_boxes - 1;
-  for (std::uint32_t v = 0; v < n_vars; ++v) {
-    S[v * kTile + lane] = make_double2(lb[v * ld + ee], ub[v * ld + ee]);
+  for (std::uint32_t v = wave; v < n_vars; v += n_groups) {
+    S[v * kTile + lane] = make_double2(-lb[v * ld + ee], ub[v * ld + ee]);
   }
+  __syncthreads();
+  std::uint32_t const begin = groups.begin[wave];
   int const t = enter_round_up();
-  IntervalResult const res =
-      run_interval(as_constant(code_g), n_instr, S, lane, t, active ? roots : nullptr, n_boxes, e);
+  IntervalResult const res = run_interval(as_constant(code_g + begin), groups.begin[wave + 1] - begin, S, lane, t,
+                                          active ? roots : nullptr, n_boxes, e);
+  double2 const obj = to_output(res.obj, t);
   leave_round_up(t);
-  if (active) {
-    feasible[e] = res.feasible ? 1 : 0;
-    obj_lb[e] = res.obj.lo;
+  // per-group results after the slots: feasibility [n_groups][64], objective [64]
+  auto* feas_g = reinterpret_cast<std::uint32_t*>(S + n_slots * kTile);
+  auto* obj_s = reinterpret_cast<double2*>(feas_g + kMaxGroups * kTile);
+  feas_g[wave * kTile + lane] = res.feasible ? 1u : 0u;
+  if (wave == groups.objective_group) {
+    obj_s[lane] = obj;
+  }
+  __syncthreads();
+  if (wave == 0 && active) {
+    bool f = true;
+    for (unsigned g = 0; g < n_groups; ++g) {
+      f = f && feas_g[g * kTile + lane] != 0u;
+    }
+    double2 const o = obj_s[lane];
+    feasible[e] = f ? 1 : 0;
+    obj_lb[e] = o.x;
     if (obj_ub != nullptr) {
-      obj_ub[e] = res.obj.hi;
+      obj_ub[e] = o.y;
     }
   }
 }
@@ -613,7 +658,7 @@ __global__ __launch_bounds__(kTile) void k_items(const Item* __restrict__ items,
   if constexpr (M == static_cast<int>(Mode::Bound)) {
     double2* const Sl = lds;
     for (std::uint32_t v = 0; v < n_vars; ++v) {
-      Sl[v * kTile + lane] = make_double2(it.parent[2 * v], it.parent[2 * v + 1]);
+      Sl[v * kTile + lane] = make_double2(-it.parent[2 * v], it.parent[2 * v + 1]);
     }
     std::uint32_t idx = region;
     for (std::uint32_t j = 0; j < it.slot_count; ++j) {
@@ -621,15 +666,16 @@ __global__ __launch_bounds__(kTile) void k_items(const Item* __restrict__ items,
       std::uint32_t const part = idx % w;
       idx /= w;
       std::uint32_t const vid = it.slot_var[j];
-      Iv const b = slot_bounds(it.slot_kind[j], {it.parent[2 * vid], it.parent[2 * vid + 1]}, part, w);
-      Sl[vid * kTile + lane] = make_double2(b.lo, b.hi);
+      Bounds const b = slot_bounds(it.slot_kind[j], {it.parent[2 * vid], it.parent[2 * vid + 1]}, part, w);
+      Sl[vid * kTile + lane] = make_double2(-b.lo, b.hi);
     }
     int const t = enter_round_up();
     IntervalResult const res = run_interval(code, it.n_instr, Sl, lane, t, nullptr, 0, 0);
+    double2 const obj = to_output(res.obj, t);
     leave_round_up(t);
     if (active) {
       it.feasible[region] = res.feasible ? 1 : 0;
-      it.obj_lb[region] = res.obj.lo;
+      it.obj_lb[region] = obj.x;
     }
     return;
   } else {
@@ -646,7 +692,7 @@ __global__ __launch_bounds__(kTile) void k_items(const Item* __restrict__ items,
       std::uint32_t const part = idx % w;
       idx /= w;
       std::uint32_t const vid = it.slot_var[j];
-      Iv const b = slot_bounds(it.slot_kind[j], {it.parent[2 * vid], it.parent[2 * vid + 1]}, part, w);
+      Bounds const b = slot_bounds(it.slot_kind[j], {it.parent[2 * vid], it.parent[2 * vid + 1]}, part, w);
       P[vid * kTile + lane] = sampling ? draw(it.var_kinds[vid] != 0, b.lo, b.hi, vid, sample, region, it.salt) : b.lo;
     }
     PointResult const res = run_point(code, it.n_instr, P, lane, nullptr, 0, 0);
@@ -701,7 +747,7 @@ __global__ __launch_bounds__(kTile) void k_items(const Item* __restrict__ items,
         std::uint32_t const part = bidx % w;
         bidx /= w;
         if (it.slot_var[j] == v) {
-          Iv const b = slot_bounds(it.slot_kind[j], {lo, hi}, part, w);
+          Bounds const b = slot_bounds(it.slot_kind[j], {lo, hi}, part, w);
           lo = b.lo;
           hi = b.hi;
         }
@@ -712,8 +758,7 @@ __global__ __launch_bounds__(kTile) void k_items(const Item* __restrict__ items,
   }
 }
 
-template<typename K>
-void allow_large_lds(K kernel)
+int lds_optin_bytes()
 {
   int dev = 0;
   (void)hipGetDevice(&dev);
@@ -721,22 +766,29 @@ void allow_large_lds(K kernel)
   if (hipDeviceGetAttribute(&optin, hipDeviceAttributeSharedMemPerBlockOptin, dev) != hipSuccess || optin <= 0) {
     (void)hipDeviceGetAttribute(&optin, hipDeviceAttributeMaxSharedMemoryPerBlock, dev);
   }
+  return optin;
+}
+
+template<typename K>
+void allow_large_lds(K kernel)
+{
   (void)hipFuncSetAttribute(reinterpret_cast<const void*>(kernel), hipFuncAttributeMaxDynamicSharedMemorySize,
-                            optin);
+                            lds_optin_bytes());
 }
 
 }  // namespace
 
 std::uint32_t max_lds_slots(bool interval)
 {
-  int dev = 0;
-  (void)hipGetDevice(&dev);
-  int optin = 0;
-  if (hipDeviceGetAttribute(&optin, hipDeviceAttributeSharedMemPerBlockOptin, dev) != hipSuccess || optin <= 0) {
-    (void)hipDeviceGetAttribute(&optin, hipDeviceAttributeMaxSharedMemoryPerBlock, dev);
-  }
-  return static_cast<std::uint32_t>(optin) / (kTile * (interval ? 16u : 8u));
+  return static_cast<std::uint32_t>(lds_optin_bytes()) / (kTile * (interval ? 16u : 8u));
 }
+
+std::size_t boxes_lds_bytes(std::uint32_t n_slots)
+{
+  return static_cast<std::size_t>(n_slots) * kTile * 16 + kMaxGroups * kTile * 4 + kTile * 16;
+}
+
+std::size_t lds_limit_bytes() { return static_cast<std::size_t>(lds_optin_bytes()); }
 
 void launch_items(Mode mode, const Item* items, const std::uint32_t* tile_item, std::uint32_t n_tiles,
                   std::uint32_t lds_slots, hipStream_t stream)
@@ -767,7 +819,7 @@ void launch_items(Mode mode, const Item* items, const std::uint32_t* tile_item, 
   }
 }
 
-void launch_boxes(const Instr* code, std::uint32_t n_instr, std::uint32_t n_slots, std::uint32_t n_vars,
+void launch_boxes(const Instr* code, const GroupTable& groups, std::uint32_t n_slots, std::uint32_t n_vars,
                   const double* lb, const double* ub, std::uint64_t ld, std::uint64_t n_boxes, std::uint8_t* feasible,
                   double* obj_lb, double* obj_ub, double* roots, hipStream_t stream)
 {
@@ -777,9 +829,9 @@ void launch_boxes(const Instr* code, std::uint32_t n_instr, std::uint32_t n_slot
   static bool once = (allow_large_lds(k_boxes), true);
   (void)once;
   std::uint64_t const blocks = (n_boxes + kTile - 1) / kTile;
-  std::size_t const bytes = static_cast<std::size_t>(n_slots) * kTile * 16;
-  hipLaunchKernelGGL(k_boxes, dim3(static_cast<unsigned>(blocks)), dim3(kTile), bytes, stream, code, n_instr,
-                     n_vars, lb, ub, ld, n_boxes, feasible, obj_lb, obj_ub, reinterpret_cast<double2*>(roots));
+  hipLaunchKernelGGL(k_boxes, dim3(static_cast<unsigned>(blocks)), dim3(kTile * groups.n_groups),
+                     boxes_lds_bytes(n_slots), stream, code, groups, n_vars, n_slots, lb, ub, ld, n_boxes, feasible,
+                     obj_lb, obj_ub, reinterpret_cast<double2*>(roots));
 }
 
 void launch_points(const Instr* code, std::uint32_t n_instr, std::uint32_t n_slots, std::uint32_t n_vars,

@@ -17,6 +17,11 @@ def _handle() -> C.c_void_p:
     return C.c_void_p()
 
 
+# One tape instruction as the device reads it (16 bytes).
+TAPE_WORD = np.dtype([("op", "u1"), ("pad", "u1"), ("dst", "<u2"), ("a", "<u2"), ("b", "<u2"), ("imm", "<f8")])
+assert TAPE_WORD.itemsize == 16
+
+
 class EngineProblem:
     """A validated Problem held by the engine (cmi_problem)."""
 
@@ -69,6 +74,17 @@ class EngineProblem:
         _c.check(_c.lib().cmi_tape_compile_info(self._h, C.byref(out), buf, (1 << 22) if listing else 0))
         return out, (buf.value.decode() if listing else None)
 
+    def tape_words(self, groups: int = 1):
+        """Host-only compile into `groups` function groups: (words, group_begin,
+        TapeInfo); words is a structured array with the 16-byte tape layout."""
+        info = _c.TapeInfo()
+        _c.check(_c.lib().cmi_tape_compile_words(self._h, groups, None, 0, None, C.byref(info)))
+        words = np.zeros(info.box_instr, TAPE_WORD)
+        begin = np.zeros(info.box_groups + 1, np.uint32)
+        _c.check(_c.lib().cmi_tape_compile_words(self._h, groups, _c.ptr(words), words.size,
+                                                 begin.ctypes.data_as(C.POINTER(C.c_uint32)), C.byref(info)))
+        return words, begin, info
+
     @property
     def handle(self) -> C.c_void_p:
         return self._h
@@ -116,6 +132,12 @@ class Tape:
         out = _c.TapeInfo()
         _c.check(_c.lib().cmi_tape_info_get(self._h, C.byref(out)))
         return out
+
+    def set_box_groups(self, k: int = 0) -> int:
+        """Waves (function groups) per explicit-box workgroup; 0 = automatic.
+        Returns the count in effect after clamping."""
+        _c.check(_c.lib().cmi_tape_set_box_groups(self._h, self.problem.handle, k))
+        return int(self.info().box_groups)
 
     def eval_boxes(self, lb: np.ndarray, ub: np.ndarray, roots: bool = False):
         """lb, ub: (n_vars, n_boxes) float64. Returns (feasible, obj_lb, obj_ub, roots or None)."""

@@ -170,6 +170,9 @@ typedef struct
   size_t reachable_nodes;
   uint64_t flops_per_box; /* algorithmic FP64 ops per box of the interval tape */
   size_t bytes_per_box;   /* algorithmic HBM bytes per explicit box: 16 n_vars + 16 + 1 */
+  size_t box_groups;      /* waves per explicit-box workgroup (function groups) */
+  size_t box_instr;       /* tape words over all groups of the explicit-box code */
+  size_t box_slots;       /* LDS slots per lane of the explicit-box code */
 } cmi_tape_info;
 
 /* Affine device cost per role, reference backend/cost_model.hpp:26-76 */
@@ -272,10 +275,27 @@ void cmi_ctx_destroy(cmi_ctx* ctx);
 cmi_status cmi_tape_create(cmi_ctx* ctx, const cmi_problem* p, cmi_tape** out);
 cmi_status cmi_tape_info_get(const cmi_tape* t, cmi_tape_info* out);
 void cmi_tape_destroy(cmi_tape* t);
+/* Split the explicit-box code of tape `t` (compiled from `p`) into k function
+ * groups, one wave each over the same 64 boxes (0 = chosen from the tape's
+ * size). k is clamped to [1, min(8, 1 + n_constraints)] and lowered until the
+ * workgroup fits in LDS. Results do not depend on k. No reference
+ * counterpart: the reference launches one kernel per DAG node
+ * (backend/graph/builder.cuh:261-514); this is its wave-shape knob. Must not
+ * overlap a launch on the tape. */
+cmi_status cmi_tape_set_box_groups(cmi_tape* t, const cmi_problem* p, size_t k);
 /* Compile the tape on the host only (no device) and report its shape; also
  * writes the listing into `listing` (up to `cap` bytes, NUL-terminated) when
  * non-NULL. Same checks and errors as cmi_tape_create. */
 cmi_status cmi_tape_compile_info(const cmi_problem* p, cmi_tape_info* out, char* listing, size_t cap);
+/* Host-only compile into k function groups (k <= 1: one group, the tape the
+ * decode-mode roles run; k is clamped to min(8, 1 + n_constraints)). Writes
+ * the shape into `out` (box_groups = the k used), the 16-byte tape words into
+ * `words` when non-NULL (SHAPE_MISMATCH if cap_words < out->box_instr) and
+ * the box_groups + 1 group offsets into `group_begin` when non-NULL. Lets a
+ * host-side interpreter replay exactly the words a device runs. No reference
+ * counterpart (the reference has no tape). */
+cmi_status cmi_tape_compile_words(const cmi_problem* p, size_t k, void* words, size_t cap_words,
+                                  uint32_t* group_begin, cmi_tape_info* out);
 
 /* Evaluate explicit boxes (box e, variable v: [lb[v*ld+e], ub[v*ld+e]]):
  * feasible[e] and obj_lb[e] as the bounder defines them (kernels.cuh:325-372),

@@ -25,6 +25,9 @@ def lib() -> C.CDLL:
         for name in ("orc_add_rd", "orc_add_ru", "orc_sub_rd", "orc_sub_ru", "orc_mul_rd", "orc_mul_ru"):
             getattr(h, name).restype = d
             getattr(h, name).argtypes = [d, d]
+        for name in ("orc_add_ru_n", "orc_mul_ru_n"):
+            getattr(h, name).restype = None
+            getattr(h, name).argtypes = [P, P, P, C.c_size_t]
         for name in ("orc_iv_mul", "orc_iv_mul_case"):
             getattr(h, name).restype = None
             getattr(h, name).argtypes = [P, P, P]
@@ -133,3 +136,21 @@ def solve(arrays, partition_num=2, enumerate_cap=2, max_cycle_size=0, sample_poi
     _ok(lib().orc_solve(C.byref(d), partition_num, enumerate_cap, max_cycle_size, sample_points, iter_limit,
                         tolerance, C.byref(out), _p(best)))
     return out, best
+
+
+def _vec(fn, a, b):
+    a = np.ascontiguousarray(np.broadcast_to(np.asarray(a, np.float64), np.broadcast_shapes(np.shape(a), np.shape(b))))
+    b = np.ascontiguousarray(np.broadcast_to(np.asarray(b, np.float64), a.shape))
+    out = np.empty(a.shape)
+    fn(a.ctypes.data, b.ctypes.data, out.ctypes.data, a.size)
+    return out
+
+
+def add_ru_n(a, b) -> np.ndarray:
+    """Element-wise RU(a + b)."""
+    return _vec(lib().orc_add_ru_n, a, b)
+
+
+def mul_ru_n(a, b) -> np.ndarray:
+    """Element-wise RU(a * b)."""
+    return _vec(lib().orc_mul_ru_n, a, b)

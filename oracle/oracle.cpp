@@ -143,7 +143,9 @@ struct Iv
 };
 
 // ---------------------------------------------------------------------------
-// Interval ops (ops.cuh semantics; Mul/Sqr/scalar Mul report a zero bound as +0)
+// Interval ops (ops.cuh semantics; Mul/Sqr/scalar Mul report a zero bound as +0,
+// and every interval handed out -- objective bounds, root intervals -- reports
+// a zero bound as +0; the sign of an intermediate zero never changes a value)
 // ---------------------------------------------------------------------------
 
 Iv iv_add(Iv a, Iv b) { return {add_rd(a.lb, b.lb), add_ru(a.ub, b.ub)}; }
@@ -737,6 +739,21 @@ double orc_sub_ru(double a, double b) { return sub_ru(a, b); }
 double orc_mul_rd(double a, double b) { return mul_rd(a, b); }
 double orc_mul_ru(double a, double b) { return mul_ru(a, b); }
 
+// Element-wise RU(a + b) and RU(a * b) over arrays (one rounding-mode switch
+// per element, like the scalar primitives) for host-side replays of tapes.
+void orc_add_ru_n(const double* a, const double* b, double* out, size_t n)
+{
+  for (size_t i = 0; i < n; ++i) {
+    out[i] = add_ru(a[i], b[i]);
+  }
+}
+void orc_mul_ru_n(const double* a, const double* b, double* out, size_t n)
+{
+  for (size_t i = 0; i < n; ++i) {
+    out[i] = mul_ru(a[i], b[i]);
+  }
+}
+
 void orc_iv_mul(const double* x, const double* y, double* out)
 {
   Iv const r = iv_mul({x[0], x[1]}, {y[0], y[1]});
@@ -791,15 +808,15 @@ int orc_eval_boxes(const cmi_problem_desc* d, const double* lb, const double* ub
       }
       eval_interval(p, x.data(), val);
       feasible[e] = feasible_interval(p, val) ? 1 : 0;
-      obj_lb[e] = val[p.obj].lb;
+      obj_lb[e] = plus_zero(val[p.obj].lb);
       if (obj_ub != nullptr) {
-        obj_ub[e] = val[p.obj].ub;
+        obj_ub[e] = plus_zero(val[p.obj].ub);
       }
       if (roots != nullptr) {
         for (std::size_t k = 0; k <= p.n_con; ++k) {
           Iv const r = val[k == 0 ? p.obj : p.con_root[k - 1]];
-          roots[2 * (k * n + e)] = r.lb;
-          roots[2 * (k * n + e) + 1] = r.ub;
+          roots[2 * (k * n + e)] = plus_zero(r.lb);
+          roots[2 * (k * n + e) + 1] = plus_zero(r.ub);
         }
       }
     }
@@ -853,7 +870,7 @@ int orc_bound(const cmi_problem_desc* d, const cmi_ivl* box, const size_t* var_i
       materialise(parent, r, var_ids, widths, kinds, slots, sub);
       eval_interval(p, sub.data(), val);
       feasible[r] = feasible_interval(p, val) ? 1 : 0;
-      obj_lb[r] = val[p.obj].lb;
+      obj_lb[r] = plus_zero(val[p.obj].lb);
     }
     return 0;
   } catch (const std::exception& ex) {
@@ -971,7 +988,7 @@ int orc_solve(const cmi_problem_desc* d, size_t partition_num, size_t enumerate_
           ++p_inf;
           continue;
         }
-        double const lb = val[p.obj].lb;
+        double const lb = plus_zero(val[p.obj].lb);
         if (lb > gub) {
           ++p_dom;
           continue;

@@ -14,6 +14,7 @@ from cuminlpsearch_amd import capi as C
 from cuminlpsearch_amd.model import Problem
 from oracle import binding as orc
 
+import op_cases
 import problems as P
 
 pytestmark = pytest.mark.gpu
@@ -38,15 +39,14 @@ def binary_problem(op):
 @pytest.mark.parametrize("op", ["add", "sub", "mul"])
 def test_device_directed_primitives_match_golden(ctx, op):
     """[a,a] op [b,b] on the device gives [RD(a op b), RU(a op b)] bit for bit
-    (products report a zero bound as +0)."""
+    (a zero bound is reported as +0)."""
     rows = [r for r in ROWS if all(math.isfinite(float.fromhex(r[k])) for k in ("a", "b", f"{op}_rd", f"{op}_ru"))]
     a = np.array([float.fromhex(r["a"]) for r in rows])
     b = np.array([float.fromhex(r["b"]) for r in rows])
     lo = np.array([float.fromhex(r[f"{op}_rd"]) for r in rows])
     hi = np.array([float.fromhex(r[f"{op}_ru"]) for r in rows])
-    if op == "mul":
-        lo = np.where(lo == 0.0, 0.0, lo)
-        hi = np.where(hi == 0.0, 0.0, hi)
+    lo = np.where(lo == 0.0, 0.0, lo)
+    hi = np.where(hi == 0.0, 0.0, hi)
     prob = cmi.EngineProblem.from_problem(binary_problem(op))
     tape = cmi.Tape(ctx, prob)
     xs = np.stack([a, b])
@@ -67,6 +67,14 @@ def compare_with_oracle(ctx, eng: cmi.EngineProblem, lb, ub):
     return f_d, lo_d, hi_d, r_d
 
 
+@pytest.mark.parametrize("name", sorted(op_cases.CASES))
+def test_op_case_bitexact(ctx, name):
+    """One instruction form at a time (every tape op code over all cases)."""
+    eng = cmi.EngineProblem.from_problem(op_cases.CASES[name]())
+    lb, ub = P.random_boxes(5, 3, 400, -3.0, 4.0, 1e-6, 6.0)
+    compare_with_oracle(ctx, eng, lb, ub)
+
+
 def test_c2_kernel_alone_bitexact(ctx):
     """BASELINE config 2 at full size: n=32, m=16, 600 reachable nodes, 65,536
     boxes with log-uniform widths in [1e-4, 10]."""
@@ -84,6 +92,29 @@ def test_random_dags_bitexact(ctx, seed):
     eng = cmi.EngineProblem.from_problem(p)
     lb, ub = P.random_boxes(100 + seed, eng.info().n_vars, 777, -3.0, 3.0, 1e-6, 6.0)
     compare_with_oracle(ctx, eng, lb, ub)
+
+
+def test_function_groups_do_not_change_results(ctx):
+    """Every split of the functions over the waves of a workgroup gives the
+    oracle's results bit for bit, on C2 and on a random DAG with many constraints."""
+    cases = [(P.c2(), P.random_boxes(7, 32, 3000, -5.0, 5.0))]
+    p = P.random_dag(77, n_vars=6, n_ops=120, n_con=11)
+    e2 = cmi.EngineProblem.from_problem(p)
+    cases.append((e2, P.random_boxes(8, 6, 1001, -3.0, 3.0, 1e-5, 6.0)))
+    for eng, (lb, ub) in cases:
+        arrays = eng.arrays()
+        f_o, lo_o, hi_o, r_o = orc.eval_boxes(arrays, lb, ub, roots=True)
+        tape = cmi.Tape(ctx, eng)
+        seen = set()
+        for k in range(1, 9):
+            got = tape.set_box_groups(k)
+            seen.add(got)
+            f_d, lo_d, hi_d, r_d = tape.eval_boxes(lb, ub, roots=True)
+            assert np.array_equal(f_d, f_o), k
+            assert same_bits(lo_d, lo_o), k
+            assert same_bits(hi_d, hi_o), k
+            assert same_bits(r_d, r_o), k
+        assert len(seen) >= 4
 
 
 @pytest.mark.parametrize("n", [1, 2, 63, 64, 65, 129, 1000])
