@@ -16,7 +16,9 @@ from __future__ import annotations
 import argparse
 import json
 import os
+import pathlib
 import platform
+import re
 import sys
 import threading
 import time
@@ -107,6 +109,19 @@ def serve_comparison(ctx, n_req: int = 96, rate: float = 400.0, seed: int = 7):
         "iteration_cap": opts["iter_limit"],
         "results_identical": bool(same),
     }
+
+
+TRAFFIC_PROFILE = pathlib.Path(__file__).resolve().parent / "profiles" / \
+    "k_boxes_qcqp-n32-m16-600node-65536box_hbm-traffic.md"
+
+
+def measured_traffic():
+    """HBM bytes per launch from the committed FETCH_SIZE/WRITE_SIZE summary
+    (counter passes scaled by a 512 MiB copy), or None if it is absent."""
+    if not TRAFFIC_PROFILE.exists():
+        return None
+    m = re.search(r"^traffic_bytes_per_launch = (\d+)$", TRAFFIC_PROFILE.read_text(), re.M)
+    return int(m.group(1)) if m else None
 
 
 def main(argv=None) -> int:
@@ -269,7 +284,8 @@ def main(argv=None) -> int:
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved_gbs / HBM_PEAK_GBS,
-            "traffic": None,
+            "traffic": measured_traffic(),
+            "traffic_source": TRAFFIC_PROFILE.name,
             "kernel": "k_boxes",
             "kernel_ms": kern_ms,
             "bytes_per_box": bytes_per_box,
