@@ -111,16 +111,20 @@ def serve_comparison(ctx, n_req: int = 96, rate: float = 400.0, seed: int = 7):
     }
 
 
-TRAFFIC_PROFILE = pathlib.Path(__file__).resolve().parent / "profiles" / \
-    "k_boxes_qcqp-n32-m16-600node-65536box_hbm-traffic.md"
+PROFILES = pathlib.Path(__file__).resolve().parent / "profiles"
 
 
-def measured_traffic():
-    """HBM bytes per launch from the committed FETCH_SIZE/WRITE_SIZE summary
-    (counter passes scaled by a 512 MiB copy), or None if it is absent."""
-    if not TRAFFIC_PROFILE.exists():
+def traffic_profile(kernel: str) -> pathlib.Path:
+    return PROFILES / f"{kernel}_qcqp-n32-m16-600node-65536box_hbm-traffic.md"
+
+
+def measured_traffic(kernel: str):
+    """HBM bytes per launch of `kernel` from the committed FETCH_SIZE/WRITE_SIZE
+    summary (counter passes scaled by a 512 MiB copy), or None if absent."""
+    path = traffic_profile(kernel)
+    if not path.exists():
         return None
-    m = re.search(r"^traffic_bytes_per_launch = (\d+)$", TRAFFIC_PROFILE.read_text(), re.M)
+    m = re.search(r"^traffic_bytes_per_launch = (\d+)$", path.read_text(), re.M)
     return int(m.group(1)) if m else None
 
 
@@ -133,8 +137,13 @@ def main(argv=None) -> int:
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU-baseline leg")
     ap.add_argument("--box-groups", type=int, default=0,
                     help="waves (function groups) per 64-box workgroup; 0 = chosen from the tape")
+    ap.add_argument("--box-engine", choices=["tuned", "compiled", "interpreted", "auto"], default="tuned",
+                    help="explicit-box kernel: the autotuner's pick over interpreter and compiled shapes on this "
+                         "workload (default), the tape compiled to a straight-line kernel at run time, the tape "
+                         "interpreter, or the engine's automatic choice")
     ap.add_argument("--sweep-groups", action="store_true",
-                    help="print the kernel time for each group count to stderr before the timed run")
+                    help="print the interpreter's kernel time for each group count, and the compiled kernel's, "
+                         "to stderr before the timed run")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run one 512 MiB device copy (read 512 MiB, write 512 MiB) after the timed region, as "
                          "the reference kernel that scales hardware byte counters")
@@ -198,7 +207,9 @@ def main(argv=None) -> int:
         torch.cuda.synchronize()
         return float(np.mean([s.elapsed_time(e) for s, e in zip(starts, ends)]))
 
+    engines = {"compiled": capi.BOX_COMPILED, "interpreted": capi.BOX_INTERPRETED, "auto": capi.BOX_AUTO}
     if a.sweep_groups:
+        tape.set_box_engine(capi.BOX_INTERPRETED)
         for g in range(1, 9):
             got = tape.set_box_groups(g)
             if got != g:
@@ -206,9 +217,30 @@ def main(argv=None) -> int:
             for k in range(3):
                 launch(k, stream)
             i = tape.info()
-            print(f"box_groups={g} instr={i.box_instr} slots={i.box_slots} kernel_ms={kernel_ms(20):.4f}",
+            print(f"interpreted box_groups={g} instr={i.box_instr} slots={i.box_slots} kernel_ms={kernel_ms(20):.4f}",
                   file=sys.stderr, flush=True)
-    tape.set_box_groups(a.box_groups)
+        tape.set_box_engine(capi.BOX_COMPILED)
+        for g in (4, 8):
+            if tape.set_box_groups(g) != g:
+                continue
+            for waves in (1, 2, 4, 8):
+                if waves > g:
+                    continue
+                for persistent, wgs in ((False, 1), (True, 1), (True, 2)):
+                    tape.set_compiled_shape(persistent, waves, wgs)
+                    for k in range(3):
+                        launch(k, stream)
+                    i = tape.info()
+                    print(f"compiled box_groups={g} waves={i.compiled_waves} persistent={i.compiled_persistent} "
+                          f"wgs_per_cu={wgs} kernel_ms={kernel_ms(20):.4f} compile_s={i.compile_seconds:.3f}",
+                          file=sys.stderr, flush=True)
+        tape.set_compiled_shape(True, 0, 1)
+    tuning = None
+    if a.box_engine == "tuned":
+        tuning = tape.autotune(N_BOXES)
+    else:
+        tape.set_box_groups(a.box_groups)
+        tape.set_box_engine(engines[a.box_engine])
     info = tape.info()
     for k in range(3):
         launch(k, stream)
@@ -247,6 +279,7 @@ def main(argv=None) -> int:
         torch.cuda.synchronize()
         del src, dst
 
+    kernel = "cmi_boxes_compiled" if info.box_compiled else "k_boxes"
     bytes_per_box = info.bytes_per_box
     flops_per_box = info.flops_per_box
     achieved_gbs = bytes_per_box * N_BOXES / (kern_ms * 1e-3) / 1e9
@@ -273,6 +306,11 @@ def main(argv=None) -> int:
             "timed_region": "one HIP-graph replay of K launches",
             "tape_instructions": info.n_instr,
             "tape_slots": info.n_slots,
+            "box_engine": ("compiled" if info.box_compiled else "interpreted"),
+            "box_engine_choice": a.box_engine,
+            "compiled_waves": info.compiled_waves if info.box_compiled else None,
+            "compiled_persistent": bool(info.compiled_persistent) if info.box_compiled else None,
+            "compile_seconds": info.compile_seconds,
             "box_groups": info.box_groups,
             "box_instructions": info.box_instr,
             "box_slots": info.box_slots,
@@ -284,9 +322,9 @@ def main(argv=None) -> int:
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved_gbs / HBM_PEAK_GBS,
-            "traffic": measured_traffic(),
-            "traffic_source": TRAFFIC_PROFILE.name,
-            "kernel": "k_boxes",
+            "traffic": measured_traffic(kernel),
+            "traffic_source": traffic_profile(kernel).name,
+            "kernel": kernel,
             "kernel_ms": kern_ms,
             "bytes_per_box": bytes_per_box,
             "flops_per_box": flops_per_box,
@@ -294,6 +332,8 @@ def main(argv=None) -> int:
             "fp64_frac": achieved_tflops / FP64_PEAK_TFLOPS,
         },
     }
+    if tuning is not None:
+        line["autotune"] = sorted(tuning, key=lambda c: c["kernel_ms"])
 
     if not a.no_cpu:
         arrays = eng.arrays()

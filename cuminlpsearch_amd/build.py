@@ -41,7 +41,24 @@ HIP_FLAGS = [
     f"-I{ROOT / 'include'}",
 ]
 
-ENGINE_SOURCES = ["kernels.hip", "hip_backend.cpp", "batch.cpp", "capi.cpp", "server.cpp"]
+ENGINE_SOURCES = ["kernels.hip", "hip_backend.cpp", "batch.cpp", "capi.cpp", "server.cpp", "jit.cpp"]
+# interval_ops.hpp is also the prelude of every kernel compiled at run time
+# from a tape; its text is embedded into the library.
+EMBEDDED_TEXT = {"cmi_interval_ops_source": "interval_ops.hpp"}
+
+
+def _embed_sources() -> list[pathlib.Path]:
+    out = []
+    for symbol, name in EMBEDDED_TEXT.items():
+        text = (CSRC / name).read_text()
+        if ")CMI_EMBED" in text:
+            raise RuntimeError(f"{name} contains the raw-string delimiter")
+        path = OBJ_DIR / f"embed_{symbol}.cpp"
+        body = f'extern "C" const char {symbol}[] = R"CMI_EMBED({text})CMI_EMBED";\n'
+        if not path.exists() or path.read_text() != body:
+            path.write_text(body)
+        out.append(path)
+    return out
 
 
 def _run(cmd: list[str]) -> None:
@@ -65,10 +82,14 @@ def build_engine(jobs: int = 8) -> pathlib.Path:
         lang = [] if src.endswith(".hip") else ["-x", "hip"]
         cmds.append([HIPCC, *HIP_FLAGS, *lang, "-c", str(path), "-o", str(obj)])
         objs.append(obj)
+    for path in _embed_sources():
+        obj = path.with_suffix(".o")
+        cmds.append(["g++", "-O2", "-fPIC", "-c", str(path), "-o", str(obj)])
+        objs.append(obj)
     with cf.ThreadPoolExecutor(max_workers=max(1, jobs)) as ex:
         list(ex.map(_run, cmds))
     tmp = ENGINE_SO.with_suffix(".so.tmp")
-    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", str(tmp), *map(str, objs)])
+    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", str(tmp), *map(str, objs), "-lhiprtc"])
     os.replace(tmp, ENGINE_SO)
     return ENGINE_SO
 

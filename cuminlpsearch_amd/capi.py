@@ -13,6 +13,7 @@ import numpy as np
 LIB_PATH = pathlib.Path(__file__).resolve().parent / "lib" / "libcmi.so"
 
 # status codes (cmi_status)
+BOX_AUTO, BOX_INTERPRETED, BOX_COMPILED = range(3)
 OK, SHAPE_MISMATCH, INVALID_CONFIG, OVER_BUDGET, DEVICE_ERROR, INVALID_HANDLE, INVALID_DAG, INVALID_PROBLEM, ERROR = range(9)
 STATUS_NAMES = {
     OK: "CMI_OK",
@@ -79,7 +80,15 @@ class ProblemInfo(C.Structure):
 class TapeInfo(C.Structure):
     _fields_ = [("n_instr", C.c_size_t), ("n_slots", C.c_size_t), ("n_vars", C.c_size_t), ("n_outputs", C.c_size_t),
                 ("reachable_nodes", C.c_size_t), ("flops_per_box", C.c_uint64), ("bytes_per_box", C.c_size_t),
-                ("box_groups", C.c_size_t), ("box_instr", C.c_size_t), ("box_slots", C.c_size_t)]
+                ("box_groups", C.c_size_t), ("box_instr", C.c_size_t), ("box_slots", C.c_size_t),
+                ("box_engine", C.c_int), ("box_compiled", C.c_int), ("compile_seconds", C.c_double),
+                ("compiled_persistent", C.c_int), ("compiled_wgs_per_cu", C.c_size_t),
+                ("compiled_waves", C.c_size_t)]
+
+
+class BoxCandidate(C.Structure):
+    _fields_ = [("engine", C.c_int), ("groups", C.c_size_t), ("waves", C.c_size_t), ("persistent", C.c_int),
+                ("wgs_per_cu", C.c_size_t), ("kernel_ms", C.c_double)]
 
 
 class CostModel(C.Structure):
@@ -155,6 +164,12 @@ _SIGNATURES = {
     "cmi_tape_info_get": (C.c_int, [_P, C.POINTER(TapeInfo)]),
     "cmi_tape_destroy": (None, [_P]),
     "cmi_tape_set_box_groups": (C.c_int, [_P, _P, C.c_size_t]),
+    "cmi_tape_set_box_engine": (C.c_int, [_P, C.c_int]),
+    "cmi_tape_autotune": (C.c_int, [_P, _P, C.c_size_t, C.POINTER(BoxCandidate), C.c_size_t,
+                                   C.POINTER(C.c_size_t)]),
+    "cmi_tape_set_compiled_shape": (C.c_int, [_P, C.c_int, C.c_size_t, C.c_size_t]),
+    "cmi_tape_compiled_source": (C.c_int, [_P, C.c_int, C.c_size_t, C.c_char_p, C.c_size_t, C.POINTER(C.c_size_t)]),
+    "cmi_tape_compiled_object": (C.c_int, [_P, C.c_int, C.c_size_t, _P, C.c_size_t, C.POINTER(C.c_size_t)]),
     "cmi_tape_compile_words": (C.c_int, [_P, C.c_size_t, _P, C.c_size_t, C.POINTER(C.c_uint32), C.POINTER(TapeInfo)]),
     "cmi_tape_compile_info": (C.c_int, [_P, C.POINTER(TapeInfo), C.c_char_p, C.c_size_t]),
     "cmi_eval_boxes": (C.c_int, [_P, _P, _P, C.c_size_t, C.c_size_t, _P, _P, _P, _P]),

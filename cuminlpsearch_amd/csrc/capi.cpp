@@ -12,6 +12,7 @@
 #include <vector>
 
 #include "batch.hpp"
+#include "jit.hpp"
 #include "cmi/model.hpp"
 #include "cmi/policy.hpp"
 #include "cmi/qcqp.hpp"
@@ -359,6 +360,120 @@ cmi_status cmi_tape_info_get(const cmi_tape* t, cmi_tape_info* out)
     out->box_groups = gt.n_groups;
     out->box_instr = gt.code.size();
     out->box_slots = gt.n_slots;
+    out->box_engine = static_cast<int>(t->tape->box_engine());
+    auto const c = t->tape->compiled();
+    out->box_compiled = c ? 1 : 0;
+    out->compile_seconds = c ? c->compile_seconds() : 0.0;
+    auto const sh = t->tape->compiled_shape();
+    out->compiled_persistent = sh.persistent ? 1 : 0;
+    out->compiled_wgs_per_cu = t->tape->compiled_wgs_per_cu();
+    out->compiled_waves = sh.waves;
+  });
+}
+
+cmi_status cmi_tape_autotune(cmi_tape* t, const cmi_problem* p, size_t n_boxes, cmi_box_candidate* results, size_t cap,
+                             size_t* n_results)
+{
+  if (t == nullptr || p == nullptr) {
+    return fail(CMI_INVALID_HANDLE, "null argument");
+  }
+  return guarded([&] {
+    if (p->problem.n_vars() != t->tape->n_vars()) {
+      throw cmi::ShapeMismatch("problem does not match the tape");
+    }
+    auto const all = t->tape->autotune(p->problem, n_boxes);
+    if (n_results != nullptr) {
+      *n_results = all.size();
+    }
+    for (std::size_t i = 0; i < all.size() && results != nullptr && i < cap; ++i) {
+      results[i].engine = static_cast<int>(all[i].engine);
+      results[i].groups = all[i].groups;
+      results[i].waves = all[i].waves;
+      results[i].persistent = all[i].persistent ? 1 : 0;
+      results[i].wgs_per_cu = all[i].wgs_per_cu;
+      results[i].kernel_ms = all[i].kernel_ms;
+    }
+  });
+}
+
+cmi_status cmi_tape_set_compiled_shape(cmi_tape* t, int persistent, size_t waves, size_t wgs_per_cu)
+{
+  if (t == nullptr) {
+    return fail(CMI_INVALID_HANDLE, "null argument");
+  }
+  return guarded([&] {
+    t->tape->set_compiled_shape(persistent != 0, static_cast<std::uint32_t>(std::min<size_t>(waves, 1u << 16)),
+                                static_cast<std::uint32_t>(std::min<size_t>(wgs_per_cu, 1u << 16)));
+  });
+}
+
+cmi_status cmi_tape_set_box_engine(cmi_tape* t, int engine)
+{
+  if (t == nullptr) {
+    return fail(CMI_INVALID_HANDLE, "null argument");
+  }
+  return guarded([&] { t->tape->set_box_engine(static_cast<cmi::hip::DeviceTape::BoxEngine>(engine)); });
+}
+
+namespace
+{
+/// The grouping a new device tape uses for explicit boxes.
+cmi::tape::Tape grouped_for_boxes(const cmi::model::Problem<double>& p)
+{
+  return cmi::tape::compile_groups(p, cmi::tape::auto_groups(cmi::tape::compile(p)));
+}
+
+cmi::jit::Shape export_shape(const cmi::tape::Tape& g, int persistent, size_t waves)
+{
+  if (waves > 8) {
+    throw cmi::InvalidConfiguration("at most 8 waves per workgroup");
+  }
+  cmi::jit::Shape sh;
+  sh.waves = cmi::jit::shape_waves(g, {false, static_cast<std::uint32_t>(waves)});
+  sh.persistent = persistent < 0 ? cmi::jit::persistent_fits(g, sh.waves) : persistent != 0;
+  return sh;
+}
+}  // namespace
+
+cmi_status cmi_tape_compiled_object(const cmi_problem* p, int persistent, size_t waves, void* buf, size_t cap,
+                                    size_t* len)
+{
+  if (p == nullptr || len == nullptr) {
+    return fail(CMI_INVALID_HANDLE, "null argument");
+  }
+  return guarded([&] {
+    p->problem.validate();
+    auto const g = grouped_for_boxes(p->problem);
+    auto const code = cmi::jit::compile_code_object(cmi::jit::boxes_source(g, export_shape(g, persistent, waves)));
+    *len = code.size();
+    if (buf != nullptr) {
+      if (cap < code.size()) {
+        throw cmi::ShapeMismatch("buffer of " + std::to_string(cap) + " bytes; the code object has "
+                                 + std::to_string(code.size()));
+      }
+      std::memcpy(buf, code.data(), code.size());
+    }
+  });
+}
+
+cmi_status cmi_tape_compiled_source(const cmi_problem* p, int persistent, size_t waves, char* buf, size_t cap,
+                                    size_t* len)
+{
+  if (p == nullptr) {
+    return fail(CMI_INVALID_HANDLE, "null argument");
+  }
+  return guarded([&] {
+    p->problem.validate();
+    auto const g = grouped_for_boxes(p->problem);
+    std::string const src = cmi::jit::boxes_source(g, export_shape(g, persistent, waves));
+    if (len != nullptr) {
+      *len = src.size();
+    }
+    if (buf != nullptr && cap > 0) {
+      std::size_t const k = std::min(cap - 1, src.size());
+      std::memcpy(buf, src.data(), k);
+      buf[k] = '\0';
+    }
   });
 }
 
@@ -399,6 +514,12 @@ cmi_status cmi_tape_compile_words(const cmi_problem* p, size_t k, void* words, s
     out->box_groups = tp.n_groups;
     out->box_instr = tp.code.size();
     out->box_slots = tp.n_slots;
+    out->box_engine = CMI_BOX_AUTO;
+    out->box_compiled = 0;
+    out->compile_seconds = 0.0;
+    out->compiled_persistent = 0;
+    out->compiled_wgs_per_cu = 1;
+    out->compiled_waves = 1;
     if (words != nullptr) {
       if (cap_words < tp.code.size()) {
         throw cmi::ShapeMismatch("word buffer holds " + std::to_string(cap_words) + " words; the tape has "
@@ -432,6 +553,12 @@ cmi_status cmi_tape_compile_info(const cmi_problem* p, cmi_tape_info* out, char*
     out->box_groups = 1;
     out->box_instr = tp.code.size();
     out->box_slots = tp.n_slots;
+    out->box_engine = CMI_BOX_AUTO;
+    out->box_compiled = 0;
+    out->compile_seconds = 0.0;
+    out->compiled_persistent = 0;
+    out->compiled_wgs_per_cu = 1;
+    out->compiled_waves = 1;
     if (listing != nullptr && cap > 0) {
       std::string const text = cmi::tape::disassemble(tp);
       std::size_t const k = std::min(cap - 1, text.size());

@@ -794,6 +794,15 @@ inline Tape compile_groups(const model::Problem<double>& p, std::uint32_t n_grou
   return t;
 }
 
+/// Function groups chosen for explicit boxes: about one per 32 tape words, at
+/// most 8 (and at most one per function).
+inline std::uint32_t auto_groups(const Tape& t)
+{
+  std::size_t const by_size = (t.code.size() + 31) / 32;
+  std::size_t const roots = 1 + static_cast<std::size_t>(t.n_constraints);
+  return static_cast<std::uint32_t>(std::clamp<std::size_t>(std::min(by_size, roots), 1, 8));
+}
+
 /// Human-readable listing, one instruction per line.
 inline std::string disassemble(const Tape& t)
 {

@@ -1,9 +1,12 @@
 #include "hip_backend.hpp"
 
 #include <algorithm>
+#include <cmath>
 #include <cstring>
 #include <limits>
 #include <stdexcept>
+
+#include "cmi/qcqp.hpp"
 
 namespace cmi::hip
 {
@@ -70,13 +73,6 @@ void upload_code(const std::vector<tape::Instr>& code, DeviceBuffer& buf)
         "hipMemcpy(tape)");
 }
 
-/// Groups for explicit boxes: about one wave per 32 tape words, at most 8.
-std::uint32_t auto_box_groups(const tape::Tape& t)
-{
-  std::size_t const by_size = (t.code.size() + 31) / 32;
-  return static_cast<std::uint32_t>(std::clamp<std::size_t>(by_size, 1, dev::kMaxGroups));
-}
-
 }  // namespace
 
 DeviceTape::DeviceTape(std::shared_ptr<Context> ctx, const model::Problem<double>& problem,
@@ -104,7 +100,7 @@ DeviceTape::DeviceTape(std::shared_ptr<Context> ctx, const model::Problem<double
 void DeviceTape::set_box_groups(const model::Problem<double>& problem, std::uint32_t k)
 {
   if (k == 0) {
-    k = auto_box_groups(tape_);
+    k = tape::auto_groups(tape_);
   }
   std::uint32_t const n_roots = 1 + tape_.n_constraints;
   k = std::clamp<std::uint32_t>(k, 1, std::min<std::uint32_t>(dev::kMaxGroups, n_roots));
@@ -121,6 +117,15 @@ void DeviceTape::set_box_groups(const model::Problem<double>& problem, std::uint
   ctx_->activate();
   upload_code(g.code, grouped_code_);
   grouped_ = std::move(g);
+  {
+    // the compiled kernel follows the grouping; compile again when needed
+    std::lock_guard<std::mutex> lock(jit_mu_);
+    compiled_.reset();
+    jit_failed_ = false;
+  }
+  if (engine_ == BoxEngine::Compiled) {
+    set_box_engine(BoxEngine::Compiled);
+  }
   groups_ = {};
   groups_.n_groups = grouped_.n_groups;
   groups_.objective_group = grouped_.objective_group;
@@ -129,13 +134,210 @@ void DeviceTape::set_box_groups(const model::Problem<double>& problem, std::uint
   }
 }
 
+void DeviceTape::set_box_engine(BoxEngine e)
+{
+  if (e != BoxEngine::Auto && e != BoxEngine::Interpreted && e != BoxEngine::Compiled) {
+    throw InvalidConfiguration("unknown box engine");
+  }
+  if (e == BoxEngine::Compiled) {
+    if (tape_.code.size() > kMaxCompiledWords) {
+      throw InvalidConfiguration("tape of " + std::to_string(tape_.code.size()) + " words is above the "
+                                 + std::to_string(kMaxCompiledWords) + "-word limit of compiled kernels");
+    }
+    auto c = compile_now();
+    std::lock_guard<std::mutex> lock(jit_mu_);
+    compiled_ = std::move(c);
+  }
+  engine_ = e;
+}
+
+jit::Shape DeviceTape::compiled_shape() const
+{
+  jit::Shape sh;
+  sh.waves = jit::shape_waves(grouped_, {false, want_waves_});
+  sh.persistent = want_persistent_ && jit::persistent_fits(grouped_, sh.waves);
+  return sh;
+}
+
+std::shared_ptr<const jit::CompiledBoxes> DeviceTape::compile_now() const
+{
+  ctx_->activate();
+  auto c = jit::compile_boxes(grouped_, ctx_->device(), compiled_shape());
+  c->set_wgs_per_cu(wgs_per_cu_);
+  return c;
+}
+
+void DeviceTape::set_compiled_shape(bool persistent, std::uint32_t waves, std::uint32_t wgs_per_cu)
+{
+  if (wgs_per_cu < 1 || wgs_per_cu > 8) {
+    throw InvalidConfiguration("workgroups per CU must be in [1, 8]");
+  }
+  if (waves > dev::kMaxGroups) {
+    throw InvalidConfiguration("at most 8 waves per workgroup");
+  }
+  want_persistent_ = persistent;
+  want_waves_ = waves;
+  wgs_per_cu_ = wgs_per_cu;
+  {
+    std::lock_guard<std::mutex> lock(jit_mu_);
+    compiled_.reset();
+    jit_failed_ = false;
+  }
+  if (engine_ == BoxEngine::Compiled) {
+    set_box_engine(BoxEngine::Compiled);
+  }
+}
+
+std::shared_ptr<const jit::CompiledBoxes> DeviceTape::compiled() const
+{
+  std::lock_guard<std::mutex> lock(jit_mu_);
+  return compiled_;
+}
+
 void DeviceTape::eval_boxes_device(const double* lb, const double* ub, std::size_t ld, std::size_t n,
                                    std::uint8_t* feasible, double* obj_lb, double* obj_ub, double* roots,
                                    hipStream_t stream) const
 {
+  std::shared_ptr<const jit::CompiledBoxes> c;
+  if (engine_ == BoxEngine::Compiled) {
+    c = compiled();
+  } else if (engine_ == BoxEngine::Auto && n >= kAutoCompileBoxes && n <= jit::kMaxBoxes
+             && tape_.code.size() <= kMaxCompiledWords) {
+    std::lock_guard<std::mutex> lock(jit_mu_);
+    if (!compiled_ && !jit_failed_) {
+      try {
+        compiled_ = compile_now();
+      } catch (const DeviceError&) {
+        jit_failed_ = true;  // Auto keeps the interpreter, which gives the same results
+      }
+    }
+    c = compiled_;
+  }
+  if (c) {
+    c->launch(lb, ub, ld, n, feasible, obj_lb, obj_ub, roots, stream);
+    return;
+  }
   dev::launch_boxes(grouped_code_.as<tape::Instr>(), groups_, grouped_.n_slots, n_vars(), lb, ub, ld, n, feasible,
                     obj_lb, obj_ub, roots, stream);
   check(hipGetLastError(), "launch(k_boxes)");
+}
+
+std::vector<DeviceTape::BoxCandidate> DeviceTape::autotune(const model::Problem<double>& problem, std::size_t n)
+{
+  if (n == 0 || n > jit::kMaxBoxes) {
+    throw InvalidConfiguration("autotune needs between 1 and 2^29 boxes");
+  }
+  std::uint32_t const nv = n_vars();
+  ctx_->activate();
+  // Synthetic boxes inside the variable bounds: log-uniform widths, clipped.
+  std::vector<double> lb(static_cast<std::size_t>(nv) * n), ub(lb.size());
+  qcqp::Rng rng(0x5eedULL);
+  for (std::uint32_t v = 0; v < nv; ++v) {
+    double const lo = problem.box_bounds[v].lb, hi = problem.box_bounds[v].ub;
+    double const span = std::isfinite(hi - lo) && hi > lo ? hi - lo : 1.0;
+    for (std::size_t e = 0; e < n; ++e) {
+      double const w = span * std::exp(rng.uniform(std::log(1e-5), 0.0));
+      double const a = (std::isfinite(lo) ? lo : -1.0) + rng.uniform() * (span - w);
+      lb[v * n + e] = a;
+      ub[v * n + e] = a + w;
+    }
+  }
+  DeviceBuffer d_lb(lb.size() * sizeof(double)), d_ub(ub.size() * sizeof(double));
+  DeviceBuffer d_f(n), d_lo(n * sizeof(double)), d_hi(n * sizeof(double));
+  check(hipMemcpy(d_lb.get(), lb.data(), lb.size() * sizeof(double), hipMemcpyHostToDevice), "hipMemcpy(autotune)");
+  check(hipMemcpy(d_ub.get(), ub.data(), ub.size() * sizeof(double), hipMemcpyHostToDevice), "hipMemcpy(autotune)");
+  hipStream_t const s = ctx_->stream();
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  check(hipEventCreate(&e0), "hipEventCreate");
+  check(hipEventCreate(&e1), "hipEventCreate");
+  auto time_it = [&]() {
+    for (int k = 0; k < 3; ++k) {
+      eval_boxes_device(d_lb.as<double>(), d_ub.as<double>(), n, n, d_f.as<std::uint8_t>(), d_lo.as<double>(),
+                        d_hi.as<double>(), nullptr, s);
+    }
+    std::vector<float> ms;
+    for (int k = 0; k < 9; ++k) {
+      check(hipEventRecord(e0, s), "hipEventRecord");
+      eval_boxes_device(d_lb.as<double>(), d_ub.as<double>(), n, n, d_f.as<std::uint8_t>(), d_lo.as<double>(),
+                        d_hi.as<double>(), nullptr, s);
+      check(hipEventRecord(e1, s), "hipEventRecord");
+      check(hipEventSynchronize(e1), "hipEventSynchronize");
+      float t = 0.0f;
+      check(hipEventElapsedTime(&t, e0, e1), "hipEventElapsedTime");
+      ms.push_back(t);
+    }
+    std::nth_element(ms.begin(), ms.begin() + ms.size() / 2, ms.end());
+    return static_cast<double>(ms[ms.size() / 2]);
+  };
+
+  std::uint32_t const n_roots = 1 + tape_.n_constraints;
+  std::vector<BoxCandidate> cands;
+  for (std::uint32_t k : {4u, 8u}) {
+    if (k > n_roots && k != 4u) {
+      continue;
+    }
+    BoxCandidate c;
+    c.engine = BoxEngine::Interpreted;
+    c.groups = k;
+    cands.push_back(c);
+  }
+  bool const compilable = tape_.code.size() <= kMaxCompiledWords;
+  for (std::uint32_t k : {1u, 4u, 8u}) {
+    if (!compilable || (k > n_roots && k != 1u)) {
+      continue;
+    }
+    for (std::uint32_t w : {1u, 2u, 4u, 8u}) {
+      if (w > k) {
+        continue;
+      }
+      for (int form = 0; form < 3; ++form) {
+        BoxCandidate c;
+        c.engine = BoxEngine::Compiled;
+        c.groups = k;
+        c.waves = w;
+        c.persistent = form > 0;
+        c.wgs_per_cu = form == 2 ? 2 : 1;
+        cands.push_back(c);
+      }
+    }
+  }
+  auto apply = [&](const BoxCandidate& c) {
+    engine_ = BoxEngine::Interpreted;  // no compile while regrouping
+    set_box_groups(problem, c.groups);
+    if (c.engine == BoxEngine::Compiled) {
+      want_persistent_ = c.persistent;
+      want_waves_ = c.waves;
+      wgs_per_cu_ = c.wgs_per_cu;
+      set_box_engine(BoxEngine::Compiled);
+    }
+  };
+  std::vector<BoxCandidate> done;
+  for (auto c : cands) {
+    try {
+      apply(c);
+      if (c.engine == BoxEngine::Compiled) {
+        auto const sh = compiled_shape();
+        if (c.persistent && !sh.persistent) {
+          continue;  // tiles do not fit: same kernel as the direct form
+        }
+        c.waves = sh.waves;
+      }
+      c.groups = grouped_.n_groups;
+      c.kernel_ms = time_it();
+      done.push_back(c);
+    } catch (const DeviceError&) {
+      // a shape the run-time compiler rejects is left out
+    }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (done.empty()) {
+    throw DeviceError(0, "autotune: no configuration ran");
+  }
+  auto const best = *std::min_element(done.begin(), done.end(),
+                                      [](auto const& a, auto const& b) { return a.kernel_ms < b.kernel_ms; });
+  apply(best);
+  return done;
 }
 
 void DeviceTape::eval_points_device(const double* x, std::size_t ld, std::size_t n, std::uint8_t* feasible,

@@ -3,6 +3,7 @@
 #include <cstddef>
 #include <cstdint>
 #include <memory>
+#include <mutex>
 #include <span>
 #include <string>
 #include <vector>
@@ -13,6 +14,7 @@
 #include "cmi/model.hpp"
 #include "cmi/tape.hpp"
 #include "device.hpp"
+#include "jit.hpp"
 
 // The HIP backend: device-resident tapes and the three roles implemented over
 // the tape kernels. HipBackendFactory is a drop-in RegionBackendFactory
@@ -161,6 +163,50 @@ public:
   std::uint32_t box_groups() const { return grouped_.n_groups; }
   const tape::Tape& grouped_tape() const { return grouped_; }
 
+  /// Which kernel evaluates explicit boxes: the tape interpreter, the tape
+  /// compiled to a straight-line kernel at run time (jit.hpp), or Auto --
+  /// compiled once a single call has at least kAutoCompileBoxes boxes and the
+  /// tape has at most kMaxCompiledWords words (the first such call waits for
+  /// the compiler), the interpreter otherwise. Results are identical.
+  enum class BoxEngine : int
+  {
+    Auto = 0,
+    Interpreted = 1,
+    Compiled = 2,
+  };
+  static constexpr std::size_t kAutoCompileBoxes = 32768;
+  static constexpr std::size_t kMaxCompiledWords = 20000;
+  /// Compiled compiles now (and throws if that fails).
+  void set_box_engine(BoxEngine e);
+  BoxEngine box_engine() const { return engine_; }
+  /// The compiled kernel, if one is loaded.
+  std::shared_ptr<const jit::CompiledBoxes> compiled() const;
+
+  /// Shape of the compiled explicit-box kernel: persistent (double-buffered
+  /// tiles, grid sized to wgs_per_cu workgroups per CU; only when its LDS
+  /// tiles fit) or direct (one workgroup per tile), and waves per workgroup
+  /// (0 = one per function group; fewer waves interleave several groups).
+  void set_compiled_shape(bool persistent, std::uint32_t waves, std::uint32_t wgs_per_cu);
+  jit::Shape compiled_shape() const;
+  std::uint32_t compiled_wgs_per_cu() const { return wgs_per_cu_; }
+
+  /// One configuration of the explicit-box kernel and its measured time.
+  struct BoxCandidate
+  {
+    BoxEngine engine = BoxEngine::Interpreted;
+    std::uint32_t groups = 1;
+    std::uint32_t waves = 0;
+    bool persistent = false;
+    std::uint32_t wgs_per_cu = 1;
+    double kernel_ms = 0.0;
+  };
+  /// Autotuner: time the interpreter and compiled shapes (function groups,
+  /// waves per workgroup, persistent or direct, workgroups per CU) on n
+  /// synthetic boxes inside the problem's bounds, keep the fastest and
+  /// return every candidate with its median kernel time. `problem` must be
+  /// the one this tape was built from.
+  std::vector<BoxCandidate> autotune(const model::Problem<double>& problem, std::size_t n);
+
   const tape::Tape& tape() const { return tape_; }
   const tape::Instr* code() const { return code_.as<tape::Instr>(); }
   const std::uint8_t* var_kinds() const { return kinds_.as<std::uint8_t>(); }
@@ -181,6 +227,15 @@ private:
   tape::Tape tape_;
   tape::Tape grouped_;
   dev::GroupTable groups_ {};
+  BoxEngine engine_ = BoxEngine::Auto;
+  bool want_persistent_ = true;
+  std::uint32_t want_waves_ = 0;
+  std::uint32_t wgs_per_cu_ = 1;
+  mutable std::mutex jit_mu_;
+  mutable std::shared_ptr<const jit::CompiledBoxes> compiled_;
+  mutable bool jit_failed_ = false;
+
+  std::shared_ptr<const jit::CompiledBoxes> compile_now() const;
   DeviceBuffer code_;
   DeviceBuffer grouped_code_;
   DeviceBuffer kinds_;

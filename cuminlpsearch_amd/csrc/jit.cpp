@@ -1,0 +1,465 @@
+#include "jit.hpp"
+
+#include <hip/hiprtc.h>
+
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <algorithm>
+#include <map>
+#include <mutex>
+#include <set>
+#include <sstream>
+#include <tuple>
+#include <vector>
+
+#include "cmi/errors.hpp"
+
+// Text of interval_ops.hpp, embedded at build time.
+extern "C" const char cmi_interval_ops_source[];
+
+namespace cmi::jit
+{
+
+namespace
+{
+
+using tape::AccMode;
+using tape::Instr;
+using tape::Term;
+using tape::TOp;
+
+/// An exact C++ literal for a double (hexadecimal significand).
+std::string lit(double x)
+{
+  if (std::isnan(x)) {
+    return "__builtin_nan(\"\")";
+  }
+  if (std::isinf(x)) {
+    return x > 0 ? "__builtin_huge_val()" : "(-__builtin_huge_val())";
+  }
+  char buf[64];
+  std::snprintf(buf, sizeof buf, "%a", x);
+  return std::string("(") + buf + ")";
+}
+
+std::string slot(std::uint32_t s) { return "s" + std::to_string(s); }
+
+/// [a] * c with the sign of c known when the source is written.
+std::string mul_imm(const std::string& a, double c)
+{
+  if (c > 0.0) {
+    return "iv_mul_pos(" + a + ", " + lit(c) + ", t)";
+  }
+  if (c < 0.0) {
+    return "iv_mul_neg(" + a + ", " + lit(c) + ", t)";
+  }
+  if (std::isnan(c)) {
+    return "iv_mul_imm(" + a + ", " + lit(c) + ", t)";
+  }
+  return "Iv{0.0, 0.0}";
+}
+
+/// One tape word as one statement; `acc` names the group's accumulator.
+std::string emit_word(const Instr& w, const std::string& acc)
+{
+  std::string const a = slot(w.a), b = slot(w.b), d = slot(w.dst), c = lit(w.imm);
+  auto set = [](const std::string& lhs, const std::string& rhs) { return "  " + lhs + " = " + rhs + ";\n"; };
+  auto call = [](const std::string& fn, const std::string& x, const std::string& y) {
+    return fn + "(" + x + ", " + y + ", t)";
+  };
+  auto output = [&](const std::string& x) {
+    return "  if (rt != nullptr) out_store(rt, " + std::to_string(w.b) + ", n, e, to_output(" + x + ", t));\n";
+  };
+  if (tape::is_term_op(w.op)) {
+    std::string x;
+    switch (static_cast<Term>(w.op & 7u)) {
+      case Term::S: x = a; break;
+      case Term::SC: x = mul_imm(a, w.imm); break;
+      case Term::SS: x = call("iv_mul", a, b); break;
+      case Term::Q: x = "iv_sqr(" + a + ", t)"; break;
+      case Term::SSC: x = mul_imm(call("iv_mul", a, b), w.imm); break;
+      case Term::QC: x = mul_imm("iv_sqr(" + a + ", t)", w.imm); break;
+    }
+    switch (static_cast<AccMode>((w.op - tape::kTermBase) >> 3)) {
+      case AccMode::Set: return set(acc, x);
+      case AccMode::Add: return set(acc, call("iv_add", acc, x));
+      case AccMode::Sub: return set(acc, call("iv_sub", acc, x));
+    }
+    throw InvalidConfiguration("tape word with an unknown accumulate mode");
+  }
+  switch (static_cast<TOp>(w.op)) {
+    case TOp::Nop: return {};
+    case TOp::LoadConst: return set(d, "iv_const(" + c + ", t)");
+    case TOp::Add: return set(d, call("iv_add", a, b));
+    case TOp::AddImm: return set(d, call("iv_add_imm", a, c));
+    case TOp::Sub: return set(d, call("iv_sub", a, b));
+    case TOp::SubImm: return set(d, call("iv_sub_imm", a, c));
+    case TOp::ImmSub: return set(d, call("iv_imm_sub", c, a));
+    case TOp::Mul: return set(d, call("iv_mul", a, b));
+    case TOp::MulImm: return set(d, mul_imm(a, w.imm));
+    case TOp::Sqr: return set(d, "iv_sqr(" + a + ", t)");
+    case TOp::Neg: return set(d, "iv_neg(" + a + ")");
+    case TOp::CheckLE: return set("f", "f && check_le(" + a + ", " + c + ")");
+    case TOp::CheckEQ: return set("f", "f && check_eq(" + a + ", " + c + ")");
+    case TOp::Objective: return set("obj", a);
+    case TOp::Output: return output(a);
+    case TOp::AccStore: return set(d, acc);
+    case TOp::AccAddImm: return set(acc, call("iv_add_imm", acc, c));
+    case TOp::AccSubImm: return set(acc, call("iv_sub_imm", acc, c));
+    case TOp::AccImmSub: return set(acc, call("iv_imm_sub", c, acc));
+    case TOp::AccScale: return set(acc, mul_imm(acc, w.imm));
+    case TOp::AccNeg: return set(acc, "iv_neg(" + acc + ")");
+    case TOp::AccCheckLE: return set("f", "f && check_le(" + acc + ", " + c + ")");
+    case TOp::AccCheckEQ: return set("f", "f && check_eq(" + acc + ", " + c + ")");
+    case TOp::AccObjective: return set("obj", acc);
+    case TOp::AccOutput: return output(acc);
+    default: throw InvalidConfiguration("tape word with an op the compiled kernel does not know");
+  }
+}
+
+std::uint64_t fnv1a(const void* p, std::size_t n, std::uint64_t h = 0xcbf29ce484222325ULL)
+{
+  auto const* b = static_cast<const unsigned char*>(p);
+  for (std::size_t i = 0; i < n; ++i) {
+    h ^= b[i];
+    h *= 0x100000001b3ULL;
+  }
+  return h;
+}
+
+void rtc_check(hiprtcResult r, const char* what)
+{
+  if (r != HIPRTC_SUCCESS) {
+    throw DeviceError(static_cast<int>(r), std::string(what) + ": " + hiprtcGetErrorString(r));
+  }
+}
+
+void hip_check(hipError_t e, const char* what)
+{
+  if (e != hipSuccess) {
+    throw DeviceError(static_cast<int>(e), std::string(what) + ": " + hipGetErrorString(e));
+  }
+}
+
+// Same code-generation flags as the library's own kernels: no contraction,
+// packed FP32 off; gfx950 only.
+const char* const kOptions[] = {
+    "--offload-arch=gfx950", "-O3", "-std=c++17", "-ffp-contract=off",
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+};
+
+}  // namespace
+
+namespace
+{
+
+/// Slots group code [lo, hi) reads before writing (its variables) and writes.
+void slot_use(const tape::Tape& t, std::uint32_t lo, std::uint32_t hi, std::set<std::uint32_t>& vars,
+              std::set<std::uint32_t>& temps)
+{
+  for (std::uint32_t i = lo; i < hi; ++i) {
+    Instr const& w = t.code[i];
+    auto note = [&](std::uint32_t s) { (s < t.n_vars ? vars : temps).insert(s); };
+    bool const term = tape::is_term_op(w.op);
+    auto const op = static_cast<TOp>(w.op);
+    unsigned const kind = w.op & 7u;
+    bool const reads_a = term || (w.op >= static_cast<std::uint8_t>(TOp::Add) && w.op <= static_cast<std::uint8_t>(TOp::Output));
+    bool const reads_b = (term && (kind == static_cast<unsigned>(Term::SS) || kind == static_cast<unsigned>(Term::SSC)))
+        || op == TOp::Add || op == TOp::Sub || op == TOp::Mul;
+    bool const writes = (w.op >= static_cast<std::uint8_t>(TOp::LoadConst) && w.op <= static_cast<std::uint8_t>(TOp::Neg))
+        || op == TOp::AccStore;
+    if (reads_a) {
+      note(w.a);
+    }
+    if (reads_b) {
+      note(w.b);
+    }
+    if (writes) {
+      note(w.dst);
+    }
+  }
+}
+
+}  // namespace
+
+std::uint32_t shape_waves(const tape::Tape& t, const Shape& shape)
+{
+  std::uint32_t const k = std::max<std::uint32_t>(t.n_groups, 1);
+  return shape.waves == 0 ? k : std::clamp<std::uint32_t>(shape.waves, 1, k);
+}
+
+bool persistent_fits(const tape::Tape& t, std::uint32_t waves)
+{
+  std::uint64_t const w = std::max<std::uint32_t>(waves, 1);
+  std::uint64_t const bytes = 2ull * t.n_vars * 64 * 16 + 2 * w * 64 * 4 + 2 * 128 * 8;
+  return t.n_vars > 0 && bytes <= kLdsBytes;
+}
+
+std::string boxes_source(const tape::Tape& t, const Shape& shape)
+{
+  std::uint32_t const k = std::max<std::uint32_t>(t.n_groups, 1);
+  std::uint32_t const w = shape_waves(t, shape);
+  std::uint32_t const nv = t.n_vars;
+  bool const persistent = shape.persistent;
+  if (persistent && !persistent_fits(t, w)) {
+    throw InvalidConfiguration("the double-buffered box tiles of this tape do not fit in LDS");
+  }
+  std::ostringstream o;
+  o << cmi_interval_ops_source << "\n";
+  o << "using namespace cmi_iv;\n\n";
+  o << "__device__ __forceinline__ void out_store(double* roots, unsigned long k, unsigned long n, unsigned long e,\n"
+       "                                          Bnd v)\n{\n"
+       "  roots[2 * (k * n + e)] = v.lb;\n  roots[2 * (k * n + e) + 1] = v.ub;\n}\n\n";
+  o << "extern \"C\" __global__ void __launch_bounds__(" << 64 * w << ") " << kBoxesKernel
+    << "(const double* __restrict__ lb, const double* __restrict__ ub, unsigned long ld, unsigned long n,\n"
+       "    unsigned char* __restrict__ feasible, double* __restrict__ obj_lb, double* __restrict__ obj_ub,\n"
+       "    double* __restrict__ roots)\n{\n";
+  o << "  const unsigned lane = threadIdx.x & 63u;\n";
+  o << "  const unsigned wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);\n";
+  if (persistent) {
+    // Persistent workgroups walk the 64-box tiles with a stride of the grid.
+    // While the waves evaluate tile i from one LDS buffer, each wave's share
+    // of tile i+1's variables is already in flight from HBM into registers,
+    // and goes to the other buffer before the barrier that ends the step.
+    std::uint32_t const share = (nv + w - 1) / w;
+    o << "  __shared__ double2 xs[" << 2 * nv * 64 << "];\n";
+    o << "  __shared__ unsigned feas_s[" << 2 * 64 * w << "];\n";
+    o << "  __shared__ double obj_s[256];\n";
+    o << "  const unsigned long n_tiles = (n + 63ul) / 64ul;\n";
+    o << "  unsigned long tile = blockIdx.x;\n";
+    o << "  if (tile >= n_tiles) {\n    return;\n  }\n";
+    o << "  {\n    const unsigned long e0 = tile * 64ul + lane;\n"
+         "    const unsigned off = 8u * (unsigned)(e0 < n ? e0 : n - 1);\n";
+    o << "    for (unsigned v = wave; v < " << nv << "u; v += " << w << "u) {\n"
+         "      const Iv x = load_box(lb + v * ld, ub + v * ld, off);\n"
+         "      xs[v * 64u + lane] = make_double2(x.nl, x.ub);\n    }\n  }\n";
+    o << "  __syncthreads();\n";
+    o << "  for (unsigned step = 0;; ++step) {\n";
+    o << "    const unsigned cur = step & 1u;\n";
+    o << "    const unsigned long e = tile * 64ul + lane;\n";
+    o << "    const bool active = e < n;\n";
+    o << "    const unsigned long next = tile + gridDim.x;\n";
+    o << "    const bool has_next = next < n_tiles;\n";
+    o << "    const unsigned long en = next * 64ul + lane;\n";
+    o << "    const unsigned offn = 8u * (unsigned)(en < n ? en : n - 1);\n";
+    o << "    Iv pre[" << share << "];\n";
+    o << "    if (has_next) {\n";
+    for (std::uint32_t j = 0; j < share; ++j) {
+      o << "      if (wave + " << j * w << "u < " << nv << "u) {\n"
+        << "        pre[" << j << "] = load_box(lb + (wave + " << j * w << "ul) * ld, ub + (wave + " << j * w
+        << "ul) * ld, offn);\n      }\n";
+    }
+    o << "    }\n";
+    o << "    const double2* const X = xs + cur * " << nv * 64 << "u;\n";
+  } else {
+    // One workgroup per 64 boxes; each wave reads the variables its groups
+    // need straight from the box arrays.
+    o << "  __shared__ unsigned feas_s[" << 64 * w << "];\n";
+    o << "  __shared__ double obj_s[128];\n";
+    o << "  const unsigned cur = 0u;\n";
+    o << "  const unsigned long e = (unsigned long)blockIdx.x * 64ul + lane;\n";
+    o << "  const bool active = e < n;\n";
+    o << "  const unsigned eo = 8u * (unsigned)(active ? e : n - 1);\n";
+  }
+  std::string const ind = persistent ? "    " : "  ";
+  o << ind << "double* const rt = active ? roots : nullptr;\n";
+  o << ind << "bool f = true;\n";
+  o << ind << "const int t = enter_round_up();\n";
+  // Wave v runs groups v, v + w, v + 2w, ...; their statements are
+  // interleaved one by one, each group keeping its own order and accumulator.
+  // The groups share nothing they write, so the results are those of the
+  // groups run one after another, while neighbouring statements are
+  // independent (FP64 results take several issue slots to come back).
+  for (std::uint32_t v = 0; v < w; ++v) {
+    std::set<std::uint32_t> vars, temps;
+    std::vector<std::vector<std::string>> streams;
+    bool has_obj = false;
+    for (std::uint32_t g = v; g < k; g += w) {
+      std::uint32_t const lo = t.group_begin.size() > g ? t.group_begin[g] : 0;
+      std::uint32_t const hi =
+          t.group_begin.size() > g + 1 ? t.group_begin[g + 1] : static_cast<std::uint32_t>(t.code.size());
+      slot_use(t, lo, hi, vars, temps);
+      std::vector<std::string> st;
+      std::string const acc = "acc" + std::to_string(streams.size());
+      for (std::uint32_t i = lo; i < hi; ++i) {
+        std::string line = emit_word(t.code[i], acc);
+        if (!line.empty()) {
+          st.push_back(ind + line);
+        }
+      }
+      streams.push_back(std::move(st));
+      has_obj = has_obj || g == t.objective_group;
+    }
+    o << ind << (v == 0 ? "if" : "else if") << " (wave == " << v << "u) {\n";
+    for (std::uint32_t x : vars) {
+      if (persistent) {
+        o << ind << "  const Iv " << slot(x) << " = {X[" << x * 64 << "u + lane].x, X[" << x * 64 << "u + lane].y};\n";
+      } else {
+        o << ind << "  const Iv " << slot(x) << " = load_box(lb + " << x << "ul * ld, ub + " << x << "ul * ld, eo);\n";
+      }
+    }
+    for (std::uint32_t x : temps) {
+      o << ind << "  Iv " << slot(x) << ";\n";
+    }
+    for (std::size_t j = 0; j < streams.size(); ++j) {
+      o << ind << "  Iv acc" << j << " = {0.0, 0.0};\n";
+    }
+    o << ind << "  Iv obj = {0.0, 0.0};\n";
+    for (std::size_t i = 0;; ++i) {
+      bool any = false;
+      for (auto const& st : streams) {
+        if (i < st.size()) {
+          o << st[i];
+          any = true;
+        }
+      }
+      if (!any) {
+        break;
+      }
+    }
+    if (has_obj) {
+      o << ind << "  const Bnd ob = to_output(obj, t);\n";
+      o << ind << "  obj_s[cur * 128u + lane] = ob.lb;\n" << ind << "  obj_s[cur * 128u + 64u + lane] = ob.ub;\n";
+    }
+    o << ind << "}\n";
+  }
+  o << ind << "leave_round_up(t);\n";
+  o << ind << "feas_s[cur * " << 64 * w << "u + wave * 64u + lane] = f ? 1u : 0u;\n";
+  if (persistent) {
+    std::uint32_t const share = (nv + w - 1) / w;
+    o << "    if (has_next) {\n";
+    for (std::uint32_t j = 0; j < share; ++j) {
+      o << "      if (wave + " << j * w << "u < " << nv << "u) {\n"
+        << "        xs[(1u - cur) * " << nv * 64 << "u + (wave + " << j * w << "u) * 64u + lane] = make_double2(pre["
+        << j << "].nl, pre[" << j << "].ub);\n      }\n";
+    }
+    o << "    }\n";
+  }
+  o << ind << "__syncthreads();\n";
+  o << ind << "if (wave == 0u && active) {\n";
+  o << ind << "  unsigned all = 1u;\n";
+  for (std::uint32_t v = 0; v < w; ++v) {
+    o << ind << "  all &= feas_s[cur * " << 64 * w << "u + " << v * 64 << "u + lane];\n";
+  }
+  o << ind << "  feasible[e] = all ? 1 : 0;\n" << ind << "  obj_lb[e] = obj_s[cur * 128u + lane];\n"
+    << ind << "  if (obj_ub != nullptr) {\n" << ind << "    obj_ub[e] = obj_s[cur * 128u + 64u + lane];\n" << ind
+    << "  }\n" << ind << "}\n";
+  if (persistent) {
+    o << "    if (!has_next) {\n      break;\n    }\n    tile = next;\n  }\n";
+  }
+  o << "}\n";
+  return o.str();
+}
+
+/// A loaded code object; unloaded when the last handle goes.
+struct CompiledBoxes::Module
+{
+  int device = 0;
+  std::vector<Instr> code;
+  bool persistent = false;
+  hipModule_t module = nullptr;
+  hipFunction_t fn = nullptr;
+
+  ~Module()
+  {
+    if (module != nullptr) {
+      int prev = 0;
+      (void)hipGetDevice(&prev);
+      (void)hipSetDevice(device);
+      (void)hipModuleUnload(module);
+      (void)hipSetDevice(prev);
+    }
+  }
+};
+
+void CompiledBoxes::launch(const double* lb, const double* ub, std::uint64_t ld, std::uint64_t n,
+                           std::uint8_t* feasible, double* obj_lb, double* obj_ub, double* roots,
+                           hipStream_t stream) const
+{
+  if (n == 0) {
+    return;
+  }
+  if (n > kMaxBoxes) {
+    throw ShapeMismatch("compiled box kernels take at most 2^29 boxes per call");
+  }
+  unsigned long ld_ = ld, n_ = n;
+  void* args[] = {&lb, &ub, &ld_, &n_, &feasible, &obj_lb, &obj_ub, &roots};
+  std::uint64_t const tiles = (n + 63) / 64;
+  std::uint64_t const blocks = persistent_ ? std::min<std::uint64_t>(tiles, std::uint64_t(n_cus_) * wgs_per_cu_) : tiles;
+  hip_check(hipModuleLaunchKernel(mod_->fn, static_cast<unsigned>(blocks), 1, 1, 64 * waves_, 1, 1, 0, stream, args,
+                                  nullptr),
+            "hipModuleLaunchKernel(compiled boxes)");
+}
+
+std::shared_ptr<CompiledBoxes> compile_boxes(const tape::Tape& t, int device, const Shape& shape)
+{
+  static std::mutex mu;
+  static std::map<std::tuple<int, bool, std::uint32_t, std::uint64_t, std::size_t>,
+                  std::weak_ptr<const CompiledBoxes::Module>>
+      cache;
+  bool const persistent = shape.persistent;
+  std::uint32_t const waves = shape_waves(t, shape);
+  std::uint64_t h = fnv1a(t.code.data(), t.code.size() * sizeof(Instr));
+  h = fnv1a(&t.n_vars, sizeof t.n_vars, h);
+  h = fnv1a(t.group_begin.data(), t.group_begin.size() * sizeof(std::uint32_t), h);
+  h = fnv1a(&t.objective_group, sizeof t.objective_group, h);
+  auto const key = std::make_tuple(device, persistent, waves, h, t.code.size());
+
+  std::shared_ptr<CompiledBoxes> cb(new CompiledBoxes());
+  cb->groups_ = std::max<std::uint32_t>(t.n_groups, 1);
+  cb->waves_ = waves;
+  cb->persistent_ = persistent;
+  int cus = 0;
+  hip_check(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device), "hipDeviceGetAttribute(CUs)");
+  cb->n_cus_ = static_cast<std::uint32_t>(std::max(cus, 1));
+
+  std::lock_guard<std::mutex> lock(mu);
+  if (auto it = cache.find(key); it != cache.end()) {
+    if (auto sp = it->second.lock();
+        sp && sp->code.size() == t.code.size()
+        && std::memcmp(sp->code.data(), t.code.data(), t.code.size() * sizeof(Instr)) == 0) {
+      cb->mod_ = sp;
+      return cb;
+    }
+  }
+  auto const t0 = std::chrono::steady_clock::now();
+  std::vector<char> const code = compile_code_object(boxes_source(t, shape));
+  auto m = std::make_shared<CompiledBoxes::Module>();
+  m->device = device;
+  m->code = t.code;
+  m->persistent = persistent;
+  hip_check(hipModuleLoadData(&m->module, code.data()), "hipModuleLoadData(compiled boxes)");
+  hip_check(hipModuleGetFunction(&m->fn, m->module, kBoxesKernel), "hipModuleGetFunction(compiled boxes)");
+  cb->mod_ = m;
+  cb->code_bytes_ = code.size();
+  cb->compile_seconds_ = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  cache[key] = m;
+  return cb;
+}
+
+std::vector<char> compile_code_object(const std::string& src)
+{
+  hiprtcProgram prog = nullptr;
+  rtc_check(hiprtcCreateProgram(&prog, src.c_str(), "cmi_boxes.hip", 0, nullptr, nullptr), "hiprtcCreateProgram");
+  hiprtcResult const rc = hiprtcCompileProgram(prog, static_cast<int>(std::size(kOptions)), kOptions);
+  if (rc != HIPRTC_SUCCESS) {
+    std::size_t n = 0;
+    (void)hiprtcGetProgramLogSize(prog, &n);
+    std::string log(n, '\0');
+    if (n > 0) {
+      (void)hiprtcGetProgramLog(prog, log.data());
+    }
+    (void)hiprtcDestroyProgram(&prog);
+    throw DeviceError(static_cast<int>(rc), "compiling the tape kernel failed: " + log);
+  }
+  std::size_t size = 0;
+  rtc_check(hiprtcGetCodeSize(prog, &size), "hiprtcGetCodeSize");
+  std::vector<char> code(size);
+  rtc_check(hiprtcGetCode(prog, code.data()), "hiprtcGetCode");
+  (void)hiprtcDestroyProgram(&prog);
+  return code;
+}
+
+}  // namespace cmi::jit

@@ -10,17 +10,10 @@
 // waves of a workgroup evaluate the same 64 boxes together and share the
 // variable slots.
 //
-// Outward rounding. gfx950 has no per-instruction rounding modes for FP64, so
-// the interval interpreter runs inside a region where the MODE register's FP64
-// rounding field is set to round-toward-+infinity, and intervals are held as
-// (-lb, ub): lower bounds are then computed as -RD(x) = RU(-x) through VALU
-// source negation, which is exact, and every interval operation is one
-// instruction per bound. Every rounding FP64 instruction of the interval path
-// is written as inline asm that names the region's token, so the compiler can
-// neither fold the negations away nor move an operation out of the region; the
-// region is entered and left by volatile asm with a memory clobber. The point
-// (sampler/enumerator) interpreter never enters the region and is ordinary
-// round-to-nearest code.
+// Outward rounding: the interval path runs inside the round-toward-+infinity
+// region of interval_ops.hpp, on intervals held as (-lb, ub), one asm
+// instruction per bound. The point (sampler/enumerator) interpreter never
+// enters the region and is ordinary round-to-nearest code.
 //
 // The op semantics follow the reference's interval and point functors
 // (backend/graph/ops.cuh:24-355, kernels.cuh:177-340) for the QCQP op set.
@@ -36,6 +29,7 @@
 
 #include "cmi/tape.hpp"
 #include "device.hpp"
+#include "interval_ops.hpp"
 
 namespace cmi::dev
 {
@@ -46,179 +40,9 @@ using tape::TOp;
 namespace
 {
 
-// ---------------------------------------------------------------------------
-// Round-toward-+infinity region and the interval operations on (-lb, ub)
-// ---------------------------------------------------------------------------
+using namespace cmi_iv;
 
-__device__ __forceinline__ int enter_round_up()
-{
-  int tok;
-  asm volatile(
-      "s_nop 1\n\t"
-      "s_setreg_imm32_b32 hwreg(HW_REG_MODE, 2, 2), 1\n\t"
-      "s_mov_b32 %0, 0"
-      : "=s"(tok)
-      :
-      : "memory");
-  return tok;
-}
-
-__device__ __forceinline__ void leave_round_up(int tok)
-{
-  asm volatile(
-      "s_nop 1\n\t"
-      "s_setreg_imm32_b32 hwreg(HW_REG_MODE, 2, 2), 0 ; %0"
-      :
-      : "s"(tok)
-      : "memory");
-}
-
-/// An interval [lb, ub] held as nl = -lb and ub.
-struct Iv
-{
-  double nl, ub;
-};
-
-// [a] + [b]: -lb = RU(-al + -bl), ub = RU(au + bu)
-__device__ __forceinline__ Iv iv_add(Iv a, Iv b, int t)
-{
-  Iv r;
-  asm("v_add_f64 %0, %2, %3\n\t"
-      "v_add_f64 %1, %4, %5 ; %6"
-      : "=&v"(r.nl), "=&v"(r.ub)
-      : "v"(a.nl), "v"(b.nl), "v"(a.ub), "v"(b.ub), "s"(t));
-  return r;
-}
-
-// [a] - [b]: -lb = RU(-al + bu), ub = RU(au + -bl)
-__device__ __forceinline__ Iv iv_sub(Iv a, Iv b, int t)
-{
-  Iv r;
-  asm("v_add_f64 %0, %2, %3\n\t"
-      "v_add_f64 %1, %4, %5 ; %6"
-      : "=&v"(r.nl), "=&v"(r.ub)
-      : "v"(a.nl), "v"(b.ub), "v"(a.ub), "v"(b.nl), "s"(t));
-  return r;
-}
-
-// [a] + c: -lb = RU(-al - c), ub = RU(au + c)
-__device__ __forceinline__ Iv iv_add_imm(Iv a, double c, int t)
-{
-  Iv r;
-  asm("v_add_f64 %0, %2, -%4\n\t"
-      "v_add_f64 %1, %3, %4 ; %5"
-      : "=&v"(r.nl), "=&v"(r.ub)
-      : "v"(a.nl), "v"(a.ub), "v"(c), "s"(t));
-  return r;
-}
-
-// [a] - c: -lb = RU(-al + c), ub = RU(au - c)
-__device__ __forceinline__ Iv iv_sub_imm(Iv a, double c, int t)
-{
-  Iv r;
-  asm("v_add_f64 %0, %2, %4\n\t"
-      "v_add_f64 %1, %3, -%4 ; %5"
-      : "=&v"(r.nl), "=&v"(r.ub)
-      : "v"(a.nl), "v"(a.ub), "v"(c), "s"(t));
-  return r;
-}
-
-// c - [a]: -lb = RU(au - c), ub = RU(c + -al)
-__device__ __forceinline__ Iv iv_imm_sub(double c, Iv a, int t)
-{
-  Iv r;
-  asm("v_add_f64 %0, %3, -%4\n\t"
-      "v_add_f64 %1, %4, %2 ; %5"
-      : "=&v"(r.nl), "=&v"(r.ub)
-      : "v"(a.nl), "v"(a.ub), "v"(c), "s"(t));
-  return r;
-}
-
-__device__ __forceinline__ Iv iv_neg(Iv a) { return {a.ub, a.nl}; }
-
-// The point interval [c, c] (exact: c + 0 and -c + 0 are c and -c, or +0).
-__device__ __forceinline__ Iv iv_const(double c, int t)
-{
-  Iv r;
-  asm("v_add_f64 %0, -%2, 0\n\t"
-      "v_add_f64 %1, %2, 0 ; %3"
-      : "=&v"(r.nl), "=&v"(r.ub)
-      : "v"(c), "s"(t));
-  return r;
-}
-
-// [x] * [y] from the four endpoint products (xl = -xnl, yl = -ynl):
-// -lb = max RU(-xi*yj), ub = max RU(xi*yj).
-__device__ __forceinline__ Iv iv_mul(Iv x, Iv y, int t)
-{
-  Iv r;
-  double t0, t1, t2, t3;
-  asm("v_mul_f64 %2, -%6, %8\n\t"  // -(xl*yl) = -(xnl*ynl)
-      "v_mul_f64 %3, %6, %9\n\t"   // -(xl*yu) =   xnl*yu
-      "v_mul_f64 %4, %7, %8\n\t"   // -(xu*yl) =   xu*ynl
-      "v_mul_f64 %5, -%7, %9\n\t"  // -(xu*yu)
-      "v_max_f64 %2, %2, %3\n\t"
-      "v_max_f64 %4, %4, %5\n\t"
-      "v_max_f64 %0, %2, %4\n\t"
-      "v_mul_f64 %2, %6, %8\n\t"   // xl*yl = xnl*ynl
-      "v_mul_f64 %3, -%6, %9\n\t"  // xl*yu = -xnl*yu
-      "v_mul_f64 %4, -%7, %8\n\t"  // xu*yl = -xu*ynl
-      "v_mul_f64 %5, %7, %9\n\t"   // xu*yu
-      "v_max_f64 %2, %2, %3\n\t"
-      "v_max_f64 %4, %4, %5\n\t"
-      "v_max_f64 %1, %2, %4 ; %10"
-      : "=&v"(r.nl), "=&v"(r.ub), "=&v"(t0), "=&v"(t1), "=&v"(t2), "=&v"(t3)
-      : "v"(x.nl), "v"(x.ub), "v"(y.nl), "v"(y.ub), "s"(t));
-  return r;
-}
-
-// [a] * c for a (wave-uniform) scalar c: c > 0 -> [RD(al*c), RU(au*c)],
-// c < 0 -> [RD(au*c), RU(al*c)], otherwise [0, 0].
-__device__ __forceinline__ Iv iv_mul_imm(Iv a, double c, int t)
-{
-  Iv r;
-  if (c > 0.0) {
-    asm("v_mul_f64 %0, %2, %4\n\t"  // -(al*c) = anl*c
-        "v_mul_f64 %1, %3, %4 ; %5"
-        : "=&v"(r.nl), "=&v"(r.ub)
-        : "v"(a.nl), "v"(a.ub), "v"(c), "s"(t));
-  } else if (c < 0.0) {
-    asm("v_mul_f64 %0, -%3, %4\n\t"  // -(au*c)
-        "v_mul_f64 %1, -%2, %4 ; %5"  // al*c = -anl*c
-        : "=&v"(r.nl), "=&v"(r.ub)
-        : "v"(a.nl), "v"(a.ub), "v"(c), "s"(t));
-  } else {
-    r = {0.0, 0.0};
-  }
-  return r;
-}
-
-// [a]^2: s = max(lb, min(ub, 0)) is the endpoint nearest zero, or 0 when the
-// interval straddles zero; -lb = RU(-s*s); ub = RU(m*m), m = max(|lb|, |ub|).
-__device__ __forceinline__ Iv iv_sqr(Iv a, int t)
-{
-  Iv r;
-  double s, m;
-  asm("v_min_f64 %2, %5, 0\n\t"
-      "v_max_f64 %2, -%4, %2\n\t"
-      "v_mul_f64 %0, -%2, %2\n\t"
-      "v_max_f64 %3, |%4|, |%5|\n\t"
-      "v_mul_f64 %1, %3, %3 ; %6"
-      : "=&v"(r.nl), "=&v"(r.ub), "=&v"(s), "=&v"(m)
-      : "v"(a.nl), "v"(a.ub), "s"(t));
-  return r;
-}
-
-/// [lb, ub] with a zero bound reported as +0 (x + 0 is exact otherwise).
-__device__ __forceinline__ double2 to_output(Iv a, int t)
-{
-  double2 o;
-  asm("v_add_f64 %0, -%2, 0\n\t"
-      "v_add_f64 %1, %3, 0 ; %4"
-      : "=&v"(o.x), "=&v"(o.y)
-      : "v"(a.nl), "v"(a.ub), "s"(t));
-  return o;
-}
+__device__ __forceinline__ double2 to_d2(Bnd b) { return make_double2(b.lb, b.ub); }
 
 // ---------------------------------------------------------------------------
 // Region decode and sampling (same arithmetic as region::decode::slot_bounds
@@ -387,12 +211,12 @@ __device__ __forceinline__ IntervalResult run_interval(ConstInstrPtr code, std::
         case TOp::AccImmSub: acc = iv_imm_sub(ins.imm, acc, t); break;
         case TOp::AccScale: acc = iv_mul_imm(acc, ins.imm, t); break;
         case TOp::AccNeg: acc = iv_neg(acc); break;
-        case TOp::AccCheckLE: res.feasible = res.feasible && (-acc.nl <= ins.imm); break;
-        case TOp::AccCheckEQ: res.feasible = res.feasible && (-acc.nl <= ins.imm) && (ins.imm <= acc.ub); break;
+        case TOp::AccCheckLE: res.feasible = res.feasible && check_le(acc, ins.imm); break;
+        case TOp::AccCheckEQ: res.feasible = res.feasible && check_eq(acc, ins.imm); break;
         case TOp::AccObjective: res.obj = acc; break;
         case TOp::AccOutput:
           if (roots != nullptr) {
-            roots[static_cast<std::uint64_t>(ins.b) * roots_stride + e] = to_output(acc, t);
+            roots[static_cast<std::uint64_t>(ins.b) * roots_stride + e] = to_d2(to_output(acc, t));
           }
           break;
         default: break;
@@ -413,17 +237,17 @@ __device__ __forceinline__ IntervalResult run_interval(ConstInstrPtr code, std::
       case TOp::Sqr: r = iv_sqr(a, t); break;
       case TOp::Neg: r = iv_neg(a); break;
       case TOp::CheckLE:
-        res.feasible = res.feasible && (-a.nl <= ins.imm);
+        res.feasible = res.feasible && check_le(a, ins.imm);
         continue;
       case TOp::CheckEQ:
-        res.feasible = res.feasible && (-a.nl <= ins.imm) && (ins.imm <= a.ub);
+        res.feasible = res.feasible && check_eq(a, ins.imm);
         continue;
       case TOp::Objective:
         res.obj = a;
         continue;
       case TOp::Output:
         if (roots != nullptr) {
-          roots[static_cast<std::uint64_t>(ins.b) * roots_stride + e] = to_output(a, t);
+          roots[static_cast<std::uint64_t>(ins.b) * roots_stride + e] = to_d2(to_output(a, t));
         }
         continue;
       default:
@@ -591,14 +415,14 @@ __global__ __launch_bounds__(kTile * kMaxGroups) void k_boxes(const Instr* __res
   int const t = enter_round_up();
   IntervalResult const res = run_interval(as_constant(code_g + begin), groups.begin[wave + 1] - begin, S, lane, t,
                                           active ? roots : nullptr, n_boxes, e);
-  double2 const obj = to_output(res.obj, t);
-  leave_round_up(t);
+  Bnd const obj = to_output(res.obj, t);
+  leave_round_up(t, obj.lb, obj.ub);
   // per-group results after the slots: feasibility [n_groups][64], objective [64]
   auto* feas_g = reinterpret_cast<std::uint32_t*>(S + n_slots * kTile);
   auto* obj_s = reinterpret_cast<double2*>(feas_g + kMaxGroups * kTile);
   feas_g[wave * kTile + lane] = res.feasible ? 1u : 0u;
   if (wave == groups.objective_group) {
-    obj_s[lane] = obj;
+    obj_s[lane] = to_d2(obj);
   }
   __syncthreads();
   if (wave == 0 && active) {
@@ -671,11 +495,11 @@ __global__ __launch_bounds__(kTile) void k_items(const Item* __restrict__ items,
     }
     int const t = enter_round_up();
     IntervalResult const res = run_interval(code, it.n_instr, Sl, lane, t, nullptr, 0, 0);
-    double2 const obj = to_output(res.obj, t);
-    leave_round_up(t);
+    Bnd const obj = to_output(res.obj, t);
+    leave_round_up(t, obj.lb, obj.ub);
     if (active) {
       it.feasible[region] = res.feasible ? 1 : 0;
-      it.obj_lb[region] = obj.x;
+      it.obj_lb[region] = obj.lb;
     }
     return;
   } else {

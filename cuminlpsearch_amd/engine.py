@@ -74,6 +74,23 @@ class EngineProblem:
         _c.check(_c.lib().cmi_tape_compile_info(self._h, C.byref(out), buf, (1 << 22) if listing else 0))
         return out, (buf.value.decode() if listing else None)
 
+    def compiled_source(self, persistent: int = -1, waves: int = 0) -> str:
+        """Host-only: HIP source of the compiled explicit-box kernel
+        (persistent: 1 persistent form, 0 direct form, -1 the default)."""
+        n = C.c_size_t()
+        _c.check(_c.lib().cmi_tape_compiled_source(self._h, persistent, waves, None, 0, C.byref(n)))
+        buf = C.create_string_buffer(n.value + 1)
+        _c.check(_c.lib().cmi_tape_compiled_source(self._h, persistent, waves, buf, n.value + 1, C.byref(n)))
+        return buf.value.decode()
+
+    def compiled_object(self, persistent: int = -1, waves: int = 0) -> bytes:
+        """Host-only: the gfx950 code object of the compiled explicit-box kernel."""
+        n = C.c_size_t()
+        _c.check(_c.lib().cmi_tape_compiled_object(self._h, persistent, waves, None, 0, C.byref(n)))
+        buf = C.create_string_buffer(n.value)
+        _c.check(_c.lib().cmi_tape_compiled_object(self._h, persistent, waves, buf, n.value, C.byref(n)))
+        return buf.raw[:n.value]
+
     def tape_words(self, groups: int = 1):
         """Host-only compile into `groups` function groups: (words, group_begin,
         TapeInfo); words is a structured array with the 16-byte tape layout."""
@@ -132,6 +149,27 @@ class Tape:
         out = _c.TapeInfo()
         _c.check(_c.lib().cmi_tape_info_get(self._h, C.byref(out)))
         return out
+
+    def set_box_engine(self, engine: int) -> None:
+        """capi.BOX_AUTO, BOX_INTERPRETED or BOX_COMPILED (compiles now)."""
+        _c.check(_c.lib().cmi_tape_set_box_engine(self._h, engine))
+
+    def set_compiled_shape(self, persistent: bool = True, waves: int = 0, wgs_per_cu: int = 1) -> None:
+        """Shape of the compiled explicit-box kernel (persistent only when its LDS tiles fit;
+        waves per workgroup, 0 = one per function group)."""
+        _c.check(_c.lib().cmi_tape_set_compiled_shape(self._h, int(persistent), waves, wgs_per_cu))
+
+    def autotune(self, n_boxes: int = 65536) -> list[dict]:
+        """Time the explicit-box kernel configurations on n_boxes synthetic boxes,
+        keep the fastest; returns every candidate with its median kernel time."""
+        cap = 64
+        out = (_c.BoxCandidate * cap)()
+        n = C.c_size_t()
+        _c.check(_c.lib().cmi_tape_autotune(self._h, self.problem.handle, n_boxes, out, cap, C.byref(n)))
+        names = {_c.BOX_INTERPRETED: "interpreted", _c.BOX_COMPILED: "compiled"}
+        return [dict(engine=names.get(r.engine, r.engine), groups=r.groups, waves=r.waves,
+                     persistent=bool(r.persistent), wgs_per_cu=r.wgs_per_cu, kernel_ms=r.kernel_ms)
+                for r in out[:min(n.value, cap)]]
 
     def set_box_groups(self, k: int = 0) -> int:
         """Waves (function groups) per explicit-box workgroup; 0 = automatic.

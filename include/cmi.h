@@ -173,7 +173,23 @@ typedef struct
   size_t box_groups;      /* waves per explicit-box workgroup (function groups) */
   size_t box_instr;       /* tape words over all groups of the explicit-box code */
   size_t box_slots;       /* LDS slots per lane of the explicit-box code */
+  int box_engine;         /* cmi_box_engine in effect */
+  int box_compiled;       /* 1 when a compiled explicit-box kernel is loaded */
+  double compile_seconds; /* time spent compiling that kernel (0 if none) */
+  int compiled_persistent;  /* compiled kernel form: 1 persistent, 0 direct */
+  size_t compiled_wgs_per_cu;
+  size_t compiled_waves;    /* waves per workgroup of the compiled kernel */
 } cmi_tape_info;
+
+/* Kernel for explicit boxes: the tape interpreter, or the tape compiled to a
+ * straight-line gfx950 kernel at run time; AUTO compiles once a call has at
+ * least 32,768 boxes (tapes of at most 20,000 words). Results are identical. */
+typedef enum
+{
+  CMI_BOX_AUTO = 0,
+  CMI_BOX_INTERPRETED = 1,
+  CMI_BOX_COMPILED = 2
+} cmi_box_engine;
 
 /* Affine device cost per role, reference backend/cost_model.hpp:26-76 */
 typedef struct
@@ -283,6 +299,54 @@ void cmi_tape_destroy(cmi_tape* t);
  * (backend/graph/builder.cuh:261-514); this is its wave-shape knob. Must not
  * overlap a launch on the tape. */
 cmi_status cmi_tape_set_box_groups(cmi_tape* t, const cmi_problem* p, size_t k);
+/* Select the explicit-box kernel (cmi_box_engine). CMI_BOX_COMPILED compiles
+ * now; a compiler failure returns CMI_DEVICE_ERROR with the compiler log in
+ * cmi_last_error(). No reference counterpart (the reference replays a CUDA
+ * graph of per-node kernels, backend/graph/replay.cuh:493); must not overlap a
+ * launch on the tape. */
+cmi_status cmi_tape_set_box_engine(cmi_tape* t, int engine);
+/* Shape of the compiled explicit-box kernel. persistent = 1 walks the 64-box
+ * tiles with workgroups that stay resident (wgs_per_cu per CU) and stage the
+ * next tile into LDS while evaluating the current one (used only when two
+ * tiles of the problem's variables fit in LDS); 0 launches one workgroup per
+ * tile. waves = waves per workgroup (0 = one per function group, else at most
+ * 8): wave v runs the function groups v, v + waves, ..., interleaving their
+ * statements. Results do not depend on the shape. */
+cmi_status cmi_tape_set_compiled_shape(cmi_tape* t, int persistent, size_t waves, size_t wgs_per_cu);
+/* One explicit-box kernel configuration and its median kernel time. */
+typedef struct
+{
+  int engine;        /* cmi_box_engine: INTERPRETED or COMPILED */
+  size_t groups;     /* function groups */
+  size_t waves;      /* waves per workgroup (compiled) */
+  int persistent;    /* compiled form */
+  size_t wgs_per_cu; /* persistent form: workgroups per CU */
+  double kernel_ms;
+} cmi_box_candidate;
+
+/* Autotuner: time the interpreter and the compiled shapes (function groups,
+ * waves per workgroup, persistent or direct, workgroups per CU) on n_boxes
+ * synthetic boxes inside the bounds of `p` (the problem the tape was built
+ * from), leave the tape on the fastest, and report up to `cap` candidates in
+ * `results` (*n_results = how many ran). Takes seconds (each compiled shape is
+ * compiled once). No reference counterpart: the reference has one fixed
+ * launch shape per node kernel (backend/graph/kernels.cuh:177-281). */
+cmi_status cmi_tape_autotune(cmi_tape* t, const cmi_problem* p, size_t n_boxes, cmi_box_candidate* results, size_t cap,
+                             size_t* n_results);
+
+/* Host-only: the HIP source of the compiled explicit-box kernel of `p`'s tape
+ * (with the function groups a new tape uses), NUL-terminated into `buf` (up
+ * to `cap` bytes); *len receives the full length without the NUL. persistent:
+ * 1 the persistent form (INVALID_CONFIG if its tiles do not fit in LDS), 0 the
+ * direct form, -1 the form a new tape uses; waves as for
+ * cmi_tape_set_compiled_shape. */
+cmi_status cmi_tape_compiled_source(const cmi_problem* p, int persistent, size_t waves, char* buf, size_t cap,
+                                    size_t* len);
+/* Host-only: compile that source with the run-time compiler, exactly as
+ * CMI_BOX_COMPILED does, and copy the gfx950 code object into `buf` (when
+ * non-NULL and cap >= *len; SHAPE_MISMATCH otherwise); *len receives its size. */
+cmi_status cmi_tape_compiled_object(const cmi_problem* p, int persistent, size_t waves, void* buf, size_t cap,
+                                    size_t* len);
 /* Compile the tape on the host only (no device) and report its shape; also
  * writes the listing into `listing` (up to `cap` bytes, NUL-terminated) when
  * non-NULL. Same checks and errors as cmi_tape_create. */

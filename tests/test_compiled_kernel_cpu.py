@@ -1,0 +1,81 @@
+"""CPU checks of the explicit-box kernels compiled at run time from a tape:
+the generated source and the code object the run-time compiler makes from it
+(exactly as the engine does on a GPU) keep every rounding FP64 instruction
+inside the one round-toward-+infinity region, and every instruction there comes
+from the interval operations' inline asm."""
+import pathlib
+import re
+import shutil
+import subprocess
+
+import pytest
+
+import cuminlpsearch_amd as cmi
+from isa_tools import ENTER, LEAVE, PACKED_F32, ROUNDING, check_asm_regions
+
+import op_cases
+import problems as P
+
+HIPCC = "/opt/rocm/bin/hipcc"
+OBJDUMP = "/opt/rocm/llvm/bin/llvm-objdump"
+# the run-time compiler's options (jit.cpp)
+RTC_FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-ffp-contract=off",
+             "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
+
+
+def problems():
+    yield "c2", P.c2()
+    for name in ("const_root", "shared_generic", "acc_forms", "eq_acc", "mul_imm_zero"):
+        yield name, cmi.EngineProblem.from_problem(op_cases.CASES[name]())
+
+
+@pytest.mark.parametrize("persistent", [0, 1])
+@pytest.mark.parametrize("case", [name for name, _ in problems()])
+def test_generated_source_regions(tmp_path, case, persistent):
+    if not pathlib.Path(HIPCC).exists():
+        pytest.skip("hipcc not available")
+    eng = dict(problems())[case]
+    src = tmp_path / "k.hip"
+    # hiprtc includes the HIP device header by itself; hipcc needs it named
+    src.write_text("#include <hip/hip_runtime.h>\n" + eng.compiled_source(persistent))
+    out = tmp_path / "k.s"
+    subprocess.run([HIPCC, *RTC_FLAGS, "--cuda-device-only", "-S", "-o", str(out), str(src)], check=True,
+                   capture_output=True)
+    assert check_asm_regions(out.read_text().splitlines()) == 1
+
+
+@pytest.mark.parametrize("persistent", [0, 1])
+@pytest.mark.parametrize("case", ["c2", "shared_generic"])
+def test_runtime_code_object_rounds_only_inside_the_region(tmp_path, case, persistent):
+    if not pathlib.Path(OBJDUMP).exists():
+        pytest.skip("llvm-objdump not available")
+    eng = dict(problems())[case]
+    co = tmp_path / "k.co"
+    co.write_bytes(eng.compiled_object(persistent))
+    dis = subprocess.run([OBJDUMP, "-d", "--mcpu=gfx950", str(co)], check=True, capture_output=True,
+                         text=True).stdout.splitlines()
+    enters = [i for i, ln in enumerate(dis) if ENTER in ln]
+    leaves = [i for i, ln in enumerate(dis) if LEAVE in ln]
+    assert len(enters) == 1 and len(leaves) == 1 and enters[0] < leaves[0]
+    rounding = [i for i, ln in enumerate(dis) if ROUNDING.match(ln.strip())]
+    assert rounding and all(enters[0] < i < leaves[0] for i in rounding)
+    assert not any(PACKED_F32.match(ln.strip()) for ln in dis)
+    assert not any(re.match(r"\s*scratch_", ln) for ln in dis), "register spills in the compiled kernel"
+
+
+def test_source_is_deterministic_and_exact():
+    eng = P.c2()
+    a, b = eng.compiled_source(), eng.compiled_source()
+    assert a == b
+    words, _, _ = eng.tape_words(1)
+    # every immediate appears as an exact hexadecimal literal
+    for imm in {float(w["imm"]) for w in words if float(w["imm"]) != 0.0}:
+        assert f"({_c_hex(imm)})" in a, imm
+
+
+def _c_hex(x: float) -> str:
+    """printf("%a") spelling of a finite nonzero double."""
+    m, e = float.hex(x).split("p")
+    if "." in m:
+        m = m.rstrip("0").rstrip(".")
+    return f"{m}p{e}"

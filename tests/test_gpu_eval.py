@@ -55,9 +55,10 @@ def test_device_directed_primitives_match_golden(ctx, op):
     assert same_bits(oub, hi)
 
 
-def compare_with_oracle(ctx, eng: cmi.EngineProblem, lb, ub):
+def compare_with_oracle(ctx, eng: cmi.EngineProblem, lb, ub, engine=C.BOX_INTERPRETED):
     arrays = eng.arrays()
     tape = cmi.Tape(ctx, eng)
+    tape.set_box_engine(engine)
     f_d, lo_d, hi_d, r_d = tape.eval_boxes(lb, ub, roots=True)
     f_o, lo_o, hi_o, r_o = orc.eval_boxes(arrays, lb, ub, roots=True)
     assert np.array_equal(f_d, f_o)
@@ -75,15 +76,55 @@ def test_op_case_bitexact(ctx, name):
     compare_with_oracle(ctx, eng, lb, ub)
 
 
-def test_c2_kernel_alone_bitexact(ctx):
-    """BASELINE config 2 at full size: n=32, m=16, 600 reachable nodes, 65,536
-    boxes with log-uniform widths in [1e-4, 10]."""
+@pytest.mark.parametrize("engine", [C.BOX_INTERPRETED, C.BOX_COMPILED, C.BOX_AUTO])
+def test_c2_kernel_alone_bitexact(ctx, engine):
+    """The 600-node QCQP (n=32, m=16) at full size: 65,536 boxes with
+    log-uniform widths in [1e-4, 10], each box kernel."""
     eng = P.c2()
     assert eng.info().reachable_nodes == 600
     lb, ub = P.random_boxes(2026, 32, 65536, -5.0, 5.0)
-    f, lo, hi, roots = compare_with_oracle(ctx, eng, lb, ub)
+    f, lo, hi, roots = compare_with_oracle(ctx, eng, lb, ub, engine)
     assert 0 < f.sum() < f.size
     assert np.all(lo <= hi)
+
+
+def test_compiled_op_cases_bitexact(ctx):
+    """Every instruction form through the kernel compiled from the tape."""
+    lb, ub = P.random_boxes(5, 3, 400, -3.0, 4.0, 1e-6, 6.0)
+    for name in sorted(op_cases.CASES):
+        eng = cmi.EngineProblem.from_problem(op_cases.CASES[name]())
+        compare_with_oracle(ctx, eng, lb, ub, C.BOX_COMPILED)
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_compiled_random_dags_bitexact(ctx, seed):
+    p = P.random_dag(seed, n_vars=4 + seed % 5, n_ops=40 + 10 * (seed % 4), n_con=4)
+    eng = cmi.EngineProblem.from_problem(p)
+    lb, ub = P.random_boxes(100 + seed, eng.info().n_vars, 777, -3.0, 3.0, 1e-6, 6.0)
+    compare_with_oracle(ctx, eng, lb, ub, C.BOX_COMPILED)
+
+
+def test_compiled_ragged_counts_and_auto_switch(ctx):
+    eng = P.c1(3)
+    tape = cmi.Tape(ctx, eng)
+    tape.set_box_engine(C.BOX_COMPILED)
+    info = tape.info()
+    assert info.box_compiled == 1 and info.compile_seconds > 0
+    for n in (1, 63, 64, 65, 255, 256, 257, 1000):
+        lb, ub = P.random_boxes(n, 4, n, -10.0, 10.0)
+        f_d, lo_d, hi_d, r_d = tape.eval_boxes(lb, ub, roots=True)
+        f_o, lo_o, hi_o, r_o = orc.eval_boxes(eng.arrays(), lb, ub, roots=True)
+        assert np.array_equal(f_d, f_o) and same_bits(lo_d, lo_o) and same_bits(hi_d, hi_o) and same_bits(r_d, r_o)
+    # Auto: small calls stay on the interpreter, a large call compiles
+    t2 = cmi.Tape(ctx, P.c1(4))
+    lb, ub = P.random_boxes(1, 4, 100, -10.0, 10.0)
+    t2.eval_boxes(lb, ub)
+    assert t2.info().box_compiled == 0
+    lb, ub = P.random_boxes(2, 4, 40000, -10.0, 10.0)
+    f_d, lo_d, hi_d, _ = t2.eval_boxes(lb, ub)
+    assert t2.info().box_compiled == 1
+    f_o, lo_o, hi_o, _ = orc.eval_boxes(P.c1(4).arrays(), lb, ub)
+    assert np.array_equal(f_d, f_o) and same_bits(lo_d, lo_o) and same_bits(hi_d, hi_o)
 
 
 @pytest.mark.parametrize("seed", range(12))
@@ -182,3 +223,38 @@ def test_zero_boxes_is_a_noop(ctx):
     tape = cmi.Tape(ctx, eng)
     f, lo, hi, _ = tape.eval_boxes(np.zeros((4, 0)), np.zeros((4, 0)))
     assert f.size == 0
+
+
+def test_autotune_keeps_results_and_picks_the_fastest(ctx):
+    eng = P.c2()
+    tape = cmi.Tape(ctx, eng)
+    table = tape.autotune(8192)
+    assert len(table) >= 4
+    assert {c["engine"] for c in table} == {"interpreted", "compiled"}
+    best = min(table, key=lambda c: c["kernel_ms"])
+    info = tape.info()
+    assert (info.box_compiled == 1) == (best["engine"] == "compiled")
+    lb, ub = P.random_boxes(99, 32, 5000, -5.0, 5.0)
+    f_d, lo_d, hi_d, r_d = tape.eval_boxes(lb, ub, roots=True)
+    f_o, lo_o, hi_o, r_o = orc.eval_boxes(eng.arrays(), lb, ub, roots=True)
+    assert np.array_equal(f_d, f_o) and same_bits(lo_d, lo_o) and same_bits(hi_d, hi_o) and same_bits(r_d, r_o)
+
+
+@pytest.mark.parametrize("waves", [1, 2, 8])
+@pytest.mark.parametrize("persistent", [False, True])
+def test_compiled_shapes_bitexact(ctx, waves, persistent):
+    """Every compiled shape (groups interleaved in a wave, persistent tiles) gives
+    the oracle's results, including a ragged last tile and several tiles per
+    persistent workgroup."""
+    eng = P.c2()
+    tape = cmi.Tape(ctx, eng)
+    tape.set_box_groups(8)
+    tape.set_box_engine(C.BOX_COMPILED)
+    tape.set_compiled_shape(persistent, waves, 1)
+    info = tape.info()
+    assert info.compiled_waves == waves and info.compiled_persistent == int(persistent)
+    n = 256 * 64 * 3 + 17  # more tiles than one workgroup per CU: each persistent workgroup walks several
+    lb, ub = P.random_boxes(123, 32, n, -5.0, 5.0)
+    f_d, lo_d, hi_d, r_d = tape.eval_boxes(lb, ub, roots=True)
+    f_o, lo_o, hi_o, r_o = orc.eval_boxes(eng.arrays(), lb, ub, roots=True)
+    assert np.array_equal(f_d, f_o) and same_bits(lo_d, lo_o) and same_bits(hi_d, hi_o) and same_bits(r_d, r_o)

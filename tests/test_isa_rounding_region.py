@@ -4,16 +4,14 @@ engine's own inline asm (which computes lower bounds by negation); none is
 compiler-generated, since compiler-generated arithmetic there would silently
 round upward. Compiles kernels.hip to assembly on the host (no GPU needed)."""
 import pathlib
-import re
-import shutil
 import subprocess
 
 import pytest
 
+from isa_tools import check_asm_regions
+
 ROOT = pathlib.Path(__file__).resolve().parent.parent
 HIPCC = "/opt/rocm/bin/hipcc"
-
-ROUNDING = re.compile(r"^\s*(v_(add|mul|fma|sub|subrev|div_fmas|div_scale|rcp|sqrt|ldexp|cvt)[a-z0-9_]*f64)\b")
 
 
 @pytest.fixture(scope="module")
@@ -29,36 +27,6 @@ def asm(tmp_path_factory):
     return out.read_text().splitlines()
 
 
-def regions(lines):
-    inside = False
-    cur = []
-    for ln in lines:
-        if "s_setreg_imm32_b32 hwreg(HW_REG_MODE, 2, 2), 1" in ln:
-            inside = True
-            cur = []
-            continue
-        if "s_setreg_imm32_b32 hwreg(HW_REG_MODE, 2, 2), 0" in ln and inside:
-            inside = False
-            yield cur
-            continue
-        if inside:
-            cur.append(ln)
-
-
 def test_round_up_regions_hold_only_asm_arithmetic(asm):
-    found = 0
-    for body in regions(asm):
-        found += 1
-        in_asm = False
-        for ln in body:
-            if ";;#ASMSTART" in ln:
-                in_asm = True
-                continue
-            if ";;#ASMEND" in ln:
-                in_asm = False
-                continue
-            m = ROUNDING.match(ln)
-            assert not (m and not in_asm), f"compiler-generated rounding op inside the round-up region: {ln.strip()}"
-            assert "s_swappc" not in ln and "s_setpc" not in ln, "call inside the round-up region"
-            assert "s_setreg" not in ln or in_asm, "stray mode write inside the round-up region"
-    assert found >= 2  # the explicit-box kernel and the decode bounder
+    # the explicit-box kernel and the decode bounder
+    assert check_asm_regions(asm) >= 2
