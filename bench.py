@@ -6,10 +6,13 @@ bilinear pairs, 600 reachable DAG nodes, evaluated over 65,536 sub-boxes
 (log-uniform widths 1e-4..10) resident in HBM. One step = one launch over all
 65,536 boxes. value = boxes/s.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--serve]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-serve] [--no-batch] [--no-cpu]
 
---serve adds a served-vs-plain comparison (continuous batching against one
-solve at a time) on a Poisson stream of small QCQPs, reported under "serving".
+Also reported, on the same line: "serving" -- served (continuous batching) vs
+plain (one solve at a time) solves/s and p50/p99 latency on a Poisson stream of
+seeded QCQPs -- and "batching" -- 512 seeded QCQPs solved in shared launches vs
+one after another; "autotune" -- every explicit-box kernel shape the autotuner
+timed on this workload; "cpu_baseline" -- the CPU restatement on the host.
 """
 from __future__ import annotations
 
@@ -78,26 +81,40 @@ def cpu_baseline(arrays, lb, ub, budget_s: float = 12.0):
     return n / dt, threads, n, (feas, olb, oub)
 
 
-def serve_comparison(ctx, n_req: int = 96, rate: float = 400.0, seed: int = 7):
-    """Served (continuous batching) vs plain (one at a time) on the same Poisson stream."""
+CONVERGED_STOPS = (0, 1)  # gap closed, or every node fathomed
+
+
+def _converged(outs) -> float:
+    return float(np.mean([o.stop_reason in CONVERGED_STOPS for o in outs]))
+
+
+def _same_results(a_outs, b_outs) -> bool:
+    return all(a.iterations == b.iterations and a.upper_bound == b.upper_bound and a.lower_bound == b.lower_bound
+               and a.stop_reason == b.stop_reason for a, b in zip(a_outs, b_outs))
+
+
+def serve_comparison(ctx, n_req: int = 128, rate: float = 200.0, seed: int = 7, n_max: int = 16,
+                     iter_limit: int = 1000):
+    """Served (continuous batching) vs plain (one solve at a time) on the same
+    Poisson stream of seeded QCQPs: n log-uniform in [4, n_max], m = n/2,
+    bilinear density U[0.1, 0.3], gap 1e-6, iteration cap per request."""
     import cuminlpsearch_amd as cmi
 
     rng = np.random.default_rng(seed)
-    sizes = np.floor(np.exp(rng.uniform(np.log(4), np.log(12), n_req))).astype(int)
+    sizes = np.floor(np.exp(rng.uniform(np.log(4), np.log(n_max + 1), n_req))).astype(int)
     probs = [cmi.EngineProblem.qcqp(1000 + i, n=int(n), m=int(n) // 2, cont_lb=-5.0, cont_ub=5.0,
                                     pair_density=float(rng.uniform(0.1, 0.3))) for i, n in enumerate(sizes)]
     arrival = np.cumsum(rng.exponential(1.0 / rate, n_req))
     arrival -= arrival[0]
-    opts = dict(partition_num=2, enumerate_cap=2, max_cycle_size=4, sample_points=16, iter_limit=200,
+    opts = dict(partition_num=2, enumerate_cap=2, max_cycle_size=4, sample_points=16, iter_limit=iter_limit,
                 tolerance=1e-6)
     outs_s, fin_s, lat_s, st_s = cmi.serve_stream(ctx, probs, arrival, max_active=256, **opts)
     outs_p, fin_p, lat_p, st_p = cmi.serve_plain(ctx, probs, arrival, **opts)
-    same = all(a.iterations == b.iterations and a.upper_bound == b.upper_bound and a.lower_bound == b.lower_bound
-               for a, b in zip(outs_s, outs_p))
     return {
+        "workload": f"Poisson stream, {n_req} QCQPs, n log-uniform [4,{n_max}], m=n/2, density U[0.1,0.3], "
+                    f"gap 1e-6, cap {iter_limit} iterations, seed {seed}",
         "requests": n_req,
         "arrival_rate_per_s": rate,
-        "n_range": [int(sizes.min()), int(sizes.max())],
         "served_solves_per_s": n_req / st_s.wall_seconds,
         "plain_solves_per_s": n_req / st_p.wall_seconds,
         "speedup": st_p.wall_seconds / st_s.wall_seconds,
@@ -105,9 +122,40 @@ def serve_comparison(ctx, n_req: int = 96, rate: float = 400.0, seed: int = 7):
         "served_p99_ms": float(np.percentile(lat_s, 99) * 1e3),
         "plain_p50_ms": float(np.percentile(lat_p, 50) * 1e3),
         "plain_p99_ms": float(np.percentile(lat_p, 99) * 1e3),
-        "converged_fraction": float(np.mean([o.proven_optimal for o in outs_s])),
-        "iteration_cap": opts["iter_limit"],
-        "results_identical": bool(same),
+        "converged_fraction": _converged(outs_s),
+        "nodes_enqueued_total": int(sum(o.enqueued for o in outs_s)),
+        "results_identical": _same_results(outs_s, outs_p),
+    }
+
+
+def batch_comparison(ctx, n_inst: int = 512, seed: int = 11, iter_limit: int = 100):
+    """Batched (every instance's regions in the same launches) vs sequential
+    plain solves of the same seeded QCQPs: n uniform in [4, 48], m = n/2,
+    bilinear density 0.15, gap 1e-6, iteration cap per instance."""
+    import cuminlpsearch_amd as cmi
+
+    rng = np.random.default_rng(seed)
+    sizes = rng.integers(4, 49, n_inst)
+    probs = [cmi.EngineProblem.qcqp(5000 + i, n=int(n), m=int(n) // 2, cont_lb=-5.0, cont_ub=5.0,
+                                    pair_density=0.15) for i, n in enumerate(sizes)]
+    opts = dict(partition_num=2, enumerate_cap=2, max_cycle_size=4, sample_points=16, iter_limit=iter_limit,
+                tolerance=1e-6)
+    t0 = time.perf_counter()
+    outs_b, _ = cmi.solve_batch(ctx, probs, **opts)
+    t_b = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    outs_p = [cmi.solve(ctx, p, **opts)[0] for p in probs]
+    t_p = time.perf_counter() - t0
+    return {
+        "workload": f"{n_inst} QCQPs, n uniform [4,48], m=n/2, density 0.15, gap 1e-6, cap {iter_limit} "
+                    f"iterations, seed {seed}",
+        "instances": n_inst,
+        "batched_solves_per_s": n_inst / t_b,
+        "plain_solves_per_s": n_inst / t_p,
+        "speedup": t_p / t_b,
+        "converged_fraction": _converged(outs_b),
+        "iterations_total": int(sum(o.iterations for o in outs_b)),
+        "results_identical": _same_results(outs_b, outs_p),
     }
 
 
@@ -133,7 +181,10 @@ def main(argv=None) -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--serve", action="store_true")
+    ap.add_argument("--no-serve", action="store_true",
+                    help="skip the served-vs-plain Poisson-stream comparison (continuous batching)")
+    ap.add_argument("--no-batch", action="store_true",
+                    help="skip the batched-vs-sequential comparison over 512 seeded instances")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU-baseline leg")
     ap.add_argument("--box-groups", type=int, default=0,
                     help="waves (function groups) per 64-box workgroup; 0 = chosen from the tape")
@@ -141,6 +192,10 @@ def main(argv=None) -> int:
                     help="explicit-box kernel: the autotuner's pick over interpreter and compiled shapes on this "
                          "workload (default), the tape compiled to a straight-line kernel at run time, the tape "
                          "interpreter, or the engine's automatic choice")
+    ap.add_argument("--waves", type=int, default=0,
+                    help="compiled kernel: waves per workgroup (0 = one per function group)")
+    ap.add_argument("--persistent", action="store_true",
+                    help="compiled kernel: persistent double-buffered form instead of one workgroup per tile")
     ap.add_argument("--sweep-groups", action="store_true",
                     help="print the interpreter's kernel time for each group count, and the compiled kernel's, "
                          "to stderr before the timed run")
@@ -191,23 +246,35 @@ def main(argv=None) -> int:
     # (a null handle would select the engine context's internal stream) and the
     # HIP events below time exactly these launches.
     stream = torch.cuda.Stream(dev)
+    engines = {"compiled": capi.BOX_COMPILED, "interpreted": capi.BOX_INTERPRETED, "auto": capi.BOX_AUTO}
+    # The kernel configuration is fixed before any launch (the autotuner runs on
+    # its own synthetic boxes), so every launch below is the measured kernel.
+    tuning = None
+    if not a.sweep_groups:
+        if a.box_engine == "tuned":
+            tuning = tape.autotune(N_BOXES)
+        else:
+            tape.set_box_groups(a.box_groups)
+            tape.set_compiled_shape(a.persistent, a.waves, 1)
+            tape.set_box_engine(engines[a.box_engine])
     torch.cuda.synchronize()
     for k in range(a.warmup):
         launch(k, stream)
     torch.cuda.synchronize()
 
     def kernel_ms(steps):
-        """Mean kernel duration: HIP events around single launches on the stream they run on."""
-        starts = [torch.cuda.Event(enable_timing=True) for _ in range(steps)]
-        ends = [torch.cuda.Event(enable_timing=True) for _ in range(steps)]
+        """Mean kernel duration: HIP events on the launch stream around `steps`
+        back-to-back launches (the gap between two launches on one stream is
+        well under a microsecond, so this tracks the kernels' own duration)."""
+        start = torch.cuda.Event(enable_timing=True)
+        end = torch.cuda.Event(enable_timing=True)
+        start.record(stream)
         for k in range(steps):
-            starts[k].record(stream)
             launch(k, stream)
-            ends[k].record(stream)
+        end.record(stream)
         torch.cuda.synchronize()
-        return float(np.mean([s.elapsed_time(e) for s, e in zip(starts, ends)]))
+        return float(start.elapsed_time(end)) / steps
 
-    engines = {"compiled": capi.BOX_COMPILED, "interpreted": capi.BOX_INTERPRETED, "auto": capi.BOX_AUTO}
     if a.sweep_groups:
         tape.set_box_engine(capi.BOX_INTERPRETED)
         for g in range(1, 9):
@@ -234,13 +301,12 @@ def main(argv=None) -> int:
                     print(f"compiled box_groups={g} waves={i.compiled_waves} persistent={i.compiled_persistent} "
                           f"wgs_per_cu={wgs} kernel_ms={kernel_ms(20):.4f} compile_s={i.compile_seconds:.3f}",
                           file=sys.stderr, flush=True)
-        tape.set_compiled_shape(True, 0, 1)
-    tuning = None
-    if a.box_engine == "tuned":
-        tuning = tape.autotune(N_BOXES)
-    else:
-        tape.set_box_groups(a.box_groups)
-        tape.set_box_engine(engines[a.box_engine])
+        if a.box_engine == "tuned":
+            tuning = tape.autotune(N_BOXES)
+        else:
+            tape.set_box_groups(a.box_groups)
+            tape.set_compiled_shape(a.persistent, a.waves, 1)
+            tape.set_box_engine(engines[a.box_engine])
     info = tape.info()
     for k in range(3):
         launch(k, stream)
@@ -351,8 +417,10 @@ def main(argv=None) -> int:
         }
         line["parity_vs_cpu"] = "bit-exact" if parity else "MISMATCH"
 
-    if a.serve:
+    if not a.no_serve:
         line["serving"] = serve_comparison(ctx)
+    if not a.no_batch:
+        line["batching"] = batch_comparison(ctx)
 
     print(json.dumps(line))
     return 0
