@@ -5,7 +5,7 @@ The engine (HIP kernels for gfx950 + C++ search host + C-ABI) is
 fallback: ``capi.lib()`` raises if the library has not been built.
 """
 from . import capi
-from .engine import (Candidate, Context, EngineProblem, Role, Tape, qcqp_spec, serve_plain, serve_stream, solve,
+from .engine import (AggregateBounder, Candidate, Context, EngineProblem, Role, Tape, qcqp_spec, serve_plain, serve_stream, solve,
                      solve_batch, solve_options)
 from .model import Expr, Problem, ProblemArrays, evaluate_point, satisfies, sqr
 

@@ -91,6 +91,10 @@ class BoxCandidate(C.Structure):
                 ("wgs_per_cu", C.c_size_t), ("kernel_ms", C.c_double)]
 
 
+class AggregateBound(C.Structure):
+    _fields_ = [("lb", C.c_double), ("hull_ub", C.c_double), ("feasible", C.c_int)]
+
+
 class CostModel(C.Structure):
     _fields_ = [("sampler_fixed", C.c_size_t), ("sampler_per_region", C.c_size_t),
                 ("bounder_fixed", C.c_size_t), ("bounder_per_region", C.c_size_t),
@@ -165,6 +169,10 @@ _SIGNATURES = {
     "cmi_tape_destroy": (None, [_P]),
     "cmi_tape_set_box_groups": (C.c_int, [_P, _P, C.c_size_t]),
     "cmi_tape_set_box_engine": (C.c_int, [_P, C.c_int]),
+    "cmi_aggregate_build": (C.c_int, [_P, _P, C.c_size_t, C.c_size_t, C.c_size_t, C.c_size_t, C.c_size_t,
+                                     C.POINTER(_P), C.POINTER(OverBudget)]),
+    "cmi_bound_children": (C.c_int, [_P, C.c_size_t, _P, C.c_size_t, _P, _P, C.c_size_t,
+                                    C.POINTER(AggregateBound)]),
     "cmi_tape_autotune": (C.c_int, [_P, _P, C.c_size_t, C.POINTER(BoxCandidate), C.c_size_t,
                                    C.POINTER(C.c_size_t)]),
     "cmi_tape_set_compiled_shape": (C.c_int, [_P, C.c_int, C.c_size_t, C.c_size_t]),

@@ -720,7 +720,7 @@ cmi_status cmi_role_build(cmi_tape* t, int role_kind, const uint8_t* slot_kinds,
     }
     auto impl = std::make_unique<cmi::hip::RoleImpl>(t->tape, mode, comp, n_regions, samples_per_region,
                                                      budget_bytes);
-    *out = new cmi_role {std::move(impl), comp};
+    *out = new cmi_role {std::move(impl), comp, nullptr};
     g_last_error.clear();
     return CMI_OK;
   } catch (const cmi::backend::OverBudgetError& e) {
@@ -805,7 +805,68 @@ cmi_status cmi_enumerate(cmi_role* r, const cmi_ivl* box, size_t n_vars, const s
   return s;
 }
 
-size_t cmi_role_size(const cmi_role* r) { return r != nullptr && r->impl != nullptr ? r->impl->n_elems() : 0; }
+size_t cmi_role_size(const cmi_role* r)
+{
+  if (r == nullptr) {
+    return 0;
+  }
+  if (r->aggregate != nullptr) {
+    return r->aggregate->n_subregions();
+  }
+  return r->impl != nullptr ? r->impl->n_elems() : 0;
+}
+
+cmi_status cmi_aggregate_build(cmi_tape* t, const uint8_t* slot_kinds, size_t slot_count, size_t n_regions, size_t k,
+                               size_t max_batch, size_t budget_bytes, cmi_role** out, cmi_over_budget* facts)
+{
+  if (t == nullptr || out == nullptr || (slot_count > 0 && slot_kinds == nullptr)) {
+    return fail(CMI_INVALID_HANDLE, "null argument");
+  }
+  try {
+    auto comp = composition_of(slot_kinds, slot_count);
+    auto agg = std::make_unique<cmi::hip::AggregateBounderImpl>(t->tape, comp, n_regions, k, max_batch, budget_bytes);
+    *out = new cmi_role {nullptr, comp, std::move(agg)};
+    g_last_error.clear();
+    return CMI_OK;
+  } catch (const cmi::backend::OverBudgetError& e) {
+    if (facts != nullptr) {
+      std::memset(facts, 0, sizeof(*facts));
+      std::strncpy(facts->role, e.facts().role.c_str(), sizeof(facts->role) - 1);
+      facts->needed_bytes = e.facts().needed_bytes;
+      facts->budget_bytes = e.facts().budget_bytes;
+      facts->n_regions = e.facts().n_regions;
+      facts->bytes_per_element = e.facts().bytes_per_element;
+    }
+    return fail(CMI_OVER_BUDGET, e.what());
+  } catch (...) {
+    return guarded([] { throw; });
+  }
+}
+
+cmi_status cmi_bound_children(cmi_role* r, size_t n_batch, const cmi_ivl* boxes, size_t n_vars, const size_t* var_ids,
+                              const size_t* widths, size_t slot_count, cmi_aggregate_bound* out)
+{
+  if (r == nullptr || r->aggregate == nullptr) {
+    return fail(CMI_INVALID_HANDLE, "role is not an aggregate bounder");
+  }
+  if (n_batch > 0 && (boxes == nullptr || out == nullptr || (slot_count > 0 && (var_ids == nullptr || widths == nullptr)))) {
+    return fail(CMI_SHAPE_MISMATCH, "null boxes, slot arrays or output");
+  }
+  return guarded([&] {
+    std::vector<cmi::hip::AggregateBounderImpl::RegionView> views(n_batch);
+    for (size_t i = 0; i < n_batch; ++i) {
+      views[i].box = {reinterpret_cast<const cmi::interval*>(boxes) + i * n_vars, n_vars};
+      views[i].var_ids = {var_ids + i * slot_count, slot_count};
+      views[i].widths = {widths + i * slot_count, slot_count};
+    }
+    auto const res = r->aggregate->bound_children(views);
+    for (size_t i = 0; i < res.size(); ++i) {
+      out[i].lb = res[i].lb;
+      out[i].hull_ub = res[i].hull_ub;
+      out[i].feasible = res[i].feasible ? 1 : 0;
+    }
+  });
+}
 
 void cmi_role_destroy(cmi_role* r) { delete r; }
 

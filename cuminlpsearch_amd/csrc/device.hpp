@@ -56,6 +56,10 @@ struct Item
   double* cand_value;  // 1
   std::int64_t* cand_index;  // 1
   double* witness;  // n_vars
+  // aggregate outputs: child c of this item owns regions [c*refine, (c+1)*refine)
+  double* child_lb;  // k entries, +inf before the launch; min of masked obj_lb
+  double* child_ub;  // k entries, -inf before the launch; max of masked obj_ub
+  std::uint64_t refine;
 };
 
 enum class Mode : int
@@ -63,6 +67,7 @@ enum class Mode : int
   Bound = 0,
   Sample = 1,
   Enumerate = 2,
+  Aggregate = 3,
 };
 
 /// Launch `n_tiles` workgroups over `items`; `tile_item[t]` names the item of

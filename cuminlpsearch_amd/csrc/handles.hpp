@@ -6,6 +6,7 @@
 #include "cmi/model.hpp"
 #include "cmi/region.hpp"
 #include "cmi/search.hpp"
+#include "aggregate_backend.hpp"
 #include "hip_backend.hpp"
 
 // Definitions of the C-ABI's opaque handle types, shared by the C-ABI
@@ -29,6 +30,7 @@ struct cmi_role
 {
   std::unique_ptr<cmi::hip::RoleImpl> impl;
   cmi::region::Composition composition;
+  std::unique_ptr<cmi::hip::AggregateBounderImpl> aggregate;
 };
 
 namespace cmi::capi

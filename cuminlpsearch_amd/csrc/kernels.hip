@@ -479,7 +479,7 @@ __global__ __launch_bounds__(kTile) void k_items(const Item* __restrict__ items,
   ConstInstrPtr const code = as_constant(it.code);
   std::uint32_t const n_vars = it.n_vars;
 
-  if constexpr (M == static_cast<int>(Mode::Bound)) {
+  if constexpr (M == static_cast<int>(Mode::Bound) || M == static_cast<int>(Mode::Aggregate)) {
     double2* const Sl = lds;
     for (std::uint32_t v = 0; v < n_vars; ++v) {
       Sl[v * kTile + lane] = make_double2(-it.parent[2 * v], it.parent[2 * v + 1]);
@@ -497,9 +497,37 @@ __global__ __launch_bounds__(kTile) void k_items(const Item* __restrict__ items,
     IntervalResult const res = run_interval(code, it.n_instr, Sl, lane, t, nullptr, 0, 0);
     Bnd const obj = to_output(res.obj, t);
     leave_round_up(t, obj.lb, obj.ub);
-    if (active) {
-      it.feasible[region] = res.feasible ? 1 : 0;
-      it.obj_lb[region] = obj.lb;
+    if constexpr (M == static_cast<int>(Mode::Bound)) {
+      if (active) {
+        it.feasible[region] = res.feasible ? 1 : 0;
+        it.obj_lb[region] = obj.lb;
+      }
+    } else {
+      // Mask by feasibility with the reductions' identities (reference
+      // backend/aggregate/kernels.cuh:32-47), reduce each child's run of
+      // lanes with shuffles, and fold the run into the child's bound with one
+      // atomic min and max per child and tile. Min and max are exact, so the
+      // result does not depend on the order the atomics land in.
+      bool const live = active && res.feasible;
+      double lo = live ? obj.lb : INFINITY;
+      double hi = live ? obj.ub : -INFINITY;
+      std::uint64_t const child = static_cast<std::uint64_t>(region) / it.refine;
+#pragma unroll
+      for (int off = 1; off < 64; off <<= 1) {
+        double const olo = __shfl_down(lo, off, 64);
+        double const ohi = __shfl_down(hi, off, 64);
+        std::uint64_t const oc = __shfl_down(child, off, 64);
+        if (lane + off < 64 && oc == child) {
+          lo = fmin(lo, olo);
+          hi = fmax(hi, ohi);
+        }
+      }
+      std::uint64_t const prev = __shfl_up(child, 1, 64);
+      bool const head = lane == 0 || prev != child;
+      if (active && head) {
+        atomicMin(it.child_lb + child, lo);
+        atomicMax(it.child_ub + child, hi);
+      }
     }
     return;
   } else {
@@ -620,7 +648,8 @@ void launch_items(Mode mode, const Item* items, const std::uint32_t* tile_item, 
   if (n_tiles == 0) {
     return;
   }
-  std::size_t const bytes = static_cast<std::size_t>(lds_slots) * kTile * (mode == Mode::Bound ? 16 : 8);
+  bool const interval = mode == Mode::Bound || mode == Mode::Aggregate;
+  std::size_t const bytes = static_cast<std::size_t>(lds_slots) * kTile * (interval ? 16 : 8);
   switch (mode) {
     case Mode::Bound: {
       static bool once = (allow_large_lds(k_items<0>), true);
@@ -638,6 +667,12 @@ void launch_items(Mode mode, const Item* items, const std::uint32_t* tile_item, 
       static bool once = (allow_large_lds(k_items<2>), true);
       (void)once;
       hipLaunchKernelGGL(k_items<2>, dim3(n_tiles), dim3(kTile), bytes, stream, items, tile_item);
+      break;
+    }
+    case Mode::Aggregate: {
+      static bool once = (allow_large_lds(k_items<3>), true);
+      (void)once;
+      hipLaunchKernelGGL(k_items<3>, dim3(n_tiles), dim3(kTile), bytes, stream, items, tile_item);
       break;
     }
   }

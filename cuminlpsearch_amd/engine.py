@@ -214,6 +214,44 @@ class Tape:
             self._h = None
 
 
+class AggregateBounder:
+    """The k-child aggregate bounder (cmi_aggregate_build / cmi_bound_children)."""
+
+    def __init__(self, tape: Tape, slot_kinds, n_regions: int, k: int, max_batch: int = 1, budget_bytes: int = 0):
+        self.tape = tape
+        self.k = k
+        self.slot_kinds = np.ascontiguousarray(slot_kinds, np.uint8)
+        self._h = _handle()
+        self.facts = _c.OverBudget()
+        _c.check(_c.lib().cmi_aggregate_build(tape.handle, _c.ptr(self.slot_kinds), len(self.slot_kinds), n_regions,
+                                              k, max_batch, budget_bytes, C.byref(self._h), C.byref(self.facts)))
+
+    def bound_children(self, boxes, var_ids, widths):
+        """boxes (R, n_vars, 2), var_ids and widths (R, slot_count) -> (lb, hull_ub, feasible), each (R, k)."""
+        boxes = np.ascontiguousarray(boxes, np.float64)
+        if boxes.ndim == 2:
+            boxes = boxes[None]
+        r, nv = boxes.shape[0], boxes.shape[1]
+        slots = len(self.slot_kinds)
+        vid = np.ascontiguousarray(np.broadcast_to(np.asarray(var_ids, np.uint64).reshape(-1, slots), (r, slots)))
+        wid = np.ascontiguousarray(np.broadcast_to(np.asarray(widths, np.uint64).reshape(-1, slots), (r, slots)))
+        out = (_c.AggregateBound * (r * self.k))()
+        _c.check(_c.lib().cmi_bound_children(self._h, r, _c.ptr(boxes), nv, _c.ptr(vid), _c.ptr(wid),
+                                             len(self.slot_kinds), out))
+        lb = np.array([o.lb for o in out]).reshape(r, self.k)
+        ub = np.array([o.hull_ub for o in out]).reshape(r, self.k)
+        fe = np.array([o.feasible for o in out], np.int32).reshape(r, self.k)
+        return lb, ub, fe
+
+    def size(self) -> int:
+        return int(_c.lib().cmi_role_size(self._h))
+
+    def __del__(self):
+        if getattr(self, "_h", None) and _c._lib is not None:
+            _c.lib().cmi_role_destroy(self._h)
+            self._h = None
+
+
 @dataclass
 class Candidate:
     found: bool

@@ -396,6 +396,37 @@ cmi_status cmi_enumerate(cmi_role* r, const cmi_ivl* box, size_t n_vars, const s
 size_t cmi_role_size(const cmi_role* r); /* n_regions / n_samples / n_points */
 void cmi_role_destroy(cmi_role* r);
 
+/* -- aggregate (k-child) bounder ------------------------------------------------ */
+
+/* AggregateBound, reference aggregate/bound.hpp:42-64: min over a child's
+ * unexcluded subregions of the objective's lower bound, max of the upper
+ * bound (hull_ub, never an incumbent), and whether any subregion survived.
+ * An infeasible child is {+inf, +inf, 0}. */
+typedef struct
+{
+  double lb;
+  double hull_ub;
+  int feasible;
+} cmi_aggregate_bound;
+
+/* AggregateBackendFactory::build_bounder, reference
+ * backend/aggregate/factory.cuh and replay.cuh:49-118: a bounder for launch
+ * assignments with these slot kinds, N = n_regions subregions per region in k
+ * children of M = N / k (k must divide N), taking up to max_batch regions per
+ * call. Its device bytes do not grow with N beyond one u32 per 64 subregions.
+ * CMI_OVER_BUDGET with `facts` before allocating. The result is a cmi_role
+ * that only cmi_bound_children accepts. */
+cmi_status cmi_aggregate_build(cmi_tape* t, const uint8_t* slot_kinds, size_t slot_count, size_t n_regions, size_t k,
+                               size_t max_batch, size_t budget_bytes, cmi_role** out, cmi_over_budget* facts);
+/* AggregateBounder::bound_children, reference aggregate/bound.hpp:88-91, for
+ * n_batch regions in one launch (the reference builds one region per launch,
+ * backend/aggregate/replay.cuh:255-270): region i is boxes[i*n_vars ..] with
+ * launch slots var_ids[i*slot_count ..], widths[i*slot_count ..] (product N;
+ * the branch slots last, so child c owns subregions [c*M, (c+1)*M)). Writes
+ * n_batch * k bounds to `out`, region-major. */
+cmi_status cmi_bound_children(cmi_role* r, size_t n_batch, const cmi_ivl* boxes, size_t n_vars, const size_t* var_ids,
+                              const size_t* widths, size_t slot_count, cmi_aggregate_bound* out);
+
 /* -- search ------------------------------------------------------------------- */
 
 void cmi_solve_options_default(cmi_solve_options* o);

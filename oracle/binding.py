@@ -48,6 +48,8 @@ def lib() -> C.CDLL:
         h.orc_eval_points.argtypes = [P, P, C.c_size_t, C.c_size_t, P, P, P]
         h.orc_bound.restype = C.c_int
         h.orc_bound.argtypes = [P, P, P, P, P, C.c_size_t, P, P]
+        h.orc_aggregate.restype = C.c_int
+        h.orc_aggregate.argtypes = [P, P, P, P, P, C.c_size_t, C.c_size_t, P, P, P, P, P, P]
         h.orc_candidate.restype = C.c_int
         h.orc_candidate.argtypes = [P, P, P, P, P, C.c_size_t, C.c_size_t, C.c_uint64, P, P, P, P]
         h.orc_solve.restype = C.c_int
@@ -154,3 +156,25 @@ def add_ru_n(a, b) -> np.ndarray:
 def mul_ru_n(a, b) -> np.ndarray:
     """Element-wise RU(a * b)."""
     return _vec(lib().orc_mul_ru_n, a, b)
+
+
+def aggregate(arrays, box, var_ids, widths, kinds, k, per_region=False):
+    """k-child aggregate of one launch: (lb, hull_ub, feasible) per child, and
+    with per_region the per-subregion (feasible, obj_lb, obj_ub)."""
+    d = arrays.desc()
+    box = np.ascontiguousarray(box, np.float64).reshape(-1, 2)
+    vid = np.ascontiguousarray(var_ids, np.uint64)
+    wid = np.ascontiguousarray(widths, np.uint64)
+    kin = np.ascontiguousarray(kinds, np.uint8)
+    n = int(np.prod(wid)) if wid.size else 1
+    lb = np.zeros(k)
+    ub = np.zeros(k)
+    fe = np.zeros(k, np.int32)
+    rf = np.zeros(n, np.uint8) if per_region else None
+    rl = np.zeros(n) if per_region else None
+    ru = np.zeros(n) if per_region else None
+    _ok(lib().orc_aggregate(C.byref(d), _p(box), _p(vid), _p(wid), _p(kin), vid.size, k, _p(lb), _p(ub), _p(fe),
+                            _p(rf), _p(rl), _p(ru)))
+    if per_region:
+        return lb, ub, fe, (rf, rl, ru)
+    return lb, ub, fe

@@ -879,6 +879,59 @@ int orc_bound(const cmi_problem_desc* d, const cmi_ivl* box, const size_t* var_i
   }
 }
 
+// The k-child aggregate (reference backend/aggregate/kernels.cuh:32-47 and
+// replay.cuh:303-322): per child c, the min of obj_lb and the max of obj_ub
+// over its feasible subregions [c*M, (c+1)*M); a child with none is
+// {+inf, +inf, infeasible}. Also returns the per-subregion verdicts and
+// objective intervals when the arrays are non-null.
+int orc_aggregate(const cmi_problem_desc* d, const cmi_ivl* box, const size_t* var_ids, const size_t* widths,
+                  const uint8_t* kinds, size_t slots, size_t k, double* child_lb, double* child_ub, int* child_feasible,
+                  uint8_t* feasible, double* obj_lb, double* obj_ub)
+{
+  try {
+    Prob const p = load(d);
+    std::vector<Iv> parent(p.n_vars), sub, val;
+    for (std::size_t v = 0; v < p.n_vars; ++v) {
+      parent[v] = {box[v].lb, box[v].ub};
+    }
+    std::size_t n_regions = 1;
+    for (std::size_t j = 0; j < slots; ++j) {
+      n_regions *= widths[j];
+    }
+    if (k == 0 || n_regions % k != 0) {
+      throw std::runtime_error("k must divide the region count");
+    }
+    std::size_t const m = n_regions / k;
+    double const inf = std::numeric_limits<double>::infinity();
+    std::vector<double> lo(k, inf), hi(k, -inf);
+    for (std::size_t r = 0; r < n_regions; ++r) {
+      materialise(parent, r, var_ids, widths, kinds, slots, sub);
+      eval_interval(p, sub.data(), val);
+      bool const f = feasible_interval(p, val);
+      double const l = plus_zero(val[p.obj].lb), u = plus_zero(val[p.obj].ub);
+      if (feasible != nullptr) {
+        feasible[r] = f ? 1 : 0;
+        obj_lb[r] = l;
+        obj_ub[r] = u;
+      }
+      std::size_t const c = r / m;
+      double const ml = f ? l : inf, mu = f ? u : -inf;
+      lo[c] = ml < lo[c] ? ml : lo[c];
+      hi[c] = mu > hi[c] ? mu : hi[c];
+    }
+    for (std::size_t c = 0; c < k; ++c) {
+      bool const any = hi[c] > -inf;
+      child_lb[c] = any ? lo[c] : inf;
+      child_ub[c] = any ? hi[c] : inf;
+      child_feasible[c] = any ? 1 : 0;
+    }
+    return 0;
+  } catch (const std::exception& ex) {
+    g_err = ex.what();
+    return 1;
+  }
+}
+
 // samples == 0: the enumerator (lattice points); otherwise the sampler.
 int orc_candidate(const cmi_problem_desc* d, const cmi_ivl* box, const size_t* var_ids, const size_t* widths,
                   const uint8_t* kinds, size_t slots, size_t samples, uint64_t salt, int* found, double* value,
