@@ -14,7 +14,8 @@ LIB_PATH = pathlib.Path(__file__).resolve().parent / "lib" / "libcmi.so"
 
 # status codes (cmi_status)
 BOX_AUTO, BOX_INTERPRETED, BOX_COMPILED = range(3)
-OK, SHAPE_MISMATCH, INVALID_CONFIG, OVER_BUDGET, DEVICE_ERROR, INVALID_HANDLE, INVALID_DAG, INVALID_PROBLEM, ERROR = range(9)
+(OK, SHAPE_MISMATCH, INVALID_CONFIG, OVER_BUDGET, DEVICE_ERROR, INVALID_HANDLE, INVALID_DAG, INVALID_PROBLEM, ERROR,
+ PARSE_ERROR) = range(10)
 STATUS_NAMES = {
     OK: "CMI_OK",
     SHAPE_MISMATCH: "CMI_SHAPE_MISMATCH",
@@ -25,6 +26,7 @@ STATUS_NAMES = {
     INVALID_DAG: "CMI_INVALID_DAG",
     INVALID_PROBLEM: "CMI_INVALID_PROBLEM",
     ERROR: "CMI_ERROR",
+    PARSE_ERROR: "CMI_PARSE_ERROR",
 }
 
 # model::Op
@@ -147,6 +149,19 @@ class ServeStats(C.Structure):
                 ("wall_seconds", C.c_double)]
 
 
+class GamsOptions(C.Structure):
+    _fields_ = [("default_bound", C.c_double), ("fold_fixed_to_const", C.c_int), ("reject_discrete", C.c_int)]
+
+
+class GamsError(C.Structure):
+    _fields_ = [("line", C.c_int), ("kind", C.c_int)]
+
+
+class GamsInfo(C.Structure):
+    _fields_ = [("sense", C.c_int), ("n_vars", C.c_size_t), ("n_constraints", C.c_size_t),
+                ("n_warnings", C.c_size_t), ("objvar_kept", C.c_int), ("has_initial_point", C.c_int)]
+
+
 class CmiError(RuntimeError):
     def __init__(self, status: int, message: str):
         super().__init__(f"{STATUS_NAMES.get(status, status)}: {message}")
@@ -198,6 +213,19 @@ _SIGNATURES = {
     "cmi_serve_stream": (C.c_int, [_P, C.c_size_t, C.POINTER(_P), _P, C.POINTER(SolveOptions),
                                    C.POINTER(ServerOptions), C.POINTER(SolveOutcome), _P, _P,
                                    C.POINTER(ServeStats)]),
+    "cmi_gams_options_default": (None, [C.POINTER(GamsOptions)]),
+    "cmi_gams_parse": (C.c_int, [C.c_char_p, C.c_size_t, C.POINTER(GamsOptions), C.POINTER(_P), C.POINTER(_P),
+                                 C.POINTER(GamsError)]),
+    "cmi_gams_parse_file": (C.c_int, [C.c_char_p, C.POINTER(GamsOptions), C.POINTER(_P), C.POINTER(_P),
+                                      C.POINTER(GamsError)]),
+    "cmi_gams_info_get": (C.c_int, [_P, C.POINTER(GamsInfo)]),
+    "cmi_gams_var_name": (C.c_char_p, [_P, C.c_size_t]),
+    "cmi_gams_constraint_name": (C.c_char_p, [_P, C.c_size_t]),
+    "cmi_gams_warning": (C.c_char_p, [_P, C.c_size_t, C.POINTER(C.c_int)]),
+    "cmi_gams_initial_point": (C.c_int, [_P, _P, C.c_size_t]),
+    "cmi_gams_destroy": (None, [_P]),
+    "cmi_gams_result_line": (C.c_int, [C.c_int, C.c_double, C.c_double, C.c_char_p, C.c_size_t,
+                                       C.POINTER(C.c_size_t)]),
     "cmi_serve_plain": (C.c_int, [_P, C.c_size_t, C.POINTER(_P), _P, C.POINTER(SolveOptions),
                                   C.POINTER(SolveOutcome), _P, _P, C.POINTER(ServeStats)]),
 }

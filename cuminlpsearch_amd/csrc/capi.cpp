@@ -4,8 +4,10 @@
 
 #include "cmi.h"
 
+#include <algorithm>
 #include <chrono>
 #include <cstring>
+#include <functional>
 #include <memory>
 #include <new>
 #include <string>
@@ -13,6 +15,7 @@
 
 #include "batch.hpp"
 #include "jit.hpp"
+#include "cmi/gams.hpp"
 #include "cmi/model.hpp"
 #include "cmi/policy.hpp"
 #include "cmi/qcqp.hpp"
@@ -52,6 +55,8 @@ cmi_status guarded(F&& f)
     return fail(CMI_INVALID_PROBLEM, e.what());
   } catch (const cmi::DeviceError& e) {
     return fail(CMI_DEVICE_ERROR, e.what());
+  } catch (const cmi::gams::ParseError& e) {
+    return fail(CMI_PARSE_ERROR, e.what());
   } catch (const std::bad_alloc&) {
     return fail(CMI_ERROR, "host allocation failed");
   } catch (const std::exception& e) {
@@ -929,6 +934,158 @@ cmi_status cmi_solve_batch(cmi_ctx* ctx, size_t n, const cmi_problem* const* pro
                    best_points != nullptr ? best_points[i] : nullptr, ps[i]->box_bounds.size());
     }
   });
+}
+
+// -- GAMS scalar-format models ---------------------------------------------------
+
+void cmi_gams_options_default(cmi_gams_options* o)
+{
+  if (o == nullptr) {
+    return;
+  }
+  cmi::gams::ParseOptions const d;
+  o->default_bound = d.default_bound;
+  o->fold_fixed_to_const = d.fold_fixed_to_const ? 1 : 0;
+  o->reject_discrete = d.reject_discrete ? 1 : 0;
+}
+
+namespace
+{
+
+cmi_status gams_parse_into(const std::function<cmi::gams::ParsedModel(const cmi::gams::ParseOptions&)>& parse,
+                           const cmi_gams_options* options, cmi_problem** problem, cmi_gams_model** model,
+                           cmi_gams_error* err)
+{
+  if (problem == nullptr || model == nullptr) {
+    return fail(CMI_INVALID_HANDLE, "null argument");
+  }
+  cmi::gams::ParseOptions opt;
+  if (options != nullptr) {
+    opt.default_bound = options->default_bound;
+    opt.fold_fixed_to_const = options->fold_fixed_to_const != 0;
+    opt.reject_discrete = options->reject_discrete != 0;
+  }
+  if (err != nullptr) {
+    err->line = 0;
+    err->kind = CMI_GAMS_SYNTAX;
+  }
+  try {
+    cmi::gams::ParsedModel parsed = parse(opt);
+    auto p = std::make_unique<cmi_problem>();
+    auto m = std::make_unique<cmi_gams_model>();
+    p->problem = std::move(parsed.problem);
+    m->model = std::move(parsed);
+    *problem = p.release();
+    *model = m.release();
+    g_last_error.clear();
+    return CMI_OK;
+  } catch (const cmi::gams::ParseError& e) {
+    if (err != nullptr) {
+      err->line = e.line;
+      err->kind = static_cast<int>(e.kind);
+    }
+    return fail(CMI_PARSE_ERROR, e.what());
+  } catch (const std::bad_alloc&) {
+    return fail(CMI_ERROR, "host allocation failed");
+  } catch (const std::exception& e) {
+    return fail(CMI_ERROR, e.what());
+  }
+}
+
+}  // namespace
+
+cmi_status cmi_gams_parse(const char* text, size_t len, const cmi_gams_options* options, cmi_problem** problem,
+                          cmi_gams_model** model, cmi_gams_error* err)
+{
+  if (text == nullptr && len > 0) {
+    return fail(CMI_INVALID_HANDLE, "null text");
+  }
+  std::string_view const src(text == nullptr ? "" : text, len);
+  return gams_parse_into([src](const cmi::gams::ParseOptions& o) { return cmi::gams::parse(src, o); }, options,
+                         problem, model, err);
+}
+
+cmi_status cmi_gams_parse_file(const char* path, const cmi_gams_options* options, cmi_problem** problem,
+                               cmi_gams_model** model, cmi_gams_error* err)
+{
+  if (path == nullptr) {
+    return fail(CMI_INVALID_HANDLE, "null path");
+  }
+  std::string const file(path);
+  return gams_parse_into([&file](const cmi::gams::ParseOptions& o) { return cmi::gams::parse_file(file, o); },
+                         options, problem, model, err);
+}
+
+cmi_status cmi_gams_info_get(const cmi_gams_model* m, cmi_gams_info* out)
+{
+  if (m == nullptr || out == nullptr) {
+    return fail(CMI_INVALID_HANDLE, "null argument");
+  }
+  const cmi::gams::ParsedModel& pm = m->model;
+  out->sense = pm.sense == cmi::gams::Sense::Maximise ? CMI_SENSE_MAX : CMI_SENSE_MIN;
+  out->n_vars = pm.var_names.size();
+  out->n_constraints = pm.constraint_names.size();
+  out->n_warnings = pm.warnings.size();
+  out->objvar_kept = pm.objvar_kept ? 1 : 0;
+  out->has_initial_point = pm.initial_point.empty() ? 0 : 1;
+  return CMI_OK;
+}
+
+const char* cmi_gams_var_name(const cmi_gams_model* m, size_t i)
+{
+  return (m != nullptr && i < m->model.var_names.size()) ? m->model.var_names[i].c_str() : nullptr;
+}
+
+const char* cmi_gams_constraint_name(const cmi_gams_model* m, size_t i)
+{
+  return (m != nullptr && i < m->model.constraint_names.size()) ? m->model.constraint_names[i].c_str() : nullptr;
+}
+
+const char* cmi_gams_warning(const cmi_gams_model* m, size_t i, int* line)
+{
+  if (m == nullptr || i >= m->model.warnings.size()) {
+    return nullptr;
+  }
+  if (line != nullptr) {
+    *line = m->model.warnings[i].line;
+  }
+  return m->model.warnings[i].message.c_str();
+}
+
+cmi_status cmi_gams_initial_point(const cmi_gams_model* m, double* out, size_t n)
+{
+  if (m == nullptr || out == nullptr) {
+    return fail(CMI_INVALID_HANDLE, "null argument");
+  }
+  const std::vector<double>& x = m->model.initial_point;
+  if (x.empty()) {
+    return fail(CMI_INVALID_CONFIG, "the model gave no initial levels");
+  }
+  if (n != x.size()) {
+    return fail(CMI_SHAPE_MISMATCH, "initial point length differs from the number of variables");
+  }
+  std::copy(x.begin(), x.end(), out);
+  return CMI_OK;
+}
+
+void cmi_gams_destroy(cmi_gams_model* m) { delete m; }
+
+cmi_status cmi_gams_result_line(int sense, double lower, double upper, char* buf, size_t cap, size_t* len)
+{
+  if (sense != CMI_SENSE_MIN && sense != CMI_SENSE_MAX) {
+    return fail(CMI_INVALID_CONFIG, "sense must be CMI_SENSE_MIN or CMI_SENSE_MAX");
+  }
+  std::string const line = cmi::gams::result_line(
+      sense == CMI_SENSE_MAX ? cmi::gams::Sense::Maximise : cmi::gams::Sense::Minimise, lower, upper);
+  if (len != nullptr) {
+    *len = line.size();
+  }
+  if (buf != nullptr && cap > 0) {
+    std::size_t const k = std::min(cap - 1, line.size());
+    std::memcpy(buf, line.data(), k);
+    buf[k] = '\0';
+  }
+  return CMI_OK;
 }
 
 }  // extern "C"

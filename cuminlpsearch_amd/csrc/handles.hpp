@@ -3,6 +3,7 @@
 #include <memory>
 
 #include "cmi.h"
+#include "cmi/gams.hpp"
 #include "cmi/model.hpp"
 #include "cmi/region.hpp"
 #include "cmi/search.hpp"
@@ -14,6 +15,11 @@
 struct cmi_problem
 {
   cmi::model::Problem<double> problem;
+};
+
+struct cmi_gams_model
+{
+  cmi::gams::ParsedModel model;  // its problem has been moved into a cmi_problem
 };
 
 struct cmi_ctx

@@ -193,34 +193,58 @@ class ProblemArrays:
         return d
 
 
+_UNARY = {
+    _c.OP_EXP: np.exp,
+    _c.OP_LOG: np.log,
+    _c.OP_SQRT: np.sqrt,
+    _c.OP_SIN: np.sin,
+    _c.OP_COS: np.cos,
+    _c.OP_TANH: np.tanh,
+    _c.OP_ABS: np.fabs,
+}
+_BINARY = {
+    _c.OP_ADD: np.add,
+    _c.OP_SUB: np.subtract,
+    _c.OP_MUL: np.multiply,
+    _c.OP_DIV: np.divide,
+    _c.OP_MIN: np.fmin,
+    _c.OP_MAX: np.fmax,
+    _c.OP_POW: np.power,
+}
+
+
 def evaluate_point(a: ProblemArrays, x) -> np.ndarray:
-    """Round-to-nearest value of every node at point x (reference model/eval.hpp:38-116)."""
+    """Round-to-nearest value of every node at point x, with C semantics for
+    out-of-domain arguments (log(0) = -inf, sqrt(-1) = nan); a value, not a
+    bound (reference model/eval.hpp:38-116)."""
     n = a.op.shape[0]
     val = np.zeros(n)
-    for i in range(n):
-        op = int(a.op[i])
-        if op == _c.OP_VAR:
-            val[i] = x[int(a.var_index[i])]
-        elif op == _c.OP_CONST:
-            val[i] = a.constant[i]
-        else:
-            u = val[int(a.in0[i])]
-            w = val[int(a.in1[i])] if op in (_c.OP_ADD, _c.OP_SUB, _c.OP_MUL, _c.OP_DIV) else 0.0
-            if op == _c.OP_ADD:
-                val[i] = u + w
-            elif op == _c.OP_SUB:
-                val[i] = u - w
-            elif op == _c.OP_MUL:
-                val[i] = u * w
-            elif op == _c.OP_DIV:
-                val[i] = u / w
+    with np.errstate(all="ignore"):
+        for i in range(n):
+            op = int(a.op[i])
+            if op == _c.OP_VAR:
+                val[i] = x[int(a.var_index[i])]
+            elif op == _c.OP_CONST:
+                val[i] = a.constant[i]
+            elif op in _BINARY:
+                val[i] = _BINARY[op](val[int(a.in0[i])], val[int(a.in1[i])])
+            elif op in _UNARY:
+                val[i] = _UNARY[op](val[int(a.in0[i])])
             elif op == _c.OP_SQR:
+                u = val[int(a.in0[i])]
                 val[i] = u * u
             elif op == _c.OP_NEG:
-                val[i] = -u
+                val[i] = -val[int(a.in0[i])]
+            elif op == _c.OP_POWN:
+                val[i] = np.power(val[int(a.in0[i])], float(a.int_exp[i]))
             else:
-                raise ValueError(f"evaluate_point: op {op} not supported here")
+                raise ValueError(f"evaluate_point: unknown op {op}")
     return val
+
+
+def evaluate_objective(a: ProblemArrays, x) -> float:
+    """Value of the objective at x (reference model/eval.hpp:165-171)."""
+    return float(evaluate_point(a, x)[int(a.objective_root)])
 
 
 def satisfies(lhs: float, cmp: int, rhs: float) -> bool:
@@ -230,4 +254,4 @@ def satisfies(lhs: float, cmp: int, rhs: float) -> bool:
     return diff <= 1e-8 or diff <= 1e-6 * max(abs(lhs), abs(rhs))
 
 
-__all__ = ["Expr", "Problem", "ProblemArrays", "sqr", "evaluate_point", "satisfies", "math"]
+__all__ = ["Expr", "Problem", "ProblemArrays", "sqr", "evaluate_point", "evaluate_objective", "satisfies", "math"]

@@ -44,7 +44,8 @@ typedef enum
   CMI_INVALID_HANDLE = 5,   /* null or destroyed handle (moved-from role) */
   CMI_INVALID_DAG = 6,      /* cuminlp::InvalidDAG */
   CMI_INVALID_PROBLEM = 7,  /* cuminlp::InvalidProblem */
-  CMI_ERROR = 8             /* any other failure */
+  CMI_ERROR = 8,            /* any other failure */
+  CMI_PARSE_ERROR = 9       /* gams::ParseError */
 } cmi_status;
 
 /* model::Op, reference model/dag.hpp:12-33 (same order) */
@@ -426,6 +427,83 @@ cmi_status cmi_aggregate_build(cmi_tape* t, const uint8_t* slot_kinds, size_t sl
  * n_batch * k bounds to `out`, region-major. */
 cmi_status cmi_bound_children(cmi_role* r, size_t n_batch, const cmi_ivl* boxes, size_t n_vars, const size_t* var_ids,
                               const size_t* widths, size_t slot_count, cmi_aggregate_bound* out);
+
+/* -- GAMS scalar-format models (host) ----------------------------------------- */
+
+typedef struct cmi_gams_model cmi_gams_model;
+
+/* gams::ParseOptions, reference include/cuminlp/gams.hpp:21-50 */
+typedef struct
+{
+  double default_bound;    /* box half-width for a variable with no finite bound (1e6) */
+  int fold_fixed_to_const; /* x.fx variables become literals (1) */
+  int reject_discrete;     /* refuse Binary/Integer declarations (0) */
+} cmi_gams_options;
+
+/* gams::ErrorKind, reference gams.hpp:59-70 (same order) */
+enum
+{
+  CMI_GAMS_SYNTAX = 0,
+  CMI_GAMS_NON_SCALAR,
+  CMI_GAMS_DISCRETE,
+  CMI_GAMS_UNSUPPORTED_FUNCTION,
+  CMI_GAMS_UNREPRESENTABLE,
+  CMI_GAMS_IO
+};
+
+/* gams::Sense, reference gams.hpp:14-19 */
+enum
+{
+  CMI_SENSE_MIN = 0,
+  CMI_SENSE_MAX = 1
+};
+
+/* ParseError's line and kind, reference gams.hpp:72-83 */
+typedef struct
+{
+  int line;
+  int kind;
+} cmi_gams_error;
+
+/* ParsedModel's metadata, reference gams.hpp:85-116 */
+typedef struct
+{
+  int sense;               /* CMI_SENSE_*; under MAX the objective is already negated */
+  size_t n_vars;
+  size_t n_constraints;
+  size_t n_warnings;
+  int objvar_kept;         /* the objective variable stayed a search dimension */
+  int has_initial_point;   /* the file gave x.l levels */
+} cmi_gams_info;
+
+void cmi_gams_options_default(cmi_gams_options* o);
+/* gams::parse / gams::parse_file, reference gams.hpp:118-126 and
+ * source/gams/frontend.cpp:1592-1681. On CMI_OK, *problem is a new validated
+ * problem (release with cmi_problem_destroy) and *model its names, sense,
+ * warnings and initial point (release with cmi_gams_destroy). On
+ * CMI_PARSE_ERROR nothing is allocated, `err` (may be NULL) receives the line
+ * (0 = whole model) and kind, and cmi_last_error() the message. `options` may
+ * be NULL for the defaults. */
+cmi_status cmi_gams_parse(const char* text, size_t len, const cmi_gams_options* options, cmi_problem** problem,
+                          cmi_gams_model** model, cmi_gams_error* err);
+cmi_status cmi_gams_parse_file(const char* path, const cmi_gams_options* options, cmi_problem** problem,
+                               cmi_gams_model** model, cmi_gams_error* err);
+cmi_status cmi_gams_info_get(const cmi_gams_model* m, cmi_gams_info* out);
+/* ParsedModel::var_names[i] / constraint_names[i] / warnings[i]; NULL when i
+ * is out of range. The strings live until cmi_gams_destroy. */
+const char* cmi_gams_var_name(const cmi_gams_model* m, size_t i);
+const char* cmi_gams_constraint_name(const cmi_gams_model* m, size_t i);
+const char* cmi_gams_warning(const cmi_gams_model* m, size_t i, int* line);
+/* ParsedModel::initial_point (n = n_vars values); CMI_INVALID_CONFIG if the
+ * file gave no levels. */
+cmi_status cmi_gams_initial_point(const cmi_gams_model* m, double* out, size_t n);
+void cmi_gams_destroy(cmi_gams_model* m);
+/* The solver's RESULT line, reference source/gams/solve.cu:836-867:
+ * "RESULT\tsense=min|max\tprimal=<v|none>\tdual=<v|none>" (%.17g) from the
+ * minimisation bounds a solve reports (cmi_solve_outcome lower_bound /
+ * upper_bound), negated back under CMI_SENSE_MAX. Writes at most cap bytes
+ * including the terminating NUL; *len (may be NULL) gets the full length. */
+cmi_status cmi_gams_result_line(int sense, double lower, double upper, char* buf, size_t cap, size_t* len);
 
 /* -- search ------------------------------------------------------------------- */
 

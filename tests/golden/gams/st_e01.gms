@@ -1,0 +1,43 @@
+$offlisting
+*  
+*  Equation counts
+*      Total        E        G        L        N        X        C        B
+*          2        1        0        1        0        0        0        0
+*  
+*  Variable counts
+*                   x        b        i      s1s      s2s       sc       si
+*      Total     cont   binary  integer     sos1     sos2    scont     sint
+*          3        3        0        0        0        0        0        0
+*  FX      0
+*  
+*  Nonzero counts
+*      Total    const       NL      DLL
+*          5        3        2        0
+*
+*  Solve m using NLP minimizing objvar;
+
+
+Variables  x1,x2,objvar;
+
+Positive Variables  x1,x2;
+
+Equations  e1,e2;
+
+
+e1.. x1*x2 =L= 4;
+
+e2..    x1 + x2 + objvar =E= 0;
+
+* set non-default bounds
+x1.up = 6;
+x2.up = 4;
+
+Model m / all /;
+
+m.limrow=0; m.limcol=0;
+m.tolproj=0.0;
+
+$if NOT '%gams.u1%' == '' $include '%gams.u1%'
+
+$if not set NLP $set NLP NLP
+Solve m using %NLP% minimizing objvar;
