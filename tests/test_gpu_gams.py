@@ -1,0 +1,128 @@
+"""MINLPLib models read from their .gms files and solved on the device.
+
+Each model the device op set covers is solved on the device and by the
+oracle's restated search with the same settings: the iterates (counts,
+bracket, incumbent) must agree bit for bit, and the bracket must be consistent
+with MINLPLib's published bounds (tests/golden/gams_corpus.json). The small
+ones are solved to the published optimum; the whole set also runs as one
+batch of uneven instances. Reference: source/gams/solve.cu:82-122 (parse,
+solve), :836-867 (RESULT line)."""
+from __future__ import annotations
+
+import json
+import pathlib
+
+import numpy as np
+import pytest
+
+import cuminlpsearch_amd as cmi
+from cuminlpsearch_amd import gams
+from oracle import binding as orc
+
+pytestmark = pytest.mark.gpu
+
+GOLDEN = pathlib.Path(__file__).resolve().parent / "golden"
+TABLE = json.loads((GOLDEN / "gams_corpus.json").read_text())["table"]
+
+# models whose DAGs use only Add/Sub/Mul/Sqr/Neg
+QCQP_MODELS = ["ex2_1_1", "st_e01", "nvs04", "autocorr_bern20-03", "alkyl", "house", "ex3_1_1", "ex5_2_2_case1",
+               "st_bpaf1a", "ex14_1_2", "circle"]
+OPTS = dict(partition_num=4, enumerate_cap=4, max_cycle_size=2, sample_points=32, iter_limit=300)
+
+FIELDS = ("iterations", "fathomed", "bounded", "sampled", "enqueued", "pruned_infeasible", "pruned_dominated",
+          "pending", "stop_reason", "proven_optimal", "infeasible", "found_incumbent")
+
+
+def same_outcome(d, o):
+    for f in FIELDS:
+        assert getattr(d, f) == getattr(o, f), f
+    assert np.float64(d.lower_bound).tobytes() == np.float64(o.lower_bound).tobytes()
+    assert np.float64(d.upper_bound).tobytes() == np.float64(o.upper_bound).tobytes()
+
+
+def parsed(name):
+    return gams.parse_file(GOLDEN / "gams" / f"{name}.gms")
+
+
+def consistent_with_published(name, out, tol=1e-6):
+    """lower <= optimum <= upper, with the optimum bracketed by MINLPLib's
+    published primal/dual bounds (all these models minimise)."""
+    row = TABLE.get(name)
+    if row is None:
+        return
+    if row["ref_primal"] is not None:
+        assert out.lower_bound <= row["ref_primal"] + tol * max(1.0, abs(row["ref_primal"]))
+    if out.found_incumbent and row["ref_dual"] is not None:
+        assert out.upper_bound >= row["ref_dual"] - tol * max(1.0, abs(row["ref_dual"]))
+
+
+@pytest.mark.parametrize("name", QCQP_MODELS)
+def test_solve_matches_oracle(ctx, name):
+    pm = parsed(name)
+    print(f"solving {name}", flush=True)
+    d, best_d = cmi.solve(ctx, pm.problem, **OPTS)
+    o, best_o = orc.solve(pm.problem.arrays(), **OPTS)
+    same_outcome(d, o)
+    assert np.array_equal(best_d.view(np.uint64), best_o.view(np.uint64))
+    consistent_with_published(name, d)
+
+
+@pytest.mark.parametrize("name,opts,optimum", [
+    ("nvs04", OPTS, 0.72),
+    ("st_e01", OPTS, -6.666666667),
+    ("circle", dict(partition_num=4, enumerate_cap=4, max_cycle_size=3, sample_points=32, iter_limit=2000), 4.57424899),
+    # all 20 binaries enumerated in one launch of 2^20 regions
+    ("autocorr_bern20-03", dict(partition_num=2, enumerate_cap=2, max_cycle_size=20, sample_points=1, iter_limit=10),
+     -72.0),
+])
+def test_solved_to_the_published_optimum(ctx, name, opts, optimum):
+    pm = parsed(name)
+    print(f"solving {name}", flush=True)
+    d, best = cmi.solve(ctx, pm.problem, **opts)
+    assert d.proven_optimal or d.stop_reason == 0
+    assert abs(d.upper_bound - optimum) <= 1e-6 * max(1.0, abs(optimum))
+    assert d.lower_bound <= d.upper_bound
+    # the incumbent is a point of the model with that objective value
+    from cuminlpsearch_amd.model import evaluate_objective
+    assert abs(evaluate_objective(pm.problem.arrays(), best) - d.upper_bound) <= 1e-9 * max(1.0, abs(optimum))
+    line = gams.result_line(pm.sense, d.lower_bound, d.upper_bound)
+    assert line.startswith("RESULT\tsense=min\tprimal=")
+    assert float(line.split("primal=")[1].split("\t")[0]) == d.upper_bound
+
+
+def test_ex2_1_1_brackets_its_optimum(ctx):
+    opts = dict(partition_num=2, enumerate_cap=2, max_cycle_size=5, sample_points=64, iter_limit=4000)
+    pm = parsed("ex2_1_1")
+    d, _ = cmi.solve(ctx, pm.problem, **opts)
+    o, _ = orc.solve(pm.problem.arrays(), **opts)
+    same_outcome(d, o)
+    assert d.lower_bound <= -17.0 <= d.upper_bound
+    assert d.upper_bound - d.lower_bound < 0.5
+
+
+def test_corpus_batch_equals_plain(ctx):
+    """The whole set as one batch of uneven instances (2 to 20 variables, 0 to
+    10 constraints)."""
+    models = [parsed(n).problem for n in QCQP_MODELS]
+    outs, bests = cmi.solve_batch(ctx, models, **OPTS)
+    for name, p, d, b in zip(QCQP_MODELS, models, outs, bests):
+        s, sb = cmi.solve(ctx, p, **OPTS)
+        same_outcome(d, s)
+        assert np.array_equal(b.view(np.uint64), sb.view(np.uint64)), name
+
+
+def test_maximise_result_line_reports_the_files_sense(ctx):
+    src = ("Variables x1,x2,objvar;\nEquations e1,e2;\n"
+           "e1.. objvar =E= x1*x2 + x1;\ne2.. x1 + x2 =L= 3;\n"
+           "x1.lo = 0; x1.up = 2; x2.lo = 0; x2.up = 2;\nSolve m using NLP maximizing objvar;\n")
+    pm = gams.parse(src)
+    d, best = cmi.solve(ctx, pm.problem, partition_num=4, enumerate_cap=4, max_cycle_size=2, sample_points=64,
+                        iter_limit=3000, tolerance=1e-7)
+    # max x1*x2 + x1 on x1 + x2 <= 3 is 4 at (2, 1)
+    assert -d.upper_bound <= 4.0 + 1e-9 and -d.lower_bound >= 4.0 - 1e-9
+    line = gams.result_line(pm.sense, d.lower_bound, d.upper_bound)
+    primal = float(line.split("primal=")[1].split("\t")[0])
+    dual = float(line.split("dual=")[1])
+    assert line.startswith("RESULT\tsense=max\t")
+    assert primal <= 4.0 + 1e-9 <= dual + 2e-9
+    assert abs(primal - 4.0) < 1e-3
