@@ -52,6 +52,19 @@ def lib() -> C.CDLL:
         h.orc_aggregate.argtypes = [P, P, P, P, P, C.c_size_t, C.c_size_t, P, P, P, P, P, P]
         h.orc_candidate.restype = C.c_int
         h.orc_candidate.argtypes = [P, P, P, P, P, C.c_size_t, C.c_size_t, C.c_uint64, P, P, P, P]
+        for name in ("orc_fma_ru", "orc_fma_rd"):
+            getattr(h, name).restype = d
+            getattr(h, name).argtypes = [d, d, d]
+        for name in ("orc_div_ru", "orc_div_rd"):
+            getattr(h, name).restype = d
+            getattr(h, name).argtypes = [d, d]
+        for name in ("orc_sqrt_ru", "orc_sqrt_rd"):
+            getattr(h, name).restype = d
+            getattr(h, name).argtypes = [d]
+        h.orc_iv_ext.restype = C.c_int
+        h.orc_iv_ext.argtypes = [C.c_int, P, P, P, P, C.c_int, C.c_size_t, P, P, P]
+        h.orc_pt_ext.restype = C.c_int
+        h.orc_pt_ext.argtypes = [C.c_int, P, P, C.c_int, C.c_size_t, P]
         h.orc_solve.restype = C.c_int
         h.orc_solve.argtypes = [P, C.c_size_t, C.c_size_t, C.c_size_t, C.c_size_t, C.c_uint32, d, P, P]
         _lib = h
@@ -178,3 +191,30 @@ def aggregate(arrays, box, var_ids, widths, kinds, k, per_region=False):
     if per_region:
         return lb, ub, fe, (rf, rl, ru)
     return lb, ub, fe
+
+
+def iv_ext(op: int, a_lb, a_ub, b_lb=None, b_ub=None, n_exp: int = 0):
+    """Extended interval op n at a time: (lb, ub, ok) arrays."""
+    a_lb = np.ascontiguousarray(a_lb, np.float64)
+    a_ub = np.ascontiguousarray(a_ub, np.float64)
+    n = a_lb.shape[0]
+    if b_lb is not None:
+        b_lb = np.ascontiguousarray(b_lb, np.float64)
+        b_ub = np.ascontiguousarray(b_ub, np.float64)
+    lo, hi, ok = np.zeros(n), np.zeros(n), np.zeros(n, np.uint8)
+    rc = lib().orc_iv_ext(op, _p(a_lb), _p(a_ub), _p(b_lb), _p(b_ub), int(n_exp), n, _p(lo), _p(hi), _p(ok))
+    if rc != 0:
+        raise ValueError(f"oracle: op {op} has no extended interval form")
+    return lo, hi, ok.astype(bool)
+
+
+def pt_ext(op: int, a, b=None, n_exp: int = 0) -> np.ndarray:
+    """Extended point op (round to nearest), n at a time."""
+    a = np.ascontiguousarray(a, np.float64)
+    if b is not None:
+        b = np.ascontiguousarray(b, np.float64)
+    out = np.zeros(a.shape[0])
+    rc = lib().orc_pt_ext(op, _p(a), _p(b), int(n_exp), a.shape[0], _p(out))
+    if rc != 0:
+        raise ValueError(f"oracle: op {op} has no extended point form")
+    return out

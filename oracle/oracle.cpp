@@ -161,10 +161,12 @@ Iv iv_mul(Iv x, Iv y)
 {
   double const l[4] = {mul_rd(x.lb, y.lb), mul_rd(x.lb, y.ub), mul_rd(x.ub, y.lb), mul_rd(x.ub, y.ub)};
   double const u[4] = {mul_ru(x.lb, y.lb), mul_ru(x.lb, y.ub), mul_ru(x.ub, y.lb), mul_ru(x.ub, y.ub)};
+  // fmin/fmax skip a NaN product (0 * inf, whose IEEE-1788 value is 0 and
+  // never the extreme one unless all four are NaN)
   double lo = l[0], hi = u[0];
   for (int k = 1; k < 4; ++k) {
-    lo = l[k] < lo ? l[k] : lo;
-    hi = u[k] > hi ? u[k] : hi;
+    lo = std::fmin(lo, l[k]);
+    hi = std::fmax(hi, u[k]);
   }
   return {plus_zero(lo), plus_zero(hi)};
 }
@@ -230,6 +232,558 @@ Iv iv_sqr(Iv a)
 }
 
 // ---------------------------------------------------------------------------
+// Extended op set (Div, PowN, Sqrt, Exp, Log, Sin, Cos, Tanh, Abs, Min, Max;
+// Pow as exp(y log x)). The reference takes these from cuinterval, which is
+// not in the snapshot; the enclosures restated here are the engine's own:
+// division and square root correctly rounded in each direction (MXCSR divsd
+// and sqrtsd), the transcendental functions by argument reduction with
+// directed error terms and directed Horner schemes plus a tail bound, on the
+// constants of enclosure_constants.inc. Pinned by mpmath containment and
+// tightness checks (tests/test_oracle_extended.py). Empty results (division by
+// [0, 0], sqrt or log below the domain) clear `ok`.
+// ---------------------------------------------------------------------------
+
+#define CMI_CONST static constexpr
+#include "enclosure_constants.inc"
+
+__attribute__((noinline)) double fma_rd(double a, double b, double c)
+{
+  RoundGuard g(_MM_ROUND_DOWN);
+  volatile double va = a, vb = b, vc = c;
+  volatile double r = std::fma(va, vb, vc);
+  return r;
+}
+__attribute__((noinline)) double fma_ru(double a, double b, double c)
+{
+  RoundGuard g(_MM_ROUND_UP);
+  volatile double va = a, vb = b, vc = c;
+  volatile double r = std::fma(va, vb, vc);
+  return r;
+}
+__attribute__((noinline)) double div_rd(double a, double b)
+{
+  RoundGuard g(_MM_ROUND_DOWN);
+  volatile double va = a, vb = b;
+  volatile double r = va / vb;
+  return r;
+}
+__attribute__((noinline)) double div_ru(double a, double b)
+{
+  RoundGuard g(_MM_ROUND_UP);
+  volatile double va = a, vb = b;
+  volatile double r = va / vb;
+  return r;
+}
+__attribute__((noinline)) double sqrt_rd(double a)
+{
+  RoundGuard g(_MM_ROUND_DOWN);
+  volatile double va = a;
+  volatile double r = std::sqrt(va);
+  return r;
+}
+__attribute__((noinline)) double sqrt_ru(double a)
+{
+  RoundGuard g(_MM_ROUND_UP);
+  volatile double va = a;
+  volatile double r = std::sqrt(va);
+  return r;
+}
+
+constexpr double kInfO = std::numeric_limits<double>::infinity();
+
+double pow2_exact(int k)  // 2^k, -1074 <= k <= 1023
+{
+  return std::ldexp(1.0, k);
+}
+
+// x * 2^k in the given direction, |x| in [1/4, 4]: one exact scaling that
+// keeps the value normal, then one rounded one.
+double scale_dir(double x, int k, bool up)
+{
+  k = k < -2100 ? -2100 : (k > 1030 ? 1030 : k);
+  int const k1 = k < -1018 ? -1018 : (k > 1018 ? 1018 : k);
+  int k2 = k - k1;
+  k2 = k2 < -1074 ? -1074 : k2;
+  double const y = x * pow2_exact(k1);  // exact
+  return up ? mul_ru(y, pow2_exact(k2)) : mul_rd(y, pow2_exact(k2));
+}
+
+struct DivOut
+{
+  Iv v;
+  bool ok;
+};
+
+// x / d, d = [dl, du] with dl > 0
+Iv div_by_positive(Iv x, double dl, double du)
+{
+  return {div_rd(x.lb, x.lb <= 0.0 ? dl : du), div_ru(x.ub, x.ub >= 0.0 ? dl : du)};
+}
+
+DivOut iv_div(Iv x, Iv y)
+{
+  if (y.lb > 0.0) {
+    return {div_by_positive(x, y.lb, y.ub), true};
+  }
+  if (y.ub < 0.0) {
+    Iv const z = div_by_positive(x, -y.ub, -y.lb);
+    return {{-z.ub, -z.lb}, true};
+  }
+  bool const y0l = y.lb == 0.0, y0u = y.ub == 0.0;
+  if (y0l && y0u) {
+    return {{0.0, 0.0}, false};
+  }
+  if (x.lb == 0.0 && x.ub == 0.0) {
+    return {{0.0, 0.0}, true};
+  }
+  if (x.lb > 0.0) {
+    if (y0l) {
+      return {{div_rd(x.lb, y.ub), kInfO}, true};
+    }
+    if (y0u) {
+      return {{-kInfO, div_ru(x.lb, y.lb)}, true};
+    }
+  } else if (x.ub < 0.0) {
+    if (y0l) {
+      return {{-kInfO, div_ru(x.ub, y.ub)}, true};
+    }
+    if (y0u) {
+      return {{div_rd(x.ub, y.lb), kInfO}, true};
+    }
+  }
+  return {{-kInfO, kInfO}, true};
+}
+
+DivOut iv_sqrt(Iv x)
+{
+  if (!(x.ub >= 0.0)) {
+    return {{0.0, 0.0}, false};
+  }
+  double const ub = (x.ub > 0.0 && x.ub < kInfO) ? sqrt_ru(x.ub) : x.ub;
+  double const lb = x.lb > 0.0 ? sqrt_rd(x.lb) : 0.0;
+  return {{lb, ub}, true};
+}
+
+Iv iv_abs(Iv x)
+{
+  if (x.lb >= 0.0) {
+    return x;
+  }
+  if (x.ub <= 0.0) {
+    return {-x.ub, -x.lb};
+  }
+  return {0.0, std::fmax(-x.lb, x.ub)};
+}
+
+Iv iv_min(Iv x, Iv y) { return {std::fmin(x.lb, y.lb), std::fmin(x.ub, y.ub)}; }
+Iv iv_max(Iv x, Iv y) { return {std::fmax(x.lb, y.lb), std::fmax(x.ub, y.ub)}; }
+
+// m^n (m >= 0, n >= 1) by binary powering from the low bit, every product
+// rounded in one direction
+double pow_dir(double m, unsigned n, bool up)
+{
+  double u = m, acc = 0.0;
+  bool have = false;
+  for (;;) {
+    if (n & 1u) {
+      acc = have ? (up ? mul_ru(acc, u) : mul_rd(acc, u)) : u;
+      have = true;
+    }
+    n >>= 1;
+    if (n == 0u) {
+      break;
+    }
+    u = up ? mul_ru(u, u) : mul_rd(u, u);
+  }
+  return acc;
+}
+
+DivOut iv_pown(Iv x, int n)
+{
+  if (n == 0) {
+    return {{1.0, 1.0}, true};
+  }
+  if (n == 1) {
+    return {x, true};
+  }
+  unsigned const m = static_cast<unsigned>(n < 0 ? -n : n);
+  Iv r;
+  if (m & 1u) {
+    r.lb = x.lb >= 0.0 ? pow_dir(x.lb, m, false) : -pow_dir(-x.lb, m, true);
+    r.ub = x.ub >= 0.0 ? pow_dir(x.ub, m, true) : -pow_dir(-x.ub, m, false);
+  } else {
+    double const near = x.lb > 0.0 ? x.lb : (x.ub < 0.0 ? -x.ub : 0.0);
+    r = {pow_dir(near, m, false), pow_dir(std::fmax(std::fabs(x.lb), std::fabs(x.ub)), m, true)};
+  }
+  if (n < 0) {
+    return iv_div({1.0, 1.0}, r);
+  }
+  return {r, true};
+}
+
+// e^x bounded above / below
+double exp_hi(double x)
+{
+  if (x != x) {
+    return x;
+  }
+  if (x >= 709.8) {
+    return kInfO;
+  }
+  x = x < -1100.0 ? -1100.0 : x;
+  double const kd = std::floor(mul_ru(x, kInvLn2));
+  double r = fma_ru(-kd, kLn2Hi, x);
+  r = fma_ru(-kd, kLn2Lo, r);
+  r = fma_ru(std::fabs(kd), kLn2Err, r);
+  r = r > 0.0 ? r : 0.0;
+  double p = kExpUp[17];
+  for (int j = 16; j >= 0; --j) {
+    p = fma_ru(p, r, kExpUp[j]);
+  }
+  p = add_ru(p, kExpTail);
+  return scale_dir(p, static_cast<int>(kd), true);
+}
+
+double exp_lo(double x)
+{
+  if (x != x) {
+    return x;
+  }
+  if (x <= -1100.0) {
+    return 0.0;
+  }
+  x = x > 710.0 ? 710.0 : x;
+  double const kd = std::floor(mul_ru(x, kInvLn2));
+  double r = fma_rd(-kd, kLn2Hi, x);
+  r = fma_rd(-kd, kLn2Lo, r);
+  r = fma_rd(-std::fabs(kd), kLn2Err, r);
+  double p;
+  if (r >= 0.0) {
+    p = kExpDn[17];
+    for (int j = 16; j >= 0; --j) {
+      p = fma_rd(p, r, kExpDn[j]);
+    }
+  } else {
+    p = add_rd(1.0, r);  // e^r >= 1 + r
+  }
+  return scale_dir(p, static_cast<int>(kd), false);
+}
+
+Iv iv_exp(Iv x) { return {exp_lo(x.lb), exp_hi(x.ub)}; }
+
+// atanh(u) = u P(u^2), P(w) = sum w^j/(2j+1): upper bound for |u| <= 0.1716
+double atanh_hi(double u)
+{
+  double const wu = mul_ru(u, u), wd = mul_rd(u, u);
+  double p = kAtanhUp[11], q = kAtanhDn[11];
+  for (int j = 10; j >= 0; --j) {
+    p = fma_ru(p, wu, kAtanhUp[j]);
+    q = fma_rd(q, wd, kAtanhDn[j]);
+  }
+  p = add_ru(p, kAtanhTail);
+  return u >= 0.0 ? mul_ru(u, p) : mul_ru(u, q);
+}
+
+// v = m 2^e with m in [1/sqrt2, sqrt2)
+void log_split(double v, double& m, int& e)
+{
+  m = std::frexp(v, &e);
+  if (m < 0.70710678118654752) {
+    m = m + m;
+    e -= 1;
+  }
+}
+
+double log_hi(double v)
+{
+  double m;
+  int e;
+  log_split(v, m, e);
+  double const num = m - 1.0;  // exact
+  double const s = div_ru(num, num >= 0.0 ? add_rd(m, 1.0) : add_ru(m, 1.0));
+  double const a = atanh_hi(s);
+  double const ed = static_cast<double>(e);
+  double const el = ed >= 0.0 ? mul_ru(ed, kLn2Up) : mul_ru(ed, kLn2Dn);
+  return fma_ru(a, 2.0, el);
+}
+
+double log_lo(double v)
+{
+  double m;
+  int e;
+  log_split(v, m, e);
+  double const num = m - 1.0;
+  double const ns = div_ru(-num, num >= 0.0 ? add_ru(m, 1.0) : add_rd(m, 1.0));  // -s bounded above
+  double const na = atanh_hi(ns);
+  double const ed = static_cast<double>(e);
+  double const el = ed >= 0.0 ? mul_ru(-ed, kLn2Dn) : mul_ru(-ed, kLn2Up);
+  return -fma_ru(na, 2.0, el);
+}
+
+DivOut iv_log(Iv x)
+{
+  if (!(x.ub > 0.0)) {
+    return {{0.0, 0.0}, false};
+  }
+  double const ub = x.ub == kInfO ? kInfO : log_hi(x.ub);
+  double const lb = x.lb > 0.0 ? log_lo(x.lb) : -kInfO;
+  return {{lb, ub}, true};
+}
+
+struct TrigEnd
+{
+  double lo, hi;
+  int k;
+  bool r_lo_nonpos, r_hi_nonneg;
+};
+
+// sin(x + shift*pi/2) enclosed at one endpoint x = k pi/2 + r
+TrigEnd trig_end(double x, int shift)
+{
+  double const kd = std::nearbyint(mul_ru(x, kTwoOverPi));
+  double const ak = std::fabs(kd);
+  double const t0 = fma_ru(-kd, kPio2A, x);  // exact
+  double rh = fma_ru(-kd, kPio2B, t0);
+  rh = fma_ru(-kd, kPio2C, rh);
+  rh = fma_ru(ak, kPio2Err, rh);
+  double rl = fma_rd(-kd, kPio2B, t0);
+  rl = fma_rd(-kd, kPio2C, rl);
+  rl = fma_rd(-ak, kPio2Err, rl);
+  double const big = std::fmax(std::fabs(rl), std::fabs(rh));
+  double const wh = mul_ru(big, big);
+  double const near = rl > 0.0 ? rl : (rh < 0.0 ? -rh : 0.0);
+  double const wl = mul_rd(near, near);
+  // partial sums alternate in sign with the coefficients
+  double sl = kSinDn[9], sh = kSinUp[9];
+  for (int j = 8; j >= 0; --j) {
+    bool const next_pos = ((j + 1) & 1) == 0;
+    sl = fma_rd(next_pos ? wl : wh, sl, kSinDn[j]);
+    sh = fma_ru(next_pos ? wh : wl, sh, kSinUp[j]);
+  }
+  double cl = kCosDn[10], ch = kCosUp[10];
+  for (int j = 9; j >= 0; --j) {
+    bool const next_pos = ((j + 1) & 1) == 0;
+    cl = fma_rd(next_pos ? wl : wh, cl, kCosDn[j]);
+    ch = fma_ru(next_pos ? wh : wl, ch, kCosUp[j]);
+  }
+  sl = sub_rd(sl, kSinTail);
+  sh = add_ru(sh, kSinTail);
+  cl = sub_rd(cl, kCosTail);
+  ch = add_ru(ch, kCosTail);
+  double const sin_hi = rh >= 0.0 ? mul_ru(rh, sh) : mul_ru(rh, sl);
+  double const sin_lo = rl >= 0.0 ? mul_rd(rl, sl) : mul_rd(rl, sh);
+  int const k = static_cast<int>(kd);
+  int const q = (k + shift) & 3;
+  TrigEnd out;
+  switch (q) {
+    case 0: out.lo = sin_lo; out.hi = sin_hi; break;
+    case 1: out.lo = cl; out.hi = ch; break;
+    case 2: out.lo = -sin_hi; out.hi = -sin_lo; break;
+    default: out.lo = -ch; out.hi = -cl; break;
+  }
+  out.k = k + shift;
+  out.r_lo_nonpos = rl <= 0.0;
+  out.r_hi_nonneg = rh >= 0.0;
+  return out;
+}
+
+Iv iv_sincos(Iv x, int shift)
+{
+  double const a = x.lb, b = x.ub;
+  if (!(std::fabs(a) <= 1048576.0) || !(std::fabs(b) <= 1048576.0) || sub_ru(b, a) >= 6.28) {
+    return {-1.0, 1.0};
+  }
+  TrigEnd const ea = trig_end(a, shift), eb = trig_end(b, shift);
+  int const lo_k = ea.r_lo_nonpos ? ea.k : ea.k + 1;
+  int const hi_k = eb.r_hi_nonneg ? eb.k : eb.k - 1;
+  auto first = [lo_k](int res) { return lo_k + (((res - lo_k) % 4) + 4) % 4; };
+  double hi = first(1) <= hi_k ? 1.0 : std::fmax(ea.hi, eb.hi);
+  double lo = first(3) <= hi_k ? -1.0 : std::fmin(ea.lo, eb.lo);
+  return {std::fmax(lo, -1.0), std::fmin(hi, 1.0)};
+}
+
+// tanh(u) bounded above / below for 0 <= u <= 40, from M = e^(2u) - 1:
+// tanh = M / (M + 2), increasing in M
+double tanh_pos_hi(double u)
+{
+  double const v = u + u;
+  double mh;
+  if (v <= 1.0) {
+    double p = kExpm1Up[18];
+    for (int j = 17; j >= 0; --j) {
+      p = fma_ru(p, v, kExpm1Up[j]);
+    }
+    mh = mul_ru(v, add_ru(p, kExpm1Tail));
+  } else {
+    mh = sub_ru(exp_hi(v), 1.0);
+  }
+  double const r = div_ru(mh, add_rd(mh, 2.0));
+  return r < 1.0 ? r : 1.0;
+}
+
+double tanh_pos_lo(double u)
+{
+  double const v = u + u;
+  double ml;
+  if (v <= 1.0) {
+    double q = kExpm1Dn[18];
+    for (int j = 17; j >= 0; --j) {
+      q = fma_rd(q, v, kExpm1Dn[j]);
+    }
+    ml = mul_rd(v, q);
+  } else {
+    ml = sub_rd(exp_lo(v), 1.0);
+  }
+  return div_rd(ml, add_ru(ml, 2.0));
+}
+
+double tanh_hi(double x)
+{
+  if (x != x) {
+    return x;
+  }
+  if (x >= 0.0) {
+    return tanh_pos_hi(x < 40.0 ? x : 40.0);
+  }
+  double const u = -x;
+  return -tanh_pos_lo(u < 40.0 ? u : 40.0);
+}
+
+Iv iv_tanh(Iv x) { return {-tanh_hi(-x.lb), tanh_hi(x.ub)}; }
+
+// Round-to-nearest point versions
+double scale_rn(double x, int k)
+{
+  k = k < -2100 ? -2100 : (k > 1030 ? 1030 : k);
+  int const k1 = k < -1018 ? -1018 : (k > 1018 ? 1018 : k);
+  int k2 = k - k1;
+  k2 = k2 < -1074 ? -1074 : k2;
+  return (x * pow2_exact(k1)) * pow2_exact(k2);
+}
+
+double pt_exp(double x)
+{
+  if (x != x) {
+    return x;
+  }
+  if (x >= 709.8) {
+    return kInfO;
+  }
+  if (x <= -746.0) {
+    return 0.0;
+  }
+  double const kd = std::nearbyint(x * kInvLn2);
+  double r = std::fma(-kd, kLn2Hi, x);
+  r = std::fma(-kd, kLn2Lo, r);
+  double p = kExpRn[13];
+  for (int j = 12; j >= 0; --j) {
+    p = std::fma(p, r, kExpRn[j]);
+  }
+  return scale_rn(p, static_cast<int>(kd));
+}
+
+double pt_log(double x)
+{
+  if (x != x || x == kInfO) {
+    return x;
+  }
+  if (x < 0.0) {
+    return std::numeric_limits<double>::quiet_NaN();
+  }
+  if (x == 0.0) {
+    return -kInfO;
+  }
+  double m;
+  int e;
+  log_split(x, m, e);
+  double const s = (m - 1.0) / (m + 1.0);
+  double const w = s * s;
+  double p = kAtanhRn[11];
+  for (int j = 10; j >= 0; --j) {
+    p = std::fma(p, w, kAtanhRn[j]);
+  }
+  double const ed = static_cast<double>(e);
+  double r = (s + s) * p;
+  r = std::fma(ed, kLn2Lo, r);
+  return std::fma(ed, kLn2Hi, r);
+}
+
+double pt_sincos(double x, int shift)
+{
+  if (!(std::fabs(x) <= 1125899906842624.0)) {
+    return std::numeric_limits<double>::quiet_NaN();
+  }
+  double const kd = std::nearbyint(x * kTwoOverPi);
+  double r = std::fma(-kd, kPio2A, x);
+  r = std::fma(-kd, kPio2B, r);
+  r = std::fma(-kd, kPio2C, r);
+  double const w = r * r;
+  double s = kSinRn[9];
+  for (int j = 8; j >= 0; --j) {
+    s = std::fma(s, w, kSinRn[j]);
+  }
+  double c = kCosRn[10];
+  for (int j = 9; j >= 0; --j) {
+    c = std::fma(c, w, kCosRn[j]);
+  }
+  s = r * s;
+  int const q = (static_cast<int>(static_cast<long long>(kd) & 3) + shift) & 3;
+  return q == 0 ? s : (q == 1 ? c : (q == 2 ? -s : -c));
+}
+
+double pt_tanh(double x)
+{
+  if (x != x) {
+    return x;
+  }
+  double const ax = std::fabs(x);
+  double r;
+  if (ax > 40.0) {
+    r = 1.0;
+  } else {
+    double const v = ax + ax;
+    double mm;
+    if (v <= 1.0) {
+      double p = kExpm1Rn[18];
+      for (int j = 17; j >= 0; --j) {
+        p = std::fma(p, v, kExpm1Rn[j]);
+      }
+      mm = v * p;
+    } else {
+      mm = pt_exp(v) - 1.0;
+    }
+    r = mm / (mm + 2.0);
+  }
+  return x < 0.0 ? -r : r;
+}
+
+double pt_pown(double x, int n)
+{
+  if (n == 0) {
+    return 1.0;
+  }
+  unsigned k = static_cast<unsigned>(n < 0 ? -n : n);
+  double u = x, acc = 1.0;
+  bool have = false;
+  for (;;) {
+    if (k & 1u) {
+      acc = have ? acc * u : u;
+      have = true;
+    }
+    k >>= 1;
+    if (k == 0u) {
+      break;
+    }
+    u = u * u;
+  }
+  return n < 0 ? 1.0 / acc : acc;
+}
+
+/// Output form of a bound: +0 for a zero, the infinite bound on its side for NaN.
+double out_lb(double x) { return x != x ? -kInfO : plus_zero(x); }
+double out_ub(double x) { return x != x ? kInfO : plus_zero(x); }
+
+// ---------------------------------------------------------------------------
 // Problem view
 // ---------------------------------------------------------------------------
 
@@ -240,6 +794,7 @@ struct Prob
   std::vector<std::uint64_t> in0, in1;
   std::vector<double> cst;
   std::vector<std::uint64_t> vidx;
+  std::vector<int> iexp;
   std::vector<Iv> box;
   std::vector<std::uint8_t> kind;
   std::uint64_t obj;
@@ -260,7 +815,11 @@ Prob load(const cmi_problem_desc* d)
   p.in1.assign(d->n_nodes, 0);
   p.cst.assign(d->n_nodes, 0.0);
   p.vidx.assign(d->n_nodes, 0);
+  p.iexp.assign(d->n_nodes, 0);
   for (std::size_t i = 0; i < d->n_nodes; ++i) {
+    if (d->int_exp != nullptr) {
+      p.iexp[i] = d->int_exp[i];
+    }
     if (d->in0 != nullptr) {
       p.in0[i] = d->in0[i];
     }
@@ -308,9 +867,17 @@ Prob load(const cmi_problem_desc* d)
 // DAG evaluation, node by node (one "kernel" per reachable node)
 // ---------------------------------------------------------------------------
 
-void eval_interval(const Prob& p, const Iv* x, std::vector<Iv>& val)
+/// Returns false when some value is empty (IEEE 1788: division by [0, 0],
+/// sqrt or log entirely below the domain); every value reaches a root, so the
+/// box is then infeasible.
+bool eval_interval(const Prob& p, const Iv* x, std::vector<Iv>& val)
 {
   val.resize(p.n_nodes);
+  bool ok = true;
+  auto take = [&ok](DivOut d) {
+    ok = ok && d.ok;
+    return d.v;
+  };
   for (std::size_t i = 0; i < p.n_nodes; ++i) {
     if (!p.reach[i]) {
       continue;
@@ -327,6 +894,11 @@ void eval_interval(const Prob& p, const Iv* x, std::vector<Iv>& val)
     bool const ca = p.op[a] == CMI_OP_CONST;
     bool const cb = arity(op) > 1 && p.op[b] == CMI_OP_CONST;
     double const ka = p.cst[a], kb = arity(op) > 1 ? p.cst[b] : 0.0;
+    if (arity(op) == 1 && ca) {
+      throw std::runtime_error("unary op on a Const");
+    }
+    Iv const va = ca ? Iv {ka, ka} : val[a];
+    Iv const vb = arity(op) > 1 ? (cb ? Iv {kb, kb} : val[b]) : Iv {0.0, 0.0};
     switch (op) {
       case CMI_OP_ADD:
         val[i] = (ca && cb) ? iv_add({ka, ka}, {kb, kb})
@@ -346,22 +918,30 @@ void eval_interval(const Prob& p, const Iv* x, std::vector<Iv>& val)
             : cb            ? iv_mul_s(val[a], kb)
                             : iv_mul(val[a], val[b]);
         break;
-      case CMI_OP_SQR:
-        if (ca) {
-          throw std::runtime_error("unary op on a Const");
-        }
-        val[i] = iv_sqr(val[a]);
+      case CMI_OP_SQR: val[i] = iv_sqr(va); break;
+      case CMI_OP_NEG: val[i] = iv_neg(va); break;
+      case CMI_OP_DIV: val[i] = take(iv_div(va, vb)); break;
+      case CMI_OP_MIN: val[i] = iv_min(va, vb); break;
+      case CMI_OP_MAX: val[i] = iv_max(va, vb); break;
+      case CMI_OP_POWN: val[i] = take(iv_pown(va, p.iexp[i])); break;
+      case CMI_OP_SQRT: val[i] = take(iv_sqrt(va)); break;
+      case CMI_OP_EXP: val[i] = iv_exp(va); break;
+      case CMI_OP_LOG: val[i] = take(iv_log(va)); break;
+      case CMI_OP_SIN: val[i] = iv_sincos(va, 0); break;
+      case CMI_OP_COS: val[i] = iv_sincos(va, 1); break;
+      case CMI_OP_TANH: val[i] = iv_tanh(va); break;
+      case CMI_OP_ABS: val[i] = iv_abs(va); break;
+      case CMI_OP_POW: {
+        // x**y = exp(y log x): the same three steps the tape runs
+        Iv const l = take(iv_log(va));
+        Iv const m = cb ? iv_mul_s(l, kb) : iv_mul(l, vb);
+        val[i] = iv_exp(m);
         break;
-      case CMI_OP_NEG:
-        if (ca) {
-          throw std::runtime_error("unary op on a Const");
-        }
-        val[i] = iv_neg(val[a]);
-        break;
-      default:
-        throw std::runtime_error("oracle: op outside the QCQP set");
+      }
+      default: throw std::runtime_error("oracle: unknown op");
     }
   }
+  return ok;
 }
 
 void eval_point(const Prob& p, const double* x, std::vector<double>& val)
@@ -388,7 +968,19 @@ void eval_point(const Prob& p, const double* x, std::vector<double>& val)
       case CMI_OP_MUL: val[i] = a * b; break;
       case CMI_OP_SQR: val[i] = a * a; break;
       case CMI_OP_NEG: val[i] = -a; break;
-      default: throw std::runtime_error("oracle: op outside the QCQP set");
+      case CMI_OP_DIV: val[i] = a / b; break;
+      case CMI_OP_MIN: val[i] = std::fmin(a, b); break;
+      case CMI_OP_MAX: val[i] = std::fmax(a, b); break;
+      case CMI_OP_POWN: val[i] = pt_pown(a, p.iexp[i]); break;
+      case CMI_OP_SQRT: val[i] = std::sqrt(a); break;
+      case CMI_OP_EXP: val[i] = pt_exp(a); break;
+      case CMI_OP_LOG: val[i] = pt_log(a); break;
+      case CMI_OP_SIN: val[i] = pt_sincos(a, 0); break;
+      case CMI_OP_COS: val[i] = pt_sincos(a, 1); break;
+      case CMI_OP_TANH: val[i] = pt_tanh(a); break;
+      case CMI_OP_ABS: val[i] = std::fabs(a); break;
+      case CMI_OP_POW: val[i] = pt_exp(b * pt_log(a)); break;
+      default: throw std::runtime_error("oracle: unknown op");
     }
   }
 }
@@ -408,7 +1000,8 @@ bool feasible_interval(const Prob& p, const std::vector<Iv>& val)
   for (std::size_t k = 0; k < p.n_con; ++k) {
     Iv const v = val[p.con_root[k]];
     double const rhs = p.con_rhs[k];
-    bool const ok = p.con_cmp[k] == CMI_LE ? v.lb <= rhs : (v.lb <= rhs && rhs <= v.ub);
+    // a NaN bound (no information) never fails a check
+    bool const ok = p.con_cmp[k] == CMI_LE ? !(v.lb > rhs) : (!(v.lb > rhs) && !(rhs > v.ub));
     f = f && ok;
   }
   return f;
@@ -518,8 +1111,9 @@ Cand points_argmin(const Prob& p, const std::vector<Iv>& box, const std::size_t*
         x[v] = sampling ? draw(p.kind[v] != CMI_CONTINUOUS, sub[v].lb, sub[v].ub, v, i, r, salt) : sub[v].lb;
       }
       eval_point(p, x.data(), val);
+      // a point needs a finite objective (log(0) = -inf is no incumbent)
       double m = feasible_point(p, val) ? val[p.obj] : INFINITY;
-      if (m != m) {
+      if (m != m || m == -INFINITY) {
         m = INFINITY;
       }
       if (m < best) {
@@ -778,6 +1372,62 @@ void orc_iv_sqr(const double* x, double* out)
   out[0] = r.lb;
   out[1] = r.ub;
 }
+// Directed primitives and the extended interval / point operations, n at a
+// time, for the tests. `op` is a CMI_OP_* code (Div, Min, Max binary; the
+// others unary; n_exp is PowN's exponent); ok[i] = 0 marks an empty result.
+double orc_fma_ru(double a, double b, double c) { return fma_ru(a, b, c); }
+double orc_fma_rd(double a, double b, double c) { return fma_rd(a, b, c); }
+double orc_div_ru(double a, double b) { return div_ru(a, b); }
+double orc_div_rd(double a, double b) { return div_rd(a, b); }
+double orc_sqrt_ru(double a) { return sqrt_ru(a); }
+double orc_sqrt_rd(double a) { return sqrt_rd(a); }
+
+int orc_iv_ext(int op, const double* alb, const double* aub, const double* blb, const double* bub, int n_exp,
+               size_t n, double* out_lb_, double* out_ub_, uint8_t* ok)
+{
+  for (size_t i = 0; i < n; ++i) {
+    Iv const a {alb[i], aub[i]};
+    Iv const b = blb != nullptr ? Iv {blb[i], bub[i]} : Iv {0.0, 0.0};
+    DivOut r {{0.0, 0.0}, true};
+    switch (op) {
+      case CMI_OP_DIV: r = iv_div(a, b); break;
+      case CMI_OP_MIN: r.v = iv_min(a, b); break;
+      case CMI_OP_MAX: r.v = iv_max(a, b); break;
+      case CMI_OP_POWN: r = iv_pown(a, n_exp); break;
+      case CMI_OP_SQRT: r = iv_sqrt(a); break;
+      case CMI_OP_EXP: r.v = iv_exp(a); break;
+      case CMI_OP_LOG: r = iv_log(a); break;
+      case CMI_OP_SIN: r.v = iv_sincos(a, 0); break;
+      case CMI_OP_COS: r.v = iv_sincos(a, 1); break;
+      case CMI_OP_TANH: r.v = iv_tanh(a); break;
+      case CMI_OP_ABS: r.v = iv_abs(a); break;
+      default: return 1;
+    }
+    out_lb_[i] = r.v.lb;
+    out_ub_[i] = r.v.ub;
+    ok[i] = r.ok ? 1 : 0;
+  }
+  return 0;
+}
+
+int orc_pt_ext(int op, const double* a, const double* b, int n_exp, size_t n, double* out)
+{
+  for (size_t i = 0; i < n; ++i) {
+    switch (op) {
+      case CMI_OP_DIV: out[i] = a[i] / b[i]; break;
+      case CMI_OP_POWN: out[i] = pt_pown(a[i], n_exp); break;
+      case CMI_OP_SQRT: out[i] = std::sqrt(a[i]); break;
+      case CMI_OP_EXP: out[i] = pt_exp(a[i]); break;
+      case CMI_OP_LOG: out[i] = pt_log(a[i]); break;
+      case CMI_OP_SIN: out[i] = pt_sincos(a[i], 0); break;
+      case CMI_OP_COS: out[i] = pt_sincos(a[i], 1); break;
+      case CMI_OP_TANH: out[i] = pt_tanh(a[i]); break;
+      default: return 1;
+    }
+  }
+  return 0;
+}
+
 void orc_slot_bounds(int kind, const double* parent, size_t part, size_t fan_out, double* out)
 {
   Iv const r = slot_bounds(static_cast<std::uint8_t>(kind), {parent[0], parent[1]}, part, fan_out);
@@ -806,17 +1456,17 @@ int orc_eval_boxes(const cmi_problem_desc* d, const double* lb, const double* ub
       for (std::size_t v = 0; v < p.n_vars; ++v) {
         x[v] = {lb[v * ld + e], ub[v * ld + e]};
       }
-      eval_interval(p, x.data(), val);
-      feasible[e] = feasible_interval(p, val) ? 1 : 0;
-      obj_lb[e] = plus_zero(val[p.obj].lb);
+      bool const ok = eval_interval(p, x.data(), val);
+      feasible[e] = (ok && feasible_interval(p, val)) ? 1 : 0;
+      obj_lb[e] = out_lb(val[p.obj].lb);
       if (obj_ub != nullptr) {
-        obj_ub[e] = plus_zero(val[p.obj].ub);
+        obj_ub[e] = out_ub(val[p.obj].ub);
       }
       if (roots != nullptr) {
         for (std::size_t k = 0; k <= p.n_con; ++k) {
           Iv const r = val[k == 0 ? p.obj : p.con_root[k - 1]];
-          roots[2 * (k * n + e)] = plus_zero(r.lb);
-          roots[2 * (k * n + e) + 1] = plus_zero(r.ub);
+          roots[2 * (k * n + e)] = out_lb(r.lb);
+          roots[2 * (k * n + e) + 1] = out_ub(r.ub);
         }
       }
     }
@@ -868,9 +1518,9 @@ int orc_bound(const cmi_problem_desc* d, const cmi_ivl* box, const size_t* var_i
     }
     for (std::size_t r = 0; r < n_regions; ++r) {
       materialise(parent, r, var_ids, widths, kinds, slots, sub);
-      eval_interval(p, sub.data(), val);
-      feasible[r] = feasible_interval(p, val) ? 1 : 0;
-      obj_lb[r] = plus_zero(val[p.obj].lb);
+      bool const ok = eval_interval(p, sub.data(), val);
+      feasible[r] = (ok && feasible_interval(p, val)) ? 1 : 0;
+      obj_lb[r] = out_lb(val[p.obj].lb);
     }
     return 0;
   } catch (const std::exception& ex) {
@@ -906,9 +1556,9 @@ int orc_aggregate(const cmi_problem_desc* d, const cmi_ivl* box, const size_t* v
     std::vector<double> lo(k, inf), hi(k, -inf);
     for (std::size_t r = 0; r < n_regions; ++r) {
       materialise(parent, r, var_ids, widths, kinds, slots, sub);
-      eval_interval(p, sub.data(), val);
-      bool const f = feasible_interval(p, val);
-      double const l = plus_zero(val[p.obj].lb), u = plus_zero(val[p.obj].ub);
+      bool const ok = eval_interval(p, sub.data(), val);
+      bool const f = ok && feasible_interval(p, val);
+      double const l = out_lb(val[p.obj].lb), u = out_ub(val[p.obj].ub);
       if (feasible != nullptr) {
         feasible[r] = f ? 1 : 0;
         obj_lb[r] = l;
@@ -1035,13 +1685,13 @@ int orc_solve(const cmi_problem_desc* d, size_t partition_num, size_t enumerate_
       std::vector<Iv> sub, val;
       for (std::size_t r = 0; r < n_regions; ++r) {
         materialise(box, r, a.vars.data(), a.widths.data(), a.kinds.data(), a.vars.size(), sub);
-        eval_interval(p, sub.data(), val);
+        bool const ok = eval_interval(p, sub.data(), val);
         ++bounded;
-        if (!feasible_interval(p, val)) {
+        if (!(ok && feasible_interval(p, val))) {
           ++p_inf;
           continue;
         }
-        double const lb = plus_zero(val[p.obj].lb);
+        double const lb = out_lb(val[p.obj].lb);
         if (lb > gub) {
           ++p_dom;
           continue;
