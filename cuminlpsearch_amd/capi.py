@@ -85,7 +85,7 @@ class TapeInfo(C.Structure):
                 ("box_groups", C.c_size_t), ("box_instr", C.c_size_t), ("box_slots", C.c_size_t),
                 ("box_engine", C.c_int), ("box_compiled", C.c_int), ("compile_seconds", C.c_double),
                 ("compiled_persistent", C.c_int), ("compiled_wgs_per_cu", C.c_size_t),
-                ("compiled_waves", C.c_size_t)]
+                ("compiled_waves", C.c_size_t), ("extended", C.c_int)]
 
 
 class BoxCandidate(C.Structure):

@@ -177,7 +177,7 @@ std::span<const aggregate::AggregateBound> AggregateBounderImpl::bound_children(
   std::size_t const used = off_data_ + r_count * item_stride_;
   check(hipMemcpyAsync(staging_d_.get(), h, used, hipMemcpyHostToDevice, s), "hipMemcpyAsync(aggregate staging)");
   std::uint32_t const n_tiles = static_cast<std::uint32_t>(r_count * tiles_per_region_);
-  dev::launch_items(dev::Mode::Aggregate, staging_d_.as<dev::Item>(), tile_item_d_.as<std::uint32_t>(), n_tiles,
+  dev::launch_items(dev::Mode::Aggregate, tape_->extended(), staging_d_.as<dev::Item>(), tile_item_d_.as<std::uint32_t>(), n_tiles,
                     tape_->n_slots(), s);
   check(hipGetLastError(), "launch(k_items aggregate)");
   check(hipMemcpyAsync(child_h_.data(), staging_d_.as<unsigned char>() + off_child_, r_count * 2 * k_ * sizeof(double),

@@ -75,6 +75,7 @@ void Runner::run(std::span<const Request> reqs)
   std::size_t tiles[3] = {0, 0, 0};
   std::size_t total_regions = 0, n_cand = 0, witness_doubles = 0, data_bytes = 0;
   std::uint32_t lds[3] = {1, 1, 1};
+  bool ext[3] = {false, false, false};
   for (std::size_t k = 0; k < n; ++k) {
     std::size_t const i = order[k];
     Request const& r = reqs[i];
@@ -99,6 +100,7 @@ void Runner::run(std::span<const Request> reqs)
     p.n_vars = r.tape->n_vars();
     tiles[pass] += (n_elems + dev::kTile - 1) / dev::kTile;
     lds[pass] = std::max(lds[pass], r.tape->n_slots());
+    ext[pass] = ext[pass] || r.tape->extended();
     if (r.mode == dev::Mode::Bound) {
       p.region_off = total_regions;
       total_regions += n_regions;
@@ -228,7 +230,7 @@ void Runner::run(std::span<const Request> reqs)
   for (int pass = 0; pass < 3; ++pass) {
     dev::Mode const m = pass == 0 ? dev::Mode::Sample : pass == 1 ? dev::Mode::Enumerate : dev::Mode::Bound;
     if (counts[pass] > 0) {
-      dev::launch_items(m, reinterpret_cast<const dev::Item*>(d + off_items) + first_item,
+      dev::launch_items(m, ext[pass], reinterpret_cast<const dev::Item*>(d + off_items) + first_item,
                         reinterpret_cast<const std::uint32_t*>(d + off_tile_item) + first_tile,
                         static_cast<std::uint32_t>(tiles[pass]), lds[pass], s);
       check(hipGetLastError(), "launch(k_items batch)");

@@ -373,6 +373,7 @@ cmi_status cmi_tape_info_get(const cmi_tape* t, cmi_tape_info* out)
     out->compiled_persistent = sh.persistent ? 1 : 0;
     out->compiled_wgs_per_cu = t->tape->compiled_wgs_per_cu();
     out->compiled_waves = sh.waves;
+    out->extended = tp.extended ? 1 : 0;
   });
 }
 
@@ -525,6 +526,7 @@ cmi_status cmi_tape_compile_words(const cmi_problem* p, size_t k, void* words, s
     out->compiled_persistent = 0;
     out->compiled_wgs_per_cu = 1;
     out->compiled_waves = 1;
+    out->extended = whole.extended ? 1 : 0;
     if (words != nullptr) {
       if (cap_words < tp.code.size()) {
         throw cmi::ShapeMismatch("word buffer holds " + std::to_string(cap_words) + " words; the tape has "
@@ -564,6 +566,7 @@ cmi_status cmi_tape_compile_info(const cmi_problem* p, cmi_tape_info* out, char*
     out->compiled_persistent = 0;
     out->compiled_wgs_per_cu = 1;
     out->compiled_waves = 1;
+    out->extended = tp.extended ? 1 : 0;
     if (listing != nullptr && cap > 0) {
       std::string const text = cmi::tape::disassemble(tp);
       std::size_t const k = std::min(cap - 1, text.size());

@@ -68,8 +68,29 @@ enum class TOp : std::uint8_t
   AccCheckEQ,     // feasible &= ACC == imm
   AccObjective,   // objective <- ACC
   AccOutput,      // roots_out[b] <- ACC
+  // Extended op set (interpreted by the extended kernels only).
+  Div = 32,   // dst <- a / b (IEEE 1788; division by [0, 0] empties the box)
+  DivImm,     // dst <- a / imm
+  ImmDiv,     // dst <- imm / a
+  PowN,       // dst <- a^n, n = imm (an integer)
+  Sqrt,       // dst <- sqrt(a) on a's part in [0, inf)
+  Exp,        // dst <- exp(a)
+  Log,        // dst <- log(a) on a's part in (0, inf)
+  Sin,        // dst <- sin(a)
+  Cos,        // dst <- cos(a)
+  Tanh,       // dst <- tanh(a)
+  Abs,        // dst <- |a|
+  Min,        // dst <- min(a, b)
+  MinImm,     // dst <- min(a, imm)
+  Max,        // dst <- max(a, b)
+  MaxImm,     // dst <- max(a, imm)
   Count
 };
+
+/// First and one-past-last op code of the extended op set.
+inline constexpr std::uint8_t kExtBase = 32;
+inline constexpr std::uint8_t kExtEnd = 47;
+inline bool is_ext_op(std::uint8_t op) { return op >= kExtBase && op < kExtEnd; }
 
 // Term instructions: op = kTermBase + mode * 8 + term.
 inline constexpr std::uint8_t kTermBase = 64;
@@ -114,6 +135,11 @@ inline std::string op_name(std::uint8_t op)
   if (is_term_op(op)) {
     unsigned const k = op - kTermBase;
     return std::string("acc.") + modes[k >> 3] + "." + terms[k & 7];
+  }
+  static const char* ext[] = {"div", "divi", "idiv", "pown", "sqrt", "exp", "log", "sin",
+                              "cos", "tanh", "abs", "min", "mini", "max", "maxi"};
+  if (is_ext_op(op)) {
+    return ext[op - kExtBase];
   }
   return "?";
 }
@@ -172,6 +198,24 @@ inline std::uint32_t interval_flops(std::uint8_t op)
     case TOp::CheckEQ:
     case TOp::AccCheckEQ:
       return 2;
+    // extended ops: the directed FP64 operations of their enclosures
+    case TOp::Div:
+    case TOp::DivImm:
+    case TOp::ImmDiv:
+      return 24;
+    case TOp::Sqrt:
+      return 30;
+    case TOp::PowN:
+      return 12;
+    case TOp::Exp:
+      return 44;
+    case TOp::Log:
+      return 100;
+    case TOp::Sin:
+    case TOp::Cos:
+      return 110;
+    case TOp::Tanh:
+      return 80;
     default:
       return 0;
   }
@@ -193,6 +237,7 @@ struct Tape
   std::vector<std::uint32_t> group_begin;
   std::uint32_t temps_per_group = 0;
   std::uint32_t objective_group = 0;  // the group holding the Objective instruction
+  bool extended = false;              // uses the extended op set
 };
 
 struct CompileOptions
@@ -282,6 +327,11 @@ private:
   using Op = model::Op;
 
   bool is_var(std::size_t id) const { return nodes_[id].op == Op::Var; }
+  /// Ops cheap enough to re-evaluate at every use rather than keep live.
+  static bool cheap(Op op)
+  {
+    return op == Op::Add || op == Op::Sub || op == Op::Mul || op == Op::Sqr || op == Op::Neg;
+  }
   bool is_const(std::size_t id) const { return nodes_[id].op == Op::Const; }
   double cval(std::size_t id) const { return nodes_[id].payload.constant; }
 
@@ -312,9 +362,21 @@ private:
         case Op::Add:
         case Op::Sub:
         case Op::Mul:
+        case Op::Div:
+        case Op::Min:
+        case Op::Max:
+        case Op::Pow:
           break;
         case Op::Sqr:
         case Op::Neg:
+        case Op::Sqrt:
+        case Op::Exp:
+        case Op::Log:
+        case Op::Sin:
+        case Op::Cos:
+        case Op::Tanh:
+        case Op::Abs:
+        case Op::PowN:
           if (is_const(nd.in[0])) {
             throw InvalidDAG("unary op applied directly to an Op::Const");
           }
@@ -333,7 +395,7 @@ private:
       for (std::size_t in : nodes_[i].in) {
         all = all && (is_var(in) || is_const(in));
       }
-      leaf_[i] = all && opt_.rematerialise_leaves;
+      leaf_[i] = all && opt_.rematerialise_leaves && cheap(nodes_[i].op);
     }
   }
 
@@ -381,7 +443,7 @@ private:
     bool const c0 = is_const(nd.in[0]);
     bool const c1 = arity > 1 && is_const(nd.in[1]);
     std::int64_t a = -1, b = -1;
-    if (c0 && c1) {
+    if ((c0 && c1) || (c0 && nd.op == Op::Pow)) {
       a = fresh();
       emit({static_cast<std::uint8_t>(TOp::LoadConst), a, -1, -1, cval(nd.in[0])});
     } else if (!c0) {
@@ -424,6 +486,48 @@ private:
         break;
       case Op::Sqr: out(TOp::Sqr, a, -1, 0.0); break;
       case Op::Neg: out(TOp::Neg, a, -1, 0.0); break;
+      case Op::Div:
+        if (c1) {
+          out(TOp::DivImm, a, -1, cval(nd.in[1]));
+        } else if (c0) {
+          out(TOp::ImmDiv, b, -1, cval(nd.in[0]));
+        } else {
+          out(TOp::Div, a, b, 0.0);
+        }
+        break;
+      case Op::Min:
+      case Op::Max: {
+        bool const mn = nd.op == Op::Min;
+        if (c1) {
+          out(mn ? TOp::MinImm : TOp::MaxImm, a, -1, cval(nd.in[1]));
+        } else if (c0) {
+          out(mn ? TOp::MinImm : TOp::MaxImm, b, -1, cval(nd.in[0]));
+        } else {
+          out(mn ? TOp::Min : TOp::Max, a, b, 0.0);
+        }
+        break;
+      }
+      case Op::PowN: out(TOp::PowN, a, -1, static_cast<double>(nd.payload.int_exp)); break;
+      case Op::Sqrt: out(TOp::Sqrt, a, -1, 0.0); break;
+      case Op::Exp: out(TOp::Exp, a, -1, 0.0); break;
+      case Op::Log: out(TOp::Log, a, -1, 0.0); break;
+      case Op::Sin: out(TOp::Sin, a, -1, 0.0); break;
+      case Op::Cos: out(TOp::Cos, a, -1, 0.0); break;
+      case Op::Tanh: out(TOp::Tanh, a, -1, 0.0); break;
+      case Op::Abs: out(TOp::Abs, a, -1, 0.0); break;
+      case Op::Pow: {
+        // x**y = exp(y log x), in three words through two temporaries
+        std::int64_t const l = fresh();
+        emit({static_cast<std::uint8_t>(TOp::Log), l, a, -1, 0.0});
+        std::int64_t const m = fresh();
+        if (c1) {
+          emit({static_cast<std::uint8_t>(TOp::MulImm), m, l, -1, cval(nd.in[1])});
+        } else {
+          emit({static_cast<std::uint8_t>(TOp::Mul), m, l, b, 0.0});
+        }
+        out(TOp::Exp, m, -1, 0.0);
+        break;
+      }
       default: throw InvalidConfiguration("unsupported op reached the tape emitter");
     }
     return d;
@@ -683,6 +787,7 @@ private:
       }
       t.code.push_back(ins);
       t.flops += interval_flops(ins.op);
+      t.extended = t.extended || is_ext_op(ins.op);
     }
     if (next_slot > 65535) {
       throw InvalidConfiguration("tape needs more slots than the 16-bit encoding holds");
@@ -772,7 +877,8 @@ inline Tape compile_groups(const model::Problem<double>& p, std::uint32_t n_grou
     for (Instr ins : parts[g].code) {
       bool const out = ins.op == static_cast<std::uint8_t>(TOp::Output)
           || ins.op == static_cast<std::uint8_t>(TOp::AccOutput);
-      bool const writes = ins.op >= static_cast<std::uint8_t>(TOp::LoadConst) && ins.op <= static_cast<std::uint8_t>(TOp::Neg);
+      bool const writes = (ins.op >= static_cast<std::uint8_t>(TOp::LoadConst) && ins.op <= static_cast<std::uint8_t>(TOp::Neg))
+          || is_ext_op(ins.op);
       if (writes || ins.op == static_cast<std::uint8_t>(TOp::AccStore)) {
         ins.dst = remap(ins.dst);
       }
@@ -785,6 +891,7 @@ inline Tape compile_groups(const model::Problem<double>& p, std::uint32_t n_grou
     t.node_ops += parts[g].node_ops;
   }
   t.flops = whole.flops;  // algorithmic work: every DAG operation once
+  t.extended = whole.extended;
   t.group_begin.push_back(static_cast<std::uint32_t>(t.code.size()));
   std::uint64_t const slots = std::uint64_t(t.n_vars) + std::uint64_t(n_groups) * t.temps_per_group;
   if (slots > 65535) {

@@ -72,7 +72,9 @@ enum class Mode : int
 
 /// Launch `n_tiles` workgroups over `items`; `tile_item[t]` names the item of
 /// global tile t. `lds_slots` must be >= every item's tape slot count.
-void launch_items(Mode mode, const Item* items, const std::uint32_t* tile_item, std::uint32_t n_tiles,
+/// `extended`: some item's tape uses the extended op set (then the kernels
+/// that interpret it run; results of the other tapes are the same either way).
+void launch_items(Mode mode, bool extended, const Item* items, const std::uint32_t* tile_item, std::uint32_t n_tiles,
                   std::uint32_t lds_slots, hipStream_t stream);
 
 /// Explicit-box evaluation: box e's variable v is [lb[v*ld+e], ub[v*ld+e]].
@@ -80,7 +82,7 @@ void launch_items(Mode mode, const Item* items, const std::uint32_t* tile_item, 
 /// output-major: roots[k*n_boxes+e]) when non-null. One workgroup of
 /// groups.n_groups waves per 64 boxes; `code` holds every group's code
 /// followed by one Nop word.
-void launch_boxes(const tape::Instr* code, const GroupTable& groups, std::uint32_t n_slots, std::uint32_t n_vars,
+void launch_boxes(const tape::Instr* code, bool extended, const GroupTable& groups, std::uint32_t n_slots, std::uint32_t n_vars,
                   const double* lb, const double* ub, std::uint64_t ld, std::uint64_t n_boxes,
                   std::uint8_t* feasible, double* obj_lb, double* obj_ub, double* roots, hipStream_t stream);
 
@@ -92,7 +94,7 @@ std::size_t lds_limit_bytes();
 
 /// Points (round to nearest): x[v*ld+e], one wave per 64 points over an
 /// ungrouped tape of n_instr words followed by one Nop word.
-void launch_points(const tape::Instr* code, std::uint32_t n_instr, std::uint32_t n_slots, std::uint32_t n_vars,
+void launch_points(const tape::Instr* code, bool extended, std::uint32_t n_instr, std::uint32_t n_slots, std::uint32_t n_vars,
                    const double* x, std::uint64_t ld, std::uint64_t n_points, std::uint8_t* feasible,
                    double* obj, double* roots, hipStream_t stream);
 

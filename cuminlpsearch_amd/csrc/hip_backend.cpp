@@ -217,7 +217,7 @@ void DeviceTape::eval_boxes_device(const double* lb, const double* ub, std::size
     c->launch(lb, ub, ld, n, feasible, obj_lb, obj_ub, roots, stream);
     return;
   }
-  dev::launch_boxes(grouped_code_.as<tape::Instr>(), groups_, grouped_.n_slots, n_vars(), lb, ub, ld, n, feasible,
+  dev::launch_boxes(grouped_code_.as<tape::Instr>(), grouped_.extended, groups_, grouped_.n_slots, n_vars(), lb, ub, ld, n, feasible,
                     obj_lb, obj_ub, roots, stream);
   check(hipGetLastError(), "launch(k_boxes)");
 }
@@ -346,7 +346,7 @@ void DeviceTape::eval_points_device(const double* x, std::size_t ld, std::size_t
   if (tape_.n_slots > ctx_->lds_point_slots()) {
     throw ResourceExhausted("tape needs more point slots than LDS holds");
   }
-  dev::launch_points(code(), n_instr(), n_slots(), n_vars(), x, ld, n, feasible, obj, roots, stream);
+  dev::launch_points(code(), extended(), n_instr(), n_slots(), n_vars(), x, ld, n, feasible, obj, roots, stream);
   check(hipGetLastError(), "launch(k_points)");
 }
 
@@ -530,7 +530,7 @@ void RoleImpl::run(std::span<const interval> box, std::span<const std::size_t> v
     running *= widths[j];
   }
   check(hipMemcpyAsync(staging_d_.get(), h, staging_bytes_, hipMemcpyHostToDevice, s), "hipMemcpyAsync(staging)");
-  dev::launch_items(mode_, staging_d_.as<dev::Item>(), tile_item_d_.as<std::uint32_t>(), n_tiles_, tape_->n_slots(),
+  dev::launch_items(mode_, tape_->extended(), staging_d_.as<dev::Item>(), tile_item_d_.as<std::uint32_t>(), n_tiles_, tape_->n_slots(),
                     s);
   check(hipGetLastError(), "launch(k_items)");
   if (mode_ == dev::Mode::Bound) {

@@ -213,6 +213,8 @@ public:
   std::uint32_t n_vars() const { return tape_.n_vars; }
   std::uint32_t n_instr() const { return static_cast<std::uint32_t>(tape_.code.size()); }
   std::uint32_t n_slots() const { return tape_.n_slots; }
+  /// The tape uses the extended op set (Div, PowN, transcendental, ...).
+  bool extended() const { return tape_.extended; }
   const std::shared_ptr<Context>& context() const { return ctx_; }
 
   /// C2-style evaluation of explicit boxes held in device memory.

@@ -62,6 +62,30 @@ __device__ __forceinline__ void leave_round_up(int tok, double a, double b)
       : "memory");
 }
 
+/// Leave the region after a, b and the flag f are computed.
+__device__ __forceinline__ void leave_round_up(int tok, double a, double b, bool f)
+{
+  int const fi = f ? 1 : 0;
+  asm volatile(
+      "s_nop 1\n\t"
+      "s_setreg_imm32_b32 hwreg(HW_REG_MODE, 2, 2), 0 ; %0 %1 %2 %3"
+      :
+      : "s"(tok), "v"(a), "v"(b), "v"(fi)
+      : "memory");
+}
+
+/// Leave the region after the flag f is computed.
+__device__ __forceinline__ void leave_round_up(int tok, bool f)
+{
+  int const fi = f ? 1 : 0;
+  asm volatile(
+      "s_nop 1\n\t"
+      "s_setreg_imm32_b32 hwreg(HW_REG_MODE, 2, 2), 0 ; %0 %1"
+      :
+      : "s"(tok), "v"(fi)
+      : "memory");
+}
+
 /// An interval [lb, ub] held as nl = -lb and ub.
 struct Iv
 {

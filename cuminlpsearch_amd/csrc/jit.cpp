@@ -115,6 +115,22 @@ std::string emit_word(const Instr& w, const std::string& acc)
     case TOp::AccCheckEQ: return set("f", "f && check_eq(" + acc + ", " + c + ")");
     case TOp::AccObjective: return set("obj", acc);
     case TOp::AccOutput: return output(acc);
+    // extended op set; `f` (the group's feasibility) takes the empty results
+    case TOp::Div: return set(d, "iv_div(" + a + ", " + b + ", t, f)");
+    case TOp::DivImm: return set(d, "iv_div_imm(" + a + ", " + c + ", t, f)");
+    case TOp::ImmDiv: return set(d, "iv_imm_div(" + c + ", " + a + ", t, f)");
+    case TOp::PowN: return set(d, "iv_pown(" + a + ", " + std::to_string(static_cast<int>(w.imm)) + ", t, f)");
+    case TOp::Sqrt: return set(d, "iv_sqrt(" + a + ", t, f)");
+    case TOp::Exp: return set(d, "iv_exp(" + a + ", t)");
+    case TOp::Log: return set(d, "iv_log(" + a + ", t, f)");
+    case TOp::Sin: return set(d, "iv_sincos(" + a + ", 0, t)");
+    case TOp::Cos: return set(d, "iv_sincos(" + a + ", 1, t)");
+    case TOp::Tanh: return set(d, "iv_tanh(" + a + ", t)");
+    case TOp::Abs: return set(d, "iv_abs(" + a + ")");
+    case TOp::Min: return set(d, "iv_min(" + a + ", " + b + ")");
+    case TOp::MinImm: return set(d, "iv_min(" + a + ", iv_const(" + c + ", t))");
+    case TOp::Max: return set(d, "iv_max(" + a + ", " + b + ")");
+    case TOp::MaxImm: return set(d, "iv_max(" + a + ", iv_const(" + c + ", t))");
     default: throw InvalidConfiguration("tape word with an op the compiled kernel does not know");
   }
 }
@@ -165,11 +181,13 @@ void slot_use(const tape::Tape& t, std::uint32_t lo, std::uint32_t hi, std::set<
     bool const term = tape::is_term_op(w.op);
     auto const op = static_cast<TOp>(w.op);
     unsigned const kind = w.op & 7u;
-    bool const reads_a = term || (w.op >= static_cast<std::uint8_t>(TOp::Add) && w.op <= static_cast<std::uint8_t>(TOp::Output));
+    bool const ext = tape::is_ext_op(w.op);
+    bool const reads_a =
+        term || ext || (w.op >= static_cast<std::uint8_t>(TOp::Add) && w.op <= static_cast<std::uint8_t>(TOp::Output));
     bool const reads_b = (term && (kind == static_cast<unsigned>(Term::SS) || kind == static_cast<unsigned>(Term::SSC)))
-        || op == TOp::Add || op == TOp::Sub || op == TOp::Mul;
+        || op == TOp::Add || op == TOp::Sub || op == TOp::Mul || op == TOp::Div || op == TOp::Min || op == TOp::Max;
     bool const writes = (w.op >= static_cast<std::uint8_t>(TOp::LoadConst) && w.op <= static_cast<std::uint8_t>(TOp::Neg))
-        || op == TOp::AccStore;
+        || op == TOp::AccStore || ext;
     if (reads_a) {
       note(w.a);
     }
@@ -325,7 +343,7 @@ std::string boxes_source(const tape::Tape& t, const Shape& shape)
     }
     o << ind << "}\n";
   }
-  o << ind << "leave_round_up(t);\n";
+  o << ind << "leave_round_up(t, f);\n";
   o << ind << "feas_s[cur * " << 64 * w << "u + wave * 64u + lane] = f ? 1u : 0u;\n";
   if (persistent) {
     std::uint32_t const share = (nv + w - 1) / w;

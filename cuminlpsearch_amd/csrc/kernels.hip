@@ -160,10 +160,37 @@ __device__ __forceinline__ void store_iv(double2* S, std::uint32_t slot, unsigne
   S[slot * kTile + lane] = make_double2(v.nl, v.ub);
 }
 
+// One word of the extended op set (interval): the result, and `ok` cleared
+// when it is empty.
+__device__ __forceinline__ Iv ext_interval(std::uint32_t op, Iv a, const double2* S, const Word& ins, unsigned lane,
+                                           int t, bool& ok)
+{
+  switch (static_cast<TOp>(op)) {
+    case TOp::Div: return iv_div(a, load_iv(S, ins.b, lane), t, ok);
+    case TOp::DivImm: return iv_div_imm(a, ins.imm, t, ok);
+    case TOp::ImmDiv: return iv_imm_div(ins.imm, a, t, ok);
+    case TOp::PowN: return iv_pown(a, static_cast<int>(ins.imm), t, ok);
+    case TOp::Sqrt: return iv_sqrt(a, t, ok);
+    case TOp::Exp: return iv_exp(a, t);
+    case TOp::Log: return iv_log(a, t, ok);
+    case TOp::Sin: return iv_sincos(a, 0, t);
+    case TOp::Cos: return iv_sincos(a, 1, t);
+    case TOp::Tanh: return iv_tanh(a, t);
+    case TOp::Abs: return iv_abs(a);
+    case TOp::Min: return iv_min(a, load_iv(S, ins.b, lane));
+    case TOp::MinImm: return iv_min(a, iv_const(ins.imm, t));
+    case TOp::Max: return iv_max(a, load_iv(S, ins.b, lane));
+    case TOp::MaxImm: return iv_max(a, iv_const(ins.imm, t));
+    default: return a;
+  }
+}
+
 // Interval tape code[0, n_instr) over LDS slots S[slot * kTile + lane] holding
 // (-lb, ub), and the accumulator register. Must run inside the round-up region
 // (token t). Output instructions store [lb, ub] into roots[b*stride + e] when
 // roots is non-null.
+// With Ext the extended op set is interpreted too.
+template<bool Ext>
 __device__ __forceinline__ IntervalResult run_interval(ConstInstrPtr code, std::uint32_t n_instr, double2* S,
                                                        unsigned lane, int t, double2* roots,
                                                        std::uint64_t roots_stride, std::uint64_t e)
@@ -203,7 +230,7 @@ __device__ __forceinline__ IntervalResult run_interval(ConstInstrPtr code, std::
       }
       continue;
     }
-    if (op >= static_cast<std::uint32_t>(TOp::AccStore)) {
+    if (op >= static_cast<std::uint32_t>(TOp::AccStore) && op < tape::kExtBase) {
       switch (static_cast<TOp>(op)) {
         case TOp::AccStore: store_iv(S, ins.dst, lane, acc); break;
         case TOp::AccAddImm: acc = iv_add_imm(acc, ins.imm, t); break;
@@ -224,6 +251,15 @@ __device__ __forceinline__ IntervalResult run_interval(ConstInstrPtr code, std::
       continue;
     }
     Iv const a = load_iv(S, ins.a, lane);
+    if constexpr (Ext) {
+      if (op >= tape::kExtBase && op < tape::kExtEnd) {
+        bool ok = true;
+        Iv const r = ext_interval(op, a, S, ins, lane, t, ok);
+        res.feasible = res.feasible && ok;
+        store_iv(S, ins.dst, lane, r);
+        continue;
+      }
+    }
     Iv r;
     switch (static_cast<TOp>(op)) {
       case TOp::LoadConst: r = iv_const(ins.imm, t); break;
@@ -273,8 +309,33 @@ struct PointResult
   bool feasible;
 };
 
+// One word of the extended op set (round to nearest).
+__device__ __forceinline__ double ext_point(std::uint32_t op, double a, const double* P, const Word& ins,
+                                            unsigned lane)
+{
+  switch (static_cast<TOp>(op)) {
+    case TOp::Div: return a / P[ins.b * kTile + lane];
+    case TOp::DivImm: return a / ins.imm;
+    case TOp::ImmDiv: return ins.imm / a;
+    case TOp::PowN: return pt_pown(a, static_cast<int>(ins.imm));
+    case TOp::Sqrt: return __builtin_sqrt(a);
+    case TOp::Exp: return pt_exp(a);
+    case TOp::Log: return pt_log(a);
+    case TOp::Sin: return pt_sincos(a, 0);
+    case TOp::Cos: return pt_sincos(a, 1);
+    case TOp::Tanh: return pt_tanh(a);
+    case TOp::Abs: return __builtin_fabs(a);
+    case TOp::Min: return __builtin_fmin(a, P[ins.b * kTile + lane]);
+    case TOp::MinImm: return __builtin_fmin(a, ins.imm);
+    case TOp::Max: return __builtin_fmax(a, P[ins.b * kTile + lane]);
+    case TOp::MaxImm: return __builtin_fmax(a, ins.imm);
+    default: return a;
+  }
+}
+
 // Round-to-nearest point tape over LDS slots P[slot * kTile + lane] and the
 // accumulator register.
+template<bool Ext>
 __device__ __forceinline__ PointResult run_point(ConstInstrPtr code, std::uint32_t n_instr, double* P, unsigned lane,
                                                  double* roots, std::uint64_t roots_stride, std::uint64_t e)
 {
@@ -312,7 +373,7 @@ __device__ __forceinline__ PointResult run_point(ConstInstrPtr code, std::uint32
       }
       continue;
     }
-    if (op >= static_cast<std::uint32_t>(TOp::AccStore)) {
+    if (op >= static_cast<std::uint32_t>(TOp::AccStore) && op < tape::kExtBase) {
       switch (static_cast<TOp>(op)) {
         case TOp::AccStore: P[ins.dst * kTile + lane] = acc; break;
         case TOp::AccAddImm: acc = acc + ins.imm; break;
@@ -333,6 +394,12 @@ __device__ __forceinline__ PointResult run_point(ConstInstrPtr code, std::uint32
       continue;
     }
     double const a = P[ins.a * kTile + lane];
+    if constexpr (Ext) {
+      if (op >= tape::kExtBase && op < tape::kExtEnd) {
+        P[ins.dst * kTile + lane] = ext_point(op, a, P, ins, lane);
+        continue;
+      }
+    }
     double r;
     switch (static_cast<TOp>(op)) {
       case TOp::LoadConst: r = ins.imm; break;
@@ -389,6 +456,7 @@ __device__ __forceinline__ void wave_argmin(double& v, std::int64_t& i)
 // the same 64 boxes. Variables are loaded once per workgroup into the shared
 // slots; each wave runs its group's code; wave 0 combines the groups'
 // feasibility and writes the outputs.
+template<bool Ext>
 __global__ __launch_bounds__(kTile * kMaxGroups) void k_boxes(const Instr* __restrict__ code_g, GroupTable groups,
                                                                std::uint32_t n_vars, std::uint32_t n_slots,
                                                                const double* __restrict__ lb,
@@ -413,10 +481,10 @@ __global__ __launch_bounds__(kTile * kMaxGroups) void k_boxes(const Instr* __res
   __syncthreads();
   std::uint32_t const begin = groups.begin[wave];
   int const t = enter_round_up();
-  IntervalResult const res = run_interval(as_constant(code_g + begin), groups.begin[wave + 1] - begin, S, lane, t,
-                                          active ? roots : nullptr, n_boxes, e);
+  IntervalResult const res = run_interval<Ext>(as_constant(code_g + begin), groups.begin[wave + 1] - begin, S, lane,
+                                               t, active ? roots : nullptr, n_boxes, e);
   Bnd const obj = to_output(res.obj, t);
-  leave_round_up(t, obj.lb, obj.ub);
+  leave_round_up(t, obj.lb, obj.ub, res.feasible);
   // per-group results after the slots: feasibility [n_groups][64], objective [64]
   auto* feas_g = reinterpret_cast<std::uint32_t*>(S + n_slots * kTile);
   auto* obj_s = reinterpret_cast<double2*>(feas_g + kMaxGroups * kTile);
@@ -439,6 +507,7 @@ __global__ __launch_bounds__(kTile * kMaxGroups) void k_boxes(const Instr* __res
   }
 }
 
+template<bool Ext>
 __global__ __launch_bounds__(kTile) void k_points(const Instr* __restrict__ code_g, std::uint32_t n_instr,
                                                    std::uint32_t n_vars, const double* __restrict__ x,
                                                    std::uint64_t ld, std::uint64_t n_points,
@@ -453,7 +522,8 @@ __global__ __launch_bounds__(kTile) void k_points(const Instr* __restrict__ code
   for (std::uint32_t v = 0; v < n_vars; ++v) {
     P[v * kTile + lane] = x[v * ld + ee];
   }
-  PointResult const res = run_point(as_constant(code_g), n_instr, P, lane, active ? roots : nullptr, n_points, e);
+  PointResult const res =
+      run_point<Ext>(as_constant(code_g), n_instr, P, lane, active ? roots : nullptr, n_points, e);
   if (active) {
     feasible[e] = res.feasible ? 1 : 0;
     obj[e] = res.obj;
@@ -461,7 +531,7 @@ __global__ __launch_bounds__(kTile) void k_points(const Instr* __restrict__ code
 }
 
 // Decode-mode item kernels. Each workgroup is one 64-element tile of one item.
-template<int M>
+template<int M, bool Ext>
 __global__ __launch_bounds__(kTile) void k_items(const Item* __restrict__ items,
                                                   const std::uint32_t* __restrict__ tile_item)
 {
@@ -494,9 +564,9 @@ __global__ __launch_bounds__(kTile) void k_items(const Item* __restrict__ items,
       Sl[vid * kTile + lane] = make_double2(-b.lo, b.hi);
     }
     int const t = enter_round_up();
-    IntervalResult const res = run_interval(code, it.n_instr, Sl, lane, t, nullptr, 0, 0);
+    IntervalResult const res = run_interval<Ext>(code, it.n_instr, Sl, lane, t, nullptr, 0, 0);
     Bnd const obj = to_output(res.obj, t);
-    leave_round_up(t, obj.lb, obj.ub);
+    leave_round_up(t, obj.lb, obj.ub, res.feasible);
     if constexpr (M == static_cast<int>(Mode::Bound)) {
       if (active) {
         it.feasible[region] = res.feasible ? 1 : 0;
@@ -547,9 +617,10 @@ __global__ __launch_bounds__(kTile) void k_items(const Item* __restrict__ items,
       Bounds const b = slot_bounds(it.slot_kind[j], {it.parent[2 * vid], it.parent[2 * vid + 1]}, part, w);
       P[vid * kTile + lane] = sampling ? draw(it.var_kinds[vid] != 0, b.lo, b.hi, vid, sample, region, it.salt) : b.lo;
     }
-    PointResult const res = run_point(code, it.n_instr, P, lane, nullptr, 0, 0);
+    PointResult const res = run_point<Ext>(code, it.n_instr, P, lane, nullptr, 0, 0);
+    // a point needs a finite objective (log(0) = -inf is no incumbent)
     double v = (active && res.feasible) ? res.obj : INFINITY;
-    if (v != v) {
+    if (v != v || v == -INFINITY) {
       v = INFINITY;
     }
     std::int64_t i = active ? static_cast<std::int64_t>(e) : INT64_MAX;
@@ -642,7 +713,59 @@ std::size_t boxes_lds_bytes(std::uint32_t n_slots)
 
 std::size_t lds_limit_bytes() { return static_cast<std::size_t>(lds_optin_bytes()); }
 
-void launch_items(Mode mode, const Item* items, const std::uint32_t* tile_item, std::uint32_t n_tiles,
+namespace
+{
+
+template<int M, bool Ext>
+void launch_items_as(const Item* items, const std::uint32_t* tile_item, std::uint32_t n_tiles, std::size_t bytes,
+                     hipStream_t stream)
+{
+  static bool once = (allow_large_lds(k_items<M, Ext>), true);
+  (void)once;
+  hipLaunchKernelGGL((k_items<M, Ext>), dim3(n_tiles), dim3(kTile), bytes, stream, items, tile_item);
+}
+
+template<bool Ext>
+void launch_items_mode(Mode mode, const Item* items, const std::uint32_t* tile_item, std::uint32_t n_tiles,
+                       std::size_t bytes, hipStream_t stream)
+{
+  switch (mode) {
+    case Mode::Bound: launch_items_as<0, Ext>(items, tile_item, n_tiles, bytes, stream); break;
+    case Mode::Sample: launch_items_as<1, Ext>(items, tile_item, n_tiles, bytes, stream); break;
+    case Mode::Enumerate: launch_items_as<2, Ext>(items, tile_item, n_tiles, bytes, stream); break;
+    case Mode::Aggregate: launch_items_as<3, Ext>(items, tile_item, n_tiles, bytes, stream); break;
+  }
+}
+
+template<bool Ext>
+void launch_boxes_as(const Instr* code, const GroupTable& groups, std::uint32_t n_slots, std::uint32_t n_vars,
+                     const double* lb, const double* ub, std::uint64_t ld, std::uint64_t n_boxes,
+                     std::uint8_t* feasible, double* obj_lb, double* obj_ub, double* roots, hipStream_t stream)
+{
+  static bool once = (allow_large_lds(k_boxes<Ext>), true);
+  (void)once;
+  std::uint64_t const blocks = (n_boxes + kTile - 1) / kTile;
+  hipLaunchKernelGGL(k_boxes<Ext>, dim3(static_cast<unsigned>(blocks)), dim3(kTile * groups.n_groups),
+                     boxes_lds_bytes(n_slots), stream, code, groups, n_vars, n_slots, lb, ub, ld, n_boxes, feasible,
+                     obj_lb, obj_ub, reinterpret_cast<double2*>(roots));
+}
+
+template<bool Ext>
+void launch_points_as(const Instr* code, std::uint32_t n_instr, std::uint32_t n_slots, std::uint32_t n_vars,
+                      const double* x, std::uint64_t ld, std::uint64_t n_points, std::uint8_t* feasible,
+                      double* obj, double* roots, hipStream_t stream)
+{
+  static bool once = (allow_large_lds(k_points<Ext>), true);
+  (void)once;
+  std::uint64_t const blocks = (n_points + kTile - 1) / kTile;
+  std::size_t const bytes = static_cast<std::size_t>(n_slots) * kTile * 8;
+  hipLaunchKernelGGL(k_points<Ext>, dim3(static_cast<unsigned>(blocks)), dim3(kTile), bytes, stream, code, n_instr,
+                     n_vars, x, ld, n_points, feasible, obj, roots);
+}
+
+}  // namespace
+
+void launch_items(Mode mode, bool extended, const Item* items, const std::uint32_t* tile_item, std::uint32_t n_tiles,
                   std::uint32_t lds_slots, hipStream_t stream)
 {
   if (n_tiles == 0) {
@@ -650,62 +773,39 @@ void launch_items(Mode mode, const Item* items, const std::uint32_t* tile_item, 
   }
   bool const interval = mode == Mode::Bound || mode == Mode::Aggregate;
   std::size_t const bytes = static_cast<std::size_t>(lds_slots) * kTile * (interval ? 16 : 8);
-  switch (mode) {
-    case Mode::Bound: {
-      static bool once = (allow_large_lds(k_items<0>), true);
-      (void)once;
-      hipLaunchKernelGGL(k_items<0>, dim3(n_tiles), dim3(kTile), bytes, stream, items, tile_item);
-      break;
-    }
-    case Mode::Sample: {
-      static bool once = (allow_large_lds(k_items<1>), true);
-      (void)once;
-      hipLaunchKernelGGL(k_items<1>, dim3(n_tiles), dim3(kTile), bytes, stream, items, tile_item);
-      break;
-    }
-    case Mode::Enumerate: {
-      static bool once = (allow_large_lds(k_items<2>), true);
-      (void)once;
-      hipLaunchKernelGGL(k_items<2>, dim3(n_tiles), dim3(kTile), bytes, stream, items, tile_item);
-      break;
-    }
-    case Mode::Aggregate: {
-      static bool once = (allow_large_lds(k_items<3>), true);
-      (void)once;
-      hipLaunchKernelGGL(k_items<3>, dim3(n_tiles), dim3(kTile), bytes, stream, items, tile_item);
-      break;
-    }
+  if (extended) {
+    launch_items_mode<true>(mode, items, tile_item, n_tiles, bytes, stream);
+  } else {
+    launch_items_mode<false>(mode, items, tile_item, n_tiles, bytes, stream);
   }
 }
 
-void launch_boxes(const Instr* code, const GroupTable& groups, std::uint32_t n_slots, std::uint32_t n_vars,
-                  const double* lb, const double* ub, std::uint64_t ld, std::uint64_t n_boxes, std::uint8_t* feasible,
-                  double* obj_lb, double* obj_ub, double* roots, hipStream_t stream)
+void launch_boxes(const Instr* code, bool extended, const GroupTable& groups, std::uint32_t n_slots,
+                  std::uint32_t n_vars, const double* lb, const double* ub, std::uint64_t ld, std::uint64_t n_boxes,
+                  std::uint8_t* feasible, double* obj_lb, double* obj_ub, double* roots, hipStream_t stream)
 {
   if (n_boxes == 0) {
     return;
   }
-  static bool once = (allow_large_lds(k_boxes), true);
-  (void)once;
-  std::uint64_t const blocks = (n_boxes + kTile - 1) / kTile;
-  hipLaunchKernelGGL(k_boxes, dim3(static_cast<unsigned>(blocks)), dim3(kTile * groups.n_groups),
-                     boxes_lds_bytes(n_slots), stream, code, groups, n_vars, n_slots, lb, ub, ld, n_boxes, feasible,
-                     obj_lb, obj_ub, reinterpret_cast<double2*>(roots));
+  if (extended) {
+    launch_boxes_as<true>(code, groups, n_slots, n_vars, lb, ub, ld, n_boxes, feasible, obj_lb, obj_ub, roots, stream);
+  } else {
+    launch_boxes_as<false>(code, groups, n_slots, n_vars, lb, ub, ld, n_boxes, feasible, obj_lb, obj_ub, roots, stream);
+  }
 }
 
-void launch_points(const Instr* code, std::uint32_t n_instr, std::uint32_t n_slots, std::uint32_t n_vars,
-                   const double* x, std::uint64_t ld, std::uint64_t n_points, std::uint8_t* feasible, double* obj,
-                   double* roots, hipStream_t stream)
+void launch_points(const Instr* code, bool extended, std::uint32_t n_instr, std::uint32_t n_slots,
+                   std::uint32_t n_vars, const double* x, std::uint64_t ld, std::uint64_t n_points,
+                   std::uint8_t* feasible, double* obj, double* roots, hipStream_t stream)
 {
   if (n_points == 0) {
     return;
   }
-  static bool once = (allow_large_lds(k_points), true);
-  (void)once;
-  std::uint64_t const blocks = (n_points + kTile - 1) / kTile;
-  std::size_t const bytes = static_cast<std::size_t>(n_slots) * kTile * 8;
-  hipLaunchKernelGGL(k_points, dim3(static_cast<unsigned>(blocks)), dim3(kTile), bytes, stream, code, n_instr,
-                     n_vars, x, ld, n_points, feasible, obj, roots);
+  if (extended) {
+    launch_points_as<true>(code, n_instr, n_slots, n_vars, x, ld, n_points, feasible, obj, roots, stream);
+  } else {
+    launch_points_as<false>(code, n_instr, n_slots, n_vars, x, ld, n_points, feasible, obj, roots, stream);
+  }
 }
 
 }  // namespace cmi::dev

@@ -85,6 +85,79 @@ def sqr(e: Expr) -> Expr:
     return e * e
 
 
+def _unary(op: int, fold, e: Expr) -> Expr:
+    """A unary op, folded to a Const on a Const operand (model/problem.hpp)."""
+    if e._is_const():
+        return e.p.fixed(fold(e.p.nodes[e.id].constant))
+    return Expr(e.p, e.p._emit(op, (e.id,)))
+
+
+def _fold(fn):
+    def safe(v):
+        with np.errstate(all="ignore"):
+            return float(fn(np.float64(v)))
+    return safe
+
+
+def exp(e: Expr) -> Expr:
+    return _unary(_c.OP_EXP, _fold(np.exp), e)
+
+
+def log(e: Expr) -> Expr:
+    return _unary(_c.OP_LOG, _fold(np.log), e)
+
+
+def sqrt(e: Expr) -> Expr:
+    return _unary(_c.OP_SQRT, _fold(np.sqrt), e)
+
+
+def sin(e: Expr) -> Expr:
+    return _unary(_c.OP_SIN, _fold(np.sin), e)
+
+
+def cos(e: Expr) -> Expr:
+    return _unary(_c.OP_COS, _fold(np.cos), e)
+
+
+def tanh(e: Expr) -> Expr:
+    return _unary(_c.OP_TANH, _fold(np.tanh), e)
+
+
+def fabs(e: Expr) -> Expr:
+    return _unary(_c.OP_ABS, _fold(np.fabs), e)
+
+
+def pown(e: Expr, n: int) -> Expr:
+    """e**n for an integer n (Op::PowN)."""
+    if e._is_const():
+        return e.p.fixed(_fold(lambda v: np.power(v, float(n)))(e.p.nodes[e.id].constant))
+    return Expr(e.p, e.p._emit(_c.OP_POWN, (e.id,), int_exp=int(n)))
+
+
+def power(a: Expr, b) -> Expr:
+    """a**b for an expression exponent (Op::Pow)."""
+    b = a._wrap(b)
+    if a._is_const() and b._is_const():
+        return a.p.fixed(_fold(lambda v: np.power(v, a.p.nodes[b.id].constant))(a.p.nodes[a.id].constant))
+    return a._binary(_c.OP_POW, b)
+
+
+def _pair(a, b):
+    if not isinstance(a, Expr):
+        a = b._wrap(a)
+    return a, a._wrap(b)
+
+
+def minimum(a, b) -> Expr:
+    a, b = _pair(a, b)
+    return a._binary(_c.OP_MIN, b)
+
+
+def maximum(a, b) -> Expr:
+    a, b = _pair(a, b)
+    return a._binary(_c.OP_MAX, b)
+
+
 @dataclass
 class Problem:
     nodes: list = field(default_factory=list)
@@ -254,4 +327,5 @@ def satisfies(lhs: float, cmp: int, rhs: float) -> bool:
     return diff <= 1e-8 or diff <= 1e-6 * max(abs(lhs), abs(rhs))
 
 
-__all__ = ["Expr", "Problem", "ProblemArrays", "sqr", "evaluate_point", "evaluate_objective", "satisfies", "math"]
+__all__ = ["Expr", "Problem", "ProblemArrays", "sqr", "exp", "log", "sqrt", "sin", "cos", "tanh", "fabs", "pown",
+           "power", "minimum", "maximum", "evaluate_point", "evaluate_objective", "satisfies", "math"]

@@ -180,6 +180,8 @@ typedef struct
   int compiled_persistent;  /* compiled kernel form: 1 persistent, 0 direct */
   size_t compiled_wgs_per_cu;
   size_t compiled_waves;    /* waves per workgroup of the compiled kernel */
+  int extended;             /* the tape uses the extended op set (Div, PowN, Sqrt, Exp, Log, Sin, Cos, Tanh,
+                               Abs, Min, Max; Pow as exp(y log x)) */
 } cmi_tape_info;
 
 /* Kernel for explicit boxes: the tape interpreter, or the tape compiled to a

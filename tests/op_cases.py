@@ -5,6 +5,7 @@ device test."""
 from __future__ import annotations
 
 from cuminlpsearch_amd import capi as C
+from cuminlpsearch_amd import model as M
 from cuminlpsearch_amd.model import Problem
 
 
@@ -66,6 +67,37 @@ CASES = {
     "acc_forms": lambda: _separate(lambda p, x, y, z: x + 2.0 * y - 1.5 * z + 3.0 * (y * y) - x * x
                                    - 0.5 * (x * z) + y * z - z * z,
                                    lambda p, x, y, z: 2.0 * x - y * 0.5 + (x * y) * 1.5 - 2.0 * (z * z)),
+}
+
+
+# The extended op set: one case per word form, over boxes that reach every
+# domain case (partly or wholly outside the domain, zero inside or at an end).
+EXT_CASES = {
+    "div": _case(lambda p, x, y, z: x / y),
+    "div_pos": _case(lambda p, x, y, z: (x + y) / z),
+    "div_imm": _case(lambda p, x, y, z: x / 1.5),
+    "imm_div": _case(lambda p, x, y, z: 2.0 / y),
+    "pown_even": _case(lambda p, x, y, z: M.pown(x, 4) - y),
+    "pown_odd": _case(lambda p, x, y, z: M.pown(y, 3) + x),
+    "pown_neg_even": _case(lambda p, x, y, z: M.pown(y, -2)),
+    "pown_neg_odd": _case(lambda p, x, y, z: M.pown(x, -3) + z),
+    "pown_big": _case(lambda p, x, y, z: M.pown(z, 17) * 1e-6),
+    "sqrt": _case(lambda p, x, y, z: M.sqrt(x) + M.sqrt(z)),
+    "exp": _case(lambda p, x, y, z: M.exp(x * y) - z),
+    "log": _case(lambda p, x, y, z: M.log(y) + M.log(z)),
+    "sin": _case(lambda p, x, y, z: M.sin(3.0 * x) + y),
+    "cos": _case(lambda p, x, y, z: M.cos(x * z) - 0.5),
+    "tanh": _case(lambda p, x, y, z: M.tanh(x * y), cmp=C.EQ, rhs=0.25),
+    "abs": _case(lambda p, x, y, z: M.fabs(x) - M.fabs(y)),
+    "min": _case(lambda p, x, y, z: M.minimum(x, y)),
+    "min_imm": _case(lambda p, x, y, z: M.minimum(x, 0.5) + M.minimum(0.25, z)),
+    "max": _case(lambda p, x, y, z: M.maximum(x, y) * z),
+    "max_imm": _case(lambda p, x, y, z: M.maximum(y, -0.5) + M.maximum(1.0, x)),
+    "pow": _case(lambda p, x, y, z: M.power(z, x)),
+    "pow_const_exp": _case(lambda p, x, y, z: M.power(y, 2.5)),
+    "pow_const_base": _case(lambda p, x, y, z: M.power(p.fixed(1.5), x * z)),
+    "ext_mixed": _case(lambda p, x, y, z: M.exp(M.sin(x) / z) + M.log(z + M.fabs(y)) * M.tanh(x),
+                       second=lambda p, x, y, z: M.sqrt(z * z + x * x) - M.pown(y, 2)),
 }
 
 

@@ -92,6 +92,69 @@ def random_dag(seed: int, n_vars: int = 5, n_ops: int = 60, n_con: int = 3, with
     return p
 
 
+def random_ext_dag(seed: int, n_vars: int = 4, n_ops: int = 40, n_con: int = 3, lo: float = -3.0,
+                   hi: float = 3.0) -> Problem:
+    """A random DAG mixing the extended op set (division, powers, sqrt, exp,
+    log, sin, cos, tanh, abs, min, max, x**y) with the QCQP ops. Magnitudes are
+    kept in range by squashing: exp and powers take tanh'd or sin'd operands."""
+    from cuminlpsearch_amd import model as M
+    rng = np.random.default_rng(seed)
+    p = Problem()
+    xs = []
+    for _ in range(n_vars):
+        a = float(rng.uniform(lo, 0.5 * (lo + hi)))
+        b = float(rng.uniform(a + 0.1, hi))
+        xs.append(p.var(a, b))
+    pool = list(xs)
+    for _ in range(n_ops):
+        kind = rng.integers(0, 16)
+        a = pool[rng.integers(0, len(pool))]
+        b = pool[rng.integers(0, len(pool))]
+        c = float(np.round(rng.normal(), 3))
+        if kind == 0:
+            e = a + b
+        elif kind == 1:
+            e = a * b if a.id != b.id else a * a
+        elif kind == 2:
+            e = a / b if rng.random() < 0.7 else (a / (c if c != 0 else 1.5) if rng.random() < 0.5 else c / b)
+        elif kind == 3:
+            e = M.pown(M.tanh(a), int(rng.integers(-3, 6)))
+        elif kind == 4:
+            e = M.sqrt(a)
+        elif kind == 5:
+            e = M.exp(M.sin(a) * 2.0)
+        elif kind == 6:
+            e = M.log(a)
+        elif kind == 7:
+            e = M.sin(a * 1.7)
+        elif kind == 8:
+            e = M.cos(a - b)
+        elif kind == 9:
+            e = M.tanh(a)
+        elif kind == 10:
+            e = M.fabs(a) - 0.5
+        elif kind == 11:
+            e = M.minimum(a, b) if rng.random() < 0.5 else M.minimum(c, a)
+        elif kind == 12:
+            e = M.maximum(a, b) if rng.random() < 0.5 else M.maximum(a, c)
+        elif kind == 13:
+            e = M.power(M.fabs(a) + 0.5, M.sin(b))
+        elif kind == 14:
+            e = a - b
+        else:
+            e = c * a + b
+        pool.append(e)
+    tail = pool[n_vars:]
+    p.set_objective(tail[-1])
+    for k in range(n_con):
+        e = tail[rng.integers(0, len(tail))]
+        if k % 3 == 2:
+            p.add_constraint(e, C.EQ, float(np.round(rng.normal(), 2)))
+        else:
+            p.add_constraint(e, C.LE, float(np.round(rng.uniform(0, 5), 2)))
+    return p
+
+
 C1_SPEC = dict(n=4, m=2, cont_lb=-10.0, cont_ub=10.0, symmetric_pairs=True)
 C2_SPEC = dict(n=32, m=16, cont_lb=-5.0, cont_ub=5.0, pair_density=0.2, target_nodes=600)
 C2_SEED = 2026

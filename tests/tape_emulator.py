@@ -14,11 +14,15 @@ from __future__ import annotations
 
 import numpy as np
 
+from cuminlpsearch_amd import capi as _c
 from oracle import binding as orc
 
 OPS = dict(Nop=0, LoadConst=1, Add=2, AddImm=3, Sub=4, SubImm=5, ImmSub=6, Mul=7, MulImm=8, Sqr=9, Neg=10,
            CheckLE=11, CheckEQ=12, Objective=13, Output=14, AccStore=16, AccAddImm=17, AccSubImm=18,
-           AccImmSub=19, AccScale=20, AccNeg=21, AccCheckLE=22, AccCheckEQ=23, AccObjective=24, AccOutput=25)
+           AccImmSub=19, AccScale=20, AccNeg=21, AccCheckLE=22, AccCheckEQ=23, AccObjective=24, AccOutput=25,
+           Div=32, DivImm=33, ImmDiv=34, PowN=35, Sqrt=36, Exp=37, Log=38, Sin=39, Cos=40, Tanh=41, Abs=42,
+           Min=43, MinImm=44, Max=45, MaxImm=46)
+EXT_BASE, EXT_END = 32, 47
 TERM_BASE = 64
 TERM_S, TERM_SC, TERM_SS, TERM_Q, TERM_SSC, TERM_QC = range(6)
 MODE_SET, MODE_ADD, MODE_SUB = range(3)
@@ -54,8 +58,9 @@ def neg(a):
 def mul(x, y):
     xn, xu = x
     yn, yu = y
-    nl = np.maximum.reduce([ru_mul(-xn, yn), ru_mul(xn, yu), ru_mul(xu, yn), ru_mul(-xu, yu)])
-    ub = np.maximum.reduce([ru_mul(xn, yn), ru_mul(-xn, yu), ru_mul(-xu, yn), ru_mul(xu, yu)])
+    # max skipping NaN products (0 * inf), as the device's v_max_f64
+    nl = np.fmax.reduce([ru_mul(-xn, yn), ru_mul(xn, yu), ru_mul(xu, yn), ru_mul(-xu, yu)])
+    ub = np.fmax.reduce([ru_mul(xn, yn), ru_mul(-xn, yu), ru_mul(-xu, yn), ru_mul(xu, yu)])
     return nl, ub
 
 
@@ -69,13 +74,47 @@ def mul_imm(a, c):
 
 
 def sqr(a):
-    s = np.maximum(-a[0], np.minimum(a[1], 0.0))
-    m = np.maximum(np.abs(a[0]), np.abs(a[1]))
+    s = np.fmax(-a[0], np.fmin(a[1], 0.0))
+    m = np.fmax(np.abs(a[0]), np.abs(a[1]))
     return ru_mul(-s, s), ru_mul(m, m)
 
 
 def output(a):
-    return ru_add(-a[0], 0.0), ru_add(a[1], 0.0)
+    """[lb, ub] with +0 for a zero bound and the infinite bound for a NaN one."""
+    nl = np.where(np.isnan(a[0]), np.inf, a[0])
+    ub = np.where(np.isnan(a[1]), np.inf, a[1])
+    return ru_add(-nl, 0.0), ru_add(ub, 0.0)
+
+
+def check_le(a, imm):
+    return ~(-a[0] > imm)
+
+
+def check_eq(a, imm):
+    return ~(-a[0] > imm) & ~(imm > a[1])
+
+
+_EXT = {OPS["Div"]: _c.OP_DIV, OPS["DivImm"]: _c.OP_DIV, OPS["ImmDiv"]: _c.OP_DIV, OPS["PowN"]: _c.OP_POWN,
+        OPS["Sqrt"]: _c.OP_SQRT, OPS["Exp"]: _c.OP_EXP, OPS["Log"]: _c.OP_LOG, OPS["Sin"]: _c.OP_SIN,
+        OPS["Cos"]: _c.OP_COS, OPS["Tanh"]: _c.OP_TANH, OPS["Abs"]: _c.OP_ABS, OPS["Min"]: _c.OP_MIN,
+        OPS["MinImm"]: _c.OP_MIN, OPS["Max"]: _c.OP_MAX, OPS["MaxImm"]: _c.OP_MAX}
+
+
+def ext(op, a, b, imm):
+    """An extended-op word through the oracle's interval op: ((nl, ub), ok)."""
+    n = a[0].shape[0]
+    point = (np.full(n, imm), np.full(n, imm))
+    x = (-a[0], a[1])
+    y = None
+    if op in (OPS["Div"], OPS["Min"], OPS["Max"]):
+        y = (-b[0], b[1])
+    elif op in (OPS["DivImm"], OPS["MinImm"], OPS["MaxImm"]):
+        y = point
+    elif op == OPS["ImmDiv"]:
+        x, y = point, (-a[0], a[1])
+    lo, hi, ok = orc.iv_ext(_EXT[op], x[0], x[1], None if y is None else y[0], None if y is None else y[1],
+                            n_exp=int(imm) if op == OPS["PowN"] else 0)
+    return (-lo, hi), ok
 
 
 def run_group(words, slots, n, n_outputs, roots):
@@ -97,7 +136,7 @@ def run_group(words, slots, n, n_outputs, roots):
                  TERM_QC: lambda: mul_imm(sqr(a), imm)}[term]()
             acc = x if mode == MODE_SET else add(acc, x) if mode == MODE_ADD else sub(acc, x)
             continue
-        if op >= OPS["AccStore"]:
+        if OPS["AccStore"] <= op <= OPS["AccOutput"]:
             if op == OPS["AccStore"]:
                 slots[d_i] = acc
             elif op == OPS["AccAddImm"]:
@@ -111,9 +150,9 @@ def run_group(words, slots, n, n_outputs, roots):
             elif op == OPS["AccNeg"]:
                 acc = neg(acc)
             elif op == OPS["AccCheckLE"]:
-                feas &= -acc[0] <= imm
+                feas &= check_le(acc, imm)
             elif op == OPS["AccCheckEQ"]:
-                feas &= (-acc[0] <= imm) & (imm <= acc[1])
+                feas &= check_eq(acc, imm)
             elif op == OPS["AccObjective"]:
                 obj = acc
             elif op == OPS["AccOutput"]:
@@ -122,6 +161,11 @@ def run_group(words, slots, n, n_outputs, roots):
         if op == OPS["Nop"]:
             continue
         a = slots[a_i]
+        if EXT_BASE <= op < EXT_END:
+            r, ok = ext(op, a, slots[b_i] if op in (OPS["Div"], OPS["Min"], OPS["Max"]) else None, imm)
+            feas &= ok
+            slots[d_i] = r
+            continue
         if op == OPS["LoadConst"]:
             r = (np.full(n, -imm), np.full(n, imm))
         elif op == OPS["Add"]:
@@ -143,10 +187,10 @@ def run_group(words, slots, n, n_outputs, roots):
         elif op == OPS["Neg"]:
             r = neg(a)
         elif op == OPS["CheckLE"]:
-            feas &= -a[0] <= imm
+            feas &= check_le(a, imm)
             continue
         elif op == OPS["CheckEQ"]:
-            feas &= (-a[0] <= imm) & (imm <= a[1])
+            feas &= check_eq(a, imm)
             continue
         elif op == OPS["Objective"]:
             obj = a

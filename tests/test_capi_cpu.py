@@ -71,14 +71,28 @@ def test_validation_errors_map_to_status_codes():
     assert e.value.status == C.INVALID_DAG
 
 
-def test_unsupported_op_is_rejected_by_the_tape_compiler():
+def test_extended_ops_compile_to_an_extended_tape():
     p = Problem()
     x = p.var(1.0, 2.0)
     p.set_objective(x / (x + 1.0))
+    info, _ = cmi.EngineProblem.from_problem(p).tape_info()
+    assert info.extended == 1
+    q = Problem()
+    y = q.var(1.0, 2.0)
+    q.set_objective(y * y + y)
+    assert cmi.EngineProblem.from_problem(q).tape_info()[0].extended == 0
+
+
+def test_extended_unary_on_const_is_rejected():
+    p = Problem()
+    x = p.var(1.0, 2.0)
+    c = p.fixed(2.0)
+    e = cmi.Expr(p, p._emit(C.OP_EXP, (c.id,)))
+    p.set_objective(x + e)
     eng = cmi.EngineProblem.from_problem(p)
-    with pytest.raises(C.CmiError) as e:
+    with pytest.raises(C.CmiError) as err:
         eng.tape_info()
-    assert e.value.status == C.INVALID_CONFIG
+    assert err.value.status == C.INVALID_DAG
 
 
 def test_unary_on_const_is_rejected():

@@ -27,6 +27,8 @@ def problems():
     yield "c2", P.c2()
     for name in ("const_root", "shared_generic", "acc_forms", "eq_acc", "mul_imm_zero"):
         yield name, cmi.EngineProblem.from_problem(op_cases.CASES[name]())
+    for name in ("ext_mixed", "div", "pown_neg_odd", "sin", "log", "tanh", "pow"):
+        yield name, cmi.EngineProblem.from_problem(op_cases.EXT_CASES[name]())
 
 
 @pytest.mark.parametrize("persistent", [0, 1])

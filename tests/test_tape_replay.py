@@ -26,22 +26,26 @@ def check(eng, lb, ub, groups):
     assert same_bits(r_e, r_o)
 
 
-@pytest.mark.parametrize("name", sorted(op_cases.CASES))
+ALL_CASES = {**op_cases.CASES, **op_cases.EXT_CASES}
+
+
+@pytest.mark.parametrize("name", sorted(ALL_CASES))
 def test_op_case_replay_matches_oracle(name):
-    eng = cmi.EngineProblem.from_problem(op_cases.CASES[name]())
+    eng = cmi.EngineProblem.from_problem(ALL_CASES[name]())
     lb, ub = P.random_boxes(5, 3, 400, -3.0, 4.0, 1e-6, 6.0)
     check(eng, lb, ub, 1)
 
 
 def test_op_cases_reach_every_op_code():
     seen = set()
-    for build in op_cases.CASES.values():
+    for build in ALL_CASES.values():
         words, _, _ = cmi.EngineProblem.from_problem(build()).tape_words(1)
         seen |= {int(o) for o in words["op"]}
     generic = set(range(T.OPS["LoadConst"], T.OPS["Output"] + 1))
     acc = set(range(T.OPS["AccStore"], T.OPS["AccOutput"] + 1))
     terms = {T.TERM_BASE + 8 * m + t for m in range(3) for t in range(6)}
-    assert generic | acc | terms <= seen
+    ext = set(range(T.EXT_BASE, T.EXT_END))
+    assert generic | acc | terms | ext <= seen
 
 
 @pytest.mark.parametrize("seed", range(8))
@@ -91,3 +95,12 @@ def test_compile_words_rejects_small_buffer():
     buf = np.zeros(2, TAPE_WORD)
     rc = capi.lib().cmi_tape_compile_words(eng.handle, 1, capi.ptr(buf), 2, None, C.byref(info))
     assert rc == capi.SHAPE_MISMATCH
+
+
+@pytest.mark.parametrize("seed", range(10))
+@pytest.mark.parametrize("groups", [1, 3])
+def test_random_extended_dag_replay_matches_oracle(seed, groups):
+    p = P.random_ext_dag(seed, n_vars=3 + seed % 4, n_ops=30 + 5 * (seed % 3), n_con=5)
+    eng = cmi.EngineProblem.from_problem(p)
+    lb, ub = P.random_boxes(200 + seed, eng.info().n_vars, 300, -3.0, 3.0, 1e-9, 6.0)
+    check(eng, lb, ub, groups)
