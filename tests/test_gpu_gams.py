@@ -27,6 +27,8 @@ TABLE = json.loads((GOLDEN / "gams_corpus.json").read_text())["table"]
 # models whose DAGs use only Add/Sub/Mul/Sqr/Neg
 QCQP_MODELS = ["ex2_1_1", "st_e01", "nvs04", "autocorr_bern20-03", "alkyl", "house", "ex3_1_1", "ex5_2_2_case1",
                "st_bpaf1a", "ex14_1_2", "circle"]
+# models that need the extended op set (division, powers, sqrt, exp, log, sin, cos)
+EXT_MODELS = ["ex8_1_1", "ex4_1_2", "ex4_1_5", "ex14_1_1", "nvs01", "hs62", "chem", "ex6_1_1", "ex7_2_1", "ex8_6_2"]
 OPTS = dict(partition_num=4, enumerate_cap=4, max_cycle_size=2, sample_points=32, iter_limit=300)
 
 FIELDS = ("iterations", "fathomed", "bounded", "sampled", "enqueued", "pruned_infeasible", "pruned_dominated",
@@ -56,7 +58,7 @@ def consistent_with_published(name, out, tol=1e-6):
         assert out.upper_bound >= row["ref_dual"] - tol * max(1.0, abs(row["ref_dual"]))
 
 
-@pytest.mark.parametrize("name", QCQP_MODELS)
+@pytest.mark.parametrize("name", QCQP_MODELS + EXT_MODELS)
 def test_solve_matches_oracle(ctx, name):
     pm = parsed(name)
     print(f"solving {name}", flush=True)
@@ -74,6 +76,14 @@ def test_solve_matches_oracle(ctx, name):
     # all 20 binaries enumerated in one launch of 2^20 regions
     ("autocorr_bern20-03", dict(partition_num=2, enumerate_cap=2, max_cycle_size=20, sample_points=1, iter_limit=10),
      -72.0),
+    # sin, cos and division
+    ("ex8_1_1", dict(partition_num=4, enumerate_cap=4, max_cycle_size=2, sample_points=32, iter_limit=3000),
+     -2.021806783),
+    # a degree-50 polynomial in PowN terms
+    ("ex4_1_2", dict(partition_num=8, enumerate_cap=8, max_cycle_size=1, sample_points=32, iter_limit=2000),
+     -663.5000966),
+    # integers, sqrt and division
+    ("nvs01", OPTS, 12.46966882),
 ])
 def test_solved_to_the_published_optimum(ctx, name, opts, optimum):
     pm = parsed(name)
@@ -101,11 +111,12 @@ def test_ex2_1_1_brackets_its_optimum(ctx):
 
 
 def test_corpus_batch_equals_plain(ctx):
-    """The whole set as one batch of uneven instances (2 to 20 variables, 0 to
-    10 constraints)."""
-    models = [parsed(n).problem for n in QCQP_MODELS]
+    """The whole set as one batch of uneven instances (1 to 24 variables, 0 to
+    14 constraints, with and without the extended op set in the same launch)."""
+    names = QCQP_MODELS + EXT_MODELS
+    models = [parsed(n).problem for n in names]
     outs, bests = cmi.solve_batch(ctx, models, **OPTS)
-    for name, p, d, b in zip(QCQP_MODELS, models, outs, bests):
+    for name, p, d, b in zip(names, models, outs, bests):
         s, sb = cmi.solve(ctx, p, **OPTS)
         same_outcome(d, s)
         assert np.array_equal(b.view(np.uint64), sb.view(np.uint64)), name
