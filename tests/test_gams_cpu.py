@@ -570,3 +570,11 @@ def test_result_line():
     for v in (1.0 / 3.0, -1e-300, 6.02214076e23, 12.46966882):
         line = gams.result_line(gams.MINIMISE, v, v)
         assert float(line.split("primal=")[1].split("\t")[0]) == v  # 17 digits round-trip
+
+
+def test_gams_solve_cli_reports_parse_errors(tmp_path, capsys):
+    from cuminlpsearch_amd import gams_solve
+    bad = tmp_path / "bad.gms"
+    bad.write_text("Set i /1*3/;\n")
+    assert gams_solve.main([str(bad), "10"]) == 1
+    assert "parse error: line 1:" in capsys.readouterr().err

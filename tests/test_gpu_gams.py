@@ -137,3 +137,14 @@ def test_maximise_result_line_reports_the_files_sense(ctx):
     assert line.startswith("RESULT\tsense=max\t")
     assert primal <= 4.0 + 1e-9 <= dual + 2e-9
     assert abs(primal - 4.0) < 1e-3
+
+
+def test_gams_solve_cli(capsys):
+    from cuminlpsearch_amd import gams_solve
+    rc = gams_solve.main([str(GOLDEN / "gams" / "st_e01.gms"), "300"])
+    out = capsys.readouterr().out
+    assert rc == 0
+    line = [ln for ln in out.splitlines() if ln.startswith("RESULT\t")][0]
+    assert line.startswith("RESULT\tsense=min\tprimal=")
+    assert abs(float(line.split("primal=")[1].split("\t")[0]) + 6.666666667) < 1e-6
+    assert "  x1 = " in out and "  x2 = " in out
