@@ -146,5 +146,5 @@ def test_gams_solve_cli(capsys):
     assert rc == 0
     line = [ln for ln in out.splitlines() if ln.startswith("RESULT\t")][0]
     assert line.startswith("RESULT\tsense=min\tprimal=")
-    assert abs(float(line.split("primal=")[1].split("\t")[0]) + 6.666666667) < 1e-6
+    assert abs(float(line.split("primal=")[1].split("\t")[0]) + 6.666666667) < 1e-5  # relative gap 1e-6
     assert "  x1 = " in out and "  x2 = " in out
