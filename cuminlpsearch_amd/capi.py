@@ -100,12 +100,13 @@ class AggregateBound(C.Structure):
 class CostModel(C.Structure):
     _fields_ = [("sampler_fixed", C.c_size_t), ("sampler_per_region", C.c_size_t),
                 ("bounder_fixed", C.c_size_t), ("bounder_per_region", C.c_size_t),
-                ("enumerator_fixed", C.c_size_t), ("enumerator_per_region", C.c_size_t)]
+                ("enumerator_fixed", C.c_size_t), ("enumerator_per_region", C.c_size_t),
+                ("max_regions", C.c_size_t)]
 
 
 class OverBudget(C.Structure):
     _fields_ = [("role", C.c_char * 32), ("needed_bytes", C.c_size_t), ("budget_bytes", C.c_size_t),
-                ("n_regions", C.c_size_t), ("bytes_per_element", C.c_size_t)]
+                ("n_regions", C.c_size_t), ("bytes_per_element", C.c_size_t), ("composition", C.c_char * 64)]
 
 
 class BoundView(C.Structure):
@@ -131,12 +132,17 @@ class SolveOutcome(C.Structure):
         ("proven_optimal", C.c_int), ("infeasible", C.c_int), ("found_incumbent", C.c_int),
         ("iterations", C.c_uint64), ("fathomed", C.c_uint64), ("bounded", C.c_uint64), ("sampled", C.c_uint64),
         ("enqueued", C.c_uint64), ("pruned_infeasible", C.c_uint64), ("pruned_dominated", C.c_uint64),
-        ("pending", C.c_uint64), ("wall_seconds", C.c_double),
+        ("pending", C.c_uint64), ("wall_seconds", C.c_double), ("has_over_budget", C.c_int),
+        ("over_budget", OverBudget),
     ]
 
     def as_dict(self) -> dict:
         d = {name: getattr(self, name) for name, _ in self._fields_}
         d["stop_reason"] = STOP_NAMES[self.stop_reason]
+        ob = self.over_budget
+        d["over_budget"] = ({name: (v.decode() if isinstance(v, bytes) else v)
+                             for name, v in ((n, getattr(ob, n)) for n, _ in ob._fields_)}
+                            if self.has_over_budget else None)
         return d
 
 
@@ -146,7 +152,7 @@ class ServerOptions(C.Structure):
 
 class ServeStats(C.Structure):
     _fields_ = [("steps", C.c_uint64), ("launches", C.c_uint64), ("peak_active", C.c_size_t),
-                ("wall_seconds", C.c_double)]
+                ("wall_seconds", C.c_double), ("peak_resident_bytes", C.c_size_t)]
 
 
 class GamsOptions(C.Structure):
@@ -160,6 +166,27 @@ class GamsError(C.Structure):
 class GamsInfo(C.Structure):
     _fields_ = [("sense", C.c_int), ("n_vars", C.c_size_t), ("n_constraints", C.c_size_t),
                 ("n_warnings", C.c_size_t), ("objvar_kept", C.c_int), ("has_initial_point", C.c_int)]
+
+
+class ProblemProfile(C.Structure):
+    _fields_ = [("num_binary", C.c_size_t), ("num_integer", C.c_size_t), ("num_continuous", C.c_size_t),
+                ("largest_integer_domain", C.c_size_t), ("objvar_kept", C.c_int)]
+
+
+SOURCE_AUTO, SOURCE_NAMED, SOURCE_OVERRIDDEN = 0, 1, 2
+SOURCE_NAMES = {SOURCE_AUTO: "auto", SOURCE_NAMED: "named", SOURCE_OVERRIDDEN: "overridden"}
+
+
+class ResolveRequest(C.Structure):
+    _fields_ = [("rules", C.c_char_p), ("objvar_kept", C.c_int), ("free_device_bytes", C.c_size_t),
+                ("partition_num", C.c_size_t), ("enumerate_cap", C.c_size_t), ("sample_points", C.c_size_t),
+                ("max_cycle_size", C.c_size_t)]
+
+
+class ResolvedShape(C.Structure):
+    _fields_ = [("partition_num", C.c_size_t), ("enumerate_cap", C.c_size_t), ("sample_points", C.c_size_t),
+                ("max_cycle_size", C.c_size_t), ("budget_bytes", C.c_size_t), ("footprint_bytes", C.c_size_t),
+                ("source", C.c_int), ("rules", C.c_char * 32)]
 
 
 class CmiError(RuntimeError):
@@ -226,6 +253,15 @@ _SIGNATURES = {
     "cmi_gams_destroy": (None, [_P]),
     "cmi_gams_result_line": (C.c_int, [C.c_int, C.c_double, C.c_double, C.c_char_p, C.c_size_t,
                                        C.POINTER(C.c_size_t)]),
+    "cmi_problem_profile_get": (C.c_int, [_P, C.POINTER(ProblemProfile)]),
+    "cmi_ctx_memory": (C.c_int, [_P, C.POINTER(C.c_size_t), C.POINTER(C.c_size_t)]),
+    "cmi_worst_footprint": (C.c_int, [_P, C.POINTER(ProblemProfile), C.c_size_t, C.c_size_t, C.c_size_t, C.c_size_t,
+                                      C.POINTER(C.c_size_t)]),
+    "cmi_auto_max_cycle_size": (C.c_int, [_P, C.POINTER(ProblemProfile), C.c_size_t, C.c_size_t, C.c_size_t,
+                                          C.c_size_t, C.c_size_t, C.POINTER(C.c_size_t)]),
+    "cmi_explain_over_budget": (C.c_int, [_P, C.c_int, C.POINTER(OverBudget), C.c_size_t, C.c_size_t, C.c_size_t,
+                                          C.c_char_p, C.c_size_t, C.POINTER(C.c_size_t)]),
+    "cmi_resolve_shape": (C.c_int, [_P, C.POINTER(ResolveRequest), C.POINTER(ResolvedShape)]),
     "cmi_serve_plain": (C.c_int, [_P, C.c_size_t, C.POINTER(_P), _P, C.POINTER(SolveOptions),
                                   C.POINTER(SolveOutcome), _P, _P, C.POINTER(ServeStats)]),
 }

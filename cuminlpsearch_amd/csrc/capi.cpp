@@ -15,6 +15,7 @@
 
 #include "batch.hpp"
 #include "jit.hpp"
+#include "cmi/config.hpp"
 #include "cmi/gams.hpp"
 #include "cmi/model.hpp"
 #include "cmi/policy.hpp"
@@ -162,6 +163,17 @@ cmi::region::Composition composition_of(const std::uint8_t* kinds, std::size_t n
   return c;
 }
 
+void fill_facts(const cmi::backend::OverBudget& f, cmi_over_budget* out)
+{
+  std::memset(out, 0, sizeof(*out));
+  std::strncpy(out->role, f.role.c_str(), sizeof(out->role) - 1);
+  out->needed_bytes = f.needed_bytes;
+  out->budget_bytes = f.budget_bytes;
+  out->n_regions = f.n_regions;
+  out->bytes_per_element = f.bytes_per_element;
+  std::strncpy(out->composition, f.composition.c_str(), sizeof(out->composition) - 1);
+}
+
 void fill_outcome(const cmi::search::SolveOutcome& o, double seconds, cmi_solve_outcome* out, double* best,
                   std::size_t n_vars)
 {
@@ -180,6 +192,12 @@ void fill_outcome(const cmi::search::SolveOutcome& o, double seconds, cmi_solve_
   out->pruned_dominated = o.counters.pruned_dominated;
   out->pending = o.counters.pending;
   out->wall_seconds = seconds;
+  out->has_over_budget = o.over_budget ? 1 : 0;
+  if (o.over_budget) {
+    fill_facts(*o.over_budget, &out->over_budget);
+  } else {
+    std::memset(&out->over_budget, 0, sizeof(out->over_budget));
+  }
   if (best != nullptr) {
     for (std::size_t v = 0; v < n_vars; ++v) {
       best[v] = v < o.best_point.size() ? o.best_point[v] : __builtin_nan("");
@@ -703,6 +721,7 @@ cmi_status cmi_cost_model_get(const cmi_tape* t, size_t samples_per_region, cmi_
     out->bounder_per_region = m.bounder_per_region;
     out->enumerator_fixed = m.enumerator_fixed;
     out->enumerator_per_region = m.enumerator_per_region;
+    out->max_regions = m.max_regions;
   });
 }
 
@@ -733,12 +752,7 @@ cmi_status cmi_role_build(cmi_tape* t, int role_kind, const uint8_t* slot_kinds,
     return CMI_OK;
   } catch (const cmi::backend::OverBudgetError& e) {
     if (facts != nullptr) {
-      std::memset(facts, 0, sizeof(*facts));
-      std::strncpy(facts->role, e.facts().role.c_str(), sizeof(facts->role) - 1);
-      facts->needed_bytes = e.facts().needed_bytes;
-      facts->budget_bytes = e.facts().budget_bytes;
-      facts->n_regions = e.facts().n_regions;
-      facts->bytes_per_element = e.facts().bytes_per_element;
+      fill_facts(e.facts(), facts);
     }
     return fail(CMI_OVER_BUDGET, e.what());
   } catch (...) {
@@ -838,12 +852,7 @@ cmi_status cmi_aggregate_build(cmi_tape* t, const uint8_t* slot_kinds, size_t sl
     return CMI_OK;
   } catch (const cmi::backend::OverBudgetError& e) {
     if (facts != nullptr) {
-      std::memset(facts, 0, sizeof(*facts));
-      std::strncpy(facts->role, e.facts().role.c_str(), sizeof(facts->role) - 1);
-      facts->needed_bytes = e.facts().needed_bytes;
-      facts->budget_bytes = e.facts().budget_bytes;
-      facts->n_regions = e.facts().n_regions;
-      facts->bytes_per_element = e.facts().bytes_per_element;
+      fill_facts(e.facts(), facts);
     }
     return fail(CMI_OVER_BUDGET, e.what());
   } catch (...) {
@@ -1089,6 +1098,178 @@ cmi_status cmi_gams_result_line(int sense, double lower, double upper, char* buf
     buf[k] = '\0';
   }
   return CMI_OK;
+}
+
+// -- memory-fit resolver --------------------------------------------------------
+
+namespace
+{
+cmi::config::ProblemProfile profile_arg(const cmi_problem* p, const cmi_problem_profile* pp)
+{
+  if (pp == nullptr) {
+    return cmi::config::profile_of(p->problem);
+  }
+  cmi::config::ProblemProfile out;
+  out.num_binary = pp->num_binary;
+  out.num_integer = pp->num_integer;
+  out.num_continuous = pp->num_continuous;
+  out.largest_integer_domain = pp->largest_integer_domain;
+  out.objvar_kept = pp->objvar_kept != 0;
+  return out;
+}
+
+/// The device roles' cost coefficients, priced from the problem's host tape.
+cmi::backend::RegionCostModel host_cost(const cmi_problem* p, std::size_t sample_points)
+{
+  p->problem.validate();
+  return cmi::hip::cost_model_for(cmi::tape::compile(p->problem), std::max<std::size_t>(sample_points, 1));
+}
+}  // namespace
+
+cmi_status cmi_problem_profile_get(const cmi_problem* p, cmi_problem_profile* out)
+{
+  if (p == nullptr || out == nullptr) {
+    return fail(CMI_INVALID_HANDLE, "null argument");
+  }
+  return guarded([&] {
+    auto const pp = cmi::config::profile_of(p->problem);
+    out->num_binary = pp.num_binary;
+    out->num_integer = pp.num_integer;
+    out->num_continuous = pp.num_continuous;
+    out->largest_integer_domain = pp.largest_integer_domain;
+    out->objvar_kept = 0;
+  });
+}
+
+cmi_status cmi_ctx_memory(cmi_ctx* ctx, size_t* free_bytes, size_t* total_bytes)
+{
+  if (ctx == nullptr) {
+    return fail(CMI_INVALID_HANDLE, "null argument");
+  }
+  return guarded([&] {
+    ctx->ctx->activate();
+    std::size_t f = 0, t = 0;
+    cmi::hip::check(hipMemGetInfo(&f, &t), "hipMemGetInfo");
+    if (free_bytes != nullptr) {
+      *free_bytes = f;
+    }
+    if (total_bytes != nullptr) {
+      *total_bytes = t;
+    }
+  });
+}
+
+cmi_status cmi_worst_footprint(const cmi_problem* p, const cmi_problem_profile* profile, size_t cap,
+                               size_t partition_num, size_t enumerate_cap, size_t sample_points, size_t* bytes)
+{
+  if (p == nullptr || bytes == nullptr) {
+    return fail(CMI_INVALID_HANDLE, "null argument");
+  }
+  return guarded([&] {
+    cmi::region::FanOutSpec {partition_num, enumerate_cap};
+    *bytes = cmi::config::worst_footprint(cap, profile_arg(p, profile), host_cost(p, sample_points), partition_num,
+                                          enumerate_cap);
+  });
+}
+
+cmi_status cmi_auto_max_cycle_size(const cmi_problem* p, const cmi_problem_profile* profile, size_t partition_num,
+                                   size_t enumerate_cap, size_t sample_points, size_t budget_bytes, size_t ceiling,
+                                   size_t* out)
+{
+  if (p == nullptr || out == nullptr) {
+    return fail(CMI_INVALID_HANDLE, "null argument");
+  }
+  return guarded([&] {
+    cmi::region::FanOutSpec {partition_num, enumerate_cap};
+    *out = cmi::config::auto_max_cycle_size(profile_arg(p, profile), host_cost(p, sample_points), partition_num,
+                                            enumerate_cap, budget_bytes, ceiling);
+  });
+}
+
+cmi_status cmi_explain_over_budget(const cmi_problem* p, int objvar_kept, const cmi_over_budget* facts,
+                                   size_t partition_num, size_t enumerate_cap, size_t sample_points, char* buf,
+                                   size_t cap, size_t* len)
+{
+  if (p == nullptr || facts == nullptr) {
+    return fail(CMI_INVALID_HANDLE, "null argument");
+  }
+  return guarded([&] {
+    cmi::region::FanOutSpec {partition_num, enumerate_cap};
+    cmi::backend::OverBudget f;
+    f.role.assign(facts->role, strnlen(facts->role, sizeof(facts->role)));
+    f.needed_bytes = facts->needed_bytes;
+    f.budget_bytes = facts->budget_bytes;
+    f.n_regions = facts->n_regions;
+    f.bytes_per_element = facts->bytes_per_element;
+    f.composition.assign(facts->composition, strnlen(facts->composition, sizeof(facts->composition)));
+    cmi::config::ProblemProfile pp = cmi::config::profile_of(p->problem);
+    pp.objvar_kept = objvar_kept != 0;
+    std::string const msg = cmi::config::explain_over_budget(f, pp, host_cost(p, sample_points), partition_num,
+                                                             enumerate_cap);
+    if (len != nullptr) {
+      *len = msg.size();
+    }
+    if (buf != nullptr && cap > 0) {
+      std::size_t const k = std::min(cap - 1, msg.size());
+      std::memcpy(buf, msg.data(), k);
+      buf[k] = '\0';
+    }
+  });
+}
+
+cmi_status cmi_resolve_shape(const cmi_problem* p, const cmi_resolve_request* req, cmi_resolved_shape* out)
+{
+  if (p == nullptr || req == nullptr || out == nullptr) {
+    return fail(CMI_INVALID_HANDLE, "null argument");
+  }
+  return guarded([&] {
+    cmi::config::ProblemProfile pp = cmi::config::profile_of(p->problem);
+    pp.objvar_kept = req->objvar_kept != 0;
+    cmi::config::ShapeRules rules;
+    cmi::config::Source source = cmi::config::Source::Auto;
+    if (req->rules != nullptr) {
+      auto const found = cmi::config::lookup_rules(req->rules);
+      if (!found) {
+        std::string names;
+        for (const auto& r : cmi::config::kRoster) {
+          names += (names.empty() ? "" : ", ") + std::string(r.name);
+        }
+        throw cmi::InvalidConfiguration("unknown rules '" + std::string(req->rules) + "'; the roster has " + names);
+      }
+      if (!cmi::config::is_applicable(*found, pp)) {
+        throw cmi::InvalidConfiguration(
+            "rules '" + std::string(req->rules) + "' do not apply to a problem with " + std::to_string(pp.num_binary)
+            + " binary, " + std::to_string(pp.num_integer) + " integer and " + std::to_string(pp.num_continuous)
+            + " continuous variable(s)" + (pp.objvar_kept ? " (one continuous is the kept objective variable)" : ""));
+      }
+      rules = *found;
+      source = cmi::config::Source::Named;
+    } else {
+      rules = cmi::config::select_rules(pp);
+    }
+    cmi::config::Overrides ov;
+    ov.partition_num = req->partition_num;
+    ov.enumerate_cap = req->enumerate_cap;
+    ov.sample_points = req->sample_points;
+    ov.max_cycle_size = req->max_cycle_size;
+    auto const fitted =
+        cmi::config::resolve_shape(rules, pp, req->free_device_bytes, host_cost(p, rules.sample_points));
+    auto const s = cmi::config::apply_overrides(fitted, ov);
+    out->partition_num = s.partition_num;
+    out->enumerate_cap = s.enumerate_cap;
+    out->sample_points = s.sample_points;
+    out->max_cycle_size = s.max_cycle_size;
+    out->budget_bytes = req->free_device_bytes > 0
+        ? static_cast<std::size_t>(static_cast<double>(req->free_device_bytes) * cmi::config::kBudgetFraction)
+        : 0;
+    out->footprint_bytes = cmi::config::worst_footprint(std::min(s.max_cycle_size, cmi::config::kMaxSlots), pp,
+                                                        host_cost(p, s.sample_points), s.partition_num,
+                                                        s.enumerate_cap);
+    out->source = ov.any() ? CMI_SOURCE_OVERRIDDEN
+                           : (source == cmi::config::Source::Named ? CMI_SOURCE_NAMED : CMI_SOURCE_AUTO);
+    std::memset(out->rules, 0, sizeof(out->rules));
+    std::memcpy(out->rules, rules.name.data(), std::min(rules.name.size(), sizeof(out->rules) - 1));
+  });
 }
 
 }  // extern "C"

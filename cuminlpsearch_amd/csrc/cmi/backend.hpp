@@ -1,6 +1,7 @@
 #pragma once
 
 #include <cstddef>
+#include <limits>
 #include <cstdint>
 #include <memory>
 #include <span>
@@ -105,6 +106,10 @@ struct RegionCostModel
   std::size_t bounder_per_region = 0;
   std::size_t enumerator_fixed = 0;
   std::size_t enumerator_per_region = 0;
+  /// Most regions one role launch can index (the kernels address
+  /// regions x samples with 32-bit lanes); a wider composition never fits,
+  /// whatever the bytes.
+  std::size_t max_regions = std::numeric_limits<std::size_t>::max();
 };
 
 /// Facts of a build that did not fit (reference backend.hpp:236-287).

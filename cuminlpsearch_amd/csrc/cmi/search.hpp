@@ -421,6 +421,9 @@ struct SolveOutcome
   bool proven_optimal = false;
   bool infeasible = false;
   RunTelemetry counters;
+  /// Facts of the role build that stopped the search with DeviceMemory, if a
+  /// build over its budget did (reference driver.hpp:408-416 reports them).
+  std::optional<backend::OverBudget> over_budget;
 
   bool found_incumbent() const { return upper_bound < std::numeric_limits<double>::max(); }
 };
@@ -668,6 +671,13 @@ public:
 
   void set_graphs_built(std::uint64_t n) { t_.graphs_built = n; }
 
+  /// Stop with DeviceMemory, keeping the facts of the build that did not fit.
+  void stop_over_budget(backend::OverBudget facts)
+  {
+    over_budget_ = std::move(facts);
+    stop(StopReason::DeviceMemory);
+  }
+
   SolveOutcome finish()
   {
     if (!stopped_early_) {
@@ -693,6 +703,7 @@ public:
     out.proven_optimal = fin.proven_optimal;
     out.infeasible = fin.infeasible;
     out.counters = t_;
+    out.over_budget = over_budget_;
     return out;
   }
 
@@ -715,6 +726,7 @@ private:
   bool done_ = false;
   bool enumerate_capable_ = true;
   StopReason stop_ = StopReason::IterationLimit;
+  std::optional<backend::OverBudget> over_budget_;
   std::vector<interval> box_;
   region::SlotAssignment assignment_;
   std::size_t depth_ = 0;
@@ -776,6 +788,8 @@ public:
         backend::BoundResult const bounds = roles.bounder->bound(reg);
         st.on_subdivide(drawn, roles.sampler->n_samples(), bounds);
       }
+    } catch (const backend::OverBudgetError& e) {
+      st.stop_over_budget(e.facts());
     } catch (const ResourceExhausted&) {
       st.stop(StopReason::DeviceMemory);
     } catch (const std::bad_alloc&) {

@@ -364,6 +364,7 @@ backend::RegionCostModel cost_model_for(const tape::Tape& t, std::size_t samples
   m.sampler_per_region = (samples * 16 + 63) / 64 + 1;
   m.enumerator_fixed = fixed + t.n_vars * 8;
   m.enumerator_per_region = 1;
+  m.max_regions = ((std::size_t(1) << 31) - 1) / std::max<std::size_t>(samples, 1);
   return m;
 }
 

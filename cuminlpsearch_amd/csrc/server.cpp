@@ -12,6 +12,7 @@
 
 #include "batch.hpp"
 #include "cmi.h"
+#include "cmi/config.hpp"
 #include "cmi/policy.hpp"
 #include "cmi/search.hpp"
 #include "handles.hpp"
@@ -42,27 +43,19 @@ cmi::search::SolveOptions solve_options(const cmi_solve_options& o)
   return so;
 }
 
-/// Device bytes one resident solve holds between steps plus what one step of
-/// it writes: the tape, its variable kinds, and its share of the step's
-/// staging/result/partial buffers for the widest composition the policy can
-/// produce.
-std::size_t resident_bytes(const cmi::hip::DeviceTape& t, const cmi_solve_options& o, std::size_t max_regions)
+/// Device bytes one resident solve can come to hold: its tape and variable
+/// kinds, plus the roles' bytes for the worst composition its search can
+/// reach at this fan-out and slot cap (the memory-fit resolver's footprint,
+/// so admission and shape fitting price a solve alike).
+std::size_t resident_bytes(const cmi::hip::DeviceTape& t, const cmi::model::Problem<double>& p,
+                           const cmi_solve_options& o)
 {
   std::size_t const n_vars = t.n_vars();
-  std::size_t const samples = std::max<std::size_t>(o.sample_points, 1);
-  return t.n_instr() * sizeof(cmi::tape::Instr) + n_vars + 3 * (512 + n_vars * 16) + max_regions * 9
-      + (max_regions * samples / 64 + 1) * 16 + n_vars * 8 + 64;
-}
-
-std::size_t max_regions_of(const cmi_solve_options& o, std::size_t n_vars)
-{
-  std::size_t const slots = std::min<std::size_t>(o.max_cycle_size == 0 ? 64 : o.max_cycle_size, n_vars);
-  std::size_t const w = std::max(o.partition_num, std::max<std::size_t>(o.enumerate_cap, 2));
-  std::size_t r = 1;
-  for (std::size_t j = 0; j < slots && r < (std::size_t(1) << 31); ++j) {
-    r *= w;
-  }
-  return r;
+  std::size_t const cap = std::min<std::size_t>(o.max_cycle_size == 0 ? cmi::config::kMaxSlots : o.max_cycle_size,
+                                                n_vars);
+  auto const cost = cmi::hip::cost_model_for(t.tape(), std::max<std::size_t>(o.sample_points, 1));
+  return t.n_instr() * sizeof(cmi::tape::Instr) + n_vars
+      + cmi::config::worst_footprint(cap, cmi::config::profile_of(p), cost, o.partition_num, o.enumerate_cap);
 }
 
 struct Active
@@ -117,7 +110,7 @@ extern "C" cmi_status cmi_serve_stream(cmi_ctx* ctx, size_t n, const cmi_problem
 
         cmi::batch::Runner runner(a.ctx->ctx);
         std::vector<Active> active;
-        std::size_t next = 0, completed = 0, used_bytes = 0, peak = 0;
+        std::size_t next = 0, completed = 0, used_bytes = 0, peak = 0, peak_bytes = 0;
         std::uint64_t steps = 0;
         std::vector<cmi::batch::Request> reqs;
         struct Owner
@@ -135,8 +128,7 @@ extern "C" cmi_status cmi_serve_stream(cmi_ctx* ctx, size_t n, const cmi_problem
           while (next < a.n && active.size() < a.so->max_active && arrive(order[next]) <= now) {
             std::size_t const i = order[next];
             auto tape = std::make_unique<cmi::hip::DeviceTape>(a.ctx->ctx, a.problems[i]->problem);
-            std::size_t const bytes =
-                resident_bytes(*tape, *a.o, max_regions_of(*a.o, a.problems[i]->problem.box_bounds.size()));
+            std::size_t const bytes = resident_bytes(*tape, a.problems[i]->problem, *a.o);
             if (a.so->hbm_budget_bytes != 0 && !active.empty() && used_bytes + bytes > a.so->hbm_budget_bytes) {
               break;  // wait for a resident solve to finish
             }
@@ -147,6 +139,7 @@ extern "C" cmi_status cmi_serve_stream(cmi_ctx* ctx, size_t n, const cmi_problem
             ++next;
           }
           peak = std::max(peak, active.size());
+          peak_bytes = std::max(peak_bytes, used_bytes);
           if (active.empty()) {
             if (next < a.n) {
               wait_until(t0, arrive(order[next]));
@@ -208,6 +201,7 @@ extern "C" cmi_status cmi_serve_stream(cmi_ctx* ctx, size_t n, const cmi_problem
           a.stats->launches = runner.launches();
           a.stats->peak_active = peak;
           a.stats->wall_seconds = elapsed();
+          a.stats->peak_resident_bytes = peak_bytes;
         }
       },
       &a);
@@ -274,6 +268,7 @@ extern "C" cmi_status cmi_serve_plain(cmi_ctx* ctx, size_t n, const cmi_problem*
           a.stats->launches = 0;
           a.stats->peak_active = a.n > 0 ? 1 : 0;
           a.stats->wall_seconds = elapsed();
+          a.stats->peak_resident_bytes = 0;
         }
       },
       &a);

@@ -200,6 +200,7 @@ typedef struct
   size_t sampler_fixed, sampler_per_region;
   size_t bounder_fixed, bounder_per_region;
   size_t enumerator_fixed, enumerator_per_region;
+  size_t max_regions; /* most regions one role launch indexes at this sample count */
 } cmi_cost_model;
 
 /* OverBudget facts, reference backend/backend.hpp:236-262 */
@@ -210,6 +211,7 @@ typedef struct
   size_t budget_bytes;
   size_t n_regions;
   size_t bytes_per_element;
+  char composition[64]; /* the composition's slot kinds, e.g. "C2 I1" (truncated if longer) */
 } cmi_over_budget;
 
 /* BoundResult, reference backend/backend.hpp:68-74 */
@@ -265,6 +267,8 @@ typedef struct
   uint64_t pruned_dominated;
   uint64_t pending;
   double wall_seconds;
+  int has_over_budget;         /* stop_reason is CMI_STOP_DEVICE_MEMORY because a role build did not fit */
+  cmi_over_budget over_budget; /* its facts (reference driver.hpp:408-416) */
 } cmi_solve_outcome;
 
 const char* cmi_last_error(void);
@@ -507,6 +511,73 @@ void cmi_gams_destroy(cmi_gams_model* m);
  * including the terminating NUL; *len (may be NULL) gets the full length. */
 cmi_status cmi_gams_result_line(int sense, double lower, double upper, char* buf, size_t cap, size_t* len);
 
+/* -- memory-fit resolver (reference config/problem_profile.hpp,
+ * config/footprint.hpp, config/catalogue.hpp, config/resolve.hpp,
+ * config/run_spec.hpp). Host arithmetic only: no device is touched. -------- */
+
+/* ProblemProfile, reference config/problem_profile.hpp:21-32 */
+typedef struct
+{
+  size_t num_binary, num_integer, num_continuous;
+  size_t largest_integer_domain; /* integer points in the widest integer box, 0 if none */
+  int objvar_kept;               /* a GAMS model whose objective variable stayed a variable */
+} cmi_problem_profile;
+
+enum
+{
+  CMI_SOURCE_AUTO = 0,      /* rules selected by the problem's variable kinds */
+  CMI_SOURCE_NAMED = 1,     /* rules named by the caller */
+  CMI_SOURCE_OVERRIDDEN = 2 /* at least one number given by the caller */
+};
+
+typedef struct
+{
+  const char* rules;        /* roster row ("all-binary", "discrete", "mixed-binary", "mixed-all-small",
+                               "mixed-all-large"), NULL = select by the problem's variable kinds */
+  int objvar_kept;          /* cmi_gams_info.objvar_kept of the model, 0 for a hand-built problem */
+  size_t free_device_bytes; /* free device memory to fit against, 0 = no budget (the rows' fallbacks) */
+  /* overrides, 0 = resolve */
+  size_t partition_num, enumerate_cap, sample_points, max_cycle_size;
+} cmi_resolve_request;
+
+typedef struct
+{
+  size_t partition_num, enumerate_cap, sample_points, max_cycle_size;
+  size_t budget_bytes;    /* what the fit spent against: 0.67 of free_device_bytes */
+  size_t footprint_bytes; /* device bytes of the worst composition the shape can reach */
+  int source;             /* CMI_SOURCE_* */
+  char rules[32];         /* the roster row used */
+} cmi_resolved_shape;
+
+/* profile_problem, reference config/problem_profile.hpp:43-70 (objvar_kept = 0). */
+cmi_status cmi_problem_profile_get(const cmi_problem* p, cmi_problem_profile* out);
+/* Free and total device memory of the context's device (hipMemGetInfo). */
+cmi_status cmi_ctx_memory(cmi_ctx* ctx, size_t* free_bytes, size_t* total_bytes);
+/* worst_composition_footprint, reference config/footprint.hpp:61-97: the
+ * device bytes (the roles' cost model, cmi_cost_model_get, at sample_points)
+ * of the largest composition any box of the search reaches with cap slots.
+ * profile may be NULL (then it is taken from p). */
+cmi_status cmi_worst_footprint(const cmi_problem* p, const cmi_problem_profile* profile, size_t cap,
+                               size_t partition_num, size_t enumerate_cap, size_t sample_points, size_t* bytes);
+/* auto_max_cycle_size, reference config/footprint.hpp:99-149: the widest
+ * slot cap, at most ceiling, whose worst composition fits budget_bytes; 0 if
+ * none does. */
+cmi_status cmi_auto_max_cycle_size(const cmi_problem* p, const cmi_problem_profile* profile, size_t partition_num,
+                                   size_t enumerate_cap, size_t sample_points, size_t budget_bytes, size_t ceiling,
+                                   size_t* out);
+/* explain_over_budget, reference config/resolve.hpp:247-330: a role build that
+ * did not fit (cmi_solve_outcome.over_budget, or cmi_role_build's facts),
+ * explained with the widest slot cap that fits its budget at the solve's
+ * fan-out, every role of a composition charged. Writes at most cap bytes
+ * including the NUL; *len (may be NULL) gets the full length. */
+cmi_status cmi_explain_over_budget(const cmi_problem* p, int objvar_kept, const cmi_over_budget* facts,
+                                   size_t partition_num, size_t enumerate_cap, size_t sample_points, char* buf,
+                                   size_t cap, size_t* len);
+/* select_policy + resolve_shape + resolve, reference config/catalogue.hpp:182-245,
+ * config/resolve.hpp:95-218, config/run_spec.hpp:122-166. An unknown or
+ * inapplicable named row is CMI_INVALID_CONFIG. */
+cmi_status cmi_resolve_shape(const cmi_problem* p, const cmi_resolve_request* req, cmi_resolved_shape* out);
+
 /* -- search ------------------------------------------------------------------- */
 
 void cmi_solve_options_default(cmi_solve_options* o);
@@ -537,10 +608,13 @@ typedef struct
   uint64_t launches;    /* kernel launches issued */
   size_t peak_active;   /* most solves resident at once */
   double wall_seconds;  /* first admission to last completion */
+  size_t peak_resident_bytes; /* largest sum of the resident solves' admission footprints (0 for serve_plain) */
 } cmi_serve_stats;
 
 /* Continuous batching: requests are admitted as they arrive while a slot and
- * the HBM budget are free; every step advances every resident solve by one
+ * the HBM budget are free (a solve is charged its tape plus the memory-fit
+ * resolver's worst-composition footprint at the options' shape, see
+ * cmi_worst_footprint; the first resident solve is always admitted); every step advances every resident solve by one
  * search iteration in one launch per role kind; a finished solve frees its
  * slot at once. finish[i] (seconds since the call started) and latency[i]
  * (finish - arrival) may be NULL. Per-request results equal cmi_solve's. */

@@ -141,7 +141,8 @@ def test_maximise_result_line_reports_the_files_sense(ctx):
 
 def test_gams_solve_cli(capsys):
     from cuminlpsearch_amd import gams_solve
-    rc = gams_solve.main([str(GOLDEN / "gams" / "st_e01.gms"), "300"])
+    rc = gams_solve.main([str(GOLDEN / "gams" / "st_e01.gms"), "300", "--partition-num", "4", "--enumerate-cap", "4",
+                          "--max-slots", "2", "--sample-points", "32"])
     out = capsys.readouterr().out
     assert rc == 0
     line = [ln for ln in out.splitlines() if ln.startswith("RESULT\t")][0]
