@@ -196,6 +196,8 @@ def main(argv=None) -> int:
                     help="compiled kernel: waves per workgroup (0 = one per function group)")
     ap.add_argument("--persistent", action="store_true",
                     help="compiled kernel: persistent double-buffered form instead of one workgroup per tile")
+    ap.add_argument("--wgs-per-cu", type=int, default=1,
+                    help="compiled kernel, persistent form: resident workgroups per CU (grid = CUs x this)")
     ap.add_argument("--sweep-groups", action="store_true",
                     help="print the interpreter's kernel time for each group count, and the compiled kernel's, "
                          "to stderr before the timed run")
@@ -255,7 +257,7 @@ def main(argv=None) -> int:
             tuning = tape.autotune(N_BOXES)
         else:
             tape.set_box_groups(a.box_groups)
-            tape.set_compiled_shape(a.persistent, a.waves, 1)
+            tape.set_compiled_shape(a.persistent, a.waves, a.wgs_per_cu)
             tape.set_box_engine(engines[a.box_engine])
     torch.cuda.synchronize()
     for k in range(a.warmup):
@@ -305,7 +307,7 @@ def main(argv=None) -> int:
             tuning = tape.autotune(N_BOXES)
         else:
             tape.set_box_groups(a.box_groups)
-            tape.set_compiled_shape(a.persistent, a.waves, 1)
+            tape.set_compiled_shape(a.persistent, a.waves, a.wgs_per_cu)
             tape.set_box_engine(engines[a.box_engine])
     info = tape.info()
     for k in range(3):

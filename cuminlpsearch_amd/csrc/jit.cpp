@@ -61,8 +61,39 @@ std::string mul_imm(const std::string& a, double c)
   return "Iv{0.0, 0.0}";
 }
 
-/// One tape word as one statement; `acc` names the group's accumulator.
-std::string emit_word(const Instr& w, const std::string& acc)
+/// The product a term word forms from variable slots before scaling and
+/// accumulating it, iv_mul(a, b) or iv_sqr(a), as a key (0: the word forms
+/// none, or reads a temporary slot, which the tape may rewrite). Variable
+/// slots are never written, so words with equal keys compute bit-identical
+/// intervals.
+std::uint64_t product_key(const Instr& w, std::uint32_t n_vars)
+{
+  if (!tape::is_term_op(w.op) || w.a >= n_vars) {
+    return 0;
+  }
+  switch (static_cast<Term>(w.op & 7u)) {
+    case Term::SS:
+    case Term::SSC: return w.b < n_vars ? (std::uint64_t {2} << 40) | (std::uint64_t {w.a} << 20) | w.b : 0;
+    case Term::Q:
+    case Term::QC: return (std::uint64_t {1} << 40) | w.a;
+    default: return 0;
+  }
+}
+
+/// The expression of the product a term word forms (product_key).
+std::string product_expr(const Instr& w)
+{
+  auto const kind = static_cast<Term>(w.op & 7u);
+  if (kind == Term::SS || kind == Term::SSC) {
+    return "iv_mul(" + slot(w.a) + ", " + slot(w.b) + ", t)";
+  }
+  return "iv_sqr(" + slot(w.a) + ", t)";
+}
+
+/// One tape word as one statement; `acc` names the group's accumulator. A
+/// term word whose product is already held in a variable takes its name as
+/// `prod` instead of forming the product again.
+std::string emit_word(const Instr& w, const std::string& acc, const std::string& prod = {})
 {
   std::string const a = slot(w.a), b = slot(w.b), d = slot(w.dst), c = lit(w.imm);
   auto set = [](const std::string& lhs, const std::string& rhs) { return "  " + lhs + " = " + rhs + ";\n"; };
@@ -77,10 +108,10 @@ std::string emit_word(const Instr& w, const std::string& acc)
     switch (static_cast<Term>(w.op & 7u)) {
       case Term::S: x = a; break;
       case Term::SC: x = mul_imm(a, w.imm); break;
-      case Term::SS: x = call("iv_mul", a, b); break;
-      case Term::Q: x = "iv_sqr(" + a + ", t)"; break;
-      case Term::SSC: x = mul_imm(call("iv_mul", a, b), w.imm); break;
-      case Term::QC: x = mul_imm("iv_sqr(" + a + ", t)", w.imm); break;
+      case Term::SS: x = prod.empty() ? call("iv_mul", a, b) : prod; break;
+      case Term::Q: x = prod.empty() ? "iv_sqr(" + a + ", t)" : prod; break;
+      case Term::SSC: x = mul_imm(prod.empty() ? call("iv_mul", a, b) : prod, w.imm); break;
+      case Term::QC: x = mul_imm(prod.empty() ? "iv_sqr(" + a + ", t)" : prod, w.imm); break;
     }
     switch (static_cast<AccMode>((w.op - tape::kTermBase) >> 3)) {
       case AccMode::Set: return set(acc, x);
@@ -171,6 +202,28 @@ const char* const kOptions[] = {
 namespace
 {
 
+/// Which slot fields of one tape word are read and written.
+struct WordUse
+{
+  bool reads_a, reads_b, writes;
+};
+
+WordUse word_use(const Instr& w)
+{
+  bool const term = tape::is_term_op(w.op);
+  auto const op = static_cast<TOp>(w.op);
+  unsigned const kind = w.op & 7u;
+  bool const ext = tape::is_ext_op(w.op);
+  WordUse u;
+  u.reads_a =
+      term || ext || (w.op >= static_cast<std::uint8_t>(TOp::Add) && w.op <= static_cast<std::uint8_t>(TOp::Output));
+  u.reads_b = (term && (kind == static_cast<unsigned>(Term::SS) || kind == static_cast<unsigned>(Term::SSC)))
+      || op == TOp::Add || op == TOp::Sub || op == TOp::Mul || op == TOp::Div || op == TOp::Min || op == TOp::Max;
+  u.writes = (w.op >= static_cast<std::uint8_t>(TOp::LoadConst) && w.op <= static_cast<std::uint8_t>(TOp::Neg))
+      || op == TOp::AccStore || ext;
+  return u;
+}
+
 /// Slots group code [lo, hi) reads before writing (its variables) and writes.
 void slot_use(const tape::Tape& t, std::uint32_t lo, std::uint32_t hi, std::set<std::uint32_t>& vars,
               std::set<std::uint32_t>& temps)
@@ -178,23 +231,14 @@ void slot_use(const tape::Tape& t, std::uint32_t lo, std::uint32_t hi, std::set<
   for (std::uint32_t i = lo; i < hi; ++i) {
     Instr const& w = t.code[i];
     auto note = [&](std::uint32_t s) { (s < t.n_vars ? vars : temps).insert(s); };
-    bool const term = tape::is_term_op(w.op);
-    auto const op = static_cast<TOp>(w.op);
-    unsigned const kind = w.op & 7u;
-    bool const ext = tape::is_ext_op(w.op);
-    bool const reads_a =
-        term || ext || (w.op >= static_cast<std::uint8_t>(TOp::Add) && w.op <= static_cast<std::uint8_t>(TOp::Output));
-    bool const reads_b = (term && (kind == static_cast<unsigned>(Term::SS) || kind == static_cast<unsigned>(Term::SSC)))
-        || op == TOp::Add || op == TOp::Sub || op == TOp::Mul || op == TOp::Div || op == TOp::Min || op == TOp::Max;
-    bool const writes = (w.op >= static_cast<std::uint8_t>(TOp::LoadConst) && w.op <= static_cast<std::uint8_t>(TOp::Neg))
-        || op == TOp::AccStore || ext;
-    if (reads_a) {
+    WordUse const u = word_use(w);
+    if (u.reads_a) {
       note(w.a);
     }
-    if (reads_b) {
+    if (u.reads_b) {
       note(w.b);
     }
-    if (writes) {
+    if (u.writes) {
       note(w.dst);
     }
   }
@@ -290,28 +334,92 @@ std::string boxes_source(const tape::Tape& t, const Shape& shape)
   // The groups share nothing they write, so the results are those of the
   // groups run one after another, while neighbouring statements are
   // independent (FP64 results take several issue slots to come back).
+  // A product that several term words of the wave form (a product node the
+  // DAG shares between functions) is formed once, at its first use in that
+  // order, and held for the others: the same operation on the same slots, so
+  // the same bits.
   for (std::uint32_t v = 0; v < w; ++v) {
     std::set<std::uint32_t> vars, temps;
-    std::vector<std::vector<std::string>> streams;
+    std::vector<std::vector<std::uint32_t>> words;
     bool has_obj = false;
     for (std::uint32_t g = v; g < k; g += w) {
       std::uint32_t const lo = t.group_begin.size() > g ? t.group_begin[g] : 0;
       std::uint32_t const hi =
           t.group_begin.size() > g + 1 ? t.group_begin[g + 1] : static_cast<std::uint32_t>(t.code.size());
       slot_use(t, lo, hi, vars, temps);
-      std::vector<std::string> st;
-      std::string const acc = "acc" + std::to_string(streams.size());
+      std::vector<std::uint32_t> st;
       for (std::uint32_t i = lo; i < hi; ++i) {
-        std::string line = emit_word(t.code[i], acc);
-        if (!line.empty()) {
-          st.push_back(ind + line);
-        }
+        st.push_back(i);
       }
-      streams.push_back(std::move(st));
+      words.push_back(std::move(st));
       has_obj = has_obj || g == t.objective_group;
     }
+    std::vector<std::pair<std::size_t, std::uint32_t>> order;  // (stream, word) in emission order
+    for (std::size_t i = 0;; ++i) {
+      bool any = false;
+      for (std::size_t j = 0; j < words.size(); ++j) {
+        if (i < words[j].size()) {
+          order.emplace_back(j, words[j][i]);
+          any = true;
+        }
+      }
+      if (!any) {
+        break;
+      }
+    }
+    std::map<std::uint64_t, std::uint32_t> uses;
+    for (auto const& [j, i] : order) {
+      if (std::uint64_t const key = product_key(t.code[i], nv)) {
+        ++uses[key];
+      }
+    }
+    std::map<std::uint64_t, std::string> held;
+    std::vector<std::string> body;
+    for (auto const& [j, i] : order) {
+      Instr const& word = t.code[i];
+      std::string const acc = "acc" + std::to_string(j);
+      std::uint64_t const key = product_key(word, nv);
+      std::string line;
+      if (key != 0 && uses[key] > 1) {
+        auto it = held.find(key);
+        if (it == held.end()) {
+          std::string const name = "p" + std::to_string(held.size());
+          body.push_back(ind + "  const Iv " + name + " = " + product_expr(word) + ";\n");
+          it = held.emplace(key, name).first;
+        }
+        line = emit_word(word, acc, it->second);
+      } else {
+        line = emit_word(word, acc);
+      }
+      if (!line.empty()) {
+        body.push_back(ind + line);
+      }
+    }
     o << ind << (v == 0 ? "if" : "else if") << " (wave == " << v << "u) {\n";
-    for (std::uint32_t x : vars) {
+    // variables in the order the wave's statements first read them, so the
+    // waits for their loads come due in the order the loads were issued
+    std::vector<std::uint32_t> first_use;
+    {
+      std::set<std::uint32_t> seen;
+      auto see = [&](std::uint32_t x) {
+        if (vars.count(x) != 0 && seen.insert(x).second) {
+          first_use.push_back(x);
+        }
+      };
+      for (auto const& [j, i] : order) {
+        WordUse const u = word_use(t.code[i]);
+        if (u.reads_a) {
+          see(t.code[i].a);
+        }
+        if (u.reads_b) {
+          see(t.code[i].b);
+        }
+      }
+      for (std::uint32_t x : vars) {
+        see(x);
+      }
+    }
+    for (std::uint32_t x : first_use) {
       if (persistent) {
         o << ind << "  const Iv " << slot(x) << " = {X[" << x * 64 << "u + lane].x, X[" << x * 64 << "u + lane].y};\n";
       } else {
@@ -321,21 +429,12 @@ std::string boxes_source(const tape::Tape& t, const Shape& shape)
     for (std::uint32_t x : temps) {
       o << ind << "  Iv " << slot(x) << ";\n";
     }
-    for (std::size_t j = 0; j < streams.size(); ++j) {
+    for (std::size_t j = 0; j < words.size(); ++j) {
       o << ind << "  Iv acc" << j << " = {0.0, 0.0};\n";
     }
     o << ind << "  Iv obj = {0.0, 0.0};\n";
-    for (std::size_t i = 0;; ++i) {
-      bool any = false;
-      for (auto const& st : streams) {
-        if (i < st.size()) {
-          o << st[i];
-          any = true;
-        }
-      }
-      if (!any) {
-        break;
-      }
+    for (std::string const& line : body) {
+      o << line;
     }
     if (has_obj) {
       o << ind << "  const Bnd ob = to_output(obj, t);\n";

@@ -81,3 +81,34 @@ def _c_hex(x: float) -> str:
     if "." in m:
         m = m.rstrip("0").rstrip(".")
     return f"{m}p{e}"
+
+
+def _wave_bodies(src: str):
+    body = src[src.index("cmi_boxes_compiled("):]
+    parts = re.split(r"(?:else )?if \(wave == \d+u\) \{", body)[1:]
+    return parts
+
+
+def test_shared_products_are_formed_once_per_wave():
+    """The QCQP's product nodes are shared between functions: a wave forms
+    each distinct product of variables once and reuses it (c2: 145 bilinear
+    and 48 square terms over 77 + 32 distinct products)."""
+    eng = P.c2()
+    one = _wave_bodies(eng.compiled_source(0, 1))
+    assert len(one) == 1
+    formed = len(re.findall(r"iv_mul\(s\d+, s\d+, t\)", one[0])) + len(re.findall(r"iv_sqr\(s\d+, t\)", one[0]))
+    assert formed == 109
+    # split over waves, each wave still forms each of its products once
+    for body in _wave_bodies(eng.compiled_source(0, 2)):
+        calls = re.findall(r"iv_mul\(s\d+, s\d+, t\)|iv_sqr\(s\d+, t\)", body)
+        assert len(calls) == len(set(calls))
+
+
+def test_products_of_rewritten_slots_are_not_shared():
+    """A term over a temporary slot is formed where it stands: the tape may
+    write that slot again between two such terms."""
+    eng = cmi.EngineProblem.from_problem(op_cases.CASES["shared_generic"]())
+    src = eng.compiled_source(0, 1)
+    n_vars = eng.info().n_vars
+    for m in re.finditer(r"const Iv p\d+ = iv_(?:mul\(s(\d+), s(\d+)|sqr\(s(\d+)), t\)", src):
+        assert all(int(x) < n_vars for x in m.groups() if x is not None)
