@@ -98,14 +98,10 @@ struct Bnd
   double lb, ub;
 };
 
-/// Box variable as (-lb, ub) from the double at byte offset `off` of the
-/// columns lb and ub. The sign flip is opaque to the compiler, so it is done
-/// once and not repeated at every use. With lb, ub uniform and a 32-bit byte
-/// offset, the loads take the scalar-base + vector-offset form.
-__device__ __forceinline__ Iv load_box(const double* lb, const double* ub, unsigned off)
+/// (-lb, ub) from the two doubles lb, ub. The sign flip is opaque to the
+/// compiler, so it is done once and not repeated at every use.
+__device__ __forceinline__ Iv box_iv(double l, double u)
 {
-  double const l = *reinterpret_cast<const double*>(reinterpret_cast<const char*>(lb) + off);
-  double const u = *reinterpret_cast<const double*>(reinterpret_cast<const char*>(ub) + off);
   unsigned long long const bits = __builtin_bit_cast(unsigned long long, l);
   unsigned hi = static_cast<unsigned>(bits >> 32);
   unsigned const lo = static_cast<unsigned>(bits);
@@ -114,6 +110,26 @@ __device__ __forceinline__ Iv load_box(const double* lb, const double* ub, unsig
   r.nl = __builtin_bit_cast(double, (static_cast<unsigned long long>(hi) << 32) | lo);
   r.ub = u;
   return r;
+}
+
+/// Box variable as (-lb, ub) from the double at byte offset `off` of the
+/// columns lb and ub. With lb, ub uniform and a 32-bit byte offset, the loads
+/// take the scalar-base + vector-offset form. A box variable that only one
+/// wave of a workgroup reads is read once per launch, so these loads are
+/// non-temporal (streamed past the caches).
+__device__ __forceinline__ Iv load_box(const double* lb, const double* ub, unsigned off)
+{
+  return box_iv(
+      __builtin_nontemporal_load(reinterpret_cast<const double*>(reinterpret_cast<const char*>(lb) + off)),
+      __builtin_nontemporal_load(reinterpret_cast<const double*>(reinterpret_cast<const char*>(ub) + off)));
+}
+
+/// The same for a box variable that several waves of a workgroup read: plain
+/// loads, so the second wave finds the lines in L2.
+__device__ __forceinline__ Iv load_box_shared(const double* lb, const double* ub, unsigned off)
+{
+  return box_iv(*reinterpret_cast<const double*>(reinterpret_cast<const char*>(lb) + off),
+                *reinterpret_cast<const double*>(reinterpret_cast<const char*>(ub) + off));
 }
 
 // ---------------------------------------------------------------------------

@@ -338,6 +338,21 @@ std::string boxes_source(const tape::Tape& t, const Shape& shape)
   // DAG shares between functions) is formed once, at its first use in that
   // order, and held for the others: the same operation on the same slots, so
   // the same bits.
+  // How many waves of the workgroup read each variable: one reader streams
+  // it past the caches, several let the later ones find it in L2.
+  std::vector<std::uint32_t> readers(nv, 0);
+  for (std::uint32_t v = 0; v < w; ++v) {
+    std::set<std::uint32_t> vars, temps;
+    for (std::uint32_t g = v; g < k; g += w) {
+      std::uint32_t const lo = t.group_begin.size() > g ? t.group_begin[g] : 0;
+      std::uint32_t const hi =
+          t.group_begin.size() > g + 1 ? t.group_begin[g + 1] : static_cast<std::uint32_t>(t.code.size());
+      slot_use(t, lo, hi, vars, temps);
+    }
+    for (std::uint32_t x : vars) {
+      ++readers[x];
+    }
+  }
   for (std::uint32_t v = 0; v < w; ++v) {
     std::set<std::uint32_t> vars, temps;
     std::vector<std::vector<std::uint32_t>> words;
@@ -423,7 +438,8 @@ std::string boxes_source(const tape::Tape& t, const Shape& shape)
       if (persistent) {
         o << ind << "  const Iv " << slot(x) << " = {X[" << x * 64 << "u + lane].x, X[" << x * 64 << "u + lane].y};\n";
       } else {
-        o << ind << "  const Iv " << slot(x) << " = load_box(lb + " << x << "ul * ld, ub + " << x << "ul * ld, eo);\n";
+        o << ind << "  const Iv " << slot(x) << " = " << (readers[x] > 1 ? "load_box_shared" : "load_box") << "(lb + "
+          << x << "ul * ld, ub + " << x << "ul * ld, eo);\n";
       }
     }
     for (std::uint32_t x : temps) {

@@ -476,7 +476,9 @@ __global__ __launch_bounds__(kTile * kMaxGroups) void k_boxes(const Instr* __res
   bool const active = e < n_boxes;
   std::uint64_t const ee = active ? e : n_boxes - 1;
   for (std::uint32_t v = wave; v < n_vars; v += n_groups) {
-    S[v * kTile + lane] = make_double2(-lb[v * ld + ee], ub[v * ld + ee]);
+    // each box is read once per launch: stream it past the caches
+    S[v * kTile + lane] = make_double2(-__builtin_nontemporal_load(lb + v * ld + ee),
+                                       __builtin_nontemporal_load(ub + v * ld + ee));
   }
   __syncthreads();
   std::uint32_t const begin = groups.begin[wave];
