@@ -196,6 +196,10 @@ def main(argv=None) -> int:
                     help="compiled kernel: waves per workgroup (0 = one per function group)")
     ap.add_argument("--persistent", action="store_true",
                     help="compiled kernel: persistent double-buffered form instead of one workgroup per tile")
+    ap.add_argument("--schedule", choices=["default", "max-ilp"], default="default",
+                    help="compiled kernel: instruction scheduler of the run-time compiler")
+    ap.add_argument("--load-batch", type=int, default=8,
+                    help="compiled direct form: variables per load batch (0 = every load at the start)")
     ap.add_argument("--wgs-per-cu", type=int, default=1,
                     help="compiled kernel, persistent form: resident workgroups per CU (grid = CUs x this)")
     ap.add_argument("--sweep-groups", action="store_true",
@@ -249,6 +253,7 @@ def main(argv=None) -> int:
     # HIP events below time exactly these launches.
     stream = torch.cuda.Stream(dev)
     engines = {"compiled": capi.BOX_COMPILED, "interpreted": capi.BOX_INTERPRETED, "auto": capi.BOX_AUTO}
+    schedules = {"default": capi.SCHED_DEFAULT, "max-ilp": capi.SCHED_MAX_ILP}
     # The kernel configuration is fixed before any launch (the autotuner runs on
     # its own synthetic boxes), so every launch below is the measured kernel.
     tuning = None
@@ -258,6 +263,7 @@ def main(argv=None) -> int:
         else:
             tape.set_box_groups(a.box_groups)
             tape.set_compiled_shape(a.persistent, a.waves, a.wgs_per_cu)
+            tape.set_compiled_codegen(schedules[a.schedule], a.load_batch)
             tape.set_box_engine(engines[a.box_engine])
     torch.cuda.synchronize()
     for k in range(a.warmup):
@@ -308,6 +314,7 @@ def main(argv=None) -> int:
         else:
             tape.set_box_groups(a.box_groups)
             tape.set_compiled_shape(a.persistent, a.waves, a.wgs_per_cu)
+            tape.set_compiled_codegen(schedules[a.schedule], a.load_batch)
             tape.set_box_engine(engines[a.box_engine])
     info = tape.info()
     for k in range(3):
@@ -386,6 +393,9 @@ def main(argv=None) -> int:
             "box_engine_choice": a.box_engine,
             "compiled_waves": info.compiled_waves if info.box_compiled else None,
             "compiled_persistent": bool(info.compiled_persistent) if info.box_compiled else None,
+            "compiled_schedule": ("max-ilp" if info.compiled_schedule == capi.SCHED_MAX_ILP else "default")
+            if info.box_compiled else None,
+            "compiled_load_batch": info.compiled_load_batch if info.box_compiled else None,
             "compile_seconds": info.compile_seconds,
             "box_groups": info.box_groups,
             "box_instructions": info.box_instr,

@@ -14,6 +14,7 @@ LIB_PATH = pathlib.Path(__file__).resolve().parent / "lib" / "libcmi.so"
 
 # status codes (cmi_status)
 BOX_AUTO, BOX_INTERPRETED, BOX_COMPILED = range(3)
+SCHED_DEFAULT, SCHED_MAX_ILP = range(2)
 (OK, SHAPE_MISMATCH, INVALID_CONFIG, OVER_BUDGET, DEVICE_ERROR, INVALID_HANDLE, INVALID_DAG, INVALID_PROBLEM, ERROR,
  PARSE_ERROR) = range(10)
 STATUS_NAMES = {
@@ -85,12 +86,19 @@ class TapeInfo(C.Structure):
                 ("box_groups", C.c_size_t), ("box_instr", C.c_size_t), ("box_slots", C.c_size_t),
                 ("box_engine", C.c_int), ("box_compiled", C.c_int), ("compile_seconds", C.c_double),
                 ("compiled_persistent", C.c_int), ("compiled_wgs_per_cu", C.c_size_t),
-                ("compiled_waves", C.c_size_t), ("extended", C.c_int)]
+                ("compiled_waves", C.c_size_t), ("extended", C.c_int), ("compiled_schedule", C.c_int),
+                ("compiled_load_batch", C.c_size_t)]
 
 
 class BoxCandidate(C.Structure):
     _fields_ = [("engine", C.c_int), ("groups", C.c_size_t), ("waves", C.c_size_t), ("persistent", C.c_int),
-                ("wgs_per_cu", C.c_size_t), ("kernel_ms", C.c_double)]
+                ("wgs_per_cu", C.c_size_t), ("schedule", C.c_int), ("load_batch", C.c_size_t),
+                ("kernel_ms", C.c_double)]
+
+
+class CompiledShape(C.Structure):
+    _fields_ = [("groups", C.c_size_t), ("persistent", C.c_int), ("waves", C.c_size_t), ("schedule", C.c_int),
+                ("load_batch", C.c_size_t), ("roots", C.c_int)]
 
 
 class AggregateBound(C.Structure):
@@ -218,8 +226,10 @@ _SIGNATURES = {
     "cmi_tape_autotune": (C.c_int, [_P, _P, C.c_size_t, C.POINTER(BoxCandidate), C.c_size_t,
                                    C.POINTER(C.c_size_t)]),
     "cmi_tape_set_compiled_shape": (C.c_int, [_P, C.c_int, C.c_size_t, C.c_size_t]),
-    "cmi_tape_compiled_source": (C.c_int, [_P, C.c_int, C.c_size_t, C.c_char_p, C.c_size_t, C.POINTER(C.c_size_t)]),
-    "cmi_tape_compiled_object": (C.c_int, [_P, C.c_int, C.c_size_t, _P, C.c_size_t, C.POINTER(C.c_size_t)]),
+    "cmi_tape_compiled_source": (C.c_int, [_P, C.POINTER(CompiledShape), C.c_char_p, C.c_size_t,
+                                           C.POINTER(C.c_size_t)]),
+    "cmi_tape_compiled_object": (C.c_int, [_P, C.POINTER(CompiledShape), _P, C.c_size_t, C.POINTER(C.c_size_t)]),
+    "cmi_tape_set_compiled_codegen": (C.c_int, [_P, C.c_int, C.c_size_t]),
     "cmi_tape_compile_words": (C.c_int, [_P, C.c_size_t, _P, C.c_size_t, C.POINTER(C.c_uint32), C.POINTER(TapeInfo)]),
     "cmi_tape_compile_info": (C.c_int, [_P, C.POINTER(TapeInfo), C.c_char_p, C.c_size_t]),
     "cmi_eval_boxes": (C.c_int, [_P, _P, _P, C.c_size_t, C.c_size_t, _P, _P, _P, _P]),

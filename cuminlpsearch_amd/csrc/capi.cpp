@@ -392,6 +392,8 @@ cmi_status cmi_tape_info_get(const cmi_tape* t, cmi_tape_info* out)
     out->compiled_wgs_per_cu = t->tape->compiled_wgs_per_cu();
     out->compiled_waves = sh.waves;
     out->extended = tp.extended ? 1 : 0;
+    out->compiled_schedule = sh.ilp_schedule ? CMI_SCHED_MAX_ILP : CMI_SCHED_DEFAULT;
+    out->compiled_load_batch = sh.persistent ? 0 : sh.load_batch;
   });
 }
 
@@ -415,6 +417,8 @@ cmi_status cmi_tape_autotune(cmi_tape* t, const cmi_problem* p, size_t n_boxes, 
       results[i].waves = all[i].waves;
       results[i].persistent = all[i].persistent ? 1 : 0;
       results[i].wgs_per_cu = all[i].wgs_per_cu;
+      results[i].schedule = all[i].ilp_schedule ? CMI_SCHED_MAX_ILP : CMI_SCHED_DEFAULT;
+      results[i].load_batch = all[i].persistent ? 0 : all[i].load_batch;
       results[i].kernel_ms = all[i].kernel_ms;
     }
   });
@@ -431,6 +435,22 @@ cmi_status cmi_tape_set_compiled_shape(cmi_tape* t, int persistent, size_t waves
   });
 }
 
+cmi_status cmi_tape_set_compiled_codegen(cmi_tape* t, int schedule, size_t load_batch)
+{
+  if (t == nullptr) {
+    return fail(CMI_INVALID_HANDLE, "null argument");
+  }
+  if (schedule != CMI_SCHED_DEFAULT && schedule != CMI_SCHED_MAX_ILP) {
+    return fail(CMI_INVALID_CONFIG, "unknown compiled-kernel schedule");
+  }
+  if (load_batch > 64) {
+    return fail(CMI_INVALID_CONFIG, "at most 64 variables per load batch");
+  }
+  return guarded([&] {
+    t->tape->set_compiled_codegen(schedule == CMI_SCHED_MAX_ILP, static_cast<std::uint32_t>(load_batch));
+  });
+}
+
 cmi_status cmi_tape_set_box_engine(cmi_tape* t, int engine)
 {
   if (t == nullptr) {
@@ -442,33 +462,55 @@ cmi_status cmi_tape_set_box_engine(cmi_tape* t, int engine)
 namespace
 {
 /// The grouping a new device tape uses for explicit boxes.
-cmi::tape::Tape grouped_for_boxes(const cmi::model::Problem<double>& p)
+/// The function grouping a tape given `groups` (0 = automatic) evaluates
+/// explicit boxes with: the same clamping and LDS fit as set_box_groups, with
+/// gfx950's 160 KB of LDS.
+cmi::tape::Tape grouped_for_boxes(const cmi::model::Problem<double>& p, size_t groups)
 {
-  return cmi::tape::compile_groups(p, cmi::tape::auto_groups(cmi::tape::compile(p)));
+  cmi::tape::Tape whole = cmi::tape::compile(p);
+  std::uint32_t k = groups == 0 ? cmi::tape::auto_groups(whole) : static_cast<std::uint32_t>(std::min<size_t>(groups, 8));
+  k = std::clamp<std::uint32_t>(k, 1, std::min<std::uint32_t>(cmi::dev::kMaxGroups, 1 + whole.n_constraints));
+  for (; k > 1; --k) {
+    cmi::tape::Tape g = cmi::tape::compile_groups(p, k);
+    if (cmi::dev::boxes_lds_bytes(g.n_slots) <= cmi::jit::kLdsBytes) {
+      return g;
+    }
+  }
+  return whole;
 }
 
-cmi::jit::Shape export_shape(const cmi::tape::Tape& g, int persistent, size_t waves)
+cmi::jit::Shape export_shape(const cmi::tape::Tape& g, const cmi_compiled_shape& in)
 {
-  if (waves > 8) {
+  if (in.waves > 8) {
     throw cmi::InvalidConfiguration("at most 8 waves per workgroup");
   }
+  if (in.schedule != CMI_SCHED_DEFAULT && in.schedule != CMI_SCHED_MAX_ILP) {
+    throw cmi::InvalidConfiguration("unknown compiled-kernel schedule");
+  }
+  if (in.load_batch > 64) {
+    throw cmi::InvalidConfiguration("at most 64 variables per load batch");
+  }
   cmi::jit::Shape sh;
-  sh.waves = cmi::jit::shape_waves(g, {false, static_cast<std::uint32_t>(waves)});
-  sh.persistent = persistent < 0 ? cmi::jit::persistent_fits(g, sh.waves) : persistent != 0;
+  sh.waves = cmi::jit::shape_waves(g, {false, static_cast<std::uint32_t>(in.waves)});
+  sh.persistent = in.persistent < 0 ? cmi::jit::persistent_fits(g, sh.waves) : in.persistent != 0;
+  sh.ilp_schedule = in.schedule == CMI_SCHED_MAX_ILP;
+  sh.load_batch = static_cast<std::uint32_t>(in.load_batch);
+  sh.roots = in.roots != 0;
   return sh;
 }
 }  // namespace
 
-cmi_status cmi_tape_compiled_object(const cmi_problem* p, int persistent, size_t waves, void* buf, size_t cap,
+cmi_status cmi_tape_compiled_object(const cmi_problem* p, const cmi_compiled_shape* shape, void* buf, size_t cap,
                                     size_t* len)
 {
-  if (p == nullptr || len == nullptr) {
+  if (p == nullptr || shape == nullptr || len == nullptr) {
     return fail(CMI_INVALID_HANDLE, "null argument");
   }
   return guarded([&] {
     p->problem.validate();
-    auto const g = grouped_for_boxes(p->problem);
-    auto const code = cmi::jit::compile_code_object(cmi::jit::boxes_source(g, export_shape(g, persistent, waves)));
+    auto const g = grouped_for_boxes(p->problem, shape->groups);
+    auto const sh = export_shape(g, *shape);
+    auto const code = cmi::jit::compile_code_object(cmi::jit::boxes_source(g, sh), sh.ilp_schedule);
     *len = code.size();
     if (buf != nullptr) {
       if (cap < code.size()) {
@@ -480,16 +522,16 @@ cmi_status cmi_tape_compiled_object(const cmi_problem* p, int persistent, size_t
   });
 }
 
-cmi_status cmi_tape_compiled_source(const cmi_problem* p, int persistent, size_t waves, char* buf, size_t cap,
+cmi_status cmi_tape_compiled_source(const cmi_problem* p, const cmi_compiled_shape* shape, char* buf, size_t cap,
                                     size_t* len)
 {
-  if (p == nullptr) {
+  if (p == nullptr || shape == nullptr) {
     return fail(CMI_INVALID_HANDLE, "null argument");
   }
   return guarded([&] {
     p->problem.validate();
-    auto const g = grouped_for_boxes(p->problem);
-    std::string const src = cmi::jit::boxes_source(g, export_shape(g, persistent, waves));
+    auto const g = grouped_for_boxes(p->problem, shape->groups);
+    std::string const src = cmi::jit::boxes_source(g, export_shape(g, *shape));
     if (len != nullptr) {
       *len = src.size();
     }
@@ -545,6 +587,8 @@ cmi_status cmi_tape_compile_words(const cmi_problem* p, size_t k, void* words, s
     out->compiled_wgs_per_cu = 1;
     out->compiled_waves = 1;
     out->extended = whole.extended ? 1 : 0;
+    out->compiled_schedule = CMI_SCHED_DEFAULT;
+    out->compiled_load_batch = 0;
     if (words != nullptr) {
       if (cap_words < tp.code.size()) {
         throw cmi::ShapeMismatch("word buffer holds " + std::to_string(cap_words) + " words; the tape has "
@@ -585,6 +629,8 @@ cmi_status cmi_tape_compile_info(const cmi_problem* p, cmi_tape_info* out, char*
     out->compiled_wgs_per_cu = 1;
     out->compiled_waves = 1;
     out->extended = tp.extended ? 1 : 0;
+    out->compiled_schedule = CMI_SCHED_DEFAULT;
+    out->compiled_load_batch = 0;
     if (listing != nullptr && cap > 0) {
       std::string const text = cmi::tape::disassemble(tp);
       std::size_t const k = std::min(cap - 1, text.size());

@@ -176,10 +176,12 @@ public:
   };
   static constexpr std::size_t kAutoCompileBoxes = 32768;
   static constexpr std::size_t kMaxCompiledWords = 20000;
+  /// Variables per load batch of a new tape's compiled direct form.
+  static constexpr std::uint32_t kDefaultLoadBatch = 8;
   /// Compiled compiles now (and throws if that fails).
   void set_box_engine(BoxEngine e);
   BoxEngine box_engine() const { return engine_; }
-  /// The compiled kernel, if one is loaded.
+  /// The compiled kernel without root-interval stores, if one is loaded.
   std::shared_ptr<const jit::CompiledBoxes> compiled() const;
 
   /// Shape of the compiled explicit-box kernel: persistent (double-buffered
@@ -187,6 +189,10 @@ public:
   /// tiles fit) or direct (one workgroup per tile), and waves per workgroup
   /// (0 = one per function group; fewer waves interleave several groups).
   void set_compiled_shape(bool persistent, std::uint32_t waves, std::uint32_t wgs_per_cu);
+  /// Code generation of the compiled kernel: the instruction scheduler
+  /// (jit::Shape::ilp_schedule) and the direct form's load batching
+  /// (jit::Shape::load_batch, at most 64 variables per batch).
+  void set_compiled_codegen(bool ilp_schedule, std::uint32_t load_batch);
   jit::Shape compiled_shape() const;
   std::uint32_t compiled_wgs_per_cu() const { return wgs_per_cu_; }
 
@@ -198,6 +204,8 @@ public:
     std::uint32_t waves = 0;
     bool persistent = false;
     std::uint32_t wgs_per_cu = 1;
+    bool ilp_schedule = false;
+    std::uint32_t load_batch = 0;
     double kernel_ms = 0.0;
   };
   /// Autotuner: time the interpreter and compiled shapes (function groups,
@@ -232,12 +240,18 @@ private:
   BoxEngine engine_ = BoxEngine::Auto;
   bool want_persistent_ = true;
   std::uint32_t want_waves_ = 0;
+  bool want_ilp_ = false;
+  std::uint32_t want_load_batch_ = kDefaultLoadBatch;
   std::uint32_t wgs_per_cu_ = 1;
   mutable std::mutex jit_mu_;
+  // compiled kernels without and with root-interval stores (the second is
+  // compiled the first time a call asks for root intervals)
   mutable std::shared_ptr<const jit::CompiledBoxes> compiled_;
+  mutable std::shared_ptr<const jit::CompiledBoxes> compiled_roots_;
   mutable bool jit_failed_ = false;
 
-  std::shared_ptr<const jit::CompiledBoxes> compile_now() const;
+  std::shared_ptr<const jit::CompiledBoxes> compile_now(bool roots) const;
+  void drop_compiled() const;
   DeviceBuffer code_;
   DeviceBuffer grouped_code_;
   DeviceBuffer kinds_;

@@ -93,14 +93,17 @@ std::string product_expr(const Instr& w)
 /// One tape word as one statement; `acc` names the group's accumulator. A
 /// term word whose product is already held in a variable takes its name as
 /// `prod` instead of forming the product again.
-std::string emit_word(const Instr& w, const std::string& acc, const std::string& prod = {})
+std::string emit_word(const Instr& w, const std::string& acc, bool roots, const std::string& prod = {})
 {
   std::string const a = slot(w.a), b = slot(w.b), d = slot(w.dst), c = lit(w.imm);
   auto set = [](const std::string& lhs, const std::string& rhs) { return "  " + lhs + " = " + rhs + ";\n"; };
   auto call = [](const std::string& fn, const std::string& x, const std::string& y) {
     return fn + "(" + x + ", " + y + ", t)";
   };
-  auto output = [&](const std::string& x) {
+  auto output = [&](const std::string& x) -> std::string {
+    if (!roots) {
+      return {};
+    }
     return "  if (rt != nullptr) out_store(rt, " + std::to_string(w.b) + ", n, e, to_output(" + x + ", t));\n";
   };
   if (tape::is_term_op(w.op)) {
@@ -196,6 +199,9 @@ const char* const kOptions[] = {
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-ffp-contract=off",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
 };
+// The scheduler that orders each block for instruction-level parallelism
+// rather than for occupancy (Shape::ilp_schedule).
+const char* const kIlpOptions[] = {"-mllvm", "-amdgpu-sched-strategy=max-ilp"};
 
 }  // namespace
 
@@ -326,7 +332,9 @@ std::string boxes_source(const tape::Tape& t, const Shape& shape)
     o << "  const unsigned eo = 8u * (unsigned)(active ? e : n - 1);\n";
   }
   std::string const ind = persistent ? "    " : "  ";
-  o << ind << "double* const rt = active ? roots : nullptr;\n";
+  if (shape.roots) {
+    o << ind << "double* const rt = active ? roots : nullptr;\n";
+  }
   o << ind << "bool f = true;\n";
   o << ind << "const int t = enter_round_up();\n";
   // Wave v runs groups v, v + w, v + 2w, ...; their statements are
@@ -388,36 +396,13 @@ std::string boxes_source(const tape::Tape& t, const Shape& shape)
         ++uses[key];
       }
     }
-    std::map<std::uint64_t, std::string> held;
-    std::vector<std::string> body;
-    for (auto const& [j, i] : order) {
-      Instr const& word = t.code[i];
-      std::string const acc = "acc" + std::to_string(j);
-      std::uint64_t const key = product_key(word, nv);
-      std::string line;
-      if (key != 0 && uses[key] > 1) {
-        auto it = held.find(key);
-        if (it == held.end()) {
-          std::string const name = "p" + std::to_string(held.size());
-          body.push_back(ind + "  const Iv " + name + " = " + product_expr(word) + ";\n");
-          it = held.emplace(key, name).first;
-        }
-        line = emit_word(word, acc, it->second);
-      } else {
-        line = emit_word(word, acc);
-      }
-      if (!line.empty()) {
-        body.push_back(ind + line);
-      }
-    }
-    o << ind << (v == 0 ? "if" : "else if") << " (wave == " << v << "u) {\n";
     // variables in the order the wave's statements first read them, so the
     // waits for their loads come due in the order the loads were issued
     std::vector<std::uint32_t> first_use;
+    std::map<std::uint32_t, std::size_t> rank;  // variable -> position in first_use
     {
-      std::set<std::uint32_t> seen;
       auto see = [&](std::uint32_t x) {
-        if (vars.count(x) != 0 && seen.insert(x).second) {
+        if (vars.count(x) != 0 && rank.emplace(x, first_use.size()).second) {
           first_use.push_back(x);
         }
       };
@@ -434,14 +419,68 @@ std::string boxes_source(const tape::Tape& t, const Shape& shape)
         see(x);
       }
     }
-    for (std::uint32_t x : first_use) {
-      if (persistent) {
-        o << ind << "  const Iv " << slot(x) << " = {X[" << x * 64 << "u + lane].x, X[" << x * 64 << "u + lane].y};\n";
+    // The statements, each with 1 + the first-use position of the last
+    // variable it reads (0: it reads none).
+    std::map<std::uint64_t, std::string> held;
+    std::vector<std::pair<std::string, std::size_t>> body;
+    for (auto const& [j, i] : order) {
+      Instr const& word = t.code[i];
+      std::string const acc = "acc" + std::to_string(j);
+      std::size_t need = 0;
+      WordUse const u = word_use(word);
+      if (u.reads_a && word.a < nv) {
+        need = std::max(need, rank.at(word.a) + 1);
+      }
+      if (u.reads_b && word.b < nv) {
+        need = std::max(need, rank.at(word.b) + 1);
+      }
+      std::uint64_t const key = product_key(word, nv);
+      std::string line;
+      if (key != 0 && uses[key] > 1) {
+        auto it = held.find(key);
+        if (it == held.end()) {
+          std::string const name = "p" + std::to_string(held.size());
+          body.emplace_back(ind + "  const Iv " + name + " = " + product_expr(word) + ";\n", need);
+          it = held.emplace(key, name).first;
+        }
+        line = emit_word(word, acc, shape.roots, it->second);
       } else {
-        o << ind << "  const Iv " << slot(x) << " = " << (readers[x] > 1 ? "load_box_shared" : "load_box") << "(lb + "
-          << x << "ul * ld, ub + " << x << "ul * ld, eo);\n";
+        line = emit_word(word, acc, shape.roots);
+      }
+      if (!line.empty()) {
+        body.emplace_back(ind + line, need);
       }
     }
+    o << ind << (v == 0 ? "if" : "else if") << " (wave == " << v << "u) {\n";
+    // Variable loads. Batched (direct form): the first two batches up front,
+    // then batch b+1 just before the first statement that reads batch b; a
+    // scheduling barrier on each side keeps the compiler from hoisting them.
+    std::uint32_t const batch = persistent ? 0u : shape.load_batch;
+    std::size_t const n_batches = batch == 0 ? 1 : (first_use.size() + batch - 1) / batch;
+    std::size_t issued = 0;
+    auto issue_until = [&](std::size_t upto) {
+      for (upto = std::min(upto, n_batches); issued < upto; ++issued) {
+        std::size_t const lo = batch == 0 ? 0 : issued * batch;
+        std::size_t const hi = batch == 0 ? first_use.size() : std::min(first_use.size(), lo + batch);
+        if (batch != 0) {
+          o << ind << "  __builtin_amdgcn_sched_barrier(0);\n";
+        }
+        for (std::size_t q = lo; q < hi; ++q) {
+          std::uint32_t const x = first_use[q];
+          if (persistent) {
+            o << ind << "  const Iv " << slot(x) << " = {X[" << x * 64 << "u + lane].x, X[" << x * 64
+              << "u + lane].y};\n";
+          } else {
+            o << ind << "  const Iv " << slot(x) << " = " << (readers[x] > 1 ? "load_box_shared" : "load_box")
+              << "(lb + " << x << "ul * ld, ub + " << x << "ul * ld, eo);\n";
+          }
+        }
+        if (batch != 0) {
+          o << ind << "  __builtin_amdgcn_sched_barrier(0);\n";
+        }
+      }
+    };
+    issue_until(batch == 0 ? 1 : 2);
     for (std::uint32_t x : temps) {
       o << ind << "  Iv " << slot(x) << ";\n";
     }
@@ -449,9 +488,13 @@ std::string boxes_source(const tape::Tape& t, const Shape& shape)
       o << ind << "  Iv acc" << j << " = {0.0, 0.0};\n";
     }
     o << ind << "  Iv obj = {0.0, 0.0};\n";
-    for (std::string const& line : body) {
+    for (auto const& [line, need] : body) {
+      if (batch != 0 && need > 0) {
+        issue_until((need - 1) / batch + 2);
+      }
       o << line;
     }
+    issue_until(n_batches);  // variables no statement reads: dead loads the compiler drops
     if (has_obj) {
       o << ind << "  const Bnd ob = to_output(obj, t);\n";
       o << ind << "  obj_s[cur * 128u + lane] = ob.lb;\n" << ind << "  obj_s[cur * 128u + 64u + lane] = ob.ub;\n";
@@ -529,7 +572,7 @@ void CompiledBoxes::launch(const double* lb, const double* ub, std::uint64_t ld,
 std::shared_ptr<CompiledBoxes> compile_boxes(const tape::Tape& t, int device, const Shape& shape)
 {
   static std::mutex mu;
-  static std::map<std::tuple<int, bool, std::uint32_t, std::uint64_t, std::size_t>,
+  static std::map<std::tuple<int, bool, bool, bool, std::uint32_t, std::uint32_t, std::uint64_t, std::size_t>,
                   std::weak_ptr<const CompiledBoxes::Module>>
       cache;
   bool const persistent = shape.persistent;
@@ -538,12 +581,17 @@ std::shared_ptr<CompiledBoxes> compile_boxes(const tape::Tape& t, int device, co
   h = fnv1a(&t.n_vars, sizeof t.n_vars, h);
   h = fnv1a(t.group_begin.data(), t.group_begin.size() * sizeof(std::uint32_t), h);
   h = fnv1a(&t.objective_group, sizeof t.objective_group, h);
-  auto const key = std::make_tuple(device, persistent, waves, h, t.code.size());
+  std::uint32_t const load_batch = persistent ? 0u : shape.load_batch;
+  auto const key =
+      std::make_tuple(device, persistent, shape.ilp_schedule, shape.roots, waves, load_batch, h, t.code.size());
 
   std::shared_ptr<CompiledBoxes> cb(new CompiledBoxes());
   cb->groups_ = std::max<std::uint32_t>(t.n_groups, 1);
   cb->waves_ = waves;
   cb->persistent_ = persistent;
+  cb->ilp_schedule_ = shape.ilp_schedule;
+  cb->roots_ = shape.roots;
+  cb->load_batch_ = load_batch;
   int cus = 0;
   hip_check(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device), "hipDeviceGetAttribute(CUs)");
   cb->n_cus_ = static_cast<std::uint32_t>(std::max(cus, 1));
@@ -558,7 +606,7 @@ std::shared_ptr<CompiledBoxes> compile_boxes(const tape::Tape& t, int device, co
     }
   }
   auto const t0 = std::chrono::steady_clock::now();
-  std::vector<char> const code = compile_code_object(boxes_source(t, shape));
+  std::vector<char> const code = compile_code_object(boxes_source(t, shape), shape.ilp_schedule);
   auto m = std::make_shared<CompiledBoxes::Module>();
   m->device = device;
   m->code = t.code;
@@ -572,11 +620,15 @@ std::shared_ptr<CompiledBoxes> compile_boxes(const tape::Tape& t, int device, co
   return cb;
 }
 
-std::vector<char> compile_code_object(const std::string& src)
+std::vector<char> compile_code_object(const std::string& src, bool ilp_schedule)
 {
+  std::vector<const char*> opts(std::begin(kOptions), std::end(kOptions));
+  if (ilp_schedule) {
+    opts.insert(opts.end(), std::begin(kIlpOptions), std::end(kIlpOptions));
+  }
   hiprtcProgram prog = nullptr;
   rtc_check(hiprtcCreateProgram(&prog, src.c_str(), "cmi_boxes.hip", 0, nullptr, nullptr), "hiprtcCreateProgram");
-  hiprtcResult const rc = hiprtcCompileProgram(prog, static_cast<int>(std::size(kOptions)), kOptions);
+  hiprtcResult const rc = hiprtcCompileProgram(prog, static_cast<int>(opts.size()), opts.data());
   if (rc != HIPRTC_SUCCESS) {
     std::size_t n = 0;
     (void)hiprtcGetProgramLogSize(prog, &n);

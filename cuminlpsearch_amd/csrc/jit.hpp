@@ -44,6 +44,21 @@ struct Shape
   /// Waves per workgroup (0 = one per function group). Wave v runs groups
   /// v, v + waves, ...; a wave's groups are interleaved statement by statement.
   std::uint32_t waves = 0;
+  /// Compile with the instruction scheduler that maximises instruction-level
+  /// parallelism (-amdgpu-sched-strategy=max-ilp) rather than the default,
+  /// occupancy-first one. The source, and so every result, is the same.
+  bool ilp_schedule = false;
+  /// Direct form: 0 issues every variable load at the start of the wave; k > 0
+  /// issues them k variables at a time, in the order the statements first
+  /// read them, each batch as the statements start reading the one before
+  /// it, so a wave's first variables arrive first rather than in whatever
+  /// order the memory system serves the whole grid's loads.
+  std::uint32_t load_batch = 0;
+  /// Write the root intervals when the caller asks for them (the kernel then
+  /// computes both bounds of every function). Without them only what the
+  /// outputs depend on is computed: the compiler drops the upper bounds of
+  /// inequality constraints, which nothing reads.
+  bool roots = true;
 };
 
 /// Waves per workgroup `shape` gives for `t`.
@@ -76,6 +91,10 @@ public:
   bool persistent() const { return persistent_; }
   std::uint32_t groups() const { return groups_; }
   std::uint32_t waves() const { return waves_; }
+  bool ilp_schedule() const { return ilp_schedule_; }
+  /// Whether this kernel writes root intervals (Shape::roots).
+  bool roots() const { return roots_; }
+  std::uint32_t load_batch() const { return load_batch_; }
   /// Persistent form: resident workgroups per CU the grid is sized for.
   std::uint32_t wgs_per_cu() const { return wgs_per_cu_; }
   void set_wgs_per_cu(std::uint32_t w) { wgs_per_cu_ = w < 1 ? 1 : w; }
@@ -89,15 +108,19 @@ private:
   std::uint32_t groups_ = 1;
   std::uint32_t waves_ = 1;
   bool persistent_ = false;
+  bool ilp_schedule_ = false;
+  bool roots_ = true;
+  std::uint32_t load_batch_ = 0;
   std::uint32_t wgs_per_cu_ = 1;
   std::uint32_t n_cus_ = 1;
   double compile_seconds_ = 0.0;
   std::size_t code_bytes_ = 0;
 };
 
-/// Compile HIP source with hiprtc for gfx950 (host only, no device needed);
-/// returns the code object. Throws DeviceError with the compiler log.
-std::vector<char> compile_code_object(const std::string& src);
+/// Compile HIP source with hiprtc for gfx950 (host only, no device needed),
+/// with the max-ILP instruction scheduler when `ilp_schedule`; returns the
+/// code object. Throws DeviceError with the compiler log.
+std::vector<char> compile_code_object(const std::string& src, bool ilp_schedule = false);
 
 /// Compile (or fetch from the process-wide cache, keyed by device, form and
 /// tape contents) the explicit-box kernel of `t` and load it on `device`,

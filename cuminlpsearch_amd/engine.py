@@ -74,21 +74,33 @@ class EngineProblem:
         _c.check(_c.lib().cmi_tape_compile_info(self._h, C.byref(out), buf, (1 << 22) if listing else 0))
         return out, (buf.value.decode() if listing else None)
 
-    def compiled_source(self, persistent: int = -1, waves: int = 0) -> str:
-        """Host-only: HIP source of the compiled explicit-box kernel
-        (persistent: 1 persistent form, 0 direct form, -1 the default)."""
+    @staticmethod
+    def _shape(persistent, waves, groups, schedule, load_batch, roots):
+        return _c.CompiledShape(groups, persistent, waves, schedule, load_batch, int(roots))
+
+    def compiled_source(self, persistent: int = -1, waves: int = 0, groups: int = 0, schedule: int = 0,
+                        load_batch: int = 0, roots: bool = True) -> str:
+        """Host-only: HIP source of the compiled explicit-box kernel (persistent: 1
+        persistent form, 0 direct form, -1 the default; groups: function groups,
+        0 = automatic; load_batch: variables per load batch, 0 = all at once;
+        roots: the kernel that writes root intervals)."""
+        sh = self._shape(persistent, waves, groups, schedule, load_batch, roots)
         n = C.c_size_t()
-        _c.check(_c.lib().cmi_tape_compiled_source(self._h, persistent, waves, None, 0, C.byref(n)))
+        _c.check(_c.lib().cmi_tape_compiled_source(self._h, C.byref(sh), None, 0, C.byref(n)))
         buf = C.create_string_buffer(n.value + 1)
-        _c.check(_c.lib().cmi_tape_compiled_source(self._h, persistent, waves, buf, n.value + 1, C.byref(n)))
+        _c.check(_c.lib().cmi_tape_compiled_source(self._h, C.byref(sh), buf, n.value + 1, C.byref(n)))
         return buf.value.decode()
 
-    def compiled_object(self, persistent: int = -1, waves: int = 0) -> bytes:
-        """Host-only: the gfx950 code object of the compiled explicit-box kernel."""
+    def compiled_object(self, persistent: int = -1, waves: int = 0, schedule: int = 0, groups: int = 0,
+                        load_batch: int = 0, roots: bool = True) -> bytes:
+        """Host-only: the gfx950 code object of the compiled explicit-box kernel
+        (schedule: capi.SCHED_DEFAULT or capi.SCHED_MAX_ILP; other arguments as
+        for compiled_source)."""
+        sh = self._shape(persistent, waves, groups, schedule, load_batch, roots)
         n = C.c_size_t()
-        _c.check(_c.lib().cmi_tape_compiled_object(self._h, persistent, waves, None, 0, C.byref(n)))
+        _c.check(_c.lib().cmi_tape_compiled_object(self._h, C.byref(sh), None, 0, C.byref(n)))
         buf = C.create_string_buffer(n.value)
-        _c.check(_c.lib().cmi_tape_compiled_object(self._h, persistent, waves, buf, n.value, C.byref(n)))
+        _c.check(_c.lib().cmi_tape_compiled_object(self._h, C.byref(sh), buf, n.value, C.byref(n)))
         return buf.raw[:n.value]
 
     def tape_words(self, groups: int = 1):
@@ -159,6 +171,12 @@ class Tape:
         waves per workgroup, 0 = one per function group)."""
         _c.check(_c.lib().cmi_tape_set_compiled_shape(self._h, int(persistent), waves, wgs_per_cu))
 
+    def set_compiled_codegen(self, schedule: int = 0, load_batch: int = 8) -> None:
+        """Scheduler (capi.SCHED_DEFAULT or capi.SCHED_MAX_ILP) and load batching
+        (variables per batch, 0 = all at once) of the compiled kernel; results
+        are the same."""
+        _c.check(_c.lib().cmi_tape_set_compiled_codegen(self._h, schedule, load_batch))
+
     def autotune(self, n_boxes: int = 65536) -> list[dict]:
         """Time the explicit-box kernel configurations on n_boxes synthetic boxes,
         keep the fastest; returns every candidate with its median kernel time."""
@@ -168,7 +186,9 @@ class Tape:
         _c.check(_c.lib().cmi_tape_autotune(self._h, self.problem.handle, n_boxes, out, cap, C.byref(n)))
         names = {_c.BOX_INTERPRETED: "interpreted", _c.BOX_COMPILED: "compiled"}
         return [dict(engine=names.get(r.engine, r.engine), groups=r.groups, waves=r.waves,
-                     persistent=bool(r.persistent), wgs_per_cu=r.wgs_per_cu, kernel_ms=r.kernel_ms)
+                     persistent=bool(r.persistent), wgs_per_cu=r.wgs_per_cu,
+                     schedule="max-ilp" if r.schedule == _c.SCHED_MAX_ILP else "default", load_batch=r.load_batch,
+                     kernel_ms=r.kernel_ms)
                 for r in out[:min(n.value, cap)]]
 
     def set_box_groups(self, k: int = 0) -> int:

@@ -182,6 +182,8 @@ typedef struct
   size_t compiled_waves;    /* waves per workgroup of the compiled kernel */
   int extended;             /* the tape uses the extended op set (Div, PowN, Sqrt, Exp, Log, Sin, Cos, Tanh,
                                Abs, Min, Max; Pow as exp(y log x)) */
+  int compiled_schedule;    /* cmi_compiled_schedule of the compiled kernel */
+  size_t compiled_load_batch; /* variables per load batch of the compiled direct form (0: all at once) */
 } cmi_tape_info;
 
 /* Kernel for explicit boxes: the tape interpreter, or the tape compiled to a
@@ -320,6 +322,26 @@ cmi_status cmi_tape_set_box_engine(cmi_tape* t, int engine);
  * 8): wave v runs the function groups v, v + waves, ..., interleaving their
  * statements. Results do not depend on the shape. */
 cmi_status cmi_tape_set_compiled_shape(cmi_tape* t, int persistent, size_t waves, size_t wgs_per_cu);
+/* Instruction scheduler the run-time compiler orders the compiled kernel's
+ * code with: the default, occupancy-first one, or the one that maximises
+ * instruction-level parallelism. The kernel source, and so every result, is
+ * the same either way. */
+typedef enum
+{
+  CMI_SCHED_DEFAULT = 0,
+  CMI_SCHED_MAX_ILP = 1
+} cmi_compiled_schedule;
+/* Code generation of the compiled kernel: its scheduler (cmi_compiled_schedule)
+ * and, for the direct form, its load batching: load_batch = 0 issues every
+ * variable load at the start of the wave; k in [1, 64] issues them k
+ * variables at a time in the order the statements first read them, each
+ * batch as the statements start on the one before it. Recompiles when the
+ * compiled engine is in effect. Results do not depend on either. A call that
+ * asks for root intervals runs a second compiled kernel that writes them,
+ * compiled on the first such call; without them the kernel computes only
+ * what its outputs read. No reference counterpart; must not overlap a launch
+ * on the tape. */
+cmi_status cmi_tape_set_compiled_codegen(cmi_tape* t, int schedule, size_t load_batch);
 /* One explicit-box kernel configuration and its median kernel time. */
 typedef struct
 {
@@ -328,11 +350,14 @@ typedef struct
   size_t waves;      /* waves per workgroup (compiled) */
   int persistent;    /* compiled form */
   size_t wgs_per_cu; /* persistent form: workgroups per CU */
+  int schedule;      /* compiled: cmi_compiled_schedule */
+  size_t load_batch; /* compiled direct form: variables per load batch (0: all at once) */
   double kernel_ms;
 } cmi_box_candidate;
 
 /* Autotuner: time the interpreter and the compiled shapes (function groups,
- * waves per workgroup, persistent or direct, workgroups per CU) on n_boxes
+ * waves per workgroup, persistent or direct, workgroups per CU, and the direct
+ * form's scheduler and load batching) on n_boxes
  * synthetic boxes inside the bounds of `p` (the problem the tape was built
  * from), leave the tape on the fastest, and report up to `cap` candidates in
  * `results` (*n_results = how many ran). Takes seconds (each compiled shape is
@@ -341,18 +366,26 @@ typedef struct
 cmi_status cmi_tape_autotune(cmi_tape* t, const cmi_problem* p, size_t n_boxes, cmi_box_candidate* results, size_t cap,
                              size_t* n_results);
 
+/* One compiled explicit-box kernel, for the host-only views below. */
+typedef struct
+{
+  size_t groups;     /* function groups as cmi_tape_set_box_groups makes them; 0 = those of a new tape */
+  int persistent;    /* 1 persistent form (INVALID_CONFIG if its tiles do not fit in LDS), 0 direct, -1 a new
+                        tape's form */
+  size_t waves;      /* as for cmi_tape_set_compiled_shape */
+  int schedule;      /* cmi_compiled_schedule */
+  size_t load_batch; /* as for cmi_tape_set_compiled_codegen */
+  int roots;         /* 1: the kernel that writes root intervals */
+} cmi_compiled_shape;
 /* Host-only: the HIP source of the compiled explicit-box kernel of `p`'s tape
- * (with the function groups a new tape uses), NUL-terminated into `buf` (up
- * to `cap` bytes); *len receives the full length without the NUL. persistent:
- * 1 the persistent form (INVALID_CONFIG if its tiles do not fit in LDS), 0 the
- * direct form, -1 the form a new tape uses; waves as for
- * cmi_tape_set_compiled_shape. */
-cmi_status cmi_tape_compiled_source(const cmi_problem* p, int persistent, size_t waves, char* buf, size_t cap,
+ * in `shape`, NUL-terminated into `buf` (up to `cap` bytes); *len receives
+ * the full length without the NUL. */
+cmi_status cmi_tape_compiled_source(const cmi_problem* p, const cmi_compiled_shape* shape, char* buf, size_t cap,
                                     size_t* len);
 /* Host-only: compile that source with the run-time compiler, exactly as
  * CMI_BOX_COMPILED does, and copy the gfx950 code object into `buf` (when
  * non-NULL and cap >= *len; SHAPE_MISMATCH otherwise); *len receives its size. */
-cmi_status cmi_tape_compiled_object(const cmi_problem* p, int persistent, size_t waves, void* buf, size_t cap,
+cmi_status cmi_tape_compiled_object(const cmi_problem* p, const cmi_compiled_shape* shape, void* buf, size_t cap,
                                     size_t* len);
 /* Compile the tape on the host only (no device) and report its shape; also
  * writes the listing into `listing` (up to `cap` bytes, NUL-terminated) when

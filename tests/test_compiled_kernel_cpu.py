@@ -46,14 +46,17 @@ def test_generated_source_regions(tmp_path, case, persistent):
     assert check_asm_regions(out.read_text().splitlines()) == 1
 
 
+@pytest.mark.parametrize("schedule", [0, 1])
 @pytest.mark.parametrize("persistent", [0, 1])
 @pytest.mark.parametrize("case", ["c2", "shared_generic"])
-def test_runtime_code_object_rounds_only_inside_the_region(tmp_path, case, persistent):
+def test_runtime_code_object_rounds_only_inside_the_region(tmp_path, case, persistent, schedule):
+    """schedule 1: the max-ILP instruction scheduler, which reorders independent
+    instructions but keeps the region's entry and exit around every rounding one."""
     if not pathlib.Path(OBJDUMP).exists():
         pytest.skip("llvm-objdump not available")
     eng = dict(problems())[case]
     co = tmp_path / "k.co"
-    co.write_bytes(eng.compiled_object(persistent))
+    co.write_bytes(eng.compiled_object(persistent, 0, schedule, load_batch=8 * schedule, roots=bool(schedule)))
     dis = subprocess.run([OBJDUMP, "-d", "--mcpu=gfx950", str(co)], check=True, capture_output=True,
                          text=True).stdout.splitlines()
     enters = [i for i, ln in enumerate(dis) if ENTER in ln]
@@ -63,6 +66,54 @@ def test_runtime_code_object_rounds_only_inside_the_region(tmp_path, case, persi
     assert rounding and all(enters[0] < i < leaves[0] for i in rounding)
     assert not any(PACKED_F32.match(ln.strip()) for ln in dis)
     assert not any(re.match(r"\s*scratch_", ln) for ln in dis), "register spills in the compiled kernel"
+
+
+@pytest.mark.parametrize("batch", [1, 3, 8, 64])
+@pytest.mark.parametrize("groups,waves", [(1, 1), (8, 2)])
+def test_batched_loads_follow_first_use(groups, waves, batch):
+    """Direct form with load batching: each wave loads every variable once,
+    batches of `batch` in first-use order between scheduling barriers, the
+    first two up front, and never reads a variable before its load."""
+    eng = P.c2()
+    src = eng.compiled_source(0, waves, groups, 0, batch, True)
+    kern = src[src.index("__global__"):]
+    blocks = re.split(r"\n  (?:else )?if \(wave == \d+u\) \{\n", kern)[1:]
+    assert len(blocks) == waves
+    for blk in blocks:
+        lines = blk.split("\n")
+        loads = [re.match(r"\s+const Iv (s\d+) = load_box", ln) for ln in lines]
+        names = [m.group(1) for m in loads if m]
+        assert len(names) == len(set(names))
+        loaded = set()
+        first = []
+        for ln, m in zip(lines, loads):
+            if m:
+                loaded.add(m.group(1))
+                continue
+            for v in re.findall(r"\b(s\d+)\b", ln):
+                assert v in loaded, (v, ln)
+                if v not in first:
+                    first.append(v)
+        assert names[:len(first)] == first  # loads issued in first-use order
+        barriers = [i for i, ln in enumerate(lines) if "sched_barrier" in ln]
+        assert len(barriers) == 2 * ((len(names) + batch - 1) // batch)
+
+
+def test_no_roots_kernel_writes_no_roots():
+    """The kernel for calls without root intervals has no root stores, and the
+    compiler then drops the constraint upper bounds nothing reads: fewer FP64
+    instructions than the kernel that writes them."""
+    if not pathlib.Path(OBJDUMP).exists():
+        pytest.skip("llvm-objdump not available")
+    eng = P.c2()
+    assert "out_store(rt" in eng.compiled_source(0, 1, 1, 0, 8, True)
+    assert "out_store(rt" not in eng.compiled_source(0, 1, 1, 0, 8, False)
+    counts = []
+    for roots in (True, False):
+        dis = subprocess.run([OBJDUMP, "-d", "--mcpu=gfx950", "/dev/stdin"], input=eng.compiled_object(
+            0, 1, 0, groups=1, load_batch=8, roots=roots), check=True, capture_output=True).stdout.decode()
+        counts.append(len(re.findall(r"\sv_\w+_f64", dis)))
+    assert counts[1] < 0.8 * counts[0], counts
 
 
 def test_source_is_deterministic_and_exact():

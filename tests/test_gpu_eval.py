@@ -240,6 +240,31 @@ def test_autotune_keeps_results_and_picks_the_fastest(ctx):
     assert np.array_equal(f_d, f_o) and same_bits(lo_d, lo_o) and same_bits(hi_d, hi_o) and same_bits(r_d, r_o)
 
 
+@pytest.mark.parametrize("groups,waves", [(1, 1), (4, 1), (8, 2), (8, 4)])
+@pytest.mark.parametrize("schedule", [0, 1])
+@pytest.mark.parametrize("load_batch", [0, 3, 8])
+def test_compiled_codegen_bitexact(ctx, groups, waves, schedule, load_batch):
+    """Every scheduler and load batching of the compiled kernel gives the
+    oracle's results bit for bit, both the kernel that writes root intervals
+    and the one that computes only what feasibility and the objective read."""
+    eng = P.c2()
+    tape = cmi.Tape(ctx, eng)
+    tape.set_box_groups(groups)
+    tape.set_box_engine(C.BOX_COMPILED)
+    tape.set_compiled_shape(False, waves, 1)
+    tape.set_compiled_codegen(schedule, load_batch)
+    info = tape.info()
+    assert info.compiled_schedule == schedule and info.compiled_waves == waves
+    assert info.compiled_load_batch == load_batch
+    n = 64 * 700 + 5
+    lb, ub = P.random_boxes(321 + load_batch, 32, n, -5.0, 5.0)
+    f_o, lo_o, hi_o, r_o = orc.eval_boxes(eng.arrays(), lb, ub, roots=True)
+    f_n, lo_n, hi_n, _ = tape.eval_boxes(lb, ub)
+    assert np.array_equal(f_n, f_o) and same_bits(lo_n, lo_o) and same_bits(hi_n, hi_o)
+    f_d, lo_d, hi_d, r_d = tape.eval_boxes(lb, ub, roots=True)
+    assert np.array_equal(f_d, f_o) and same_bits(lo_d, lo_o) and same_bits(hi_d, hi_o) and same_bits(r_d, r_o)
+
+
 @pytest.mark.parametrize("waves", [1, 2, 8])
 @pytest.mark.parametrize("persistent", [False, True])
 def test_compiled_shapes_bitexact(ctx, waves, persistent):
