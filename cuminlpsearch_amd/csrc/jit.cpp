@@ -250,6 +250,75 @@ void slot_use(const tape::Tape& t, std::uint32_t lo, std::uint32_t hi, std::set<
   }
 }
 
+/// The words [lo, hi) of one function group in the order a wave whose
+/// variables arrive in batches should run them. A group is a run of function
+/// blocks, each starting at a term word that sets the accumulator. When no
+/// word touches a temporary slot, the blocks share nothing but variables
+/// (read-only) and products of variables (recomputed or held by value), so
+/// any block order gives the same values; the blocks then go greedily, each
+/// next the one that reads the fewest variables no earlier block read (ties:
+/// tape order), so the first statements need the fewest loads. Otherwise the
+/// tape order is kept.
+std::vector<std::uint32_t> arrival_order(const tape::Tape& t, std::uint32_t lo, std::uint32_t hi)
+{
+  std::vector<std::uint32_t> same;
+  for (std::uint32_t i = lo; i < hi; ++i) {
+    same.push_back(i);
+  }
+  std::uint32_t const nv = t.n_vars;
+  std::vector<std::pair<std::uint32_t, std::uint32_t>> blocks;  // [begin, end)
+  std::vector<std::set<std::uint32_t>> reads;
+  for (std::uint32_t i = lo; i < hi; ++i) {
+    Instr const& w = t.code[i];
+    WordUse const u = word_use(w);
+    if (u.writes || (u.reads_a && w.a >= nv) || (u.reads_b && w.b >= nv)) {
+      return same;
+    }
+    bool const starts =
+        tape::is_term_op(w.op) && static_cast<AccMode>((w.op - tape::kTermBase) >> 3) == AccMode::Set;
+    if (starts) {
+      blocks.emplace_back(i, i);
+      reads.emplace_back();
+    }
+    if (blocks.empty()) {
+      return same;
+    }
+    blocks.back().second = i + 1;
+    if (u.reads_a) {
+      reads.back().insert(w.a);
+    }
+    if (u.reads_b) {
+      reads.back().insert(w.b);
+    }
+  }
+  std::vector<std::uint32_t> out;
+  std::vector<bool> used(blocks.size(), false);
+  std::set<std::uint32_t> have;
+  for (std::size_t step = 0; step < blocks.size(); ++step) {
+    std::size_t best = blocks.size();
+    std::size_t best_new = 0;
+    for (std::size_t b = 0; b < blocks.size(); ++b) {
+      if (used[b]) {
+        continue;
+      }
+      std::size_t fresh = 0;
+      for (std::uint32_t x : reads[b]) {
+        fresh += have.count(x) == 0 ? 1 : 0;
+      }
+      if (best == blocks.size() || fresh < best_new) {
+        best = b;
+        best_new = fresh;
+      }
+    }
+    used[best] = true;
+    have.insert(reads[best].begin(), reads[best].end());
+    for (std::uint32_t i = blocks[best].first; i < blocks[best].second; ++i) {
+      out.push_back(i);
+    }
+  }
+  return out;
+}
+
 }  // namespace
 
 std::uint32_t shape_waves(const tape::Tape& t, const Shape& shape)
@@ -370,11 +439,15 @@ std::string boxes_source(const tape::Tape& t, const Shape& shape)
       std::uint32_t const hi =
           t.group_begin.size() > g + 1 ? t.group_begin[g + 1] : static_cast<std::uint32_t>(t.code.size());
       slot_use(t, lo, hi, vars, temps);
-      std::vector<std::uint32_t> st;
-      for (std::uint32_t i = lo; i < hi; ++i) {
-        st.push_back(i);
+      if (!persistent && shape.load_batch != 0) {
+        words.push_back(arrival_order(t, lo, hi));
+      } else {
+        std::vector<std::uint32_t> st;
+        for (std::uint32_t i = lo; i < hi; ++i) {
+          st.push_back(i);
+        }
+        words.push_back(std::move(st));
       }
-      words.push_back(std::move(st));
       has_obj = has_obj || g == t.objective_group;
     }
     std::vector<std::pair<std::size_t, std::uint32_t>> order;  // (stream, word) in emission order

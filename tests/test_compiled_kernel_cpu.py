@@ -99,6 +99,51 @@ def test_batched_loads_follow_first_use(groups, waves, batch):
         assert len(barriers) == 2 * ((len(names) + batch - 1) // batch)
 
 
+def _statements(src):
+    """The kernel's statements without loads and barriers, with every held
+    product (const Iv pN = ...) written back into the statements that use it."""
+    kern = src[src.index("__global__"):]
+    held = {}
+    out = []
+    for ln in kern.split("\n"):
+        ln = ln.strip()
+        if not ln or "load_box" in ln or "sched_barrier" in ln:
+            continue
+        m = re.match(r"const Iv (p\d+) = (.*);$", ln)
+        if m:
+            held[m.group(1)] = m.group(2)
+            continue
+        out.append(re.sub(r"\b(p\d+)\b", lambda q: held.get(q.group(1), q.group(1)), ln))
+    return out
+
+
+@pytest.mark.parametrize("case", ["c2", "shared_generic", "acc_forms", "ext_mixed"])
+def test_batched_form_reorders_whole_functions_only(case):
+    """With batched loads the statements are the same multiset as in tape
+    order. For a tape whose functions share no temporary slot (c2) whole
+    function blocks move, the one reading the fewest variables first; a tape
+    with temporaries keeps its order."""
+    eng = dict(problems())[case]
+    plain = _statements(eng.compiled_source(0, 1, 1, 0, 0, True))
+    batched = _statements(eng.compiled_source(0, 1, 1, 0, 8, True))
+    assert sorted(plain) == sorted(batched)
+    words, _, _ = eng.tape_words(1)
+    nv = eng.info().n_vars
+    temps = any(int(w["dst"]) >= nv and (int(w["op"]) < 16 or 32 <= int(w["op"]) < 47 or int(w["op"]) == 16)
+                for w in words)
+    if temps:
+        assert plain == batched
+    if case == "c2":
+        assert plain != batched
+        first_block = []
+        for ln in batched[batched.index("Iv obj = {0.0, 0.0};") + 1:]:
+            if first_block and not ln.startswith("acc0 = iv_add(acc0"):
+                break
+            first_block.append(ln)
+        vars_first = set(re.findall(r"\bs(\d+)\b", " ".join(first_block)))
+        assert len(vars_first) == 7  # the constraint of the 600-node DAG that reads the fewest variables
+
+
 def test_no_roots_kernel_writes_no_roots():
     """The kernel for calls without root intervals has no root stores, and the
     compiler then drops the constraint upper bounds nothing reads: fewer FP64
