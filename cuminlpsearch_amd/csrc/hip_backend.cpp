@@ -295,23 +295,27 @@ std::vector<DeviceTape::BoxCandidate> DeviceTape::autotune(const model::Problem<
     eval_boxes_device(d_lb[k].as<double>(), d_ub[k].as<double>(), n, n, d_f.as<std::uint8_t>(), d_lo.as<double>(),
                       d_hi.as<double>(), nullptr, s);
   };
-  // Per-launch time of runs of 8 back-to-back launches (as callers streaming
-  // boxes issue them), the median over `runs` runs.
+  // Per-launch time of runs of 64 back-to-back launches, the median over
+  // `runs` runs. Callers streaming boxes keep the device busy for long
+  // stretches, and a kernel's sustained rate can differ from its rate in a
+  // short burst after an idle gap (measured: up to 8 % on the 600-node DAG),
+  // so short runs would rank the shapes for the wrong case.
+  constexpr int kRun = 64;
   auto time_it = [&](int runs) {
-    for (int k = 0; k < 3; ++k) {
+    for (int k = 0; k < 8; ++k) {
       launch();
     }
     std::vector<float> ms;
     for (int k = 0; k < runs; ++k) {
       check(hipEventRecord(e0, s), "hipEventRecord");
-      for (int q = 0; q < 8; ++q) {
+      for (int q = 0; q < kRun; ++q) {
         launch();
       }
       check(hipEventRecord(e1, s), "hipEventRecord");
       check(hipEventSynchronize(e1), "hipEventSynchronize");
       float t = 0.0f;
       check(hipEventElapsedTime(&t, e0, e1), "hipEventElapsedTime");
-      ms.push_back(t / 8.0f);
+      ms.push_back(t / static_cast<float>(kRun));
     }
     std::nth_element(ms.begin(), ms.begin() + ms.size() / 2, ms.end());
     return static_cast<double>(ms[ms.size() / 2]);
@@ -387,7 +391,7 @@ std::vector<DeviceTape::BoxCandidate> DeviceTape::autotune(const model::Problem<
         c.waves = sh.waves;
       }
       c.groups = grouped_.n_groups;
-      c.kernel_ms = time_it(5);
+      c.kernel_ms = time_it(3);
       done.push_back(c);
     } catch (const DeviceError&) {
       // a shape the run-time compiler rejects is left out
@@ -411,7 +415,7 @@ std::vector<DeviceTape::BoxCandidate> DeviceTape::autotune(const model::Problem<
   for (std::size_t i : rank) {
     try {
       apply(done[i]);
-      double const ms = time_it(15);
+      double const ms = time_it(9);
       done[i].kernel_ms = ms;
       if (ms < best_ms) {
         best_ms = ms;
