@@ -93,8 +93,7 @@ AggregateBounderImpl::AggregateBounderImpl(std::shared_ptr<const DeviceTape> tap
     std::size_t const o_width = align_up(o_var + slot_count_ * 4, 16);
     std::size_t const o_prefix = align_up(o_width + slot_count_ * 4, 16);
     std::size_t const o_kind = align_up(o_prefix + slot_count_ * 8, 16);
-    item->code = tape_->code();
-    item->n_instr = tape_->n_instr();
+    tape_->item_code(*item, false);
     item->n_vars = static_cast<std::uint32_t>(n_vars_);
     item->parent = reinterpret_cast<const double*>(base + o_parent);
     item->var_kinds = tape_->var_kinds();
@@ -150,6 +149,11 @@ std::span<const aggregate::AggregateBound> AggregateBounderImpl::bound_children(
 
   unsigned char* h = staging_h_.data();
   double const inf = std::numeric_limits<double>::infinity();
+  // the tape's current function groups (a regrouping replaces their code)
+  bool const grouped = tape_->items_grouped_fit(dev::Mode::Aggregate);
+  for (std::size_t i = 0; i < r_count; ++i) {
+    tape_->item_code(reinterpret_cast<dev::Item*>(h + off_items_)[i], grouped);
+  }
   for (std::size_t i = 0; i < r_count; ++i) {
     auto const& r = regions[i];
     auto* child = reinterpret_cast<double*>(h + off_child_) + i * 2 * k_;
@@ -178,7 +182,7 @@ std::span<const aggregate::AggregateBound> AggregateBounderImpl::bound_children(
   check(hipMemcpyAsync(staging_d_.get(), h, used, hipMemcpyHostToDevice, s), "hipMemcpyAsync(aggregate staging)");
   std::uint32_t const n_tiles = static_cast<std::uint32_t>(r_count * tiles_per_region_);
   dev::launch_items(dev::Mode::Aggregate, tape_->extended(), staging_d_.as<dev::Item>(), tile_item_d_.as<std::uint32_t>(), n_tiles,
-                    tape_->n_slots(), s);
+                    tape_->item_slots(grouped), tape_->item_waves(grouped), s);
   check(hipGetLastError(), "launch(k_items aggregate)");
   check(hipMemcpyAsync(child_h_.data(), staging_d_.as<unsigned char>() + off_child_, r_count * 2 * k_ * sizeof(double),
                        hipMemcpyDeviceToHost, s),

@@ -74,7 +74,9 @@ void Runner::run(std::span<const Request> reqs)
   // Sizes.
   std::size_t tiles[3] = {0, 0, 0};
   std::size_t total_regions = 0, n_cand = 0, witness_doubles = 0, data_bytes = 0;
-  std::uint32_t lds[3] = {1, 1, 1};
+  std::uint32_t lds[3] = {1, 1, 1};        // slots of the whole tapes
+  std::uint32_t lds_g[3] = {1, 1, 1};      // slots of the grouped tapes
+  std::uint32_t waves_g[3] = {1, 1, 1};    // most groups of any item
   bool ext[3] = {false, false, false};
   for (std::size_t k = 0; k < n; ++k) {
     std::size_t const i = order[k];
@@ -99,7 +101,9 @@ void Runner::run(std::span<const Request> reqs)
     p.n_elems = n_elems;
     p.n_vars = r.tape->n_vars();
     tiles[pass] += (n_elems + dev::kTile - 1) / dev::kTile;
-    lds[pass] = std::max(lds[pass], r.tape->n_slots());
+    lds[pass] = std::max(lds[pass], r.tape->item_slots(false));
+    lds_g[pass] = std::max(lds_g[pass], r.tape->item_slots(true));
+    waves_g[pass] = std::max(waves_g[pass], r.tape->item_waves(true));
     ext[pass] = ext[pass] || r.tape->extended();
     if (r.mode == dev::Mode::Bound) {
       p.region_off = total_regions;
@@ -114,6 +118,14 @@ void Runner::run(std::span<const Request> reqs)
   }
   if (lds[2] > ctx_->lds_interval_slots() || std::max(lds[0], lds[1]) > ctx_->lds_point_slots()) {
     throw ResourceExhausted("batched request needs more LDS slots than a workgroup holds");
+  }
+  // Each pass runs its items' tapes in their function groups (one wave per
+  // group, as many waves as the most grouped item has) when that workgroup
+  // fits in LDS, else each tape whole on one wave.
+  bool grouped[3];
+  for (int pass = 0; pass < 3; ++pass) {
+    dev::Mode const m = pass == 0 ? dev::Mode::Sample : pass == 1 ? dev::Mode::Enumerate : dev::Mode::Bound;
+    grouped[pass] = waves_g[pass] > 1 && dev::items_lds_bytes(m, lds_g[pass], waves_g[pass]) <= ctx_->lds_bytes();
   }
   std::size_t const total_tiles = tiles[0] + tiles[1] + tiles[2];
   std::size_t const cand_tiles = tiles[0] + tiles[1];
@@ -192,8 +204,7 @@ void Runner::run(std::span<const Request> reqs)
     std::uint32_t const n_tiles = static_cast<std::uint32_t>((p.n_elems + dev::kTile - 1) / dev::kTile);
     dev::Item& it = items[k];
     std::memset(&it, 0, sizeof(it));
-    it.code = r.tape->code();
-    it.n_instr = r.tape->n_instr();
+    r.tape->item_code(it, grouped[pass]);
     it.n_vars = static_cast<std::uint32_t>(p.n_vars);
     it.parent = reinterpret_cast<const double*>(d + off_parent);
     it.var_kinds = r.tape->var_kinds();
@@ -232,7 +243,8 @@ void Runner::run(std::span<const Request> reqs)
     if (counts[pass] > 0) {
       dev::launch_items(m, ext[pass], reinterpret_cast<const dev::Item*>(d + off_items) + first_item,
                         reinterpret_cast<const std::uint32_t*>(d + off_tile_item) + first_tile,
-                        static_cast<std::uint32_t>(tiles[pass]), lds[pass], s);
+                        static_cast<std::uint32_t>(tiles[pass]), grouped[pass] ? lds_g[pass] : lds[pass],
+                        grouped[pass] ? waves_g[pass] : 1u, s);
       check(hipGetLastError(), "launch(k_items batch)");
       ++launches_;
     }

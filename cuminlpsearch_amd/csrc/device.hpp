@@ -31,7 +31,8 @@ struct GroupTable
 /// run of consecutive 64-element tiles, possibly from different problems.
 struct Item
 {
-  const tape::Instr* code;
+  const tape::Instr* code;  // every group's code, then one Nop word
+  GroupTable groups;        // wave g of the item's workgroup runs code[begin[g], begin[g+1])
   std::uint32_t n_instr;
   std::uint32_t n_vars;
   const double* parent;  // n_vars (lb, ub) pairs
@@ -70,12 +71,16 @@ enum class Mode : int
   Aggregate = 3,
 };
 
-/// Launch `n_tiles` workgroups over `items`; `tile_item[t]` names the item of
-/// global tile t. `lds_slots` must be >= every item's tape slot count.
+/// Launch `n_tiles` workgroups of `waves` waves over `items`; `tile_item[t]`
+/// names the item of global tile t. `lds_slots` must be >= every item's tape
+/// slot count and `waves` >= every item's group count (at most kMaxGroups).
 /// `extended`: some item's tape uses the extended op set (then the kernels
 /// that interpret it run; results of the other tapes are the same either way).
 void launch_items(Mode mode, bool extended, const Item* items, const std::uint32_t* tile_item, std::uint32_t n_tiles,
-                  std::uint32_t lds_slots, hipStream_t stream);
+                  std::uint32_t lds_slots, std::uint32_t waves, hipStream_t stream);
+
+/// LDS bytes of one decode-mode workgroup of `waves` waves over `lds_slots` slots.
+std::size_t items_lds_bytes(Mode mode, std::uint32_t lds_slots, std::uint32_t waves);
 
 /// Explicit-box evaluation: box e's variable v is [lb[v*ld+e], ub[v*ld+e]].
 /// Writes feasible[e] and obj_lb[e]; obj_ub and roots (n_outputs intervals,

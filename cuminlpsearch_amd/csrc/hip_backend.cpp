@@ -35,6 +35,7 @@ Context::Context(int device)
   check(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking), "hipStreamCreateWithFlags");
   lds_interval_slots_ = dev::max_lds_slots(true);
   lds_point_slots_ = dev::max_lds_slots(false);
+  lds_bytes_ = dev::lds_limit_bytes();
 }
 
 Context::~Context()
@@ -127,6 +128,28 @@ void DeviceTape::set_box_groups(const model::Problem<double>& problem, std::uint
   for (std::uint32_t i = 0; i <= grouped_.n_groups; ++i) {
     groups_.begin[i] = grouped_.group_begin[i];
   }
+}
+
+void DeviceTape::item_code(dev::Item& it, bool grouped) const
+{
+  if (grouped) {
+    it.code = grouped_code_.as<tape::Instr>();
+    it.groups = groups_;
+    it.n_instr = static_cast<std::uint32_t>(grouped_.code.size());
+  } else {
+    it.code = code_.as<tape::Instr>();
+    it.groups = {};
+    it.groups.n_groups = 1;
+    it.groups.objective_group = 0;
+    it.groups.begin[0] = 0;
+    it.groups.begin[1] = static_cast<std::uint32_t>(tape_.code.size());
+    it.n_instr = static_cast<std::uint32_t>(tape_.code.size());
+  }
+}
+
+bool DeviceTape::items_grouped_fit(dev::Mode mode) const
+{
+  return dev::items_lds_bytes(mode, item_slots(true), item_waves(true)) <= ctx_->lds_bytes();
 }
 
 void DeviceTape::set_box_engine(BoxEngine e)
@@ -557,8 +580,7 @@ RoleImpl::RoleImpl(std::shared_ptr<const DeviceTape> tape, dev::Mode mode, const
   // are rewritten per call.
   auto* item = reinterpret_cast<dev::Item*>(staging_h_.data());
   auto* base = staging_d_.as<unsigned char>();
-  item->code = tape_->code();
-  item->n_instr = tape_->n_instr();
+  tape_->item_code(*item, false);
   item->n_vars = static_cast<std::uint32_t>(n_vars_);
   item->parent = reinterpret_cast<const double*>(base + off_parent_);
   item->var_kinds = tape_->var_kinds();
@@ -621,9 +643,12 @@ void RoleImpl::run(std::span<const interval> box, std::span<const std::size_t> v
     prefix[j] = running;
     running *= widths[j];
   }
+  // the tape's current function groups (a regrouping replaces their code)
+  bool const grouped = tape_->items_grouped_fit(mode_);
+  tape_->item_code(*item, grouped);
   check(hipMemcpyAsync(staging_d_.get(), h, staging_bytes_, hipMemcpyHostToDevice, s), "hipMemcpyAsync(staging)");
-  dev::launch_items(mode_, tape_->extended(), staging_d_.as<dev::Item>(), tile_item_d_.as<std::uint32_t>(), n_tiles_, tape_->n_slots(),
-                    s);
+  dev::launch_items(mode_, tape_->extended(), staging_d_.as<dev::Item>(), tile_item_d_.as<std::uint32_t>(), n_tiles_,
+                    tape_->item_slots(grouped), tape_->item_waves(grouped), s);
   check(hipGetLastError(), "launch(k_items)");
   if (mode_ == dev::Mode::Bound) {
     check(hipMemcpyAsync(feasible_h_.data(), feasible_d_.get(), n_regions_, hipMemcpyDeviceToHost, s),

@@ -138,6 +138,8 @@ public:
   void activate() const;
   std::uint32_t lds_interval_slots() const { return lds_interval_slots_; }
   std::uint32_t lds_point_slots() const { return lds_point_slots_; }
+  /// LDS one workgroup may use.
+  std::size_t lds_bytes() const { return lds_bytes_; }
   std::size_t free_bytes() const;
 
 private:
@@ -145,6 +147,7 @@ private:
   hipStream_t stream_ = nullptr;
   std::uint32_t lds_interval_slots_ = 0;
   std::uint32_t lds_point_slots_ = 0;
+  std::size_t lds_bytes_ = 0;
 };
 
 /// A problem compiled to a tape and uploaded once.
@@ -221,6 +224,14 @@ public:
   std::uint32_t n_vars() const { return tape_.n_vars; }
   std::uint32_t n_instr() const { return static_cast<std::uint32_t>(tape_.code.size()); }
   std::uint32_t n_slots() const { return tape_.n_slots; }
+  /// What a decode-mode item runs: the tape in its explicit-box function
+  /// groups (one wave per group; grouped = true) or the whole tape on one
+  /// wave. The values are the same; the groups shorten each wave's run.
+  void item_code(dev::Item& it, bool grouped) const;
+  std::uint32_t item_slots(bool grouped) const { return grouped ? grouped_.n_slots : tape_.n_slots; }
+  std::uint32_t item_waves(bool grouped) const { return grouped ? groups_.n_groups : 1u; }
+  /// Whether a decode-mode workgroup of this tape's groups fits in LDS.
+  bool items_grouped_fit(dev::Mode mode) const;
   /// The tape uses the extended op set (Div, PowN, transcendental, ...).
   bool extended() const { return tape_.extended; }
   const std::shared_ptr<Context>& context() const { return ctx_; }

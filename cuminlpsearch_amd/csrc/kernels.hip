@@ -23,6 +23,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include <cfloat>
 #include <cmath>
 #include <cstdint>
@@ -532,13 +534,23 @@ __global__ __launch_bounds__(kTile) void k_points(const Instr* __restrict__ code
   }
 }
 
-// Decode-mode item kernels. Each workgroup is one 64-element tile of one item.
+// Decode-mode item kernels. Each workgroup is one 64-element tile of one item,
+// one wave per function group of the item's tape (blockDim.x / 64 waves, the
+// most groups of any item in the launch; a wave past its item's groups runs
+// no code). Wave 0 decodes the tile's sub-boxes or points into the shared
+// variable slots; each wave then runs its group's code over them; the groups'
+// verdicts and the objective meet in LDS after the lds_slots slots, and wave
+// 0 writes the results. Groups share only the (read-only) variable slots, so
+// the values are those of the whole tape run by one wave.
 template<int M, bool Ext>
-__global__ __launch_bounds__(kTile) void k_items(const Item* __restrict__ items,
-                                                  const std::uint32_t* __restrict__ tile_item)
+__global__ __launch_bounds__(kTile * kMaxGroups) void k_items(const Item* __restrict__ items,
+                                                               const std::uint32_t* __restrict__ tile_item,
+                                                               std::uint32_t lds_slots)
 {
   extern __shared__ double2 lds[];
-  unsigned const lane = threadIdx.x;
+  unsigned const lane = threadIdx.x % kTile;
+  unsigned const wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kTile);
+  unsigned const n_waves = blockDim.x / kTile;
   std::uint32_t const tile = blockIdx.x;
   Item const& it = items[tile_item[tile]];
   std::uint32_t const lt = tile - it.tile_begin;
@@ -548,27 +560,57 @@ __global__ __launch_bounds__(kTile) void k_items(const Item* __restrict__ items,
   std::uint32_t const S = it.samples;
   std::uint32_t const region = (M == static_cast<int>(Mode::Sample)) ? ee / S : ee;
   std::uint32_t const sample = (M == static_cast<int>(Mode::Sample)) ? ee - region * S : 0;
-  ConstInstrPtr const code = as_constant(it.code);
+  std::uint32_t const n_groups = it.groups.n_groups;
+  std::uint32_t const g_begin = wave < n_groups ? it.groups.begin[wave] : 0u;
+  std::uint32_t const g_len = wave < n_groups ? it.groups.begin[wave + 1] - g_begin : 0u;
+  ConstInstrPtr const code = as_constant(it.code) + g_begin;
   std::uint32_t const n_vars = it.n_vars;
+  constexpr bool kInterval = M == static_cast<int>(Mode::Bound) || M == static_cast<int>(Mode::Aggregate);
+  // per-group verdicts [kMaxGroups][64] and the objective [64], after the slots
+  auto* const feas_g = reinterpret_cast<std::uint32_t*>(reinterpret_cast<unsigned char*>(lds)
+                                                        + std::size_t {lds_slots} * kTile * (kInterval ? 16u : 8u));
 
-  if constexpr (M == static_cast<int>(Mode::Bound) || M == static_cast<int>(Mode::Aggregate)) {
+  if constexpr (kInterval) {
     double2* const Sl = lds;
-    for (std::uint32_t v = 0; v < n_vars; ++v) {
-      Sl[v * kTile + lane] = make_double2(-it.parent[2 * v], it.parent[2 * v + 1]);
+    if (wave == 0) {
+      for (std::uint32_t v = 0; v < n_vars; ++v) {
+        Sl[v * kTile + lane] = make_double2(-it.parent[2 * v], it.parent[2 * v + 1]);
+      }
+      std::uint32_t idx = region;
+      for (std::uint32_t j = 0; j < it.slot_count; ++j) {
+        std::uint32_t const w = it.slot_width[j];
+        std::uint32_t const part = idx % w;
+        idx /= w;
+        std::uint32_t const vid = it.slot_var[j];
+        Bounds const b = slot_bounds(it.slot_kind[j], {it.parent[2 * vid], it.parent[2 * vid + 1]}, part, w);
+        Sl[vid * kTile + lane] = make_double2(-b.lo, b.hi);
+      }
     }
-    std::uint32_t idx = region;
-    for (std::uint32_t j = 0; j < it.slot_count; ++j) {
-      std::uint32_t const w = it.slot_width[j];
-      std::uint32_t const part = idx % w;
-      idx /= w;
-      std::uint32_t const vid = it.slot_var[j];
-      Bounds const b = slot_bounds(it.slot_kind[j], {it.parent[2 * vid], it.parent[2 * vid + 1]}, part, w);
-      Sl[vid * kTile + lane] = make_double2(-b.lo, b.hi);
+    if (n_waves > 1) {
+      __syncthreads();
     }
     int const t = enter_round_up();
-    IntervalResult const res = run_interval<Ext>(code, it.n_instr, Sl, lane, t, nullptr, 0, 0);
-    Bnd const obj = to_output(res.obj, t);
+    IntervalResult res = run_interval<Ext>(code, g_len, Sl, lane, t, nullptr, 0, 0);
+    Bnd obj = to_output(res.obj, t);
     leave_round_up(t, obj.lb, obj.ub, res.feasible);
+    if (n_waves > 1) {
+      auto* const obj_s = reinterpret_cast<double2*>(feas_g + kMaxGroups * kTile);
+      feas_g[wave * kTile + lane] = res.feasible ? 1u : 0u;
+      if (wave == it.groups.objective_group) {
+        obj_s[lane] = to_d2(obj);
+      }
+      __syncthreads();
+      if (wave != 0) {
+        return;
+      }
+      bool f = true;
+      for (unsigned g = 0; g < n_waves; ++g) {
+        f = f && feas_g[g * kTile + lane] != 0u;
+      }
+      res.feasible = f;
+      double2 const o = obj_s[lane];
+      obj = {o.x, o.y};
+    }
     if constexpr (M == static_cast<int>(Mode::Bound)) {
       if (active) {
         it.feasible[region] = res.feasible ? 1 : 0;
@@ -605,21 +647,44 @@ __global__ __launch_bounds__(kTile) void k_items(const Item* __restrict__ items,
   } else {
     double* const P = reinterpret_cast<double*>(lds);
     bool const sampling = M == static_cast<int>(Mode::Sample);
-    for (std::uint32_t v = 0; v < n_vars; ++v) {
-      double const lo = it.parent[2 * v];
-      double const hi = it.parent[2 * v + 1];
-      P[v * kTile + lane] = sampling ? draw(it.var_kinds[v] != 0, lo, hi, v, sample, region, it.salt) : lo;
+    if (wave == 0) {
+      for (std::uint32_t v = 0; v < n_vars; ++v) {
+        double const lo = it.parent[2 * v];
+        double const hi = it.parent[2 * v + 1];
+        P[v * kTile + lane] = sampling ? draw(it.var_kinds[v] != 0, lo, hi, v, sample, region, it.salt) : lo;
+      }
+      std::uint32_t idx = region;
+      for (std::uint32_t j = 0; j < it.slot_count; ++j) {
+        std::uint32_t const w = it.slot_width[j];
+        std::uint32_t const part = idx % w;
+        idx /= w;
+        std::uint32_t const vid = it.slot_var[j];
+        Bounds const b = slot_bounds(it.slot_kind[j], {it.parent[2 * vid], it.parent[2 * vid + 1]}, part, w);
+        P[vid * kTile + lane] =
+            sampling ? draw(it.var_kinds[vid] != 0, b.lo, b.hi, vid, sample, region, it.salt) : b.lo;
+      }
     }
-    std::uint32_t idx = region;
-    for (std::uint32_t j = 0; j < it.slot_count; ++j) {
-      std::uint32_t const w = it.slot_width[j];
-      std::uint32_t const part = idx % w;
-      idx /= w;
-      std::uint32_t const vid = it.slot_var[j];
-      Bounds const b = slot_bounds(it.slot_kind[j], {it.parent[2 * vid], it.parent[2 * vid + 1]}, part, w);
-      P[vid * kTile + lane] = sampling ? draw(it.var_kinds[vid] != 0, b.lo, b.hi, vid, sample, region, it.salt) : b.lo;
+    if (n_waves > 1) {
+      __syncthreads();
     }
-    PointResult const res = run_point<Ext>(code, it.n_instr, P, lane, nullptr, 0, 0);
+    PointResult res = run_point<Ext>(code, g_len, P, lane, nullptr, 0, 0);
+    if (n_waves > 1) {
+      auto* const obj_s = reinterpret_cast<double*>(feas_g + kMaxGroups * kTile);
+      feas_g[wave * kTile + lane] = res.feasible ? 1u : 0u;
+      if (wave == it.groups.objective_group) {
+        obj_s[lane] = res.obj;
+      }
+      __syncthreads();
+      if (wave != 0) {
+        return;
+      }
+      bool f = true;
+      for (unsigned g = 0; g < n_waves; ++g) {
+        f = f && feas_g[g * kTile + lane] != 0u;
+      }
+      res.feasible = f;
+      res.obj = obj_s[lane];
+    }
     // a point needs a finite objective (log(0) = -inf is no incumbent)
     double v = (active && res.feasible) ? res.obj : INFINITY;
     if (v != v || v == -INFINITY) {
@@ -720,22 +785,27 @@ namespace
 
 template<int M, bool Ext>
 void launch_items_as(const Item* items, const std::uint32_t* tile_item, std::uint32_t n_tiles, std::size_t bytes,
-                     hipStream_t stream)
+                     std::uint32_t lds_slots, std::uint32_t waves, hipStream_t stream)
 {
   static bool once = (allow_large_lds(k_items<M, Ext>), true);
   (void)once;
-  hipLaunchKernelGGL((k_items<M, Ext>), dim3(n_tiles), dim3(kTile), bytes, stream, items, tile_item);
+  hipLaunchKernelGGL((k_items<M, Ext>), dim3(n_tiles), dim3(kTile * waves), bytes, stream, items, tile_item,
+                     lds_slots);
 }
 
 template<bool Ext>
 void launch_items_mode(Mode mode, const Item* items, const std::uint32_t* tile_item, std::uint32_t n_tiles,
-                       std::size_t bytes, hipStream_t stream)
+                       std::size_t bytes, std::uint32_t lds_slots, std::uint32_t waves, hipStream_t stream)
 {
   switch (mode) {
-    case Mode::Bound: launch_items_as<0, Ext>(items, tile_item, n_tiles, bytes, stream); break;
-    case Mode::Sample: launch_items_as<1, Ext>(items, tile_item, n_tiles, bytes, stream); break;
-    case Mode::Enumerate: launch_items_as<2, Ext>(items, tile_item, n_tiles, bytes, stream); break;
-    case Mode::Aggregate: launch_items_as<3, Ext>(items, tile_item, n_tiles, bytes, stream); break;
+    case Mode::Bound: launch_items_as<0, Ext>(items, tile_item, n_tiles, bytes, lds_slots, waves, stream); break;
+    case Mode::Sample: launch_items_as<1, Ext>(items, tile_item, n_tiles, bytes, lds_slots, waves, stream); break;
+    case Mode::Enumerate:
+      launch_items_as<2, Ext>(items, tile_item, n_tiles, bytes, lds_slots, waves, stream);
+      break;
+    case Mode::Aggregate:
+      launch_items_as<3, Ext>(items, tile_item, n_tiles, bytes, lds_slots, waves, stream);
+      break;
   }
 }
 
@@ -767,18 +837,25 @@ void launch_points_as(const Instr* code, std::uint32_t n_instr, std::uint32_t n_
 
 }  // namespace
 
+std::size_t items_lds_bytes(Mode mode, std::uint32_t lds_slots, std::uint32_t waves)
+{
+  bool const interval = mode == Mode::Bound || mode == Mode::Aggregate;
+  std::size_t const slots = static_cast<std::size_t>(lds_slots) * kTile * (interval ? 16 : 8);
+  return waves > 1 ? slots + kMaxGroups * kTile * 4 + kTile * 16 : slots;
+}
+
 void launch_items(Mode mode, bool extended, const Item* items, const std::uint32_t* tile_item, std::uint32_t n_tiles,
-                  std::uint32_t lds_slots, hipStream_t stream)
+                  std::uint32_t lds_slots, std::uint32_t waves, hipStream_t stream)
 {
   if (n_tiles == 0) {
     return;
   }
-  bool const interval = mode == Mode::Bound || mode == Mode::Aggregate;
-  std::size_t const bytes = static_cast<std::size_t>(lds_slots) * kTile * (interval ? 16 : 8);
+  waves = std::clamp<std::uint32_t>(waves, 1, kMaxGroups);
+  std::size_t const bytes = items_lds_bytes(mode, lds_slots, waves);
   if (extended) {
-    launch_items_mode<true>(mode, items, tile_item, n_tiles, bytes, stream);
+    launch_items_mode<true>(mode, items, tile_item, n_tiles, bytes, lds_slots, waves, stream);
   } else {
-    launch_items_mode<false>(mode, items, tile_item, n_tiles, bytes, stream);
+    launch_items_mode<false>(mode, items, tile_item, n_tiles, bytes, lds_slots, waves, stream);
   }
 }
 

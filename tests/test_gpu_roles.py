@@ -162,3 +162,51 @@ def test_shape_errors(ctx):
     with pytest.raises(C.CmiError) as ei:
         cmi.Role(tape, C.ROLE_ENUMERATOR, [C.SLOT_CONTINUOUS], 4)
     assert ei.value.status == C.INVALID_CONFIG
+
+
+@pytest.mark.parametrize("groups", [1, 3, 8])
+def test_roles_in_function_groups_match_oracle(ctx, groups):
+    """Decode-mode items run the tape in its function groups, one wave per
+    group: the bounder, sampler and enumerator give the oracle's results for
+    every grouping of a 600-node DAG (groups of about 30 tape words)."""
+    eng = P.c2()
+    a = eng.arrays()
+    tape = cmi.Tape(ctx, eng)
+    assert tape.set_box_groups(groups) == groups
+    kinds, vids, widths = [C.SLOT_CONTINUOUS] * 3, [3, 17, 29], [4, 3, 5]
+    n = int(np.prod(widths))
+    bnd = cmi.Role(tape, C.ROLE_BOUNDER, kinds, n)
+    f_d, lb_d = bnd.bound(a.box, vids, widths)
+    f_o, lb_o = orc.bound(a, a.box, vids, widths, kinds)
+    assert np.array_equal(f_d, f_o) and same_bits(lb_d, lb_o)
+    smp = cmi.Role(tape, C.ROLE_SAMPLER, kinds, n, samples=33)
+    for salt in (1, 7):
+        d = smp.sample(a.box, vids, widths, salt)
+        found, value, index, point = orc.candidate(a, a.box, vids, widths, kinds, 33, salt)
+        assert d.found == found and same_bits([d.value], [value]) and d.index == index and same_bits(d.point, point)
+    # the enumerator needs integer slots: a QCQP whose first 8 variables are integers in [0, 3]
+    ieng = cmi.EngineProblem.qcqp(17, n=24, m=12, n_int=8, int_lb=0.0, int_ub=3.0, cont_lb=-5.0, cont_ub=5.0,
+                                  pair_density=0.3)
+    ia = ieng.arrays()
+    itape = cmi.Tape(ctx, ieng)
+    itape.set_box_groups(groups)
+    ekinds, evids, ewidths = [C.SLOT_INTEGER_ENUMERATE] * 3, [0, 3, 6], [4, 4, 4]
+    enu = cmi.Role(itape, C.ROLE_ENUMERATOR, ekinds, 64)
+    d = enu.enumerate(ia.box, evids, ewidths)
+    found, value, index, point = orc.candidate(ia, ia.box, evids, ewidths, ekinds, 0, 0)
+    assert d.found == found and same_bits([d.value], [value]) and d.index == index and same_bits(d.point, point)
+
+
+def test_regrouping_after_role_build(ctx):
+    """A role built before the tape is regrouped runs the new groups' code."""
+    eng = P.c2()
+    a = eng.arrays()
+    tape = cmi.Tape(ctx, eng)
+    tape.set_box_groups(8)
+    kinds, vids, widths = [C.SLOT_CONTINUOUS] * 2, [0, 5], [8, 8]
+    bnd = cmi.Role(tape, C.ROLE_BOUNDER, kinds, 64)
+    f_o, lb_o = orc.bound(a, a.box, vids, widths, kinds)
+    for g in (2, 8, 1):
+        tape.set_box_groups(g)
+        f_d, lb_d = bnd.bound(a.box, vids, widths)
+        assert np.array_equal(f_d, f_o) and same_bits(lb_d, lb_o)
