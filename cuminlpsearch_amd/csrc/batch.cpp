@@ -1,9 +1,14 @@
 #include "batch.hpp"
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cstring>
+#include <exception>
+#include <functional>
 #include <limits>
+#include <mutex>
+#include <thread>
 
 #include "cmi/policy.hpp"
 
@@ -18,6 +23,46 @@ namespace cmi::batch
 namespace
 {
 std::size_t align_up(std::size_t x, std::size_t a) { return (x + a - 1) / a * a; }
+
+/// fn(i) for i in [0, n) on up to min(16, hardware threads) host threads;
+/// rethrows the first exception after every thread has stopped.
+void parallel_for(std::size_t n, const std::function<void(std::size_t)>& fn)
+{
+  std::size_t const threads = std::min<std::size_t>(n, std::clamp<std::size_t>(std::thread::hardware_concurrency(), 1, 16));
+  if (threads <= 1) {
+    for (std::size_t i = 0; i < n; ++i) {
+      fn(i);
+    }
+    return;
+  }
+  std::atomic<std::size_t> next {0};
+  std::mutex mu;
+  std::exception_ptr error;
+  auto work = [&] {
+    for (std::size_t i; (i = next.fetch_add(1)) < n;) {
+      try {
+        fn(i);
+      } catch (...) {
+        std::lock_guard<std::mutex> lock(mu);
+        if (!error) {
+          error = std::current_exception();
+        }
+        next = n;
+      }
+    }
+  };
+  std::vector<std::thread> pool;
+  for (std::size_t t = 1; t < threads; ++t) {
+    pool.emplace_back(work);
+  }
+  work();
+  for (std::thread& t : pool) {
+    t.join();
+  }
+  if (error) {
+    std::rethrow_exception(error);
+  }
+}
 
 struct Candidate
 {
@@ -297,14 +342,14 @@ std::vector<Result> solve_batch(std::shared_ptr<hip::Context> ctx,
   if (o.has_known_primal) {
     so.known_primal_bound = o.known_primal;
   }
-  std::vector<std::unique_ptr<hip::DeviceTape>> tapes;
-  std::vector<std::unique_ptr<search::SolveState>> states;
-  tapes.reserve(n);
-  states.reserve(n);
-  for (std::size_t i = 0; i < n; ++i) {
-    tapes.push_back(std::make_unique<hip::DeviceTape>(ctx, *problems[i]));
-    states.push_back(std::make_unique<search::SolveState>(*problems[i], pol, so));
-  }
+  // Each solve's tape (compiled on the host, uploaded once) and search state
+  // are independent of the others': build them on several host threads.
+  std::vector<std::unique_ptr<hip::DeviceTape>> tapes(n);
+  std::vector<std::unique_ptr<search::SolveState>> states(n);
+  parallel_for(n, [&](std::size_t i) {
+    tapes[i] = std::make_unique<hip::DeviceTape>(ctx, *problems[i]);
+    states[i] = std::make_unique<search::SolveState>(*problems[i], pol, so);
+  });
   std::vector<Result> results(n);
   std::vector<bool> done(n, false);
   Runner runner(ctx);

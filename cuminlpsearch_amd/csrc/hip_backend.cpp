@@ -82,12 +82,12 @@ DeviceTape::DeviceTape(std::shared_ptr<Context> ctx, const model::Problem<double
     , opt_(opt)
 {
   problem.validate();
+  ctx_->activate();  // the calling thread may be any host thread
   tape_ = tape::compile(problem, opt);
-  if (tape_.n_slots > ctx_->lds_interval_slots() || dev::boxes_lds_bytes(tape_.n_slots) > dev::lds_limit_bytes()) {
+  if (tape_.n_slots > ctx_->lds_interval_slots() || dev::boxes_lds_bytes(tape_.n_slots) > ctx_->lds_bytes()) {
     throw ResourceExhausted("tape needs " + std::to_string(tape_.n_slots) + " slots per lane; LDS holds "
                             + std::to_string(ctx_->lds_interval_slots()) + " interval slots per 64-lane workgroup");
   }
-  ctx_->activate();
   upload_code(tape_.code, code_);
   std::vector<std::uint8_t> kinds(problem.var_kinds.size());
   for (std::size_t i = 0; i < kinds.size(); ++i) {
