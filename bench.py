@@ -9,8 +9,9 @@ bilinear pairs, 600 reachable DAG nodes, evaluated over 65,536 sub-boxes
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-serve] [--no-batch] [--no-cpu]
 
 Also reported, on the same line: "serving" -- served (continuous batching) vs
-plain (one solve at a time) solves/s and p50/p99 latency on a Poisson stream of
-seeded QCQPs -- and "batching" -- 512 seeded QCQPs solved in shared launches vs
+plain (one solve at a time) solves/s and p50/p99 latency on Poisson streams of
+seeded QCQPs at 50, 200 and 2,000 req/s -- and "batching" -- 512 seeded QCQPs
+solved in shared launches vs
 one after another; "autotune" -- every explicit-box kernel shape the autotuner
 timed on this workload; "cpu_baseline" -- the CPU restatement on the host.
 """
@@ -93,38 +94,49 @@ def _same_results(a_outs, b_outs) -> bool:
                and a.stop_reason == b.stop_reason for a, b in zip(a_outs, b_outs))
 
 
-def serve_comparison(ctx, n_req: int = 128, rate: float = 200.0, seed: int = 7, n_max: int = 16,
+def serve_comparison(ctx, n_req: int = 128, rates=(50.0, 200.0, 2000.0), seed: int = 7, n_max: int = 16,
                      iter_limit: int = 1000):
     """Served (continuous batching) vs plain (one solve at a time) on the same
-    Poisson stream of seeded QCQPs: n log-uniform in [4, n_max], m = n/2,
-    bilinear density U[0.1, 0.3], gap 1e-6, iteration cap per request."""
+    Poisson streams of seeded QCQPs, one stream per arrival rate: n
+    log-uniform in [4, n_max], m = n/2, bilinear density U[0.1, 0.3], gap 1e-6,
+    iteration cap per request. The first fields are those of the 200 req/s
+    stream; "rates" holds every stream."""
     import cuminlpsearch_amd as cmi
 
     rng = np.random.default_rng(seed)
     sizes = np.floor(np.exp(rng.uniform(np.log(4), np.log(n_max + 1), n_req))).astype(int)
     probs = [cmi.EngineProblem.qcqp(1000 + i, n=int(n), m=int(n) // 2, cont_lb=-5.0, cont_ub=5.0,
                                     pair_density=float(rng.uniform(0.1, 0.3))) for i, n in enumerate(sizes)]
-    arrival = np.cumsum(rng.exponential(1.0 / rate, n_req))
-    arrival -= arrival[0]
+    gaps = rng.exponential(1.0, n_req)
     opts = dict(partition_num=2, enumerate_cap=2, max_cycle_size=4, sample_points=16, iter_limit=iter_limit,
                 tolerance=1e-6)
-    outs_s, fin_s, lat_s, st_s = cmi.serve_stream(ctx, probs, arrival, max_active=256, **opts)
-    outs_p, fin_p, lat_p, st_p = cmi.serve_plain(ctx, probs, arrival, **opts)
+    per_rate = []
+    for rate in rates:
+        arrival = np.cumsum(gaps / rate)
+        arrival -= arrival[0]
+        outs_s, fin_s, lat_s, st_s = cmi.serve_stream(ctx, probs, arrival, max_active=256, **opts)
+        outs_p, fin_p, lat_p, st_p = cmi.serve_plain(ctx, probs, arrival, **opts)
+        per_rate.append({
+            "arrival_rate_per_s": rate,
+            "served_solves_per_s": n_req / st_s.wall_seconds,
+            "plain_solves_per_s": n_req / st_p.wall_seconds,
+            "speedup": st_p.wall_seconds / st_s.wall_seconds,
+            "served_p50_ms": float(np.percentile(lat_s, 50) * 1e3),
+            "served_p99_ms": float(np.percentile(lat_s, 99) * 1e3),
+            "plain_p50_ms": float(np.percentile(lat_p, 50) * 1e3),
+            "plain_p99_ms": float(np.percentile(lat_p, 99) * 1e3),
+            "results_identical": _same_results(outs_s, outs_p),
+            "converged_fraction": _converged(outs_s),
+        })
+    main = next((r for r in per_rate if r["arrival_rate_per_s"] == 200.0), per_rate[0])
     return {
-        "workload": f"Poisson stream, {n_req} QCQPs, n log-uniform [4,{n_max}], m=n/2, density U[0.1,0.3], "
-                    f"gap 1e-6, cap {iter_limit} iterations, seed {seed}",
+        "workload": f"Poisson streams, {n_req} QCQPs each, n log-uniform [4,{n_max}], m=n/2, density U[0.1,0.3], "
+                    f"gap 1e-6, cap {iter_limit} iterations, seed {seed}, rates {list(rates)} req/s",
         "requests": n_req,
-        "arrival_rate_per_s": rate,
-        "served_solves_per_s": n_req / st_s.wall_seconds,
-        "plain_solves_per_s": n_req / st_p.wall_seconds,
-        "speedup": st_p.wall_seconds / st_s.wall_seconds,
-        "served_p50_ms": float(np.percentile(lat_s, 50) * 1e3),
-        "served_p99_ms": float(np.percentile(lat_s, 99) * 1e3),
-        "plain_p50_ms": float(np.percentile(lat_p, 50) * 1e3),
-        "plain_p99_ms": float(np.percentile(lat_p, 99) * 1e3),
-        "converged_fraction": _converged(outs_s),
+        **main,
         "nodes_enqueued_total": int(sum(o.enqueued for o in outs_s)),
-        "results_identical": _same_results(outs_s, outs_p),
+        "results_identical": all(r["results_identical"] for r in per_rate),
+        "rates": per_rate,
     }
 
 
