@@ -158,6 +158,10 @@ def random_ext_dag(seed: int, n_vars: int = 4, n_ops: int = 40, n_con: int = 3, 
 C1_SPEC = dict(n=4, m=2, cont_lb=-10.0, cont_ub=10.0, symmetric_pairs=True)
 C2_SPEC = dict(n=32, m=16, cont_lb=-5.0, cont_ub=5.0, pair_density=0.2, target_nodes=600)
 C2_SEED = 2026
+# MINLP B&B: 64 variables, the first 16 integer in [0, 10], the rest continuous
+# in [-5, 5]; 32 constraints; 10 % bilinear pairs plus squares; about 1,500 DAG nodes
+C3_SPEC = dict(n=64, m=32, n_int=16, int_lb=0.0, int_ub=10.0, cont_lb=-5.0, cont_ub=5.0, pair_density=0.1,
+               target_nodes=1500)
 
 
 def c1(seed: int) -> cmi.EngineProblem:
@@ -166,6 +170,10 @@ def c1(seed: int) -> cmi.EngineProblem:
 
 def c2() -> cmi.EngineProblem:
     return cmi.EngineProblem.qcqp(C2_SEED, **C2_SPEC)
+
+
+def c3(seed: int) -> cmi.EngineProblem:
+    return cmi.EngineProblem.qcqp(seed, **C3_SPEC)
 
 
 def random_boxes(seed: int, n_vars: int, n: int, lo: float, hi: float, wmin: float = 1e-4, wmax: float = 10.0):

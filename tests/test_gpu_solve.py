@@ -34,6 +34,23 @@ def test_c1_solve_matches_oracle(ctx, seed):
     assert np.array_equal(best_d.view(np.uint64), best_o.view(np.uint64))
 
 
+C3_OPTS = dict(partition_num=2, enumerate_cap=2, max_cycle_size=4, sample_points=64, iter_limit=300)
+
+
+@pytest.mark.parametrize("seed", [0, 1])
+def test_c3_minlp_matches_oracle(ctx, seed):
+    """The 64-variable MINLP (16 integers in [0, 10], 32 constraints, 64 samples
+    per region): 300 iterations of the device search equal the oracle's, every
+    counter and both bounds bit for bit."""
+    eng = P.c3(seed)
+    assert eng.info().n_vars == 64
+    d, best_d = cmi.solve(ctx, eng, **C3_OPTS)
+    o, best_o = orc.solve(eng.arrays(), **C3_OPTS)
+    same_outcome(d, o)
+    assert d.iterations == 300 and d.enqueued > 1000
+    assert np.array_equal(best_d.view(np.uint64), best_o.view(np.uint64))
+
+
 def test_ex2_1_2_brackets_published_optimum(ctx):
     """qps.cu:98-162 settings: FanOut 20, max_cycle_size 2, 100 samples, tol 1e-6."""
     eng = cmi.EngineProblem.from_problem(P.ex2_1_2())
