@@ -118,3 +118,19 @@ def test_over_budget_and_shape_errors(ctx):
     with pytest.raises(C.CmiError) as ei:
         agg.bound_children(np.stack([a.box] * 3), [1, 0], [16, 4])  # 3 regions > max_batch 2
     assert ei.value.status == C.SHAPE_MISMATCH
+
+
+@pytest.mark.parametrize("groups", [1, 4, 8])
+def test_aggregate_in_function_groups_equals_oracle(ctx, groups):
+    """The aggregate tail over a 600-node DAG run in 1, 4 or 8 function groups
+    (one wave per group per tile): every child's lower bound, hull upper bound
+    and feasibility equal the oracle's bit for bit."""
+    eng = P.c2()
+    a = eng.arrays()
+    tape = cmi.Tape(ctx, eng)
+    assert tape.set_box_groups(groups) == groups
+    kinds, vids, widths = A.partition(a.box, 8, 1024)
+    agg = cmi.AggregateBounder(tape, kinds, 1024, 8)
+    lb, ub, fe = agg.bound_children(a.box, vids, widths)
+    lb_o, ub_o, fe_o = orc.aggregate(a, a.box, vids, widths, kinds, 8)
+    assert same_bits(lb[0], lb_o) and same_bits(ub[0], ub_o) and np.array_equal(fe[0], fe_o)
