@@ -31,15 +31,22 @@ def problems():
         yield name, cmi.EngineProblem.from_problem(op_cases.EXT_CASES[name]())
 
 
-@pytest.mark.parametrize("persistent", [0, 1])
+@pytest.mark.parametrize("form", ["direct", "persistent", "batched_no_roots"])
 @pytest.mark.parametrize("case", [name for name, _ in problems()])
-def test_generated_source_regions(tmp_path, case, persistent):
+def test_generated_source_regions(tmp_path, case, form):
+    """Every form of the generated source -- direct, persistent, and the direct
+    form with batched loads (scheduling barriers) and no root stores -- keeps
+    every rounding instruction inside the one round-up region."""
     if not pathlib.Path(HIPCC).exists():
         pytest.skip("hipcc not available")
     eng = dict(problems())[case]
     src = tmp_path / "k.hip"
+    if form == "batched_no_roots":
+        text = eng.compiled_source(0, 0, 0, 0, 8, False)
+    else:
+        text = eng.compiled_source(1 if form == "persistent" else 0)
     # hiprtc includes the HIP device header by itself; hipcc needs it named
-    src.write_text("#include <hip/hip_runtime.h>\n" + eng.compiled_source(persistent))
+    src.write_text("#include <hip/hip_runtime.h>\n" + text)
     out = tmp_path / "k.s"
     subprocess.run([HIPCC, *RTC_FLAGS, "--cuda-device-only", "-S", "-o", str(out), str(src)], check=True,
                    capture_output=True)
