@@ -10,7 +10,8 @@ bilinear pairs, 600 reachable DAG nodes, evaluated over 65,536 sub-boxes
 
 Also reported, on the same line: "serving" -- served (continuous batching) vs
 plain (one solve at a time) solves/s and p50/p99 latency on Poisson streams of
-seeded QCQPs at 50, 200 and 2,000 req/s -- and "batching" -- 512 seeded QCQPs
+seeded QCQPs at 50, 200 and 2,000 req/s (n up to 16), and at 200 req/s over
+the full size range n in [4, 96] ("wide") -- and "batching" -- 512 seeded QCQPs
 solved in shared launches vs
 one after another; "autotune" -- every explicit-box kernel shape the autotuner
 timed on this workload; "cpu_baseline" -- the CPU restatement on the host.
@@ -452,6 +453,9 @@ def main(argv=None) -> int:
 
     if not a.no_serve:
         line["serving"] = serve_comparison(ctx)
+        # the survey's size range (n up to 96), on a shorter stream and cap
+        wide = serve_comparison(ctx, n_req=32, rates=(200.0,), seed=8, n_max=96, iter_limit=100)
+        line["serving"]["wide"] = {k: v for k, v in wide.items() if k != "rates"}
     if not a.no_batch:
         line["batching"] = batch_comparison(ctx)
 
