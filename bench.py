@@ -13,8 +13,8 @@ plain (one solve at a time) solves/s and p50/p99 latency on Poisson streams of
 seeded QCQPs at 50, 200 and 2,000 req/s (n up to 16), and at 200 req/s over
 the full size range n in [4, 96] ("wide") -- and "batching" -- 512 seeded QCQPs
 solved in shared launches vs one after another; "autotune" -- every
-explicit-box kernel shape the autotuner
-timed on this workload; "cpu_baseline" -- the CPU restatement on the host.
+explicit-box kernel shape the autotuner timed on this workload;
+"cpu_baseline" -- the CPU restatement on the host.
 """
 from __future__ import annotations
 
