@@ -357,12 +357,14 @@ typedef struct
 
 /* Autotuner: time the interpreter and the compiled shapes (function groups,
  * waves per workgroup, persistent or direct, workgroups per CU, and the direct
- * form's scheduler and load batching) on n_boxes
- * synthetic boxes inside the bounds of `p` (the problem the tape was built
- * from), leave the tape on the fastest, and report up to `cap` candidates in
- * `results` (*n_results = how many ran). Takes seconds (each compiled shape is
- * compiled once). No reference counterpart: the reference has one fixed
- * launch shape per node kernel (backend/graph/kernels.cuh:177-281). */
+ * form's scheduler and load batching) on n_boxes synthetic boxes inside the
+ * bounds of `p` (the problem the tape was built from), streamed from HBM
+ * (rotating copies of up to 640 MB in all) in sustained runs of 64
+ * back-to-back launches; time the four fastest again, leave the tape on the
+ * fastest of those, and report up to `cap` candidates in `results`
+ * (*n_results = how many ran; kernel_ms per launch). Takes seconds (each
+ * compiled shape is compiled once). No reference counterpart: the reference
+ * has one fixed launch shape per node kernel (backend/graph/kernels.cuh:177-281). */
 cmi_status cmi_tape_autotune(cmi_tape* t, const cmi_problem* p, size_t n_boxes, cmi_box_candidate* results, size_t cap,
                              size_t* n_results);
 
