@@ -90,3 +90,17 @@ def test_served_equals_plain(ctx):
         same_outcome(d, s)
     assert stats.peak_active <= 4
     assert np.all(lat >= 0)
+
+
+def test_served_large_instances_equal_plain(ctx):
+    """Served solves of QCQPs with up to 96 variables (tapes of tens of
+    thousands of words, run in 8 function groups) equal plain ones."""
+    probs = [cmi.EngineProblem.qcqp(70 + i, n=n, m=n // 2, cont_lb=-5.0, cont_ub=5.0, pair_density=0.2)
+             for i, n in enumerate([40, 96, 12, 64])]
+    opts = dict(partition_num=2, enumerate_cap=2, max_cycle_size=4, sample_points=16, iter_limit=15)
+    arrival = np.zeros(len(probs))
+    outs, fin, lat, stats = cmi.serve_stream(ctx, probs, arrival, max_active=8, **opts)
+    for p, d in zip(probs, outs):
+        s, _ = cmi.solve(ctx, p, **opts)
+        same_outcome(d, s)
+    assert stats.peak_active == len(probs)
