@@ -114,7 +114,27 @@ class CostModel(C.Structure):
 
 class OverBudget(C.Structure):
     _fields_ = [("role", C.c_char * 32), ("needed_bytes", C.c_size_t), ("budget_bytes", C.c_size_t),
-                ("n_regions", C.c_size_t), ("bytes_per_element", C.c_size_t), ("composition", C.c_char * 64)]
+                ("n_regions", C.c_size_t), ("elements_per_region", C.c_size_t), ("bytes_per_element", C.c_size_t),
+                ("element_breakdown", C.c_char * 96), ("element_inventory", C.c_char * 64), ("cost", CostModel),
+                ("composition_kinds", C.c_uint8 * 64), ("composition_size", C.c_size_t),
+                ("composition", C.c_char * 68), ("has_fan_out", C.c_int), ("fan_out_partition_num", C.c_size_t),
+                ("fan_out_enumerate_cap", C.c_size_t), ("solve_samples_per_region", C.c_size_t),
+                ("sampler_bytes", C.c_size_t)]
+
+
+def over_budget_dict(ob: OverBudget) -> dict:
+    """The facts as plain Python values (composition as its kinds list)."""
+    out = {}
+    for name, _ in ob._fields_:
+        v = getattr(ob, name)
+        if isinstance(v, bytes):
+            v = v.decode()
+        elif name == "cost":
+            v = {n: getattr(v, n) for n, _ in v._fields_}
+        elif name == "composition_kinds":
+            v = list(v)[:ob.composition_size]
+        out[name] = v
+    return out
 
 
 class BoundView(C.Structure):
@@ -141,16 +161,17 @@ class SolveOutcome(C.Structure):
         ("iterations", C.c_uint64), ("fathomed", C.c_uint64), ("bounded", C.c_uint64), ("sampled", C.c_uint64),
         ("enqueued", C.c_uint64), ("pruned_infeasible", C.c_uint64), ("pruned_dominated", C.c_uint64),
         ("pending", C.c_uint64), ("wall_seconds", C.c_double), ("has_over_budget", C.c_int),
-        ("over_budget", OverBudget),
+        ("over_budget", OverBudget), ("fathomed_points", C.c_uint64), ("witness_rejected", C.c_uint64),
+        ("pruned_duplicate", C.c_uint64), ("viable", C.c_uint64), ("dropped_viable", C.c_uint64),
+        ("dropped_dominated", C.c_uint64), ("dropped_floor", C.c_double), ("history_freed", C.c_uint64),
+        ("history_peak_live", C.c_uint64),
     ]
 
     def as_dict(self) -> dict:
         d = {name: getattr(self, name) for name, _ in self._fields_}
         d["stop_reason"] = STOP_NAMES[self.stop_reason]
         ob = self.over_budget
-        d["over_budget"] = ({name: (v.decode() if isinstance(v, bytes) else v)
-                             for name, v in ((n, getattr(ob, n)) for n, _ in ob._fields_)}
-                            if self.has_over_budget else None)
+        d["over_budget"] = over_budget_dict(ob) if self.has_over_budget else None
         return d
 
 

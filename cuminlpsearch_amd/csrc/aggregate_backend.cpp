@@ -66,7 +66,9 @@ AggregateBounderImpl::AggregateBounderImpl(std::shared_ptr<const DeviceTape> tap
     f.budget_bytes = budget;
     f.n_regions = n_;
     f.bytes_per_element = 0;
-    f.composition = region::spell(launch_);
+    f.element_breakdown = "per-batch staging of parent boxes and slot data, plus one ticket per tile";
+    f.element_inventory = std::to_string(max_batch_) + " regions per batch";
+    f.composition = launch_;
     throw backend::OverBudgetError(f);
   }
 

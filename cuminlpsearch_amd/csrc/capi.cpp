@@ -163,15 +163,76 @@ cmi::region::Composition composition_of(const std::uint8_t* kinds, std::size_t n
   return c;
 }
 
+void copy_text(char* dst, std::size_t cap, const std::string& src)
+{
+  std::size_t const k = std::min(cap - 1, src.size());
+  std::memcpy(dst, src.data(), k);
+  dst[k] = '\0';
+}
+
 void fill_facts(const cmi::backend::OverBudget& f, cmi_over_budget* out)
 {
   std::memset(out, 0, sizeof(*out));
-  std::strncpy(out->role, f.role.c_str(), sizeof(out->role) - 1);
+  copy_text(out->role, sizeof(out->role), f.role);
   out->needed_bytes = f.needed_bytes;
   out->budget_bytes = f.budget_bytes;
   out->n_regions = f.n_regions;
+  out->elements_per_region = f.elements_per_region;
   out->bytes_per_element = f.bytes_per_element;
-  std::strncpy(out->composition, f.composition.c_str(), sizeof(out->composition) - 1);
+  copy_text(out->element_breakdown, sizeof(out->element_breakdown), f.element_breakdown);
+  copy_text(out->element_inventory, sizeof(out->element_inventory), f.element_inventory);
+  out->cost.sampler_fixed = f.cost.sampler_fixed;
+  out->cost.sampler_per_region = f.cost.sampler_per_region;
+  out->cost.bounder_fixed = f.cost.bounder_fixed;
+  out->cost.bounder_per_region = f.cost.bounder_per_region;
+  out->cost.enumerator_fixed = f.cost.enumerator_fixed;
+  out->cost.enumerator_per_region = f.cost.enumerator_per_region;
+  out->cost.max_regions = f.cost.max_regions;
+  std::size_t const n = std::min<std::size_t>(f.composition.size(), sizeof(out->composition_kinds));
+  for (std::size_t j = 0; j < n; ++j) {
+    out->composition_kinds[j] = static_cast<std::uint8_t>(f.composition[j]);
+  }
+  out->composition_size = n;
+  copy_text(out->composition, sizeof(out->composition), cmi::region::spell(f.composition));
+  out->has_fan_out = f.fan_out ? 1 : 0;
+  out->fan_out_partition_num = f.fan_out ? f.fan_out->partition_num() : 0;
+  out->fan_out_enumerate_cap = f.fan_out ? f.fan_out->enumerate_cap() : 0;
+  out->solve_samples_per_region = f.solve_samples_per_region;
+  out->sampler_bytes = f.sampler_bytes;
+}
+
+/// The inverse of fill_facts, for the explain call.
+cmi::backend::OverBudget facts_from(const cmi_over_budget& in)
+{
+  cmi::backend::OverBudget f;
+  f.role.assign(in.role, strnlen(in.role, sizeof(in.role)));
+  f.needed_bytes = in.needed_bytes;
+  f.budget_bytes = in.budget_bytes;
+  f.n_regions = in.n_regions;
+  f.elements_per_region = in.elements_per_region;
+  f.bytes_per_element = in.bytes_per_element;
+  f.element_breakdown.assign(in.element_breakdown, strnlen(in.element_breakdown, sizeof(in.element_breakdown)));
+  f.element_inventory.assign(in.element_inventory, strnlen(in.element_inventory, sizeof(in.element_inventory)));
+  f.cost.sampler_fixed = in.cost.sampler_fixed;
+  f.cost.sampler_per_region = in.cost.sampler_per_region;
+  f.cost.bounder_fixed = in.cost.bounder_fixed;
+  f.cost.bounder_per_region = in.cost.bounder_per_region;
+  f.cost.enumerator_fixed = in.cost.enumerator_fixed;
+  f.cost.enumerator_per_region = in.cost.enumerator_per_region;
+  f.cost.max_regions = in.cost.max_regions;
+  std::size_t const n = std::min<std::size_t>(in.composition_size, sizeof(in.composition_kinds));
+  for (std::size_t j = 0; j < n; ++j) {
+    if (in.composition_kinds[j] > CMI_SLOT_BINARY_ENUMERATE) {
+      throw cmi::InvalidConfiguration("over-budget facts name an unknown slot kind");
+    }
+    f.composition.kinds.push_back(static_cast<cmi::region::SlotKind>(in.composition_kinds[j]));
+  }
+  if (in.has_fan_out) {
+    f.fan_out = cmi::region::FanOutSpec(in.fan_out_partition_num, in.fan_out_enumerate_cap);
+  }
+  f.solve_samples_per_region = in.solve_samples_per_region;
+  f.sampler_bytes = in.sampler_bytes;
+  return f;
 }
 
 void fill_outcome(const cmi::search::SolveOutcome& o, double seconds, cmi_solve_outcome* out, double* best,
@@ -191,6 +252,15 @@ void fill_outcome(const cmi::search::SolveOutcome& o, double seconds, cmi_solve_
   out->pruned_infeasible = o.counters.pruned_infeasible;
   out->pruned_dominated = o.counters.pruned_dominated;
   out->pending = o.counters.pending;
+  out->fathomed_points = o.counters.fathomed_points;
+  out->witness_rejected = o.counters.witness_rejected;
+  out->pruned_duplicate = o.counters.pruned_duplicate;
+  out->viable = o.counters.viable;
+  out->dropped_viable = o.counters.dropped.viable;
+  out->dropped_dominated = o.counters.dropped.dominated;
+  out->dropped_floor = o.counters.dropped.lb_min;
+  out->history_freed = o.counters.history_freed;
+  out->history_peak_live = o.counters.history_peak_live;
   out->wall_seconds = seconds;
   out->has_over_budget = o.over_budget ? 1 : 0;
   if (o.over_budget) {
@@ -1241,13 +1311,7 @@ cmi_status cmi_explain_over_budget(const cmi_problem* p, int objvar_kept, const 
   }
   return guarded([&] {
     cmi::region::FanOutSpec {partition_num, enumerate_cap};
-    cmi::backend::OverBudget f;
-    f.role.assign(facts->role, strnlen(facts->role, sizeof(facts->role)));
-    f.needed_bytes = facts->needed_bytes;
-    f.budget_bytes = facts->budget_bytes;
-    f.n_regions = facts->n_regions;
-    f.bytes_per_element = facts->bytes_per_element;
-    f.composition.assign(facts->composition, strnlen(facts->composition, sizeof(facts->composition)));
+    cmi::backend::OverBudget const f = facts_from(*facts);
     cmi::config::ProblemProfile pp = cmi::config::profile_of(p->problem);
     pp.objvar_kept = objvar_kept != 0;
     std::string const msg = cmi::config::explain_over_budget(f, pp, host_cost(p, sample_points), partition_num,

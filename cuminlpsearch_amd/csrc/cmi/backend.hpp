@@ -4,6 +4,7 @@
 #include <limits>
 #include <cstdint>
 #include <memory>
+#include <optional>
 #include <span>
 #include <string>
 
@@ -112,15 +113,24 @@ struct RegionCostModel
   std::size_t max_regions = std::numeric_limits<std::size_t>::max();
 };
 
-/// Facts of a build that did not fit (reference backend.hpp:236-287).
+/// Facts of a build that did not fit, member for member the reference's
+/// OverBudget (backend.hpp:236-262): thrown before anything is allocated, so
+/// a caller can explain the failure or retry with a smaller composition.
 struct OverBudget
 {
   std::string role;
   std::size_t needed_bytes = 0;
   std::size_t budget_bytes = 0;
   std::size_t n_regions = 0;
+  std::size_t elements_per_region = 1;  // > 1 only for the sampler
   std::size_t bytes_per_element = 0;
-  std::string composition;
+  std::string element_breakdown;  // what one element's bytes pay for
+  std::string element_inventory;  // what the per-element cost grows with
+  RegionCostModel cost;
+  region::Composition composition;
+  std::optional<region::FanOutSpec> fan_out;
+  std::size_t solve_samples_per_region = 1;
+  std::size_t sampler_bytes = 0;
 };
 
 class OverBudgetError : public ResourceExhausted
@@ -139,7 +149,7 @@ private:
   {
     return "out of device memory for the " + f.role + ": needs " + std::to_string(f.needed_bytes)
         + " bytes against a budget of " + std::to_string(f.budget_bytes) + " (" + std::to_string(f.n_regions)
-        + " regions, composition " + f.composition + ")";
+        + " regions, composition " + region::spell(f.composition) + ")";
   }
 
   OverBudget facts_;

@@ -358,7 +358,7 @@ inline std::string explain_over_budget(const backend::OverBudget& f, const Probl
                                        const backend::RegionCostModel& cost, std::size_t partition_num,
                                        std::size_t enumerate_cap)
 {
-  std::string out = "the " + f.role + " for composition " + f.composition + " needs " +
+  std::string out = "the " + f.role + " for composition " + region::spell(f.composition) + " needs " +
                     std::to_string(f.needed_bytes) + " bytes (" + std::to_string(f.n_regions) + " regions x " +
                     std::to_string(f.bytes_per_element) + " bytes per element) against a budget of " +
                     std::to_string(f.budget_bytes) + " bytes";

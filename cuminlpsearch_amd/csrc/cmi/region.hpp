@@ -69,7 +69,10 @@ struct SlotAssignment
   std::size_t size() const { return composition.size(); }
 };
 
-/// Uniform per-kind fan-out (reference region/fan_out.hpp:22-75).
+/// The width of a slot as a function of its kind only: every partition slot
+/// (continuous or integer) splits into partition_num children, an enumerated
+/// integer slot has enumerate_cap children, a binary slot 2 (reference
+/// region/fan_out.hpp:22-75).
 class FanOutSpec
 {
 public:
@@ -77,16 +80,16 @@ public:
       : partition_num_(partition_num)
       , enumerate_cap_(enumerate_cap)
   {
-    if (partition_num < 2) {
-      throw InvalidConfiguration(
-          "partition_num must be at least 2; a fan-out of 1 subdivides nothing, so bisection "
-          "could never narrow a box");
+    if (partition_num_ <= 1) {
+      throw InvalidConfiguration("partition_num " + std::to_string(partition_num_)
+                                 + " would not split a box; it must be 2 or more");
     }
-    if (enumerate_cap < 1) {
-      throw InvalidConfiguration("enumerate_cap must be at least 1; a slot must produce at least one child");
+    if (enumerate_cap_ == 0) {
+      throw InvalidConfiguration("enumerate_cap 0 would give an enumerated slot no children; it must be 1 or more");
     }
   }
 
+  /// enumerate_cap defaults to partition_num.
   explicit FanOutSpec(std::size_t partition_num)
       : FanOutSpec(partition_num, partition_num)
   {

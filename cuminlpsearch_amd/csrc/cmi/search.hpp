@@ -4,7 +4,6 @@
 #include <cmath>
 #include <cstddef>
 #include <cstdint>
-#include <fstream>
 #include <limits>
 #include <memory>
 #include <new>
@@ -16,381 +15,21 @@
 
 #include "cmi/backend.hpp"
 #include "cmi/errors.hpp"
+#include "cmi/frontier.hpp"
 #include "cmi/interval.hpp"
 #include "cmi/model.hpp"
 #include "cmi/policy.hpp"
 #include "cmi/region.hpp"
 
-// Host best-first branch and bound: the frontier heap, the parent-box
-// history, the host budget, the epilogue and the driver loop.
-//
-// Node order is part of the result (iteration counts, the incumbent found
-// under an iteration cap), so everything that decides it is restated exactly
-// from the reference: the comparator (search/frontier.hpp:66-89), the binary
-// heap's sift rules including its tie behaviour (frontier.hpp:152-338), the
-// history's LIFO slot reuse that produces pidx values (search/history.hpp:
-// 41-104) and the driver's per-iteration protocol (search/driver.hpp:221-407).
+// Host best-first branch and bound: one solve's resumable state machine, the
+// per-composition role cache and the plain driver that runs one solve with one
+// device round trip per role call. The heap, the parent-box store, the host
+// budget and the end-of-run rules are in cmi/frontier.hpp. The per-iteration
+// protocol (which node is taken, when the run stops, which children are
+// pruned and in which order survivors are enqueued) follows the reference's
+// loop (search/driver.hpp:221-407), because it decides the node order.
 namespace cmi::search
 {
-
-enum class StopReason
-{
-  Converged,
-  Exhausted,
-  IterationLimit,
-  HostMemory,
-  AllocationFailure,
-  DeviceMemory,
-};
-
-inline const char* to_string(StopReason r)
-{
-  switch (r) {
-    case StopReason::Converged: return "converged";
-    case StopReason::Exhausted: return "exhausted";
-    case StopReason::IterationLimit: return "iteration-limit";
-    case StopReason::HostMemory: return "host-memory";
-    case StopReason::AllocationFailure: return "allocation-failure";
-    case StopReason::DeviceMemory: return "device-memory";
-  }
-  return "unknown";
-}
-
-enum class FrontierPolicy
-{
-  StopAtBudget,
-  Compact,
-};
-
-/// gub - bound <= tol * max(1, |gub|) (reference search/budget.hpp:215-218).
-inline bool gap_closed(double gub, double bound, double tolerance)
-{
-  return gub - bound <= tolerance * std::max(1.0, std::fabs(gub));
-}
-
-inline std::size_t available_host_bytes()
-{
-  std::ifstream meminfo("/proc/meminfo");
-  std::string key;
-  while (meminfo >> key) {
-    if (key != "MemAvailable:") {
-      meminfo.ignore(std::numeric_limits<std::streamsize>::max(), '\n');
-      continue;
-    }
-    unsigned long long value = 0;
-    std::string unit;
-    if (!(meminfo >> value)) {
-      return 0;
-    }
-    meminfo >> unit;
-    return static_cast<std::size_t>(value) * (unit == "kB" ? 1024u : 1u);
-  }
-  return 0;
-}
-
-inline std::size_t resolve_host_budget(std::size_t requested)
-{
-  return requested != 0 ? requested : available_host_bytes() / 2;
-}
-
-inline bool over_host_budget(std::size_t live_bytes, std::size_t budget_bytes)
-{
-  return budget_bytes != 0 && live_bytes >= budget_bytes;
-}
-
-inline std::size_t compact_keep_count(std::size_t budget_bytes, std::size_t history_bytes, std::size_t node_bytes)
-{
-  std::size_t const room = budget_bytes - std::min(budget_bytes, history_bytes);
-  return std::max<std::size_t>(1, room / (4 * node_bytes));
-}
-
-struct DropAccounting
-{
-  double lb_min = std::numeric_limits<double>::infinity();
-  std::size_t viable = 0;
-  std::size_t dominated = 0;
-
-  void fold(double lb, double gub)
-  {
-    if (lb <= gub) {
-      lb_min = std::min(lb_min, lb);
-      ++viable;
-    } else {
-      ++dominated;
-    }
-  }
-
-  bool lost_nothing() const { return viable == 0; }
-};
-
-/// Box containment plus a 1e-6 integrality check (reference epilogue.hpp:35-59).
-inline bool witness_is_admissible(std::span<const double> point, std::span<const model::VarKind> kinds,
-                                  std::span<const interval> root_box)
-{
-  constexpr double kIntegralityTolerance = 1e-6;
-  if (point.size() != kinds.size() || point.size() != root_box.size()) {
-    return false;
-  }
-  for (std::size_t i = 0; i < point.size(); ++i) {
-    double const v = point[i];
-    if (!(v >= root_box[i].lb) || !(v <= root_box[i].ub)) {
-      return false;
-    }
-    if (kinds[i] == model::VarKind::Integer || kinds[i] == model::VarKind::Binary) {
-      if (std::fabs(v - std::round(v)) > kIntegralityTolerance) {
-        return false;
-      }
-    }
-  }
-  return true;
-}
-
-struct FinalBounds
-{
-  double glb;
-  bool proven_optimal;
-  bool infeasible;
-};
-
-/// Final bracket and claims (reference epilogue.hpp:89-127).
-inline FinalBounds finalise_bounds(double glb, double gub, double tolerance, bool found_incumbent, bool converged,
-                                   bool pending_empty, double frontier_min, std::size_t viable,
-                                   const DropAccounting& dropped)
-{
-  FinalBounds out {glb, false, false};
-  out.proven_optimal = found_incumbent && (converged || pending_empty || viable == 0)
-      && gap_closed(gub, dropped.lb_min, tolerance);
-  if (out.proven_optimal) {
-    out.glb = gub;
-  } else if (pending_empty && dropped.lost_nothing()) {
-    out.infeasible = true;
-  } else {
-    double const frontier = pending_empty ? std::numeric_limits<double>::max() : frontier_min;
-    out.glb = std::min(frontier, dropped.lb_min);
-    if (found_incumbent) {
-      out.glb = std::min(out.glb, gub);
-    }
-  }
-  return out;
-}
-
-/// Parent boxes, refcounted, slots reused LIFO; slot 0 is the root sentinel.
-struct IntervalHistory
-{
-  std::vector<std::vector<interval>> intervals;
-
-  std::size_t enqueue(std::vector<interval> box)
-  {
-    std::size_t const bytes = entry_bytes(box.size());
-    std::size_t idx = 0;
-    if (!free_slots_.empty()) {
-      idx = free_slots_.back();
-      free_slots_.pop_back();
-      intervals[idx] = std::move(box);
-      refs_[idx] = 1;
-    } else {
-      idx = intervals.size();
-      intervals.push_back(std::move(box));
-      refs_.push_back(1);
-    }
-    live_bytes_ += bytes;
-    peak_live_ = std::max(peak_live_, live_count());
-    return idx;
-  }
-
-  void add_ref(std::size_t idx)
-  {
-    if (idx != 0) {
-      ++refs_[idx];
-    }
-  }
-
-  void release(std::size_t idx)
-  {
-    if (idx == 0) {
-      return;
-    }
-    if (--refs_[idx] > 0) {
-      return;
-    }
-    live_bytes_ -= entry_bytes(intervals[idx].size());
-    std::vector<interval>().swap(intervals[idx]);
-    free_slots_.push_back(idx);
-    ++freed_;
-  }
-
-  std::size_t live_bytes() const { return live_bytes_; }
-  std::size_t live_count() const { return intervals.size() - free_slots_.size(); }
-  std::size_t ref_count(std::size_t idx) const { return refs_[idx]; }
-  std::uint64_t freed_count() const { return freed_; }
-  std::size_t peak_live() const { return peak_live_; }
-
-private:
-  static constexpr std::size_t entry_bytes(std::size_t n_vars)
-  {
-    return n_vars * sizeof(interval) + sizeof(std::vector<interval>);
-  }
-
-  std::vector<std::size_t> refs_;
-  std::vector<std::size_t> free_slots_;
-  std::size_t live_bytes_ = 0;
-  std::uint64_t freed_ = 0;
-  std::size_t peak_live_ = 0;
-};
-
-struct Node
-{
-  std::size_t sidx;
-  std::size_t pidx;
-  std::size_t depth;
-  double lb;
-  std::uint64_t assignment_hash = 0;
-
-  /// Least lb first, then deeper first, then larger pidx first.
-  bool operator<(const Node& o) const
-  {
-    if (lb < o.lb) {
-      return true;
-    }
-    if (o.lb < lb) {
-      return false;
-    }
-    if (depth != o.depth) {
-      return depth > o.depth;
-    }
-    return pidx > o.pidx;
-  }
-
-  /// Rebuild this node's box from its parent in `history` by re-running the
-  /// policy on the parent and decoding sidx; the hash check rejects a policy
-  /// that is not pure (reference frontier.hpp:113-144).
-  void materialise(const IntervalHistory& history, std::vector<interval>& out, const policy::CompositionPolicy& pol,
-                   std::span<const model::VarKind> kinds, std::span<const interval> root_box) const
-  {
-    if (pidx == 0) {
-      out.assign(root_box.begin(), root_box.end());
-      return;
-    }
-    const std::vector<interval>& parent = history.intervals[pidx];
-    region::SlotAssignment const a = pol.choose(parent, kinds);
-    if (region::assignment_hash(a) != assignment_hash) {
-      throw InvalidConfiguration(
-          "CompositionPolicy::choose is not a pure function of (box, var_kinds): re-invoking it on this "
-          "node's parent returned an assignment whose hash does not match the one used to encode the "
-          "node's sidx");
-    }
-    region::materialise(parent, sidx, a, out);
-  }
-};
-
-/// Binary min-heap of pending nodes with the reference's exact sift rules.
-class IntervalPQueue
-{
-public:
-  explicit IntervalPQueue(std::size_t capacity)
-      : elems_(capacity)
-  {
-  }
-
-  void enqueue(const Node& e)
-  {
-    if (n_ == elems_.size()) {
-      elems_.resize(elems_.empty() ? 1 : elems_.size() * 2);
-    }
-    elems_[n_] = e;
-    sift_up(n_);
-    ++n_;
-  }
-
-  Node dequeue()
-  {
-    Node res = elems_[0];
-    --n_;
-    if (n_ > 0) {
-      elems_[0] = elems_[n_];
-      sift_down(0);
-    }
-    return res;
-  }
-
-  bool empty() const { return n_ == 0; }
-  std::size_t size() const { return n_; }
-  const Node& peek() const { return elems_[0]; }
-  std::size_t capacity_bytes() const { return elems_.capacity() * sizeof(Node); }
-
-  std::size_t count_viable(double gub) const
-  {
-    // iterative form of the reference's downward-closed recursive count
-    std::size_t count = 0;
-    std::vector<std::size_t> stack;
-    if (n_ > 0) {
-      stack.push_back(0);
-    }
-    while (!stack.empty()) {
-      std::size_t const i = stack.back();
-      stack.pop_back();
-      if (i >= n_ || elems_[i].lb > gub) {
-        continue;
-      }
-      ++count;
-      stack.push_back(2 * i + 2);
-      stack.push_back(2 * i + 1);
-    }
-    return count;
-  }
-
-  template<typename OnEvict>
-  void compact(std::size_t n_keep, OnEvict&& on_evict)
-  {
-    if (n_ <= n_keep) {
-      return;
-    }
-    std::nth_element(elems_.begin(), elems_.begin() + static_cast<std::ptrdiff_t>(n_keep),
-                     elems_.begin() + static_cast<std::ptrdiff_t>(n_));
-    for (std::size_t i = n_keep; i < n_; ++i) {
-      on_evict(static_cast<const Node&>(elems_[i]));
-    }
-    n_ = n_keep;
-    for (std::size_t i = n_ / 2; i-- > 0;) {
-      sift_down(i);
-    }
-    if (elems_.size() > 2 * n_keep) {
-      std::vector<Node> shrunk(2 * n_keep);
-      std::copy(elems_.begin(), elems_.begin() + static_cast<std::ptrdiff_t>(n_), shrunk.begin());
-      elems_.swap(shrunk);
-    }
-  }
-
-private:
-  void sift_up(std::size_t idx)
-  {
-    Node const e = elems_[idx];
-    while (idx != 0 && e < elems_[(idx - 1) / 2]) {
-      elems_[idx] = elems_[(idx - 1) / 2];
-      idx = (idx - 1) / 2;
-    }
-    elems_[idx] = e;
-  }
-
-  void sift_down(std::size_t idx)
-  {
-    Node const e = elems_[idx];
-    while (2 * idx + 1 < n_) {
-      std::size_t const l = 2 * idx + 1;
-      std::size_t const r = 2 * idx + 2;
-      std::size_t const child = (r < n_ && elems_[r] < elems_[l]) ? r : l;
-      if (!(elems_[child] < e)) {
-        break;
-      }
-      elems_[idx] = elems_[child];
-      idx = child;
-    }
-    elems_[idx] = e;
-  }
-
-  std::vector<Node> elems_;
-  std::size_t n_ = 0;
-};
 
 struct RunTelemetry
 {

@@ -494,6 +494,7 @@ RoleImpl::RoleImpl(std::shared_ptr<const DeviceTape> tape, dev::Mode mode, const
     , composition_(composition)
     , n_regions_(n_regions)
     , samples_(mode == dev::Mode::Sample ? samples_per_region : 1)
+    , solve_samples_(samples_per_region)
     , slot_count_(composition.size())
     , n_vars_(tape_->n_vars())
 {
@@ -538,8 +539,20 @@ RoleImpl::RoleImpl(std::shared_ptr<const DeviceTape> tape, dev::Mode mode, const
     f.needed_bytes = needed;
     f.budget_bytes = budget;
     f.n_regions = n_regions_;
-    f.bytes_per_element = mode_ == dev::Mode::Bound ? cm.bounder_per_region : cm.sampler_per_region;
-    f.composition = region::spell(composition_);
+    f.elements_per_region = samples_;
+    std::size_t const per_region = mode_ == dev::Mode::Bound ? cm.bounder_per_region
+        : mode_ == dev::Mode::Sample                          ? cm.sampler_per_region
+                                                              : cm.enumerator_per_region;
+    f.bytes_per_element = (per_region + samples_ - 1) / samples_;
+    f.element_breakdown = mode_ == dev::Mode::Bound
+        ? "one lower bound (8 B) and one feasibility byte read back per region, plus tile minima"
+        : "one 16 B (value, index) partial per 64-element tile, plus one byte per region";
+    f.element_inventory = std::to_string(tape_->tape().code.size()) + " tape instructions over "
+        + std::to_string(n_vars_) + " variables";
+    f.cost = cm;
+    f.composition = composition_;
+    f.solve_samples_per_region = solve_samples_;
+    f.sampler_bytes = 0;
     throw backend::OverBudgetError(f);
   }
 
@@ -765,11 +778,12 @@ backend::SubdivisionBundle HipBackendFactory::build_subdivision(const model::Pro
   }
   if (roles.bounder) {
     out.bounder = std::make_unique<HipBounder>(
-        std::make_unique<RoleImpl>(t, dev::Mode::Bound, composition, n_regions, 1, budget.bytes));
+        std::make_unique<RoleImpl>(t, dev::Mode::Bound, composition, n_regions, budget.samples_per_region, budget.bytes));
   }
   if (roles.enumerator && region::is_fully_enumerable(composition)) {
     out.enumerator = std::make_unique<HipEnumerator>(
-        std::make_unique<RoleImpl>(t, dev::Mode::Enumerate, composition, n_regions, 1, budget.bytes));
+        std::make_unique<RoleImpl>(t, dev::Mode::Enumerate, composition, n_regions, budget.samples_per_region,
+                                   budget.bytes));
   }
   return out;
 }

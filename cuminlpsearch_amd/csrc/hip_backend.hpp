@@ -308,6 +308,7 @@ private:
   region::Composition composition_;
   std::size_t n_regions_;
   std::size_t samples_;
+  std::size_t solve_samples_;
   std::size_t n_elems_;
   std::size_t slot_count_;
   std::size_t n_vars_;

@@ -205,15 +205,28 @@ typedef struct
   size_t max_regions; /* most regions one role launch indexes at this sample count */
 } cmi_cost_model;
 
-/* OverBudget facts, reference backend/backend.hpp:236-262 */
+/* OverBudget facts, member for member the reference's OverBudget
+ * (backend/backend.hpp:236-262); filled by a build that does not fit, before
+ * anything is allocated. */
 typedef struct
 {
-  char role[32];
+  char role[32];                  /* e.g. "interval bounder" */
   size_t needed_bytes;
   size_t budget_bytes;
   size_t n_regions;
+  size_t elements_per_region;     /* > 1 only for the sampler (its samples per region) */
   size_t bytes_per_element;
-  char composition[64]; /* the composition's slot kinds, e.g. "C2 I1" (truncated if longer) */
+  char element_breakdown[96];     /* what one element's bytes pay for */
+  char element_inventory[64];     /* what the per-element cost grows with */
+  cmi_cost_model cost;            /* the problem's affine role costs */
+  uint8_t composition_kinds[64];  /* cmi_slot_kind of each slot */
+  size_t composition_size;        /* slots in composition_kinds */
+  char composition[68];           /* the same kinds spelled, e.g. "CCPE" */
+  int has_fan_out;                /* fan_out below is set */
+  size_t fan_out_partition_num;
+  size_t fan_out_enumerate_cap;
+  size_t solve_samples_per_region; /* the solve-wide sample count */
+  size_t sampler_bytes;           /* device bytes already held by the sampler */
 } cmi_over_budget;
 
 /* BoundResult, reference backend/backend.hpp:68-74 */
@@ -271,6 +284,16 @@ typedef struct
   double wall_seconds;
   int has_over_budget;         /* stop_reason is CMI_STOP_DEVICE_MEMORY because a role build did not fit */
   cmi_over_budget over_budget; /* its facts (reference driver.hpp:408-416) */
+  /* RunTelemetry beyond the counters above (reference report/telemetry.hpp) */
+  uint64_t fathomed_points;
+  uint64_t witness_rejected;
+  uint64_t pruned_duplicate;
+  uint64_t viable;             /* pending nodes with lb <= upper_bound at the end */
+  uint64_t dropped_viable;     /* compaction evictions the incumbent did not dominate */
+  uint64_t dropped_dominated;  /* compaction evictions it did */
+  double dropped_floor;        /* least lb among the viable evictions; +inf if none */
+  uint64_t history_freed;
+  uint64_t history_peak_live;
 } cmi_solve_outcome;
 
 const char* cmi_last_error(void);

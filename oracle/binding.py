@@ -67,6 +67,8 @@ def lib() -> C.CDLL:
         h.orc_pt_ext.argtypes = [C.c_int, P, P, C.c_int, C.c_size_t, P]
         h.orc_solve.restype = C.c_int
         h.orc_solve.argtypes = [P, C.c_size_t, C.c_size_t, C.c_size_t, C.c_size_t, C.c_uint32, d, P, P]
+        h.orc_solve_opts.restype = C.c_int
+        h.orc_solve_opts.argtypes = [P, P, P, P]
         _lib = h
     return _lib
 
@@ -142,14 +144,27 @@ def candidate(arrays, box, var_ids, widths, kinds, samples, salt):
 
 
 def solve(arrays, partition_num=2, enumerate_cap=2, max_cycle_size=0, sample_points=1, iter_limit=1000000,
-          tolerance=1e-6):
+          tolerance=1e-6, host_budget_bytes=0, compact_frontier=False, known_primal=None):
+    """The oracle's best-first search with the engine's solve options. A host
+    budget of 0 means none here (the engine reads 0 as half of MemAvailable)."""
     from cuminlpsearch_amd import capi as _c
 
     d = arrays.desc()
+    o = _c.SolveOptions()
+    o.partition_num = partition_num
+    o.enumerate_cap = enumerate_cap
+    o.max_cycle_size = max_cycle_size
+    o.sample_points = sample_points
+    o.iter_limit = iter_limit
+    o.tolerance = tolerance
+    o.host_budget_bytes = host_budget_bytes
+    o.compact_frontier = int(compact_frontier)
+    if known_primal is not None:
+        o.has_known_primal = 1
+        o.known_primal = known_primal
     out = _c.SolveOutcome()
     best = np.zeros(arrays.n_vars)
-    _ok(lib().orc_solve(C.byref(d), partition_num, enumerate_cap, max_cycle_size, sample_points, iter_limit,
-                        tolerance, C.byref(out), _p(best)))
+    _ok(lib().orc_solve_opts(C.byref(d), C.byref(o), C.byref(out), _p(best)))
     return out, best
 
 

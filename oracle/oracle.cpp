@@ -43,9 +43,12 @@
 #include <xmmintrin.h>
 
 #include "cmi.h"
+#include "search_core.hpp"
 
 namespace
 {
+
+using namespace orc;
 
 // ---------------------------------------------------------------------------
 // Directed rounding
@@ -137,10 +140,6 @@ int arity(std::uint8_t op)
   }
 }
 
-struct Iv
-{
-  double lb, ub;
-};
 
 // ---------------------------------------------------------------------------
 // Interval ops (ops.cuh semantics; Mul/Sqr/scalar Mul report a zero bound as +0,
@@ -1135,192 +1134,6 @@ Cand points_argmin(const Prob& p, const std::vector<Iv>& box, const std::size_t*
 // Search (GreedyEnumerate + best-first driver)
 // ---------------------------------------------------------------------------
 
-struct Assign
-{
-  std::vector<std::uint8_t> kinds;
-  std::vector<std::size_t> vars, widths;
-};
-
-std::size_t int_domain(Iv b)
-{
-  double const lo = std::ceil(b.lb), hi = std::floor(b.ub);
-  return hi < lo ? 0 : static_cast<std::size_t>(hi - lo) + 1;
-}
-
-Assign choose(const std::vector<Iv>& box, const std::vector<std::uint8_t>& vk, std::size_t part, std::size_t cap,
-              std::size_t max_slots)
-{
-  Assign a;
-  auto full = [&] { return a.vars.size() >= max_slots; };
-  auto push = [&](std::uint8_t k, std::size_t v, std::size_t w) {
-    a.kinds.push_back(k);
-    a.vars.push_back(v);
-    a.widths.push_back(w);
-  };
-  for (std::size_t v = 0; v < vk.size() && !full(); ++v) {
-    if (vk[v] == CMI_BINARY && box[v].ub > box[v].lb) {
-      push(CMI_SLOT_BINARY_ENUMERATE, v, 2);
-    }
-  }
-  std::vector<std::size_t> ints;
-  for (std::size_t v = 0; v < vk.size(); ++v) {
-    if (vk[v] == CMI_INTEGER && box[v].ub > box[v].lb) {
-      ints.push_back(v);
-    }
-  }
-  std::sort(ints.begin(), ints.end(), [&](std::size_t x, std::size_t y) { return int_domain(box[x]) < int_domain(box[y]); });
-  for (std::size_t v : ints) {
-    if (full()) {
-      break;
-    }
-    if (int_domain(box[v]) <= cap) {
-      push(CMI_SLOT_INTEGER_ENUMERATE, v, cap);
-    } else {
-      push(CMI_SLOT_INTEGER_PARTITION, v, part);
-    }
-  }
-  std::vector<std::size_t> cont;
-  for (std::size_t v = 0; v < vk.size(); ++v) {
-    if (vk[v] == CMI_CONTINUOUS && box[v].ub > box[v].lb) {
-      cont.push_back(v);
-    }
-  }
-  std::sort(cont.begin(), cont.end(),
-            [&](std::size_t x, std::size_t y) { return (box[x].ub - box[x].lb) > (box[y].ub - box[y].lb); });
-  for (std::size_t v : cont) {
-    if (full()) {
-      break;
-    }
-    push(CMI_SLOT_CONTINUOUS, v, part);
-  }
-  return a;
-}
-
-struct Node
-{
-  std::size_t sidx, pidx, depth;
-  double lb;
-};
-
-bool before(const Node& a, const Node& b)
-{
-  if (a.lb < b.lb) {
-    return true;
-  }
-  if (b.lb < a.lb) {
-    return false;
-  }
-  if (a.depth != b.depth) {
-    return a.depth > b.depth;
-  }
-  return a.pidx > b.pidx;
-}
-
-struct Heap
-{
-  std::vector<Node> e;
-  std::size_t n = 0;
-  void push(Node x)
-  {
-    if (n == e.size()) {
-      e.resize(e.empty() ? 1 : 2 * e.size());
-    }
-    std::size_t i = n++;
-    while (i > 0 && before(x, e[(i - 1) / 2])) {
-      e[i] = e[(i - 1) / 2];
-      i = (i - 1) / 2;
-    }
-    e[i] = x;
-  }
-  Node pop()
-  {
-    Node top = e[0];
-    --n;
-    if (n > 0) {
-      Node x = e[n];
-      std::size_t i = 0;
-      while (2 * i + 1 < n) {
-        std::size_t c = 2 * i + 1;
-        if (c + 1 < n && before(e[c + 1], e[c])) {
-          ++c;
-        }
-        if (!before(e[c], x)) {
-          break;
-        }
-        e[i] = e[c];
-        i = c;
-      }
-      e[i] = x;
-    }
-    return top;
-  }
-  std::size_t viable(double gub) const
-  {
-    std::size_t c = 0;
-    std::vector<std::size_t> st {0};
-    while (!st.empty()) {
-      std::size_t i = st.back();
-      st.pop_back();
-      if (i >= n || e[i].lb > gub) {
-        continue;
-      }
-      ++c;
-      st.push_back(2 * i + 1);
-      st.push_back(2 * i + 2);
-    }
-    return c;
-  }
-};
-
-struct History
-{
-  std::vector<std::vector<Iv>> box;
-  std::vector<std::size_t> refs, free_list;
-  std::size_t put(std::vector<Iv> b)
-  {
-    if (!free_list.empty()) {
-      std::size_t i = free_list.back();
-      free_list.pop_back();
-      box[i] = std::move(b);
-      refs[i] = 1;
-      return i;
-    }
-    box.push_back(std::move(b));
-    refs.push_back(1);
-    return box.size() - 1;
-  }
-  void ref(std::size_t i)
-  {
-    if (i != 0) {
-      ++refs[i];
-    }
-  }
-  void unref(std::size_t i)
-  {
-    if (i != 0 && --refs[i] == 0) {
-      std::vector<Iv>().swap(box[i]);
-      free_list.push_back(i);
-    }
-  }
-};
-
-bool admissible(const std::vector<double>& x, const Prob& p)
-{
-  if (x.size() != p.n_vars) {
-    return false;
-  }
-  for (std::size_t i = 0; i < x.size(); ++i) {
-    if (!(x[i] >= p.box[i].lb) || !(x[i] <= p.box[i].ub)) {
-      return false;
-    }
-    if (p.kind[i] != CMI_CONTINUOUS && std::fabs(x[i] - std::round(x[i])) > 1e-6) {
-      return false;
-    }
-  }
-  return true;
-}
-
-bool closed(double gub, double bound, double tol) { return gub - bound <= tol * std::max(1.0, std::fabs(gub)); }
 
 }  // namespace
 
@@ -1611,29 +1424,33 @@ int orc_candidate(const cmi_problem_desc* d, const cmi_ivl* box, const size_t* v
   }
 }
 
-// The best-first search, restated. Host budget: unbounded (the engine's
-// default budget is half of MemAvailable; the small parity solves never reach
-// it).
-int orc_solve(const cmi_problem_desc* d, size_t partition_num, size_t enumerate_cap, size_t max_cycle_size,
-              size_t sample_points, uint32_t iter_limit, double tol, cmi_solve_outcome* out, double* best_point)
+// The best-first search, restated (search/driver.hpp:221-407) over the pieces
+// in search_core.hpp. A host budget of 0 is unbounded here (the engine reads
+// 0 as half of MemAvailable); parity runs that exercise the budget pass an
+// explicit one to both.
+int orc_solve_opts(const cmi_problem_desc* d, const cmi_solve_options* o, cmi_solve_outcome* out,
+                   double* best_point)
 {
   try {
     Prob const p = load(d);
-    std::size_t const max_slots = max_cycle_size == 0 ? 64 : max_cycle_size;
-    double gub = DBL_MAX, glb = -DBL_MAX;
+    std::size_t const max_slots = o->max_cycle_size == 0 ? 64 : o->max_cycle_size;
+    std::size_t const part = o->partition_num, cap = o->enumerate_cap;
+    double const tol = o->tolerance;
+    double gub = o->has_known_primal ? o->known_primal : DBL_MAX;
+    double glb = -DBL_MAX;
     std::uint32_t iter = 0;
     std::vector<double> best;
-    Heap pending;
-    pending.e.resize(10000);
+    Heap pending(10000);
     History hist;
     hist.put({});
-    pending.push({0, 0, 0, glb});
+    pending.push({0, 0, 0, glb, 0});
+    Dropped dropped;
     bool converged = false;
     int stop = CMI_STOP_ITERATION_LIMIT;
     bool early = false;
     std::uint64_t fathomed = 0, bounded = 0, sampled = 0, enqueued = 0, p_inf = 0, p_dom = 0;
     std::vector<Iv> box;
-    while (iter < iter_limit && pending.n > 0 && !closed(gub, glb, tol)) {
+    while (iter < o->iter_limit && pending.n > 0 && !closed(gub, glb, tol)) {
       ++iter;
       Node const cur = pending.pop();
       if (closed(gub, cur.lb, tol)) {
@@ -1647,11 +1464,11 @@ int orc_solve(const cmi_problem_desc* d, size_t partition_num, size_t enumerate_
         box = p.box;
       } else {
         std::vector<Iv> const& parent = hist.box[cur.pidx];
-        Assign const pa = choose(parent, p.kind, partition_num, enumerate_cap, max_slots);
+        Assign const pa = choose(parent, p.kind, part, cap, max_slots);
         materialise(parent, cur.sidx, pa.vars.data(), pa.widths.data(), pa.kinds.data(), pa.vars.size(), box);
       }
       hist.unref(cur.pidx);
-      Assign const a = choose(box, p.kind, partition_num, enumerate_cap, max_slots);
+      Assign const a = choose(box, p.kind, part, cap, max_slots);
       std::size_t live = 0;
       for (Iv const& b : box) {
         live += b.ub > b.lb ? 1 : 0;
@@ -1667,7 +1484,7 @@ int orc_solve(const cmi_problem_desc* d, size_t partition_num, size_t enumerate_
       if (live <= a.kinds.size() && fully) {
         Cand const c = points_argmin(p, box, a.vars.data(), a.widths.data(), a.kinds.data(), a.vars.size(),
                                      n_regions, 0, 0);
-        if (c.found && c.value < gub && admissible(c.point, p)) {
+        if (c.found && c.value < gub && admissible(c.point, p.kind, p.box)) {
           gub = c.value;
           best = c.point;
         }
@@ -1676,9 +1493,9 @@ int orc_solve(const cmi_problem_desc* d, size_t partition_num, size_t enumerate_
       }
       std::size_t const box_idx = hist.put(box);
       Cand const c = points_argmin(p, box, a.vars.data(), a.widths.data(), a.kinds.data(), a.vars.size(), n_regions,
-                                   sample_points, iter);
-      sampled += n_regions * sample_points;
-      if (c.value < gub && admissible(c.point, p)) {
+                                   o->sample_points, iter);
+      sampled += n_regions * o->sample_points;
+      if (c.value < gub && admissible(c.point, p.kind, p.box)) {
         gub = c.value;
         best = c.point;
       }
@@ -1696,36 +1513,42 @@ int orc_solve(const cmi_problem_desc* d, size_t partition_num, size_t enumerate_
           ++p_dom;
           continue;
         }
-        pending.push({r, box_idx, cur.depth + 1, lb});
+        pending.push({r, box_idx, cur.depth + 1, lb, 0});
         hist.ref(box_idx);
         ++enqueued;
       }
       hist.unref(box_idx);
+      std::size_t const budget = o->host_budget_bytes;
+      if (over_budget(pending.cap_bytes() + hist.bytes, budget)) {
+        if (!o->compact_frontier) {
+          stop = CMI_STOP_HOST_MEMORY;
+          early = true;
+          break;
+        }
+        pending.compact(keep_count(budget, hist.bytes, sizeof(Node)), [&](const Node& e) {
+          dropped.fold(e.lb, gub);
+          hist.unref(e.pidx);
+        });
+        if (over_budget(pending.cap_bytes() + hist.bytes, budget)) {
+          stop = CMI_STOP_HOST_MEMORY;
+          early = true;
+          break;
+        }
+      }
     }
     if (!early) {
       stop = pending.n == 0 ? CMI_STOP_EXHAUSTED : closed(gub, glb, tol) ? CMI_STOP_CONVERGED : CMI_STOP_ITERATION_LIMIT;
     }
     bool const found = gub < DBL_MAX;
     std::size_t const viable = pending.viable(gub);
-    bool const proven = found && (converged || pending.n == 0 || viable == 0);
-    bool infeasible = false;
-    double out_glb = glb;
-    if (proven) {
-      out_glb = gub;
-    } else if (pending.n == 0) {
-      infeasible = true;
-    } else {
-      out_glb = pending.e[0].lb;
-      if (found) {
-        out_glb = std::min(out_glb, gub);
-      }
-    }
+    Final const fin = finalise(glb, gub, tol, found, converged, pending.n == 0, pending.n ? pending.e[0].lb : DBL_MAX,
+                               viable, dropped);
     std::memset(out, 0, sizeof(*out));
-    out->lower_bound = out_glb;
+    out->lower_bound = fin.glb;
     out->upper_bound = gub;
     out->stop_reason = stop;
-    out->proven_optimal = proven ? 1 : 0;
-    out->infeasible = infeasible ? 1 : 0;
+    out->proven_optimal = fin.proven ? 1 : 0;
+    out->infeasible = fin.infeasible ? 1 : 0;
     out->found_incumbent = found ? 1 : 0;
     out->iterations = iter;
     out->fathomed = fathomed;
@@ -1735,6 +1558,12 @@ int orc_solve(const cmi_problem_desc* d, size_t partition_num, size_t enumerate_
     out->pruned_infeasible = p_inf;
     out->pruned_dominated = p_dom;
     out->pending = pending.n;
+    out->viable = viable;
+    out->dropped_viable = dropped.viable;
+    out->dropped_dominated = dropped.dominated;
+    out->dropped_floor = dropped.floor;
+    out->history_freed = hist.freed;
+    out->history_peak_live = hist.peak;
     if (best_point != nullptr) {
       for (std::size_t v = 0; v < p.n_vars; ++v) {
         best_point[v] = v < best.size() ? best[v] : NAN;
@@ -1745,6 +1574,19 @@ int orc_solve(const cmi_problem_desc* d, size_t partition_num, size_t enumerate_
     g_err = ex.what();
     return 1;
   }
+}
+
+int orc_solve(const cmi_problem_desc* d, size_t partition_num, size_t enumerate_cap, size_t max_cycle_size,
+              size_t sample_points, uint32_t iter_limit, double tol, cmi_solve_outcome* out, double* best_point)
+{
+  cmi_solve_options o {};
+  o.partition_num = partition_num;
+  o.enumerate_cap = enumerate_cap;
+  o.max_cycle_size = max_cycle_size;
+  o.sample_points = sample_points;
+  o.iter_limit = iter_limit;
+  o.tolerance = tol;
+  return orc_solve_opts(d, &o, out, best_point);
 }
 
 }  // extern "C"

@@ -1,0 +1,328 @@
+// CPU oracle -- TEST INFRASTRUCTURE ONLY. The search pieces of the oracle's
+// best-first restatement, in a header so that the restated host tests
+// (tests/search_pins.cpp) can check them case by case; oracle.cpp's orc_solve
+// is built on them. Nothing in the product includes this file.
+//
+// Restated from the reference: policy/greedy_enum.hpp:54-186 (composition),
+// search/frontier.hpp:66-338 (node order, heap, compaction),
+// search/history.hpp:20-137 (parent boxes), search/budget.hpp:60-215 (host
+// budget), search/epilogue.hpp:35-127 (incumbent admission, final claims).
+#pragma once
+
+#include <algorithm>
+#include <cfloat>
+#include <cmath>
+#include <cstddef>
+#include <cstdint>
+#include <vector>
+
+#include "cmi.h"
+
+namespace orc
+{
+
+struct Iv
+{
+  double lb, ub;
+};
+
+// ---- composition (GreedyEnumerate) ----------------------------------------
+
+struct Assign
+{
+  std::vector<std::uint8_t> kinds;
+  std::vector<std::size_t> vars, widths;
+};
+
+inline std::size_t int_domain(Iv b)
+{
+  double const lo = std::ceil(b.lb), hi = std::floor(b.ub);
+  return hi < lo ? 0 : static_cast<std::size_t>(hi - lo) + 1;
+}
+
+inline Assign choose(const std::vector<Iv>& box, const std::vector<std::uint8_t>& vk, std::size_t part,
+                     std::size_t cap, std::size_t max_slots)
+{
+  Assign a;
+  auto full = [&] { return a.vars.size() >= max_slots; };
+  auto push = [&](std::uint8_t k, std::size_t v, std::size_t w) {
+    a.kinds.push_back(k);
+    a.vars.push_back(v);
+    a.widths.push_back(w);
+  };
+  for (std::size_t v = 0; v < vk.size() && !full(); ++v) {
+    if (vk[v] == CMI_BINARY && box[v].ub > box[v].lb) {
+      push(CMI_SLOT_BINARY_ENUMERATE, v, 2);
+    }
+  }
+  std::vector<std::size_t> ints;
+  for (std::size_t v = 0; v < vk.size(); ++v) {
+    if (vk[v] == CMI_INTEGER && box[v].ub > box[v].lb) {
+      ints.push_back(v);
+    }
+  }
+  std::sort(ints.begin(), ints.end(),
+            [&](std::size_t x, std::size_t y) { return int_domain(box[x]) < int_domain(box[y]); });
+  for (std::size_t v : ints) {
+    if (full()) {
+      break;
+    }
+    if (int_domain(box[v]) <= cap) {
+      push(CMI_SLOT_INTEGER_ENUMERATE, v, cap);
+    } else {
+      push(CMI_SLOT_INTEGER_PARTITION, v, part);
+    }
+  }
+  std::vector<std::size_t> cont;
+  for (std::size_t v = 0; v < vk.size(); ++v) {
+    if (vk[v] == CMI_CONTINUOUS && box[v].ub > box[v].lb) {
+      cont.push_back(v);
+    }
+  }
+  std::sort(cont.begin(), cont.end(),
+            [&](std::size_t x, std::size_t y) { return (box[x].ub - box[x].lb) > (box[y].ub - box[y].lb); });
+  for (std::size_t v : cont) {
+    if (full()) {
+      break;
+    }
+    push(CMI_SLOT_CONTINUOUS, v, part);
+  }
+  return a;
+}
+
+// ---- pending nodes ----------------------------------------------------------
+
+struct Node
+{
+  std::size_t sidx, pidx, depth;
+  double lb;
+  std::uint64_t hash;  // unused by the oracle; keeps the reference's 40-byte node
+};
+
+inline bool before(const Node& a, const Node& b)
+{
+  if (a.lb < b.lb) {
+    return true;
+  }
+  if (b.lb < a.lb) {
+    return false;
+  }
+  if (a.depth != b.depth) {
+    return a.depth > b.depth;
+  }
+  return a.pidx > b.pidx;
+}
+
+struct NodeLess
+{
+  bool operator()(const Node& a, const Node& b) const { return before(a, b); }
+};
+
+struct Heap
+{
+  std::vector<Node> e;
+  std::size_t n = 0;
+
+  explicit Heap(std::size_t initial = 0)
+      : e(initial)
+  {
+  }
+
+  void push(Node x)
+  {
+    if (n == e.size()) {
+      e.resize(e.empty() ? 1 : 2 * e.size());
+    }
+    std::size_t i = n++;
+    while (i > 0 && before(x, e[(i - 1) / 2])) {
+      e[i] = e[(i - 1) / 2];
+      i = (i - 1) / 2;
+    }
+    e[i] = x;
+  }
+
+  void down(std::size_t i)
+  {
+    Node const x = e[i];
+    while (2 * i + 1 < n) {
+      std::size_t c = 2 * i + 1;
+      if (c + 1 < n && before(e[c + 1], e[c])) {
+        ++c;
+      }
+      if (!before(e[c], x)) {
+        break;
+      }
+      e[i] = e[c];
+      i = c;
+    }
+    e[i] = x;
+  }
+
+  Node pop()
+  {
+    Node top = e[0];
+    --n;
+    if (n > 0) {
+      e[0] = e[n];
+      down(0);
+    }
+    return top;
+  }
+
+  std::size_t viable(double gub) const
+  {
+    std::size_t c = 0;
+    std::vector<std::size_t> st {0};
+    while (!st.empty()) {
+      std::size_t i = st.back();
+      st.pop_back();
+      if (i >= n || e[i].lb > gub) {
+        continue;
+      }
+      ++c;
+      st.push_back(2 * i + 1);
+      st.push_back(2 * i + 2);
+    }
+    return c;
+  }
+
+  std::size_t cap_bytes() const { return e.capacity() * sizeof(Node); }
+
+  // keep the `keep` best; `gone` sees the rest in the order nth_element leaves
+  // them; heapify bottom-up; cut the array to 2 * keep entries if larger
+  template<typename F>
+  void compact(std::size_t keep, F&& gone)
+  {
+    if (n <= keep) {
+      return;
+    }
+    std::nth_element(e.begin(), e.begin() + static_cast<std::ptrdiff_t>(keep),
+                     e.begin() + static_cast<std::ptrdiff_t>(n), NodeLess {});
+    for (std::size_t i = keep; i < n; ++i) {
+      gone(e[i]);
+    }
+    n = keep;
+    for (std::size_t i = n / 2; i-- > 0;) {
+      down(i);
+    }
+    if (e.size() > 2 * keep) {
+      std::vector<Node> cut(2 * keep);
+      for (std::size_t i = 0; i < n; ++i) {
+        cut[i] = e[i];
+      }
+      e.swap(cut);
+    }
+  }
+};
+
+// ---- parent boxes -------------------------------------------------------------
+
+struct History
+{
+  std::vector<std::vector<Iv>> box;
+  std::vector<std::size_t> refs, free_list;
+  std::size_t bytes = 0, freed = 0, peak = 0;
+
+  static std::size_t cost(std::size_t nv) { return nv * sizeof(Iv) + sizeof(std::vector<Iv>); }
+  std::size_t live() const { return box.size() - free_list.size(); }
+
+  std::size_t put(std::vector<Iv> b)
+  {
+    bytes += cost(b.size());
+    std::size_t i;
+    if (!free_list.empty()) {
+      i = free_list.back();
+      free_list.pop_back();
+      box[i] = std::move(b);
+      refs[i] = 1;
+    } else {
+      box.push_back(std::move(b));
+      refs.push_back(1);
+      i = box.size() - 1;
+    }
+    peak = std::max(peak, live());
+    return i;
+  }
+  void ref(std::size_t i)
+  {
+    if (i != 0) {
+      ++refs[i];
+    }
+  }
+  void unref(std::size_t i)
+  {
+    if (i != 0 && --refs[i] == 0) {
+      bytes -= cost(box[i].size());
+      std::vector<Iv>().swap(box[i]);
+      free_list.push_back(i);
+      ++freed;
+    }
+  }
+};
+
+// ---- host budget and the end-of-run claims ------------------------------------
+
+inline bool closed(double gub, double bound, double tol) { return gub - bound <= tol * std::max(1.0, std::fabs(gub)); }
+
+inline bool over_budget(std::size_t live, std::size_t budget) { return budget != 0 && live >= budget; }
+
+inline std::size_t keep_count(std::size_t budget, std::size_t hist, std::size_t node)
+{
+  std::size_t const room = budget > hist ? budget - hist : 0;
+  return std::max<std::size_t>(1, room / (4 * node));
+}
+
+struct Dropped
+{
+  double floor = INFINITY;
+  std::size_t viable = 0, dominated = 0;
+  void fold(double lb, double gub)
+  {
+    if (lb <= gub) {
+      floor = std::min(floor, lb);
+      ++viable;
+    } else {
+      ++dominated;
+    }
+  }
+};
+
+struct Final
+{
+  double glb;
+  bool proven, infeasible;
+};
+
+inline Final finalise(double glb, double gub, double tol, bool found, bool converged, bool empty, double front_min,
+                      std::size_t viable, const Dropped& d)
+{
+  if (found && (converged || empty || viable == 0) && closed(gub, d.floor, tol)) {
+    return {gub, true, false};
+  }
+  if (empty && d.viable == 0) {
+    return {glb, false, true};
+  }
+  double g = std::min(empty ? DBL_MAX : front_min, d.floor);
+  if (found) {
+    g = std::min(g, gub);
+  }
+  return {g, false, false};
+}
+
+inline bool admissible(const std::vector<double>& x, const std::vector<std::uint8_t>& kind, const std::vector<Iv>& root)
+{
+  if (x.size() != kind.size() || x.size() != root.size()) {
+    return false;
+  }
+  for (std::size_t i = 0; i < x.size(); ++i) {
+    if (!(x[i] >= root[i].lb) || !(x[i] <= root[i].ub)) {
+      return false;
+    }
+    if (kind[i] != CMI_CONTINUOUS && std::fabs(x[i] - std::round(x[i])) > 1e-6) {
+      return false;
+    }
+  }
+  return true;
+}
+
+}  // namespace orc

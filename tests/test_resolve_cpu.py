@@ -338,11 +338,14 @@ def test_cli_needs_a_model_and_an_iteration_limit(capsys):
 # -- explain_over_budget (reference config/resolve.hpp:247-330) ----------------------------
 
 
-def facts(role="point sampler", needed=5_000_000, budget=1_000_000, n_regions=250_000, per=20, comp="C C"):
+def facts(role="point sampler", needed=5_000_000, budget=1_000_000, n_regions=250_000, per=20,
+          comp=(C.SLOT_CONTINUOUS, C.SLOT_CONTINUOUS)):
     f = C.OverBudget()
     f.role = role.encode()
     f.needed_bytes, f.budget_bytes, f.n_regions, f.bytes_per_element = needed, budget, n_regions, per
-    f.composition = comp.encode()
+    for j, k in enumerate(comp):
+        f.composition_kinds[j] = k
+    f.composition_size = len(comp)
     return f
 
 
@@ -351,7 +354,7 @@ def test_explanation_names_the_failure_and_the_cap_that_fits():
     msg = R.explain_over_budget(p, facts(budget=100_000), 8, 8, 10)
     k = R.auto_max_cycle_size(p, 8, 8, 10, 100_000, 64)
     assert 0 < k < 5
-    assert "point sampler" in msg and "C C" in msg and "5000000 bytes" in msg and "100000 bytes" in msg
+    assert "point sampler" in msg and "composition CC " in msg and "5000000 bytes" in msg and "100000 bytes" in msg
     assert f"at most {k} slot(s) per iteration fit" in msg
 
 
