@@ -1449,6 +1449,7 @@ int orc_solve_opts(const cmi_problem_desc* d, const cmi_solve_options* o, cmi_so
     int stop = CMI_STOP_ITERATION_LIMIT;
     bool early = false;
     std::uint64_t fathomed = 0, bounded = 0, sampled = 0, enqueued = 0, p_inf = 0, p_dom = 0;
+    std::uint64_t fathomed_points = 0, rejected = 0;
     std::vector<Iv> box;
     while (iter < o->iter_limit && pending.n > 0 && !closed(gub, glb, tol)) {
       ++iter;
@@ -1484,20 +1485,29 @@ int orc_solve_opts(const cmi_problem_desc* d, const cmi_solve_options* o, cmi_so
       if (live <= a.kinds.size() && fully) {
         Cand const c = points_argmin(p, box, a.vars.data(), a.widths.data(), a.kinds.data(), a.vars.size(),
                                      n_regions, 0, 0);
-        if (c.found && c.value < gub && admissible(c.point, p.kind, p.box)) {
-          gub = c.value;
-          best = c.point;
+        if (c.found && c.value < gub) {
+          if (admissible(c.point, p.kind, p.box)) {
+            gub = c.value;
+            best = c.point;
+          } else {
+            ++rejected;
+          }
         }
         ++fathomed;
+        fathomed_points += n_regions;
         continue;
       }
       std::size_t const box_idx = hist.put(box);
       Cand const c = points_argmin(p, box, a.vars.data(), a.widths.data(), a.kinds.data(), a.vars.size(), n_regions,
                                    o->sample_points, iter);
       sampled += n_regions * o->sample_points;
-      if (c.value < gub && admissible(c.point, p.kind, p.box)) {
-        gub = c.value;
-        best = c.point;
+      if (c.value < gub) {
+        if (admissible(c.point, p.kind, p.box)) {
+          gub = c.value;
+          best = c.point;
+        } else {
+          ++rejected;
+        }
       }
       std::vector<Iv> sub, val;
       for (std::size_t r = 0; r < n_regions; ++r) {
@@ -1559,6 +1569,8 @@ int orc_solve_opts(const cmi_problem_desc* d, const cmi_solve_options* o, cmi_so
     out->pruned_dominated = p_dom;
     out->pending = pending.n;
     out->viable = viable;
+    out->fathomed_points = fathomed_points;
+    out->witness_rejected = rejected;
     out->dropped_viable = dropped.viable;
     out->dropped_dominated = dropped.dominated;
     out->dropped_floor = dropped.floor;

@@ -149,3 +149,21 @@ def test_gams_solve_cli(capsys):
     assert line.startswith("RESULT\tsense=min\tprimal=")
     assert abs(float(line.split("primal=")[1].split("\t")[0]) + 6.666666667) < 1e-5  # relative gap 1e-6
     assert "  x1 = " in out and "  x2 = " in out
+
+
+def test_autocorr_all_binary_run_record(ctx):
+    """The reference's run record for autocorr_bern20-03 under --policy=all-binary
+    (MINLP_STATUS.md:217): converged at iteration 1 with primal = dual = -72.
+    The all-binary row pins its shape whatever the device memory (partition 2,
+    enumerate cap 2, 1 sample, 20 slots: catalogue.hpp:92-99), so the record
+    is reproducible: the 20 binaries are enumerated in the first iteration."""
+    from cuminlpsearch_amd import resolve as R
+    pm = parsed("autocorr_bern20-03")
+    shape = R.resolve_shape(pm.problem, "all-binary", free_device_bytes=8 << 30)
+    opts = shape.solve_options()
+    assert opts == dict(partition_num=2, enumerate_cap=2, sample_points=1, max_cycle_size=20)
+    d, _ = cmi.solve(ctx, pm.problem, iter_limit=1000, **opts)
+    o, _ = orc.solve(pm.problem.arrays(), iter_limit=1000, **opts)
+    same_outcome(d, o)
+    assert d.iterations == 1 and d.proven_optimal
+    assert d.upper_bound == -72.0 and d.lower_bound == -72.0

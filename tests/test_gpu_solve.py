@@ -12,14 +12,16 @@ import problems as P
 pytestmark = pytest.mark.gpu
 
 FIELDS = ("iterations", "fathomed", "bounded", "sampled", "enqueued", "pruned_infeasible", "pruned_dominated",
-          "pending", "stop_reason", "proven_optimal", "infeasible", "found_incumbent")
+          "pending", "stop_reason", "proven_optimal", "infeasible", "found_incumbent", "fathomed_points",
+          "witness_rejected", "pruned_duplicate", "viable", "dropped_viable", "dropped_dominated", "history_freed",
+          "history_peak_live")
 
 
 def same_outcome(d, o):
     for f in FIELDS:
         assert getattr(d, f) == getattr(o, f), f
-    assert np.float64(d.lower_bound).tobytes() == np.float64(o.lower_bound).tobytes()
-    assert np.float64(d.upper_bound).tobytes() == np.float64(o.upper_bound).tobytes()
+    for f in ("lower_bound", "upper_bound", "dropped_floor"):
+        assert np.float64(getattr(d, f)).tobytes() == np.float64(getattr(o, f)).tobytes(), f
 
 
 C1_OPTS = dict(partition_num=4, enumerate_cap=4, max_cycle_size=2, sample_points=32, iter_limit=400)
@@ -104,3 +106,48 @@ def test_served_large_instances_equal_plain(ctx):
         s, _ = cmi.solve(ctx, p, **opts)
         same_outcome(d, s)
     assert stats.peak_active == len(probs)
+
+
+# -- host budget, compaction and a known primal bound (search/driver.hpp:355-400, budget.hpp) --------------------
+
+BUDGET_OPTS = dict(partition_num=8, enumerate_cap=8, max_cycle_size=2, sample_points=8, iter_limit=3000)
+
+
+def test_host_budget_stops_with_the_frontier_intact(ctx):
+    """StopAtBudget: the run ends at the first iteration whose heap capacity plus
+    live history reaches the budget, with nothing discarded; the device search
+    and the oracle stop at the same iteration with the same bracket."""
+    eng = P.c1(0)
+    opts = dict(BUDGET_OPTS, host_budget_bytes=600_000)
+    d, best_d = cmi.solve(ctx, eng, **opts)
+    o, best_o = orc.solve(eng.arrays(), **opts)
+    same_outcome(d, o)
+    assert d.stop_reason == 3 and d.iterations < 3000  # host-memory
+    assert d.dropped_viable == 0 and d.dropped_dominated == 0 and d.pending > 10000
+    assert np.array_equal(best_d.view(np.uint64), best_o.view(np.uint64))
+
+
+def test_compaction_evicts_in_the_same_order_as_the_oracle(ctx):
+    """Compact: repeated compactions evict the worst nodes; the iterates after
+    them, the eviction tallies and the dropped floor equal the oracle's."""
+    eng = P.c1(0)
+    opts = dict(BUDGET_OPTS, host_budget_bytes=600_000, compact_frontier=True)
+    d, _ = cmi.solve(ctx, eng, **opts)
+    o, _ = orc.solve(eng.arrays(), **opts)
+    same_outcome(d, o)
+    assert d.iterations == 3000 and d.dropped_viable > 0 and d.dropped_dominated > 0
+    assert d.lower_bound <= d.dropped_floor
+
+
+def test_known_primal_bound_prunes_like_an_incumbent(ctx):
+    """known_primal_bound seeds the incumbent value (driver.hpp:101-117): with
+    the optimum of a plain run as the bound, the search prunes from the start
+    and reaches the same optimum in no more iterations."""
+    eng = P.c1(1)
+    plain, _ = cmi.solve(ctx, eng, **C1_OPTS)
+    opts = dict(C1_OPTS, known_primal=plain.upper_bound + 1e-3)
+    d, _ = cmi.solve(ctx, eng, **opts)
+    o, _ = orc.solve(eng.arrays(), **opts)
+    same_outcome(d, o)
+    assert d.iterations <= plain.iterations
+    assert d.upper_bound <= plain.upper_bound + 1e-3
