@@ -326,6 +326,114 @@ backend::CandidateResult Runner::candidate(std::size_t i) const
   return out;
 }
 
+std::size_t shared_host_budget(const cmi_solve_options& o, std::size_t residents)
+{
+  if (o.host_budget_bytes != 0) {
+    return o.host_budget_bytes;
+  }
+  std::size_t const total = search::resolve_host_budget(0);
+  if (total == 0) {
+    return 0;  // meminfo unreadable: unbudgeted, as a plain solve would be
+  }
+  return std::max<std::size_t>(1, total / std::max<std::size_t>(residents, 1));
+}
+
+search::SolveOptions search_options(const cmi_solve_options& o, std::size_t host_budget)
+{
+  search::SolveOptions so;
+  so.iter_limit = o.iter_limit;
+  so.tolerance = o.tolerance;
+  so.sample_points = o.sample_points;
+  so.host_budget_bytes = host_budget;
+  so.frontier_policy = o.compact_frontier ? search::FrontierPolicy::Compact : search::FrontierPolicy::StopAtBudget;
+  if (o.has_known_primal) {
+    so.known_primal_bound = o.known_primal;
+  }
+  return so;
+}
+
+std::size_t step(Runner& runner, std::span<Member> members, const cmi_solve_options& o, hip::Context& ctx)
+{
+  // One device budget for the whole step: the free memory is read once, as
+  // a plain solve reads it once per role build.
+  std::size_t const budget = o.budget_bytes != 0 ? o.budget_bytes : ctx.free_bytes();
+  struct Owner
+  {
+    Member* member;
+    std::size_t first_req;
+    bool enumerate;
+  };
+  std::vector<Request> reqs;
+  std::vector<Owner> owners;
+  reqs.reserve(2 * members.size());
+  owners.reserve(members.size());
+  for (Member& m : members) {
+    search::SolveState& st = *m.state;
+    if (st.finished()) {
+      continue;
+    }
+    const search::SolveState::Request* q = nullptr;
+    try {
+      q = st.next();
+    } catch (const std::bad_alloc&) {
+      st.stop(search::StopReason::AllocationFailure);
+      continue;
+    }
+    if (q == nullptr) {
+      continue;
+    }
+    bool const enumerate = q->kind == search::SolveState::Kind::Enumerate;
+    std::size_t const n_regions = region::composition_fan_out(q->assignment->widths);
+    // The plain driver builds (and prices) the sampler, then the bounder, or
+    // the enumerator on a fathom; a build that does not fit stops the solve.
+    try {
+      if (m.tape == nullptr) {
+        throw ResourceExhausted("the problem's device tape could not be built");
+      }
+      region::Composition const& c = q->assignment->composition;
+      if (enumerate) {
+        hip::check_role_fits(*m.tape, dev::Mode::Enumerate, c, n_regions, o.sample_points, budget);
+      } else {
+        hip::check_role_fits(*m.tape, dev::Mode::Sample, c, n_regions, o.sample_points, budget);
+        hip::check_role_fits(*m.tape, dev::Mode::Bound, c, n_regions, o.sample_points, budget);
+      }
+    } catch (const backend::OverBudgetError& e) {
+      st.stop_over_budget(e.facts());
+      continue;
+    } catch (const ResourceExhausted&) {
+      st.stop(search::StopReason::DeviceMemory);
+      continue;
+    }
+    owners.push_back({&m, reqs.size(), enumerate});
+    if (enumerate) {
+      reqs.push_back({m.tape, dev::Mode::Enumerate, q->box, q->assignment, q->salt, 1});
+    } else {
+      reqs.push_back({m.tape, dev::Mode::Sample, q->box, q->assignment, q->salt,
+                      static_cast<std::uint32_t>(o.sample_points)});
+      reqs.push_back({m.tape, dev::Mode::Bound, q->box, q->assignment, q->salt, 1});
+    }
+  }
+  if (reqs.empty()) {
+    return 0;
+  }
+  runner.run(reqs);
+  for (Owner const& w : owners) {
+    search::SolveState& st = *w.member->state;
+    try {
+      if (w.enumerate) {
+        st.on_enumerate(runner.candidate(w.first_req), runner.n_elems(w.first_req));
+      } else {
+        backend::BoundResult const b {runner.feasible(w.first_req + 1), runner.obj_lb(w.first_req + 1),
+                                      runner.n_elems(w.first_req + 1)};
+        st.on_subdivide(runner.candidate(w.first_req), runner.n_elems(w.first_req), b);
+      }
+    } catch (const std::bad_alloc&) {
+      st.stop(search::StopReason::AllocationFailure);
+    }
+  }
+  return reqs.size();
+}
+
 std::vector<Result> solve_batch(std::shared_ptr<hip::Context> ctx,
                                 std::span<const model::Problem<double>* const> problems,
                                 const cmi_solve_options& o)
@@ -333,71 +441,41 @@ std::vector<Result> solve_batch(std::shared_ptr<hip::Context> ctx,
   auto const t0 = std::chrono::steady_clock::now();
   std::size_t const n = problems.size();
   auto pol = capi::make_policy(o);
-  search::SolveOptions so;
-  so.iter_limit = o.iter_limit;
-  so.tolerance = o.tolerance;
-  so.sample_points = o.sample_points;
-  so.host_budget_bytes = o.host_budget_bytes;
-  so.frontier_policy = o.compact_frontier ? search::FrontierPolicy::Compact : search::FrontierPolicy::StopAtBudget;
-  if (o.has_known_primal) {
-    so.known_primal_bound = o.known_primal;
-  }
+  search::SolveOptions const so = search_options(o, shared_host_budget(o, n));
   // Each solve's tape (compiled on the host, uploaded once) and search state
-  // are independent of the others': build them on several host threads.
+  // are independent of the others': build them on several host threads. A
+  // tape that cannot be built stops only its own solve, at its first
+  // iteration, with DeviceMemory (where a plain solve's first role build
+  // would have failed).
   std::vector<std::unique_ptr<hip::DeviceTape>> tapes(n);
   std::vector<std::unique_ptr<search::SolveState>> states(n);
   parallel_for(n, [&](std::size_t i) {
-    tapes[i] = std::make_unique<hip::DeviceTape>(ctx, *problems[i]);
     states[i] = std::make_unique<search::SolveState>(*problems[i], pol, so);
+    try {
+      tapes[i] = std::make_unique<hip::DeviceTape>(ctx, *problems[i]);
+    } catch (const ResourceExhausted&) {
+      tapes[i].reset();
+    } catch (const std::bad_alloc&) {
+      tapes[i].reset();
+    }
   });
+  std::vector<Member> members(n);
+  for (std::size_t i = 0; i < n; ++i) {
+    members[i] = Member {states[i].get(), tapes[i].get()};
+  }
   std::vector<Result> results(n);
   std::vector<bool> done(n, false);
   Runner runner(ctx);
-  std::vector<Request> reqs;
-  struct Owner
-  {
-    std::size_t solve;
-    std::size_t first_req;
-    bool enumerate;
-  };
-  std::vector<Owner> owners;
   std::size_t remaining = n;
   while (remaining > 0) {
-    reqs.clear();
-    owners.clear();
+    step(runner, members, o, *ctx);
+    double const now = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     for (std::size_t i = 0; i < n; ++i) {
-      if (done[i]) {
-        continue;
-      }
-      const search::SolveState::Request* q = states[i]->next();
-      if (q == nullptr) {
+      if (!done[i] && states[i]->finished()) {
         results[i].outcome = states[i]->finish();
-        results[i].seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        results[i].seconds = now;
         done[i] = true;
         --remaining;
-        continue;
-      }
-      owners.push_back({i, reqs.size(), q->kind == search::SolveState::Kind::Enumerate});
-      if (q->kind == search::SolveState::Kind::Enumerate) {
-        reqs.push_back({tapes[i].get(), dev::Mode::Enumerate, q->box, q->assignment, q->salt, 1});
-      } else {
-        reqs.push_back({tapes[i].get(), dev::Mode::Sample, q->box, q->assignment, q->salt,
-                        static_cast<std::uint32_t>(o.sample_points)});
-        reqs.push_back({tapes[i].get(), dev::Mode::Bound, q->box, q->assignment, q->salt, 1});
-      }
-    }
-    if (reqs.empty()) {
-      continue;
-    }
-    runner.run(reqs);
-    for (Owner const& w : owners) {
-      search::SolveState& st = *states[w.solve];
-      if (w.enumerate) {
-        st.on_enumerate(runner.candidate(w.first_req), runner.n_elems(w.first_req));
-      } else {
-        backend::BoundResult const b {runner.feasible(w.first_req + 1), runner.obj_lb(w.first_req + 1),
-                                      runner.n_elems(w.first_req + 1)};
-        st.on_subdivide(runner.candidate(w.first_req), runner.n_elems(w.first_req), b);
       }
     }
   }

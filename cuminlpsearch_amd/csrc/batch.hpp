@@ -69,6 +69,29 @@ private:
   std::uint64_t launches_ = 0;
 };
 
+/// One solve advanced by the lock-step loops: its search and its device
+/// tape (null when the tape could not be built).
+struct Member
+{
+  search::SolveState* state = nullptr;
+  const hip::DeviceTape* tape = nullptr;
+};
+
+/// One lock-step iteration of every unfinished member: the host part of its
+/// iteration, the plain driver's role-build checks for its request (a solve
+/// whose roles would not fit, or whose host allocation fails, stops exactly as
+/// a plain solve stops, and the others go on), one Runner::run over all the
+/// requests, and the results handed back. Returns the requests run.
+std::size_t step(Runner& runner, std::span<Member> members, const cmi_solve_options& o, hip::Context& ctx);
+
+/// The host budget of each resident solve: host_budget_bytes as given (the
+/// per-solve budget a plain solve has), or, when it is 0, half of MemAvailable
+/// read once and shared equally among `residents` solves.
+std::size_t shared_host_budget(const cmi_solve_options& o, std::size_t residents);
+
+/// SearchDriver settings from the C options, with the given host budget.
+search::SolveOptions search_options(const cmi_solve_options& o, std::size_t host_budget);
+
 struct Result
 {
   search::SolveOutcome outcome;

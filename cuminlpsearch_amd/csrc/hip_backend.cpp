@@ -487,6 +487,69 @@ namespace
 std::size_t align_up(std::size_t x, std::size_t a) { return (x + a - 1) / a * a; }
 }  // namespace
 
+void check_role_fits(const DeviceTape& tape, dev::Mode mode, const region::Composition& composition,
+                     std::size_t n_regions, std::size_t samples_per_region, std::size_t budget_bytes)
+{
+  std::size_t const samples = mode == dev::Mode::Sample ? samples_per_region : 1;
+  if (samples < 1) {
+    throw InvalidConfiguration("sample_points must be at least 1");
+  }
+  if (n_regions == 0 || n_regions > (std::size_t(1) << 31) / samples) {
+    throw InvalidConfiguration("n_regions * samples must be in [1, 2^31)");
+  }
+  if (composition.size() > dev::kMaxSlotsPerItem) {
+    throw InvalidConfiguration("composition has more slots than the device decode supports");
+  }
+  Context const& ctx = *tape.context();
+  if (mode != dev::Mode::Bound && tape.n_slots() > ctx.lds_point_slots()) {
+    throw ResourceExhausted("tape needs more point slots than LDS holds");
+  }
+
+  // Priced before anything is allocated (reference replay.cuh:285-304).
+  backend::RegionCostModel const cm = cost_model_for(tape.tape(), samples);
+  std::size_t needed = 0;
+  std::size_t per_region = 0;
+  std::string role;
+  switch (mode) {
+    case dev::Mode::Bound:
+      per_region = cm.bounder_per_region;
+      needed = cm.bounder_fixed + per_region * n_regions;
+      role = "interval bounder";
+      break;
+    case dev::Mode::Sample:
+      per_region = cm.sampler_per_region;
+      needed = cm.sampler_fixed + per_region * n_regions;
+      role = "point sampler";
+      break;
+    case dev::Mode::Enumerate:
+      per_region = cm.enumerator_per_region;
+      needed = cm.enumerator_fixed + per_region * n_regions;
+      role = "exact enumerator";
+      break;
+  }
+  std::size_t const budget = budget_bytes != 0 ? budget_bytes : ctx.free_bytes();
+  if (needed <= budget) {
+    return;
+  }
+  backend::OverBudget f;
+  f.role = role;
+  f.needed_bytes = needed;
+  f.budget_bytes = budget;
+  f.n_regions = n_regions;
+  f.elements_per_region = samples;
+  f.bytes_per_element = (per_region + samples - 1) / samples;
+  f.element_breakdown = mode == dev::Mode::Bound
+      ? "one lower bound (8 B) and one feasibility byte read back per region, plus tile minima"
+      : "one 16 B (value, index) partial per 64-element tile, plus one byte per region";
+  f.element_inventory = std::to_string(tape.tape().code.size()) + " tape instructions over "
+      + std::to_string(tape.n_vars()) + " variables";
+  f.cost = cm;
+  f.composition = composition;
+  f.solve_samples_per_region = samples_per_region;
+  f.sampler_bytes = 0;
+  throw backend::OverBudgetError(f);
+}
+
 RoleImpl::RoleImpl(std::shared_ptr<const DeviceTape> tape, dev::Mode mode, const region::Composition& composition,
                    std::size_t n_regions, std::size_t samples_per_region, std::size_t budget_bytes)
     : tape_(std::move(tape))
@@ -498,63 +561,10 @@ RoleImpl::RoleImpl(std::shared_ptr<const DeviceTape> tape, dev::Mode mode, const
     , slot_count_(composition.size())
     , n_vars_(tape_->n_vars())
 {
-  if (samples_ < 1) {
-    throw InvalidConfiguration("sample_points must be at least 1");
-  }
-  if (n_regions_ == 0 || n_regions_ > (std::size_t(1) << 31) / samples_) {
-    throw InvalidConfiguration("n_regions * samples must be in [1, 2^31)");
-  }
-  if (slot_count_ > dev::kMaxSlotsPerItem) {
-    throw InvalidConfiguration("composition has more slots than the device decode supports");
-  }
+  check_role_fits(*tape_, mode_, composition_, n_regions_, solve_samples_, budget_bytes);
   Context const& ctx = *tape_->context();
-  if (mode_ != dev::Mode::Bound && tape_->n_slots() > ctx.lds_point_slots()) {
-    throw ResourceExhausted("tape needs more point slots than LDS holds");
-  }
   n_elems_ = n_regions_ * samples_;
   n_tiles_ = static_cast<std::uint32_t>((n_elems_ + dev::kTile - 1) / dev::kTile);
-
-  // Budget check before any allocation (reference replay.cuh:285-304).
-  backend::RegionCostModel const cm = cost_model_for(tape_->tape(), samples_);
-  std::size_t needed = 0;
-  std::string role;
-  switch (mode_) {
-    case dev::Mode::Bound:
-      needed = cm.bounder_fixed + cm.bounder_per_region * n_regions_;
-      role = "interval bounder";
-      break;
-    case dev::Mode::Sample:
-      needed = cm.sampler_fixed + cm.sampler_per_region * n_regions_;
-      role = "point sampler";
-      break;
-    case dev::Mode::Enumerate:
-      needed = cm.enumerator_fixed + cm.enumerator_per_region * n_regions_;
-      role = "exact enumerator";
-      break;
-  }
-  std::size_t const budget = budget_bytes != 0 ? budget_bytes : ctx.free_bytes();
-  if (needed > budget) {
-    backend::OverBudget f;
-    f.role = role;
-    f.needed_bytes = needed;
-    f.budget_bytes = budget;
-    f.n_regions = n_regions_;
-    f.elements_per_region = samples_;
-    std::size_t const per_region = mode_ == dev::Mode::Bound ? cm.bounder_per_region
-        : mode_ == dev::Mode::Sample                          ? cm.sampler_per_region
-                                                              : cm.enumerator_per_region;
-    f.bytes_per_element = (per_region + samples_ - 1) / samples_;
-    f.element_breakdown = mode_ == dev::Mode::Bound
-        ? "one lower bound (8 B) and one feasibility byte read back per region, plus tile minima"
-        : "one 16 B (value, index) partial per 64-element tile, plus one byte per region";
-    f.element_inventory = std::to_string(tape_->tape().code.size()) + " tape instructions over "
-        + std::to_string(n_vars_) + " variables";
-    f.cost = cm;
-    f.composition = composition_;
-    f.solve_samples_per_region = solve_samples_;
-    f.sampler_bytes = 0;
-    throw backend::OverBudgetError(f);
-  }
 
   ctx.activate();
   off_parent_ = align_up(sizeof(dev::Item), 256);

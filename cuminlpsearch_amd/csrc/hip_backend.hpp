@@ -271,6 +271,16 @@ private:
 /// Bytes a role of each kind needs for n_regions (affine in n_regions).
 backend::RegionCostModel cost_model_for(const tape::Tape& t, std::size_t samples_per_region);
 
+/// Every check a role build makes before it allocates, in the same order:
+/// the shape (InvalidConfiguration), the LDS the point roles need
+/// (ResourceExhausted) and the device budget (OverBudgetError with its facts;
+/// budget_bytes 0 = the device's free memory now). samples_per_region is the
+/// solve-wide sample count; only the sampler allocates per sample. The
+/// batched and served paths call it per request, so a solve that would not
+/// fit stops exactly as the plain driver stops it.
+void check_role_fits(const DeviceTape& tape, dev::Mode mode, const region::Composition& composition,
+                     std::size_t n_regions, std::size_t samples_per_region, std::size_t budget_bytes);
+
 /// The sampler, bounder and enumerator over one DeviceTape and a fixed
 /// composition shape (slot count, n_regions), reusable across calls; results
 /// are views into this object's pinned staging, valid until its next call.

@@ -5,6 +5,7 @@
 #include <chrono>
 #include <cstring>
 #include <memory>
+#include <new>
 #include <numeric>
 #include <string>
 #include <thread>
@@ -15,6 +16,7 @@
 #include "cmi/config.hpp"
 #include "cmi/policy.hpp"
 #include "cmi/search.hpp"
+#include "cmi/tape.hpp"
 #include "handles.hpp"
 #include "hip_backend.hpp"
 
@@ -28,33 +30,20 @@ namespace
 
 using Clock = std::chrono::steady_clock;
 
-cmi::search::SolveOptions solve_options(const cmi_solve_options& o)
-{
-  cmi::search::SolveOptions so;
-  so.iter_limit = o.iter_limit;
-  so.tolerance = o.tolerance;
-  so.sample_points = o.sample_points;
-  so.host_budget_bytes = o.host_budget_bytes;
-  so.frontier_policy =
-      o.compact_frontier ? cmi::search::FrontierPolicy::Compact : cmi::search::FrontierPolicy::StopAtBudget;
-  if (o.has_known_primal) {
-    so.known_primal_bound = o.known_primal;
-  }
-  return so;
-}
-
 /// Device bytes one resident solve can come to hold: its tape and variable
 /// kinds, plus the roles' bytes for the worst composition its search can
 /// reach at this fan-out and slot cap (the memory-fit resolver's footprint,
-/// so admission and shape fitting price a solve alike).
-std::size_t resident_bytes(const cmi::hip::DeviceTape& t, const cmi::model::Problem<double>& p,
-                           const cmi_solve_options& o)
+/// so admission and shape fitting price a solve alike). Priced from a host
+/// compile of the problem's tape, before any device memory is taken.
+std::size_t resident_bytes(const cmi::model::Problem<double>& p, const cmi_solve_options& o)
 {
-  std::size_t const n_vars = t.n_vars();
+  p.validate();
+  cmi::tape::Tape const t = cmi::tape::compile(p, {});
+  std::size_t const n_vars = t.n_vars;
   std::size_t const cap = std::min<std::size_t>(o.max_cycle_size == 0 ? cmi::config::kMaxSlots : o.max_cycle_size,
                                                 n_vars);
-  auto const cost = cmi::hip::cost_model_for(t.tape(), std::max<std::size_t>(o.sample_points, 1));
-  return t.n_instr() * sizeof(cmi::tape::Instr) + n_vars
+  auto const cost = cmi::hip::cost_model_for(t, std::max<std::size_t>(o.sample_points, 1));
+  return t.code.size() * sizeof(cmi::tape::Instr) + n_vars
       + cmi::config::worst_footprint(cap, cmi::config::profile_of(p), cost, o.partition_num, o.enumerate_cap);
 }
 
@@ -102,7 +91,10 @@ extern "C" cmi_status cmi_serve_stream(cmi_ctx* ctx, size_t n, const cmi_problem
           throw cmi::InvalidConfiguration("server max_active must be at least 1");
         }
         auto const pol = cmi::capi::make_policy(*a.o);
-        cmi::search::SolveOptions const sopt = solve_options(*a.o);
+        // Each resident solve's host budget: as given, or an equal share of
+        // half of MemAvailable among the max_active slots.
+        cmi::search::SolveOptions const sopt =
+            cmi::batch::search_options(*a.o, cmi::batch::shared_host_budget(*a.o, a.so->max_active));
         std::vector<std::size_t> order(a.n);
         std::iota(order.begin(), order.end(), 0);
         auto arrive = [&](std::size_t i) { return (a.so->realtime && a.arrival != nullptr) ? a.arrival[i] : 0.0; };
@@ -110,15 +102,11 @@ extern "C" cmi_status cmi_serve_stream(cmi_ctx* ctx, size_t n, const cmi_problem
 
         cmi::batch::Runner runner(a.ctx->ctx);
         std::vector<Active> active;
+        std::vector<cmi::batch::Member> members;
         std::size_t next = 0, completed = 0, used_bytes = 0, peak = 0, peak_bytes = 0;
         std::uint64_t steps = 0;
-        std::vector<cmi::batch::Request> reqs;
-        struct Owner
-        {
-          std::size_t slot, first_req;
-          bool enumerate;
-        };
-        std::vector<Owner> owners;
+        // The head of the queue is priced once, however many steps it waits.
+        std::size_t priced = a.n, priced_bytes = 0;
         auto const t0 = Clock::now();
         auto elapsed = [&] { return std::chrono::duration<double>(Clock::now() - t0).count(); };
 
@@ -127,15 +115,27 @@ extern "C" cmi_status cmi_serve_stream(cmi_ctx* ctx, size_t n, const cmi_problem
           // Admission: arrived, a slot free, and the device budget fits.
           while (next < a.n && active.size() < a.so->max_active && arrive(order[next]) <= now) {
             std::size_t const i = order[next];
-            auto tape = std::make_unique<cmi::hip::DeviceTape>(a.ctx->ctx, a.problems[i]->problem);
-            std::size_t const bytes = resident_bytes(*tape, a.problems[i]->problem, *a.o);
-            if (a.so->hbm_budget_bytes != 0 && !active.empty() && used_bytes + bytes > a.so->hbm_budget_bytes) {
+            if (priced != i) {
+              priced_bytes = resident_bytes(a.problems[i]->problem, *a.o);
+              priced = i;
+            }
+            if (a.so->hbm_budget_bytes != 0 && !active.empty() && used_bytes + priced_bytes > a.so->hbm_budget_bytes) {
               break;  // wait for a resident solve to finish
             }
-            used_bytes += bytes;
+            // A tape that cannot be built stops its own solve (DeviceMemory at
+            // its first iteration, as a plain solve's first role build would).
+            std::unique_ptr<cmi::hip::DeviceTape> tape;
+            try {
+              tape = std::make_unique<cmi::hip::DeviceTape>(a.ctx->ctx, a.problems[i]->problem);
+            } catch (const cmi::ResourceExhausted&) {
+              tape.reset();
+            } catch (const std::bad_alloc&) {
+              tape.reset();
+            }
+            used_bytes += priced_bytes;
             active.push_back(Active {i, std::move(tape),
                                      std::make_unique<cmi::search::SolveState>(a.problems[i]->problem, pol, sopt),
-                                     bytes});
+                                     priced_bytes});
             ++next;
           }
           peak = std::max(peak, active.size());
@@ -146,35 +146,12 @@ extern "C" cmi_status cmi_serve_stream(cmi_ctx* ctx, size_t n, const cmi_problem
             }
             continue;
           }
-          reqs.clear();
-          owners.clear();
-          for (std::size_t s = 0; s < active.size(); ++s) {
-            const cmi::search::SolveState::Request* q = active[s].state->next();
-            if (q == nullptr) {
-              continue;
-            }
-            owners.push_back({s, reqs.size(), q->kind == cmi::search::SolveState::Kind::Enumerate});
-            if (q->kind == cmi::search::SolveState::Kind::Enumerate) {
-              reqs.push_back({active[s].tape.get(), cmi::dev::Mode::Enumerate, q->box, q->assignment, q->salt, 1});
-            } else {
-              reqs.push_back({active[s].tape.get(), cmi::dev::Mode::Sample, q->box, q->assignment, q->salt,
-                              static_cast<std::uint32_t>(a.o->sample_points)});
-              reqs.push_back({active[s].tape.get(), cmi::dev::Mode::Bound, q->box, q->assignment, q->salt, 1});
-            }
+          members.clear();
+          for (Active& r : active) {
+            members.push_back({r.state.get(), r.tape.get()});
           }
-          if (!reqs.empty()) {
-            runner.run(reqs);
+          if (cmi::batch::step(runner, members, *a.o, *a.ctx->ctx) > 0) {
             ++steps;
-            for (Owner const& w : owners) {
-              cmi::search::SolveState& st = *active[w.slot].state;
-              if (w.enumerate) {
-                st.on_enumerate(runner.candidate(w.first_req), runner.n_elems(w.first_req));
-              } else {
-                cmi::backend::BoundResult const b {runner.feasible(w.first_req + 1), runner.obj_lb(w.first_req + 1),
-                                                   runner.n_elems(w.first_req + 1)};
-                st.on_subdivide(runner.candidate(w.first_req), runner.n_elems(w.first_req), b);
-              }
-            }
           }
           // Retire finished solves; their slots are free for the next admission.
           double const t_done = elapsed();

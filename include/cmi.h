@@ -646,7 +646,12 @@ cmi_status cmi_solve(cmi_ctx* ctx, const cmi_problem* p, const cmi_solve_options
 /* The same solves for many problems, interleaved: every step runs one search
  * iteration of every unfinished solve in one sampler launch and one bounder
  * launch (the batched form of AggregateBounder::bound_children's span API,
- * aggregate/bound.hpp:85-100). Per-solve results equal cmi_solve's. */
+ * aggregate/bound.hpp:85-100). Per-solve results equal cmi_solve's, including
+ * a solve whose role builds do not fit budget_bytes (it stops with
+ * CMI_STOP_DEVICE_MEMORY and its facts; the others go on). Host budget: a
+ * non-zero host_budget_bytes is each solve's own, as in cmi_solve; 0 reads
+ * half of MemAvailable once and gives each of the n solves an equal share.
+ * With budget_bytes 0 the free device memory is read once per step. */
 cmi_status cmi_solve_batch(cmi_ctx* ctx, size_t n, const cmi_problem* const* problems, const cmi_solve_options* o,
                            cmi_solve_outcome* outs, double* const* best_points);
 
@@ -675,7 +680,13 @@ typedef struct
  * cmi_worst_footprint; the first resident solve is always admitted); every step advances every resident solve by one
  * search iteration in one launch per role kind; a finished solve frees its
  * slot at once. finish[i] (seconds since the call started) and latency[i]
- * (finish - arrival) may be NULL. Per-request results equal cmi_solve's. */
+ * (finish - arrival) may be NULL. Per-request results equal cmi_solve's, as
+ * for cmi_solve_batch. A request is priced (tape + worst-composition role
+ * footprint) from a host compile before any device memory is taken; its tape
+ * is uploaded only once it is admitted. Host budget: a non-zero
+ * host_budget_bytes is each solve's own; 0 reads half of MemAvailable once
+ * and gives each of the max_active slots an equal share. Calls that overlap
+ * on one context are not supported. */
 cmi_status cmi_serve_stream(cmi_ctx* ctx, size_t n, const cmi_problem* const* problems, const double* arrival,
                             const cmi_solve_options* o, const cmi_server_options* so, cmi_solve_outcome* outs,
                             double* finish, double* latency, cmi_serve_stats* stats);

@@ -151,3 +151,33 @@ def test_known_primal_bound_prunes_like_an_incumbent(ctx):
     same_outcome(d, o)
     assert d.iterations <= plain.iterations
     assert d.upper_bound <= plain.upper_bound + 1e-3
+
+
+def test_batched_and_served_solves_stop_over_budget_like_plain(ctx):
+    """A device budget that the larger instances' roles do not fit: each such
+    solve stops with DeviceMemory and the facts of the build that did not fit,
+    exactly as its plain solve does, and the other solves in the same batch or
+    stream are unaffected (SearchDriver's catch sites, driver.hpp:401-425)."""
+    probs = [cmi.EngineProblem.qcqp(300 + i, n=n, m=n // 2, cont_lb=-5.0, cont_ub=5.0, pair_density=0.2)
+             for i, n in enumerate([4, 24, 6, 40, 5, 32])]
+    opts = dict(partition_num=4, enumerate_cap=4, max_cycle_size=3, sample_points=64, iter_limit=20)
+    needs = []
+    for p in probs:
+        cm = cmi.Tape(ctx, p).cost_model(64)
+        needs.append(cm.sampler_fixed + cm.sampler_per_region * 64)
+    opts["budget_bytes"] = int(sorted(needs)[len(needs) // 2])
+    plain = [cmi.solve(ctx, p, **opts)[0] for p in probs]
+    stopped = [d.stop_reason == 5 for d in plain]  # device-memory
+    assert any(stopped) and not all(stopped)
+    assert all(d.has_over_budget == s for d, s in zip(plain, stopped))
+    batched, _ = cmi.solve_batch(ctx, probs, **opts)
+    served, _, _, _ = cmi.serve_stream(ctx, probs, np.zeros(len(probs)), max_active=3, **opts)
+    for d, b, s in zip(plain, batched, served):
+        same_outcome(b, d)
+        same_outcome(s, d)
+        assert b.has_over_budget == d.has_over_budget == s.has_over_budget
+        if d.has_over_budget:
+            want = d.as_dict()["over_budget"]
+            assert b.as_dict()["over_budget"] == want and s.as_dict()["over_budget"] == want
+            assert want["role"] == "point sampler" and want["elements_per_region"] == 64
+            assert want["composition"] == "CCC" and want["solve_samples_per_region"] == 64
