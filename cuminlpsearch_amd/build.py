@@ -95,6 +95,26 @@ def build_engine(jobs: int = 8) -> pathlib.Path:
     return ENGINE_SO
 
 
+ADAPTER_DIR = ROOT / "tests" / "adapter"
+ADAPTER_TEST = ADAPTER_DIR / "build" / "adapter_test"
+
+
+def adapter_test_command(out: pathlib.Path) -> list[str]:
+    """g++ line of the reference-side adapter's test program (host code only:
+    it reaches the device through libcmi.so, and checks against liboracle.so)."""
+    return ["g++", "-std=c++20", "-O1", "-Wall", "-Wextra", "-Werror", f"-I{ADAPTER_DIR}", f"-I{ROOT / 'include'}",
+            str(ADAPTER_DIR / "adapter_test.cpp"), "-o", str(out), f"-L{LIB_DIR}", f"-L{ORACLE_DIR}", "-lcmi",
+            "-loracle", "-Wl,-rpath,$ORIGIN/../../../cuminlpsearch_amd/lib:$ORIGIN/../../../oracle"]
+
+
+def build_adapter_test() -> pathlib.Path:
+    ADAPTER_TEST.parent.mkdir(parents=True, exist_ok=True)
+    tmp = ADAPTER_TEST.with_suffix(".tmp")
+    _run(adapter_test_command(tmp))
+    os.replace(tmp, ADAPTER_TEST)
+    return ADAPTER_TEST
+
+
 def build_oracle() -> pathlib.Path:
     make = shutil.which("make")
     if make is None:
@@ -114,6 +134,8 @@ def main(argv: list[str] | None = None) -> int:
         print(build_engine(a.j))
     if a.oracle or both:
         print(build_oracle())
+    if both:
+        print(build_adapter_test())
     return 0
 
 
