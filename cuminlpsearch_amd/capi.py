@@ -290,6 +290,8 @@ _SIGNATURES = {
                                       C.POINTER(C.c_size_t)]),
     "cmi_auto_max_cycle_size": (C.c_int, [_P, C.POINTER(ProblemProfile), C.c_size_t, C.c_size_t, C.c_size_t,
                                           C.c_size_t, C.c_size_t, C.POINTER(C.c_size_t)]),
+    "cmi_sample_noalloc": (C.c_int, [_P, _P, C.c_size_t, _P, C.c_size_t, _P, _P, C.c_size_t, C.c_uint64, _P, _P, _P,
+                                      _P]),
     "cmi_explain_over_budget": (C.c_int, [_P, C.c_int, C.POINTER(OverBudget), C.c_size_t, C.c_size_t, C.c_size_t,
                                           C.c_char_p, C.c_size_t, C.POINTER(C.c_size_t)]),
     "cmi_resolve_shape": (C.c_int, [_P, C.POINTER(ResolveRequest), C.POINTER(ResolvedShape)]),

@@ -943,6 +943,23 @@ cmi_status cmi_enumerate(cmi_role* r, const cmi_ivl* box, size_t n_vars, const s
   return s;
 }
 
+cmi_status cmi_sample_noalloc(cmi_tape* t, const uint8_t* slot_kinds, size_t slot_count, const cmi_ivl* box,
+                              size_t n_vars, const size_t* var_ids, const size_t* widths, size_t samples_per_region,
+                              uint64_t salt, int* found, double* value, int64_t* index, double* point)
+{
+  if (t == nullptr || box == nullptr || found == nullptr || value == nullptr || index == nullptr || point == nullptr
+      || (slot_count > 0 && (slot_kinds == nullptr || var_ids == nullptr || widths == nullptr))) {
+    return fail(CMI_INVALID_HANDLE, "null argument");
+  }
+  return guarded([&] {
+    auto const comp = composition_of(slot_kinds, slot_count);
+    std::span<const cmi::interval> const b(reinterpret_cast<const cmi::interval*>(box), n_vars);
+    cmi::hip::sample_static(*t->tape, comp, b, {var_ids, slot_count}, {widths, slot_count}, samples_per_region, salt,
+                            value, index, point);
+    *found = *value < std::numeric_limits<double>::max() ? 1 : 0;
+  });
+}
+
 size_t cmi_role_size(const cmi_role* r)
 {
   if (r == nullptr) {

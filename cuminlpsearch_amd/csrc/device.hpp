@@ -106,4 +106,21 @@ void launch_points(const tape::Instr* code, bool extended, std::uint32_t n_instr
 /// Largest per-lane slot count a 64-lane workgroup can hold in LDS.
 std::uint32_t max_lds_slots(bool interval);
 
+/// Size of the file-scope device array (kernels.hip, `scratch`) that the
+/// no-allocation entry points use instead of allocated scratch. Every device
+/// the library runs on holds one copy until the process ends.
+inline constexpr std::size_t kStaticScratchBytes = std::size_t(4) << 20;
+inline constexpr int kMaxDevices = 64;
+
+/// The current device's copy of that array, resolved with
+/// hipGetSymbolAddress the first time a device asks (the context's
+/// constructor does, with its device current) and kept per device; host code
+/// must use this address, never the array's name. Throws DeviceError if the
+/// symbol cannot be resolved.
+void* static_scratch();
+
+/// Fill n 32-bit words at p with value, as a kernel (capturable, unlike a
+/// memset node).
+void launch_fill_u32(std::uint32_t* p, std::size_t n, std::uint32_t value, hipStream_t stream);
+
 }  // namespace cmi::dev

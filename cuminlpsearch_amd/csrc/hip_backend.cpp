@@ -3,6 +3,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <limits>
 #include <stdexcept>
 
@@ -36,6 +37,9 @@ Context::Context(int device)
   lds_interval_slots_ = dev::max_lds_slots(true);
   lds_point_slots_ = dev::max_lds_slots(false);
   lds_bytes_ = dev::lds_limit_bytes();
+  // Resolve this device's copy of the static scratch now, with the device
+  // current, so the no-allocation entry points never look it up mid-call.
+  scratch_ = dev::static_scratch();
 }
 
 Context::~Context()
@@ -684,6 +688,140 @@ void RoleImpl::run(std::span<const interval> box, std::span<const std::size_t> v
     check(hipMemcpyAsync(witness_h_.data(), witness_d_.get(), n_vars_ * sizeof(double), hipMemcpyDeviceToHost, s),
           "hipMemcpyAsync(witness)");
   }
+  check(hipStreamSynchronize(s), "hipStreamSynchronize");
+}
+
+namespace
+{
+
+/// Where each piece of a no-allocation sampler launch sits in the static
+/// scratch array.
+struct StaticLayout
+{
+  std::size_t parent, var, width, prefix, kind, tile_item, tile_min, tile_arg, done, out, witness, total;
+
+  StaticLayout(std::size_t n_vars, std::size_t slots, std::size_t n_tiles)
+  {
+    parent = align_up(sizeof(dev::Item), 256);
+    var = align_up(parent + n_vars * 16, 16);
+    width = align_up(var + slots * 4, 16);
+    prefix = align_up(width + slots * 4, 16);
+    kind = align_up(prefix + slots * 8, 16);
+    tile_item = align_up(kind + slots, 256);
+    tile_min = align_up(tile_item + n_tiles * 4, 256);
+    tile_arg = align_up(tile_min + n_tiles * 8, 256);
+    done = align_up(tile_arg + n_tiles * 8, 256);
+    out = align_up(done + 4, 16);
+    witness = align_up(out + 16, 16);
+    total = align_up(witness + std::max<std::size_t>(n_vars, 1) * 8, 256);
+  }
+};
+
+std::mutex g_static_mu[dev::kMaxDevices];
+
+}  // namespace
+
+std::size_t sample_static_bytes(std::size_t n_vars, std::size_t slot_count, std::size_t n_elems)
+{
+  return StaticLayout(n_vars, slot_count, (n_elems + dev::kTile - 1) / dev::kTile).total;
+}
+
+void sample_static(const DeviceTape& tape, const region::Composition& composition, std::span<const interval> box,
+                   std::span<const std::size_t> var_ids, std::span<const std::size_t> widths,
+                   std::size_t samples_per_region, std::uint64_t salt, double* value, std::int64_t* index,
+                   double* point)
+{
+  std::size_t const n_vars = tape.n_vars();
+  std::size_t const slots = composition.size();
+  if (box.size() != n_vars) {
+    throw ShapeMismatch("set_domain: domain size does not match the problem's variable count");
+  }
+  if (var_ids.size() != slots || widths.size() != slots) {
+    throw ShapeMismatch("set_domain: var_ids/widths size does not match the composition's slot count");
+  }
+  for (std::size_t j = 0; j < slots; ++j) {
+    if (var_ids[j] >= n_vars) {
+      throw ShapeMismatch("set_domain: slot var_id out of range");
+    }
+  }
+  std::size_t const n_regions = region::composition_fan_out(widths);
+  // Shape and LDS checks of a sampler build; no budget, since nothing is allocated.
+  check_role_fits(tape, dev::Mode::Sample, composition, n_regions, samples_per_region,
+                  std::numeric_limits<std::size_t>::max());
+  std::size_t const n_elems = n_regions * samples_per_region;
+  std::size_t const n_tiles = (n_elems + dev::kTile - 1) / dev::kTile;
+  StaticLayout const L(n_vars, slots, n_tiles);
+  if (L.total > dev::kStaticScratchBytes) {
+    throw InvalidConfiguration("this sample needs " + std::to_string(L.total) + " bytes of static scratch ("
+                               + std::to_string(n_tiles) + " tiles of 64 points, " + std::to_string(n_vars)
+                               + " variables); the array holds " + std::to_string(dev::kStaticScratchBytes));
+  }
+
+  Context const& ctx = *tape.context();
+  ctx.activate();
+  std::lock_guard<std::mutex> lock(g_static_mu[ctx.device()]);
+  auto* const base = static_cast<unsigned char*>(ctx.static_scratch());
+  hipStream_t const s = ctx.stream();
+
+  // The Item and the slot arrays are assembled on the stack (at most
+  // kMaxSlotsPerItem slots), the parent box is copied from the caller's array.
+  dev::Item item {};
+  bool const grouped = tape.items_grouped_fit(dev::Mode::Sample);
+  tape.item_code(item, grouped);
+  item.n_vars = static_cast<std::uint32_t>(n_vars);
+  item.parent = reinterpret_cast<const double*>(base + L.parent);
+  item.var_kinds = tape.var_kinds();
+  item.slot_var = reinterpret_cast<const std::uint32_t*>(base + L.var);
+  item.slot_width = reinterpret_cast<const std::uint32_t*>(base + L.width);
+  item.slot_prefix = reinterpret_cast<const std::uint64_t*>(base + L.prefix);
+  item.slot_kind = reinterpret_cast<const std::uint8_t*>(base + L.kind);
+  item.slot_count = static_cast<std::uint32_t>(slots);
+  item.samples = static_cast<std::uint32_t>(samples_per_region);
+  item.n_elems = n_elems;
+  item.salt = salt;
+  item.tile_begin = 0;
+  item.n_tiles = static_cast<std::uint32_t>(n_tiles);
+  item.tile_min = reinterpret_cast<double*>(base + L.tile_min);
+  item.tile_arg = reinterpret_cast<std::int64_t*>(base + L.tile_arg);
+  item.done = reinterpret_cast<std::uint32_t*>(base + L.done);
+  item.cand_value = reinterpret_cast<double*>(base + L.out);
+  item.cand_index = reinterpret_cast<std::int64_t*>(base + L.out + 8);
+  item.witness = reinterpret_cast<double*>(base + L.witness);
+  std::uint32_t var[dev::kMaxSlotsPerItem] = {};
+  std::uint32_t width[dev::kMaxSlotsPerItem] = {};
+  std::uint64_t prefix[dev::kMaxSlotsPerItem] = {};
+  std::uint8_t kind[dev::kMaxSlotsPerItem] = {};
+  std::uint64_t running = 1;
+  for (std::size_t j = 0; j < slots; ++j) {
+    var[j] = static_cast<std::uint32_t>(var_ids[j]);
+    width[j] = static_cast<std::uint32_t>(widths[j]);
+    prefix[j] = running;
+    running *= widths[j];
+    kind[j] = static_cast<std::uint8_t>(composition[j]);
+  }
+  check(hipMemcpyAsync(base, &item, sizeof(item), hipMemcpyHostToDevice, s), "hipMemcpyAsync(static item)");
+  check(hipMemcpyAsync(base + L.parent, box.data(), n_vars * sizeof(interval), hipMemcpyHostToDevice, s),
+        "hipMemcpyAsync(static parent)");
+  if (slots > 0) {
+    check(hipMemcpyAsync(base + L.var, var, slots * 4, hipMemcpyHostToDevice, s), "hipMemcpyAsync(static var)");
+    check(hipMemcpyAsync(base + L.width, width, slots * 4, hipMemcpyHostToDevice, s), "hipMemcpyAsync(static width)");
+    check(hipMemcpyAsync(base + L.prefix, prefix, slots * 8, hipMemcpyHostToDevice, s),
+          "hipMemcpyAsync(static prefix)");
+    check(hipMemcpyAsync(base + L.kind, kind, slots, hipMemcpyHostToDevice, s), "hipMemcpyAsync(static kind)");
+  }
+  // Every tile belongs to item 0, and the done counter starts at zero: the
+  // array keeps whatever the previous call left, so both are cleared here.
+  dev::launch_fill_u32(reinterpret_cast<std::uint32_t*>(base + L.tile_item), n_tiles, 0u, s);
+  dev::launch_fill_u32(reinterpret_cast<std::uint32_t*>(base + L.done), 1, 0u, s);
+  check(hipGetLastError(), "launch(fill static scratch)");
+  dev::launch_items(dev::Mode::Sample, tape.extended(), reinterpret_cast<const dev::Item*>(base),
+                    reinterpret_cast<const std::uint32_t*>(base + L.tile_item), static_cast<std::uint32_t>(n_tiles),
+                    tape.item_slots(grouped), tape.item_waves(grouped), s);
+  check(hipGetLastError(), "launch(k_items static)");
+  check(hipMemcpyAsync(value, base + L.out, 8, hipMemcpyDeviceToHost, s), "hipMemcpyAsync(static value)");
+  check(hipMemcpyAsync(index, base + L.out + 8, 8, hipMemcpyDeviceToHost, s), "hipMemcpyAsync(static index)");
+  check(hipMemcpyAsync(point, base + L.witness, n_vars * sizeof(double), hipMemcpyDeviceToHost, s),
+        "hipMemcpyAsync(static witness)");
   check(hipStreamSynchronize(s), "hipStreamSynchronize");
 }
 

@@ -141,6 +141,9 @@ public:
   /// LDS one workgroup may use.
   std::size_t lds_bytes() const { return lds_bytes_; }
   std::size_t free_bytes() const;
+  /// This device's copy of the static scratch array (dev::static_scratch),
+  /// resolved when the context was created.
+  void* static_scratch() const { return scratch_; }
 
 private:
   int device_;
@@ -148,6 +151,7 @@ private:
   std::uint32_t lds_interval_slots_ = 0;
   std::uint32_t lds_point_slots_ = 0;
   std::size_t lds_bytes_ = 0;
+  void* scratch_ = nullptr;
 };
 
 /// A problem compiled to a tape and uploaded once.
@@ -334,6 +338,21 @@ private:
   PinnedBuffer<double> witness_h_;
   std::size_t off_parent_ = 0, off_var_ = 0, off_width_ = 0, off_prefix_ = 0, off_kind_ = 0, staging_bytes_ = 0;
 };
+
+/// The sampler's launch without a role and without allocating: staging, tile
+/// partials, the done counter and the candidate live in the device's static
+/// scratch array (dev::kStaticScratchBytes). Same checks and results as a
+/// sampler RoleImpl over the same arguments; a launch whose scratch would
+/// exceed the array throws InvalidConfiguration before anything runs. Calls
+/// on one device are serialised; value, index and point (n_vars doubles) are
+/// host memory the caller owns.
+void sample_static(const DeviceTape& tape, const region::Composition& composition, std::span<const interval> box,
+                   std::span<const std::size_t> var_ids, std::span<const std::size_t> widths,
+                   std::size_t samples_per_region, std::uint64_t salt, double* value, std::int64_t* index,
+                   double* point);
+
+/// Static-scratch bytes sample_static needs for these sizes.
+std::size_t sample_static_bytes(std::size_t n_vars, std::size_t slot_count, std::size_t n_elems);
 
 class HipBackendFactory : public backend::RegionBackendFactory
 {

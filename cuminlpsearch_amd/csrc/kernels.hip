@@ -28,7 +28,10 @@
 #include <cfloat>
 #include <cmath>
 #include <cstdint>
+#include <mutex>
+#include <string>
 
+#include "cmi/errors.hpp"
 #include "cmi/tape.hpp"
 #include "device.hpp"
 #include "interval_ops.hpp"
@@ -885,6 +888,63 @@ void launch_points(const Instr* code, bool extended, std::uint32_t n_instr, std:
   } else {
     launch_points_as<false>(code, n_instr, n_slots, n_vars, x, ld, n_points, feasible, obj, roots, stream);
   }
+}
+
+}  // namespace cmi::dev
+
+// ---------------------------------------------------------------------------
+// Static scratch for the entry points that allocate nothing
+// ---------------------------------------------------------------------------
+
+namespace
+{
+
+// Staging, tile partials and the done counter of a no-allocation launch. In
+// host code `scratch` is only a stand-in for the symbol: its device address
+// comes from hipGetSymbolAddress, per device (cmi::dev::static_scratch).
+static __device__ __align__(256) unsigned char scratch[cmi::dev::kStaticScratchBytes];
+
+__global__ void k_fill_u32(std::uint32_t* __restrict__ p, std::uint64_t n, std::uint32_t value)
+{
+  std::uint64_t const i = static_cast<std::uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < n) {
+    p[i] = value;
+  }
+}
+
+}  // namespace
+
+namespace cmi::dev
+{
+
+void* static_scratch()
+{
+  static std::mutex mu;
+  static void* address[kMaxDevices] = {};
+  int device = -1;
+  if (hipGetDevice(&device) != hipSuccess || device < 0 || device >= kMaxDevices) {
+    throw DeviceError(static_cast<int>(hipErrorInvalidDevice), "static scratch: no current device in [0, " + std::to_string(kMaxDevices) + ")");
+  }
+  std::lock_guard<std::mutex> lock(mu);
+  if (address[device] == nullptr) {
+    void* p = nullptr;
+    hipError_t const e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || p == nullptr) {
+      throw DeviceError(static_cast<int>(e), std::string("hipGetSymbolAddress(scratch) failed: ") + hipGetErrorString(e));
+    }
+    address[device] = p;
+  }
+  return address[device];
+}
+
+void launch_fill_u32(std::uint32_t* p, std::size_t n, std::uint32_t value, hipStream_t stream)
+{
+  if (n == 0) {
+    return;
+  }
+  constexpr unsigned kBlock = 256;
+  auto const blocks = static_cast<unsigned>((n + kBlock - 1) / kBlock);
+  hipLaunchKernelGGL(k_fill_u32, dim3(blocks), dim3(kBlock), 0, stream, p, static_cast<std::uint64_t>(n), value);
 }
 
 }  // namespace cmi::dev

@@ -459,6 +459,25 @@ cmi_status cmi_sample(cmi_role* r, const cmi_ivl* box, size_t n_vars, const size
 cmi_status cmi_enumerate(cmi_role* r, const cmi_ivl* box, size_t n_vars, const size_t* var_ids,
                          const size_t* widths, size_t slot_count, cmi_candidate_view* out);
 size_t cmi_role_size(const cmi_role* r); /* n_regions / n_samples / n_points */
+
+/* The sampler's search for the best sampled point of a region, without a role
+ * and without allocating: no hipMalloc/hipFree and no allocating library
+ * call. No counterpart in the reference; it is for callers that cannot
+ * allocate device memory (code under graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc). Arguments and results are cmi_sample's
+ * on a sampler built for (slot_kinds, prod(widths) regions,
+ * samples_per_region), with the candidate written to caller-owned host memory
+ * (found, value, index; point holds n_vars doubles). Staging, per-tile
+ * partial minima and the done counter live in a 4 MiB file-scope device array
+ * per GPU, held until the process ends and kept between calls (the call
+ * clears what must start at zero). A call whose scratch would exceed it (about
+ * 200,000 tiles of 64 points, fewer with many variables) returns
+ * CMI_INVALID_CONFIG before anything runs. Calls on one device are
+ * serialised inside the process; calls that overlapped on one GPU would race
+ * on the shared array. */
+cmi_status cmi_sample_noalloc(cmi_tape* t, const uint8_t* slot_kinds, size_t slot_count, const cmi_ivl* box,
+                              size_t n_vars, const size_t* var_ids, const size_t* widths, size_t samples_per_region,
+                              uint64_t salt, int* found, double* value, int64_t* index, double* point);
 void cmi_role_destroy(cmi_role* r);
 
 /* -- aggregate (k-child) bounder ------------------------------------------------ */
