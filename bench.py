@@ -6,15 +6,21 @@ bilinear pairs, 600 reachable DAG nodes, evaluated over 65,536 sub-boxes
 (log-uniform widths 1e-4..10) resident in HBM. One step = one launch over all
 65,536 boxes. value = boxes/s.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-serve] [--no-batch] [--no-cpu]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-solves] [--no-serve] [--no-batch] [--no-cpu]
+                    [--full-stream]
 
-Also reported, on the same line: "serving" -- served (continuous batching) vs
-plain (one solve at a time) solves/s and p50/p99 latency on Poisson streams of
-seeded QCQPs at 50, 200 and 2,000 req/s (n up to 16), and at 200 req/s over
-the full size range n in [4, 96] ("wide") -- and "batching" -- 512 seeded QCQPs
-solved in shared launches vs one after another; "autotune" -- every
-explicit-box kernel shape the autotuner timed on this workload;
-"cpu_baseline" -- the CPU restatement on the host.
+Also reported, on the same line:
+  "cpu_baseline"  the CPU port of the evaluation on 1 thread and on every
+                  worker thread, with the CPU model;
+  "solves"        the 4-variable QCQP solved to gap 1e-6 for 32 seeds: GPU
+                  plain and batched, and the CPU oracle's search;
+  "serving"       served (continuous batching) vs plain solves/s and p50/p99
+                  latency on Poisson streams at 50, 200 and 2,000 req/s, over
+                  the full size range n in [4, 96] ("wide"), and with
+                  --full-stream the 20,000-request stream at 2,000 req/s;
+  "batching"      512 seeded QCQPs solved in shared launches vs one after
+                  another;
+  "autotune"      every explicit-box kernel shape the autotuner timed.
 """
 from __future__ import annotations
 
@@ -50,25 +56,48 @@ def make_workload():
     return eng, lb, ub
 
 
-def cpu_baseline(arrays, lb, ub, budget_s: float = 12.0):
-    """The oracle (CPU restatement) on host threads over a bounded prefix of the
-    same boxes; returns (boxes/s, threads, sample size, per-box results)."""
+def cpu_model() -> str:
+    try:
+        for line in pathlib.Path("/proc/cpuinfo").read_text().splitlines():
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return platform.processor() or platform.machine()
+
+
+def host_threads() -> int:
+    """Worker threads for the CPU legs: the host's cores, at most 16 (one GPU's
+    share of a GPU box's CPUs)."""
+    return max(1, min(16, os.cpu_count() or 1))
+
+
+def cpu_baseline(arrays, lb, ub, budget_s: float = 8.0):
+    """The CPU port of the DAG evaluation (the oracle's node-by-node evaluation
+    with round-up set once per call, bit-identical to the oracle) over bounded
+    prefixes of the same boxes: on 1 thread, then on every worker thread.
+    Returns a dict with both rates and the all-threads results for the parity
+    check."""
     from oracle import binding as orc
 
-    threads = max(1, min(16, os.cpu_count() or 1))
-    # size the sample from a short single-thread probe
+    threads = host_threads()
     probe = 2000
     t0 = time.perf_counter()
-    orc.eval_boxes(arrays, lb[:, :probe], ub[:, :probe])
+    orc.eval_boxes_ru(arrays, lb[:, :probe], ub[:, :probe])
     per_box = (time.perf_counter() - t0) / probe
-    n = int(min(N_BOXES, max(threads * 1000, budget_s * threads / per_box)))
+    n1 = int(min(N_BOXES, max(probe, 0.25 * budget_s / per_box)))
+    t0 = time.perf_counter()
+    orc.eval_boxes_ru(arrays, np.ascontiguousarray(lb[:, :n1]), np.ascontiguousarray(ub[:, :n1]))
+    one = n1 / (time.perf_counter() - t0)
+
+    n = int(min(N_BOXES, max(threads * 1000, 0.75 * budget_s * threads / per_box)))
     n -= n % threads
     chunk = n // threads
     results = [None] * threads
 
     def work(k):
         s = slice(k * chunk, (k + 1) * chunk)
-        results[k] = orc.eval_boxes(arrays, np.ascontiguousarray(lb[:, s]), np.ascontiguousarray(ub[:, s]))
+        results[k] = orc.eval_boxes_ru(arrays, np.ascontiguousarray(lb[:, s]), np.ascontiguousarray(ub[:, s]))
 
     t0 = time.perf_counter()
     ts = [threading.Thread(target=work, args=(k,)) for k in range(threads)]
@@ -76,11 +105,9 @@ def cpu_baseline(arrays, lb, ub, budget_s: float = 12.0):
         t.start()
     for t in ts:
         t.join()
-    dt = time.perf_counter() - t0
-    feas = np.concatenate([r[0] for r in results])
-    olb = np.concatenate([r[1] for r in results])
-    oub = np.concatenate([r[2] for r in results])
-    return n / dt, threads, n, (feas, olb, oub)
+    many = n / (time.perf_counter() - t0)
+    merged = tuple(np.concatenate([r[i] for r in results]) for i in range(3))
+    return dict(one_thread=one, one_sample=n1, all_threads=many, threads=threads, sample=n, results=merged)
 
 
 CONVERGED_STOPS = (0, 1)  # gap closed, or every node fathomed
@@ -95,78 +122,165 @@ def _same_results(a_outs, b_outs) -> bool:
                and a.stop_reason == b.stop_reason for a, b in zip(a_outs, b_outs))
 
 
+C1_SOLVE_OPTS = dict(partition_num=8, enumerate_cap=8, max_cycle_size=4, sample_points=16, tolerance=1e-6)
+
+
+def c1_solves(ctx, seeds=range(32), iter_limit: int = 5000, cpu_iter_cap: int = 300):
+    """The 4-variable QCQP (benchmark config 1) solved to gap 1e-6 for 32
+    seeds: plain GPU solves one after another, the same 32 as one batch, and
+    the CPU oracle's restated search on the converged seeds that close within
+    `cpu_iter_cap` iterations (1 thread, then one seed per worker thread)."""
+    import cuminlpsearch_amd as cmi
+    from cuminlpsearch_amd import workloads as W
+    from oracle import binding as orc
+
+    opts = dict(C1_SOLVE_OPTS, iter_limit=iter_limit)
+    probs = [W.c1(s) for s in seeds]
+    rows = []
+    t_all = time.perf_counter()
+    for s, p in zip(seeds, probs):
+        t0 = time.perf_counter()
+        out, _ = cmi.solve(ctx, p, **opts)
+        rows.append(dict(seed=int(s), converged=out.stop_reason in CONVERGED_STOPS, stop=int(out.stop_reason),
+                         iterations=int(out.iterations), nodes=int(out.enqueued),
+                         bounded=int(out.bounded), seconds=time.perf_counter() - t0,
+                         lower=float(out.lower_bound), upper=float(out.upper_bound)))
+    t_plain = time.perf_counter() - t_all
+    conv = [r for r in rows if r["converged"]]
+    t0 = time.perf_counter()
+    outs_b, _ = cmi.solve_batch(ctx, probs, **opts)
+    t_batch = time.perf_counter() - t0
+    same = all(o.iterations == r["iterations"] and o.upper_bound == r["upper"] and o.lower_bound == r["lower"]
+               for o, r in zip(outs_b, rows))
+
+    cpu_seeds = [r["seed"] for r in conv if r["iterations"] <= cpu_iter_cap]
+    cpu = None
+    if cpu_seeds:
+        arrays = {s: W.c1(s).arrays() for s in cpu_seeds}
+        t0 = time.perf_counter()
+        cpu_outs = {s: orc.solve(arrays[s], **opts)[0] for s in cpu_seeds}
+        t_one = time.perf_counter() - t0
+        threads = min(host_threads(), len(cpu_seeds))
+        done = {}
+
+        def work(k):
+            for s in cpu_seeds[k::threads]:
+                done[s] = orc.solve(arrays[s], **opts)[0]
+
+        t0 = time.perf_counter()
+        ts = [threading.Thread(target=work, args=(k,)) for k in range(threads)]
+        for t in ts:
+            t.start()
+        for t in ts:
+            t.join()
+        t_many = time.perf_counter() - t0
+        gpu_time = sum(r["seconds"] for r in rows if r["seed"] in cpu_outs)
+        cpu = {
+            "seeds": cpu_seeds,
+            "solves_per_s_1_thread": len(cpu_seeds) / t_one,
+            "solves_per_s_all_threads": len(cpu_seeds) / t_many,
+            "threads": threads,
+            "gpu_plain_solves_per_s_same_seeds": len(cpu_seeds) / gpu_time,
+            "results_identical_to_gpu": all(cpu_outs[s].iterations == done[s].iterations
+                                            and next(r for r in rows if r["seed"] == s)["iterations"]
+                                            == cpu_outs[s].iterations for s in cpu_seeds),
+        }
+    return {
+        "workload": f"QCQP n=4 in [-10,10], m=2, Q and A_i symmetric N(0,1), seeds {seeds[0]}..{seeds[-1]}; "
+                    f"GreedyEnumerate partition {opts['partition_num']} over all 4 variables "
+                    f"({opts['partition_num'] ** 4} regions per iteration), {opts['sample_points']} samples per "
+                    f"region, gap 1e-6, cap {iter_limit} iterations",
+        "solves": len(rows),
+        "converged": len(conv),
+        "converged_fraction": len(conv) / len(rows),
+        "plain_solves_per_s_converged": (len(conv) / sum(r["seconds"] for r in conv)) if conv else None,
+        "plain_seconds_total": t_plain,
+        "batched_seconds_total": t_batch,
+        "batched_equals_plain": same,
+        "median_nodes_converged": float(np.median([r["nodes"] for r in conv])) if conv else None,
+        "per_seed": [{k: (round(v, 6) if isinstance(v, float) else v) for k, v in r.items()} for r in rows],
+        "cpu_oracle": cpu,
+        "cpu_model": cpu_model(),
+    }
+
+
 def serve_comparison(ctx, n_req: int = 128, rates=(50.0, 200.0, 2000.0), seed: int = 7, n_max: int = 16,
-                     iter_limit: int = 1000):
+                     iter_limit: int = 1000, plain: bool = True):
     """Served (continuous batching) vs plain (one solve at a time) on the same
     Poisson streams of seeded QCQPs, one stream per arrival rate: n
     log-uniform in [4, n_max], m = n/2, bilinear density U[0.1, 0.3], gap 1e-6,
     iteration cap per request. The first fields are those of the 200 req/s
-    stream; "rates" holds every stream."""
+    stream (or of the only rate); "rates" holds every stream. Served rates on
+    converged requests only are reported beside the all-request rates."""
     import cuminlpsearch_amd as cmi
+    from cuminlpsearch_amd import workloads as W
 
-    rng = np.random.default_rng(seed)
-    sizes = np.floor(np.exp(rng.uniform(np.log(4), np.log(n_max + 1), n_req))).astype(int)
-    probs = [cmi.EngineProblem.qcqp(1000 + i, n=int(n), m=int(n) // 2, cont_lb=-5.0, cont_ub=5.0,
-                                    pair_density=float(rng.uniform(0.1, 0.3))) for i, n in enumerate(sizes)]
-    gaps = rng.exponential(1.0, n_req)
+    st = W.c5_stream(n_req, seed=seed, n_max=n_max)
     opts = dict(partition_num=2, enumerate_cap=2, max_cycle_size=4, sample_points=16, iter_limit=iter_limit,
                 tolerance=1e-6)
     per_rate = []
     for rate in rates:
-        arrival = np.cumsum(gaps / rate)
-        arrival -= arrival[0]
-        outs_s, fin_s, lat_s, st_s = cmi.serve_stream(ctx, probs, arrival, max_active=256, **opts)
-        outs_p, fin_p, lat_p, st_p = cmi.serve_plain(ctx, probs, arrival, **opts)
-        per_rate.append({
+        arrival = st.arrivals(rate)
+        outs_s, fin_s, lat_s, st_s = cmi.serve_stream(ctx, st.problems, arrival, max_active=256, **opts)
+        conv = np.array([o.stop_reason in CONVERGED_STOPS for o in outs_s])
+        row = {
             "arrival_rate_per_s": rate,
             "served_solves_per_s": n_req / st_s.wall_seconds,
-            "plain_solves_per_s": n_req / st_p.wall_seconds,
-            "speedup": st_p.wall_seconds / st_s.wall_seconds,
+            "served_converged_solves_per_s": float(conv.sum()) / st_s.wall_seconds,
             "served_p50_ms": float(np.percentile(lat_s, 50) * 1e3),
             "served_p99_ms": float(np.percentile(lat_s, 99) * 1e3),
-            "plain_p50_ms": float(np.percentile(lat_p, 50) * 1e3),
-            "plain_p99_ms": float(np.percentile(lat_p, 99) * 1e3),
-            "results_identical": _same_results(outs_s, outs_p),
-            "converged_fraction": _converged(outs_s),
-        })
+            "converged_fraction": float(conv.mean()),
+            "served_p50_ms_converged": float(np.percentile(lat_s[conv], 50) * 1e3) if conv.any() else None,
+            "steps": int(st_s.steps),
+            "peak_active": int(st_s.peak_active),
+        }
+        if plain:
+            outs_p, fin_p, lat_p, st_p = cmi.serve_plain(ctx, st.problems, arrival, **opts)
+            row.update({
+                "plain_solves_per_s": n_req / st_p.wall_seconds,
+                "plain_converged_solves_per_s": float(conv.sum()) / st_p.wall_seconds,
+                "speedup": st_p.wall_seconds / st_s.wall_seconds,
+                "plain_p50_ms": float(np.percentile(lat_p, 50) * 1e3),
+                "plain_p99_ms": float(np.percentile(lat_p, 99) * 1e3),
+                "results_identical": _same_results(outs_s, outs_p),
+            })
+        per_rate.append(row)
     main = next((r for r in per_rate if r["arrival_rate_per_s"] == 200.0), per_rate[0])
     return {
-        "workload": f"Poisson streams, {n_req} QCQPs each, n log-uniform [4,{n_max}], m=n/2, density U[0.1,0.3], "
-                    f"gap 1e-6, cap {iter_limit} iterations, seed {seed}, rates {list(rates)} req/s",
+        "workload": st.description + f", gap 1e-6, cap {iter_limit} iterations, rates {list(rates)} req/s",
         "requests": n_req,
         **main,
-        "nodes_enqueued_total": int(sum(o.enqueued for o in outs_s)),
-        "results_identical": all(r["results_identical"] for r in per_rate),
+        "results_identical": all(r.get("results_identical", True) for r in per_rate),
         "rates": per_rate,
     }
 
 
-def batch_comparison(ctx, n_inst: int = 512, seed: int = 11, iter_limit: int = 100):
+def batch_comparison(ctx, n_inst: int = 512, iter_limit: int = 100):
     """Batched (every instance's regions in the same launches) vs sequential
     plain solves of the same seeded QCQPs: n uniform in [4, 48], m = n/2,
     bilinear density 0.15, gap 1e-6, iteration cap per instance."""
     import cuminlpsearch_amd as cmi
+    from cuminlpsearch_amd import workloads as W
 
-    rng = np.random.default_rng(seed)
-    sizes = rng.integers(4, 49, n_inst)
-    probs = [cmi.EngineProblem.qcqp(5000 + i, n=int(n), m=int(n) // 2, cont_lb=-5.0, cont_ub=5.0,
-                                    pair_density=0.15) for i, n in enumerate(sizes)]
+    b = W.c4_batch(n_inst)
     opts = dict(partition_num=2, enumerate_cap=2, max_cycle_size=4, sample_points=16, iter_limit=iter_limit,
                 tolerance=1e-6)
     t0 = time.perf_counter()
-    outs_b, _ = cmi.solve_batch(ctx, probs, **opts)
+    outs_b, _ = cmi.solve_batch(ctx, b.problems, **opts)
     t_b = time.perf_counter() - t0
     t0 = time.perf_counter()
-    outs_p = [cmi.solve(ctx, p, **opts)[0] for p in probs]
+    outs_p = [cmi.solve(ctx, p, **opts)[0] for p in b.problems]
     t_p = time.perf_counter() - t0
+    conv = sum(o.stop_reason in CONVERGED_STOPS for o in outs_b)
     return {
-        "workload": f"{n_inst} QCQPs, n uniform [4,48], m=n/2, density 0.15, gap 1e-6, cap {iter_limit} "
-                    f"iterations, seed {seed}",
+        "workload": b.description + f", gap 1e-6, cap {iter_limit} iterations",
         "instances": n_inst,
         "batched_solves_per_s": n_inst / t_b,
         "plain_solves_per_s": n_inst / t_p,
         "speedup": t_p / t_b,
-        "converged_fraction": _converged(outs_b),
+        "converged_fraction": conv / n_inst,
+        "batched_converged_solves_per_s": conv / t_b,
+        "plain_converged_solves_per_s": conv / t_p,
         "iterations_total": int(sum(o.iterations for o in outs_b)),
         "results_identical": _same_results(outs_b, outs_p),
     }
@@ -199,6 +313,12 @@ def main(argv=None) -> int:
     ap.add_argument("--no-batch", action="store_true",
                     help="skip the batched-vs-sequential comparison over 512 seeded instances")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU-baseline leg")
+    ap.add_argument("--no-solves", action="store_true",
+                    help="skip the 32 solves of the 4-variable QCQP to gap 1e-6 (GPU plain, batched, CPU oracle)")
+    ap.add_argument("--full-stream", action="store_true",
+                    help="also serve the whole 20,000-request Poisson stream (n in [4, 96]) at 2,000 req/s")
+    ap.add_argument("--full-stream-cap", type=int, default=20,
+                    help="iteration cap per request of the --full-stream run")
     ap.add_argument("--box-groups", type=int, default=0,
                     help="waves (function groups) per 64-box workgroup; 0 = chosen from the tape")
     ap.add_argument("--box-engine", choices=["tuned", "compiled", "interpreted", "auto"], default="tuned",
@@ -237,6 +357,7 @@ def main(argv=None) -> int:
     if not torch.cuda.is_available():
         print("no GPU visible", file=sys.stderr)
         return 2
+    t_start = time.perf_counter()
     dev = torch.device("cuda:0")
     ctx = cmi.Context(0)
     eng, lb, ub = make_workload()
@@ -436,27 +557,46 @@ def main(argv=None) -> int:
         line["autotune"] = sorted(tuning, key=lambda c: c["kernel_ms"])
 
     if not a.no_cpu:
+        print(f"bench: CPU baseline ({time.perf_counter() - t_start:.0f} s)", file=sys.stderr, flush=True)
         arrays = eng.arrays()
-        cpu_rate, threads, n_sample, (f_o, lo_o, hi_o) = cpu_baseline(arrays, lb, ub)
+        cpu = cpu_baseline(arrays, lb, ub)
+        f_o, lo_o, hi_o = cpu["results"]
+        n_sample = cpu["sample"]
         parity = bool(np.array_equal(f_o, feas[:n_sample]) and np.array_equal(lo_o.view(np.uint64),
                                                                               olb[:n_sample].view(np.uint64))
                       and np.array_equal(hi_o.view(np.uint64), oub[:n_sample].view(np.uint64)))
         line["cpu_baseline"] = {
-            "value": cpu_rate,
+            "value": cpu["all_threads"],
             "unit": "boxes/s",
-            "cores": threads,
+            "cores": cpu["threads"],
             "kind": "port",
-            "sample": f"first {n_sample} of the 65536 boxes, {threads} host threads, CPU oracle (C++ restatement, "
-                      f"MXCSR directed rounding per operation); {platform.processor() or platform.machine()}",
+            "value_1_thread": cpu["one_thread"],
+            "cpu_model": cpu_model(),
+            "sample": f"first {n_sample} of the 65536 boxes on {cpu['threads']} host threads "
+                      f"(first {cpu['one_sample']} on 1 thread); the oracle's node-by-node evaluation with "
+                      f"round-up set once per call, bit-identical to the oracle",
         }
         line["parity_vs_cpu"] = "bit-exact" if parity else "MISMATCH"
 
+    def progress(what):
+        print(f"bench: {what} ({time.perf_counter() - t_start:.0f} s)", file=sys.stderr, flush=True)
+
+    if not a.no_solves:
+        progress("solves of the 4-variable QCQP")
+        line["solves"] = c1_solves(ctx)
     if not a.no_serve:
+        progress("serving")
         line["serving"] = serve_comparison(ctx)
         # the survey's size range (n up to 96), on a shorter stream and cap
         wide = serve_comparison(ctx, n_req=32, rates=(200.0,), seed=8, n_max=96, iter_limit=100)
         line["serving"]["wide"] = {k: v for k, v in wide.items() if k != "rates"}
+        if a.full_stream:
+            progress("serving the 20,000-request stream")
+            full = serve_comparison(ctx, n_req=20000, rates=(2000.0,), seed=9, n_max=96, iter_limit=a.full_stream_cap,
+                                    plain=False)
+            line["serving"]["full_stream"] = {k: v for k, v in full.items() if k != "rates"}
     if not a.no_batch:
+        progress("batching")
         line["batching"] = batch_comparison(ctx)
 
     print(json.dumps(line))

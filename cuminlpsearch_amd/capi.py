@@ -37,6 +37,7 @@ CONTINUOUS, INTEGER, BINARY = range(3)
 LE, EQ = range(2)
 SLOT_CONTINUOUS, SLOT_INTEGER_PARTITION, SLOT_INTEGER_ENUMERATE, SLOT_BINARY_ENUMERATE = range(4)
 ROLE_SAMPLER, ROLE_BOUNDER, ROLE_ENUMERATOR = range(3)
+POLICY_GREEDY_ENUMERATE, POLICY_BISECTION_BUDGET, POLICY_RELATIVE_BISECTION_BUDGET = range(3)
 STOP_NAMES = ["converged", "exhausted", "iteration-limit", "host-memory", "allocation-failure", "device-memory"]
 
 
@@ -150,7 +151,8 @@ class SolveOptions(C.Structure):
         ("partition_num", C.c_size_t), ("enumerate_cap", C.c_size_t), ("max_cycle_size", C.c_size_t),
         ("sample_points", C.c_size_t), ("iter_limit", C.c_uint32), ("tolerance", C.c_double),
         ("budget_bytes", C.c_size_t), ("host_budget_bytes", C.c_size_t), ("compact_frontier", C.c_int),
-        ("has_known_primal", C.c_int), ("known_primal", C.c_double),
+        ("has_known_primal", C.c_int), ("known_primal", C.c_double), ("policy", C.c_int),
+        ("bisection_budget", C.c_size_t),
     ]
 
 

@@ -280,12 +280,22 @@ void fill_outcome(const cmi::search::SolveOutcome& o, double seconds, cmi_solve_
 namespace cmi::capi
 {
 
-std::shared_ptr<cmi::policy::GreedyEnumCompositionPolicy> make_policy(const cmi_solve_options& o)
+std::shared_ptr<const cmi::policy::CompositionPolicy> make_policy(const cmi_solve_options& o)
 {
   cmi::policy::SearchCalibration cal;
   cal.max_cycle_size = o.max_cycle_size;
-  return std::make_shared<cmi::policy::GreedyEnumCompositionPolicy>(
-      cmi::region::FanOutSpec(o.partition_num, o.enumerate_cap), cal);
+  switch (o.policy) {
+    case CMI_POLICY_GREEDY_ENUMERATE:
+      return std::make_shared<cmi::policy::GreedyEnumCompositionPolicy>(
+          cmi::region::FanOutSpec(o.partition_num, o.enumerate_cap), cal);
+    case CMI_POLICY_BISECTION_BUDGET:
+    case CMI_POLICY_RELATIVE_BISECTION_BUDGET:
+      cal.enumerate_cap = o.enumerate_cap;
+      return std::make_shared<cmi::policy::BisectionBudgetCompositionPolicy>(
+          o.bisection_budget, cal, o.policy == CMI_POLICY_RELATIVE_BISECTION_BUDGET);
+    default: break;
+  }
+  throw cmi::InvalidConfiguration("unknown composition policy " + std::to_string(o.policy));
 }
 
 cmi_status run_guarded(void (*fn)(void*), void* arg)

@@ -168,4 +168,173 @@ private:
   region::FanOutSpec fan_out_;
 };
 
+/// BisectionBudget (reference policy/bisection_budget.hpp:40-299): every
+/// non-leaf node splits into exactly N = 2^B children, the B bisections shared
+/// out per node. Enumerable discrete variables are taken first, cheapest
+/// (fewest bisections, ceil(log2(domain)), at least 1) first, while their cost
+/// fits the budget; an integer that does not fit is bisected instead. The rest
+/// of the budget goes one bisection at a time to the candidate with the widest
+/// current width (absolute, or relative to its width at the start of the node
+/// when relative_width is set), through std::make_heap / pop_heap / push_heap
+/// with a less-than on the key, so ties go where those algorithms put them.
+/// Budget left once every candidate is resolved pads the taken enumerate slot
+/// with the most bisections (the first such on ties); the padded children are
+/// duplicates (domain_sizes < widths). Candidates are gathered in variable
+/// order up to max_cycle_size; an integer with more than enumerate_cap points
+/// (calibration; 0 = no ceiling) is bisected rather than enumerated.
+class BisectionBudgetCompositionPolicy : public CompositionPolicy
+{
+public:
+  explicit BisectionBudgetCompositionPolicy(std::size_t bisection_budget, SearchCalibration calibration = {},
+                                            bool relative_width = false)
+      : CompositionPolicy(calibration)
+      , budget_(bisection_budget)
+      , relative_(relative_width)
+  {
+    if (budget_ < 1 || budget_ > 62) {
+      throw InvalidConfiguration("bisection budget " + std::to_string(budget_)
+                                 + " is outside [1, 62]: a node needs at least one bisection and 2^B regions must be "
+                                   "countable in 64 bits");
+    }
+  }
+
+  std::size_t bisection_budget() const { return budget_; }
+  bool relative_width() const { return relative_; }
+
+  std::size_t n_regions(const region::Composition& composition) const override
+  {
+    return composition.size() == 0 ? 1 : std::size_t(1) << budget_;
+  }
+
+  /// Bisections an enumerate slot over d points costs: ceil(log2(max(d, 1))),
+  /// at least 1.
+  static std::size_t enumerate_cost(std::size_t d)
+  {
+    std::size_t bits = 0;
+    while ((std::size_t(1) << bits) < std::max<std::size_t>(d, 1)) {
+      ++bits;
+    }
+    return bits == 0 ? 1 : bits;
+  }
+
+  region::SlotAssignment choose(std::span<const interval> box,
+                                std::span<const model::VarKind> var_kinds) const override
+  {
+    if (var_kinds.empty()) {
+      throw ShapeMismatch("a composition cannot be chosen for a problem without variables");
+    }
+    if (box.size() != var_kinds.size()) {
+      throw ShapeMismatch("the box has " + std::to_string(box.size()) + " coordinates for "
+                          + std::to_string(var_kinds.size()) + " variables");
+    }
+    struct Enumerated
+    {
+      std::size_t var;
+      region::SlotKind kind;
+      std::size_t domain;
+      std::size_t cost;
+    };
+    struct Split
+    {
+      std::size_t var;
+      region::SlotKind kind;
+      double width;  // halved by each bisection it receives
+      double start;  // its width when the node began
+      std::size_t bisections;
+    };
+    std::vector<Enumerated> enumerable;
+    std::vector<Split> splittable;
+    std::size_t const slots = max_cycle_size();
+    std::size_t const ceiling = calibration().enumerate_cap;
+    for (std::size_t v = 0; v < var_kinds.size() && enumerable.size() + splittable.size() < slots; ++v) {
+      if (!(box[v].ub > box[v].lb)) {
+        continue;
+      }
+      double const w = box[v].ub - box[v].lb;
+      if (var_kinds[v] == model::VarKind::Binary) {
+        enumerable.push_back({v, region::SlotKind::BinaryEnumerate, 2, 1});
+      } else if (var_kinds[v] == model::VarKind::Integer) {
+        std::size_t const d = GreedyEnumCompositionPolicy::integer_domain_size(box[v]);
+        if (ceiling == 0 || std::max<std::size_t>(d, 1) <= ceiling) {
+          enumerable.push_back({v, region::SlotKind::IntegerEnumerate, d, enumerate_cost(d)});
+        } else {
+          splittable.push_back({v, region::SlotKind::IntegerPartition, w, w, 0});
+        }
+      } else if (var_kinds[v] == model::VarKind::Continuous) {
+        splittable.push_back({v, region::SlotKind::Continuous, w, w, 0});
+      }
+    }
+    region::SlotAssignment out {};
+    if (enumerable.empty() && splittable.empty()) {
+      return out;
+    }
+
+    std::stable_sort(enumerable.begin(), enumerable.end(),
+                     [](const Enumerated& x, const Enumerated& y) { return x.cost < y.cost; });
+    std::size_t budget = budget_;
+    std::size_t taken = 0;
+    while (taken < enumerable.size() && enumerable[taken].cost <= budget) {
+      budget -= enumerable[taken].cost;
+      ++taken;
+    }
+    for (std::size_t i = taken; i < enumerable.size(); ++i) {
+      if (enumerable[i].kind == region::SlotKind::IntegerEnumerate) {
+        double const w = box[enumerable[i].var].ub - box[enumerable[i].var].lb;
+        splittable.push_back({enumerable[i].var, region::SlotKind::IntegerPartition, w, w, 0});
+      }
+    }
+
+    auto key = [&](std::size_t i) {
+      const Split& c = splittable[i];
+      return relative_ ? c.width / c.start : c.width;
+    };
+    auto narrower = [&](std::size_t x, std::size_t y) { return key(x) < key(y); };
+    std::vector<std::size_t> order(splittable.size());
+    for (std::size_t i = 0; i < order.size(); ++i) {
+      order[i] = i;
+    }
+    std::make_heap(order.begin(), order.end(), narrower);
+    std::size_t given = 0;
+    while (given < budget && !order.empty()) {
+      std::pop_heap(order.begin(), order.end(), narrower);
+      Split& widest = splittable[order.back()];
+      widest.width = widest.width / 2.0;
+      widest.bisections += 1;
+      std::push_heap(order.begin(), order.end(), narrower);
+      ++given;
+    }
+    for (; given < budget && taken > 0; ++given) {
+      std::size_t pick = 0;
+      for (std::size_t i = 1; i < taken; ++i) {
+        if (enumerable[i].cost > enumerable[pick].cost) {
+          pick = i;
+        }
+      }
+      enumerable[pick].cost += 1;
+    }
+
+    for (std::size_t i = 0; i < taken; ++i) {
+      out.composition.kinds.push_back(enumerable[i].kind);
+      out.var_ids.push_back(enumerable[i].var);
+      out.widths.push_back(std::size_t(1) << enumerable[i].cost);
+      out.domain_sizes.push_back(enumerable[i].domain);
+    }
+    for (const Split& c : splittable) {
+      if (c.bisections == 0) {
+        continue;
+      }
+      std::size_t const w = std::size_t(1) << c.bisections;
+      out.composition.kinds.push_back(c.kind);
+      out.var_ids.push_back(c.var);
+      out.widths.push_back(w);
+      out.domain_sizes.push_back(w);
+    }
+    return out;
+  }
+
+private:
+  std::size_t budget_;
+  bool relative_;
+};
+
 }  // namespace cmi::policy

@@ -258,7 +258,12 @@ public:
       }
     }
     t_.bounded += bounds.n_regions;
-    bool const may_duplicate = !assignment_.domain_sizes.empty();
+    // Only an enumerate slot padded past its domain (domain size < width) has
+    // duplicate children; without one the per-region check is skipped.
+    bool may_duplicate = false;
+    for (std::size_t j = 0; j < assignment_.domain_sizes.size(); ++j) {
+      may_duplicate = may_duplicate || assignment_.domain_sizes[j] < assignment_.widths[j];
+    }
     std::vector<std::size_t> const prefix =
         may_duplicate ? region::slot_prefixes(assignment_.widths) : std::vector<std::size_t> {};
     std::uint64_t const hash = region::assignment_hash(assignment_);

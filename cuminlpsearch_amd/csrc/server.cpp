@@ -22,7 +22,7 @@
 
 namespace cmi::capi
 {
-std::shared_ptr<cmi::policy::GreedyEnumCompositionPolicy> make_policy(const cmi_solve_options& o);
+std::shared_ptr<const cmi::policy::CompositionPolicy> make_policy(const cmi_solve_options& o);
 }  // namespace cmi::capi
 
 namespace

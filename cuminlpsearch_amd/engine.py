@@ -335,9 +335,13 @@ class Role:
 
 def solve_options(partition_num=2, enumerate_cap=2, max_cycle_size=0, sample_points=1, iter_limit=1000000,
                   tolerance=1e-6, budget_bytes=0, host_budget_bytes=0, compact_frontier=False,
-                  known_primal=None) -> _c.SolveOptions:
+                  known_primal=None, policy=0, bisection_budget=0) -> _c.SolveOptions:
+    """Solve options; policy is capi.POLICY_* (for the bisection-budget
+    policies enumerate_cap is the enumeration ceiling, 0 = none)."""
     o = _c.SolveOptions()
     _c.lib().cmi_solve_options_default(C.byref(o))
+    o.policy = policy
+    o.bisection_budget = bisection_budget
     o.partition_num = partition_num
     o.enumerate_cap = enumerate_cap
     o.max_cycle_size = max_cycle_size

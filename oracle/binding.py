@@ -44,6 +44,8 @@ def lib() -> C.CDLL:
         h.orc_last_error.restype = C.c_char_p
         h.orc_eval_boxes.restype = C.c_int
         h.orc_eval_boxes.argtypes = [P, P, P, C.c_size_t, C.c_size_t, P, P, P, P]
+        h.orc_eval_boxes_ru.restype = C.c_int
+        h.orc_eval_boxes_ru.argtypes = [P, P, P, C.c_size_t, C.c_size_t, P, P, P]
         h.orc_eval_points.restype = C.c_int
         h.orc_eval_points.argtypes = [P, P, C.c_size_t, C.c_size_t, P, P, P]
         h.orc_bound.restype = C.c_int
@@ -101,6 +103,21 @@ def eval_boxes(arrays, lb, ub, roots=False):
     r = np.zeros((1 + arrays.n_constraints, n, 2)) if roots else None
     _ok(lib().orc_eval_boxes(C.byref(d), _p(lb), _p(ub), n, n, _p(feas), _p(olb), _p(oub), _p(r)))
     return feas, olb, oub, r
+
+
+def eval_boxes_ru(arrays, lb, ub):
+    """orc_eval_boxes with the rounding mode set once per call (the CPU
+    baseline's evaluator; QCQP op set only). lb, ub: (n_vars, n) views of one
+    array may be strided: ld is their row stride."""
+    d = arrays.desc()
+    lb = np.ascontiguousarray(lb, np.float64)
+    ub = np.ascontiguousarray(ub, np.float64)
+    nv, n = lb.shape
+    feas = np.zeros(n, np.uint8)
+    olb = np.zeros(n)
+    oub = np.zeros(n)
+    _ok(lib().orc_eval_boxes_ru(C.byref(d), _p(lb), _p(ub), n, n, _p(feas), _p(olb), _p(oub)))
+    return feas, olb, oub
 
 
 def eval_points(arrays, x, roots=False):

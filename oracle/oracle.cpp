@@ -1290,6 +1290,136 @@ int orc_eval_boxes(const cmi_problem_desc* d, const double* lb, const double* ub
   }
 }
 
+// The CPU baseline's evaluator: the same node-by-node evaluation over the
+// QCQP op set (Var, Const, Add, Sub, Mul, Sqr, Neg and their constant-operand
+// forms) with the rounding mode set to round-up once per call instead of once
+// per operation. Lower bounds are carried negated (nl = -lb), so every lower
+// bound is RD(f) = -RU(-f) of an exactly negated expression. Outputs equal
+// orc_eval_boxes bit for bit (tests/test_oracle_cpu_port.py). Returns 2 for a
+// problem outside that op set.
+int orc_eval_boxes_ru(const cmi_problem_desc* d, const double* lb, const double* ub, size_t ld, size_t n,
+                      uint8_t* feasible, double* obj_lb, double* obj_ub)
+{
+  try {
+    Prob const p = load(d);
+    for (std::size_t i = 0; i < p.n_nodes; ++i) {
+      std::uint8_t const op = p.op[i];
+      bool const qcqp = op == CMI_OP_VAR || op == CMI_OP_CONST || op == CMI_OP_ADD || op == CMI_OP_SUB
+          || op == CMI_OP_MUL || op == CMI_OP_SQR || op == CMI_OP_NEG;
+      if (p.reach[i] && !qcqp) {
+        g_err = "the round-up evaluator covers the QCQP op set only";
+        return 2;
+      }
+    }
+    struct NU
+    {
+      double nl, ub;  // (-lb, ub)
+    };
+    // The reachable nodes as a flat list of steps, decided once: which form
+    // of each operation applies (constant operand or not) does not depend on
+    // the box.
+    enum Form : std::uint8_t { kVar, kConst, kAdd, kSub, kNeg, kSqr, kMulC, kMul };
+    struct Step
+    {
+      Form form;
+      std::uint32_t dst, a, b;
+      double c;
+    };
+    std::vector<Step> steps;
+    for (std::size_t i = 0; i < p.n_nodes; ++i) {
+      if (!p.reach[i]) {
+        continue;
+      }
+      std::uint8_t const op = p.op[i];
+      auto const dst = static_cast<std::uint32_t>(i);
+      auto const a = static_cast<std::uint32_t>(p.in0[i]);
+      auto const b = static_cast<std::uint32_t>(p.in1[i]);
+      switch (op) {
+        case CMI_OP_VAR: steps.push_back({kVar, dst, static_cast<std::uint32_t>(p.vidx[i]), 0, 0.0}); break;
+        case CMI_OP_CONST: steps.push_back({kConst, dst, 0, 0, p.cst[i]}); break;
+        case CMI_OP_ADD: steps.push_back({kAdd, dst, a, b, 0.0}); break;
+        case CMI_OP_SUB: steps.push_back({kSub, dst, a, b, 0.0}); break;
+        case CMI_OP_NEG: steps.push_back({kNeg, dst, a, 0, 0.0}); break;
+        case CMI_OP_SQR: steps.push_back({kSqr, dst, a, 0, 0.0}); break;
+        default: {  // Mul
+          bool const ca = p.op[a] == CMI_OP_CONST, cb = p.op[b] == CMI_OP_CONST;
+          if (ca != cb) {
+            steps.push_back({kMulC, dst, ca ? b : a, 0, ca ? p.cst[a] : p.cst[b]});
+          } else {
+            steps.push_back({kMul, dst, a, b, 0.0});
+          }
+        }
+      }
+    }
+    std::vector<NU> v(p.n_nodes);
+    RoundGuard const up(_MM_ROUND_UP);
+    for (std::size_t e = 0; e < n; ++e) {
+      for (const Step& s : steps) {
+        NU const x = v[s.a];
+        NU r {};
+        switch (s.form) {
+          case kVar: r = {-lb[s.a * ld + e], ub[s.a * ld + e]}; break;
+          case kConst: r = {-s.c, s.c}; break;
+          case kAdd: r = {x.nl + v[s.b].nl, x.ub + v[s.b].ub}; break;
+          case kSub: r = {x.nl + v[s.b].ub, x.ub + v[s.b].nl}; break;
+          case kNeg: r = {x.ub, x.nl}; break;
+          case kSqr:
+            if (!(x.nl > 0.0)) {  // lb >= 0
+              r = {x.nl * -x.nl, x.ub * x.ub};
+            } else if (!(x.ub > 0.0)) {  // ub <= 0
+              r = {x.ub * -x.ub, x.nl * x.nl};
+            } else {
+              r = {0.0, std::max(x.nl * x.nl, x.ub * x.ub)};
+            }
+            break;
+          case kMulC:  // the sign of the constant picks the ends
+            if (s.c > 0.0) {
+              r = {x.nl * s.c, x.ub * s.c};
+            } else if (s.c < 0.0) {
+              r = {x.ub * -s.c, x.nl * -s.c};
+            } else {
+              r = {0.0, 0.0};
+            }
+            break;
+          case kMul: {
+            // lb = min of the RD corner products = -(max of RU of their negations)
+            NU const y = v[s.b];
+            double const lx = -x.nl, ly = -y.nl;
+            double nl = x.nl * ly;
+            nl = std::fmax(nl, x.nl * y.ub);
+            nl = std::fmax(nl, x.ub * y.nl);
+            nl = std::fmax(nl, -x.ub * y.ub);
+            double hi = lx * ly;
+            hi = std::fmax(hi, lx * y.ub);
+            hi = std::fmax(hi, x.ub * ly);
+            hi = std::fmax(hi, x.ub * y.ub);
+            r = {nl, hi};
+            break;
+          }
+        }
+        v[s.dst] = r;
+      }
+      bool f = true;
+      for (std::size_t k = 0; k < p.n_con; ++k) {
+        NU const c = v[p.con_root[k]];
+        double const rhs = p.con_rhs[k];
+        bool const below = !(-c.nl > rhs);
+        f = f && (p.con_cmp[k] == CMI_LE ? below : (below && !(rhs > c.ub)));
+      }
+      feasible[e] = f ? 1 : 0;
+      NU const o = v[p.obj];
+      obj_lb[e] = out_lb(-o.nl);
+      if (obj_ub != nullptr) {
+        obj_ub[e] = out_ub(o.ub);
+      }
+    }
+    return 0;
+  } catch (const std::exception& ex) {
+    g_err = ex.what();
+    return 1;
+  }
+}
+
 int orc_eval_points(const cmi_problem_desc* d, const double* xs, size_t ld, size_t n, uint8_t* feasible, double* obj,
                     double* roots)
 {
@@ -1435,6 +1565,16 @@ int orc_solve_opts(const cmi_problem_desc* d, const cmi_solve_options* o, cmi_so
     Prob const p = load(d);
     std::size_t const max_slots = o->max_cycle_size == 0 ? 64 : o->max_cycle_size;
     std::size_t const part = o->partition_num, cap = o->enumerate_cap;
+    bool const bisection = o->policy == CMI_POLICY_BISECTION_BUDGET || o->policy == CMI_POLICY_RELATIVE_BISECTION_BUDGET;
+    if (o->policy < 0 || o->policy > CMI_POLICY_RELATIVE_BISECTION_BUDGET
+        || (bisection && (o->bisection_budget < 1 || o->bisection_budget > 62))) {
+      throw std::runtime_error("oracle: bad composition policy");
+    }
+    auto compose = [&](const std::vector<Iv>& b) {
+      return bisection ? choose_bisection(b, p.kind, o->bisection_budget, max_slots, cap,
+                                          o->policy == CMI_POLICY_RELATIVE_BISECTION_BUDGET)
+                       : choose(b, p.kind, part, cap, max_slots);
+    };
     double const tol = o->tolerance;
     double gub = o->has_known_primal ? o->known_primal : DBL_MAX;
     double glb = -DBL_MAX;
@@ -1449,7 +1589,7 @@ int orc_solve_opts(const cmi_problem_desc* d, const cmi_solve_options* o, cmi_so
     int stop = CMI_STOP_ITERATION_LIMIT;
     bool early = false;
     std::uint64_t fathomed = 0, bounded = 0, sampled = 0, enqueued = 0, p_inf = 0, p_dom = 0;
-    std::uint64_t fathomed_points = 0, rejected = 0;
+    std::uint64_t fathomed_points = 0, rejected = 0, p_dup = 0;
     std::vector<Iv> box;
     while (iter < o->iter_limit && pending.n > 0 && !closed(gub, glb, tol)) {
       ++iter;
@@ -1465,11 +1605,11 @@ int orc_solve_opts(const cmi_problem_desc* d, const cmi_solve_options* o, cmi_so
         box = p.box;
       } else {
         std::vector<Iv> const& parent = hist.box[cur.pidx];
-        Assign const pa = choose(parent, p.kind, part, cap, max_slots);
+        Assign const pa = compose(parent);
         materialise(parent, cur.sidx, pa.vars.data(), pa.widths.data(), pa.kinds.data(), pa.vars.size(), box);
       }
       hist.unref(cur.pidx);
-      Assign const a = choose(box, p.kind, part, cap, max_slots);
+      Assign const a = compose(box);
       std::size_t live = 0;
       for (Iv const& b : box) {
         live += b.ub > b.lb ? 1 : 0;
@@ -1511,6 +1651,11 @@ int orc_solve_opts(const cmi_problem_desc* d, const cmi_solve_options* o, cmi_so
       }
       std::vector<Iv> sub, val;
       for (std::size_t r = 0; r < n_regions; ++r) {
+        if (!a.domains.empty() && duplicate_child(r, a)) {
+          ++p_dup;
+          ++bounded;
+          continue;
+        }
         materialise(box, r, a.vars.data(), a.widths.data(), a.kinds.data(), a.vars.size(), sub);
         bool const ok = eval_interval(p, sub.data(), val);
         ++bounded;
@@ -1571,6 +1716,7 @@ int orc_solve_opts(const cmi_problem_desc* d, const cmi_solve_options* o, cmi_so
     out->viable = viable;
     out->fathomed_points = fathomed_points;
     out->witness_rejected = rejected;
+    out->pruned_duplicate = p_dup;
     out->dropped_viable = dropped.viable;
     out->dropped_dominated = dropped.dominated;
     out->dropped_floor = dropped.floor;

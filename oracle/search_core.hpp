@@ -32,6 +32,7 @@ struct Assign
 {
   std::vector<std::uint8_t> kinds;
   std::vector<std::size_t> vars, widths;
+  std::vector<std::size_t> domains;  // empty for GreedyEnumerate (no duplicate children)
 };
 
 inline std::size_t int_domain(Iv b)
@@ -88,6 +89,121 @@ inline Assign choose(const std::vector<Iv>& box, const std::vector<std::uint8_t>
     push(CMI_SLOT_CONTINUOUS, v, part);
   }
   return a;
+}
+
+// ---- composition (BisectionBudget, policy/bisection_budget.hpp:40-299) -----
+
+inline std::size_t bits_for(std::size_t d)
+{
+  std::size_t const want = d == 0 ? 1 : d;
+  std::size_t b = 0;
+  for (std::size_t cap = 1; cap < want; cap <<= 1) {
+    ++b;
+  }
+  return b < 1 ? 1 : b;
+}
+
+inline Assign choose_bisection(const std::vector<Iv>& box, const std::vector<std::uint8_t>& vk, std::size_t B,
+                               std::size_t max_slots, std::size_t enum_ceiling, bool relative)
+{
+  struct E
+  {
+    std::size_t v;
+    std::uint8_t k;
+    std::size_t d, bits;
+  };
+  struct S
+  {
+    std::size_t v;
+    std::uint8_t k;
+    double live, orig;
+    std::size_t bits;
+  };
+  std::vector<E> en;
+  std::vector<S> bi;
+  for (std::size_t v = 0; v < vk.size() && en.size() + bi.size() < max_slots; ++v) {
+    if (!(box[v].ub > box[v].lb)) {
+      continue;
+    }
+    double const w = box[v].ub - box[v].lb;
+    if (vk[v] == CMI_BINARY) {
+      en.push_back({v, CMI_SLOT_BINARY_ENUMERATE, 2, bits_for(2)});
+    } else if (vk[v] == CMI_INTEGER) {
+      std::size_t const d = int_domain(box[v]);
+      if (enum_ceiling == 0 || std::max<std::size_t>(d, 1) <= enum_ceiling) {
+        en.push_back({v, CMI_SLOT_INTEGER_ENUMERATE, d, bits_for(d)});
+      } else {
+        bi.push_back({v, CMI_SLOT_INTEGER_PARTITION, w, w, 0});
+      }
+    } else {
+      bi.push_back({v, CMI_SLOT_CONTINUOUS, w, w, 0});
+    }
+  }
+  Assign a;
+  if (en.empty() && bi.empty()) {
+    return a;
+  }
+  std::stable_sort(en.begin(), en.end(), [](const E& x, const E& y) { return x.bits < y.bits; });
+  std::size_t left = B, taken = 0;
+  for (; taken < en.size() && en[taken].bits <= left; ++taken) {
+    left -= en[taken].bits;
+  }
+  for (std::size_t i = taken; i < en.size(); ++i) {
+    if (en[i].k == CMI_SLOT_INTEGER_ENUMERATE) {
+      double const w = box[en[i].v].ub - box[en[i].v].lb;
+      bi.push_back({en[i].v, CMI_SLOT_INTEGER_PARTITION, w, w, 0});
+    }
+  }
+  std::vector<std::size_t> h(bi.size());
+  for (std::size_t i = 0; i < h.size(); ++i) {
+    h[i] = i;
+  }
+  auto lt = [&](std::size_t x, std::size_t y) {
+    return relative ? bi[x].live / bi[x].orig < bi[y].live / bi[y].orig : bi[x].live < bi[y].live;
+  };
+  std::make_heap(h.begin(), h.end(), lt);
+  std::size_t spent = 0;
+  for (; spent < left && !h.empty(); ++spent) {
+    std::pop_heap(h.begin(), h.end(), lt);
+    bi[h.back()].live /= 2.0;
+    ++bi[h.back()].bits;
+    std::push_heap(h.begin(), h.end(), lt);
+  }
+  for (; spent < left && taken > 0; ++spent) {
+    std::size_t w = 0;
+    for (std::size_t i = 1; i < taken; ++i) {
+      w = en[i].bits > en[w].bits ? i : w;
+    }
+    ++en[w].bits;
+  }
+  for (std::size_t i = 0; i < taken; ++i) {
+    a.kinds.push_back(en[i].k);
+    a.vars.push_back(en[i].v);
+    a.widths.push_back(std::size_t(1) << en[i].bits);
+    a.domains.push_back(en[i].d);
+  }
+  for (const S& c : bi) {
+    if (c.bits > 0) {
+      a.kinds.push_back(c.k);
+      a.vars.push_back(c.v);
+      a.widths.push_back(std::size_t(1) << c.bits);
+      a.domains.push_back(std::size_t(1) << c.bits);
+    }
+  }
+  return a;
+}
+
+/// Whether child r repeats a sibling: some enumerate digit beyond the domain.
+inline bool duplicate_child(std::size_t r, const Assign& a)
+{
+  std::size_t idx = r;
+  for (std::size_t j = 0; j < a.domains.size(); ++j) {
+    if (idx % a.widths[j] >= a.domains[j]) {
+      return true;
+    }
+    idx /= a.widths[j];
+  }
+  return false;
 }
 
 // ---- pending nodes ----------------------------------------------------------

@@ -159,6 +159,36 @@ struct Engine
   {
     return cmi::policy::GreedyEnumCompositionPolicy::integer_domain_size({b.lb, b.ub});
   }
+
+  struct Bisected
+  {
+    std::vector<int> kinds;
+    std::vector<std::size_t> vars, widths, domains;
+  };
+  static Bisected bisect(std::vector<Box> box, std::vector<int> kinds, std::size_t B, bool relative = false,
+                         std::size_t ceiling = 0)
+  {
+    cmi::policy::SearchCalibration cal;
+    cal.enumerate_cap = ceiling;
+    cmi::policy::BisectionBudgetCompositionPolicy pol(B, cal, relative);
+    std::vector<cmi::interval> b;
+    for (Box v : box) {
+      b.push_back({v.lb, v.ub});
+    }
+    std::vector<cmi::model::VarKind> k;
+    for (int v : kinds) {
+      k.push_back(static_cast<cmi::model::VarKind>(v));
+    }
+    auto const a = pol.choose(b, k);
+    Bisected c;
+    for (auto sk : a.composition) {
+      c.kinds.push_back(static_cast<int>(sk));
+    }
+    c.vars = a.var_ids;
+    c.widths = a.widths;
+    c.domains = a.domain_sizes;
+    return c;
+  }
 };
 
 struct Oracle
@@ -253,6 +283,28 @@ struct Oracle
     return c;
   }
   static std::size_t domain(Box b) { return orc::int_domain({b.lb, b.ub}); }
+
+  struct Bisected
+  {
+    std::vector<int> kinds;
+    std::vector<std::size_t> vars, widths, domains;
+  };
+  static Bisected bisect(std::vector<Box> box, std::vector<int> kinds, std::size_t B, bool relative = false,
+                         std::size_t ceiling = 0)
+  {
+    std::vector<orc::Iv> b;
+    for (Box v : box) {
+      b.push_back({v.lb, v.ub});
+    }
+    std::vector<std::uint8_t> k(kinds.begin(), kinds.end());
+    auto const a = orc::choose_bisection(b, k, B, 64, ceiling, relative);
+    Bisected c;
+    c.kinds.assign(a.kinds.begin(), a.kinds.end());
+    c.vars = a.vars;
+    c.widths = a.widths;
+    c.domains = a.domains;
+    return c;
+  }
 };
 
 constexpr int kCont = CMI_CONTINUOUS, kInt = CMI_INTEGER, kBin = CMI_BINARY;
@@ -807,6 +859,126 @@ void heap_agreement()
   CHECK(same);
 }
 
+// ---- composition_policy_test.cpp: BisectionBudget -------------------------------------
+
+std::size_t product(const std::vector<std::size_t>& w)
+{
+  std::size_t p = 1;
+  for (std::size_t x : w) {
+    p *= x;
+  }
+  return p;
+}
+
+/// Width of variable v after the assignment (unchanged if it has no slot).
+template<typename B>
+double width_after(const B& a, const std::vector<Box>& box, std::size_t v)
+{
+  for (std::size_t j = 0; j < a.vars.size(); ++j) {
+    if (a.vars[j] == v) {
+      return (box[v].ub - box[v].lb) / static_cast<double>(a.widths[j]);
+    }
+  }
+  return box[v].ub - box[v].lb;
+}
+
+template<typename I>
+void bisection_cases()
+{
+  g_case = "a lopsided box spends the whole budget on the wider variable (composition_policy_test.cpp:481)";
+  {
+    auto const a = I::bisect({{0.0, 1e6}, {0.0, 1.0}}, {kCont, kCont}, 10);
+    CHECK(a.vars.size() == 1 && a.vars[0] == 0 && a.widths[0] == 1024);
+  }
+  g_case = "equal-width variables split the budget evenly (:497)";
+  {
+    auto const a = I::bisect({{0.0, 1000.0}, {0.0, 1000.0}}, {kCont, kCont}, 10);
+    CHECK(a.widths == std::vector<std::size_t>({32, 32}));
+  }
+  g_case = "the product of widths equals n_regions for every assignment (:512)";
+  {
+    struct Case
+    {
+      std::vector<Box> box;
+      std::vector<int> kinds;
+    };
+    std::vector<Case> const cases = {
+        {{{0.0, 5.0}}, {kCont}},
+        {{{0.0, 1.0}, {0.0, 1.0}, {0.0, 1.0}}, {kBin, kBin, kBin}},
+        {{{0.0, 100.0}, {0.0, 3.0}, {-5.0, 5.0}}, {kInt, kInt, kCont}},
+        {{{0.0, 16.0}}, {kInt}},
+    };
+    for (const Case& c : cases) {
+      auto const a = I::bisect(c.box, c.kinds, 8);
+      CHECK(a.vars.empty() || product(a.widths) == 256);
+    }
+  }
+  g_case = "the greedy allocation is optimal for the min-max width (:547)";
+  {
+    std::vector<Box> box {{0.0, 100.0}, {0.0, 30.0}, {0.0, 7.0}};
+    double best = 1e300;
+    for (int b0 = 0; b0 <= 6; ++b0) {
+      for (int b1 = 0; b0 + b1 <= 6; ++b1) {
+        int const b2 = 6 - b0 - b1;
+        best = std::min(best, std::max({100.0 / (1 << b0), 30.0 / (1 << b1), 7.0 / (1 << b2)}));
+      }
+    }
+    auto const a = I::bisect(box, {kCont, kCont, kCont}, 6);
+    double worst = 0.0;
+    for (std::size_t v = 0; v < 3; ++v) {
+      worst = std::max(worst, width_after(a, box, v));
+    }
+    CHECK(feq(worst, best));
+  }
+  g_case = "the allocation does not depend on the order variables appear in (:580)";
+  {
+    std::vector<Box> ba {{0.0, 1000.0}, {0.0, 10.0}}, bb {{0.0, 10.0}, {0.0, 1000.0}};
+    auto const a = I::bisect(ba, {kCont, kCont}, 8);
+    auto const b = I::bisect(bb, {kCont, kCont}, 8);
+    CHECK(feq(width_after(a, ba, 0), width_after(b, bb, 1)) && feq(width_after(a, ba, 1), width_after(b, bb, 0)));
+  }
+  g_case = "an integer domain too wide for the budget partitions instead (:601)";
+  {
+    auto const a = I::bisect({{0.0, 1048575.0}}, {kInt}, 5);
+    CHECK(a.kinds.size() == 1 && a.kinds[0] == sP && a.widths[0] == 32);
+  }
+  g_case = "the cheap integer enumerates and the wide one partitions (:615)";
+  {
+    auto const a = I::bisect({{0.0, 3.0}, {0.0, 1048575.0}}, {kInt, kInt}, 5);
+    CHECK(a.kinds.size() == 2 && product(a.widths) == 32);
+    for (std::size_t j = 0; j < a.vars.size(); ++j) {
+      if (a.vars[j] == 0) {
+        CHECK(a.kinds[j] == sE && a.widths[j] == 4 && a.domains[j] == 4);
+      } else {
+        CHECK(a.kinds[j] == sP);
+      }
+    }
+  }
+  g_case = "a domain of 5 enumerates over 8 children, 3 of them duplicates (:640)";
+  {
+    auto const a = I::bisect({{0.0, 4.0}}, {kInt}, 3);
+    CHECK(a.kinds.size() == 1 && a.widths[0] == 8 && a.domains[0] == 5);
+  }
+  g_case = "leftover budget with no bisecting pool pads the widest enumerate slot (:672)";
+  {
+    auto const a = I::bisect({{0.0, 1.0}, {0.0, 1.0}}, {kBin, kBin}, 5);
+    CHECK(a.kinds.size() == 2 && product(a.widths) == 32);
+    CHECK(a.kinds == std::vector<int>({sB, sB}));
+    CHECK(a.widths == std::vector<std::size_t>({16, 2}));  // the first of two equal slots takes the padding
+  }
+  g_case = "relative width: equal relative shares whatever the absolute widths";
+  {
+    auto const a = I::bisect({{0.0, 1e6}, {0.0, 1.0}}, {kCont, kCont}, 10, true);
+    CHECK(a.widths == std::vector<std::size_t>({32, 32}));
+  }
+  g_case = "an enumeration ceiling turns a wide integer into a partition";
+  {
+    auto const a = I::bisect({{0.0, 40.0}, {0.0, 3.0}}, {kInt, kInt}, 8, false, 8);
+    CHECK(a.kinds.size() == 2 && a.vars[0] == 1 && a.kinds[0] == sE && a.kinds[1] == sP);
+    CHECK(product(a.widths) == 256);
+  }
+}
+
 template<typename I>
 void run_all()
 {
@@ -818,6 +990,7 @@ void run_all()
   budget_cases<I>();
   witness_cases<I>();
   policy_cases<I>();
+  bisection_cases<I>();
   order_cases<I>();
 }
 
@@ -958,6 +1131,48 @@ void engine_only()
     bad.var_ids = {0, 1};
     std::vector<cmi::interval> box {{0.0, 1.0}, {5.0, 5.0}};
     CHECK(!cmi::policy::assignment_is_distinct_and_live(bad, box));
+  }
+  g_case = "BisectionBudget rejects B outside [1, 62] (composition_policy_test.cpp:466)";
+  {
+    int thrown = 0;
+    for (std::size_t b : {std::size_t(0), std::size_t(63)}) {
+      try {
+        cmi::policy::BisectionBudgetCompositionPolicy pol {b};
+      } catch (const cmi::InvalidConfiguration&) {
+        ++thrown;
+      }
+    }
+    CHECK(thrown == 2);
+    cmi::policy::BisectionBudgetCompositionPolicy one {1}, top {62};
+    CHECK(one.bisection_budget() == 1 && top.bisection_budget() == 62);
+  }
+  g_case = "n_regions is 2^B whatever the composition, except the empty one (:473)";
+  {
+    cmi::policy::BisectionBudgetCompositionPolicy pol {10};
+    CHECK(pol.n_regions(Composition {{SlotKind::Continuous}}) == 1024);
+    CHECK(pol.n_regions(Composition {{SlotKind::Continuous, SlotKind::Continuous}}) == 1024);
+    CHECK(pol.n_regions(Composition {}) == 1);
+  }
+  g_case = "duplicate children are flagged and decode like their top sibling (:640)";
+  {
+    cmi::policy::BisectionBudgetCompositionPolicy pol {3};
+    std::vector<cmi::interval> box {{0.0, 4.0}};
+    std::vector<cmi::model::VarKind> kinds {cmi::model::VarKind::Integer};
+    auto const a = pol.choose(box, kinds);
+    CHECK(a.widths.size() == 1 && a.widths[0] == 8 && a.domain_sizes[0] == 5);
+    auto const prefix = cmi::region::slot_prefixes(a.widths);
+    std::vector<cmi::interval> top, child;
+    cmi::region::materialise(box, 4, a, top);
+    bool ok = true;
+    for (std::size_t r = 0; r < 8; ++r) {
+      bool const dup = r >= 5;
+      ok = ok && cmi::region::is_duplicate_child(r, a, prefix) == dup;
+      if (dup) {
+        cmi::region::materialise(box, r, a, child);
+        ok = ok && feq(child[0].lb, top[0].lb) && feq(child[0].ub, top[0].ub);
+      }
+    }
+    CHECK(ok);
   }
   g_case = "materialise rejects a policy that is not a pure function of the box (frontier.hpp:113-144)";
   {
