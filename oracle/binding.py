@@ -161,9 +161,12 @@ def candidate(arrays, box, var_ids, widths, kinds, samples, salt):
 
 
 def solve(arrays, partition_num=2, enumerate_cap=2, max_cycle_size=0, sample_points=1, iter_limit=1000000,
-          tolerance=1e-6, host_budget_bytes=0, compact_frontier=False, known_primal=None):
+          tolerance=1e-6, host_budget_bytes=0, compact_frontier=False, known_primal=None, policy=0,
+          bisection_budget=0):
     """The oracle's best-first search with the engine's solve options. A host
-    budget of 0 means none here (the engine reads 0 as half of MemAvailable)."""
+    budget of 0 means none here (the engine reads 0 as half of MemAvailable).
+    `policy` is a cmi_policy value; `bisection_budget` is the B of the
+    bisection-budget policies."""
     from cuminlpsearch_amd import capi as _c
 
     d = arrays.desc()
@@ -176,6 +179,8 @@ def solve(arrays, partition_num=2, enumerate_cap=2, max_cycle_size=0, sample_poi
     o.tolerance = tolerance
     o.host_budget_bytes = host_budget_bytes
     o.compact_frontier = int(compact_frontier)
+    o.policy = policy
+    o.bisection_budget = bisection_budget
     if known_primal is not None:
         o.has_known_primal = 1
         o.known_primal = known_primal

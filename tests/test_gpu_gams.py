@@ -167,3 +167,54 @@ def test_autocorr_all_binary_run_record(ctx):
     same_outcome(d, o)
     assert d.iterations == 1 and d.proven_optimal
     assert d.upper_bound == -72.0 and d.lower_bound == -72.0
+
+
+def test_ex4_1_2_mixed_all_small_run_record(ctx):
+    """MINLP_STATUS.md:543: --policy=mixed-all-small converged ex4_1_2 with
+    primal = dual = -663.50009661 at record 409. The policy fits one
+    variable's partition up to a ceiling that, when the record was made, sat
+    at 64 (include/cuminlp/config/catalogue.hpp:74-85 tells of it being raised
+    to 2,000,000); this one-variable model reaches the ceiling on any device,
+    so the record's shape is partition 64, enumerate cap 1, 10 samples, one
+    slot. Under that shape the search converges at iteration 410 (the record
+    plus the dequeue that finds the gap closed) on the device and in the
+    oracle alike. Under today's ceiling it reaches the same bracket."""
+    from cuminlpsearch_amd import resolve as R
+    pm = parsed("ex4_1_2")
+    shape = R.resolve_shape(pm.problem, "mixed-all-small", free_device_bytes=8 << 30)
+    opts = shape.solve_options()
+    assert opts == dict(partition_num=2_000_000, enumerate_cap=1, sample_points=10, max_cycle_size=1)
+    record = dict(opts, partition_num=64)
+    d, best_d = cmi.solve(ctx, pm.problem, iter_limit=1000, **record)
+    o, best_o = orc.solve(pm.problem.arrays(), iter_limit=1000, **record)
+    same_outcome(d, o)
+    assert np.array_equal(best_d.view(np.uint64), best_o.view(np.uint64))
+    assert d.proven_optimal and d.iterations == 410
+    assert f"{d.upper_bound:.11g}" == "-663.50009661" and f"{d.lower_bound:.11g}" == "-663.50009661"
+    now, _ = cmi.solve(ctx, pm.problem, iter_limit=1000, **opts)
+    print(f"partition 2,000,000: iterations {now.iterations} primal {now.upper_bound!r}", flush=True)
+    assert now.proven_optimal
+    assert f"{now.upper_bound:.11g}" == "-663.50009661" and f"{now.lower_bound:.11g}" == "-663.50009661"
+
+
+def test_nvs04_discrete_run_record(ctx):
+    """MINLP_STATUS.md:1021: --policy=discrete converged nvs04
+    with primal = dual = 0.72 at record 6. Today the policy's
+    coverage fit gives a partition of about 20,000 on a large device, which
+    does not converge within 1,000 iterations; when the record was made the
+    fit's ceiling was 64 (include/cuminlp/config/catalogue.hpp:74-85), and
+    under that shape (enumerate cap 16, 5 samples, 2 slots) the frontier
+    empties after iteration 6 on the device and in the oracle alike."""
+    from cuminlpsearch_amd import resolve as R
+    pm = parsed("nvs04")
+    shape = R.resolve_shape(pm.problem, "discrete", free_device_bytes=8 << 30)
+    opts = shape.solve_options()
+    assert (opts["enumerate_cap"], opts["sample_points"], opts["max_cycle_size"]) == (16, 5, 2)
+    assert opts["partition_num"] > 64
+    record = dict(opts, partition_num=64)
+    d, best_d = cmi.solve(ctx, pm.problem, iter_limit=1000, **record)
+    o, best_o = orc.solve(pm.problem.arrays(), iter_limit=1000, **record)
+    same_outcome(d, o)
+    assert np.array_equal(best_d.view(np.uint64), best_o.view(np.uint64))
+    assert d.proven_optimal and d.iterations == 6
+    assert f"{d.upper_bound:.12g}" == "0.72" and f"{d.lower_bound:.12g}" == "0.72"
