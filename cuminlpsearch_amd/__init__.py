@@ -5,11 +5,11 @@ The engine (HIP kernels for gfx950 + C++ search host + C-ABI) is
 fallback: ``capi.lib()`` raises if the library has not been built.
 """
 from . import capi
-from .engine import (AggregateBounder, Candidate, Context, EngineProblem, Role, Tape, qcqp_spec, serve_plain, serve_stream, solve,
-                     solve_batch, solve_options)
+from .engine import (AggregateBounder, Candidate, Context, EngineProblem, Role, Tape, aggregate_fit, aggregate_solve,
+                     qcqp_spec, serve_plain, serve_stream, solve, solve_batch, solve_options)
 from .model import Expr, Problem, ProblemArrays, evaluate_point, satisfies, sqr
 
 __all__ = [
-    "capi", "Candidate", "Context", "EngineProblem", "Role", "Tape", "qcqp_spec", "serve_plain", "serve_stream",
+    "capi", "aggregate_fit", "aggregate_solve", "Candidate", "Context", "EngineProblem", "Role", "Tape", "qcqp_spec", "serve_plain", "serve_stream",
     "solve", "solve_batch", "solve_options", "Expr", "Problem", "ProblemArrays", "evaluate_point", "satisfies", "sqr",
 ]

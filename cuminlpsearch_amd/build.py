@@ -42,7 +42,7 @@ HIP_FLAGS = [
 ]
 
 ENGINE_SOURCES = ["kernels.hip", "hip_backend.cpp", "batch.cpp", "capi.cpp", "server.cpp", "jit.cpp",
-                  "aggregate_backend.cpp", "gams.cpp"]
+                  "aggregate_backend.cpp", "aggregate_solve.cpp", "gams.cpp"]
 # interval_ops.hpp is also the prelude of every kernel compiled at run time
 # from a tape; its text is embedded into the library.
 EMBEDDED_TEXT = {"cmi_interval_ops_source": "interval_ops.hpp"}

@@ -38,6 +38,7 @@ LE, EQ = range(2)
 SLOT_CONTINUOUS, SLOT_INTEGER_PARTITION, SLOT_INTEGER_ENUMERATE, SLOT_BINARY_ENUMERATE = range(4)
 ROLE_SAMPLER, ROLE_BOUNDER, ROLE_ENUMERATOR = range(3)
 POLICY_GREEDY_ENUMERATE, POLICY_BISECTION_BUDGET, POLICY_RELATIVE_BISECTION_BUDGET = range(3)
+SELECT_LEAST_LOWER_BOUND, SELECT_REJECT_INDEX, SELECT_REJECT_INDEX_STALE = range(3)
 STOP_NAMES = ["converged", "exhausted", "iteration-limit", "host-memory", "allocation-failure", "device-memory"]
 
 
@@ -156,6 +157,15 @@ class SolveOptions(C.Structure):
     ]
 
 
+class AggregateOptions(C.Structure):
+    _fields_ = [
+        ("branch_fan_out", C.c_size_t), ("partition_budget", C.c_size_t), ("sample_fan_out", C.c_size_t),
+        ("sample_points", C.c_size_t), ("iter_limit", C.c_uint32), ("tolerance", C.c_double), ("selection", C.c_int),
+        ("relative_width", C.c_int), ("budget_bytes", C.c_size_t), ("host_budget_bytes", C.c_size_t),
+        ("has_known_primal", C.c_int), ("known_primal", C.c_double),
+    ]
+
+
 class SolveOutcome(C.Structure):
     _fields_ = [
         ("lower_bound", C.c_double), ("upper_bound", C.c_double), ("stop_reason", C.c_int),
@@ -268,6 +278,11 @@ _SIGNATURES = {
     "cmi_role_destroy": (None, [_P]),
     "cmi_solve_options_default": (None, [C.POINTER(SolveOptions)]),
     "cmi_solve": (C.c_int, [_P, _P, C.POINTER(SolveOptions), C.POINTER(SolveOutcome), _P]),
+    "cmi_aggregate_options_default": (None, [C.POINTER(AggregateOptions)]),
+    "cmi_aggregate_fit": (C.c_int, [_P, C.c_size_t, C.c_size_t, C.c_size_t, C.POINTER(C.c_size_t),
+                                    C.POINTER(C.c_size_t)]),
+    "cmi_aggregate_solve": (C.c_int, [_P, _P, C.POINTER(AggregateOptions), C.POINTER(SolveOutcome), _P,
+                                      C.POINTER(C.c_int)]),
     "cmi_solve_batch": (C.c_int, [_P, C.c_size_t, C.POINTER(_P), C.POINTER(SolveOptions), C.POINTER(SolveOutcome),
                                   C.POINTER(_P)]),
     "cmi_serve_stream": (C.c_int, [_P, C.c_size_t, C.POINTER(_P), _P, C.POINTER(SolveOptions),

@@ -13,6 +13,7 @@
 #include <string>
 #include <vector>
 
+#include "aggregate_solve.hpp"
 #include "batch.hpp"
 #include "jit.hpp"
 #include "cmi/config.hpp"
@@ -1066,6 +1067,50 @@ cmi_status cmi_solve(cmi_ctx* ctx, const cmi_problem* p, const cmi_solve_options
     auto const res = drv.solve(p->problem);
     double const secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     fill_outcome(res, secs, out, best_point, p->problem.box_bounds.size());
+  });
+}
+
+void cmi_aggregate_options_default(cmi_aggregate_options* o)
+{
+  if (o == nullptr) {
+    return;
+  }
+  std::memset(o, 0, sizeof(*o));
+  o->branch_fan_out = 4;
+  o->partition_budget = 1024;
+  o->sample_fan_out = 8;
+  o->sample_points = 1;
+  o->iter_limit = 1000000;
+  o->tolerance = 1e-6;
+  o->selection = CMI_SELECT_LEAST_LOWER_BOUND;
+}
+
+cmi_status cmi_aggregate_fit(const cmi_problem* p, size_t free_device_bytes, size_t branch_fan_out,
+                             size_t sample_points, size_t* partition_budget, size_t* sample_fan_out)
+{
+  if (p == nullptr || partition_budget == nullptr || sample_fan_out == nullptr) {
+    return fail(CMI_INVALID_HANDLE, "null argument");
+  }
+  return guarded([&] {
+    cmi::hip::aggregate_fit(cmi::qcqp::reachable_nodes(p->problem), free_device_bytes, branch_fan_out, sample_points,
+                            *partition_budget, *sample_fan_out);
+  });
+}
+
+cmi_status cmi_aggregate_solve(cmi_ctx* ctx, const cmi_problem* p, const cmi_aggregate_options* o,
+                               cmi_solve_outcome* out, double* best_point, int* rule_engaged)
+{
+  if (ctx == nullptr || p == nullptr || o == nullptr || out == nullptr) {
+    return fail(CMI_INVALID_HANDLE, "null argument");
+  }
+  return guarded([&] {
+    auto const t0 = std::chrono::steady_clock::now();
+    auto const run = cmi::hip::aggregate_solve(ctx->ctx, p->problem, *o);
+    double const secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    fill_outcome(run.outcome, secs, out, best_point, p->problem.box_bounds.size());
+    if (rule_engaged != nullptr) {
+      *rule_engaged = run.rule_engaged ? 1 : 0;
+    }
   });
 }
 

@@ -357,6 +357,48 @@ def solve_options(partition_num=2, enumerate_cap=2, max_cycle_size=0, sample_poi
     return o
 
 
+def aggregate_options(branch_fan_out=4, partition_budget=1024, sample_fan_out=8, sample_points=1,
+                      iter_limit=1000000, tolerance=1e-6, selection=0, relative_width=False, budget_bytes=0,
+                      host_budget_bytes=0, known_primal=None) -> _c.AggregateOptions:
+    """Settings of the k-child search; selection is capi.SELECT_*."""
+    o = _c.AggregateOptions()
+    _c.lib().cmi_aggregate_options_default(C.byref(o))
+    o.branch_fan_out = branch_fan_out
+    o.partition_budget = partition_budget
+    o.sample_fan_out = sample_fan_out
+    o.sample_points = sample_points
+    o.iter_limit = iter_limit
+    o.tolerance = tolerance
+    o.selection = selection
+    o.relative_width = int(relative_width)
+    o.budget_bytes = budget_bytes
+    o.host_budget_bytes = host_budget_bytes
+    if known_primal is not None:
+        o.has_known_primal = 1
+        o.known_primal = known_primal
+    return o
+
+
+def aggregate_fit(problem: EngineProblem, free_device_bytes: int, branch_fan_out=4, sample_points=1):
+    """The reference's device fit: (partition_budget, sample_fan_out)."""
+    n, s = C.c_size_t(), C.c_size_t()
+    _c.check(_c.lib().cmi_aggregate_fit(problem.handle, free_device_bytes, branch_fan_out, sample_points,
+                                        C.byref(n), C.byref(s)))
+    return n.value, s.value
+
+
+def aggregate_solve(ctx: Context, problem: EngineProblem, **opts):
+    """The k-child search: (outcome, best point, whether the selection rule
+    ever left its lb fallback)."""
+    o = aggregate_options(**opts)
+    out = _c.SolveOutcome()
+    best = np.full(problem.info().n_vars, np.nan)
+    engaged = C.c_int(0)
+    _c.check(_c.lib().cmi_aggregate_solve(ctx.handle, problem.handle, C.byref(o), C.byref(out), _c.ptr(best),
+                                          C.byref(engaged)))
+    return out, best, bool(engaged.value)
+
+
 def solve(ctx: Context, problem: EngineProblem, **opts):
     o = solve_options(**opts)
     out = _c.SolveOutcome()

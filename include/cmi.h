@@ -525,6 +525,64 @@ cmi_status cmi_aggregate_build(cmi_tape* t, const uint8_t* slot_kinds, size_t sl
 cmi_status cmi_bound_children(cmi_role* r, size_t n_batch, const cmi_ivl* boxes, size_t n_vars, const size_t* var_ids,
                               const size_t* widths, size_t slot_count, cmi_aggregate_bound* out);
 
+/* -- the k-child (aggregate) search --------------------------------------------- */
+
+/* SelectionRule, reference aggregate/selection.hpp:15-125: which pending
+ * child is explored next. Every rule orders by lb until an incumbent exists. */
+typedef enum
+{
+  CMI_SELECT_LEAST_LOWER_BOUND = 0,
+  CMI_SELECT_REJECT_INDEX = 1,       /* max (f_best - lb) / (hull_ub - lb), re-keyed on every new incumbent */
+  CMI_SELECT_REJECT_INDEX_STALE = 2, /* the same key, fixed at enqueue time */
+} cmi_selection_rule;
+
+/* The aggregate search's settings (reference source/gams/aggregate_solve.cu:
+ * 133-143 flags, 298-330 shape). */
+typedef struct
+{
+  size_t branch_fan_out;    /* k: children per node, a power of two >= 2 (--branch-factor, default 4) */
+  size_t partition_budget;  /* N: subregions bounded per node, a power of two > k (--partition-budget) */
+  size_t sample_fan_out;    /* subregions sampled per node, a power of two > k */
+  size_t sample_points;     /* points drawn per sampled subregion, >= 1 (default 1) */
+  uint32_t iter_limit;
+  double tolerance;         /* gap closed when gub - bound <= tolerance * max(1, |gub|) */
+  int selection;            /* cmi_selection_rule */
+  int relative_width;       /* the refine's bisections by width relative to the node's own */
+  size_t budget_bytes;      /* device bytes for both roles, 0 = free memory; the sampler may use a tenth */
+  size_t host_budget_bytes; /* 0 = half of MemAvailable; on reaching it the search stops (never compacts) */
+  int has_known_primal;
+  double known_primal;      /* a sound upper bound on the minimum, seeding the incumbent value */
+} cmi_aggregate_options;
+
+/* Defaults: k 4, N 1024, sample fan-out 8, 1 sample, iter_limit 1000000,
+ * tolerance 1e-6, least-lower-bound. */
+void cmi_aggregate_options_default(cmi_aggregate_options* o);
+
+/* The reference's device fit of N and the sample fan-out
+ * (source/gams/aggregate_solve.cu:298-315, aggregate/cost.hpp:22-99): a
+ * device budget of 0.8 * free_device_bytes split 9:1 between bounder and
+ * sampler, N the largest power of two whose per-subregion cost
+ * (16 * buffer nodes + 33 bytes) fits the bounder's share, the sample
+ * fan-out the largest power of two whose (8 * buffer nodes + 25) *
+ * sample_points bytes fit the sampler's, at least max(2, 2k). The device
+ * search itself holds no per-subregion buffer, so these sizes are the
+ * reference's choice, not this engine's memory limit. */
+cmi_status cmi_aggregate_fit(const cmi_problem* p, size_t free_device_bytes, size_t branch_fan_out,
+                             size_t sample_points, size_t* partition_budget, size_t* sample_fan_out);
+
+/* AggregateDriver::solve, reference aggregate/driver.hpp:98-462: best-first
+ * branch and bound in which each node is split k ways on its widest live
+ * variable and the device bounds each child over N / k subregions in one
+ * launch (the least lower bound and the greatest upper bound over those not
+ * proven infeasible), a child's bound clamped to its parent's; the sampler
+ * draws sample_points points in each of sample_fan_out subregions of the
+ * node. A node with no live variable is a point and is evaluated on the host.
+ * `out` counts children in `bounded`; `pending`/`viable` are children.
+ * *rule_engaged (optional) is 0 only when a re-keying rule never had a
+ * sampled incumbent, so it explored in lb order throughout. */
+cmi_status cmi_aggregate_solve(cmi_ctx* ctx, const cmi_problem* p, const cmi_aggregate_options* o,
+                               cmi_solve_outcome* out, double* best_point, int* rule_engaged);
+
 /* -- GAMS scalar-format models (host) ----------------------------------------- */
 
 typedef struct cmi_gams_model cmi_gams_model;

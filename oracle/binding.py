@@ -71,6 +71,8 @@ def lib() -> C.CDLL:
         h.orc_solve.argtypes = [P, C.c_size_t, C.c_size_t, C.c_size_t, C.c_size_t, C.c_uint32, d, P, P]
         h.orc_solve_opts.restype = C.c_int
         h.orc_solve_opts.argtypes = [P, P, P, P]
+        h.orc_aggregate_solve.restype = C.c_int
+        h.orc_aggregate_solve.argtypes = [P, P, P, P, P]
         _lib = h
     return _lib
 
@@ -188,6 +190,20 @@ def solve(arrays, partition_num=2, enumerate_cap=2, max_cycle_size=0, sample_poi
     best = np.zeros(arrays.n_vars)
     _ok(lib().orc_solve_opts(C.byref(d), C.byref(o), C.byref(out), _p(best)))
     return out, best
+
+
+def aggregate_solve(arrays, **opts):
+    """The oracle's k-child search with the engine's aggregate options:
+    (outcome, best point, whether the selection rule left its lb fallback)."""
+    from cuminlpsearch_amd import engine as _e
+    o = _e.aggregate_options(**opts)
+    from cuminlpsearch_amd import capi as _c
+    d = arrays.desc()
+    out = _c.SolveOutcome()
+    best = np.zeros(arrays.n_vars)
+    engaged = C.c_int(0)
+    _ok(lib().orc_aggregate_solve(C.byref(d), C.byref(o), C.byref(out), _p(best), C.byref(engaged)))
+    return out, best, bool(engaged.value)
 
 
 def _vec(fn, a, b):

@@ -1734,6 +1734,176 @@ int orc_solve_opts(const cmi_problem_desc* d, const cmi_solve_options* o, cmi_so
   }
 }
 
+// The k-child search, restated (aggregate/driver.hpp:176-462): per node the
+// sampler over the sampling partition, the k child bounds over the bound
+// partition's N subregions (each child's least lb and greatest ub over its
+// subregions not proven infeasible), each child's lb clamped to its
+// parent's, a child above its own hull or infeasible pruned as infeasible.
+int orc_aggregate_solve(const cmi_problem_desc* d, const cmi_aggregate_options* o, cmi_solve_outcome* out,
+                        double* best_point, int* rule_engaged)
+{
+  try {
+    Prob const p = load(d);
+    std::size_t const k = o->branch_fan_out, n = o->partition_budget, ns = o->sample_fan_out;
+    auto pow2 = [](std::size_t v) { return v != 0 && (v & (v - 1)) == 0; };
+    if (k < 2 || !pow2(k) || n <= k || !pow2(n) || ns <= k || !pow2(ns) || o->sample_points < 1
+        || o->selection < 0 || o->selection > CMI_SELECT_REJECT_INDEX_STALE) {
+      throw std::runtime_error("oracle: bad aggregate options");
+    }
+    bool const rel = o->relative_width != 0;
+    double const tol = o->tolerance;
+    double gub = o->has_known_primal ? o->known_primal : DBL_MAX;
+    double glb = -DBL_MAX;
+    std::vector<double> best;
+    AggFront front;
+    front.rule = o->selection;
+    History hist;
+    hist.put({});
+    front.push({0, 0, 0, -DBL_MAX, DBL_MAX, 0}, gub);
+    std::uint32_t iter = 0;
+    bool converged = false, early = false, engaged = false;
+    int stop = CMI_STOP_ITERATION_LIMIT;
+    std::uint64_t fathomed = 0, bounded = 0, sampled = 0, enqueued = 0, p_inf = 0, p_dom = 0, rejected = 0;
+    auto take = [&](double v, const std::vector<double>& x) {
+      gub = v;
+      best = x;
+      front.incumbent(gub);
+    };
+    std::vector<Iv> box, sub, val;
+    std::vector<double> pv;
+    double const inf = std::numeric_limits<double>::infinity();
+    while (iter < o->iter_limit && front.live > 0) {
+      double const least = front.min_lb();
+      glb = least;
+      if (closed(gub, least, tol)) {
+        converged = true;
+        early = true;
+        stop = CMI_STOP_CONVERGED;
+        break;
+      }
+      ++iter;
+      AggNode const cur = front.pop();
+      if (cur.pidx == 0) {
+        box = p.box;
+      } else {
+        std::vector<Iv> const& parent = hist.box[cur.pidx];
+        Assign const br = widest_split(parent, p.kind, k);
+        materialise(parent, cur.sidx, br.vars.data(), br.widths.data(), br.kinds.data(), br.vars.size(), box);
+      }
+      hist.unref(cur.pidx);
+      AggPart const part = agg_partition(box, p.kind, k, n, rel);
+      if (part.leaf()) {
+        ++fathomed;
+        std::vector<double> x(box.size());
+        for (std::size_t i = 0; i < box.size(); ++i) {
+          x[i] = box[i].lb;
+        }
+        eval_point(p, x.data(), pv);
+        if (feasible_point(p, pv) && pv[p.obj] < gub) {
+          if (admissible(x, p.kind, p.box)) {
+            take(pv[p.obj], x);
+          } else {
+            ++rejected;
+          }
+        }
+        continue;
+      }
+      AggPart const sp = agg_partition(box, p.kind, k, ns, rel);
+      std::size_t const box_idx = hist.put(box);
+      Cand const c = points_argmin(p, box, sp.launch.vars.data(), sp.launch.widths.data(), sp.launch.kinds.data(),
+                                   sp.launch.vars.size(), ns, o->sample_points, iter);
+      sampled += ns * o->sample_points;
+      if (c.found && c.value < gub) {
+        if (admissible(c.point, p.kind, p.box)) {
+          take(c.value, c.point);
+          engaged = true;
+        } else {
+          ++rejected;
+        }
+      }
+      std::size_t const m = n / k;
+      std::vector<double> lo(k, inf), hi(k, -inf);
+      for (std::size_t r = 0; r < n; ++r) {
+        materialise(box, r, part.launch.vars.data(), part.launch.widths.data(), part.launch.kinds.data(),
+                    part.launch.vars.size(), sub);
+        bool const ok = eval_interval(p, sub.data(), val);
+        if (!(ok && feasible_interval(p, val))) {
+          continue;
+        }
+        std::size_t const ch = r / m;
+        lo[ch] = std::min(lo[ch], out_lb(val[p.obj].lb));
+        hi[ch] = std::max(hi[ch], out_ub(val[p.obj].ub));
+      }
+      bounded += k;
+      std::uint64_t const hash = 0;
+      for (std::size_t ch = 0; ch < k; ++ch) {
+        if (!(hi[ch] > -inf)) {
+          ++p_inf;
+          continue;
+        }
+        double const lb = std::max(lo[ch], cur.lb);
+        if (lb > hi[ch]) {
+          ++p_inf;
+          continue;
+        }
+        if (lb > gub) {
+          ++p_dom;
+          continue;
+        }
+        front.push({ch, box_idx, cur.depth + 1, lb, hi[ch], hash}, gub);
+        hist.ref(box_idx);
+        ++enqueued;
+      }
+      hist.unref(box_idx);
+      if (over_budget(front.cap_bytes() + hist.bytes, o->host_budget_bytes)) {
+        stop = CMI_STOP_HOST_MEMORY;
+        early = true;
+        break;
+      }
+    }
+    if (!early) {
+      stop = front.live == 0 ? CMI_STOP_EXHAUSTED : CMI_STOP_ITERATION_LIMIT;
+    }
+    bool const found = gub < DBL_MAX;
+    std::size_t const viable = front.viable(gub);
+    double const least = front.live == 0 ? DBL_MAX : front.min_lb();
+    Final const fin = finalise(glb, gub, tol, found, converged, front.live == 0, least, viable, Dropped {});
+    std::memset(out, 0, sizeof(*out));
+    out->lower_bound = fin.glb;
+    out->upper_bound = gub;
+    out->stop_reason = stop;
+    out->proven_optimal = fin.proven ? 1 : 0;
+    out->infeasible = fin.infeasible ? 1 : 0;
+    out->found_incumbent = found ? 1 : 0;
+    out->iterations = iter;
+    out->fathomed = fathomed;
+    out->fathomed_points = fathomed;
+    out->bounded = bounded;
+    out->sampled = sampled;
+    out->enqueued = enqueued;
+    out->pruned_infeasible = p_inf;
+    out->pruned_dominated = p_dom;
+    out->witness_rejected = rejected;
+    out->pending = front.live;
+    out->viable = viable;
+    out->dropped_floor = INFINITY;
+    out->history_freed = hist.freed;
+    out->history_peak_live = hist.peak;
+    if (best_point != nullptr) {
+      for (std::size_t v = 0; v < p.n_vars; ++v) {
+        best_point[v] = v < best.size() ? best[v] : NAN;
+      }
+    }
+    if (rule_engaged != nullptr) {
+      *rule_engaged = engaged || o->selection != CMI_SELECT_REJECT_INDEX;
+    }
+    return 0;
+  } catch (const std::exception& ex) {
+    g_err = ex.what();
+    return 1;
+  }
+}
+
 int orc_solve(const cmi_problem_desc* d, size_t partition_num, size_t enumerate_cap, size_t max_cycle_size,
               size_t sample_points, uint32_t iter_limit, double tol, cmi_solve_outcome* out, double* best_point)
 {

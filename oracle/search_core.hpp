@@ -6,7 +6,10 @@
 // Restated from the reference: policy/greedy_enum.hpp:54-186 (composition),
 // search/frontier.hpp:66-338 (node order, heap, compaction),
 // search/history.hpp:20-137 (parent boxes), search/budget.hpp:60-215 (host
-// budget), search/epilogue.hpp:35-127 (incumbent admission, final claims).
+// budget), search/epilogue.hpp:35-127 (incumbent admission, final claims),
+// and for the k-child search aggregate/policy.hpp:106-330 (partition),
+// aggregate/selection.hpp:15-125 (keys) and aggregate/frontier.hpp:106-394
+// (the twice-ordered frontier).
 #pragma once
 
 #include <algorithm>
@@ -440,5 +443,236 @@ inline bool admissible(const std::vector<double>& x, const std::vector<std::uint
   }
   return true;
 }
+
+// ---- the k-child search --------------------------------------------------------
+
+/// One node's k-child split: `branch` one slot (k ways on the widest live
+/// variable, uniform), `refine` the other variables under log2(n / k)
+/// bisections padded with width-1 slots, `launch` = refine then branch, or
+/// the branch slot alone widened to n when only one variable is live.
+struct AggPart
+{
+  Assign branch, refine, launch;
+  bool nested = false;
+  bool leaf() const { return branch.vars.empty(); }
+};
+
+inline std::uint8_t split_kind(std::uint8_t var_kind)
+{
+  return var_kind == CMI_CONTINUOUS ? CMI_SLOT_CONTINUOUS : CMI_SLOT_INTEGER_PARTITION;
+}
+
+inline void put(Assign& a, std::uint8_t kind, std::size_t v, std::size_t w)
+{
+  a.kinds.push_back(kind);
+  a.vars.push_back(v);
+  a.widths.push_back(w);
+  a.domains.push_back(w);
+}
+
+inline Assign widest_split(const std::vector<Iv>& box, const std::vector<std::uint8_t>& vk, std::size_t k)
+{
+  Assign a;
+  std::size_t best = box.size();
+  for (std::size_t v = 0; v < box.size(); ++v) {
+    if (box[v].ub > box[v].lb && (best == box.size() || box[v].ub - box[v].lb > box[best].ub - box[best].lb)) {
+      best = v;
+    }
+  }
+  if (best < box.size()) {
+    put(a, split_kind(vk[best]), best, k);
+  }
+  return a;
+}
+
+inline std::size_t log2_exact(std::size_t v)
+{
+  std::size_t b = 0;
+  while ((std::size_t(1) << b) < v) {
+    ++b;
+  }
+  return b;
+}
+
+inline AggPart agg_partition(const std::vector<Iv>& box, const std::vector<std::uint8_t>& vk, std::size_t k,
+                             std::size_t n, bool relative)
+{
+  AggPart p;
+  p.branch = widest_split(box, vk, k);
+  if (p.leaf()) {
+    return p;
+  }
+  std::size_t const on = p.branch.vars[0];
+  std::size_t live = 0;
+  for (const Iv& b : box) {
+    live += b.ub > b.lb;
+  }
+  if (live == 1) {
+    p.nested = true;
+    put(p.launch, p.branch.kinds[0], on, n);
+    return p;
+  }
+  std::vector<Iv> rest = box;
+  rest[on].ub = rest[on].lb;
+  p.refine = choose_bisection(rest, vk, log2_exact(n / k), 64, 0, relative);
+  for (std::size_t v = 0; v < box.size() && p.refine.vars.size() < 63; ++v) {
+    bool const taken = std::find(p.refine.vars.begin(), p.refine.vars.end(), v) != p.refine.vars.end();
+    if (v != on && box[v].ub > box[v].lb && !taken) {
+      put(p.refine, split_kind(vk[v]), v, 1);
+    }
+  }
+  p.launch = p.refine;
+  put(p.launch, p.branch.kinds[0], on, k);
+  return p;
+}
+
+/// Exploration key, smaller first: 0 least lower bound, 1 and 2 the reject
+/// index -(f - lb) / (hub - lb) (lb until an incumbent exists).
+inline double agg_key(int rule, double lb, double hub, double f)
+{
+  if (rule == CMI_SELECT_LEAST_LOWER_BOUND || !(f < DBL_MAX)) {
+    return lb;
+  }
+  if (!(hub - lb > 0.0)) {
+    return lb < f ? -INFINITY : INFINITY;
+  }
+  return -((f - lb) / (hub - lb));
+}
+
+struct AggNode
+{
+  std::size_t sidx, pidx, depth;
+  double lb, hub;
+  std::uint64_t hash;
+};
+
+/// Pending children in a slot store, with a key heap and an lb heap of
+/// references (std heap algorithms, lazy deletion, generation-checked slot
+/// reuse, a sweep when stale references exceed 2 * live + 64).
+struct AggFront
+{
+  struct Slot
+  {
+    AggNode n;
+    std::uint64_t gen;
+    bool alive;
+  };
+  struct Ref
+  {
+    double key;
+    std::size_t depth, pidx, slot;
+    std::uint64_t gen;
+  };
+  static_assert(sizeof(Slot) == 64 && sizeof(Ref) == 40, "the reference's layout, which the host budget counts");
+
+  int rule;
+  std::vector<Slot> slots;
+  std::vector<std::size_t> spare;
+  std::vector<Ref> keyh, lbh;
+  std::size_t live = 0;
+
+  static bool cmp(const Ref& a, const Ref& b)  // "a below b" for std heaps
+  {
+    if (a.key < b.key || b.key < a.key) {
+      return a.key > b.key;
+    }
+    if (a.depth != b.depth) {
+      return a.depth < b.depth;
+    }
+    return a.pidx < b.pidx;
+  }
+  bool fresh(const Ref& r) const { return slots[r.slot].alive && slots[r.slot].gen == r.gen; }
+  Ref ref(std::size_t s, double key) const { return {key, slots[s].n.depth, slots[s].n.pidx, s, slots[s].gen}; }
+
+  void push(const AggNode& x, double f)
+  {
+    std::size_t s;
+    if (!spare.empty()) {
+      s = spare.back();
+      spare.pop_back();
+      slots[s] = {x, slots[s].gen + 1, true};
+    } else {
+      s = slots.size();
+      slots.push_back({x, 0, true});
+    }
+    ++live;
+    keyh.push_back(ref(s, agg_key(rule, x.lb, x.hub, f)));
+    std::push_heap(keyh.begin(), keyh.end(), cmp);
+    lbh.push_back(ref(s, x.lb));
+    std::push_heap(lbh.begin(), lbh.end(), cmp);
+  }
+
+  AggNode pop()
+  {
+    for (;;) {
+      Ref const r = keyh.front();
+      std::pop_heap(keyh.begin(), keyh.end(), cmp);
+      keyh.pop_back();
+      if (!fresh(r)) {
+        continue;
+      }
+      slots[r.slot].alive = false;
+      --live;
+      spare.push_back(r.slot);
+      if (keyh.size() + lbh.size() > 2 * live + 64) {
+        for (std::vector<Ref>* h : {&keyh, &lbh}) {
+          std::vector<Ref> k2;
+          k2.reserve(live);
+          for (const Ref& q : *h) {
+            if (fresh(q)) {
+              k2.push_back(q);
+            }
+          }
+          std::make_heap(k2.begin(), k2.end(), cmp);
+          h->swap(k2);
+        }
+      }
+      return slots[r.slot].n;
+    }
+  }
+
+  double min_lb()
+  {
+    while (!lbh.empty() && !fresh(lbh.front())) {
+      std::pop_heap(lbh.begin(), lbh.end(), cmp);
+      lbh.pop_back();
+    }
+    return lbh.empty() ? DBL_MAX : slots[lbh.front().slot].n.lb;
+  }
+
+  void incumbent(double f)
+  {
+    if (rule != CMI_SELECT_REJECT_INDEX) {
+      return;
+    }
+    keyh.clear();
+    lbh.clear();
+    keyh.reserve(live);
+    lbh.reserve(live);
+    for (std::size_t s = 0; s < slots.size(); ++s) {
+      if (slots[s].alive) {
+        keyh.push_back(ref(s, agg_key(rule, slots[s].n.lb, slots[s].n.hub, f)));
+        lbh.push_back(ref(s, slots[s].n.lb));
+      }
+    }
+    std::make_heap(keyh.begin(), keyh.end(), cmp);
+    std::make_heap(lbh.begin(), lbh.end(), cmp);
+  }
+
+  std::size_t viable(double gub) const
+  {
+    std::size_t c = 0;
+    for (const Slot& s : slots) {
+      c += s.alive && !(s.n.lb > gub);
+    }
+    return c;
+  }
+
+  std::size_t cap_bytes() const
+  {
+    return slots.capacity() * sizeof(Slot) + (keyh.capacity() + lbh.capacity()) * sizeof(Ref)
+           + spare.capacity() * sizeof(std::size_t);
+  }
+};
 
 }  // namespace orc
