@@ -183,9 +183,14 @@ std::span<const aggregate::AggregateBound> AggregateBounderImpl::bound_children(
   std::size_t const used = off_data_ + r_count * item_stride_;
   check(hipMemcpyAsync(staging_d_.get(), h, used, hipMemcpyHostToDevice, s), "hipMemcpyAsync(aggregate staging)");
   std::uint32_t const n_tiles = static_cast<std::uint32_t>(r_count * tiles_per_region_);
-  dev::launch_items(dev::Mode::Aggregate, tape_->extended(), staging_d_.as<dev::Item>(), tile_item_d_.as<std::uint32_t>(), n_tiles,
-                    tape_->item_slots(grouped), tape_->item_waves(grouped), s);
-  check(hipGetLastError(), "launch(k_items aggregate)");
+  if (auto const compiled = tape_->items_kernel(true, n_)) {
+    compiled->launch(staging_d_.get(), tile_item_d_.as<std::uint32_t>(), n_tiles, s);
+  } else {
+    dev::launch_items(dev::Mode::Aggregate, tape_->extended(), staging_d_.as<dev::Item>(),
+                      tile_item_d_.as<std::uint32_t>(), n_tiles, tape_->item_slots(grouped), tape_->item_waves(grouped),
+                      s);
+    check(hipGetLastError(), "launch(k_items aggregate)");
+  }
   check(hipMemcpyAsync(child_h_.data(), staging_d_.as<unsigned char>() + off_child_, r_count * 2 * k_ * sizeof(double),
                        hipMemcpyDeviceToHost, s),
         "hipMemcpyAsync(child bounds)");

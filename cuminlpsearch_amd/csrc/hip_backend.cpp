@@ -180,6 +180,34 @@ void DeviceTape::drop_compiled() const
   compiled_.reset();
   compiled_roots_.reset();
   jit_failed_ = false;
+  for (int i = 0; i < 2; ++i) {
+    items_[i].reset();
+    items_failed_[i] = false;
+  }
+}
+
+std::shared_ptr<const jit::CompiledItems> DeviceTape::items_kernel(bool aggregate, std::size_t n_elems) const
+{
+  if (engine_ == BoxEngine::Interpreted || grouped_.code.size() > kMaxCompiledItemWords || !jit::items_fit(grouped_)) {
+    return nullptr;
+  }
+  if (engine_ == BoxEngine::Auto && n_elems < kAutoCompileBoxes) {
+    return nullptr;
+  }
+  std::lock_guard<std::mutex> lock(jit_mu_);
+  int const i = aggregate ? 1 : 0;
+  if (!items_[i] && !items_failed_[i]) {
+    ctx_->activate();
+    try {
+      items_[i] = jit::compile_items(grouped_, ctx_->device(), aggregate);
+    } catch (const DeviceError&) {
+      if (engine_ == BoxEngine::Compiled) {
+        throw;
+      }
+      items_failed_[i] = true;  // Auto keeps the interpreter, which gives the same results
+    }
+  }
+  return items_[i];
 }
 
 jit::Shape DeviceTape::compiled_shape() const
@@ -674,8 +702,13 @@ void RoleImpl::run(std::span<const interval> box, std::span<const std::size_t> v
   bool const grouped = tape_->items_grouped_fit(mode_);
   tape_->item_code(*item, grouped);
   check(hipMemcpyAsync(staging_d_.get(), h, staging_bytes_, hipMemcpyHostToDevice, s), "hipMemcpyAsync(staging)");
-  dev::launch_items(mode_, tape_->extended(), staging_d_.as<dev::Item>(), tile_item_d_.as<std::uint32_t>(), n_tiles_,
-                    tape_->item_slots(grouped), tape_->item_waves(grouped), s);
+  auto const compiled = mode_ == dev::Mode::Bound ? tape_->items_kernel(false, n_elems_) : nullptr;
+  if (compiled) {
+    compiled->launch(staging_d_.get(), tile_item_d_.as<std::uint32_t>(), n_tiles_, s);
+  } else {
+    dev::launch_items(mode_, tape_->extended(), staging_d_.as<dev::Item>(), tile_item_d_.as<std::uint32_t>(), n_tiles_,
+                      tape_->item_slots(grouped), tape_->item_waves(grouped), s);
+  }
   check(hipGetLastError(), "launch(k_items)");
   if (mode_ == dev::Mode::Bound) {
     check(hipMemcpyAsync(feasible_h_.data(), feasible_d_.get(), n_regions_, hipMemcpyDeviceToHost, s),

@@ -170,7 +170,8 @@ public:
   std::uint32_t box_groups() const { return grouped_.n_groups; }
   const tape::Tape& grouped_tape() const { return grouped_; }
 
-  /// Which kernel evaluates explicit boxes: the tape interpreter, the tape
+  /// Which kernel evaluates explicit boxes (and decode-mode bounder and
+  /// aggregate launches, items_kernel): the tape interpreter, the tape
   /// compiled to a straight-line kernel at run time (jit.hpp), or Auto --
   /// compiled once a single call has at least kAutoCompileBoxes boxes and the
   /// tape has at most kMaxCompiledWords words (the first such call waits for
@@ -183,6 +184,10 @@ public:
   };
   static constexpr std::size_t kAutoCompileBoxes = 32768;
   static constexpr std::size_t kMaxCompiledWords = 20000;
+  /// Largest grouped tape whose decode-mode kernel is compiled: the
+  /// persistent tile loop makes its compile grow faster with the tape than
+  /// the explicit-box kernel's (about 6 s at 1,400 words).
+  static constexpr std::size_t kMaxCompiledItemWords = 2048;
   /// Variables per load batch of a new tape's compiled direct form.
   static constexpr std::uint32_t kDefaultLoadBatch = 8;
   /// Compiled compiles now (and throws if that fails).
@@ -190,6 +195,15 @@ public:
   BoxEngine box_engine() const { return engine_; }
   /// The compiled kernel without root-interval stores, if one is loaded.
   std::shared_ptr<const jit::CompiledBoxes> compiled() const;
+
+  /// The compiled decode-mode kernel (jit::compile_items) for bounder
+  /// (aggregate = false) or aggregate launches of n_elems elements each, or
+  /// nullptr to interpret: under Auto once a launch has at least
+  /// kAutoCompileBoxes elements (the first such launch waits for the
+  /// compiler; a failed compile keeps the interpreter), under Compiled
+  /// always, under Interpreted never; only for grouped tapes within
+  /// kMaxCompiledItemWords words whose tile fits in LDS. Results are identical.
+  std::shared_ptr<const jit::CompiledItems> items_kernel(bool aggregate, std::size_t n_elems) const;
 
   /// Shape of the compiled explicit-box kernel: persistent (double-buffered
   /// tiles, grid sized to wgs_per_cu workgroups per CU; only when its LDS
@@ -264,6 +278,9 @@ private:
   mutable std::shared_ptr<const jit::CompiledBoxes> compiled_;
   mutable std::shared_ptr<const jit::CompiledBoxes> compiled_roots_;
   mutable bool jit_failed_ = false;
+  // compiled decode-mode kernels: [0] bounder, [1] aggregate
+  mutable std::shared_ptr<const jit::CompiledItems> items_[2];
+  mutable bool items_failed_[2] = {false, false};
 
   std::shared_ptr<const jit::CompiledBoxes> compile_now(bool roots) const;
   void drop_compiled() const;

@@ -5,7 +5,9 @@
 #include <chrono>
 #include <cmath>
 #include <cstdio>
+#include <cstddef>
 #include <cstring>
+#include <functional>
 #include <algorithm>
 #include <map>
 #include <mutex>
@@ -15,6 +17,7 @@
 #include <vector>
 
 #include "cmi/errors.hpp"
+#include "device.hpp"
 
 // Text of interval_ops.hpp, embedded at build time.
 extern "C" const char cmi_interval_ops_source[];
@@ -334,78 +337,21 @@ bool persistent_fits(const tape::Tape& t, std::uint32_t waves)
   return t.n_vars > 0 && bytes <= kLdsBytes;
 }
 
-std::string boxes_source(const tape::Tape& t, const Shape& shape)
+namespace
+{
+
+/// The wave blocks of a tape kernel: "if (wave == 0u) {...} else if ..."; wave
+/// v runs groups v, v + w, ... over its variables, each bound by the
+/// expression `load(slot, readers)` gives (readers: how many waves read that
+/// variable). `batch` > 0 issues those bindings `batch` variables at a time
+/// (see Shape::load_batch); `roots` writes root intervals. Each wave block
+/// ends with the objective group's output in obj_s[cur * 128 + lane] (lb)
+/// and [cur * 128 + 64 + lane] (ub).
+void emit_waves(std::ostringstream& o, const tape::Tape& t, std::uint32_t w, const std::string& ind, bool roots,
+                std::uint32_t batch, const std::function<std::string(std::uint32_t, std::uint32_t)>& load)
 {
   std::uint32_t const k = std::max<std::uint32_t>(t.n_groups, 1);
-  std::uint32_t const w = shape_waves(t, shape);
   std::uint32_t const nv = t.n_vars;
-  bool const persistent = shape.persistent;
-  if (persistent && !persistent_fits(t, w)) {
-    throw InvalidConfiguration("the double-buffered box tiles of this tape do not fit in LDS");
-  }
-  std::ostringstream o;
-  o << cmi_interval_ops_source << "\n";
-  o << "using namespace cmi_iv;\n\n";
-  o << "__device__ __forceinline__ void out_store(double* roots, unsigned long k, unsigned long n, unsigned long e,\n"
-       "                                          Bnd v)\n{\n"
-       "  roots[2 * (k * n + e)] = v.lb;\n  roots[2 * (k * n + e) + 1] = v.ub;\n}\n\n";
-  o << "extern \"C\" __global__ void __launch_bounds__(" << 64 * w << ") " << kBoxesKernel
-    << "(const double* __restrict__ lb, const double* __restrict__ ub, unsigned long ld, unsigned long n,\n"
-       "    unsigned char* __restrict__ feasible, double* __restrict__ obj_lb, double* __restrict__ obj_ub,\n"
-       "    double* __restrict__ roots)\n{\n";
-  o << "  const unsigned lane = threadIdx.x & 63u;\n";
-  o << "  const unsigned wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);\n";
-  if (persistent) {
-    // Persistent workgroups walk the 64-box tiles with a stride of the grid.
-    // While the waves evaluate tile i from one LDS buffer, each wave's share
-    // of tile i+1's variables is already in flight from HBM into registers,
-    // and goes to the other buffer before the barrier that ends the step.
-    std::uint32_t const share = (nv + w - 1) / w;
-    o << "  __shared__ double2 xs[" << 2 * nv * 64 << "];\n";
-    o << "  __shared__ unsigned feas_s[" << 2 * 64 * w << "];\n";
-    o << "  __shared__ double obj_s[256];\n";
-    o << "  const unsigned long n_tiles = (n + 63ul) / 64ul;\n";
-    o << "  unsigned long tile = blockIdx.x;\n";
-    o << "  if (tile >= n_tiles) {\n    return;\n  }\n";
-    o << "  {\n    const unsigned long e0 = tile * 64ul + lane;\n"
-         "    const unsigned off = 8u * (unsigned)(e0 < n ? e0 : n - 1);\n";
-    o << "    for (unsigned v = wave; v < " << nv << "u; v += " << w << "u) {\n"
-         "      const Iv x = load_box(lb + v * ld, ub + v * ld, off);\n"
-         "      xs[v * 64u + lane] = make_double2(x.nl, x.ub);\n    }\n  }\n";
-    o << "  __syncthreads();\n";
-    o << "  for (unsigned step = 0;; ++step) {\n";
-    o << "    const unsigned cur = step & 1u;\n";
-    o << "    const unsigned long e = tile * 64ul + lane;\n";
-    o << "    const bool active = e < n;\n";
-    o << "    const unsigned long next = tile + gridDim.x;\n";
-    o << "    const bool has_next = next < n_tiles;\n";
-    o << "    const unsigned long en = next * 64ul + lane;\n";
-    o << "    const unsigned offn = 8u * (unsigned)(en < n ? en : n - 1);\n";
-    o << "    Iv pre[" << share << "];\n";
-    o << "    if (has_next) {\n";
-    for (std::uint32_t j = 0; j < share; ++j) {
-      o << "      if (wave + " << j * w << "u < " << nv << "u) {\n"
-        << "        pre[" << j << "] = load_box(lb + (wave + " << j * w << "ul) * ld, ub + (wave + " << j * w
-        << "ul) * ld, offn);\n      }\n";
-    }
-    o << "    }\n";
-    o << "    const double2* const X = xs + cur * " << nv * 64 << "u;\n";
-  } else {
-    // One workgroup per 64 boxes; each wave reads the variables its groups
-    // need straight from the box arrays.
-    o << "  __shared__ unsigned feas_s[" << 64 * w << "];\n";
-    o << "  __shared__ double obj_s[128];\n";
-    o << "  const unsigned cur = 0u;\n";
-    o << "  const unsigned long e = (unsigned long)blockIdx.x * 64ul + lane;\n";
-    o << "  const bool active = e < n;\n";
-    o << "  const unsigned eo = 8u * (unsigned)(active ? e : n - 1);\n";
-  }
-  std::string const ind = persistent ? "    " : "  ";
-  if (shape.roots) {
-    o << ind << "double* const rt = active ? roots : nullptr;\n";
-  }
-  o << ind << "bool f = true;\n";
-  o << ind << "const int t = enter_round_up();\n";
   // Wave v runs groups v, v + w, v + 2w, ...; their statements are
   // interleaved one by one, each group keeping its own order and accumulator.
   // The groups share nothing they write, so the results are those of the
@@ -439,7 +385,7 @@ std::string boxes_source(const tape::Tape& t, const Shape& shape)
       std::uint32_t const hi =
           t.group_begin.size() > g + 1 ? t.group_begin[g + 1] : static_cast<std::uint32_t>(t.code.size());
       slot_use(t, lo, hi, vars, temps);
-      if (!persistent && shape.load_batch != 0) {
+      if (batch != 0) {
         words.push_back(arrival_order(t, lo, hi));
       } else {
         std::vector<std::uint32_t> st;
@@ -516,9 +462,9 @@ std::string boxes_source(const tape::Tape& t, const Shape& shape)
           body.emplace_back(ind + "  const Iv " + name + " = " + product_expr(word) + ";\n", need);
           it = held.emplace(key, name).first;
         }
-        line = emit_word(word, acc, shape.roots, it->second);
+        line = emit_word(word, acc, roots, it->second);
       } else {
-        line = emit_word(word, acc, shape.roots);
+        line = emit_word(word, acc, roots);
       }
       if (!line.empty()) {
         body.emplace_back(ind + line, need);
@@ -528,7 +474,6 @@ std::string boxes_source(const tape::Tape& t, const Shape& shape)
     // Variable loads. Batched (direct form): the first two batches up front,
     // then batch b+1 just before the first statement that reads batch b; a
     // scheduling barrier on each side keeps the compiler from hoisting them.
-    std::uint32_t const batch = persistent ? 0u : shape.load_batch;
     std::size_t const n_batches = batch == 0 ? 1 : (first_use.size() + batch - 1) / batch;
     std::size_t issued = 0;
     auto issue_until = [&](std::size_t upto) {
@@ -540,13 +485,7 @@ std::string boxes_source(const tape::Tape& t, const Shape& shape)
         }
         for (std::size_t q = lo; q < hi; ++q) {
           std::uint32_t const x = first_use[q];
-          if (persistent) {
-            o << ind << "  const Iv " << slot(x) << " = {X[" << x * 64 << "u + lane].x, X[" << x * 64
-              << "u + lane].y};\n";
-          } else {
-            o << ind << "  const Iv " << slot(x) << " = " << (readers[x] > 1 ? "load_box_shared" : "load_box")
-              << "(lb + " << x << "ul * ld, ub + " << x << "ul * ld, eo);\n";
-          }
+          o << ind << "  const Iv " << slot(x) << " = " << load(x, readers[x]) << ";\n";
         }
         if (batch != 0) {
           o << ind << "  __builtin_amdgcn_sched_barrier(0);\n";
@@ -574,6 +513,90 @@ std::string boxes_source(const tape::Tape& t, const Shape& shape)
     }
     o << ind << "}\n";
   }
+}
+
+}  // namespace
+
+std::string boxes_source(const tape::Tape& t, const Shape& shape)
+{
+  std::uint32_t const k = std::max<std::uint32_t>(t.n_groups, 1);
+  std::uint32_t const w = shape_waves(t, shape);
+  std::uint32_t const nv = t.n_vars;
+  bool const persistent = shape.persistent;
+  if (persistent && !persistent_fits(t, w)) {
+    throw InvalidConfiguration("the double-buffered box tiles of this tape do not fit in LDS");
+  }
+  std::ostringstream o;
+  o << cmi_interval_ops_source << "\n";
+  o << "using namespace cmi_iv;\n\n";
+  o << "__device__ __forceinline__ void out_store(double* roots, unsigned long k, unsigned long n, unsigned long e,\n"
+       "                                          Bnd v)\n{\n"
+       "  roots[2 * (k * n + e)] = v.lb;\n  roots[2 * (k * n + e) + 1] = v.ub;\n}\n\n";
+  o << "extern \"C\" __global__ void __launch_bounds__(" << 64 * w << ") " << kBoxesKernel
+    << "(const double* __restrict__ lb, const double* __restrict__ ub, unsigned long ld, unsigned long n,\n"
+       "    unsigned char* __restrict__ feasible, double* __restrict__ obj_lb, double* __restrict__ obj_ub,\n"
+       "    double* __restrict__ roots)\n{\n";
+  o << "  const unsigned lane = threadIdx.x & 63u;\n";
+  o << "  const unsigned wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);\n";
+  if (persistent) {
+    // Persistent workgroups walk the 64-box tiles with a stride of the grid.
+    // While the waves evaluate tile i from one LDS buffer, each wave's share
+    // of tile i+1's variables is already in flight from HBM into registers,
+    // and goes to the other buffer before the barrier that ends the step.
+    std::uint32_t const share = (nv + w - 1) / w;
+    o << "  __shared__ double2 xs[" << 2 * nv * 64 << "];\n";
+    o << "  __shared__ unsigned feas_s[" << 2 * 64 * w << "];\n";
+    o << "  __shared__ double obj_s[256];\n";
+    o << "  const unsigned long n_tiles = (n + 63ul) / 64ul;\n";
+    o << "  unsigned long tile = blockIdx.x;\n";
+    o << "  if (tile >= n_tiles) {\n    return;\n  }\n";
+    o << "  {\n    const unsigned long e0 = tile * 64ul + lane;\n"
+         "    const unsigned off = 8u * (unsigned)(e0 < n ? e0 : n - 1);\n";
+    o << "    for (unsigned v = wave; v < " << nv << "u; v += " << w << "u) {\n"
+         "      const Iv x = load_box(lb + v * ld, ub + v * ld, off);\n"
+         "      xs[v * 64u + lane] = make_double2(x.nl, x.ub);\n    }\n  }\n";
+    o << "  __syncthreads();\n";
+    o << "  for (unsigned step = 0;; ++step) {\n";
+    o << "    const unsigned cur = step & 1u;\n";
+    o << "    const unsigned long e = tile * 64ul + lane;\n";
+    o << "    const bool active = e < n;\n";
+    o << "    const unsigned long next = tile + gridDim.x;\n";
+    o << "    const bool has_next = next < n_tiles;\n";
+    o << "    const unsigned long en = next * 64ul + lane;\n";
+    o << "    const unsigned offn = 8u * (unsigned)(en < n ? en : n - 1);\n";
+    o << "    Iv pre[" << share << "];\n";
+    o << "    if (has_next) {\n";
+    for (std::uint32_t j = 0; j < share; ++j) {
+      o << "      if (wave + " << j * w << "u < " << nv << "u) {\n"
+        << "        pre[" << j << "] = load_box(lb + (wave + " << j * w << "ul) * ld, ub + (wave + " << j * w
+        << "ul) * ld, offn);\n      }\n";
+    }
+    o << "    }\n";
+    o << "    const double2* const X = xs + cur * " << nv * 64 << "u;\n";
+  } else {
+    // One workgroup per 64 boxes; each wave reads the variables its groups
+    // need straight from the box arrays.
+    o << "  __shared__ unsigned feas_s[" << 64 * w << "];\n";
+    o << "  __shared__ double obj_s[128];\n";
+    o << "  const unsigned cur = 0u;\n";
+    o << "  const unsigned long e = (unsigned long)blockIdx.x * 64ul + lane;\n";
+    o << "  const bool active = e < n;\n";
+    o << "  const unsigned eo = 8u * (unsigned)(active ? e : n - 1);\n";
+  }
+  std::string const ind = persistent ? "    " : "  ";
+  if (shape.roots) {
+    o << ind << "double* const rt = active ? roots : nullptr;\n";
+  }
+  o << ind << "bool f = true;\n";
+  o << ind << "const int t = enter_round_up();\n";
+  std::uint32_t const batch = persistent ? 0u : shape.load_batch;
+  emit_waves(o, t, w, ind, shape.roots, batch, [&](std::uint32_t x, std::uint32_t readers) -> std::string {
+    if (persistent) {
+      return "{X[" + std::to_string(x * 64) + "u + lane].x, X[" + std::to_string(x * 64) + "u + lane].y}";
+    }
+    return std::string(readers > 1 ? "load_box_shared" : "load_box") + "(lb + " + std::to_string(x) + "ul * ld, ub + "
+           + std::to_string(x) + "ul * ld, eo)";
+  });
   o << ind << "leave_round_up(t, f);\n";
   o << ind << "feas_s[cur * " << 64 * w << "u + wave * 64u + lane] = f ? 1u : 0u;\n";
   if (persistent) {
@@ -600,6 +623,230 @@ std::string boxes_source(const tape::Tape& t, const Shape& shape)
   }
   o << "}\n";
   return o.str();
+}
+
+bool items_fit(const tape::Tape& t)
+{
+  std::uint64_t const w = std::max<std::uint32_t>(t.n_groups, 1);
+  return t.n_vars > 0 && w <= dev::kMaxGroups
+         && std::uint64_t {t.n_vars} * 64 * 16 + w * 64 * 4 + 128 * 8 <= kLdsBytes;
+}
+
+std::string items_source(const tape::Tape& t, bool aggregate)
+{
+  if (!items_fit(t)) {
+    throw InvalidConfiguration("the decoded variables of this tape's 64-region tile do not fit in LDS");
+  }
+  std::uint32_t const w = std::max<std::uint32_t>(t.n_groups, 1);
+  std::uint32_t const nv = t.n_vars;
+  std::ostringstream o;
+  o << cmi_interval_ops_source << "\n";
+  o << "using namespace cmi_iv;\n\n";
+  // dev::Item, field for field; the layout is checked against the host's.
+  o << "struct CmiGroups\n{\n  unsigned n_groups, objective_group, begin[" << dev::kMaxGroups + 1 << "];\n};\n"
+       "struct CmiItem\n{\n"
+       "  const void* code;\n  CmiGroups groups;\n  unsigned n_instr, n_vars;\n  const double* parent;\n"
+       "  const unsigned char* var_kinds;\n  const unsigned* slot_var;\n  const unsigned* slot_width;\n"
+       "  const unsigned long* slot_prefix;\n  const unsigned char* slot_kind;\n  unsigned slot_count, samples;\n"
+       "  unsigned long n_elems, salt;\n  unsigned tile_begin, n_tiles;\n  unsigned char* feasible;\n"
+       "  double* obj_lb;\n  double* tile_min;\n  long* tile_arg;\n  unsigned* done;\n  double* cand_value;\n"
+       "  long* cand_index;\n  double* witness;\n  double* child_lb;\n  double* child_ub;\n"
+       "  unsigned long refine;\n};\n";
+  o << "static_assert(sizeof(CmiItem) == " << sizeof(dev::Item) << ", \"dev::Item layout\");\n";
+  auto field = [&](const char* name, std::size_t off) {
+    o << "static_assert(__builtin_offsetof(CmiItem, " << name << ") == " << off << ", \"dev::Item::" << name
+      << "\");\n";
+  };
+  field("parent", offsetof(dev::Item, parent));
+  field("slot_var", offsetof(dev::Item, slot_var));
+  field("slot_width", offsetof(dev::Item, slot_width));
+  field("slot_kind", offsetof(dev::Item, slot_kind));
+  field("slot_count", offsetof(dev::Item, slot_count));
+  field("n_elems", offsetof(dev::Item, n_elems));
+  field("tile_begin", offsetof(dev::Item, tile_begin));
+  field("feasible", offsetof(dev::Item, feasible));
+  field("obj_lb", offsetof(dev::Item, obj_lb));
+  field("child_lb", offsetof(dev::Item, child_lb));
+  field("child_ub", offsetof(dev::Item, child_ub));
+  field("refine", offsetof(dev::Item, refine));
+  // The interpreter's decode of one slot (kernels.hip slot_bounds), round to
+  // nearest: an enumerated slot is the point ceil(lb) + part clamped to ub; a
+  // split slot is part `part` of `fan_out` equal parts, the last one ending
+  // at the parent's ub.
+  o << "__device__ __forceinline__ void slot_bounds(unsigned char kind, double plo, double phi, unsigned part,\n"
+       "                                            unsigned fan_out, double& lo, double& hi)\n{\n"
+       "  if (kind >= 2) {\n    double v = __builtin_ceil(plo) + (double)part;\n    if (v > phi) {\n"
+       "      v = phi;\n    }\n    lo = v;\n    hi = v;\n    return;\n  }\n"
+       "  const double width = (phi - plo) / (double)fan_out;\n  lo = plo + width * (double)part;\n"
+       "  hi = part + 1u == fan_out ? phi : plo + width * (double)(part + 1u);\n}\n\n";
+  // Persistent: workgroup g takes the contiguous tiles [g * chunk, (g + 1) *
+  // chunk) one after another. A child of M >= 64 subregions (M a multiple of
+  // 64) covers whole tiles, so wave 0 folds each tile into a running bound of
+  // its child held in registers and issues the atomics only when the child
+  // (or the item) changes; smaller children take the per-tile segmented
+  // reduction. Min and max are exact, so neither the fold order nor the
+  // atomics' order changes a bit.
+  o << "extern \"C\" __global__ void __launch_bounds__(" << 64 * w << ") " << kItemsKernel
+    << "(const CmiItem* __restrict__ items, const unsigned* __restrict__ tile_item, unsigned n_tiles)\n{\n";
+  o << "  __shared__ double2 X[" << nv * 64 << "];\n";
+  o << "  __shared__ unsigned feas_s[" << 64 * w << "];\n";
+  o << "  __shared__ double obj_s[128];\n";
+  o << "  const unsigned lane = threadIdx.x & 63u;\n";
+  o << "  const unsigned wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);\n";
+  o << "  const unsigned chunk = (n_tiles + gridDim.x - 1u) / gridDim.x;\n";
+  o << "  const unsigned first = blockIdx.x * chunk;\n";
+  o << "  const unsigned last = first + chunk < n_tiles ? first + chunk : n_tiles;\n";
+  o << "  const unsigned cur = 0u;\n";
+  if (aggregate) {
+    o << "  unsigned run_item = 0xffffffffu;\n  unsigned long run_child = 0ul;\n"
+         "  double run_lo = __builtin_huge_val(), run_hi = -__builtin_huge_val();\n";
+  }
+  o << "  for (unsigned tile = first; tile < last; ++tile) {\n";
+  o << "    const unsigned item = tile_item[tile];\n";
+  o << "    const CmiItem& it = items[item];\n";
+  o << "    const unsigned long e = (unsigned long)(tile - it.tile_begin) * 64ul + lane;\n";
+  o << "    const bool active = e < it.n_elems;\n";
+  o << "    const unsigned region = (unsigned)(active ? e : it.n_elems - 1ul);\n";
+  o << "    if (wave == 0u) {\n";
+  o << "      for (unsigned v = 0; v < " << nv << "u; ++v) {\n"
+       "        X[v * 64u + lane] = make_double2(-it.parent[2u * v], it.parent[2u * v + 1u]);\n      }\n";
+  o << "      unsigned idx = region;\n";
+  o << "      for (unsigned j = 0; j < it.slot_count; ++j) {\n"
+       "        const unsigned fw = it.slot_width[j];\n        const unsigned part = idx % fw;\n        idx /= fw;\n"
+       "        const unsigned vid = it.slot_var[j];\n        double lo, hi;\n"
+       "        slot_bounds(it.slot_kind[j], it.parent[2u * vid], it.parent[2u * vid + 1u], part, fw, lo, hi);\n"
+       "        X[vid * 64u + lane] = make_double2(-lo, hi);\n      }\n    }\n";
+  if (w > 1) {
+    o << "    __syncthreads();\n";
+  }
+  o << "    bool f = true;\n";
+  o << "    const int t = enter_round_up();\n";
+  emit_waves(o, t, w, "    ", false, 0, [](std::uint32_t x, std::uint32_t) -> std::string {
+    return "{X[" + std::to_string(x * 64) + "u + lane].x, X[" + std::to_string(x * 64) + "u + lane].y}";
+  });
+  o << "    leave_round_up(t, f);\n";
+  o << "    feas_s[wave * 64u + lane] = f ? 1u : 0u;\n";
+  if (w > 1) {
+    o << "    __syncthreads();\n";
+  }
+  o << "    if (wave == 0u) {\n";
+  o << "      unsigned all = 1u;\n";
+  for (std::uint32_t v = 0; v < w; ++v) {
+    o << "      all &= feas_s[" << v * 64 << "u + lane];\n";
+  }
+  if (!aggregate) {
+    o << "      if (active) {\n        it.feasible[region] = all ? 1 : 0;\n        it.obj_lb[region] = obj_s[lane];\n"
+         "      }\n";
+  } else {
+    o << "      const bool live = active && all != 0u;\n";
+    o << "      double lo = live ? obj_s[lane] : __builtin_huge_val();\n";
+    o << "      double hi = live ? obj_s[64u + lane] : -__builtin_huge_val();\n";
+    o << "      if ((it.refine & 63ul) == 0ul) {\n";
+    o << "#pragma unroll\n        for (int off = 32; off > 0; off >>= 1) {\n"
+         "          lo = fmin(lo, __shfl_xor(lo, off, 64));\n          hi = fmax(hi, __shfl_xor(hi, off, 64));\n"
+         "        }\n";
+    o << "        const unsigned long child = (unsigned long)(tile - it.tile_begin) * 64ul / it.refine;\n";
+    o << "        if (item != run_item || child != run_child) {\n"
+         "          if (run_item != 0xffffffffu && lane == 0u) {\n"
+         "            atomicMin(items[run_item].child_lb + run_child, run_lo);\n"
+         "            atomicMax(items[run_item].child_ub + run_child, run_hi);\n          }\n"
+         "          run_item = item;\n          run_child = child;\n          run_lo = __builtin_huge_val();\n"
+         "          run_hi = -__builtin_huge_val();\n        }\n"
+         "        run_lo = fmin(run_lo, lo);\n        run_hi = fmax(run_hi, hi);\n";
+    o << "      } else {\n";
+    o << "        const unsigned long child = (unsigned long)region / it.refine;\n";
+    o << "#pragma unroll\n        for (int off = 1; off < 64; off <<= 1) {\n"
+         "          const double olo = __shfl_down(lo, off, 64);\n          const double ohi = __shfl_down(hi, off, 64);\n"
+         "          const unsigned long oc = __shfl_down(child, off, 64);\n"
+         "          if (lane + off < 64u && oc == child) {\n            lo = fmin(lo, olo);\n"
+         "            hi = fmax(hi, ohi);\n          }\n        }\n";
+    o << "        const unsigned long prev = __shfl_up(child, 1, 64);\n";
+    o << "        const bool head = lane == 0u || prev != child;\n";
+    o << "        if (active && head) {\n          atomicMin(it.child_lb + child, lo);\n"
+         "          atomicMax(it.child_ub + child, hi);\n        }\n";
+    o << "      }\n";
+  }
+  o << "    }\n";
+  o << "  }\n";
+  if (aggregate) {
+    o << "  if (wave == 0u && run_item != 0xffffffffu && lane == 0u) {\n"
+         "    atomicMin(items[run_item].child_lb + run_child, run_lo);\n"
+         "    atomicMax(items[run_item].child_ub + run_child, run_hi);\n  }\n";
+  }
+  o << "}\n";
+  return o.str();
+}
+
+struct CompiledItems::Module
+{
+  int device = 0;
+  std::vector<Instr> code;
+  hipModule_t module = nullptr;
+  hipFunction_t fn = nullptr;
+
+  ~Module()
+  {
+    if (module != nullptr) {
+      int prev = 0;
+      (void)hipGetDevice(&prev);
+      (void)hipSetDevice(device);
+      (void)hipModuleUnload(module);
+      (void)hipSetDevice(prev);
+    }
+  }
+};
+
+void CompiledItems::launch(const void* items, const std::uint32_t* tile_item, std::uint32_t n_tiles,
+                           hipStream_t stream) const
+{
+  if (n_tiles == 0) {
+    return;
+  }
+  void* args[] = {&items, &tile_item, &n_tiles};
+  std::uint32_t const grid = std::min<std::uint32_t>(n_tiles, n_cus_ * wgs_per_cu_);
+  hip_check(hipModuleLaunchKernel(mod_->fn, grid, 1, 1, 64 * waves_, 1, 1, 0, stream, args, nullptr),
+            "hipModuleLaunchKernel(compiled items)");
+}
+
+std::shared_ptr<CompiledItems> compile_items(const tape::Tape& t, int device, bool aggregate)
+{
+  static std::mutex mu;
+  static std::map<std::tuple<int, bool, std::uint64_t, std::size_t>, std::weak_ptr<const CompiledItems::Module>> cache;
+  std::uint64_t h = fnv1a(t.code.data(), t.code.size() * sizeof(Instr));
+  h = fnv1a(&t.n_vars, sizeof t.n_vars, h);
+  h = fnv1a(t.group_begin.data(), t.group_begin.size() * sizeof(std::uint32_t), h);
+  h = fnv1a(&t.objective_group, sizeof t.objective_group, h);
+  auto const key = std::make_tuple(device, aggregate, h, t.code.size());
+  std::shared_ptr<CompiledItems> ci(new CompiledItems());
+  ci->aggregate_ = aggregate;
+  ci->waves_ = std::max<std::uint32_t>(t.n_groups, 1);
+  int cus = 0;
+  hip_check(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device), "hipDeviceGetAttribute(CUs)");
+  ci->n_cus_ = static_cast<std::uint32_t>(std::max(cus, 1));
+  // resident workgroups per CU: by LDS, and at most 32 waves
+  std::uint64_t const lds = std::uint64_t {t.n_vars} * 64 * 16 + std::uint64_t {ci->waves_} * 64 * 4 + 128 * 8;
+  ci->wgs_per_cu_ = static_cast<std::uint32_t>(
+      std::clamp<std::uint64_t>(std::min<std::uint64_t>(kLdsBytes / lds, 32 / ci->waves_), 1, 8));
+  std::lock_guard<std::mutex> lock(mu);
+  if (auto it = cache.find(key); it != cache.end()) {
+    if (auto sp = it->second.lock();
+        sp && sp->code.size() == t.code.size()
+        && std::memcmp(sp->code.data(), t.code.data(), t.code.size() * sizeof(Instr)) == 0) {
+      ci->mod_ = sp;
+      return ci;
+    }
+  }
+  auto const t0 = std::chrono::steady_clock::now();
+  std::vector<char> const code = compile_code_object(items_source(t, aggregate), false);
+  auto m = std::make_shared<CompiledItems::Module>();
+  m->device = device;
+  m->code = t.code;
+  hip_check(hipModuleLoadData(&m->module, code.data()), "hipModuleLoadData(compiled items)");
+  hip_check(hipModuleGetFunction(&m->fn, m->module, kItemsKernel), "hipModuleGetFunction(compiled items)");
+  ci->mod_ = m;
+  ci->compile_seconds_ = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  cache[key] = m;
+  return ci;
 }
 
 /// A loaded code object; unloaded when the last handle goes.

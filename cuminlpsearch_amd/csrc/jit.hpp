@@ -117,6 +117,63 @@ private:
   std::size_t code_bytes_ = 0;
 };
 
+/// Name of the decode-mode kernel in the generated source.
+inline constexpr const char* kItemsKernel = "cmi_items_compiled";
+
+/// Whether the decode-mode kernel of `t` fits in LDS (the tile's decoded
+/// variables, the groups' verdicts and the objective).
+bool items_fit(const tape::Tape& t);
+
+/// The decode-mode counterpart of boxes_source: an extern "C" kernel
+/// kItemsKernel(items, tile_item, n_tiles) over dev::Item descriptors that
+/// all use this tape; a persistent grid whose workgroups each take a
+/// contiguous run of 64-region tiles, one wave per function group. For each
+/// tile, wave 0 decodes its sub-boxes into LDS (the
+/// interpreter's decode, round to nearest); each wave runs its groups as
+/// straight-line code on registers; wave 0 then writes per-region
+/// feasibility and objective lower bound (`aggregate` false: the bounder) or
+/// folds the tile into its child's bound (`aggregate` true): in registers
+/// across the workgroup's consecutive tiles of one child when children cover
+/// whole tiles, with one atomic min and max per run, and per tile with a
+/// segmented reduction otherwise. Results equal the interpreted k_items
+/// kernel's bit for bit.
+std::string items_source(const tape::Tape& t, bool aggregate);
+
+/// A compiled, loaded decode-mode kernel for one tape on one device.
+class CompiledItems
+{
+public:
+  CompiledItems(const CompiledItems&) = delete;
+  CompiledItems& operator=(const CompiledItems&) = delete;
+
+  /// `items` are device dev::Item descriptors (their code and groups fields
+  /// are not read); `tile_item[t]` names the item of workgroup t.
+  void launch(const void* items, const std::uint32_t* tile_item, std::uint32_t n_tiles, hipStream_t stream) const;
+
+  bool aggregate() const { return aggregate_; }
+  std::uint32_t waves() const { return waves_; }
+  /// Workgroups per CU the persistent grid is sized for.
+  std::uint32_t wgs_per_cu() const { return wgs_per_cu_; }
+  double compile_seconds() const { return compile_seconds_; }
+
+private:
+  friend std::shared_ptr<CompiledItems> compile_items(const tape::Tape& t, int device, bool aggregate);
+  CompiledItems() = default;
+
+  struct Module;
+  std::shared_ptr<const Module> mod_;
+  bool aggregate_ = false;
+  std::uint32_t waves_ = 1;
+  std::uint32_t n_cus_ = 1;
+  std::uint32_t wgs_per_cu_ = 1;
+  double compile_seconds_ = 0.0;
+};
+
+/// Compile (or fetch from the process-wide cache) the decode-mode kernel of
+/// `t` and load it on `device`, which must be current. Throws DeviceError
+/// with the compiler log on failure, InvalidConfiguration if it does not fit.
+std::shared_ptr<CompiledItems> compile_items(const tape::Tape& t, int device, bool aggregate);
+
 /// Compile HIP source with hiprtc for gfx950 (host only, no device needed),
 /// with the max-ILP instruction scheduler when `ilp_schedule`; returns the
 /// code object. Throws DeviceError with the compiler log.
