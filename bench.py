@@ -7,6 +7,7 @@ bilinear pairs, 600 reachable DAG nodes, evaluated over 65,536 sub-boxes
 65,536 boxes. value = boxes/s.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-solves] [--no-serve] [--no-batch] [--no-cpu]
+                    [--no-aggregate]
                     [--full-stream]
 
 Also reported, on the same line:
@@ -20,6 +21,9 @@ Also reported, on the same line:
                   --full-stream the 20,000-request stream at 2,000 req/s;
   "batching"      512 seeded QCQPs solved in shared launches vs one after
                   another;
+  "aggregate"     the k-child search: the 4-variable QCQPs to gap 1e-6 and the
+                  MINLPLib models it closes, with the reference's recorded
+                  times for those models;
   "autotune"      every explicit-box kernel shape the autotuner timed.
 """
 from __future__ import annotations
@@ -204,6 +208,75 @@ def c1_solves(ctx, seeds=range(32), iter_limit: int = 5000, cpu_iter_cap: int = 
     }
 
 
+# The reference's k-child (aggregate) runs that converged on models of this
+# corpus, and its per-region gams_solve beside them: wall seconds as its
+# AGGREGATE_STATUS.md table records them (its own GPU, whole process: parse,
+# graph builds and search).
+REFERENCE_AGGREGATE_SECONDS = {"ex2_1_1": (5.6, 235.2), "ex8_1_1": (0.8, 35.7)}
+AGG_MODELS = ("ex2_1_1", "ex8_1_1", "circle", "st_e01", "nvs01", "ex4_1_2")
+
+
+def aggregate_solves(ctx, seeds=range(32), n_c1: int = 1 << 20, n_models: int = 1 << 24, iter_limit: int = 2000):
+    """The k-child search (k = 4 children per node, each bounded over N / 4
+    subregions in one launch): the 4-variable QCQPs' 32 seeds to gap 1e-6,
+    and the MINLPLib models of the corpus it closes, each timed on its first
+    solve (which compiles the problem's kernels once the interpreted work
+    would have paid for it) and on a repeat (kernels cached)."""
+    import cuminlpsearch_amd as cmi
+    from cuminlpsearch_amd import gams, workloads as W
+    from oracle import binding as orc
+
+    c1_opts = dict(partition_budget=n_c1, sample_fan_out=4096, sample_points=4, iter_limit=iter_limit)
+    cmi.aggregate_solve(ctx, W.c1(seeds[0]), **dict(c1_opts, iter_limit=1))
+    rows = []
+    for s in seeds:
+        t0 = time.perf_counter()
+        out, _, _ = cmi.aggregate_solve(ctx, W.c1(s), **c1_opts)
+        rows.append(dict(seed=int(s), converged=bool(out.proven_optimal), iterations=int(out.iterations),
+                         children=int(out.enqueued), seconds=time.perf_counter() - t0,
+                         lower=float(out.lower_bound), upper=float(out.upper_bound)))
+    conv = [r for r in rows if r["converged"]]
+    # the oracle's restated search on the same workload, two iterations
+    t0 = time.perf_counter()
+    o, _, _ = orc.aggregate_solve(W.c1(1).arrays(), **dict(c1_opts, iter_limit=2))
+    cpu_iter = (time.perf_counter() - t0) / max(int(o.iterations), 1)
+    gpu_iter = sum(r["seconds"] for r in rows) / max(sum(r["iterations"] for r in rows), 1)
+    models = {}
+    root = pathlib.Path(ROOT) / "tests" / "golden" / "gams"
+    m_opts = dict(partition_budget=n_models, sample_fan_out=1 << 14, sample_points=1, iter_limit=1000)
+    for name in AGG_MODELS:
+        pm = gams.parse_file(root / f"{name}.gms")
+        t0 = time.perf_counter()
+        out, _, _ = cmi.aggregate_solve(ctx, pm.problem, **m_opts)
+        first = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        again, _, _ = cmi.aggregate_solve(ctx, pm.problem, **m_opts)
+        repeat = time.perf_counter() - t0
+        assert again.iterations == out.iterations and again.upper_bound == out.upper_bound
+        row = dict(iterations=int(out.iterations), converged=bool(out.proven_optimal), seconds=round(first, 4),
+                   seconds_repeat=round(repeat, 4), lower=float(out.lower_bound), upper=float(out.upper_bound))
+        if name in REFERENCE_AGGREGATE_SECONDS:
+            agg_s, plain_s = REFERENCE_AGGREGATE_SECONDS[name]
+            row["reference_aggregate_seconds"] = agg_s
+            row["reference_gams_solve_seconds"] = plain_s
+        models[name] = row
+    return {
+        "workload": f"k-child search, k=4, N={n_c1} subregions bounded per node (N/4 per child), sampling "
+                    f"4096 subregions x 4 points, gap 1e-6, cap {iter_limit} iterations; QCQP n=4 seeds "
+                    f"{seeds[0]}..{seeds[-1]}",
+        "converged": len(conv),
+        "converged_fraction": len(conv) / len(rows),
+        "solves_per_s_converged": (len(conv) / sum(r["seconds"] for r in conv)) if conv else None,
+        "median_iterations_converged": float(np.median([r["iterations"] for r in conv])) if conv else None,
+        "gpu_seconds_per_iteration": gpu_iter,
+        "cpu_oracle_seconds_per_iteration": cpu_iter,
+        "per_seed": [{k: (round(v, 6) if isinstance(v, float) else v) for k, v in r.items()} for r in rows],
+        "minlplib": {"settings": f"N={n_models}, 16384 sampled subregions x 1 point, gap 1e-6, cap 1000; "
+                                 "seconds: first solve of the model in this process, parse excluded",
+                     "models": models},
+    }
+
+
 def serve_comparison(ctx, n_req: int = 128, rates=(50.0, 200.0, 2000.0), seed: int = 7, n_max: int = 16,
                      iter_limit: int = 1000, plain: bool = True):
     """Served (continuous batching) vs plain (one solve at a time) on the same
@@ -315,6 +388,8 @@ def main(argv=None) -> int:
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU-baseline leg")
     ap.add_argument("--no-solves", action="store_true",
                     help="skip the 32 solves of the 4-variable QCQP to gap 1e-6 (GPU plain, batched, CPU oracle)")
+    ap.add_argument("--no-aggregate", action="store_true",
+                    help="skip the k-child (aggregate) search runs")
     ap.add_argument("--full-stream", action="store_true",
                     help="also serve the whole 20,000-request Poisson stream (n in [4, 96]) at 2,000 req/s")
     ap.add_argument("--full-stream-cap", type=int, default=20,
@@ -584,6 +659,9 @@ def main(argv=None) -> int:
     if not a.no_solves:
         progress("solves of the 4-variable QCQP")
         line["solves"] = c1_solves(ctx)
+    if not a.no_aggregate:
+        progress("k-child searches")
+        line["aggregate"] = aggregate_solves(ctx)
     if not a.no_serve:
         progress("serving")
         line["serving"] = serve_comparison(ctx)

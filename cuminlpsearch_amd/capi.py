@@ -162,7 +162,7 @@ class AggregateOptions(C.Structure):
         ("branch_fan_out", C.c_size_t), ("partition_budget", C.c_size_t), ("sample_fan_out", C.c_size_t),
         ("sample_points", C.c_size_t), ("iter_limit", C.c_uint32), ("tolerance", C.c_double), ("selection", C.c_int),
         ("relative_width", C.c_int), ("budget_bytes", C.c_size_t), ("host_budget_bytes", C.c_size_t),
-        ("has_known_primal", C.c_int), ("known_primal", C.c_double),
+        ("has_known_primal", C.c_int), ("known_primal", C.c_double), ("kernel", C.c_int),
     ]
 
 

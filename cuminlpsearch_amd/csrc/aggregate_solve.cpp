@@ -122,7 +122,12 @@ AggregateRun aggregate_solve(std::shared_ptr<Context> ctx, const model::Problem<
     opt.known_primal_bound = o.known_primal;
   }
   problem.validate();
-  auto tape = std::make_shared<const DeviceTape>(ctx, problem);
+  if (o.kernel < CMI_BOX_AUTO || o.kernel > CMI_BOX_COMPILED) {
+    throw InvalidConfiguration("unknown kernel " + std::to_string(o.kernel));
+  }
+  auto built = std::make_shared<DeviceTape>(ctx, problem);
+  built->set_items_engine(static_cast<DeviceTape::BoxEngine>(o.kernel));
+  std::shared_ptr<const DeviceTape> const tape = built;
   AggregateRoles roles(tape, o.partition_budget, o.branch_fan_out, o.sample_fan_out, o.sample_points, o.budget_bytes);
   aggregate::AggregateSolveState st(problem, bound_policy, sample_policy, opt);
   try {

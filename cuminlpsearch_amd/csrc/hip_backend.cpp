@@ -183,25 +183,38 @@ void DeviceTape::drop_compiled() const
   for (int i = 0; i < 2; ++i) {
     items_[i].reset();
     items_failed_[i] = false;
+    items_work_[i] = 0.0;
   }
+}
+
+void DeviceTape::set_items_engine(BoxEngine e)
+{
+  if (e != BoxEngine::Auto && e != BoxEngine::Interpreted && e != BoxEngine::Compiled) {
+    throw InvalidConfiguration("unknown decode-mode engine");
+  }
+  items_engine_ = e;
 }
 
 std::shared_ptr<const jit::CompiledItems> DeviceTape::items_kernel(bool aggregate, std::size_t n_elems) const
 {
-  if (engine_ == BoxEngine::Interpreted || grouped_.code.size() > kMaxCompiledItemWords || !jit::items_fit(grouped_)) {
-    return nullptr;
-  }
-  if (engine_ == BoxEngine::Auto && n_elems < kAutoCompileBoxes) {
+  BoxEngine const engine = items_engine_ != BoxEngine::Auto ? items_engine_ : engine_;
+  if (engine == BoxEngine::Interpreted || grouped_.code.size() > kMaxCompiledItemWords || !jit::items_fit(grouped_)) {
     return nullptr;
   }
   std::lock_guard<std::mutex> lock(jit_mu_);
   int const i = aggregate ? 1 : 0;
+  if (engine == BoxEngine::Auto && !items_[i]) {
+    items_work_[i] += static_cast<double>(n_elems) * static_cast<double>(grouped_.code.size());
+    if (items_work_[i] < kCompileAfterWork) {
+      return nullptr;
+    }
+  }
   if (!items_[i] && !items_failed_[i]) {
     ctx_->activate();
     try {
       items_[i] = jit::compile_items(grouped_, ctx_->device(), aggregate);
     } catch (const DeviceError&) {
-      if (engine_ == BoxEngine::Compiled) {
+      if (engine == BoxEngine::Compiled) {
         throw;
       }
       items_failed_[i] = true;  // Auto keeps the interpreter, which gives the same results

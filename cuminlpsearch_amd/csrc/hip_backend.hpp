@@ -196,14 +196,21 @@ public:
   /// The compiled kernel without root-interval stores, if one is loaded.
   std::shared_ptr<const jit::CompiledBoxes> compiled() const;
 
-  /// The compiled decode-mode kernel (jit::compile_items) for bounder
-  /// (aggregate = false) or aggregate launches of n_elems elements each, or
-  /// nullptr to interpret: under Auto once a launch has at least
-  /// kAutoCompileBoxes elements (the first such launch waits for the
-  /// compiler; a failed compile keeps the interpreter), under Compiled
-  /// always, under Interpreted never; only for grouped tapes within
+  /// The compiled decode-mode kernel (jit::compile_items) for a bounder
+  /// (aggregate = false) or aggregate launch of n_elems elements, or nullptr
+  /// to interpret it. Under Auto a tape is compiled once the decode-mode
+  /// launches of that kind on it have reached kCompileAfterWork
+  /// element-words (elements times tape words, about a tenth of a second of
+  /// interpreted work, which a compile costs about as much as), so short
+  /// solves never wait for the compiler and long ones stop interpreting
+  /// soon; a failed compile keeps the interpreter. Under Compiled always,
+  /// under Interpreted never; only for grouped tapes within
   /// kMaxCompiledItemWords words whose tile fits in LDS. Results are identical.
   std::shared_ptr<const jit::CompiledItems> items_kernel(bool aggregate, std::size_t n_elems) const;
+  static constexpr double kCompileAfterWork = 17179869184.0;  // 2^34 element-words
+  /// The engine of decode-mode launches alone; Auto (the default) follows
+  /// the box engine.
+  void set_items_engine(BoxEngine e);
 
   /// Shape of the compiled explicit-box kernel: persistent (double-buffered
   /// tiles, grid sized to wgs_per_cu workgroups per CU; only when its LDS
@@ -281,6 +288,8 @@ private:
   // compiled decode-mode kernels: [0] bounder, [1] aggregate
   mutable std::shared_ptr<const jit::CompiledItems> items_[2];
   mutable bool items_failed_[2] = {false, false};
+  mutable double items_work_[2] = {0.0, 0.0};  // element-words launched, interpreted
+  BoxEngine items_engine_ = BoxEngine::Auto;
 
   std::shared_ptr<const jit::CompiledBoxes> compile_now(bool roots) const;
   void drop_compiled() const;

@@ -359,8 +359,9 @@ def solve_options(partition_num=2, enumerate_cap=2, max_cycle_size=0, sample_poi
 
 def aggregate_options(branch_fan_out=4, partition_budget=1024, sample_fan_out=8, sample_points=1,
                       iter_limit=1000000, tolerance=1e-6, selection=0, relative_width=False, budget_bytes=0,
-                      host_budget_bytes=0, known_primal=None) -> _c.AggregateOptions:
-    """Settings of the k-child search; selection is capi.SELECT_*."""
+                      host_budget_bytes=0, known_primal=None, kernel=0) -> _c.AggregateOptions:
+    """Settings of the k-child search; selection is capi.SELECT_*, kernel a
+    capi.BOX_* engine for the bound launches."""
     o = _c.AggregateOptions()
     _c.lib().cmi_aggregate_options_default(C.byref(o))
     o.branch_fan_out = branch_fan_out
@@ -373,6 +374,7 @@ def aggregate_options(branch_fan_out=4, partition_budget=1024, sample_fan_out=8,
     o.relative_width = int(relative_width)
     o.budget_bytes = budget_bytes
     o.host_budget_bytes = host_budget_bytes
+    o.kernel = kernel
     if known_primal is not None:
         o.has_known_primal = 1
         o.known_primal = known_primal

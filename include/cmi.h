@@ -552,6 +552,8 @@ typedef struct
   size_t host_budget_bytes; /* 0 = half of MemAvailable; on reaching it the search stops (never compacts) */
   int has_known_primal;
   double known_primal;      /* a sound upper bound on the minimum, seeding the incumbent value */
+  int kernel;               /* cmi_box_engine of the bound launches: AUTO (interpreted until about a tenth
+                               of a second of work, then compiled), INTERPRETED or COMPILED; same results */
 } cmi_aggregate_options;
 
 /* Defaults: k 4, N 1024, sample fan-out 8, 1 sample, iter_limit 1000000,

@@ -91,13 +91,22 @@ def test_compiled_aggregate_with_integer_and_enumerate_slots(ctx):
 
 
 def test_aggregate_search_on_the_compiled_kernel_equals_the_oracle(ctx):
-    """N = 2^15 per iteration: at and above 32,768 subregions the search's
-    launches take the compiled kernel."""
+    """The search with its bound launches on the compiled kernel from the
+    first iteration, against the oracle; and under the automatic choice,
+    which switches from the interpreter to the compiled kernel mid-solve,
+    against the interpreter throughout."""
     from test_gpu_aggregate_search import same
-    for name in ("ex8_1_1", "st_e01"):
+    for name in ("ex8_1_1", "st_e01", "nvs01"):
         pm = gams.parse_file(GOLDEN / "gams" / f"{name}.gms")
         opts = dict(partition_budget=1 << 15, sample_fan_out=64, sample_points=2, iter_limit=25)
-        d, bd, _ = cmi.aggregate_solve(ctx, pm.problem, **opts)
+        d, bd, _ = cmi.aggregate_solve(ctx, pm.problem, kernel=C.BOX_COMPILED, **opts)
         o, bo, _ = orc.aggregate_solve(pm.problem.arrays(), **opts)
         same(d, o)
         assert np.array_equal(bd.view(np.uint64), bo.view(np.uint64))
+    pm = gams.parse_file(GOLDEN / "gams" / "ex2_1_1.gms")
+    opts = dict(partition_budget=1 << 24, sample_fan_out=1 << 12, sample_points=1, iter_limit=150)
+    auto, ba, _ = cmi.aggregate_solve(ctx, pm.problem, **opts)
+    interp, bi, _ = cmi.aggregate_solve(ctx, pm.problem, kernel=C.BOX_INTERPRETED, **opts)
+    same(auto, interp)
+    assert np.array_equal(ba.view(np.uint64), bi.view(np.uint64))
+    assert auto.proven_optimal
