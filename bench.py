@@ -260,7 +260,11 @@ def aggregate_solves(ctx, seeds=range(32), n_c1: int = 1 << 20, n_models: int = 
             row["reference_aggregate_seconds"] = agg_s
             row["reference_gams_solve_seconds"] = plain_s
         models[name] = row
+    from cuminlpsearch_amd.kernel_timing import aggregate_launch
+    launch = {name: aggregate_launch(ctx, prob, n_models)
+              for name, prob in (("ex2_1_1", gams.parse_file(root / "ex2_1_1.gms").problem), ("c1_seed0", W.c1(0)))}
     return {
+        "bound_launch": launch,
         "workload": f"k-child search, k=4, N={n_c1} subregions bounded per node (N/4 per child), sampling "
                     f"4096 subregions x 4 points, gap 1e-6, cap {iter_limit} iterations; QCQP n=4 seeds "
                     f"{seeds[0]}..{seeds[-1]}",

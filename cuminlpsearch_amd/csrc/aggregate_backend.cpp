@@ -93,15 +93,15 @@ AggregateBounderImpl::AggregateBounderImpl(std::shared_ptr<const DeviceTape> tap
     std::size_t const o_parent = d;
     std::size_t const o_var = align_up(o_parent + n_vars_ * 16, 16);
     std::size_t const o_width = align_up(o_var + slot_count_ * 4, 16);
-    std::size_t const o_prefix = align_up(o_width + slot_count_ * 4, 16);
-    std::size_t const o_kind = align_up(o_prefix + slot_count_ * 8, 16);
+    std::size_t const o_step = align_up(o_width + slot_count_ * 4, 16);
+    std::size_t const o_kind = align_up(o_step + slot_count_ * 8, 16);
     tape_->item_code(*item, false);
     item->n_vars = static_cast<std::uint32_t>(n_vars_);
     item->parent = reinterpret_cast<const double*>(base + o_parent);
     item->var_kinds = tape_->var_kinds();
     item->slot_var = reinterpret_cast<const std::uint32_t*>(base + o_var);
     item->slot_width = reinterpret_cast<const std::uint32_t*>(base + o_width);
-    item->slot_prefix = reinterpret_cast<const std::uint64_t*>(base + o_prefix);
+    item->slot_step = reinterpret_cast<const double*>(base + o_step);
     item->slot_kind = reinterpret_cast<const std::uint8_t*>(base + o_kind);
     item->slot_count = static_cast<std::uint32_t>(slot_count_);
     item->samples = 1;
@@ -166,17 +166,15 @@ std::span<const aggregate::AggregateBound> AggregateBounderImpl::bound_children(
     std::size_t const d = off_data_ + i * item_stride_;
     std::size_t const o_var = align_up(d + n_vars_ * 16, 16);
     std::size_t const o_width = align_up(o_var + slot_count_ * 4, 16);
-    std::size_t const o_prefix = align_up(o_width + slot_count_ * 4, 16);
+    std::size_t const o_step = align_up(o_width + slot_count_ * 4, 16);
     std::memcpy(h + d, r.box.data(), n_vars_ * sizeof(interval));
     auto* var = reinterpret_cast<std::uint32_t*>(h + o_var);
     auto* width = reinterpret_cast<std::uint32_t*>(h + o_width);
-    auto* prefix = reinterpret_cast<std::uint64_t*>(h + o_prefix);
-    std::uint64_t running = 1;
+    auto* step = reinterpret_cast<double*>(h + o_step);
     for (std::size_t j = 0; j < slot_count_; ++j) {
       var[j] = static_cast<std::uint32_t>(r.var_ids[j]);
       width[j] = static_cast<std::uint32_t>(r.widths[j]);
-      prefix[j] = running;
-      running *= r.widths[j];
+      step[j] = (r.box[r.var_ids[j]].ub - r.box[r.var_ids[j]].lb) / static_cast<double>(r.widths[j]);
     }
   }
   // Only the used regions' items, results and data travel.

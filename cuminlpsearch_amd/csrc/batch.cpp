@@ -225,24 +225,22 @@ void Runner::run(std::span<const Request> reqs)
     std::size_t const off_parent = data_off;
     std::size_t const off_var = off_parent + align_up(p.n_vars * 16, 16);
     std::size_t const off_width = off_var + align_up(slots * 4, 16);
-    std::size_t const off_prefix = off_width + align_up(slots * 4, 16);
-    std::size_t const off_kind = off_prefix + align_up(slots * 8, 16);
+    std::size_t const off_step = off_width + align_up(slots * 4, 16);
+    std::size_t const off_kind = off_step + align_up(slots * 8, 16);
     data_off = off_kind + align_up(slots, 16);
 
     std::memcpy(h + off_parent, r.box.data(), p.n_vars * 16);
     auto* var = reinterpret_cast<std::uint32_t*>(h + off_var);
     auto* width = reinterpret_cast<std::uint32_t*>(h + off_width);
-    auto* prefix = reinterpret_cast<std::uint64_t*>(h + off_prefix);
+    auto* step = reinterpret_cast<double*>(h + off_step);
     auto* kind = reinterpret_cast<std::uint8_t*>(h + off_kind);
-    std::uint64_t running = 1;
     for (std::size_t j = 0; j < slots; ++j) {
       if (r.assignment->var_ids[j] >= p.n_vars) {
         throw ShapeMismatch("batched request: slot var_id out of range");
       }
       var[j] = static_cast<std::uint32_t>(r.assignment->var_ids[j]);
       width[j] = static_cast<std::uint32_t>(r.assignment->widths[j]);
-      prefix[j] = running;
-      running *= r.assignment->widths[j];
+      step[j] = (r.box[var[j]].ub - r.box[var[j]].lb) / static_cast<double>(width[j]);
       kind[j] = static_cast<std::uint8_t>(r.assignment->composition[j]);
     }
 
@@ -255,7 +253,7 @@ void Runner::run(std::span<const Request> reqs)
     it.var_kinds = r.tape->var_kinds();
     it.slot_var = reinterpret_cast<const std::uint32_t*>(d + off_var);
     it.slot_width = reinterpret_cast<const std::uint32_t*>(d + off_width);
-    it.slot_prefix = reinterpret_cast<const std::uint64_t*>(d + off_prefix);
+    it.slot_step = reinterpret_cast<const double*>(d + off_step);
     it.slot_kind = reinterpret_cast<const std::uint8_t*>(d + off_kind);
     it.slot_count = static_cast<std::uint32_t>(slots);
     it.samples = static_cast<std::uint32_t>(r.mode == dev::Mode::Sample ? r.samples : 1);

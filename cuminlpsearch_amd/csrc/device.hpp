@@ -39,7 +39,7 @@ struct Item
   const std::uint8_t* var_kinds;  // n_vars, VarKind
   const std::uint32_t* slot_var;  // slot_count entries each
   const std::uint32_t* slot_width;
-  const std::uint64_t* slot_prefix;
+  const double* slot_step;  // per slot: (ub - lb) / width of its variable's parent bounds
   const std::uint8_t* slot_kind;
   std::uint32_t slot_count;
   std::uint32_t samples;  // points per region (sampler); 1 otherwise

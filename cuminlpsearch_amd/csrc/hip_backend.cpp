@@ -615,8 +615,8 @@ RoleImpl::RoleImpl(std::shared_ptr<const DeviceTape> tape, dev::Mode mode, const
   off_parent_ = align_up(sizeof(dev::Item), 256);
   off_var_ = align_up(off_parent_ + n_vars_ * 16, 16);
   off_width_ = align_up(off_var_ + slot_count_ * 4, 16);
-  off_prefix_ = align_up(off_width_ + slot_count_ * 4, 16);
-  off_kind_ = align_up(off_prefix_ + slot_count_ * 8, 16);
+  off_step_ = align_up(off_width_ + slot_count_ * 4, 16);
+  off_kind_ = align_up(off_step_ + slot_count_ * 8, 16);
   staging_bytes_ = align_up(off_kind_ + slot_count_, 16);
   staging_d_.reset(staging_bytes_);
   staging_h_.reset(staging_bytes_);
@@ -654,7 +654,7 @@ RoleImpl::RoleImpl(std::shared_ptr<const DeviceTape> tape, dev::Mode mode, const
   item->var_kinds = tape_->var_kinds();
   item->slot_var = reinterpret_cast<const std::uint32_t*>(base + off_var_);
   item->slot_width = reinterpret_cast<const std::uint32_t*>(base + off_width_);
-  item->slot_prefix = reinterpret_cast<const std::uint64_t*>(base + off_prefix_);
+  item->slot_step = reinterpret_cast<const double*>(base + off_step_);
   item->slot_kind = reinterpret_cast<const std::uint8_t*>(base + off_kind_);
   item->slot_count = static_cast<std::uint32_t>(slot_count_);
   item->samples = static_cast<std::uint32_t>(samples_);
@@ -703,13 +703,11 @@ void RoleImpl::run(std::span<const interval> box, std::span<const std::size_t> v
   std::memcpy(h + off_parent_, box.data(), n_vars_ * sizeof(interval));
   auto* var = reinterpret_cast<std::uint32_t*>(h + off_var_);
   auto* width = reinterpret_cast<std::uint32_t*>(h + off_width_);
-  auto* prefix = reinterpret_cast<std::uint64_t*>(h + off_prefix_);
-  std::uint64_t running = 1;
+  auto* step = reinterpret_cast<double*>(h + off_step_);
   for (std::size_t j = 0; j < slot_count_; ++j) {
     var[j] = static_cast<std::uint32_t>(var_ids[j]);
     width[j] = static_cast<std::uint32_t>(widths[j]);
-    prefix[j] = running;
-    running *= widths[j];
+    step[j] = (box[var_ids[j]].ub - box[var_ids[j]].lb) / static_cast<double>(widths[j]);
   }
   // the tape's current function groups (a regrouping replaces their code)
   bool const grouped = tape_->items_grouped_fit(mode_);
@@ -744,15 +742,15 @@ namespace
 /// scratch array.
 struct StaticLayout
 {
-  std::size_t parent, var, width, prefix, kind, tile_item, tile_min, tile_arg, done, out, witness, total;
+  std::size_t parent, var, width, step, kind, tile_item, tile_min, tile_arg, done, out, witness, total;
 
   StaticLayout(std::size_t n_vars, std::size_t slots, std::size_t n_tiles)
   {
     parent = align_up(sizeof(dev::Item), 256);
     var = align_up(parent + n_vars * 16, 16);
     width = align_up(var + slots * 4, 16);
-    prefix = align_up(width + slots * 4, 16);
-    kind = align_up(prefix + slots * 8, 16);
+    step = align_up(width + slots * 4, 16);
+    kind = align_up(step + slots * 8, 16);
     tile_item = align_up(kind + slots, 256);
     tile_min = align_up(tile_item + n_tiles * 4, 256);
     tile_arg = align_up(tile_min + n_tiles * 8, 256);
@@ -819,7 +817,7 @@ void sample_static(const DeviceTape& tape, const region::Composition& compositio
   item.var_kinds = tape.var_kinds();
   item.slot_var = reinterpret_cast<const std::uint32_t*>(base + L.var);
   item.slot_width = reinterpret_cast<const std::uint32_t*>(base + L.width);
-  item.slot_prefix = reinterpret_cast<const std::uint64_t*>(base + L.prefix);
+  item.slot_step = reinterpret_cast<const double*>(base + L.step);
   item.slot_kind = reinterpret_cast<const std::uint8_t*>(base + L.kind);
   item.slot_count = static_cast<std::uint32_t>(slots);
   item.samples = static_cast<std::uint32_t>(samples_per_region);
@@ -835,14 +833,12 @@ void sample_static(const DeviceTape& tape, const region::Composition& compositio
   item.witness = reinterpret_cast<double*>(base + L.witness);
   std::uint32_t var[dev::kMaxSlotsPerItem] = {};
   std::uint32_t width[dev::kMaxSlotsPerItem] = {};
-  std::uint64_t prefix[dev::kMaxSlotsPerItem] = {};
+  double step[dev::kMaxSlotsPerItem] = {};
   std::uint8_t kind[dev::kMaxSlotsPerItem] = {};
-  std::uint64_t running = 1;
   for (std::size_t j = 0; j < slots; ++j) {
     var[j] = static_cast<std::uint32_t>(var_ids[j]);
     width[j] = static_cast<std::uint32_t>(widths[j]);
-    prefix[j] = running;
-    running *= widths[j];
+    step[j] = (box[var_ids[j]].ub - box[var_ids[j]].lb) / static_cast<double>(widths[j]);
     kind[j] = static_cast<std::uint8_t>(composition[j]);
   }
   check(hipMemcpyAsync(base, &item, sizeof(item), hipMemcpyHostToDevice, s), "hipMemcpyAsync(static item)");
@@ -851,8 +847,8 @@ void sample_static(const DeviceTape& tape, const region::Composition& compositio
   if (slots > 0) {
     check(hipMemcpyAsync(base + L.var, var, slots * 4, hipMemcpyHostToDevice, s), "hipMemcpyAsync(static var)");
     check(hipMemcpyAsync(base + L.width, width, slots * 4, hipMemcpyHostToDevice, s), "hipMemcpyAsync(static width)");
-    check(hipMemcpyAsync(base + L.prefix, prefix, slots * 8, hipMemcpyHostToDevice, s),
-          "hipMemcpyAsync(static prefix)");
+    check(hipMemcpyAsync(base + L.step, step, slots * 8, hipMemcpyHostToDevice, s),
+          "hipMemcpyAsync(static step)");
     check(hipMemcpyAsync(base + L.kind, kind, slots, hipMemcpyHostToDevice, s), "hipMemcpyAsync(static kind)");
   }
   // Every tile belongs to item 0, and the done counter starts at zero: the

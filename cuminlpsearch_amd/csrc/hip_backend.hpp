@@ -362,7 +362,7 @@ private:
   PinnedBuffer<double> obj_lb_h_;
   PinnedBuffer<Candidate> out_h_;
   PinnedBuffer<double> witness_h_;
-  std::size_t off_parent_ = 0, off_var_ = 0, off_width_ = 0, off_prefix_ = 0, off_kind_ = 0, staging_bytes_ = 0;
+  std::size_t off_parent_ = 0, off_var_ = 0, off_width_ = 0, off_step_ = 0, off_kind_ = 0, staging_bytes_ = 0;
 };
 
 /// The sampler's launch without a role and without allocating: staging, tile

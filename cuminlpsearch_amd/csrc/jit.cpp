@@ -647,7 +647,7 @@ std::string items_source(const tape::Tape& t, bool aggregate)
        "struct CmiItem\n{\n"
        "  const void* code;\n  CmiGroups groups;\n  unsigned n_instr, n_vars;\n  const double* parent;\n"
        "  const unsigned char* var_kinds;\n  const unsigned* slot_var;\n  const unsigned* slot_width;\n"
-       "  const unsigned long* slot_prefix;\n  const unsigned char* slot_kind;\n  unsigned slot_count, samples;\n"
+       "  const double* slot_step;\n  const unsigned char* slot_kind;\n  unsigned slot_count, samples;\n"
        "  unsigned long n_elems, salt;\n  unsigned tile_begin, n_tiles;\n  unsigned char* feasible;\n"
        "  double* obj_lb;\n  double* tile_min;\n  long* tile_arg;\n  unsigned* done;\n  double* cand_value;\n"
        "  long* cand_index;\n  double* witness;\n  double* child_lb;\n  double* child_ub;\n"
@@ -660,6 +660,7 @@ std::string items_source(const tape::Tape& t, bool aggregate)
   field("parent", offsetof(dev::Item, parent));
   field("slot_var", offsetof(dev::Item, slot_var));
   field("slot_width", offsetof(dev::Item, slot_width));
+  field("slot_step", offsetof(dev::Item, slot_step));
   field("slot_kind", offsetof(dev::Item, slot_kind));
   field("slot_count", offsetof(dev::Item, slot_count));
   field("n_elems", offsetof(dev::Item, n_elems));
@@ -672,13 +673,19 @@ std::string items_source(const tape::Tape& t, bool aggregate)
   // The interpreter's decode of one slot (kernels.hip slot_bounds), round to
   // nearest: an enumerated slot is the point ceil(lb) + part clamped to ub; a
   // split slot is part `part` of `fan_out` equal parts, the last one ending
-  // at the parent's ub.
-  o << "__device__ __forceinline__ void slot_bounds(unsigned char kind, double plo, double phi, unsigned part,\n"
-       "                                            unsigned fan_out, double& lo, double& hi)\n{\n"
+  // at the parent's ub. The part width (ub - lb) / fan_out is the item's
+  // slot_step, the same correctly rounded quotient formed once on the host.
+  o << "__device__ __forceinline__ double lane_value(double x, unsigned j)\n{\n"
+       "  const unsigned long u = __builtin_bit_cast(unsigned long, x);\n"
+       "  const unsigned lo = __builtin_amdgcn_readlane((unsigned)u, j);\n"
+       "  const unsigned hi = __builtin_amdgcn_readlane((unsigned)(u >> 32), j);\n"
+       "  return __builtin_bit_cast(double, ((unsigned long)hi << 32) | lo);\n}\n\n";
+  o << "__device__ __forceinline__ void slot_bounds(unsigned char kind, double plo, double phi, double step,\n"
+       "                                            unsigned part, unsigned fan_out, double& lo, double& hi)\n{\n"
        "  if (kind >= 2) {\n    double v = __builtin_ceil(plo) + (double)part;\n    if (v > phi) {\n"
        "      v = phi;\n    }\n    lo = v;\n    hi = v;\n    return;\n  }\n"
-       "  const double width = (phi - plo) / (double)fan_out;\n  lo = plo + width * (double)part;\n"
-       "  hi = part + 1u == fan_out ? phi : plo + width * (double)(part + 1u);\n}\n\n";
+       "  lo = plo + step * (double)part;\n"
+       "  hi = part + 1u == fan_out ? phi : plo + step * (double)(part + 1u);\n}\n\n";
   // Persistent: workgroup g takes the contiguous tiles [g * chunk, (g + 1) *
   // chunk) one after another. A child of M >= 64 subregions (M a multiple of
   // 64) covers whole tiles, so wave 0 folds each tile into a running bound of
@@ -686,6 +693,16 @@ std::string items_source(const tape::Tape& t, bool aggregate)
   // (or the item) changes; smaller children take the per-tile segmented
   // reduction. Min and max are exact, so neither the fold order nor the
   // atomics' order changes a bit.
+  if (aggregate) {
+    // one child's running bound, reduced over the wave and folded into the
+    // child with one atomic min and max
+    o << "__device__ __forceinline__ void flush(const CmiItem* items, unsigned item, unsigned long child, double lo,\n"
+         "                                      double hi, unsigned lane)\n{\n"
+         "#pragma unroll\n  for (int off = 32; off > 0; off >>= 1) {\n"
+         "    lo = fmin(lo, __shfl_xor(lo, off, 64));\n    hi = fmax(hi, __shfl_xor(hi, off, 64));\n  }\n"
+         "  if (lane == 0u) {\n    atomicMin(items[item].child_lb + child, lo);\n"
+         "    atomicMax(items[item].child_ub + child, hi);\n  }\n}\n\n";
+  }
   o << "extern \"C\" __global__ void __launch_bounds__(" << 64 * w << ") " << kItemsKernel
     << "(const CmiItem* __restrict__ items, const unsigned* __restrict__ tile_item, unsigned n_tiles)\n{\n";
   o << "  __shared__ double2 X[" << nv * 64 << "];\n";
@@ -697,6 +714,8 @@ std::string items_source(const tape::Tape& t, bool aggregate)
   o << "  const unsigned first = blockIdx.x * chunk;\n";
   o << "  const unsigned last = first + chunk < n_tiles ? first + chunk : n_tiles;\n";
   o << "  const unsigned cur = 0u;\n";
+  o << "  unsigned meta_item = 0xffffffffu, m_count = 0u, m_var = 0u, m_width = 1u, m_kind = 0u;\n"
+       "  double m_step = 0.0, m_plo = 0.0, m_phi = 0.0;\n";
   if (aggregate) {
     o << "  unsigned run_item = 0xffffffffu;\n  unsigned long run_child = 0ul;\n"
          "  double run_lo = __builtin_huge_val(), run_hi = -__builtin_huge_val();\n";
@@ -707,14 +726,25 @@ std::string items_source(const tape::Tape& t, bool aggregate)
   o << "    const unsigned long e = (unsigned long)(tile - it.tile_begin) * 64ul + lane;\n";
   o << "    const bool active = e < it.n_elems;\n";
   o << "    const unsigned region = (unsigned)(active ? e : it.n_elems - 1ul);\n";
+  // Wave 0 decodes. When the item changes it writes every variable's parent
+  // bounds and loads the item's slots, lane j holding slot j; each tile
+  // then rewrites only the slot variables, reading the slot data across
+  // lanes rather than from memory.
   o << "    if (wave == 0u) {\n";
-  o << "      for (unsigned v = 0; v < " << nv << "u; ++v) {\n"
-       "        X[v * 64u + lane] = make_double2(-it.parent[2u * v], it.parent[2u * v + 1u]);\n      }\n";
+  o << "      if (item != meta_item) {\n        meta_item = item;\n        m_count = it.slot_count;\n";
+  o << "        for (unsigned v = 0; v < " << nv << "u; ++v) {\n"
+       "          X[v * 64u + lane] = make_double2(-it.parent[2u * v], it.parent[2u * v + 1u]);\n        }\n";
+  o << "        if (lane < m_count) {\n          m_var = it.slot_var[lane];\n          m_width = it.slot_width[lane];\n"
+       "          m_kind = it.slot_kind[lane];\n          m_step = it.slot_step[lane];\n"
+       "          m_plo = it.parent[2u * m_var];\n          m_phi = it.parent[2u * m_var + 1u];\n        }\n      }\n";
   o << "      unsigned idx = region;\n";
-  o << "      for (unsigned j = 0; j < it.slot_count; ++j) {\n"
-       "        const unsigned fw = it.slot_width[j];\n        const unsigned part = idx % fw;\n        idx /= fw;\n"
-       "        const unsigned vid = it.slot_var[j];\n        double lo, hi;\n"
-       "        slot_bounds(it.slot_kind[j], it.parent[2u * vid], it.parent[2u * vid + 1u], part, fw, lo, hi);\n"
+  o << "      for (unsigned j = 0; j < m_count; ++j) {\n"
+       "        const unsigned fw = __builtin_amdgcn_readlane(m_width, j);\n        unsigned part;\n"
+       "        if ((fw & (fw - 1u)) == 0u) {\n          part = idx & (fw - 1u);\n          idx >>= __builtin_ctz(fw);\n"
+       "        } else {\n          part = idx % fw;\n          idx /= fw;\n        }\n"
+       "        const unsigned vid = __builtin_amdgcn_readlane(m_var, j);\n        double lo, hi;\n"
+       "        slot_bounds((unsigned char)__builtin_amdgcn_readlane(m_kind, j), lane_value(m_plo, j),\n"
+       "                    lane_value(m_phi, j), lane_value(m_step, j), part, fw, lo, hi);\n"
        "        X[vid * 64u + lane] = make_double2(-lo, hi);\n      }\n    }\n";
   if (w > 1) {
     o << "    __syncthreads();\n";
@@ -741,15 +771,13 @@ std::string items_source(const tape::Tape& t, bool aggregate)
     o << "      const bool live = active && all != 0u;\n";
     o << "      double lo = live ? obj_s[lane] : __builtin_huge_val();\n";
     o << "      double hi = live ? obj_s[64u + lane] : -__builtin_huge_val();\n";
+    // Each lane keeps its own running bound of the current child; the lanes
+    // meet only when the child changes.
     o << "      if ((it.refine & 63ul) == 0ul) {\n";
-    o << "#pragma unroll\n        for (int off = 32; off > 0; off >>= 1) {\n"
-         "          lo = fmin(lo, __shfl_xor(lo, off, 64));\n          hi = fmax(hi, __shfl_xor(hi, off, 64));\n"
-         "        }\n";
     o << "        const unsigned long child = (unsigned long)(tile - it.tile_begin) * 64ul / it.refine;\n";
     o << "        if (item != run_item || child != run_child) {\n"
-         "          if (run_item != 0xffffffffu && lane == 0u) {\n"
-         "            atomicMin(items[run_item].child_lb + run_child, run_lo);\n"
-         "            atomicMax(items[run_item].child_ub + run_child, run_hi);\n          }\n"
+         "          if (run_item != 0xffffffffu) {\n            flush(items, run_item, run_child, run_lo, run_hi, lane);\n"
+         "          }\n"
          "          run_item = item;\n          run_child = child;\n          run_lo = __builtin_huge_val();\n"
          "          run_hi = -__builtin_huge_val();\n        }\n"
          "        run_lo = fmin(run_lo, lo);\n        run_hi = fmax(run_hi, hi);\n";
@@ -769,9 +797,8 @@ std::string items_source(const tape::Tape& t, bool aggregate)
   o << "    }\n";
   o << "  }\n";
   if (aggregate) {
-    o << "  if (wave == 0u && run_item != 0xffffffffu && lane == 0u) {\n"
-         "    atomicMin(items[run_item].child_lb + run_child, run_lo);\n"
-         "    atomicMax(items[run_item].child_ub + run_child, run_hi);\n  }\n";
+    o << "  if (wave == 0u && run_item != 0xffffffffu) {\n"
+         "    flush(items, run_item, run_child, run_lo, run_hi, lane);\n  }\n";
   }
   o << "}\n";
   return o.str();
@@ -823,10 +850,12 @@ std::shared_ptr<CompiledItems> compile_items(const tape::Tape& t, int device, bo
   int cus = 0;
   hip_check(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device), "hipDeviceGetAttribute(CUs)");
   ci->n_cus_ = static_cast<std::uint32_t>(std::max(cus, 1));
-  // resident workgroups per CU: by LDS, and at most 32 waves
+  // resident workgroups per CU: as many as LDS and 32 waves per CU allow, so
+  // that each SIMD has several waves to switch between while one waits on
+  // its FP64 and LDS latencies
   std::uint64_t const lds = std::uint64_t {t.n_vars} * 64 * 16 + std::uint64_t {ci->waves_} * 64 * 4 + 128 * 8;
   ci->wgs_per_cu_ = static_cast<std::uint32_t>(
-      std::clamp<std::uint64_t>(std::min<std::uint64_t>(kLdsBytes / lds, 32 / ci->waves_), 1, 8));
+      std::clamp<std::uint64_t>(std::min<std::uint64_t>(kLdsBytes / lds, 32 / ci->waves_), 1, 32));
   std::lock_guard<std::mutex> lock(mu);
   if (auto it = cache.find(key); it != cache.end()) {
     if (auto sp = it->second.lock();
