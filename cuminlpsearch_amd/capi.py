@@ -5,7 +5,9 @@ library first with ``python -m cuminlpsearch_amd.build``.
 """
 from __future__ import annotations
 
+import atexit
 import ctypes as C
+import weakref
 import pathlib
 
 import numpy as np
@@ -319,6 +321,30 @@ _SIGNATURES = {
 EXPORTED = tuple(_SIGNATURES)
 
 _lib = None
+
+# Engine objects (contexts, tapes, roles, problems) in creation order, so that
+# whatever is still alive when the interpreter exits is released then, newest
+# first, while the HIP runtime (and a profiler attached to it) is still up,
+# rather than by finalisers that may run after either has shut down.
+_owners: list = []
+
+
+def track(owner) -> None:
+    """Register an object with a `_release()` method for release at exit."""
+    if len(_owners) % 1024 == 1023:
+        _owners[:] = [r for r in _owners if r() is not None]
+    _owners.append(weakref.ref(owner))
+
+
+@atexit.register
+def _release_at_exit() -> None:
+    if _lib is None:
+        return
+    for r in reversed(_owners):
+        o = r()
+        if o is not None:
+            o._release()
+    _owners.clear()
 
 
 def lib() -> C.CDLL:

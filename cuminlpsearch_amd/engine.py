@@ -27,6 +27,7 @@ class EngineProblem:
 
     def __init__(self, handle: C.c_void_p):
         self._h = handle
+        _c.track(self)
 
     @classmethod
     def from_arrays(cls, a: ProblemArrays) -> "EngineProblem":
@@ -118,10 +119,13 @@ class EngineProblem:
     def handle(self) -> C.c_void_p:
         return self._h
 
-    def __del__(self):
+    def _release(self):
         if getattr(self, "_h", None) and _c._lib is not None:
             _c.lib().cmi_problem_destroy(self._h)
             self._h = None
+
+    def __del__(self):
+        self._release()
 
 
 def qcqp_spec(n=4, m=2, n_int=0, cont_lb=-10.0, cont_ub=10.0, int_lb=0.0, int_ub=10.0, pair_density=1.0,
@@ -136,16 +140,20 @@ class Context:
 
     def __init__(self, device: int = 0):
         self._h = _handle()
+        _c.track(self)
         _c.check(_c.lib().cmi_ctx_create(device, C.byref(self._h)))
 
     @property
     def handle(self) -> C.c_void_p:
         return self._h
 
-    def __del__(self):
+    def _release(self):
         if getattr(self, "_h", None) and _c._lib is not None:
             _c.lib().cmi_ctx_destroy(self._h)
             self._h = None
+
+    def __del__(self):
+        self._release()
 
 
 class Tape:
@@ -155,6 +163,7 @@ class Tape:
         self.ctx = ctx
         self.problem = problem
         self._h = _handle()
+        _c.track(self)
         _c.check(_c.lib().cmi_tape_create(ctx.handle, problem.handle, C.byref(self._h)))
 
     def info(self) -> _c.TapeInfo:
@@ -228,10 +237,13 @@ class Tape:
     def handle(self) -> C.c_void_p:
         return self._h
 
-    def __del__(self):
+    def _release(self):
         if getattr(self, "_h", None) and _c._lib is not None:
             _c.lib().cmi_tape_destroy(self._h)
             self._h = None
+
+    def __del__(self):
+        self._release()
 
 
 class AggregateBounder:
@@ -242,6 +254,7 @@ class AggregateBounder:
         self.k = k
         self.slot_kinds = np.ascontiguousarray(slot_kinds, np.uint8)
         self._h = _handle()
+        _c.track(self)
         self.facts = _c.OverBudget()
         _c.check(_c.lib().cmi_aggregate_build(tape.handle, _c.ptr(self.slot_kinds), len(self.slot_kinds), n_regions,
                                               k, max_batch, budget_bytes, C.byref(self._h), C.byref(self.facts)))
@@ -266,10 +279,13 @@ class AggregateBounder:
     def size(self) -> int:
         return int(_c.lib().cmi_role_size(self._h))
 
-    def __del__(self):
+    def _release(self):
         if getattr(self, "_h", None) and _c._lib is not None:
             _c.lib().cmi_role_destroy(self._h)
             self._h = None
+
+    def __del__(self):
+        self._release()
 
 
 @dataclass
@@ -288,6 +304,7 @@ class Role:
         self.kind = kind
         self.slot_kinds = np.ascontiguousarray(slot_kinds, np.uint8)
         self._h = _handle()
+        _c.track(self)
         self.facts = _c.OverBudget()
         _c.check(_c.lib().cmi_role_build(tape.handle, kind, _c.ptr(self.slot_kinds), len(self.slot_kinds),
                                          n_regions, samples, budget_bytes, C.byref(self._h), C.byref(self.facts)))
@@ -327,10 +344,13 @@ class Role:
     def size(self) -> int:
         return _c.lib().cmi_role_size(self._h)
 
-    def __del__(self):
+    def _release(self):
         if getattr(self, "_h", None) and _c._lib is not None:
             _c.lib().cmi_role_destroy(self._h)
             self._h = None
+
+    def __del__(self):
+        self._release()
 
 
 def solve_options(partition_num=2, enumerate_cap=2, max_cycle_size=0, sample_points=1, iter_limit=1000000,
