@@ -108,6 +108,11 @@ def test_ex2_1_1_brackets_its_optimum(ctx):
     same_outcome(d, o)
     assert d.lower_bound <= -17.0 <= d.upper_bound
     assert d.upper_bound - d.lower_bound < 0.5
+    # to gap 1e-6 with the k-child search, on the published optimum -17
+    a, _, _ = cmi.aggregate_solve(ctx, pm.problem, partition_budget=1 << 16, sample_fan_out=1 << 10, sample_points=4,
+                                  iter_limit=3000)
+    assert a.proven_optimal
+    assert abs(a.upper_bound - (-17.0)) <= 1e-6 * 17.0 and a.lower_bound <= a.upper_bound
 
 
 def test_corpus_batch_equals_plain(ctx):
