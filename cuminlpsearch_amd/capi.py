@@ -39,7 +39,7 @@ CONTINUOUS, INTEGER, BINARY = range(3)
 LE, EQ = range(2)
 SLOT_CONTINUOUS, SLOT_INTEGER_PARTITION, SLOT_INTEGER_ENUMERATE, SLOT_BINARY_ENUMERATE = range(4)
 ROLE_SAMPLER, ROLE_BOUNDER, ROLE_ENUMERATOR = range(3)
-POLICY_GREEDY_ENUMERATE, POLICY_BISECTION_BUDGET, POLICY_RELATIVE_BISECTION_BUDGET = range(3)
+POLICY_GREEDY_ENUMERATE, POLICY_BISECTION_BUDGET, POLICY_RELATIVE_BISECTION_BUDGET, POLICY_WIDTH_FIRST = range(4)
 SELECT_LEAST_LOWER_BOUND, SELECT_REJECT_INDEX, SELECT_REJECT_INDEX_STALE = range(3)
 STOP_NAMES = ["converged", "exhausted", "iteration-limit", "host-memory", "allocation-failure", "device-memory"]
 

@@ -294,6 +294,9 @@ std::shared_ptr<const cmi::policy::CompositionPolicy> make_policy(const cmi_solv
       cal.enumerate_cap = o.enumerate_cap;
       return std::make_shared<cmi::policy::BisectionBudgetCompositionPolicy>(
           o.bisection_budget, cal, o.policy == CMI_POLICY_RELATIVE_BISECTION_BUDGET);
+    case CMI_POLICY_WIDTH_FIRST:
+      return std::make_shared<cmi::policy::WidthFirstCompositionPolicy>(
+          cmi::region::FanOutSpec(o.partition_num, o.enumerate_cap), cal);
     default: break;
   }
   throw cmi::InvalidConfiguration("unknown composition policy " + std::to_string(o.policy));

@@ -173,7 +173,10 @@ struct ShapeRules
   std::size_t cycle_pinned = 1;
 };
 
-/// The GreedyEnumerate rows of the reference's roster (catalogue.hpp:87-143).
+/// The rows of the reference's roster (catalogue.hpp:87-149): five
+/// GreedyEnumerate rows that select_rules chooses among by position, and
+/// width-first, reachable only by name, which runs the WidthFirst policy on
+/// one slot per iteration.
 inline constexpr ShapeRules kRoster[] = {
     {"all-binary", ShapeRules::Partition::Pin, 2, ShapeRules::Enumerate::Pin, 0, 2, 1,
      ShapeRules::Cycle::FitDevice, 20},
@@ -185,6 +188,8 @@ inline constexpr ShapeRules kRoster[] = {
      ShapeRules::Cycle::FitDevice, 4},
     {"mixed-all-large", ShapeRules::Partition::FitToCoverage, 0, ShapeRules::Enumerate::CoverDomains, 16, 0, 3,
      ShapeRules::Cycle::FitDevice, 4},
+    {"width-first", ShapeRules::Partition::FitToCoverage, 0, ShapeRules::Enumerate::CoverDomains, 16, 0, 10,
+     ShapeRules::Cycle::Pin, 1},
 };
 
 inline std::optional<ShapeRules> lookup_rules(std::string_view name)

@@ -17,7 +17,7 @@
 // policy must be a pure function of its inputs, because a pending node's box
 // is rebuilt later by asking the policy again about the parent box.
 //
-// GreedyEnumerate (reference policy/greedy_enum.hpp:54-186) is the one built:
+// GreedyEnumerate (reference policy/greedy_enum.hpp:54-186), the default:
 // live binaries in variable order, then live integers with the fewest lattice
 // points first (enumerated when they have at most enumerate_cap points,
 // partitioned otherwise), then live continuous variables widest first, until
@@ -160,6 +160,100 @@ public:
         return out;
       }
       add(v, region::SlotKind::Continuous);
+    }
+    return out;
+  }
+
+private:
+  region::FanOutSpec fan_out_;
+};
+
+/// WidthFirst (reference policy/width_first.hpp): every live variable is a
+/// candidate, the widest first, until max_cycle_size slots are filled. A
+/// continuous variable partitions; an integer enumerates only when it is the
+/// unique widest live variable and has at most enumerate_cap lattice points,
+/// and partitions otherwise; a binary joins only once every other live
+/// variable is at most 1 wide or is an integer that enumerates this
+/// iteration. Widths follow the slot kinds (partition_num, enumerate_cap, 2).
+/// The order among equal widths is std::sort's over the ascending variable
+/// order, kept because it decides which child is which.
+class WidthFirstCompositionPolicy : public CompositionPolicy
+{
+public:
+  explicit WidthFirstCompositionPolicy(region::FanOutSpec fan_out, SearchCalibration calibration = {})
+      : CompositionPolicy(calibration)
+      , fan_out_(fan_out)
+  {
+  }
+
+  const region::FanOutSpec& fan_out() const { return fan_out_; }
+
+  std::size_t n_regions(const region::Composition& composition) const override
+  {
+    return region::composition_fan_out(composition, fan_out_);
+  }
+
+  region::SlotAssignment choose(std::span<const interval> box,
+                                std::span<const model::VarKind> var_kinds) const override
+  {
+    if (var_kinds.empty()) {
+      throw ShapeMismatch("a composition cannot be chosen for a problem without variables");
+    }
+    if (box.size() != var_kinds.size()) {
+      throw ShapeMismatch("the box has " + std::to_string(box.size()) + " coordinates for "
+                          + std::to_string(var_kinds.size()) + " variables");
+    }
+    auto width = [&](std::size_t v) { return box[v].ub - box[v].lb; };
+    std::vector<std::size_t> live;
+    double widest = 0.0;
+    for (std::size_t v = 0; v < box.size(); ++v) {
+      if (box[v].ub > box[v].lb) {
+        live.push_back(v);
+        widest = std::max(widest, width(v));
+      }
+    }
+    std::size_t at_widest = 0;
+    for (std::size_t v : live) {
+      at_widest += width(v) < widest ? 0 : 1;
+    }
+    auto enumerates = [&](std::size_t v) {
+      return var_kinds[v] == model::VarKind::Integer && at_widest == 1 && !(width(v) < widest)
+             && GreedyEnumCompositionPolicy::integer_domain_size(box[v]) <= fan_out_.enumerate_cap();
+    };
+    auto binary_joins = [&](std::size_t v) {
+      return std::all_of(live.begin(), live.end(),
+                         [&](std::size_t u) { return u == v || !(width(u) > 1.0) || enumerates(u); });
+    };
+    struct Candidate
+    {
+      std::size_t var;
+      region::SlotKind kind;
+      double width;
+    };
+    std::vector<Candidate> cand;
+    for (std::size_t v : live) {
+      switch (var_kinds[v]) {
+        case model::VarKind::Continuous: cand.push_back({v, region::SlotKind::Continuous, width(v)}); break;
+        case model::VarKind::Integer:
+          cand.push_back(
+              {v, enumerates(v) ? region::SlotKind::IntegerEnumerate : region::SlotKind::IntegerPartition, width(v)});
+          break;
+        case model::VarKind::Binary:
+          if (binary_joins(v)) {
+            cand.push_back({v, region::SlotKind::BinaryEnumerate, width(v)});
+          }
+          break;
+      }
+    }
+    std::sort(cand.begin(), cand.end(), [](const Candidate& a, const Candidate& b) { return a.width > b.width; });
+    region::SlotAssignment out;
+    for (const Candidate& c : cand) {
+      if (out.size() >= max_cycle_size()) {
+        break;
+      }
+      out.composition.kinds.push_back(c.kind);
+      out.var_ids.push_back(c.var);
+      out.widths.push_back(region::slot_fan_out(c.kind, fan_out_));
     }
     return out;
   }

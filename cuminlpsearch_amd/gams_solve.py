@@ -7,7 +7,9 @@ The search shape comes from the memory-fit resolver, as in the reference
 solver (source/gams/solve.cu:640-800): rules selected by the model's variable
 kinds (or named with --policy), fitted to the device's free memory (or to
 --device-budget), with any of --partition-num, --enumerate-cap, --max-slots
-and --sample-points laid over the result. The chosen shape is printed as a
+and --sample-points laid over the result. --policy bisection-budget (or
+relative-bisection-budget) with --bisection-budget B runs 2^B regions per
+node instead (solve.cu:643-712). The chosen shape is printed as a
 ``policy:`` line and a machine-readable ``PARAMS`` line.
 
 The output keeps the reference solver's machine-readable contract
@@ -34,12 +36,17 @@ ROSTER_NOTES = {
     "mixed-binary": "continuous and binaries: enumerate width follows the partition",
     "mixed-all-small": "integers and continuous, at most 64 variables: 10 samples",
     "mixed-all-large": "integers and continuous, more than 64 variables: 3 samples",
+    "width-first": "named only: the single widest variable per iteration, 10 samples",
 }
+BISECTION_POLICIES = {"bisection-budget": _c.POLICY_BISECTION_BUDGET,
+                      "relative-bisection-budget": _c.POLICY_RELATIVE_BISECTION_BUDGET}
 
 
 def print_roster(out) -> None:
     for name in resolve.ROSTER:
         print(f"  {name:16s} {ROSTER_NOTES[name]}", file=out)
+    print("  bisection-budget, relative-bisection-budget: 2^B regions per node (--bisection-budget B; 0 or "
+          "absent fits B to the device)", file=out)
 
 
 def main(argv: list[str] | None = None) -> int:
@@ -48,6 +55,8 @@ def main(argv: list[str] | None = None) -> int:
     ap.add_argument("iterations", type=int, nargs="?")
     ap.add_argument("--policy", default=None, help="roster rules by name (default: selected by variable kinds)")
     ap.add_argument("--list-policies", action="store_true")
+    ap.add_argument("--bisection-budget", type=int, default=None,
+                    help="B for --policy bisection-budget / relative-bisection-budget (0 = fit to the device)")
     ap.add_argument("--partition-num", type=int, default=None)
     ap.add_argument("--enumerate-cap", type=int, default=None)
     ap.add_argument("--max-slots", type=int, default=None, help="slots (variables) split per iteration")
@@ -69,7 +78,12 @@ def main(argv: list[str] | None = None) -> int:
         ap.print_usage(sys.stderr)
         print("a model file and an iteration limit are required", file=sys.stderr)
         return 2
-    if a.policy is not None and a.policy not in resolve.ROSTER:
+    bisection = a.policy in BISECTION_POLICIES
+    if a.bisection_budget is not None and not bisection:
+        print("--bisection-budget only applies under --policy bisection-budget or relative-bisection-budget",
+              file=sys.stderr)
+        return 2
+    if a.policy is not None and a.policy not in resolve.ROSTER and not bisection:
         print(f"unknown policy '{a.policy}'; available policies:", file=sys.stderr)
         print_roster(sys.stderr)
         return 2
@@ -90,6 +104,25 @@ def main(argv: list[str] | None = None) -> int:
     try:
         ctx = engine.Context(a.device)
         free = a.device_budget if a.device_budget is not None else resolve.device_memory(ctx)[0]
+        if bisection:
+            # source/gams/solve.cu:684-712: 10 samples unless given, every slot
+            # available, the enumerate cap as the enumeration ceiling, B given
+            # or fitted to the device
+            samples = a.sample_points or 10
+            b = a.bisection_budget or resolve.bisection_budget_fit(pm.problem, samples, free)
+            print(f"policy: {a.policy} (named), bisection_budget: {b}, sample_points: {samples}")
+            print(f"PARAMS\tpolicy={a.policy}\tsource=named\tbisection_budget={b}\tsample_points={samples}")
+            sys.stdout.flush()
+            try:
+                out, best = engine.solve(ctx, pm.problem, policy=BISECTION_POLICIES[a.policy], bisection_budget=b,
+                                         enumerate_cap=a.enumerate_cap or 0, max_cycle_size=0,
+                                         sample_points=samples, iter_limit=a.iterations, tolerance=a.tolerance,
+                                         known_primal=known)
+            except _c.CmiError as e:
+                print(f"{'out of device memory' if e.status == _c.OVER_BUDGET else 'configuration error'}: {e}",
+                      file=sys.stderr)
+                return 3 if e.status == _c.OVER_BUDGET else 2
+            return report(pm, out, best, None)
         shape = resolve.resolve_shape(pm.problem, rules=a.policy, objvar_kept=pm.objvar_kept, free_device_bytes=free,
                                       partition_num=a.partition_num or 0, enumerate_cap=a.enumerate_cap or 0,
                                       sample_points=a.sample_points or 0, max_cycle_size=a.max_slots or 0)
@@ -109,7 +142,11 @@ def main(argv: list[str] | None = None) -> int:
               file=sys.stderr)
         return 3 if e.status == _c.OVER_BUDGET else 2
 
-    if out.has_over_budget:
+    return report(pm, out, best, shape)
+
+
+def report(pm, out, best, shape) -> int:
+    if out.has_over_budget and shape is not None:
         # a composition first reached mid-search did not fit; the bracket so
         # far is still reported (reference search/driver.hpp:408-416)
         print("out of device memory: " + resolve.explain_over_budget(

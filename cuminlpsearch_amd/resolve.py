@@ -8,7 +8,7 @@ model; nothing runs on a device). Mirrors the reference's config layer:
 ``auto_max_cycle_size`` is config/footprint.hpp:99-149, ``resolve_shape`` is
 select_policy + resolve_shape + resolve (config/catalogue.hpp:182-245,
 config/resolve.hpp:95-218, config/run_spec.hpp:122-166). ``ROSTER`` is the
-GreedyEnumerate rows of policy_roster (config/catalogue.hpp:87-143).
+rows of policy_roster (config/catalogue.hpp:87-149).
 """
 from __future__ import annotations
 
@@ -17,7 +17,7 @@ from dataclasses import dataclass
 
 from . import capi as _c
 
-ROSTER = ("all-binary", "discrete", "mixed-binary", "mixed-all-small", "mixed-all-large")
+ROSTER = ("all-binary", "discrete", "mixed-binary", "mixed-all-small", "mixed-all-large", "width-first")
 BUDGET_FRACTION = 0.67
 MAX_SLOTS = 64
 
@@ -47,15 +47,33 @@ class ResolvedShape:
     rules: str
 
     def solve_options(self) -> dict:
-        """Keyword arguments of engine.solve for this shape."""
-        return dict(partition_num=self.partition_num, enumerate_cap=self.enumerate_cap,
+        """Keyword arguments of engine.solve for this shape (the width-first
+        row runs the WidthFirst policy; every other row GreedyEnumerate)."""
+        opts = dict(partition_num=self.partition_num, enumerate_cap=self.enumerate_cap,
                     sample_points=self.sample_points, max_cycle_size=self.max_cycle_size)
+        if self.rules == "width-first":
+            opts["policy"] = _c.POLICY_WIDTH_FIRST
+        return opts
 
     def params_line(self) -> str:
         """The machine-readable shape line (reference source/gams/solve.cu:780-800)."""
         return (f"PARAMS\tpolicy={self.rules}\tsource={self.source}\tpartition_num={self.partition_num}"
                 f"\tenumerate_cap={self.enumerate_cap}\tsample_points={self.sample_points}"
                 f"\tmax_slots={self.max_cycle_size}")
+
+
+def bisection_budget_fit(problem, sample_points: int, free_device_bytes: int) -> int:
+    """B for the bisection-budget policies when none is given: the largest
+    B whose 2^B regions fit the budget fraction of the free bytes at the
+    reference graph backend's per-region cost -- sampler (8 b + 25) B per
+    sample, bounder 16 b + 25, enumerator 8 b + 25 for b buffer nodes
+    (backend/graph/cost.hpp:70-137, source/gams/solve.cu:703-711); at least 1."""
+    b = problem.info().reachable_nodes
+    per_region = sample_points * (8 * b + 25) + (16 * b + 25) + (8 * b + 25)
+    budget = int(free_device_bytes * BUDGET_FRACTION)
+    if budget < per_region:
+        return 1
+    return max(1, (budget // per_region).bit_length() - 1)
 
 
 def profile(problem, objvar_kept: bool = False) -> ProblemProfile:

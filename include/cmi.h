@@ -263,9 +263,10 @@ typedef struct
   int has_known_primal;
   double known_primal;
   /* Composition policy (reference policy/policy.hpp PolicyKind): GreedyEnumerate
-   * over partition_num/enumerate_cap, or BisectionBudget with 2^bisection_budget
-   * regions per node (policy/bisection_budget.hpp), for which enumerate_cap is
-   * the largest integer domain enumerated (0 = no ceiling) and partition_num is
+   * or WidthFirst over partition_num/enumerate_cap (policy/greedy_enum.hpp,
+   * policy/width_first.hpp), or BisectionBudget with 2^bisection_budget regions
+   * per node (policy/bisection_budget.hpp), for which enumerate_cap is the
+   * largest integer domain enumerated (0 = no ceiling) and partition_num is
    * not read. */
   int policy;               /* cmi_policy */
   size_t bisection_budget;  /* B in [1, 62] for the bisection-budget policies */
@@ -275,7 +276,8 @@ typedef enum
 {
   CMI_POLICY_GREEDY_ENUMERATE = 0,
   CMI_POLICY_BISECTION_BUDGET = 1,
-  CMI_POLICY_RELATIVE_BISECTION_BUDGET = 2
+  CMI_POLICY_RELATIVE_BISECTION_BUDGET = 2,
+  CMI_POLICY_WIDTH_FIRST = 3
 } cmi_policy;
 
 /* SolveOutcome + RunTelemetry, reference search/driver.hpp:36-54 */

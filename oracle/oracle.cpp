@@ -1566,14 +1566,15 @@ int orc_solve_opts(const cmi_problem_desc* d, const cmi_solve_options* o, cmi_so
     std::size_t const max_slots = o->max_cycle_size == 0 ? 64 : o->max_cycle_size;
     std::size_t const part = o->partition_num, cap = o->enumerate_cap;
     bool const bisection = o->policy == CMI_POLICY_BISECTION_BUDGET || o->policy == CMI_POLICY_RELATIVE_BISECTION_BUDGET;
-    if (o->policy < 0 || o->policy > CMI_POLICY_RELATIVE_BISECTION_BUDGET
+    if (o->policy < 0 || o->policy > CMI_POLICY_WIDTH_FIRST
         || (bisection && (o->bisection_budget < 1 || o->bisection_budget > 62))) {
       throw std::runtime_error("oracle: bad composition policy");
     }
     auto compose = [&](const std::vector<Iv>& b) {
       return bisection ? choose_bisection(b, p.kind, o->bisection_budget, max_slots, cap,
                                           o->policy == CMI_POLICY_RELATIVE_BISECTION_BUDGET)
-                       : choose(b, p.kind, part, cap, max_slots);
+                       : o->policy == CMI_POLICY_WIDTH_FIRST ? choose_width_first(b, p.kind, part, cap, max_slots)
+                                                             : choose(b, p.kind, part, cap, max_slots);
     };
     double const tol = o->tolerance;
     double gub = o->has_known_primal ? o->known_primal : DBL_MAX;

@@ -94,6 +94,63 @@ inline Assign choose(const std::vector<Iv>& box, const std::vector<std::uint8_t>
   return a;
 }
 
+// ---- composition (WidthFirst, policy/width_first.hpp) -----------------------
+
+inline Assign choose_width_first(const std::vector<Iv>& box, const std::vector<std::uint8_t>& vk, std::size_t part,
+                                 std::size_t cap, std::size_t max_slots)
+{
+  std::vector<std::size_t> live;
+  double top = 0.0;
+  for (std::size_t v = 0; v < box.size(); ++v) {
+    if (box[v].ub > box[v].lb) {
+      live.push_back(v);
+      if (box[v].ub - box[v].lb > top) {
+        top = box[v].ub - box[v].lb;
+      }
+    }
+  }
+  std::size_t n_top = 0;
+  for (std::size_t v : live) {
+    n_top += !(box[v].ub - box[v].lb < top);
+  }
+  auto collapses = [&](std::size_t v) {
+    return vk[v] == CMI_INTEGER && n_top == 1 && !(box[v].ub - box[v].lb < top) && int_domain(box[v]) <= cap;
+  };
+  struct C
+  {
+    std::size_t v;
+    std::uint8_t k;
+    double w;
+  };
+  std::vector<C> c;
+  for (std::size_t v : live) {
+    double const w = box[v].ub - box[v].lb;
+    if (vk[v] == CMI_CONTINUOUS) {
+      c.push_back({v, CMI_SLOT_CONTINUOUS, w});
+    } else if (vk[v] == CMI_INTEGER) {
+      c.push_back({v, static_cast<std::uint8_t>(collapses(v) ? CMI_SLOT_INTEGER_ENUMERATE : CMI_SLOT_INTEGER_PARTITION), w});
+    } else {
+      bool ready = true;
+      for (std::size_t u : live) {
+        if (u != v && box[u].ub - box[u].lb > 1.0 && !collapses(u)) {
+          ready = false;
+        }
+      }
+      if (ready) {
+        c.push_back({v, CMI_SLOT_BINARY_ENUMERATE, w});
+      }
+    }
+  }
+  std::sort(c.begin(), c.end(), [](const C& a, const C& b) { return a.w > b.w; });
+  Assign a;
+  for (std::size_t i = 0; i < c.size() && a.vars.size() < max_slots; ++i) {
+    a.kinds.push_back(c[i].k);
+    a.vars.push_back(c[i].v);
+    a.widths.push_back(c[i].k == CMI_SLOT_BINARY_ENUMERATE ? 2 : c[i].k == CMI_SLOT_INTEGER_ENUMERATE ? cap : part);
+  }
+  return a;
+}
+
 // ---- composition (BisectionBudget, policy/bisection_budget.hpp:40-299) -----
 
 inline std::size_t bits_for(std::size_t d)

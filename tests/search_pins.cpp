@@ -189,6 +189,30 @@ struct Engine
     c.domains = a.domain_sizes;
     return c;
   }
+
+  static Choice widthfirst(std::vector<Box> box, std::vector<int> kinds, std::size_t part, std::size_t cap,
+                           std::size_t max_slots = 0)
+  {
+    cmi::policy::SearchCalibration cal;
+    cal.max_cycle_size = max_slots;
+    cmi::policy::WidthFirstCompositionPolicy pol(cmi::region::FanOutSpec(part, cap), cal);
+    std::vector<cmi::interval> b;
+    for (Box v : box) {
+      b.push_back({v.lb, v.ub});
+    }
+    std::vector<cmi::model::VarKind> k;
+    for (int v : kinds) {
+      k.push_back(static_cast<cmi::model::VarKind>(v));
+    }
+    auto const a = pol.choose(b, k);
+    Choice c;
+    for (auto sk : a.composition) {
+      c.kinds.push_back(static_cast<int>(sk));
+    }
+    c.vars = a.var_ids;
+    c.widths = a.widths;
+    return c;
+  }
 };
 
 struct Oracle
@@ -303,6 +327,22 @@ struct Oracle
     c.vars = a.vars;
     c.widths = a.widths;
     c.domains = a.domains;
+    return c;
+  }
+
+  static Choice widthfirst(std::vector<Box> box, std::vector<int> kinds, std::size_t part, std::size_t cap,
+                           std::size_t max_slots = 0)
+  {
+    std::vector<orc::Iv> b;
+    for (Box v : box) {
+      b.push_back({v.lb, v.ub});
+    }
+    std::vector<std::uint8_t> k(kinds.begin(), kinds.end());
+    auto const a = orc::choose_width_first(b, k, part, cap, max_slots == 0 ? 64 : max_slots);
+    Choice c;
+    c.kinds.assign(a.kinds.begin(), a.kinds.end());
+    c.vars = a.vars;
+    c.widths = a.widths;
     return c;
   }
 };
@@ -980,6 +1020,56 @@ void bisection_cases()
 }
 
 template<typename I>
+void widthfirst_cases()
+{
+  // policy/width_first.hpp, restated as properties (the reference has no
+  // dedicated tests for this policy)
+  g_case = "width-first: every live variable, widest first, partition widths";
+  {
+    auto const a = I::widthfirst({{0.0, 1.0}, {0.0, 100.0}, {2.0, 2.0}, {0.0, 3.0}}, {kCont, kCont, kCont, kCont}, 5, 3);
+    CHECK(a.vars == std::vector<std::size_t>({1, 3, 0}));
+    CHECK(a.widths == std::vector<std::size_t>({5, 5, 5}));
+    CHECK(a.kinds == std::vector<int>({sC, sC, sC}));
+  }
+  g_case = "width-first: the slot cap keeps the widest";
+  {
+    auto const a = I::widthfirst({{0.0, 1.0}, {0.0, 100.0}, {0.0, 3.0}}, {kCont, kCont, kCont}, 4, 4, 2);
+    CHECK(a.vars == std::vector<std::size_t>({1, 2}));
+  }
+  g_case = "width-first: a binary waits while another live variable is wider than 1";
+  {
+    auto const a = I::widthfirst({{0.0, 1.0}, {0.0, 5.0}}, {kBin, kCont}, 4, 4);
+    CHECK(a.vars == std::vector<std::size_t>({1}) && a.kinds == std::vector<int>({sC}));
+    auto const b = I::widthfirst({{0.0, 1.0}, {0.0, 0.5}}, {kBin, kCont}, 4, 4);
+    CHECK(b.vars == std::vector<std::size_t>({0, 1}) && b.kinds == std::vector<int>({sB, sC}));
+    CHECK(b.widths == std::vector<std::size_t>({2, 4}));
+  }
+  g_case = "width-first: the unique widest integer enumerates when its domain fits the cap";
+  {
+    auto const a = I::widthfirst({{0.0, 3.0}, {0.0, 1.0}}, {kInt, kCont}, 4, 4);
+    CHECK(a.kinds == std::vector<int>({sE, sC}) && a.widths == std::vector<std::size_t>({4, 4}));
+    auto const wide = I::widthfirst({{0.0, 9.0}, {0.0, 1.0}}, {kInt, kCont}, 4, 4);
+    CHECK(wide.kinds == std::vector<int>({sP, sC}));
+    auto const tie = I::widthfirst({{0.0, 3.0}, {5.0, 8.0}}, {kInt, kInt}, 4, 4);
+    CHECK(tie.kinds == std::vector<int>({sP, sP}));
+    auto const not_widest = I::widthfirst({{0.0, 3.0}, {0.0, 10.0}}, {kInt, kCont}, 4, 4);
+    CHECK(not_widest.vars == std::vector<std::size_t>({1, 0}) && not_widest.kinds == std::vector<int>({sC, sP}));
+  }
+  g_case = "width-first: a binary joins beside an integer that enumerates this iteration";
+  {
+    auto const a = I::widthfirst({{0.0, 1.0}, {0.0, 3.0}}, {kBin, kInt}, 4, 4);
+    CHECK(a.vars == std::vector<std::size_t>({1, 0}) && a.kinds == std::vector<int>({sE, sB}));
+    auto const b = I::widthfirst({{0.0, 1.0}, {0.0, 30.0}}, {kBin, kInt}, 4, 4);
+    CHECK(b.vars == std::vector<std::size_t>({1}) && b.kinds == std::vector<int>({sP}));
+  }
+  g_case = "width-first: a resolved box has no slots";
+  {
+    auto const a = I::widthfirst({{1.0, 1.0}, {0.0, 0.0}}, {kCont, kBin}, 4, 4);
+    CHECK(a.vars.empty());
+  }
+}
+
+template<typename I>
 void run_all()
 {
   g_impl = I::name;
@@ -991,6 +1081,7 @@ void run_all()
   witness_cases<I>();
   policy_cases<I>();
   bisection_cases<I>();
+  widthfirst_cases<I>();
   order_cases<I>();
 }
 
@@ -1199,11 +1290,45 @@ void engine_only()
 
 }  // namespace
 
+/// The two implementations choose alike on random boxes with many equal
+/// widths (std::sort's arrangement of ties included).
+void policy_agreement()
+{
+  g_impl = "both";
+  g_case = "engine and oracle width-first and greedy choices agree on random tie-heavy boxes";
+  std::uint64_t x = 12345;
+  auto next = [&] {
+    x = x * 6364136223846793005ull + 1442695040888963407ull;
+    return x >> 33;
+  };
+  bool same = true;
+  for (int t = 0; t < 3000; ++t) {
+    std::size_t const n = 1 + next() % 12;
+    std::vector<Box> box(n);
+    std::vector<int> kinds(n);
+    for (std::size_t v = 0; v < n; ++v) {
+      kinds[v] = static_cast<int>(next() % 3);
+      double const lo = static_cast<double>(next() % 5);
+      double const w = kinds[v] == kBin ? static_cast<double>(next() % 2) : static_cast<double>(next() % 4);
+      box[v] = {lo, lo + w};
+    }
+    std::size_t const slots = next() % 5;
+    auto const a = Engine::widthfirst(box, kinds, 3, 4, slots);
+    auto const b = Oracle::widthfirst(box, kinds, 3, 4, slots);
+    same = same && a.vars == b.vars && a.kinds == b.kinds && a.widths == b.widths;
+    auto const c = Engine::choose(box, kinds, 3, 4, slots);
+    auto const d = Oracle::choose(box, kinds, 3, 4, slots);
+    same = same && c.vars == d.vars && c.kinds == d.kinds && c.widths == d.widths;
+  }
+  CHECK(same);
+}
+
 int main()
 {
   run_all<Engine>();
   run_all<Oracle>();
   heap_agreement();
+  policy_agreement();
   engine_only();
   std::printf("%d checks, %d failed\n", g_checks, g_failed);
   return g_failed == 0 ? 0 : 1;

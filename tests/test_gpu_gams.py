@@ -223,3 +223,34 @@ def test_nvs04_discrete_run_record(ctx):
     assert np.array_equal(best_d.view(np.uint64), best_o.view(np.uint64))
     assert d.proven_optimal and d.iterations == 6
     assert f"{d.upper_bound:.12g}" == "0.72" and f"{d.lower_bound:.12g}" == "0.72"
+
+
+@pytest.mark.parametrize("name", ["ex2_1_1", "nvs04", "nvs01", "autocorr_bern20-03", "st_e01", "ex8_1_1", "circle"])
+def test_width_first_solve_matches_oracle(ctx, name):
+    """The WidthFirst policy (policy/width_first.hpp) on the device and in the
+    oracle: equal iterates, counters and incumbents."""
+    from cuminlpsearch_amd import capi as C
+    pm = parsed(name)
+    opts = dict(policy=C.POLICY_WIDTH_FIRST, partition_num=8, enumerate_cap=4, max_cycle_size=2, sample_points=16,
+                iter_limit=200)
+    print(f"solving {name} width-first", flush=True)
+    d, best_d = cmi.solve(ctx, pm.problem, **opts)
+    o, best_o = orc.solve(pm.problem.arrays(), **opts)
+    same_outcome(d, o)
+    assert np.array_equal(best_d.view(np.uint64), best_o.view(np.uint64))
+
+
+def test_width_first_roster_row_on_the_device(ctx):
+    """--policy width-first: one slot per iteration at the fitted partition."""
+    from cuminlpsearch_amd import resolve as R
+    pm = parsed("circle")
+    shape = R.resolve_shape(pm.problem, "width-first", free_device_bytes=8 << 30)
+    opts = shape.solve_options()
+    assert opts["max_cycle_size"] == 1 and opts["partition_num"] > 2
+    d, _ = cmi.solve(ctx, pm.problem, iter_limit=5, **opts)
+    assert d.lower_bound <= 4.57424899 + 1e-6
+    consistent_with_published("circle", d)
+    small = dict(opts, partition_num=64)
+    d, _ = cmi.solve(ctx, pm.problem, iter_limit=100, **small)
+    o, _ = orc.solve(pm.problem.arrays(), iter_limit=100, **small)
+    same_outcome(d, o)

@@ -50,6 +50,7 @@ APPLICABLE = {
     "mixed-binary": dict(n_bin=3, n_cont=2),
     "mixed-all-small": dict(n_int=2, n_cont=2),
     "mixed-all-large": dict(n_int=2, n_cont=70),
+    "width-first": dict(n_int=2, n_cont=2),
 }
 
 
@@ -59,7 +60,7 @@ def test_every_roster_row_resolves_by_name(name):
     assert s.rules == name and s.source == "named"
 
 
-@pytest.mark.parametrize("name", ["quadratic", "", "mixed", "width-first"])
+@pytest.mark.parametrize("name", ["quadratic", "", "mixed", "bisection-budget"])
 def test_unknown_rules_are_a_configuration_error(name):
     with pytest.raises(C.CmiError) as e:
         R.resolve_shape(build(n_bin=2), rules=name)
@@ -75,11 +76,22 @@ def test_unknown_rules_are_a_configuration_error(name):
     ("mixed-binary", (2, 2, 5, 4)),      # FollowPartition at the floor
     ("mixed-all-small", (2, 7, 10, 4)),
     ("mixed-all-large", (2, 7, 3, 4)),
+    ("width-first", (2, 7, 10, 1)),      # named only; one slot per iteration (catalogue.hpp:132-149)
 ])
 def test_roster_constants_without_a_budget(name, expect):
     s = R.resolve_shape(build(**APPLICABLE[name]), rules=name)
     assert (s.partition_num, s.enumerate_cap, s.sample_points, s.max_cycle_size) == expect
     assert s.budget_bytes == 0
+    assert s.solve_options().get("policy", C.POLICY_GREEDY_ENUMERATE) == (
+        C.POLICY_WIDTH_FIRST if name == "width-first" else C.POLICY_GREEDY_ENUMERATE)
+
+
+def test_width_first_is_never_selected_automatically():
+    """select_policy classifies into the first five rows only (catalogue.hpp:132-141)."""
+    for kinds in APPLICABLE.values():
+        assert R.resolve_shape(build(**kinds)).rules != "width-first"
+    s = R.resolve_shape(build(n_int=2, n_cont=2), rules="width-first", free_device_bytes=8 << 30)
+    assert s.max_cycle_size == 1 and s.partition_num > 2
 
 
 # -- profile and the resolver ----------------------------------------------------------
