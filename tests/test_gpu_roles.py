@@ -141,6 +141,36 @@ def test_enumerator_matches_oracle_and_brute_force(ctx):
     assert d.value == best
 
 
+@pytest.mark.parametrize("order", ["x_first", "y_first"])
+def test_argmin_takes_the_lowest_index_among_equal_minima(ctx, order):
+    """CUB ArgMin's rule (reference replay.cuh:439-474): among equal minima the
+    lowest index wins. The objective (x - 150)^2 does not depend on y, so every
+    lattice point with x = 150 is a minimum; brute force names the lowest
+    index, across tiles and the last-tile reduction."""
+    p = Problem()
+    x, y = p.int_var(0, 300), p.int_var(0, 199)
+    p.set_objective((x - 150.0) * (x - 150.0))
+    eng = cmi.EngineProblem.from_problem(p)
+    a = eng.arrays()
+    kinds = [C.SLOT_INTEGER_ENUMERATE] * 2
+    vids, widths = ([0, 1], [301, 200]) if order == "x_first" else ([1, 0], [200, 301])
+    role = cmi.Role(cmi.Tape(ctx, eng), C.ROLE_ENUMERATOR, kinds, 301 * 200)
+    d = role.enumerate(a.box, vids, widths)
+    values = {}
+    for r in range(301 * 200):
+        digits = (r % widths[0], r // widths[0])
+        point = [0.0, 0.0]
+        for j in range(2):
+            point[vids[j]] = float(digits[j])
+        values[r] = (point[0] - 150.0) ** 2
+    best = min(values.values())
+    lowest = min(r for r, v in values.items() if v == best)
+    assert d.found and d.value == 0.0 and d.index == lowest
+    assert list(d.point) == [150.0, 0.0]
+    found, value, index, point = orc.candidate(a, a.box, vids, widths, kinds, 0, 0)
+    assert index == lowest and value == 0.0
+
+
 def test_over_budget_is_reported_before_allocation(ctx):
     eng = P.c1(0)
     tape = cmi.Tape(ctx, eng)

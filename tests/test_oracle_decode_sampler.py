@@ -106,3 +106,25 @@ def test_continuous_draw_formula():
         assert -1.0 <= got <= 2.5
         if want is not None:
             assert got == want
+
+
+def test_oracle_argmin_takes_the_lowest_index_among_equal_minima():
+    """The oracle's enumerator breaks ties as CUB ArgMin does (lowest index;
+    reference replay.cuh:439-474), checked by brute force over a lattice where
+    a whole line of points ties: (x - 150)^2 does not depend on y."""
+    from cuminlpsearch_amd import capi as C
+    from cuminlpsearch_amd.model import Problem
+    from oracle import binding as orc
+    p = Problem()
+    x, y = p.int_var(0, 300), p.int_var(0, 199)
+    p.set_objective((x - 150.0) * (x - 150.0))
+    a = p.arrays()
+    for vids, widths, lowest in (([0, 1], [301, 200], 150), ([1, 0], [200, 301], 150 * 200)):
+        found, value, index, point = orc.candidate(a, a.box, vids, widths, [C.SLOT_INTEGER_ENUMERATE] * 2, 0, 0)
+        # brute force: the lexicographic minimum of (value, index)
+        def xval(r):
+            return r % widths[0] if vids[0] == 0 else r // widths[0]
+        best = min((xval(r) - 150.0) ** 2 for r in range(301 * 200))
+        ties = [r for r in range(301 * 200) if (xval(r) - 150.0) ** 2 == best]
+        assert found and value == 0.0 and index == min(ties) == lowest
+        assert list(point) == [150.0, 0.0]
