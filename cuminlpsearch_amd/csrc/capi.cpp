@@ -566,7 +566,7 @@ cmi::tape::Tape grouped_for_boxes(const cmi::model::Problem<double>& p, size_t g
 cmi::jit::Shape export_shape(const cmi::tape::Tape& g, const cmi_compiled_shape& in)
 {
   if (in.waves > 8) {
-    throw cmi::InvalidConfiguration("at most 8 waves per workgroup");
+    throw cmi::InvalidConfiguration("at most 16 waves per workgroup");
   }
   if (in.schedule != CMI_SCHED_DEFAULT && in.schedule != CMI_SCHED_MAX_ILP) {
     throw cmi::InvalidConfiguration("unknown compiled-kernel schedule");

@@ -901,13 +901,16 @@ inline Tape compile_groups(const model::Problem<double>& p, std::uint32_t n_grou
   return t;
 }
 
-/// Function groups chosen for explicit boxes: about one per 32 tape words, at
-/// most 8 (and at most one per function).
+/// Function groups chosen for explicit boxes and decode-mode items: about one
+/// per 32 tape words and at most one per function; at most 8, or 16 for a
+/// tape of 4,096 words or more, whose per-wave share is otherwise the
+/// critical path of a small decode-mode item (one 64-lane tile).
 inline std::uint32_t auto_groups(const Tape& t)
 {
   std::size_t const by_size = (t.code.size() + 31) / 32;
   std::size_t const roots = 1 + static_cast<std::size_t>(t.n_constraints);
-  return static_cast<std::uint32_t>(std::clamp<std::size_t>(std::min(by_size, roots), 1, 8));
+  std::size_t const cap = t.code.size() >= 4096 ? 16 : 8;
+  return static_cast<std::uint32_t>(std::clamp<std::size_t>(std::min(by_size, roots), 1, cap));
 }
 
 /// Human-readable listing, one instruction per line.

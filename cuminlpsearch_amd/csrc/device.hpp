@@ -14,7 +14,7 @@ namespace cmi::dev
 
 inline constexpr unsigned kTile = 64;          // elements per workgroup: one wave64
 inline constexpr unsigned kMaxSlotsPerItem = 64;
-inline constexpr unsigned kMaxGroups = 8;      // waves per explicit-box workgroup
+inline constexpr unsigned kMaxGroups = 16;     // waves (function groups) per workgroup
 
 /// Function groups of a grouped tape (tape::compile_groups): wave g of an
 /// explicit-box workgroup runs code[begin[g], begin[g+1]).

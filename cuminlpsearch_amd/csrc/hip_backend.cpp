@@ -249,7 +249,7 @@ void DeviceTape::set_compiled_shape(bool persistent, std::uint32_t waves, std::u
     throw InvalidConfiguration("workgroups per CU must be in [1, 8]");
   }
   if (waves > dev::kMaxGroups) {
-    throw InvalidConfiguration("at most 8 waves per workgroup");
+    throw InvalidConfiguration("at most 16 waves per workgroup");
   }
   want_persistent_ = persistent;
   want_waves_ = waves;

@@ -164,7 +164,7 @@ public:
 
   /// Recompile the explicit-box code of `problem` (the one this tape was
   /// built from) into k function groups (0 = automatic); k is clamped to
-  /// [1, min(8, 1 + n_constraints)] and lowered until the workgroup's slots
+  /// [1, min(16, 1 + n_constraints)] and lowered until the workgroup's slots
   /// fit in LDS. Not safe while a launch reads the code.
   void set_box_groups(const model::Problem<double>& problem, std::uint32_t k);
   std::uint32_t box_groups() const { return grouped_.n_groups; }

@@ -341,7 +341,7 @@ cmi_status cmi_tape_info_get(const cmi_tape* t, cmi_tape_info* out);
 void cmi_tape_destroy(cmi_tape* t);
 /* Split the explicit-box code of tape `t` (compiled from `p`) into k function
  * groups, one wave each over the same 64 boxes (0 = chosen from the tape's
- * size). k is clamped to [1, min(8, 1 + n_constraints)] and lowered until the
+ * size). k is clamped to [1, min(16, 1 + n_constraints)] and lowered until the
  * workgroup fits in LDS. Results do not depend on k. No reference
  * counterpart: the reference launches one kernel per DAG node
  * (backend/graph/builder.cuh:261-514); this is its wave-shape knob. Must not
@@ -433,7 +433,7 @@ cmi_status cmi_tape_compiled_object(const cmi_problem* p, const cmi_compiled_sha
  * non-NULL. Same checks and errors as cmi_tape_create. */
 cmi_status cmi_tape_compile_info(const cmi_problem* p, cmi_tape_info* out, char* listing, size_t cap);
 /* Host-only compile into k function groups (k <= 1: one group, the tape the
- * decode-mode roles run; k is clamped to min(8, 1 + n_constraints)). Writes
+ * decode-mode roles run; k is clamped to min(16, 1 + n_constraints)). Writes
  * the shape into `out` (box_groups = the k used), the 16-byte tape words into
  * `words` when non-NULL (SHAPE_MISMATCH if cap_words < out->box_instr) and
  * the box_groups + 1 group offsets into `group_begin` when non-NULL. Lets a
