@@ -3,6 +3,7 @@
 #include <algorithm>
 #include <atomic>
 #include <chrono>
+#include <cstdlib>
 #include <cstring>
 #include <exception>
 #include <functional>
@@ -69,7 +70,118 @@ struct Candidate
   double value;
   std::int64_t index;
 };
+
+/// Steps with fewer live solves than this run their per-solve host work on
+/// the calling thread: waking the workers costs more than it saves.
+constexpr std::size_t kParallelMembers = 256;
+
+/// Host threads of a runner's workers: CMI_BATCH_HOST_THREADS when it holds
+/// a count in [1, 64], else min(16, hardware threads).
+std::size_t host_threads()
+{
+  if (const char* v = std::getenv("CMI_BATCH_HOST_THREADS")) {
+    char* end = nullptr;
+    long const t = std::strtol(v, &end, 10);
+    if (end != v && *end == '\0' && t >= 1 && t <= 64) {
+      return static_cast<std::size_t>(t);
+    }
+  }
+  return std::clamp<std::size_t>(std::thread::hardware_concurrency(), 1, 16);
+}
 }  // namespace
+
+Workers::Workers(std::size_t threads)
+{
+  for (std::size_t t = 1; t < threads; ++t) {
+    threads_.emplace_back([this] { loop(); });
+  }
+}
+
+Workers::~Workers()
+{
+  {
+    std::lock_guard<std::mutex> lock(mu_);
+    stop_ = true;
+  }
+  wake_.notify_all();
+  for (std::thread& t : threads_) {
+    t.join();
+  }
+}
+
+void Workers::work()
+{
+  for (std::size_t i; (i = next_.fetch_add(1)) < n_;) {
+    try {
+      (*fn_)(i);
+    } catch (...) {
+      std::lock_guard<std::mutex> lock(mu_);
+      if (!error_) {
+        error_ = std::current_exception();
+      }
+      next_ = n_;
+    }
+  }
+}
+
+void Workers::loop()
+{
+  std::uint64_t seen = 0;
+  for (;;) {
+    {
+      std::unique_lock<std::mutex> lock(mu_);
+      wake_.wait(lock, [&] { return stop_ || generation_ != seen; });
+      if (stop_) {
+        return;
+      }
+      seen = generation_;
+    }
+    work();
+    std::lock_guard<std::mutex> lock(mu_);
+    if (--busy_ == 0) {
+      idle_.notify_one();
+    }
+  }
+}
+
+void Workers::run(std::size_t n, const std::function<void(std::size_t)>& fn)
+{
+  if (threads_.empty() || n <= 1) {
+    for (std::size_t i = 0; i < n; ++i) {
+      fn(i);
+    }
+    return;
+  }
+  {
+    std::lock_guard<std::mutex> lock(mu_);
+    fn_ = &fn;
+    n_ = n;
+    next_ = 0;
+    error_ = nullptr;
+    busy_ = threads_.size();
+    ++generation_;
+  }
+  wake_.notify_all();
+  work();
+  std::exception_ptr error;
+  {
+    std::unique_lock<std::mutex> lock(mu_);
+    idle_.wait(lock, [&] { return busy_ == 0; });
+    fn_ = nullptr;
+    error = error_;
+  }
+  if (error) {
+    std::rethrow_exception(error);
+  }
+}
+
+Workers& Runner::workers()
+{
+  if (!workers_) {
+    workers_ = std::make_unique<Workers>(host_threads());
+  }
+  return *workers_;
+}
 
 Runner::Runner(std::shared_ptr<hip::Context> ctx)
     : ctx_(std::move(ctx))
@@ -361,24 +473,31 @@ std::size_t step(Runner& runner, std::span<Member> members, const cmi_solve_opti
     std::size_t first_req;
     bool enumerate;
   };
-  std::vector<Request> reqs;
-  std::vector<Owner> owners;
-  reqs.reserve(2 * members.size());
-  owners.reserve(members.size());
-  for (Member& m : members) {
+  // What each live solve asks for this step. next(), the budget check and,
+  // after the launch, on_subdivide/on_enumerate touch only that solve's own
+  // state, so for a large step they run on the host workers; the requests
+  // are packed in member order either way.
+  struct Pending
+  {
+    const search::SolveState::Request* q = nullptr;
+    bool enumerate = false;
+  };
+  std::vector<Pending> pending(members.size());
+  auto ask = [&](std::size_t i) {
+    Member& m = members[i];
     search::SolveState& st = *m.state;
     if (st.finished()) {
-      continue;
+      return;
     }
     const search::SolveState::Request* q = nullptr;
     try {
       q = st.next();
     } catch (const std::bad_alloc&) {
       st.stop(search::StopReason::AllocationFailure);
-      continue;
+      return;
     }
     if (q == nullptr) {
-      continue;
+      return;
     }
     bool const enumerate = q->kind == search::SolveState::Kind::Enumerate;
     std::size_t const n_regions = region::composition_fan_out(q->assignment->widths);
@@ -397,13 +516,34 @@ std::size_t step(Runner& runner, std::span<Member> members, const cmi_solve_opti
       }
     } catch (const backend::OverBudgetError& e) {
       st.stop_over_budget(e.facts());
-      continue;
+      return;
     } catch (const ResourceExhausted&) {
       st.stop(search::StopReason::DeviceMemory);
+      return;
+    }
+    pending[i] = {q, enumerate};
+  };
+  bool const parallel = members.size() >= kParallelMembers;
+  if (parallel) {
+    runner.workers().run(members.size(), ask);
+  } else {
+    for (std::size_t i = 0; i < members.size(); ++i) {
+      ask(i);
+    }
+  }
+
+  std::vector<Request> reqs;
+  std::vector<Owner> owners;
+  reqs.reserve(2 * members.size());
+  owners.reserve(members.size());
+  for (std::size_t i = 0; i < members.size(); ++i) {
+    const search::SolveState::Request* q = pending[i].q;
+    if (q == nullptr) {
       continue;
     }
-    owners.push_back({&m, reqs.size(), enumerate});
-    if (enumerate) {
+    Member& m = members[i];
+    owners.push_back({&m, reqs.size(), pending[i].enumerate});
+    if (pending[i].enumerate) {
       reqs.push_back({m.tape, dev::Mode::Enumerate, q->box, q->assignment, q->salt, 1});
     } else {
       reqs.push_back({m.tape, dev::Mode::Sample, q->box, q->assignment, q->salt,
@@ -415,7 +555,8 @@ std::size_t step(Runner& runner, std::span<Member> members, const cmi_solve_opti
     return 0;
   }
   runner.run(reqs);
-  for (Owner const& w : owners) {
+  auto answer = [&](std::size_t k) {
+    Owner const& w = owners[k];
     search::SolveState& st = *w.member->state;
     try {
       if (w.enumerate) {
@@ -427,6 +568,13 @@ std::size_t step(Runner& runner, std::span<Member> members, const cmi_solve_opti
       }
     } catch (const std::bad_alloc&) {
       st.stop(search::StopReason::AllocationFailure);
+    }
+  };
+  if (parallel) {
+    runner.workers().run(owners.size(), answer);
+  } else {
+    for (std::size_t k = 0; k < owners.size(); ++k) {
+      answer(k);
     }
   }
   return reqs.size();

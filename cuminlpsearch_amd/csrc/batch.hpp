@@ -3,6 +3,12 @@
 #include <cstddef>
 #include <cstdint>
 #include <memory>
+#include <thread>
+#include <mutex>
+#include <functional>
+#include <exception>
+#include <condition_variable>
+#include <atomic>
 #include <span>
 #include <vector>
 
@@ -30,6 +36,36 @@ struct Request
 /// role kinds) into one host->device copy, at most three kernel launches (one
 /// per role kind, each covering every request of that kind) and one
 /// device->host copy. Buffers grow on demand and are reused across steps.
+/// Host threads started once and woken per call: fn(i) for i in [0, n) on
+/// them and the caller together. The first exception is rethrown after every
+/// thread has stopped working on the call.
+class Workers
+{
+public:
+  explicit Workers(std::size_t threads);
+  ~Workers();
+  Workers(const Workers&) = delete;
+  Workers& operator=(const Workers&) = delete;
+
+  void run(std::size_t n, const std::function<void(std::size_t)>& fn);
+  std::size_t size() const { return threads_.size() + 1; }
+
+private:
+  void loop();
+  void work();
+
+  std::mutex mu_;
+  std::condition_variable wake_, idle_;
+  std::vector<std::thread> threads_;
+  const std::function<void(std::size_t)>* fn_ = nullptr;
+  std::size_t n_ = 0;
+  std::atomic<std::size_t> next_ {0};
+  std::uint64_t generation_ = 0;
+  std::size_t busy_ = 0;
+  bool stop_ = false;
+  std::exception_ptr error_;
+};
+
 class Runner
 {
 public:
@@ -43,6 +79,9 @@ public:
   std::size_t n_elems(std::size_t i) const { return slots_[i].n_elems; }
 
   std::uint64_t launches() const { return launches_; }
+
+  /// Host threads for the per-solve halves of a large step (started on first use).
+  Workers& workers();
 
 private:
   struct Placement
@@ -67,6 +106,7 @@ private:
   std::size_t done_capacity_ = 0;
   std::size_t off_obj_lb_ = 0, off_cand_ = 0, off_witness_ = 0, results_bytes_ = 0;
   std::uint64_t launches_ = 0;
+  std::unique_ptr<Workers> workers_;
 };
 
 /// One solve advanced by the lock-step loops: its search and its device
