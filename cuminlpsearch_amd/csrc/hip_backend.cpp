@@ -44,10 +44,27 @@ Context::Context(int device)
 
 Context::~Context()
 {
+  (void)hipSetDevice(device_);
+  retained_.clear();
   if (stream_ != nullptr) {
-    (void)hipSetDevice(device_);
     (void)hipStreamDestroy(stream_);
   }
+}
+
+void Context::retain(std::shared_ptr<const void> module) const
+{
+  if (!module) {
+    return;
+  }
+  std::lock_guard<std::mutex> lock(retained_mu_);
+  if (auto it = std::find(retained_.begin(), retained_.end(), module); it != retained_.end()) {
+    std::rotate(it, it + 1, retained_.end());  // most recently used last
+    return;
+  }
+  if (retained_.size() >= kRetainedModules) {
+    retained_.erase(retained_.begin());
+  }
+  retained_.push_back(std::move(module));
 }
 
 void Context::activate() const { check(hipSetDevice(device_), "hipSetDevice"); }
@@ -183,6 +200,7 @@ void DeviceTape::drop_compiled() const
   for (int i = 0; i < 2; ++i) {
     items_[i].reset();
     items_failed_[i] = false;
+    items_looked_up_[i] = false;
     items_work_[i] = 0.0;
   }
 }
@@ -203,7 +221,19 @@ std::shared_ptr<const jit::CompiledItems> DeviceTape::items_kernel(bool aggregat
   }
   std::lock_guard<std::mutex> lock(jit_mu_);
   int const i = aggregate ? 1 : 0;
-  if (engine == BoxEngine::Auto && !items_[i]) {
+  if (engine == BoxEngine::Auto && !items_[i] && !items_failed_[i]) {
+    // A kernel already compiled for these tape contents (looked up on the
+    // first launch) costs nothing to take; otherwise compile once the
+    // interpreted work reaches about what the compile costs.
+    if (!items_looked_up_[i]) {
+      items_looked_up_[i] = true;
+      ctx_->activate();
+      items_[i] = jit::compile_items(grouped_, ctx_->device(), aggregate, false);
+      if (items_[i]) {
+        ctx_->retain(items_[i]->module_handle());
+        return items_[i];
+      }
+    }
     items_work_[i] += static_cast<double>(n_elems) * static_cast<double>(grouped_.code.size());
     if (items_work_[i] < kCompileAfterWork) {
       return nullptr;
@@ -213,6 +243,7 @@ std::shared_ptr<const jit::CompiledItems> DeviceTape::items_kernel(bool aggregat
     ctx_->activate();
     try {
       items_[i] = jit::compile_items(grouped_, ctx_->device(), aggregate);
+      ctx_->retain(items_[i]->module_handle());
     } catch (const DeviceError&) {
       if (engine == BoxEngine::Compiled) {
         throw;

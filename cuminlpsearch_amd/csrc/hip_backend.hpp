@@ -145,6 +145,14 @@ public:
   /// resolved when the context was created.
   void* static_scratch() const { return scratch_; }
 
+  /// Keep a loaded run-time-compiled module alive with this context, so that
+  /// a later tape of the same contents (a repeated solve of one model) finds
+  /// it in the compile cache instead of compiling again. The last
+  /// kRetainedModules distinct modules are kept; all are released, with the
+  /// device current, when the context is destroyed.
+  void retain(std::shared_ptr<const void> module) const;
+  static constexpr std::size_t kRetainedModules = 32;
+
 private:
   int device_;
   hipStream_t stream_ = nullptr;
@@ -152,6 +160,8 @@ private:
   std::uint32_t lds_point_slots_ = 0;
   std::size_t lds_bytes_ = 0;
   void* scratch_ = nullptr;
+  mutable std::mutex retained_mu_;
+  mutable std::vector<std::shared_ptr<const void>> retained_;  // oldest first
 };
 
 /// A problem compiled to a tape and uploaded once.
@@ -203,7 +213,9 @@ public:
   /// element-words (elements times tape words, about a tenth of a second of
   /// interpreted work, which a compile costs about as much as), so short
   /// solves never wait for the compiler and long ones stop interpreting
-  /// soon; a failed compile keeps the interpreter. Under Compiled always,
+  /// soon; a kernel the context already holds for the same tape contents (a
+  /// repeated solve of one model) is taken at the first launch; a failed
+  /// compile keeps the interpreter. Under Compiled always,
   /// under Interpreted never; only for grouped tapes within
   /// kMaxCompiledItemWords words whose tile fits in LDS. Results are identical.
   std::shared_ptr<const jit::CompiledItems> items_kernel(bool aggregate, std::size_t n_elems) const;
@@ -289,6 +301,7 @@ private:
   mutable std::shared_ptr<const jit::CompiledItems> items_[2];
   mutable bool items_failed_[2] = {false, false};
   mutable double items_work_[2] = {0.0, 0.0};  // element-words launched, interpreted
+  mutable bool items_looked_up_[2] = {false, false};  // compile cache searched once
   BoxEngine items_engine_ = BoxEngine::Auto;
 
   std::shared_ptr<const jit::CompiledBoxes> compile_now(bool roots) const;

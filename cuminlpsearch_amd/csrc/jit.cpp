@@ -835,7 +835,7 @@ void CompiledItems::launch(const void* items, const std::uint32_t* tile_item, st
             "hipModuleLaunchKernel(compiled items)");
 }
 
-std::shared_ptr<CompiledItems> compile_items(const tape::Tape& t, int device, bool aggregate)
+std::shared_ptr<CompiledItems> compile_items(const tape::Tape& t, int device, bool aggregate, bool compile)
 {
   static std::mutex mu;
   static std::map<std::tuple<int, bool, std::uint64_t, std::size_t>, std::weak_ptr<const CompiledItems::Module>> cache;
@@ -864,6 +864,9 @@ std::shared_ptr<CompiledItems> compile_items(const tape::Tape& t, int device, bo
       ci->mod_ = sp;
       return ci;
     }
+  }
+  if (!compile) {
+    return nullptr;
   }
   auto const t0 = std::chrono::steady_clock::now();
   std::vector<char> const code = compile_code_object(items_source(t, aggregate), false);

@@ -155,9 +155,11 @@ public:
   /// Workgroups per CU the persistent grid is sized for.
   std::uint32_t wgs_per_cu() const { return wgs_per_cu_; }
   double compile_seconds() const { return compile_seconds_; }
+  /// The loaded module, for a holder that keeps it alive beyond this object.
+  std::shared_ptr<const void> module_handle() const { return mod_; }
 
 private:
-  friend std::shared_ptr<CompiledItems> compile_items(const tape::Tape& t, int device, bool aggregate);
+  friend std::shared_ptr<CompiledItems> compile_items(const tape::Tape& t, int device, bool aggregate, bool compile);
   CompiledItems() = default;
 
   struct Module;
@@ -170,9 +172,11 @@ private:
 };
 
 /// Compile (or fetch from the process-wide cache) the decode-mode kernel of
-/// `t` and load it on `device`, which must be current. Throws DeviceError
-/// with the compiler log on failure, InvalidConfiguration if it does not fit.
-std::shared_ptr<CompiledItems> compile_items(const tape::Tape& t, int device, bool aggregate);
+/// `t` and load it on `device`, which must be current. The cache holds a
+/// module while some holder keeps it alive. With compile = false a cache miss
+/// returns null instead of compiling. Throws DeviceError with the compiler
+/// log on failure, InvalidConfiguration if it does not fit.
+std::shared_ptr<CompiledItems> compile_items(const tape::Tape& t, int device, bool aggregate, bool compile = true);
 
 /// Compile HIP source with hiprtc for gfx950 (host only, no device needed),
 /// with the max-ILP instruction scheduler when `ilp_schedule`; returns the
