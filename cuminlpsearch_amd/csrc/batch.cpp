@@ -77,6 +77,23 @@ constexpr std::size_t kParallelMembers = 256;
 
 /// Host threads of a runner's workers: CMI_BATCH_HOST_THREADS when it holds
 /// a count in [1, 64], else min(16, hardware threads).
+/// CMI_ITEM_WAVES: waves per workgroup of every batched launch (0: each
+/// item's groups in one workgroup; 1, 2, 4, 8 or 16), or -1 when unset.
+int item_waves_override()
+{
+  static int const v = [] {
+    if (const char* e = std::getenv("CMI_ITEM_WAVES")) {
+      char* end = nullptr;
+      long const w = std::strtol(e, &end, 10);
+      if (end != e && *end == '\0' && (w == 0 || w == 1 || w == 2 || w == 4 || w == 8 || w == 16)) {
+        return static_cast<int>(w);
+      }
+    }
+    return -1;
+  }();
+  return v;
+}
+
 std::size_t host_threads()
 {
   if (const char* v = std::getenv("CMI_BATCH_HOST_THREADS")) {
@@ -228,13 +245,39 @@ void Runner::run(std::span<const Request> reqs)
     }
   }
 
+  // Waves per workgroup of each launch (DeviceTape::slice_waves): when a
+  // pass's tiles leave compute units idle, every item's function groups
+  // spread over several workgroups per tile (sliced items), so a long tape's
+  // tile no longer runs all its groups on one compute unit; otherwise each
+  // item's groups share one workgroup. CMI_ITEM_WAVES overrides (0: one
+  // workgroup per tile).
+  std::uint32_t pass_waves[3] = {0, 0, 0};
+  {
+    std::size_t pass_tiles[3] = {0, 0, 0};
+    std::uint32_t pass_groups[3] = {1, 1, 1};
+    for (std::size_t k = 0; k < n; ++k) {
+      int const pass = k < counts[0] ? 0 : k < counts[0] + counts[1] ? 1 : 2;
+      Request const& r = reqs[order[k]];
+      std::size_t const elems = region::composition_fan_out(r.assignment->widths)
+          * (r.mode == dev::Mode::Sample ? r.samples : 1);
+      pass_tiles[pass] += (elems + dev::kTile - 1) / dev::kTile;
+      pass_groups[pass] = std::max(pass_groups[pass], r.tape->item_groups_max());
+    }
+    for (int pass = 0; pass < 3; ++pass) {
+      pass_waves[pass] = hip::DeviceTape::slice_waves(pass_groups[pass], pass_tiles[pass], ctx_->n_cus());
+    }
+  }
+  if (int const forced = item_waves_override(); forced >= 0) {
+    pass_waves[0] = pass_waves[1] = pass_waves[2] = static_cast<std::uint32_t>(forced);
+  }
+
   // Sizes.
-  std::size_t tiles[3] = {0, 0, 0};
-  std::size_t total_regions = 0, n_cand = 0, witness_doubles = 0, data_bytes = 0;
-  std::uint32_t lds[3] = {1, 1, 1};        // slots of the whole tapes
-  std::uint32_t lds_g[3] = {1, 1, 1};      // slots of the grouped tapes
-  std::uint32_t waves_g[3] = {1, 1, 1};    // most groups of any item
+  std::size_t tiles[3] = {0, 0, 0};  // workgroups: tiles times slices
+  std::size_t total_regions = 0, n_cand = 0, witness_doubles = 0, data_bytes = 0, sliced_tiles = 0;
+  std::uint32_t lds[3] = {1, 1, 1};    // most slots of any item's code
+  std::uint32_t waves[3] = {1, 1, 1};  // most waves of any item's workgroup
   bool ext[3] = {false, false, false};
+  plans_.resize(n);
   for (std::size_t k = 0; k < n; ++k) {
     std::size_t const i = order[k];
     Request const& r = reqs[i];
@@ -257,10 +300,12 @@ void Runner::run(std::span<const Request> reqs)
     p.n_regions = n_regions;
     p.n_elems = n_elems;
     p.n_vars = r.tape->n_vars();
-    tiles[pass] += (n_elems + dev::kTile - 1) / dev::kTile;
-    lds[pass] = std::max(lds[pass], r.tape->item_slots(false));
-    lds_g[pass] = std::max(lds_g[pass], r.tape->item_slots(true));
-    waves_g[pass] = std::max(waves_g[pass], r.tape->item_waves(true));
+    hip::DeviceTape::ItemPlan& plan = plans_[k];
+    plan = r.tape->item_plan(r.mode, pass_waves[pass]);
+    std::size_t const n_tiles = (n_elems + dev::kTile - 1) / dev::kTile;
+    tiles[pass] += n_tiles * plan.slices;
+    lds[pass] = std::max(lds[pass], plan.slots);
+    waves[pass] = std::max(waves[pass], plan.waves);
     ext[pass] = ext[pass] || r.tape->extended();
     if (r.mode == dev::Mode::Bound) {
       p.region_off = total_regions;
@@ -273,16 +318,31 @@ void Runner::run(std::span<const Request> reqs)
     data_bytes += align_up(p.n_vars * 16, 16) + align_up(slots * 4, 16) * 2 + align_up(slots * 8, 16)
         + align_up(slots, 16);
   }
-  if (lds[2] > ctx_->lds_interval_slots() || std::max(lds[0], lds[1]) > ctx_->lds_point_slots()) {
-    throw ResourceExhausted("batched request needs more LDS slots than a workgroup holds");
-  }
-  // Each pass runs its items' tapes in their function groups (one wave per
-  // group, as many waves as the most grouped item has) when that workgroup
-  // fits in LDS, else each tape whole on one wave.
-  bool grouped[3];
+  // A launch whose largest code and widest workgroup together do not fit in
+  // LDS runs every item's whole tape on one wave instead.
   for (int pass = 0; pass < 3; ++pass) {
     dev::Mode const m = pass == 0 ? dev::Mode::Sample : pass == 1 ? dev::Mode::Enumerate : dev::Mode::Bound;
-    grouped[pass] = waves_g[pass] > 1 && dev::items_lds_bytes(m, lds_g[pass], waves_g[pass]) <= ctx_->lds_bytes();
+    if (waves[pass] > 1 && dev::items_lds_bytes(m, lds[pass], waves[pass]) > ctx_->lds_bytes()) {
+      tiles[pass] = 0;
+      lds[pass] = 1;
+      waves[pass] = 1;
+      for (std::size_t k = 0; k < n; ++k) {
+        int const kp = k < counts[0] ? 0 : k < counts[0] + counts[1] ? 1 : 2;
+        if (kp == pass) {
+          plans_[k] = reqs[order[k]].tape->item_plan_whole();
+          tiles[pass] += (slots_[order[k]].n_elems + dev::kTile - 1) / dev::kTile;
+          lds[pass] = std::max(lds[pass], plans_[k].slots);
+        }
+      }
+    }
+  }
+  for (std::size_t k = 0; k < n; ++k) {
+    if (plans_[k].slices > 1) {
+      sliced_tiles += (slots_[order[k]].n_elems + dev::kTile - 1) / dev::kTile;
+    }
+  }
+  if (lds[2] > ctx_->lds_interval_slots() || std::max(lds[0], lds[1]) > ctx_->lds_point_slots()) {
+    throw ResourceExhausted("batched request needs more LDS slots than a workgroup holds");
   }
   std::size_t const total_tiles = tiles[0] + tiles[1] + tiles[2];
   std::size_t const cand_tiles = tiles[0] + tiles[1];
@@ -309,6 +369,16 @@ void Runner::run(std::span<const Request> reqs)
   }
   ensure_host(staging_h_, staging_bytes);
   ensure_host(results_h_, results_bytes_);
+  if (slice_capacity_ < sliced_tiles) {
+    std::size_t const cap = std::max(sliced_tiles, slice_capacity_ * 2);
+    slice_d_.reset(cap * dev::kSliceTileBytes + 256);
+    check(hipMemsetAsync(slice_d_.get(), 0, slice_d_.bytes(), s), "hipMemsetAsync(slice scratch)");
+    slice_capacity_ = cap;
+  }
+  auto* const slice_inf = slice_d_.as<std::uint64_t>();
+  auto* const slice_done = reinterpret_cast<std::uint32_t*>(slice_d_.as<unsigned char>() + slice_capacity_ * 8);
+  auto* const slice_obj = reinterpret_cast<double*>(slice_d_.as<unsigned char>() + align_up(slice_capacity_ * 12, 16));
+  std::size_t slice_cursor = 0;
 
   unsigned char* const h = staging_h_.data();
   auto* const d = staging_d_.as<unsigned char>();
@@ -357,9 +427,16 @@ void Runner::run(std::span<const Request> reqs)
     }
 
     std::uint32_t const n_tiles = static_cast<std::uint32_t>((p.n_elems + dev::kTile - 1) / dev::kTile);
+    hip::DeviceTape::ItemPlan const& plan = plans_[k];
     dev::Item& it = items[k];
     std::memset(&it, 0, sizeof(it));
-    r.tape->item_code(it, grouped[pass]);
+    r.tape->item_code(it, plan);
+    if (plan.slices > 1) {
+      it.tile_infeasible = slice_inf + slice_cursor;
+      it.tile_slices_done = slice_done + slice_cursor;
+      it.tile_obj = slice_obj + slice_cursor * dev::kTile * 2;
+      slice_cursor += n_tiles;
+    }
     it.n_vars = static_cast<std::uint32_t>(p.n_vars);
     it.parent = reinterpret_cast<const double*>(d + off_parent);
     it.var_kinds = r.tape->var_kinds();
@@ -385,10 +462,11 @@ void Runner::run(std::span<const Request> reqs)
       it.witness = reinterpret_cast<double*>(res + off_witness_) + p.witness_off;
       cand_tile_cursor += n_tiles;
     }
-    for (std::uint32_t t = 0; t < n_tiles; ++t) {
+    std::size_t const blocks = std::size_t {n_tiles} * plan.slices;
+    for (std::size_t t = 0; t < blocks; ++t) {
       tile_item[tile_cursor + t] = static_cast<std::uint32_t>(k - launch_first_item);
     }
-    tile_cursor += n_tiles;
+    tile_cursor += blocks;
   }
 
   check(hipMemcpyAsync(d, h, staging_bytes, hipMemcpyHostToDevice, s), "hipMemcpyAsync(batch staging)");
@@ -398,8 +476,7 @@ void Runner::run(std::span<const Request> reqs)
     if (counts[pass] > 0) {
       dev::launch_items(m, ext[pass], reinterpret_cast<const dev::Item*>(d + off_items) + first_item,
                         reinterpret_cast<const std::uint32_t*>(d + off_tile_item) + first_tile,
-                        static_cast<std::uint32_t>(tiles[pass]), grouped[pass] ? lds_g[pass] : lds[pass],
-                        grouped[pass] ? waves_g[pass] : 1u, s);
+                        static_cast<std::uint32_t>(tiles[pass]), lds[pass], waves[pass], s);
       check(hipGetLastError(), "launch(k_items batch)");
       ++launches_;
     }

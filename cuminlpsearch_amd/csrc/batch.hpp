@@ -101,7 +101,10 @@ private:
 
   std::shared_ptr<hip::Context> ctx_;
   std::vector<Placement> slots_;
+  std::vector<hip::DeviceTape::ItemPlan> plans_;  // per launch-ordered item
   hip::DeviceBuffer staging_d_, results_d_, scratch_d_, done_d_;
+  hip::DeviceBuffer slice_d_;  // sliced items' per-tile scratch, zero between launches
+  std::size_t slice_capacity_ = 0;  // tiles slice_d_ holds
   hip::PinnedBuffer<unsigned char> staging_h_, results_h_;
   std::size_t done_capacity_ = 0;
   std::size_t off_obj_lb_ = 0, off_cand_ = 0, off_witness_ = 0, results_bytes_ = 0;

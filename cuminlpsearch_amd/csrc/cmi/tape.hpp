@@ -830,7 +830,11 @@ inline Tape compile(const model::Problem<double>& p, const CompileOptions& opt =
 /// by longest-processing-time assignment. A node that several groups need is
 /// evaluated by each of them. The groups' temporaries get disjoint slot ranges
 /// so the waves of one workgroup share the variable slots and nothing else.
-inline Tape compile_groups(const model::Problem<double>& p, std::uint32_t n_groups, const CompileOptions& opt = {})
+/// slot_period (0 = n_groups): group g's temporaries take range g mod
+/// slot_period, for code whose groups run slot_period (or a divisor of it) to
+/// a workgroup.
+inline Tape compile_groups(const model::Problem<double>& p, std::uint32_t n_groups, const CompileOptions& opt = {},
+                           std::uint32_t slot_period = 0)
 {
   std::size_t const n_roots = 1 + p.constraints.size();
   n_groups = std::max<std::uint32_t>(1, std::min<std::uint32_t>(n_groups, static_cast<std::uint32_t>(n_roots)));
@@ -868,9 +872,10 @@ inline Tape compile_groups(const model::Problem<double>& p, std::uint32_t n_grou
   }
   Tape const whole = detail::Compiler(p, opt).run();
   t.reachable_nodes = whole.reachable_nodes;
+  std::uint32_t const period = slot_period == 0 ? n_groups : std::min(slot_period, n_groups);
   for (std::uint32_t g = 0; g < n_groups; ++g) {
     t.group_begin.push_back(static_cast<std::uint32_t>(t.code.size()));
-    std::uint32_t const shift = g * t.temps_per_group;
+    std::uint32_t const shift = (g % period) * t.temps_per_group;
     auto remap = [&](std::uint16_t s) -> std::uint16_t {
       return s >= t.n_vars ? static_cast<std::uint16_t>(s + shift) : s;
     };
@@ -893,7 +898,7 @@ inline Tape compile_groups(const model::Problem<double>& p, std::uint32_t n_grou
   t.flops = whole.flops;  // algorithmic work: every DAG operation once
   t.extended = whole.extended;
   t.group_begin.push_back(static_cast<std::uint32_t>(t.code.size()));
-  std::uint64_t const slots = std::uint64_t(t.n_vars) + std::uint64_t(n_groups) * t.temps_per_group;
+  std::uint64_t const slots = std::uint64_t(t.n_vars) + std::uint64_t(period) * t.temps_per_group;
   if (slots > 65535) {
     throw InvalidConfiguration("grouped tape needs more slots than the 16-bit encoding holds");
   }

@@ -61,7 +61,23 @@ struct Item
   double* child_lb;  // k entries, +inf before the launch; min of masked obj_lb
   double* child_ub;  // k entries, -inf before the launch; max of masked obj_ub
   std::uint64_t refine;
+  // Sliced items: a long tape's function groups spread over n_slices
+  // workgroups per tile (1: every group in one workgroup, as `groups` says).
+  // Slice s of a tile runs groups [s * W, s * W + W), W = the launch's waves
+  // per workgroup; the last slice of a tile to finish combines the verdicts
+  // and carries on with the tile's epilogue.
+  std::uint32_t n_slices;
+  std::uint32_t slice_groups;        // function groups of the sliced code
+  const std::uint32_t* slice_begin;  // slice_groups + 1 word offsets into code
+  std::uint32_t slice_objective;     // the group that reads out the objective
+  std::uint32_t slice_pad;
+  std::uint64_t* tile_infeasible;    // per tile: lanes some slice found infeasible; zero between launches
+  std::uint32_t* tile_slices_done;   // per tile: slices finished; zero between launches
+  double* tile_obj;                  // per tile: the objective of each lane (intervals: (lb, ub) pairs)
 };
+
+/// Device scratch of one sliced tile (tile_infeasible, tile_slices_done, tile_obj).
+inline constexpr std::size_t kSliceTileBytes = 8 + 4 + kTile * 16;
 
 enum class Mode : int
 {

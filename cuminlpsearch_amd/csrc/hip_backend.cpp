@@ -37,6 +37,10 @@ Context::Context(int device)
   lds_interval_slots_ = dev::max_lds_slots(true);
   lds_point_slots_ = dev::max_lds_slots(false);
   lds_bytes_ = dev::lds_limit_bytes();
+  int cus = 0;
+  if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) == hipSuccess && cus > 0) {
+    n_cus_ = static_cast<std::uint32_t>(cus);
+  }
   // Resolve this device's copy of the static scratch now, with the device
   // current, so the no-allocation entry points never look it up mid-call.
   scratch_ = dev::static_scratch();
@@ -84,6 +88,16 @@ std::size_t Context::free_bytes() const
 namespace
 {
 
+/// Upload 32-bit words (a group table).
+void upload_words(const std::vector<std::uint32_t>& words, DeviceBuffer& buf)
+{
+  buf.reset(std::max<std::size_t>(words.size(), 1) * sizeof(std::uint32_t));
+  if (!words.empty()) {
+    check(hipMemcpy(buf.get(), words.data(), words.size() * sizeof(std::uint32_t), hipMemcpyHostToDevice),
+          "hipMemcpy(group table)");
+  }
+}
+
 /// Upload a tape's words followed by one Nop word (the interpreters read one
 /// word ahead).
 void upload_code(const std::vector<tape::Instr>& code, DeviceBuffer& buf)
@@ -117,6 +131,14 @@ DeviceTape::DeviceTape(std::shared_ptr<Context> ctx, const model::Problem<double
   kinds_.reset(std::max<std::size_t>(kinds.size(), 1));
   check(hipMemcpy(kinds_.get(), kinds.data(), kinds.size(), hipMemcpyHostToDevice), "hipMemcpy(var kinds)");
   set_box_groups(problem, box_groups);
+  std::uint32_t const n_roots = 1 + tape_.n_constraints;
+  if (tape_.code.size() >= kWideMinWords && n_roots > dev::kMaxGroups) {
+    auto const by_size = static_cast<std::uint32_t>((tape_.code.size() + kWideWordsPerGroup - 1) / kWideWordsPerGroup);
+    std::uint32_t const k = std::min({n_roots, kMaxWideGroups, std::max(by_size, dev::kMaxGroups + 1)});
+    wide_ = tape::compile_groups(problem, k, opt_, kWidePeriod);
+    upload_code(wide_.code, wide_code_);
+    upload_words(wide_.group_begin, wide_begin_);
+  }
 }
 
 void DeviceTape::set_box_groups(const model::Problem<double>& problem, std::uint32_t k)
@@ -138,6 +160,7 @@ void DeviceTape::set_box_groups(const model::Problem<double>& problem, std::uint
   }
   ctx_->activate();
   upload_code(g.code, grouped_code_);
+  upload_words(g.group_begin, grouped_begin_);
   grouped_ = std::move(g);
   drop_compiled();  // the compiled kernel follows the grouping; compile again when needed
   if (engine_ == BoxEngine::Compiled) {
@@ -153,6 +176,10 @@ void DeviceTape::set_box_groups(const model::Problem<double>& problem, std::uint
 
 void DeviceTape::item_code(dev::Item& it, bool grouped) const
 {
+  it.n_slices = 1;
+  it.slice_groups = 0;
+  it.slice_begin = nullptr;
+  it.slice_objective = 0;
   if (grouped) {
     it.code = grouped_code_.as<tape::Instr>();
     it.groups = groups_;
@@ -166,6 +193,109 @@ void DeviceTape::item_code(dev::Item& it, bool grouped) const
     it.groups.begin[1] = static_cast<std::uint32_t>(tape_.code.size());
     it.n_instr = static_cast<std::uint32_t>(tape_.code.size());
   }
+}
+
+DeviceTape::ItemPlan DeviceTape::item_plan(dev::Mode mode, std::uint32_t waves) const
+{
+  ItemPlan p;
+  std::size_t const lds = ctx_->lds_bytes();
+  if (waves == 0) {
+    bool const grouped = items_grouped_fit(mode);
+    p.code = grouped ? grouped_code_.as<tape::Instr>() : code_.as<tape::Instr>();
+    p.begin = grouped ? grouped_begin_.as<std::uint32_t>() : nullptr;
+    p.n_instr = static_cast<std::uint32_t>(grouped ? grouped_.code.size() : tape_.code.size());
+    p.groups = grouped ? grouped_.n_groups : 1;
+    p.objective = grouped ? grouped_.objective_group : 0;
+    p.slots = item_slots(grouped);
+    p.waves = item_waves(grouped);
+    p.code_kind = grouped ? 1 : 0;
+    return p;
+  }
+  waves = std::clamp<std::uint32_t>(waves, 1, dev::kMaxGroups);
+  if (wide_.n_groups > 1 && kWidePeriod % waves == 0
+      && dev::items_lds_bytes(mode, wide_.n_slots, std::max<std::uint32_t>(waves, 2)) <= lds) {
+    p.code = wide_code_.as<tape::Instr>();
+    p.begin = wide_begin_.as<std::uint32_t>();
+    p.n_instr = static_cast<std::uint32_t>(wide_.code.size());
+    p.groups = wide_.n_groups;
+    p.objective = wide_.objective_group;
+    p.slots = wide_.n_slots;
+    p.code_kind = 2;
+  } else if (grouped_.n_groups > 1
+             && dev::items_lds_bytes(mode, grouped_.n_slots, std::max<std::uint32_t>(std::min(waves, grouped_.n_groups), 2))
+                 <= lds) {
+    p.code = grouped_code_.as<tape::Instr>();
+    p.begin = grouped_begin_.as<std::uint32_t>();
+    p.n_instr = static_cast<std::uint32_t>(grouped_.code.size());
+    p.groups = grouped_.n_groups;
+    p.objective = grouped_.objective_group;
+    p.slots = grouped_.n_slots;
+    p.code_kind = 1;
+  } else {
+    p.code = code_.as<tape::Instr>();
+    p.n_instr = static_cast<std::uint32_t>(tape_.code.size());
+    p.slots = tape_.n_slots;
+    p.code_kind = 0;
+  }
+  p.waves = std::min(waves, p.groups);
+  p.slices = (p.groups + waves - 1) / waves;
+  return p;
+}
+
+DeviceTape::ItemPlan DeviceTape::item_plan_whole() const
+{
+  ItemPlan p;
+  p.code = code_.as<tape::Instr>();
+  p.n_instr = static_cast<std::uint32_t>(tape_.code.size());
+  p.slots = tape_.n_slots;
+  return p;
+}
+
+std::uint32_t DeviceTape::slice_waves(std::uint32_t groups, std::size_t tiles, std::uint32_t n_cus)
+{
+  if (groups <= kWidePeriod) {
+    return 0;
+  }
+  if (tiles * 2 <= n_cus) {
+    return kWidePeriod;
+  }
+  if (tiles <= std::size_t {2} * n_cus && groups > 8) {
+    return 8;
+  }
+  return 0;
+}
+
+std::uint32_t DeviceTape::item_waves_for(dev::Mode mode, std::size_t n_tiles) const
+{
+  (void)mode;
+  return slice_waves(item_groups_max(), n_tiles, ctx_->n_cus());
+}
+
+void DeviceTape::item_code(dev::Item& it, const ItemPlan& plan) const
+{
+  it.code = plan.code;
+  it.n_instr = plan.n_instr;
+  it.groups = {};
+  if (plan.slices == 1) {
+    // every group in one workgroup: the inline table
+    if (plan.code_kind == 0) {
+      it.groups.n_groups = 1;
+      it.groups.objective_group = 0;
+      it.groups.begin[0] = 0;
+      it.groups.begin[1] = plan.n_instr;
+    } else {
+      tape::Tape const& t = plan.code_kind == 1 ? grouped_ : wide_;
+      it.groups.n_groups = t.n_groups;
+      it.groups.objective_group = t.objective_group;
+      for (std::uint32_t i = 0; i <= t.n_groups; ++i) {
+        it.groups.begin[i] = t.group_begin[i];
+      }
+    }
+  }
+  it.n_slices = plan.slices;
+  it.slice_groups = plan.groups;
+  it.slice_begin = plan.slices > 1 ? plan.begin : nullptr;
+  it.slice_objective = plan.objective;
 }
 
 bool DeviceTape::items_grouped_fit(dev::Mode mode) const
@@ -653,10 +783,22 @@ RoleImpl::RoleImpl(std::shared_ptr<const DeviceTape> tape, dev::Mode mode, const
   staging_h_.reset(staging_bytes_);
   std::memset(staging_h_.data(), 0, staging_bytes_);
 
-  std::vector<std::uint32_t> tile_item(n_tiles_, 0);
-  tile_item_d_.reset(n_tiles_ * sizeof(std::uint32_t));
-  check(hipMemcpy(tile_item_d_.get(), tile_item.data(), n_tiles_ * sizeof(std::uint32_t), hipMemcpyHostToDevice),
+  // A launch over few tiles may spread each tile's function groups over
+  // several workgroups (item_waves_for); every workgroup names item 0, and
+  // the slices of a tile meet in a per-tile scratch that each launch leaves
+  // zeroed.
+  if (tape_->item_waves_for(mode_, n_tiles_) != 0) {
+    max_slices_ = (DeviceTape::kMaxWideGroups + DeviceTape::kWidePeriod - 1) / DeviceTape::kWidePeriod;
+  }
+  std::size_t const blocks = std::size_t {n_tiles_} * max_slices_;
+  std::vector<std::uint32_t> tile_item(blocks, 0);
+  tile_item_d_.reset(blocks * sizeof(std::uint32_t));
+  check(hipMemcpy(tile_item_d_.get(), tile_item.data(), blocks * sizeof(std::uint32_t), hipMemcpyHostToDevice),
         "hipMemcpy(tile_item)");
+  if (max_slices_ > 1) {
+    slice_d_.reset(std::size_t {n_tiles_} * dev::kSliceTileBytes + 256);
+    check(hipMemset(slice_d_.get(), 0, slice_d_.bytes()), "hipMemset(slice scratch)");
+  }
 
   if (mode_ == dev::Mode::Bound) {
     feasible_d_.reset(n_regions_);
@@ -741,15 +883,25 @@ void RoleImpl::run(std::span<const interval> box, std::span<const std::size_t> v
     step[j] = (box[var_ids[j]].ub - box[var_ids[j]].lb) / static_cast<double>(widths[j]);
   }
   // the tape's current function groups (a regrouping replaces their code)
-  bool const grouped = tape_->items_grouped_fit(mode_);
-  tape_->item_code(*item, grouped);
-  check(hipMemcpyAsync(staging_d_.get(), h, staging_bytes_, hipMemcpyHostToDevice, s), "hipMemcpyAsync(staging)");
   auto const compiled = mode_ == dev::Mode::Bound ? tape_->items_kernel(false, n_elems_) : nullptr;
+  std::uint32_t const waves = compiled || max_slices_ == 1 ? 0u : tape_->item_waves_for(mode_, n_tiles_);
+  DeviceTape::ItemPlan const plan = tape_->item_plan(mode_, waves);
+  if (plan.slices > max_slices_) {
+    throw InvalidConfiguration("decode-mode item: more slices per tile than this role's launch table holds");
+  }
+  tape_->item_code(*item, plan);
+  if (plan.slices > 1) {
+    auto* const sc = slice_d_.as<unsigned char>();
+    item->tile_infeasible = reinterpret_cast<std::uint64_t*>(sc);
+    item->tile_slices_done = reinterpret_cast<std::uint32_t*>(sc + std::size_t {n_tiles_} * 8);
+    item->tile_obj = reinterpret_cast<double*>(sc + align_up(std::size_t {n_tiles_} * 12, 16));
+  }
+  check(hipMemcpyAsync(staging_d_.get(), h, staging_bytes_, hipMemcpyHostToDevice, s), "hipMemcpyAsync(staging)");
   if (compiled) {
     compiled->launch(staging_d_.get(), tile_item_d_.as<std::uint32_t>(), n_tiles_, s);
   } else {
-    dev::launch_items(mode_, tape_->extended(), staging_d_.as<dev::Item>(), tile_item_d_.as<std::uint32_t>(), n_tiles_,
-                      tape_->item_slots(grouped), tape_->item_waves(grouped), s);
+    dev::launch_items(mode_, tape_->extended(), staging_d_.as<dev::Item>(), tile_item_d_.as<std::uint32_t>(),
+                      n_tiles_ * plan.slices, plan.slots, plan.waves, s);
   }
   check(hipGetLastError(), "launch(k_items)");
   if (mode_ == dev::Mode::Bound) {

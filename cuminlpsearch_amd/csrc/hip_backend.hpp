@@ -140,6 +140,8 @@ public:
   std::uint32_t lds_point_slots() const { return lds_point_slots_; }
   /// LDS one workgroup may use.
   std::size_t lds_bytes() const { return lds_bytes_; }
+  /// Compute units of the device.
+  std::uint32_t n_cus() const { return n_cus_; }
   std::size_t free_bytes() const;
   /// This device's copy of the static scratch array (dev::static_scratch),
   /// resolved when the context was created.
@@ -159,6 +161,7 @@ private:
   std::uint32_t lds_interval_slots_ = 0;
   std::uint32_t lds_point_slots_ = 0;
   std::size_t lds_bytes_ = 0;
+  std::uint32_t n_cus_ = 1;
   void* scratch_ = nullptr;
   mutable std::mutex retained_mu_;
   mutable std::vector<std::shared_ptr<const void>> retained_;  // oldest first
@@ -264,7 +267,56 @@ public:
   /// What a decode-mode item runs: the tape in its explicit-box function
   /// groups (one wave per group; grouped = true) or the whole tape on one
   /// wave. The values are the same; the groups shorten each wave's run.
+  /// Every group runs in the item's one workgroup (n_slices = 1).
   void item_code(dev::Item& it, bool grouped) const;
+
+  /// How a decode-mode item's function groups spread over workgroups of at
+  /// most `waves` waves: `slices` workgroups per 64-element tile, slice s
+  /// running groups [s * waves, s * waves + waves) of `groups`. A long tape
+  /// (kWideMinWords words or more) also has a wide grouping, one group per
+  /// function up to kMaxWideGroups, whose temporaries repeat with period
+  /// kWidePeriod so that any workgroup of up to that many of its groups holds
+  /// them; it is used when `waves` divides kWidePeriod. The values are those
+  /// of the whole tape whatever the plan.
+  struct ItemPlan
+  {
+    const tape::Instr* code = nullptr;
+    const std::uint32_t* begin = nullptr;  // device: groups + 1 word offsets
+    std::uint32_t n_instr = 0;
+    std::uint32_t groups = 1;
+    std::uint32_t objective = 0;
+    std::uint32_t slots = 0;   // LDS slots per lane
+    std::uint32_t waves = 1;   // waves per workgroup
+    std::uint32_t slices = 1;  // workgroups per tile
+    int code_kind = 0;         // 0 whole tape, 1 grouped, 2 wide
+  };
+  static constexpr std::size_t kWideMinWords = 4096;
+  static constexpr std::uint32_t kMaxWideGroups = 256;
+  static constexpr std::uint32_t kWideWordsPerGroup = 128;
+  static constexpr std::uint32_t kWidePeriod = 4;
+  /// The plan for workgroups of `waves` waves (the launch's); 0: every group
+  /// of the grouped code in one workgroup when it fits in LDS, as item_code.
+  ItemPlan item_plan(dev::Mode mode, std::uint32_t waves) const;
+  /// The whole tape on one wave, one workgroup per tile.
+  ItemPlan item_plan_whole() const;
+  /// This tape has the wide grouping (a long tape with many functions).
+  bool has_wide() const { return wide_.n_groups > 1; }
+  /// Most function groups any of this tape's codes has.
+  std::uint32_t item_groups_max() const { return std::max(grouped_.n_groups, wide_.n_groups); }
+  /// Waves per workgroup for a launch of this tape alone over n_tiles tiles
+  /// (slice_waves).
+  std::uint32_t item_waves_for(dev::Mode mode, std::size_t n_tiles) const;
+  /// Waves per workgroup of a decode-mode launch of `tiles` tiles whose
+  /// items have at most `groups` function groups, on n_cus compute units:
+  /// one wave per SIMD (4) when the tiles cover at most half the compute
+  /// units, so that each tile's groups spread over several of them; 8 when
+  /// they cover at most two workgroups per unit; else 0, every item's groups
+  /// in one workgroup. Measured on 32 served QCQPs of up to 93 variables
+  /// (4: 1.7x faster than 0) and 512 batched ones of up to 48 (8: 4 %
+  /// faster than 0; 4: 15 % slower).
+  static std::uint32_t slice_waves(std::uint32_t groups, std::size_t tiles, std::uint32_t n_cus);
+  /// Fill the Item's code fields from a plan (slice scratch is the caller's).
+  void item_code(dev::Item& it, const ItemPlan& plan) const;
   std::uint32_t item_slots(bool grouped) const { return grouped ? grouped_.n_slots : tape_.n_slots; }
   std::uint32_t item_waves(bool grouped) const { return grouped ? groups_.n_groups : 1u; }
   /// Whether a decode-mode workgroup of this tape's groups fits in LDS.
@@ -308,6 +360,10 @@ private:
   void drop_compiled() const;
   DeviceBuffer code_;
   DeviceBuffer grouped_code_;
+  DeviceBuffer grouped_begin_;  // grouped_.group_begin on the device
+  tape::Tape wide_;             // n_groups 0: none
+  DeviceBuffer wide_code_;
+  DeviceBuffer wide_begin_;
   DeviceBuffer kinds_;
 };
 
@@ -366,8 +422,10 @@ private:
   std::size_t slot_count_;
   std::size_t n_vars_;
   std::uint32_t n_tiles_;
+  std::uint32_t max_slices_ = 1;  // workgroups per tile the tile_item array covers
   DeviceBuffer staging_d_;   // Item + parent box + slot arrays
   DeviceBuffer tile_item_d_;
+  DeviceBuffer slice_d_;     // sliced items' per-tile scratch (zero between launches)
   DeviceBuffer feasible_d_, obj_lb_d_;
   DeviceBuffer tile_min_d_, tile_arg_d_, done_d_, out_d_, witness_d_;
   PinnedBuffer<unsigned char> staging_h_;

@@ -651,7 +651,9 @@ std::string items_source(const tape::Tape& t, bool aggregate)
        "  unsigned long n_elems, salt;\n  unsigned tile_begin, n_tiles;\n  unsigned char* feasible;\n"
        "  double* obj_lb;\n  double* tile_min;\n  long* tile_arg;\n  unsigned* done;\n  double* cand_value;\n"
        "  long* cand_index;\n  double* witness;\n  double* child_lb;\n  double* child_ub;\n"
-       "  unsigned long refine;\n};\n";
+       "  unsigned long refine;\n  unsigned n_slices, slice_groups;\n  const unsigned* slice_begin;\n"
+       "  unsigned slice_objective, slice_pad;\n  unsigned long* tile_infeasible;\n"
+       "  unsigned* tile_slices_done;\n  double* tile_obj;\n};\n";
   o << "static_assert(sizeof(CmiItem) == " << sizeof(dev::Item) << ", \"dev::Item layout\");\n";
   auto field = [&](const char* name, std::size_t off) {
     o << "static_assert(__builtin_offsetof(CmiItem, " << name << ") == " << off << ", \"dev::Item::" << name

@@ -2,7 +2,8 @@
 //
 // Each lane evaluates the tape for one element (a box, a region, or a sample
 // point); a wave covers 64 elements. The tape word stream is uniform across the
-// wave and is read with scalar loads, one word ahead. Per-lane slot values live
+// wave; it arrives in 64-word chunks through VGPRs, one chunk ahead, and each
+// word is read out of its lane into SGPRs. Per-lane slot values live
 // in LDS, slot-major ([slot][lane]), so every operand read or result write is
 // one conflict-free ds_read_b128/ds_write_b128 (16 B per lane, 1 KiB per wave);
 // the accumulator that carries sums lives in VGPRs. For explicit boxes the
@@ -119,16 +120,42 @@ __device__ __forceinline__ double draw(bool discrete, double lb, double ub, std:
 // Tape interpreters
 // ---------------------------------------------------------------------------
 
-// Tape words are read through the constant address space so the compiler
-// issues scalar loads (the pc is wave-uniform), as four whole dwords. Every
-// device tape carries one trailing Nop word, so reading one word past the last
-// instruction (the look-ahead fetch) stays in bounds.
+// Tape words are streamed through VGPRs in chunks of 64: lane j of a wave
+// loads word chunk*64 + j with one coalesced 16-byte load, the next chunk is
+// in flight while the current one runs, and each word is taken out of its lane
+// with v_readlane into SGPRs (the pc is wave-uniform, so the interpreter's
+// dispatch stays scalar). A tape larger than the scalar data cache would
+// otherwise wait out an L2 round trip on every word. Every device tape
+// carries one trailing Nop word, so reading one word past the last
+// instruction stays in bounds; lanes past that word load nothing.
 typedef std::uint32_t u32x4 __attribute__((ext_vector_type(4)));
-using ConstInstrPtr = const __attribute__((address_space(4))) u32x4*;
 
-__device__ __forceinline__ ConstInstrPtr as_constant(const Instr* p)
+/// A load through the global address space: the compiler then issues a
+/// global (not flat) load, which only the vector-memory counter tracks, so LDS
+/// traffic does not have to wait for it.
+template<typename T>
+__device__ __forceinline__ T gload(const T* p)
 {
-  return (ConstInstrPtr)(p);
+  using GlobalPtr = const __attribute__((address_space(1))) T*;
+  return *(GlobalPtr)(p);
+}
+
+/// Lane j's part of the 64-word tape chunk starting at word c0 = i - j;
+/// lanes past the last readable word repeat it.
+__device__ __forceinline__ u32x4 tape_chunk(const u32x4* words, std::uint32_t i, std::uint32_t limit)
+{
+  return gload(words + (i < limit ? i : limit));
+}
+
+/// Word `slot` of a chunk held one word per lane, as wave-uniform SGPRs.
+__device__ __forceinline__ u32x4 lane_word(u32x4 c, std::uint32_t slot)
+{
+  u32x4 w;
+  w.x = static_cast<std::uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(c.x), static_cast<int>(slot)));
+  w.y = static_cast<std::uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(c.y), static_cast<int>(slot)));
+  w.z = static_cast<std::uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(c.z), static_cast<int>(slot)));
+  w.w = static_cast<std::uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(c.w), static_cast<int>(slot)));
+  return w;
 }
 
 struct Word
@@ -196,105 +223,112 @@ __device__ __forceinline__ Iv ext_interval(std::uint32_t op, Iv a, const double2
 // roots is non-null.
 // With Ext the extended op set is interpreted too.
 template<bool Ext>
-__device__ __forceinline__ IntervalResult run_interval(ConstInstrPtr code, std::uint32_t n_instr, double2* S,
+__device__ __forceinline__ IntervalResult run_interval(const Instr* code, std::uint32_t n_instr, double2* S,
                                                        unsigned lane, int t, double2* roots,
                                                        std::uint64_t roots_stride, std::uint64_t e)
 {
   IntervalResult res {{0.0, 0.0}, true};
   Iv acc {0.0, 0.0};
-  u32x4 next = code[0];
-  for (std::uint32_t pc = 0; pc < n_instr; ++pc) {
-    Word const ins = decode_word(next);
-    next = code[pc + 1];
-    std::uint32_t const op = ins.op;
-    if (op >= tape::kTermBase) {
-      // acc <- acc (set|+|-) term(a, b, imm)
-      std::uint32_t const term = op & 7u;
-      std::uint32_t const mode = (op - tape::kTermBase) >> 3;
-      Iv const a = load_iv(S, ins.a, lane);
-      Iv x;
-      if (term == static_cast<std::uint32_t>(tape::Term::SSC)) {
-        x = iv_mul_imm(iv_mul(a, load_iv(S, ins.b, lane), t), ins.imm, t);
-      } else if (term == static_cast<std::uint32_t>(tape::Term::QC)) {
-        x = iv_mul_imm(iv_sqr(a, t), ins.imm, t);
-      } else if (term == static_cast<std::uint32_t>(tape::Term::SC)) {
-        x = iv_mul_imm(a, ins.imm, t);
-      } else if (term == static_cast<std::uint32_t>(tape::Term::SS)) {
-        x = iv_mul(a, load_iv(S, ins.b, lane), t);
-      } else if (term == static_cast<std::uint32_t>(tape::Term::Q)) {
-        x = iv_sqr(a, t);
-      } else {
-        x = a;
-      }
-      if (mode == static_cast<std::uint32_t>(tape::AccMode::Add)) {
-        acc = iv_add(acc, x, t);
-      } else if (mode == static_cast<std::uint32_t>(tape::AccMode::Set)) {
-        acc = x;
-      } else {
-        acc = iv_sub(acc, x, t);
-      }
-      continue;
-    }
-    if (op >= static_cast<std::uint32_t>(TOp::AccStore) && op < tape::kExtBase) {
-      switch (static_cast<TOp>(op)) {
-        case TOp::AccStore: store_iv(S, ins.dst, lane, acc); break;
-        case TOp::AccAddImm: acc = iv_add_imm(acc, ins.imm, t); break;
-        case TOp::AccSubImm: acc = iv_sub_imm(acc, ins.imm, t); break;
-        case TOp::AccImmSub: acc = iv_imm_sub(ins.imm, acc, t); break;
-        case TOp::AccScale: acc = iv_mul_imm(acc, ins.imm, t); break;
-        case TOp::AccNeg: acc = iv_neg(acc); break;
-        case TOp::AccCheckLE: res.feasible = res.feasible && check_le(acc, ins.imm); break;
-        case TOp::AccCheckEQ: res.feasible = res.feasible && check_eq(acc, ins.imm); break;
-        case TOp::AccObjective: res.obj = acc; break;
-        case TOp::AccOutput:
-          if (roots != nullptr) {
-            roots[static_cast<std::uint64_t>(ins.b) * roots_stride + e] = to_d2(to_output(acc, t));
-          }
-          break;
-        default: break;
-      }
-      continue;
-    }
-    Iv const a = load_iv(S, ins.a, lane);
-    if constexpr (Ext) {
-      if (op >= tape::kExtBase && op < tape::kExtEnd) {
-        bool ok = true;
-        Iv const r = ext_interval(op, a, S, ins, lane, t, ok);
-        res.feasible = res.feasible && ok;
-        store_iv(S, ins.dst, lane, r);
-        continue;
-      }
-    }
-    Iv r;
-    switch (static_cast<TOp>(op)) {
-      case TOp::LoadConst: r = iv_const(ins.imm, t); break;
-      case TOp::Add: r = iv_add(a, load_iv(S, ins.b, lane), t); break;
-      case TOp::AddImm: r = iv_add_imm(a, ins.imm, t); break;
-      case TOp::Sub: r = iv_sub(a, load_iv(S, ins.b, lane), t); break;
-      case TOp::SubImm: r = iv_sub_imm(a, ins.imm, t); break;
-      case TOp::ImmSub: r = iv_imm_sub(ins.imm, a, t); break;
-      case TOp::Mul: r = iv_mul(a, load_iv(S, ins.b, lane), t); break;
-      case TOp::MulImm: r = iv_mul_imm(a, ins.imm, t); break;
-      case TOp::Sqr: r = iv_sqr(a, t); break;
-      case TOp::Neg: r = iv_neg(a); break;
-      case TOp::CheckLE:
-        res.feasible = res.feasible && check_le(a, ins.imm);
-        continue;
-      case TOp::CheckEQ:
-        res.feasible = res.feasible && check_eq(a, ins.imm);
-        continue;
-      case TOp::Objective:
-        res.obj = a;
-        continue;
-      case TOp::Output:
-        if (roots != nullptr) {
-          roots[static_cast<std::uint64_t>(ins.b) * roots_stride + e] = to_d2(to_output(a, t));
+  // the tape arrives one 64-word chunk at a time, the next chunk in flight
+  // while this one runs
+  const u32x4* const words = reinterpret_cast<const u32x4*>(code);
+  u32x4 cur = tape_chunk(words, lane, n_instr);
+  for (std::uint32_t c0 = 0; c0 < n_instr; c0 += kTile) {
+    u32x4 const nxt = tape_chunk(words, c0 + kTile + lane, n_instr);
+    std::uint32_t const end = std::min<std::uint32_t>(n_instr - c0, kTile);
+    for (std::uint32_t slot = 0; slot < end; ++slot) {
+      Word const ins = decode_word(lane_word(cur, slot));
+      std::uint32_t const op = ins.op;
+      if (op >= tape::kTermBase) {
+        // acc <- acc (set|+|-) term(a, b, imm)
+        std::uint32_t const term = op & 7u;
+        std::uint32_t const mode = (op - tape::kTermBase) >> 3;
+        Iv const a = load_iv(S, ins.a, lane);
+        Iv x;
+        if (term == static_cast<std::uint32_t>(tape::Term::SSC)) {
+          x = iv_mul_imm(iv_mul(a, load_iv(S, ins.b, lane), t), ins.imm, t);
+        } else if (term == static_cast<std::uint32_t>(tape::Term::QC)) {
+          x = iv_mul_imm(iv_sqr(a, t), ins.imm, t);
+        } else if (term == static_cast<std::uint32_t>(tape::Term::SC)) {
+          x = iv_mul_imm(a, ins.imm, t);
+        } else if (term == static_cast<std::uint32_t>(tape::Term::SS)) {
+          x = iv_mul(a, load_iv(S, ins.b, lane), t);
+        } else if (term == static_cast<std::uint32_t>(tape::Term::Q)) {
+          x = iv_sqr(a, t);
+        } else {
+          x = a;
+        }
+        if (mode == static_cast<std::uint32_t>(tape::AccMode::Add)) {
+          acc = iv_add(acc, x, t);
+        } else if (mode == static_cast<std::uint32_t>(tape::AccMode::Set)) {
+          acc = x;
+        } else {
+          acc = iv_sub(acc, x, t);
         }
         continue;
-      default:
+      }
+      if (op >= static_cast<std::uint32_t>(TOp::AccStore) && op < tape::kExtBase) {
+        switch (static_cast<TOp>(op)) {
+          case TOp::AccStore: store_iv(S, ins.dst, lane, acc); break;
+          case TOp::AccAddImm: acc = iv_add_imm(acc, ins.imm, t); break;
+          case TOp::AccSubImm: acc = iv_sub_imm(acc, ins.imm, t); break;
+          case TOp::AccImmSub: acc = iv_imm_sub(ins.imm, acc, t); break;
+          case TOp::AccScale: acc = iv_mul_imm(acc, ins.imm, t); break;
+          case TOp::AccNeg: acc = iv_neg(acc); break;
+          case TOp::AccCheckLE: res.feasible = res.feasible && check_le(acc, ins.imm); break;
+          case TOp::AccCheckEQ: res.feasible = res.feasible && check_eq(acc, ins.imm); break;
+          case TOp::AccObjective: res.obj = acc; break;
+          case TOp::AccOutput:
+            if (roots != nullptr) {
+              roots[static_cast<std::uint64_t>(ins.b) * roots_stride + e] = to_d2(to_output(acc, t));
+            }
+            break;
+          default: break;
+        }
         continue;
+      }
+      Iv const a = load_iv(S, ins.a, lane);
+      if constexpr (Ext) {
+        if (op >= tape::kExtBase && op < tape::kExtEnd) {
+          bool ok = true;
+          Iv const r = ext_interval(op, a, S, ins, lane, t, ok);
+          res.feasible = res.feasible && ok;
+          store_iv(S, ins.dst, lane, r);
+          continue;
+        }
+      }
+      Iv r;
+      switch (static_cast<TOp>(op)) {
+        case TOp::LoadConst: r = iv_const(ins.imm, t); break;
+        case TOp::Add: r = iv_add(a, load_iv(S, ins.b, lane), t); break;
+        case TOp::AddImm: r = iv_add_imm(a, ins.imm, t); break;
+        case TOp::Sub: r = iv_sub(a, load_iv(S, ins.b, lane), t); break;
+        case TOp::SubImm: r = iv_sub_imm(a, ins.imm, t); break;
+        case TOp::ImmSub: r = iv_imm_sub(ins.imm, a, t); break;
+        case TOp::Mul: r = iv_mul(a, load_iv(S, ins.b, lane), t); break;
+        case TOp::MulImm: r = iv_mul_imm(a, ins.imm, t); break;
+        case TOp::Sqr: r = iv_sqr(a, t); break;
+        case TOp::Neg: r = iv_neg(a); break;
+        case TOp::CheckLE:
+          res.feasible = res.feasible && check_le(a, ins.imm);
+          continue;
+        case TOp::CheckEQ:
+          res.feasible = res.feasible && check_eq(a, ins.imm);
+          continue;
+        case TOp::Objective:
+          res.obj = a;
+          continue;
+        case TOp::Output:
+          if (roots != nullptr) {
+            roots[static_cast<std::uint64_t>(ins.b) * roots_stride + e] = to_d2(to_output(a, t));
+          }
+          continue;
+        default:
+          continue;
+      }
+      store_iv(S, ins.dst, lane, r);
     }
-    store_iv(S, ins.dst, lane, r);
+    cur = nxt;
   }
   return res;
 }
@@ -341,100 +375,107 @@ __device__ __forceinline__ double ext_point(std::uint32_t op, double a, const do
 // Round-to-nearest point tape over LDS slots P[slot * kTile + lane] and the
 // accumulator register.
 template<bool Ext>
-__device__ __forceinline__ PointResult run_point(ConstInstrPtr code, std::uint32_t n_instr, double* P, unsigned lane,
+__device__ __forceinline__ PointResult run_point(const Instr* code, std::uint32_t n_instr, double* P, unsigned lane,
                                                  double* roots, std::uint64_t roots_stride, std::uint64_t e)
 {
   PointResult res {0.0, true};
   double acc = 0.0;
-  u32x4 next = code[0];
-  for (std::uint32_t pc = 0; pc < n_instr; ++pc) {
-    Word const ins = decode_word(next);
-    next = code[pc + 1];
-    std::uint32_t const op = ins.op;
-    if (op >= tape::kTermBase) {
-      std::uint32_t const term = op & 7u;
-      std::uint32_t const mode = (op - tape::kTermBase) >> 3;
-      double const a = P[ins.a * kTile + lane];
-      double x;
-      if (term == static_cast<std::uint32_t>(tape::Term::SSC)) {
-        x = (a * P[ins.b * kTile + lane]) * ins.imm;
-      } else if (term == static_cast<std::uint32_t>(tape::Term::QC)) {
-        x = (a * a) * ins.imm;
-      } else if (term == static_cast<std::uint32_t>(tape::Term::SC)) {
-        x = a * ins.imm;
-      } else if (term == static_cast<std::uint32_t>(tape::Term::SS)) {
-        x = a * P[ins.b * kTile + lane];
-      } else if (term == static_cast<std::uint32_t>(tape::Term::Q)) {
-        x = a * a;
-      } else {
-        x = a;
-      }
-      if (mode == static_cast<std::uint32_t>(tape::AccMode::Add)) {
-        acc = acc + x;
-      } else if (mode == static_cast<std::uint32_t>(tape::AccMode::Set)) {
-        acc = x;
-      } else {
-        acc = acc - x;
-      }
-      continue;
-    }
-    if (op >= static_cast<std::uint32_t>(TOp::AccStore) && op < tape::kExtBase) {
-      switch (static_cast<TOp>(op)) {
-        case TOp::AccStore: P[ins.dst * kTile + lane] = acc; break;
-        case TOp::AccAddImm: acc = acc + ins.imm; break;
-        case TOp::AccSubImm: acc = acc - ins.imm; break;
-        case TOp::AccImmSub: acc = ins.imm - acc; break;
-        case TOp::AccScale: acc = acc * ins.imm; break;
-        case TOp::AccNeg: acc = -acc; break;
-        case TOp::AccCheckLE: res.feasible = res.feasible && (acc <= ins.imm); break;
-        case TOp::AccCheckEQ: res.feasible = res.feasible && almost_equal(acc, ins.imm); break;
-        case TOp::AccObjective: res.obj = acc; break;
-        case TOp::AccOutput:
-          if (roots != nullptr) {
-            roots[static_cast<std::uint64_t>(ins.b) * roots_stride + e] = acc;
-          }
-          break;
-        default: break;
-      }
-      continue;
-    }
-    double const a = P[ins.a * kTile + lane];
-    if constexpr (Ext) {
-      if (op >= tape::kExtBase && op < tape::kExtEnd) {
-        P[ins.dst * kTile + lane] = ext_point(op, a, P, ins, lane);
-        continue;
-      }
-    }
-    double r;
-    switch (static_cast<TOp>(op)) {
-      case TOp::LoadConst: r = ins.imm; break;
-      case TOp::Add: r = a + P[ins.b * kTile + lane]; break;
-      case TOp::AddImm: r = a + ins.imm; break;
-      case TOp::Sub: r = a - P[ins.b * kTile + lane]; break;
-      case TOp::SubImm: r = a - ins.imm; break;
-      case TOp::ImmSub: r = ins.imm - a; break;
-      case TOp::Mul: r = a * P[ins.b * kTile + lane]; break;
-      case TOp::MulImm: r = a * ins.imm; break;
-      case TOp::Sqr: r = a * a; break;
-      case TOp::Neg: r = -a; break;
-      case TOp::CheckLE:
-        res.feasible = res.feasible && (a <= ins.imm);
-        continue;
-      case TOp::CheckEQ:
-        res.feasible = res.feasible && almost_equal(a, ins.imm);
-        continue;
-      case TOp::Objective:
-        res.obj = a;
-        continue;
-      case TOp::Output:
-        if (roots != nullptr) {
-          roots[static_cast<std::uint64_t>(ins.b) * roots_stride + e] = a;
+  // the tape arrives one 64-word chunk at a time, the next chunk in flight
+  // while this one runs
+  const u32x4* const words = reinterpret_cast<const u32x4*>(code);
+  u32x4 cur = tape_chunk(words, lane, n_instr);
+  for (std::uint32_t c0 = 0; c0 < n_instr; c0 += kTile) {
+    u32x4 const nxt = tape_chunk(words, c0 + kTile + lane, n_instr);
+    std::uint32_t const end = std::min<std::uint32_t>(n_instr - c0, kTile);
+    for (std::uint32_t slot = 0; slot < end; ++slot) {
+      Word const ins = decode_word(lane_word(cur, slot));
+      std::uint32_t const op = ins.op;
+      if (op >= tape::kTermBase) {
+        std::uint32_t const term = op & 7u;
+        std::uint32_t const mode = (op - tape::kTermBase) >> 3;
+        double const a = P[ins.a * kTile + lane];
+        double x;
+        if (term == static_cast<std::uint32_t>(tape::Term::SSC)) {
+          x = (a * P[ins.b * kTile + lane]) * ins.imm;
+        } else if (term == static_cast<std::uint32_t>(tape::Term::QC)) {
+          x = (a * a) * ins.imm;
+        } else if (term == static_cast<std::uint32_t>(tape::Term::SC)) {
+          x = a * ins.imm;
+        } else if (term == static_cast<std::uint32_t>(tape::Term::SS)) {
+          x = a * P[ins.b * kTile + lane];
+        } else if (term == static_cast<std::uint32_t>(tape::Term::Q)) {
+          x = a * a;
+        } else {
+          x = a;
+        }
+        if (mode == static_cast<std::uint32_t>(tape::AccMode::Add)) {
+          acc = acc + x;
+        } else if (mode == static_cast<std::uint32_t>(tape::AccMode::Set)) {
+          acc = x;
+        } else {
+          acc = acc - x;
         }
         continue;
-      default:
+      }
+      if (op >= static_cast<std::uint32_t>(TOp::AccStore) && op < tape::kExtBase) {
+        switch (static_cast<TOp>(op)) {
+          case TOp::AccStore: P[ins.dst * kTile + lane] = acc; break;
+          case TOp::AccAddImm: acc = acc + ins.imm; break;
+          case TOp::AccSubImm: acc = acc - ins.imm; break;
+          case TOp::AccImmSub: acc = ins.imm - acc; break;
+          case TOp::AccScale: acc = acc * ins.imm; break;
+          case TOp::AccNeg: acc = -acc; break;
+          case TOp::AccCheckLE: res.feasible = res.feasible && (acc <= ins.imm); break;
+          case TOp::AccCheckEQ: res.feasible = res.feasible && almost_equal(acc, ins.imm); break;
+          case TOp::AccObjective: res.obj = acc; break;
+          case TOp::AccOutput:
+            if (roots != nullptr) {
+              roots[static_cast<std::uint64_t>(ins.b) * roots_stride + e] = acc;
+            }
+            break;
+          default: break;
+        }
         continue;
+      }
+      double const a = P[ins.a * kTile + lane];
+      if constexpr (Ext) {
+        if (op >= tape::kExtBase && op < tape::kExtEnd) {
+          P[ins.dst * kTile + lane] = ext_point(op, a, P, ins, lane);
+          continue;
+        }
+      }
+      double r;
+      switch (static_cast<TOp>(op)) {
+        case TOp::LoadConst: r = ins.imm; break;
+        case TOp::Add: r = a + P[ins.b * kTile + lane]; break;
+        case TOp::AddImm: r = a + ins.imm; break;
+        case TOp::Sub: r = a - P[ins.b * kTile + lane]; break;
+        case TOp::SubImm: r = a - ins.imm; break;
+        case TOp::ImmSub: r = ins.imm - a; break;
+        case TOp::Mul: r = a * P[ins.b * kTile + lane]; break;
+        case TOp::MulImm: r = a * ins.imm; break;
+        case TOp::Sqr: r = a * a; break;
+        case TOp::Neg: r = -a; break;
+        case TOp::CheckLE:
+          res.feasible = res.feasible && (a <= ins.imm);
+          continue;
+        case TOp::CheckEQ:
+          res.feasible = res.feasible && almost_equal(a, ins.imm);
+          continue;
+        case TOp::Objective:
+          res.obj = a;
+          continue;
+        case TOp::Output:
+          if (roots != nullptr) {
+            roots[static_cast<std::uint64_t>(ins.b) * roots_stride + e] = a;
+          }
+          continue;
+        default:
+          continue;
+      }
+      P[ins.dst * kTile + lane] = r;
     }
-    P[ins.dst * kTile + lane] = r;
+    cur = nxt;
   }
   return res;
 }
@@ -488,7 +529,7 @@ __global__ __launch_bounds__(kTile * kMaxGroups) void k_boxes(const Instr* __res
   __syncthreads();
   std::uint32_t const begin = groups.begin[wave];
   int const t = enter_round_up();
-  IntervalResult const res = run_interval<Ext>(as_constant(code_g + begin), groups.begin[wave + 1] - begin, S, lane,
+  IntervalResult const res = run_interval<Ext>(code_g + begin, groups.begin[wave + 1] - begin, S, lane,
                                                t, active ? roots : nullptr, n_boxes, e);
   Bnd const obj = to_output(res.obj, t);
   leave_round_up(t, obj.lb, obj.ub, res.feasible);
@@ -530,11 +571,96 @@ __global__ __launch_bounds__(kTile) void k_points(const Instr* __restrict__ code
     P[v * kTile + lane] = x[v * ld + ee];
   }
   PointResult const res =
-      run_point<Ext>(as_constant(code_g), n_instr, P, lane, active ? roots : nullptr, n_points, e);
+      run_point<Ext>(code_g, n_instr, P, lane, active ? roots : nullptr, n_points, e);
   if (active) {
     feasible[e] = res.feasible ? 1 : 0;
     obj[e] = res.obj;
   }
+}
+
+
+// ---------------------------------------------------------------------------
+// An item's parent box and slot data, fetched lane-parallel (one memory round
+// trip for the whole wave) and handed out by index with v_readlane: the
+// decode loops then run on registers and LDS only.
+// ---------------------------------------------------------------------------
+
+__device__ __forceinline__ std::uint32_t lane_u32(std::uint32_t x, std::uint32_t k)
+{
+  return static_cast<std::uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(x), static_cast<int>(k)));
+}
+
+__device__ __forceinline__ double lane_f64(double x, std::uint32_t k)
+{
+  std::uint64_t const b = __builtin_bit_cast(std::uint64_t, x);
+  std::uint64_t const lo = lane_u32(static_cast<std::uint32_t>(b), k);
+  std::uint64_t const hi = lane_u32(static_cast<std::uint32_t>(b >> 32), k);
+  return __builtin_bit_cast(double, (hi << 32) | lo);
+}
+
+/// Slot j of an item in lane j (slot_count <= kMaxSlotsPerItem = kTile): its
+/// variable, width and kind, and that variable's parent bounds.
+struct LaneSlots
+{
+  std::uint32_t var, width, kind;
+  double plo, phi;
+};
+
+__device__ __forceinline__ LaneSlots lane_slots(const Item& it, unsigned lane)
+{
+  LaneSlots s {0u, 1u, 0u, 0.0, 0.0};
+  if (lane < it.slot_count) {
+    s.var = gload(it.slot_var + lane);
+    s.width = gload(it.slot_width + lane);
+    s.kind = gload(it.slot_kind + lane);
+    s.plo = gload(it.parent + 2 * s.var);
+    s.phi = gload(it.parent + 2 * s.var + 1);
+  }
+  return s;
+}
+
+/// Agent-scope relaxed load of a double another workgroup wrote (past this
+/// CU's L1).
+__device__ __forceinline__ double atomic_load_f64(const double* p)
+{
+  std::uint64_t const b = __hip_atomic_load(reinterpret_cast<const std::uint64_t*>(p), __ATOMIC_RELAXED,
+                                            __HIP_MEMORY_SCOPE_AGENT);
+  return __builtin_bit_cast(double, b);
+}
+
+/// Wave 0 of one slice of tile lt, after its groups' verdicts are combined
+/// into `feasible` (and the slice that holds the objective group has stored
+/// the tile's objective values): folds the infeasible lanes into the tile's
+/// mask and counts the slice. Returns true in the last slice of the tile to
+/// finish, with `feasible` then the verdict of every group; that slice clears
+/// the tile's mask and count for the next launch. Nothing waits on another
+/// workgroup, so the slices may run in any order.
+__device__ __forceinline__ bool join_slices(const Item& it, std::uint32_t lt, std::uint32_t n_slices, bool& feasible,
+                                            unsigned lane)
+{
+  std::uint64_t const bad = __ballot(!feasible);
+  std::uint32_t ticket = 0;
+  __threadfence();
+  if (lane == 0) {
+    if (bad != 0) {
+      atomicOr(reinterpret_cast<unsigned long long*>(it.tile_infeasible + lt), static_cast<unsigned long long>(bad));
+    }
+    __threadfence();
+    ticket = atomicAdd(it.tile_slices_done + lt, 1u);
+  }
+  ticket = __shfl(ticket, 0, 64);
+  if (ticket != n_slices - 1) {
+    return false;
+  }
+  __threadfence();
+  std::uint64_t const mask =
+      __hip_atomic_load(it.tile_infeasible + lt, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  feasible = ((mask >> lane) & 1u) == 0u;
+  if (lane == 0) {
+    it.tile_infeasible[lt] = 0;
+    it.tile_slices_done[lt] = 0;
+  }
+  return true;
 }
 
 // Decode-mode item kernels. Each workgroup is one 64-element tile of one item,
@@ -554,19 +680,35 @@ __global__ __launch_bounds__(kTile * kMaxGroups) void k_items(const Item* __rest
   unsigned const lane = threadIdx.x % kTile;
   unsigned const wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kTile);
   unsigned const n_waves = blockDim.x / kTile;
-  std::uint32_t const tile = blockIdx.x;
-  Item const& it = items[tile_item[tile]];
-  std::uint32_t const lt = tile - it.tile_begin;
+  Item const& it = items[tile_item[blockIdx.x]];
+  // workgroup -> (tile, slice): the slices of one tile are consecutive
+  std::uint32_t const n_slices = it.n_slices > 1 ? it.n_slices : 1u;
+  std::uint32_t const local = blockIdx.x - it.tile_begin;
+  std::uint32_t const lt = n_slices == 1 ? local : local / n_slices;
+  std::uint32_t const slice = local - lt * n_slices;
   std::uint64_t const e = static_cast<std::uint64_t>(lt) * kTile + lane;
   bool const active = e < it.n_elems;
   std::uint32_t const ee = static_cast<std::uint32_t>(active ? e : it.n_elems - 1);
   std::uint32_t const S = it.samples;
   std::uint32_t const region = (M == static_cast<int>(Mode::Sample)) ? ee / S : ee;
   std::uint32_t const sample = (M == static_cast<int>(Mode::Sample)) ? ee - region * S : 0;
-  std::uint32_t const n_groups = it.groups.n_groups;
-  std::uint32_t const g_begin = wave < n_groups ? it.groups.begin[wave] : 0u;
-  std::uint32_t const g_len = wave < n_groups ? it.groups.begin[wave + 1] - g_begin : 0u;
-  ConstInstrPtr const code = as_constant(it.code) + g_begin;
+  // this wave's function group, its code range, and the objective's group
+  std::uint32_t const group = slice * n_waves + wave;
+  std::uint32_t g_begin = 0, g_len = 0, objective = 0;
+  if (n_slices == 1) {
+    objective = it.groups.objective_group;
+    if (group < it.groups.n_groups) {
+      g_begin = it.groups.begin[group];
+      g_len = it.groups.begin[group + 1] - g_begin;
+    }
+  } else {
+    objective = it.slice_objective;
+    if (group < it.slice_groups) {
+      g_begin = gload(it.slice_begin + group);
+      g_len = gload(it.slice_begin + group + 1) - g_begin;
+    }
+  }
+  const Instr* const code = it.code + g_begin;
   std::uint32_t const n_vars = it.n_vars;
   constexpr bool kInterval = M == static_cast<int>(Mode::Bound) || M == static_cast<int>(Mode::Aggregate);
   // per-group verdicts [kMaxGroups][64] and the objective [64], after the slots
@@ -576,16 +718,26 @@ __global__ __launch_bounds__(kTile * kMaxGroups) void k_items(const Item* __rest
   if constexpr (kInterval) {
     double2* const Sl = lds;
     if (wave == 0) {
-      for (std::uint32_t v = 0; v < n_vars; ++v) {
-        Sl[v * kTile + lane] = make_double2(-it.parent[2 * v], it.parent[2 * v + 1]);
+      LaneSlots const ls = lane_slots(it, lane);
+      for (std::uint32_t v0 = 0; v0 < n_vars; v0 += kTile) {
+        double lo = 0.0, hi = 0.0;
+        if (v0 + lane < n_vars) {
+          lo = gload(it.parent + 2 * (v0 + lane));
+          hi = gload(it.parent + 2 * (v0 + lane) + 1);
+        }
+        std::uint32_t const cnt = std::min<std::uint32_t>(kTile, n_vars - v0);
+        for (std::uint32_t k = 0; k < cnt; ++k) {
+          Sl[(v0 + k) * kTile + lane] = make_double2(-lane_f64(lo, k), lane_f64(hi, k));
+        }
       }
       std::uint32_t idx = region;
       for (std::uint32_t j = 0; j < it.slot_count; ++j) {
-        std::uint32_t const w = it.slot_width[j];
+        std::uint32_t const w = lane_u32(ls.width, j);
         std::uint32_t const part = idx % w;
         idx /= w;
-        std::uint32_t const vid = it.slot_var[j];
-        Bounds const b = slot_bounds(it.slot_kind[j], {it.parent[2 * vid], it.parent[2 * vid + 1]}, part, w);
+        std::uint32_t const vid = lane_u32(ls.var, j);
+        Bounds const b = slot_bounds(static_cast<std::uint8_t>(lane_u32(ls.kind, j)),
+                                     {lane_f64(ls.plo, j), lane_f64(ls.phi, j)}, part, w);
         Sl[vid * kTile + lane] = make_double2(-b.lo, b.hi);
       }
     }
@@ -599,7 +751,7 @@ __global__ __launch_bounds__(kTile * kMaxGroups) void k_items(const Item* __rest
     if (n_waves > 1) {
       auto* const obj_s = reinterpret_cast<double2*>(feas_g + kMaxGroups * kTile);
       feas_g[wave * kTile + lane] = res.feasible ? 1u : 0u;
-      if (wave == it.groups.objective_group) {
+      if (group == objective) {
         obj_s[lane] = to_d2(obj);
       }
       __syncthreads();
@@ -613,6 +765,16 @@ __global__ __launch_bounds__(kTile * kMaxGroups) void k_items(const Item* __rest
       res.feasible = f;
       double2 const o = obj_s[lane];
       obj = {o.x, o.y};
+    }
+    if (n_slices > 1) {
+      auto* const tobj = reinterpret_cast<double2*>(it.tile_obj) + static_cast<std::size_t>(lt) * kTile;
+      if (objective / n_waves == slice) {
+        tobj[lane] = to_d2(obj);
+      }
+      if (!join_slices(it, lt, n_slices, res.feasible, lane)) {
+        return;
+      }
+      obj = {atomic_load_f64(&tobj[lane].x), atomic_load_f64(&tobj[lane].y)};
     }
     if constexpr (M == static_cast<int>(Mode::Bound)) {
       if (active) {
@@ -651,20 +813,37 @@ __global__ __launch_bounds__(kTile * kMaxGroups) void k_items(const Item* __rest
     double* const P = reinterpret_cast<double*>(lds);
     bool const sampling = M == static_cast<int>(Mode::Sample);
     if (wave == 0) {
-      for (std::uint32_t v = 0; v < n_vars; ++v) {
-        double const lo = it.parent[2 * v];
-        double const hi = it.parent[2 * v + 1];
-        P[v * kTile + lane] = sampling ? draw(it.var_kinds[v] != 0, lo, hi, v, sample, region, it.salt) : lo;
+      LaneSlots const ls = lane_slots(it, lane);
+      std::uint32_t skind = 0;  // lane j: kind of the variable of slot j
+      if (lane < it.slot_count) {
+        skind = gload(it.var_kinds + ls.var);
+      }
+      for (std::uint32_t v0 = 0; v0 < n_vars; v0 += kTile) {
+        double lo = 0.0, hi = 0.0;
+        std::uint32_t vk = 0;
+        if (v0 + lane < n_vars) {
+          lo = gload(it.parent + 2 * (v0 + lane));
+          hi = gload(it.parent + 2 * (v0 + lane) + 1);
+          vk = gload(it.var_kinds + v0 + lane);
+        }
+        std::uint32_t const cnt = std::min<std::uint32_t>(kTile, n_vars - v0);
+        for (std::uint32_t k = 0; k < cnt; ++k) {
+          std::uint32_t const v = v0 + k;
+          double const l = lane_f64(lo, k);
+          P[v * kTile + lane] =
+              sampling ? draw(lane_u32(vk, k) != 0, l, lane_f64(hi, k), v, sample, region, it.salt) : l;
+        }
       }
       std::uint32_t idx = region;
       for (std::uint32_t j = 0; j < it.slot_count; ++j) {
-        std::uint32_t const w = it.slot_width[j];
+        std::uint32_t const w = lane_u32(ls.width, j);
         std::uint32_t const part = idx % w;
         idx /= w;
-        std::uint32_t const vid = it.slot_var[j];
-        Bounds const b = slot_bounds(it.slot_kind[j], {it.parent[2 * vid], it.parent[2 * vid + 1]}, part, w);
+        std::uint32_t const vid = lane_u32(ls.var, j);
+        Bounds const b = slot_bounds(static_cast<std::uint8_t>(lane_u32(ls.kind, j)),
+                                     {lane_f64(ls.plo, j), lane_f64(ls.phi, j)}, part, w);
         P[vid * kTile + lane] =
-            sampling ? draw(it.var_kinds[vid] != 0, b.lo, b.hi, vid, sample, region, it.salt) : b.lo;
+            sampling ? draw(lane_u32(skind, j) != 0, b.lo, b.hi, vid, sample, region, it.salt) : b.lo;
       }
     }
     if (n_waves > 1) {
@@ -674,7 +853,7 @@ __global__ __launch_bounds__(kTile * kMaxGroups) void k_items(const Item* __rest
     if (n_waves > 1) {
       auto* const obj_s = reinterpret_cast<double*>(feas_g + kMaxGroups * kTile);
       feas_g[wave * kTile + lane] = res.feasible ? 1u : 0u;
-      if (wave == it.groups.objective_group) {
+      if (group == objective) {
         obj_s[lane] = res.obj;
       }
       __syncthreads();
@@ -687,6 +866,16 @@ __global__ __launch_bounds__(kTile * kMaxGroups) void k_items(const Item* __rest
       }
       res.feasible = f;
       res.obj = obj_s[lane];
+    }
+    if (n_slices > 1) {
+      double* const tobj = it.tile_obj + static_cast<std::size_t>(lt) * kTile;
+      if (objective / n_waves == slice) {
+        tobj[lane] = res.obj;
+      }
+      if (!join_slices(it, lt, n_slices, res.feasible, lane)) {
+        return;
+      }
+      res.obj = atomic_load_f64(&tobj[lane]);
     }
     // a point needs a finite objective (log(0) = -inf is no incumbent)
     double v = (active && res.feasible) ? res.obj : INFINITY;
@@ -731,19 +920,29 @@ __global__ __launch_bounds__(kTile * kMaxGroups) void k_items(const Item* __rest
     std::uint32_t const best = found ? static_cast<std::uint32_t>(bi) : 0;
     std::uint32_t const b_region = sampling ? best / S : best;
     std::uint32_t const b_sample = sampling ? best - b_region * S : 0;
-    for (std::uint32_t v = lane; v < n_vars; v += kTile) {
-      double lo = it.parent[2 * v];
-      double hi = it.parent[2 * v + 1];
+    LaneSlots const ls = lane_slots(it, lane);
+    for (std::uint32_t v0 = 0; v0 < n_vars; v0 += kTile) {
+      std::uint32_t const v = v0 + lane;
+      double lo = 0.0, hi = 0.0;
+      if (v < n_vars) {
+        lo = gload(it.parent + 2 * v);
+        hi = gload(it.parent + 2 * v + 1);
+      }
       std::uint32_t bidx = b_region;
       for (std::uint32_t j = 0; j < it.slot_count; ++j) {
-        std::uint32_t const w = it.slot_width[j];
+        std::uint32_t const w = lane_u32(ls.width, j);
+        std::uint32_t const vj = lane_u32(ls.var, j);
+        auto const kj = static_cast<std::uint8_t>(lane_u32(ls.kind, j));
         std::uint32_t const part = bidx % w;
         bidx /= w;
-        if (it.slot_var[j] == v) {
-          Bounds const b = slot_bounds(it.slot_kind[j], {lo, hi}, part, w);
+        if (vj == v) {
+          Bounds const b = slot_bounds(kj, {lo, hi}, part, w);
           lo = b.lo;
           hi = b.hi;
         }
+      }
+      if (v >= n_vars) {
+        continue;
       }
       double const x = sampling ? draw(it.var_kinds[v] != 0, lo, hi, v, b_sample, b_region, it.salt) : lo;
       it.witness[v] = found ? x : NAN;

@@ -240,3 +240,34 @@ def test_regrouping_after_role_build(ctx):
         tape.set_box_groups(g)
         f_d, lb_d = bnd.bound(a.box, vids, widths)
         assert np.array_equal(f_d, f_o) and same_bits(lb_d, lb_o)
+
+
+@pytest.mark.parametrize("n,m", [(40, 20), (64, 32)])
+def test_sliced_items_on_a_long_tape_match_oracle(ctx, n, m):
+    """A launch of a few tiles spreads each tile's function groups over
+    several workgroups (one wave per SIMD each); a tape of 4,096 words or more
+    with more functions than a workgroup holds waves runs one group per
+    function. The slices' verdicts and objective meet across workgroups, and
+    the bounder, sampler and enumerator still give the oracle's results."""
+    eng = cmi.EngineProblem.qcqp(50 + n, n=n, m=m, n_int=8, int_lb=0.0, int_ub=3.0, cont_lb=-5.0, cont_ub=5.0,
+                                 pair_density=0.2)
+    info, _ = eng.tape_info()
+    assert info.box_instr >= 4096 and m + 1 > 16
+    a = eng.arrays()
+    tape = cmi.Tape(ctx, eng)
+    kinds, vids, widths = [C.SLOT_CONTINUOUS] * 2, [9, 30], [4, 4]
+    bnd = cmi.Role(tape, C.ROLE_BOUNDER, kinds, 16)
+    for _ in range(2):  # the per-tile scratch is left zeroed for the next launch
+        f_d, lb_d = bnd.bound(a.box, vids, widths)
+        f_o, lb_o = orc.bound(a, a.box, vids, widths, kinds)
+        assert np.array_equal(f_d, f_o) and same_bits(lb_d, lb_o)
+    smp = cmi.Role(tape, C.ROLE_SAMPLER, kinds, 16, samples=16)
+    for salt in (3, 4):
+        d = smp.sample(a.box, vids, widths, salt)
+        found, value, index, point = orc.candidate(a, a.box, vids, widths, kinds, 16, salt)
+        assert d.found == found and same_bits([d.value], [value]) and d.index == index and same_bits(d.point, point)
+    ekinds, evids, ewidths = [C.SLOT_INTEGER_ENUMERATE] * 3, [0, 3, 6], [4, 4, 4]
+    enu = cmi.Role(tape, C.ROLE_ENUMERATOR, ekinds, 64)
+    d = enu.enumerate(a.box, evids, ewidths)
+    found, value, index, point = orc.candidate(a, a.box, evids, ewidths, ekinds, 0, 0)
+    assert d.found == found and same_bits([d.value], [value]) and d.index == index and same_bits(d.point, point)
