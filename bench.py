@@ -310,6 +310,7 @@ def serve_comparison(ctx, n_req: int = 128, rates=(50.0, 200.0, 2000.0), seed: i
             "served_p50_ms_converged": float(np.percentile(lat_s[conv], 50) * 1e3) if conv.any() else None,
             "steps": int(st_s.steps),
             "peak_active": int(st_s.peak_active),
+            "step_kernels": kernel_roofline(st_s),
         }
         if plain:
             outs_p, fin_p, lat_p, st_p = cmi.serve_plain(ctx, st.problems, arrival, **opts)
@@ -343,7 +344,7 @@ def batch_comparison(ctx, n_inst: int = 512, iter_limit: int = 100):
     opts = dict(partition_num=2, enumerate_cap=2, max_cycle_size=4, sample_points=16, iter_limit=iter_limit,
                 tolerance=1e-6)
     t0 = time.perf_counter()
-    outs_b, _ = cmi.solve_batch(ctx, b.problems, **opts)
+    outs_b, _, st_b = cmi.solve_batch_stats(ctx, b.problems, **opts)
     t_b = time.perf_counter() - t0
     t0 = time.perf_counter()
     outs_p = [cmi.solve(ctx, p, **opts)[0] for p in b.problems]
@@ -360,6 +361,25 @@ def batch_comparison(ctx, n_inst: int = 512, iter_limit: int = 100):
         "plain_converged_solves_per_s": conv / t_p,
         "iterations_total": int(sum(o.iterations for o in outs_b)),
         "results_identical": _same_results(outs_b, outs_p),
+        "steps": int(st_b.steps),
+        "step_kernels": kernel_roofline(st_b),
+    }
+
+
+def kernel_roofline(st) -> dict:
+    """The decode-mode launches of a batched or served run, from the engine's
+    own HIP events and FP64 accounting: device time, its share of the wall
+    time, and the algorithmic FP64 rate against the FP64 vector peak."""
+    tflops = st.kernel_fp64_ops / st.kernel_seconds / 1e12 if st.kernel_seconds > 0 else 0.0
+    return {
+        "bound": "fp64",
+        "kernel_seconds": st.kernel_seconds,
+        "kernel_share_of_wall": st.kernel_seconds / st.wall_seconds if st.wall_seconds > 0 else None,
+        "fp64_ops": st.kernel_fp64_ops,
+        "achieved_tflops": tflops,
+        "peak_tflops": FP64_PEAK_TFLOPS,
+        "frac": tflops / FP64_PEAK_TFLOPS,
+        "ms_per_step": st.kernel_seconds * 1e3 / st.steps if st.steps else None,
     }
 
 

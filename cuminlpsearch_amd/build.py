@@ -90,7 +90,7 @@ def build_engine(jobs: int = 8) -> pathlib.Path:
     with cf.ThreadPoolExecutor(max_workers=max(1, jobs)) as ex:
         list(ex.map(_run, cmds))
     tmp = ENGINE_SO.with_suffix(".so.tmp")
-    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", str(tmp), *map(str, objs), "-lhiprtc"])
+    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-Wl,-z,defs", "-o", str(tmp), *map(str, objs), "-lhiprtc"])
     os.replace(tmp, ENGINE_SO)
     return ENGINE_SO
 

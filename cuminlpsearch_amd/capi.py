@@ -195,7 +195,8 @@ class ServerOptions(C.Structure):
 
 class ServeStats(C.Structure):
     _fields_ = [("steps", C.c_uint64), ("launches", C.c_uint64), ("peak_active", C.c_size_t),
-                ("wall_seconds", C.c_double), ("peak_resident_bytes", C.c_size_t)]
+                ("wall_seconds", C.c_double), ("peak_resident_bytes", C.c_size_t), ("kernel_seconds", C.c_double),
+                ("kernel_fp64_ops", C.c_double)]
 
 
 class GamsOptions(C.Structure):
@@ -287,6 +288,8 @@ _SIGNATURES = {
                                       C.POINTER(C.c_int)]),
     "cmi_solve_batch": (C.c_int, [_P, C.c_size_t, C.POINTER(_P), C.POINTER(SolveOptions), C.POINTER(SolveOutcome),
                                   C.POINTER(_P)]),
+    "cmi_solve_batch_stats": (C.c_int, [_P, C.c_size_t, C.POINTER(_P), C.POINTER(SolveOptions),
+                                        C.POINTER(SolveOutcome), C.POINTER(_P), C.POINTER(ServeStats)]),
     "cmi_serve_stream": (C.c_int, [_P, C.c_size_t, C.POINTER(_P), _P, C.POINTER(SolveOptions),
                                    C.POINTER(ServerOptions), C.POINTER(SolveOutcome), _P, _P,
                                    C.POINTER(ServeStats)]),

@@ -203,6 +203,18 @@ Workers& Runner::workers()
 Runner::Runner(std::shared_ptr<hip::Context> ctx)
     : ctx_(std::move(ctx))
 {
+  ctx_->activate();
+  for (hipEvent_t& e : ev_) {
+    hip::check(hipEventCreate(&e), "hipEventCreate");
+  }
+}
+
+Runner::~Runner()
+{
+  ctx_->activate();
+  for (hipEvent_t e : ev_) {
+    (void)hipEventDestroy(e);
+  }
 }
 
 void Runner::ensure(hip::DeviceBuffer& d, std::size_t bytes)
@@ -254,6 +266,7 @@ void Runner::run(std::span<const Request> reqs)
   std::uint32_t pass_waves[3] = {0, 0, 0};
   {
     std::size_t pass_tiles[3] = {0, 0, 0};
+    std::size_t pass_words[3] = {0, 0, 0};
     std::uint32_t pass_groups[3] = {1, 1, 1};
     for (std::size_t k = 0; k < n; ++k) {
       int const pass = k < counts[0] ? 0 : k < counts[0] + counts[1] ? 1 : 2;
@@ -262,9 +275,11 @@ void Runner::run(std::span<const Request> reqs)
           * (r.mode == dev::Mode::Sample ? r.samples : 1);
       pass_tiles[pass] += (elems + dev::kTile - 1) / dev::kTile;
       pass_groups[pass] = std::max(pass_groups[pass], r.tape->item_groups_max());
+      pass_words[pass] = std::max<std::size_t>(pass_words[pass], r.tape->n_instr());
     }
     for (int pass = 0; pass < 3; ++pass) {
-      pass_waves[pass] = hip::DeviceTape::slice_waves(pass_groups[pass], pass_tiles[pass], ctx_->n_cus());
+      pass_waves[pass] =
+          hip::DeviceTape::slice_waves(pass_groups[pass], pass_tiles[pass], ctx_->n_cus(), pass_words[pass]);
     }
   }
   if (int const forced = item_waves_override(); forced >= 0) {
@@ -278,6 +293,7 @@ void Runner::run(std::span<const Request> reqs)
   std::uint32_t waves[3] = {1, 1, 1};  // most waves of any item's workgroup
   bool ext[3] = {false, false, false};
   plans_.resize(n);
+  double step_ops = 0.0;
   for (std::size_t k = 0; k < n; ++k) {
     std::size_t const i = order[k];
     Request const& r = reqs[i];
@@ -300,6 +316,8 @@ void Runner::run(std::span<const Request> reqs)
     p.n_regions = n_regions;
     p.n_elems = n_elems;
     p.n_vars = r.tape->n_vars();
+    step_ops += static_cast<double>(n_elems)
+        * static_cast<double>(r.mode == dev::Mode::Bound ? r.tape->tape().flops : r.tape->tape().reachable_nodes);
     hip::DeviceTape::ItemPlan& plan = plans_[k];
     plan = r.tape->item_plan(r.mode, pass_waves[pass]);
     std::size_t const n_tiles = (n_elems + dev::kTile - 1) / dev::kTile;
@@ -470,6 +488,11 @@ void Runner::run(std::span<const Request> reqs)
   }
 
   check(hipMemcpyAsync(d, h, staging_bytes, hipMemcpyHostToDevice, s), "hipMemcpyAsync(batch staging)");
+  // the launches of every kTimedEvery-th run are bracketed by events
+  bool const timed = runs_++ % kTimedEvery == 0;
+  if (timed) {
+    check(hipEventRecord(ev_[0], s), "hipEventRecord");
+  }
   std::size_t first_item = 0, first_tile = 0;
   for (int pass = 0; pass < 3; ++pass) {
     dev::Mode const m = pass == 0 ? dev::Mode::Sample : pass == 1 ? dev::Mode::Enumerate : dev::Mode::Bound;
@@ -483,9 +506,25 @@ void Runner::run(std::span<const Request> reqs)
     first_item += counts[pass];
     first_tile += tiles[pass];
   }
+  if (timed) {
+    check(hipEventRecord(ev_[1], s), "hipEventRecord");
+  }
   check(hipMemcpyAsync(results_h_.data(), results_d_.get(), results_bytes_, hipMemcpyDeviceToHost, s),
         "hipMemcpyAsync(batch results)");
   check(hipStreamSynchronize(s), "hipStreamSynchronize");
+  if (timed) {
+    float ms = 0.0f;
+    check(hipEventElapsedTime(&ms, ev_[0], ev_[1]), "hipEventElapsedTime");
+    timed_seconds_ += static_cast<double>(ms) * 1e-3;
+    timed_ops_ += step_ops;
+  }
+  fp64_ops_ += step_ops;
+}
+
+double Runner::kernel_seconds() const
+{
+  // the timed runs' device time per FP64 operation, over all runs' operations
+  return timed_ops_ > 0.0 ? timed_seconds_ * (fp64_ops_ / timed_ops_) : timed_seconds_;
 }
 
 std::span<const unsigned char> Runner::feasible(std::size_t i) const
@@ -659,7 +698,7 @@ std::size_t step(Runner& runner, std::span<Member> members, const cmi_solve_opti
 
 std::vector<Result> solve_batch(std::shared_ptr<hip::Context> ctx,
                                 std::span<const model::Problem<double>* const> problems,
-                                const cmi_solve_options& o)
+                                const cmi_solve_options& o, BatchStats* stats)
 {
   auto const t0 = std::chrono::steady_clock::now();
   std::size_t const n = problems.size();
@@ -690,8 +729,10 @@ std::vector<Result> solve_batch(std::shared_ptr<hip::Context> ctx,
   std::vector<bool> done(n, false);
   Runner runner(ctx);
   std::size_t remaining = n;
+  std::uint64_t steps = 0;
   while (remaining > 0) {
     step(runner, members, o, *ctx);
+    ++steps;
     double const now = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     for (std::size_t i = 0; i < n; ++i) {
       if (!done[i] && states[i]->finished()) {
@@ -701,6 +742,13 @@ std::vector<Result> solve_batch(std::shared_ptr<hip::Context> ctx,
         --remaining;
       }
     }
+  }
+  if (stats != nullptr) {
+    stats->steps = steps;
+    stats->launches = runner.launches();
+    stats->wall_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    stats->kernel_seconds = runner.kernel_seconds();
+    stats->fp64_ops = runner.fp64_ops();
   }
   return results;
 }

@@ -71,6 +71,9 @@ class Runner
 public:
   explicit Runner(std::shared_ptr<hip::Context> ctx);
 
+  /// One launch per role kind over every request's items, with the waves
+  /// per workgroup of each launch from DeviceTape::slice_waves
+  /// (CMI_ITEM_WAVES fixes them).
   void run(std::span<const Request> reqs);
 
   std::span<const unsigned char> feasible(std::size_t i) const;
@@ -79,6 +82,19 @@ public:
   std::size_t n_elems(std::size_t i) const { return slots_[i].n_elems; }
 
   std::uint64_t launches() const { return launches_; }
+  /// Algorithmic FP64 work of every run's launches: the whole tape's
+  /// interval operations per region of a bound item, its DAG operations per
+  /// point of a sampler or enumerator item; and their device time, from HIP
+  /// events on the context's stream around every kTimedEvery-th run's
+  /// launches, scaled to all runs by their FP64 work (a pair of event records
+  /// on every step cost 2 % of the served rate of small requests).
+  double kernel_seconds() const;
+  double fp64_ops() const { return fp64_ops_; }
+  static constexpr std::uint64_t kTimedEvery = 8;
+
+  Runner(const Runner&) = delete;
+  Runner& operator=(const Runner&) = delete;
+  ~Runner();
 
   /// Host threads for the per-solve halves of a large step (started on first use).
   Workers& workers();
@@ -109,6 +125,10 @@ private:
   std::size_t done_capacity_ = 0;
   std::size_t off_obj_lb_ = 0, off_cand_ = 0, off_witness_ = 0, results_bytes_ = 0;
   std::uint64_t launches_ = 0;
+  std::uint64_t runs_ = 0;
+  double fp64_ops_ = 0.0;
+  double timed_seconds_ = 0.0, timed_ops_ = 0.0;  // the timed runs
+  hipEvent_t ev_[2] = {nullptr, nullptr};  // around a run's launches
   std::unique_ptr<Workers> workers_;
 };
 
@@ -141,10 +161,21 @@ struct Result
   double seconds = 0.0;  // from the batch start to this solve's completion
 };
 
+/// What a batch did: lock-step steps, launches, and the Runner's device time
+/// and FP64 work.
+struct BatchStats
+{
+  std::uint64_t steps = 0;
+  std::uint64_t launches = 0;
+  double wall_seconds = 0.0;
+  double kernel_seconds = 0.0;
+  double fp64_ops = 0.0;
+};
+
 /// Solve every problem with the same options, advancing all unfinished
 /// searches one iteration per step.
 std::vector<Result> solve_batch(std::shared_ptr<hip::Context> ctx,
                                 std::span<const model::Problem<double>* const> problems,
-                                const cmi_solve_options& options);
+                                const cmi_solve_options& options, BatchStats* stats = nullptr);
 
 }  // namespace cmi::batch

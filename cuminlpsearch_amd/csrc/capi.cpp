@@ -1120,6 +1120,13 @@ cmi_status cmi_aggregate_solve(cmi_ctx* ctx, const cmi_problem* p, const cmi_agg
 cmi_status cmi_solve_batch(cmi_ctx* ctx, size_t n, const cmi_problem* const* problems, const cmi_solve_options* o,
                            cmi_solve_outcome* outs, double* const* best_points)
 {
+  return cmi_solve_batch_stats(ctx, n, problems, o, outs, best_points, nullptr);
+}
+
+cmi_status cmi_solve_batch_stats(cmi_ctx* ctx, size_t n, const cmi_problem* const* problems,
+                                 const cmi_solve_options* o, cmi_solve_outcome* outs, double* const* best_points,
+                                 cmi_serve_stats* stats)
+{
   if (ctx == nullptr || (n > 0 && (problems == nullptr || outs == nullptr)) || o == nullptr) {
     return fail(CMI_INVALID_HANDLE, "null argument");
   }
@@ -1131,10 +1138,20 @@ cmi_status cmi_solve_batch(cmi_ctx* ctx, size_t n, const cmi_problem* const* pro
       }
       ps[i] = &problems[i]->problem;
     }
-    auto results = cmi::batch::solve_batch(ctx->ctx, ps, *o);
+    cmi::batch::BatchStats bs;
+    auto results = cmi::batch::solve_batch(ctx->ctx, ps, *o, &bs);
     for (std::size_t i = 0; i < n; ++i) {
       fill_outcome(results[i].outcome, results[i].seconds, &outs[i],
                    best_points != nullptr ? best_points[i] : nullptr, ps[i]->box_bounds.size());
+    }
+    if (stats != nullptr) {
+      *stats = cmi_serve_stats {};
+      stats->steps = bs.steps;
+      stats->launches = bs.launches;
+      stats->peak_active = n;
+      stats->wall_seconds = bs.wall_seconds;
+      stats->kernel_seconds = bs.kernel_seconds;
+      stats->kernel_fp64_ops = bs.fp64_ops;
     }
   });
 }

@@ -251,9 +251,9 @@ DeviceTape::ItemPlan DeviceTape::item_plan_whole() const
   return p;
 }
 
-std::uint32_t DeviceTape::slice_waves(std::uint32_t groups, std::size_t tiles, std::uint32_t n_cus)
+std::uint32_t DeviceTape::slice_waves(std::uint32_t groups, std::size_t tiles, std::uint32_t n_cus, std::size_t words)
 {
-  if (groups <= kWidePeriod) {
+  if (groups <= kWidePeriod || words < kSliceMinWords) {
     return 0;
   }
   if (tiles * 2 <= n_cus) {
@@ -268,7 +268,7 @@ std::uint32_t DeviceTape::slice_waves(std::uint32_t groups, std::size_t tiles, s
 std::uint32_t DeviceTape::item_waves_for(dev::Mode mode, std::size_t n_tiles) const
 {
   (void)mode;
-  return slice_waves(item_groups_max(), n_tiles, ctx_->n_cus());
+  return slice_waves(item_groups_max(), n_tiles, ctx_->n_cus(), tape_.code.size());
 }
 
 void DeviceTape::item_code(dev::Item& it, const ItemPlan& plan) const

@@ -307,14 +307,17 @@ public:
   /// (slice_waves).
   std::uint32_t item_waves_for(dev::Mode mode, std::size_t n_tiles) const;
   /// Waves per workgroup of a decode-mode launch of `tiles` tiles whose
-  /// items have at most `groups` function groups, on n_cus compute units:
-  /// one wave per SIMD (4) when the tiles cover at most half the compute
-  /// units, so that each tile's groups spread over several of them; 8 when
-  /// they cover at most two workgroups per unit; else 0, every item's groups
-  /// in one workgroup. Measured on 32 served QCQPs of up to 93 variables
-  /// (4: 1.7x faster than 0) and 512 batched ones of up to 48 (8: 4 %
-  /// faster than 0; 4: 15 % slower).
-  static std::uint32_t slice_waves(std::uint32_t groups, std::size_t tiles, std::uint32_t n_cus);
+  /// items have at most `groups` function groups and `words` tape words, on
+  /// n_cus compute units: 0 (every item's groups in one workgroup) unless
+  /// some tape has kSliceMinWords words; then one wave per SIMD (4) when the
+  /// tiles cover at most half the compute units, so that each tile's groups
+  /// spread over several of them, 8 when they cover at most two workgroups
+  /// per unit, else 0. Measured on 32 served QCQPs of up to 93 variables
+  /// (4: 1.7x faster than 0), 512 batched ones of up to 48 (8: 4 % faster
+  /// than 0; 4: 15 % slower) and streams of up to 16 variables (tapes of at
+  /// most 531 words: slicing cost 10 % of the served rate).
+  static std::uint32_t slice_waves(std::uint32_t groups, std::size_t tiles, std::uint32_t n_cus, std::size_t words);
+  static constexpr std::size_t kSliceMinWords = 2048;
   /// Fill the Item's code fields from a plan (slice scratch is the caller's).
   void item_code(dev::Item& it, const ItemPlan& plan) const;
   std::uint32_t item_slots(bool grouped) const { return grouped ? grouped_.n_slots : tape_.n_slots; }

@@ -179,6 +179,8 @@ extern "C" cmi_status cmi_serve_stream(cmi_ctx* ctx, size_t n, const cmi_problem
           a.stats->peak_active = peak;
           a.stats->wall_seconds = elapsed();
           a.stats->peak_resident_bytes = peak_bytes;
+          a.stats->kernel_seconds = runner.kernel_seconds();
+          a.stats->kernel_fp64_ops = runner.fp64_ops();
         }
       },
       &a);

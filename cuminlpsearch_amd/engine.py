@@ -431,14 +431,22 @@ def solve(ctx: Context, problem: EngineProblem, **opts):
 
 
 def solve_batch(ctx: Context, problems: list, **opts):
+    outs, bests, _ = solve_batch_stats(ctx, problems, **opts)
+    return outs, bests
+
+
+def solve_batch_stats(ctx: Context, problems: list, **opts):
+    """solve_batch, also returning its ServeStats (steps, launches, wall time,
+    device time and FP64 work of the decode-mode launches)."""
     o = solve_options(**opts)
     n = len(problems)
     outs = (_c.SolveOutcome * n)()
     hs = (C.c_void_p * n)(*[p.handle.value for p in problems])
     bests = [np.zeros(p.info().n_vars) for p in problems]
     bp = (C.c_void_p * n)(*[b.ctypes.data for b in bests])
-    _c.check(_c.lib().cmi_solve_batch(ctx.handle, n, hs, C.byref(o), outs, bp))
-    return list(outs), bests
+    st = _c.ServeStats()
+    _c.check(_c.lib().cmi_solve_batch_stats(ctx.handle, n, hs, C.byref(o), outs, bp, C.byref(st)))
+    return list(outs), bests, st
 
 
 def serve_stream(ctx: Context, problems: list, arrival, max_active: int = 256, hbm_budget_bytes: int = 0,

@@ -767,7 +767,18 @@ typedef struct
   size_t peak_active;   /* most solves resident at once */
   double wall_seconds;  /* first admission to last completion */
   size_t peak_resident_bytes; /* largest sum of the resident solves' admission footprints (0 for serve_plain) */
+  double kernel_seconds;  /* device time of the steps' decode-mode launches: HIP events around every 8th
+                             step's launches, scaled by FP64 work to all steps (0 for serve_plain) */
+  double kernel_fp64_ops; /* their algorithmic FP64 work: interval ops per region, DAG ops per point */
 } cmi_serve_stats;
+
+/* cmi_solve_batch, also reporting its steps, launches, wall time and the
+ * device time and FP64 work of its decode-mode launches (stats may be NULL;
+ * peak_active is the batch size, peak_resident_bytes 0). No reference
+ * counterpart: it instruments cmi_solve_batch. */
+cmi_status cmi_solve_batch_stats(cmi_ctx* ctx, size_t n, const cmi_problem* const* problems,
+                                 const cmi_solve_options* o, cmi_solve_outcome* outs, double* const* best_points,
+                                 cmi_serve_stats* stats);
 
 /* Continuous batching: requests are admitted as they arrive while a slot and
  * the HBM budget are free (a solve is charged its tape plus the memory-fit
