@@ -825,6 +825,54 @@ inline Tape compile(const model::Problem<double>& p, const CompileOptions& opt =
   return t;
 }
 
+namespace detail
+{
+
+/// Work of each function (0 = objective, 1 + j = constraint j) for balancing
+/// function groups: the non-constant DAG nodes it reaches, a node shared by
+/// several functions counted once for each (as each group evaluates it). One
+/// backward sweep carries a bit per function over the topologically ordered
+/// nodes, instead of one compile per function.
+inline std::vector<std::size_t> root_work(const model::Problem<double>& p)
+{
+  auto const& nodes = p.graph.nodes;
+  std::size_t const n_roots = 1 + p.constraints.size();
+  std::size_t const words = (n_roots + 63) / 64;
+  std::vector<std::uint64_t> bits(nodes.size() * words, 0);
+  auto mark = [&](std::size_t node, std::size_t k) { bits[node * words + k / 64] |= std::uint64_t(1) << (k % 64); };
+  mark(p.objective_root, 0);
+  for (std::size_t j = 0; j < p.constraints.size(); ++j) {
+    mark(p.constraints[j].root_id, 1 + j);
+  }
+  std::vector<std::size_t> work(n_roots, 0);
+  for (std::size_t i = nodes.size(); i-- > 0;) {
+    std::uint64_t const* const b = &bits[i * words];
+    bool any = false;
+    for (std::size_t w = 0; w < words; ++w) {
+      any = any || b[w] != 0;
+    }
+    if (!any) {
+      continue;
+    }
+    for (std::size_t in : nodes[i].in) {
+      for (std::size_t w = 0; w < words; ++w) {
+        bits[in * words + w] |= b[w];
+      }
+    }
+    if (nodes[i].op == model::Op::Const) {
+      continue;
+    }
+    for (std::size_t w = 0; w < words; ++w) {
+      for (std::uint64_t x = b[w]; x != 0; x &= x - 1) {
+        ++work[w * 64 + static_cast<std::size_t>(__builtin_ctzll(x))];
+      }
+    }
+  }
+  return work;
+}
+
+}  // namespace detail
+
 /// Compile the objective and constraints into `n_groups` independent groups
 /// (each one wave's program over the same box), balancing instruction counts
 /// by longest-processing-time assignment. A node that several groups need is
@@ -841,17 +889,25 @@ inline Tape compile_groups(const model::Problem<double>& p, std::uint32_t n_grou
   if (n_groups == 1) {
     return compile(p, opt);
   }
-  std::vector<std::pair<std::size_t, std::size_t>> cost;  // (instructions, root index)
-  for (std::size_t k = 0; k < n_roots; ++k) {
-    cost.push_back({detail::Compiler(p, opt).run({k}).code.size(), k});
-  }
-  std::stable_sort(cost.begin(), cost.end(), [](auto const& a, auto const& b) { return a.first > b.first; });
   std::vector<std::vector<std::size_t>> members(n_groups);
-  std::vector<std::size_t> load(n_groups, 0);
-  for (auto const& [c, k] : cost) {
-    std::size_t const g = static_cast<std::size_t>(std::min_element(load.begin(), load.end()) - load.begin());
-    members[g].push_back(k);
-    load[g] += c;
+  if (n_groups == n_roots) {
+    // one function per group
+    for (std::size_t k = 0; k < n_roots; ++k) {
+      members[k].push_back(k);
+    }
+  } else {
+    std::vector<std::pair<std::size_t, std::size_t>> cost;  // (work, root index)
+    std::vector<std::size_t> const work = detail::root_work(p);
+    for (std::size_t k = 0; k < n_roots; ++k) {
+      cost.push_back({work[k], k});
+    }
+    std::stable_sort(cost.begin(), cost.end(), [](auto const& a, auto const& b) { return a.first > b.first; });
+    std::vector<std::size_t> load(n_groups, 0);
+    for (auto const& [c, k] : cost) {
+      std::size_t const g = static_cast<std::size_t>(std::min_element(load.begin(), load.end()) - load.begin());
+      members[g].push_back(k);
+      load[g] += c;
+    }
   }
   std::vector<Tape> parts;
   Tape t;

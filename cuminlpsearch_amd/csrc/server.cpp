@@ -105,38 +105,73 @@ extern "C" cmi_status cmi_serve_stream(cmi_ctx* ctx, size_t n, const cmi_problem
         std::vector<cmi::batch::Member> members;
         std::size_t next = 0, completed = 0, used_bytes = 0, peak = 0, peak_bytes = 0;
         std::uint64_t steps = 0;
-        // The head of the queue is priced once, however many steps it waits.
-        std::size_t priced = a.n, priced_bytes = 0;
+        // Each request is priced once, however many steps it waits.
+        std::vector<std::size_t> price(a.n, 0);
+        std::vector<bool> priced(a.n, false);
+        std::vector<std::size_t> cand, admit;
+        std::vector<Active> fresh;
         auto const t0 = Clock::now();
         auto elapsed = [&] { return std::chrono::duration<double>(Clock::now() - t0).count(); };
 
         while (completed < a.n) {
           double const now = elapsed();
-          // Admission: arrived, a slot free, and the device budget fits.
-          while (next < a.n && active.size() < a.so->max_active && arrive(order[next]) <= now) {
-            std::size_t const i = order[next];
-            if (priced != i) {
-              priced_bytes = resident_bytes(a.problems[i]->problem, *a.o);
-              priced = i;
+          // Admission: arrived, a slot free, and the device budget fits. The
+          // candidates are priced, and the admitted ones' tapes compiled and
+          // uploaded, on the runner's host workers (a long tape takes tens of
+          // milliseconds to compile); admission itself stays in arrival order.
+          cand.clear();
+          while (next + cand.size() < a.n && active.size() + cand.size() < a.so->max_active
+                 && arrive(order[next + cand.size()]) <= now) {
+            cand.push_back(order[next + cand.size()]);
+          }
+          if (!cand.empty()) {
+            auto price_one = [&](std::size_t k) {
+              if (!priced[cand[k]]) {
+                price[cand[k]] = resident_bytes(a.problems[cand[k]]->problem, *a.o);
+              }
+            };
+            if (cand.size() == 1) {
+              price_one(0);
+            } else {
+              runner.workers().run(cand.size(), price_one);
             }
-            if (a.so->hbm_budget_bytes != 0 && !active.empty() && used_bytes + priced_bytes > a.so->hbm_budget_bytes) {
-              break;  // wait for a resident solve to finish
+            admit.clear();
+            for (std::size_t i : cand) {
+              priced[i] = true;
+              if (a.so->hbm_budget_bytes != 0 && !(active.empty() && admit.empty())
+                  && used_bytes + price[i] > a.so->hbm_budget_bytes) {
+                break;  // wait for a resident solve to finish
+              }
+              used_bytes += price[i];
+              admit.push_back(i);
             }
-            // A tape that cannot be built stops its own solve (DeviceMemory at
-            // its first iteration, as a plain solve's first role build would).
-            std::unique_ptr<cmi::hip::DeviceTape> tape;
-            try {
-              tape = std::make_unique<cmi::hip::DeviceTape>(a.ctx->ctx, a.problems[i]->problem);
-            } catch (const cmi::ResourceExhausted&) {
-              tape.reset();
-            } catch (const std::bad_alloc&) {
-              tape.reset();
+            fresh.clear();
+            fresh.resize(admit.size());
+            auto build_one = [&](std::size_t k) {
+              std::size_t const i = admit[k];
+              // A tape that cannot be built stops its own solve (DeviceMemory at
+              // its first iteration, as a plain solve's first role build would).
+              std::unique_ptr<cmi::hip::DeviceTape> tape;
+              try {
+                tape = std::make_unique<cmi::hip::DeviceTape>(a.ctx->ctx, a.problems[i]->problem);
+              } catch (const cmi::ResourceExhausted&) {
+                tape.reset();
+              } catch (const std::bad_alloc&) {
+                tape.reset();
+              }
+              fresh[k] = Active {i, std::move(tape),
+                                 std::make_unique<cmi::search::SolveState>(a.problems[i]->problem, pol, sopt),
+                                 price[i]};
+            };
+            if (admit.size() == 1) {
+              build_one(0);
+            } else if (!admit.empty()) {
+              runner.workers().run(admit.size(), build_one);
             }
-            used_bytes += priced_bytes;
-            active.push_back(Active {i, std::move(tape),
-                                     std::make_unique<cmi::search::SolveState>(a.problems[i]->problem, pol, sopt),
-                                     priced_bytes});
-            ++next;
+            for (Active& f : fresh) {
+              active.push_back(std::move(f));
+            }
+            next += admit.size();
           }
           peak = std::max(peak, active.size());
           peak_bytes = std::max(peak_bytes, used_bytes);
