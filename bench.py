@@ -732,8 +732,8 @@ def main(argv=None) -> int:
         progress("serving")
         line["serving"] = serve_comparison(ctx)
         # the survey's size range (n up to 96), on a shorter stream and cap
-        wide = serve_comparison(ctx, n_req=32, rates=(200.0,), seed=8, n_max=96, iter_limit=100)
-        line["serving"]["wide"] = {k: v for k, v in wide.items() if k != "rates"}
+        wide = serve_comparison(ctx, n_req=64, rates=(200.0, 2000.0), seed=8, n_max=96, iter_limit=100)
+        line["serving"]["wide"] = wide
         if not a.no_full_stream:
             progress("serving the 20,000-request stream")
             line["serving"]["full_stream"] = full_stream(ctx, a.full_stream_cap)
