@@ -91,6 +91,10 @@ public:
   double kernel_seconds() const;
   double fp64_ops() const { return fp64_ops_; }
   static constexpr std::uint64_t kTimedEvery = 8;
+  /// The device's free memory, read on this runner's first call and every
+  /// kFreeBytesEvery-th call after it.
+  std::size_t free_bytes(hip::Context& ctx);
+  static constexpr std::uint64_t kFreeBytesEvery = 32;
 
   Runner(const Runner&) = delete;
   Runner& operator=(const Runner&) = delete;
@@ -126,6 +130,8 @@ private:
   std::size_t off_obj_lb_ = 0, off_cand_ = 0, off_witness_ = 0, results_bytes_ = 0;
   std::uint64_t launches_ = 0;
   std::uint64_t runs_ = 0;
+  std::uint64_t free_steps_ = 0;
+  std::size_t free_bytes_ = 0;
   double fp64_ops_ = 0.0;
   double timed_seconds_ = 0.0, timed_ops_ = 0.0;  // the timed runs
   hipEvent_t ev_[2] = {nullptr, nullptr};  // around a run's launches

@@ -22,6 +22,17 @@ def c1(seed: int) -> EngineProblem:
     return EngineProblem.qcqp(seed, **C1_SPEC)
 
 
+# 64 variables, the first 16 integer in [0, 10], the rest continuous in
+# [-5, 5]; 32 constraints; 10 % bilinear pairs plus squares, coefficients
+# N(0, 1); about 1,500 DAG nodes.
+C3_SPEC = dict(n=64, m=32, n_int=16, int_lb=0.0, int_ub=10.0, cont_lb=-5.0, cont_ub=5.0, pair_density=0.1,
+               target_nodes=1500)
+
+
+def c3(seed: int) -> EngineProblem:
+    return EngineProblem.qcqp(seed, **C3_SPEC)
+
+
 @dataclass
 class Batch:
     problems: list

@@ -746,7 +746,8 @@ cmi_status cmi_solve(cmi_ctx* ctx, const cmi_problem* p, const cmi_solve_options
  * CMI_STOP_DEVICE_MEMORY and its facts; the others go on). Host budget: a
  * non-zero host_budget_bytes is each solve's own, as in cmi_solve; 0 reads
  * half of MemAvailable once and gives each of the n solves an equal share.
- * With budget_bytes 0 the free device memory is read once per step. */
+ * With budget_bytes 0 the free device memory is read on the first step and
+ * every 32nd step after it. */
 cmi_status cmi_solve_batch(cmi_ctx* ctx, size_t n, const cmi_problem* const* problems, const cmi_solve_options* o,
                            cmi_solve_outcome* outs, double* const* best_points);
 
