@@ -81,17 +81,27 @@ constexpr std::size_t kParallelMembers = 256;
 /// item's groups in one workgroup; 1, 2, 4, 8 or 16), or -1 when unset.
 int item_waves_override()
 {
-  static int const v = [] {
-    if (const char* e = std::getenv("CMI_ITEM_WAVES")) {
-      char* end = nullptr;
-      long const w = std::strtol(e, &end, 10);
-      if (end != e && *end == '\0' && (w == 0 || w == 1 || w == 2 || w == 4 || w == 8 || w == 16)) {
-        return static_cast<int>(w);
-      }
+  if (const char* e = std::getenv("CMI_ITEM_WAVES")) {
+    char* end = nullptr;
+    long const w = std::strtol(e, &end, 10);
+    if (end != e && *end == '\0' && (w == 0 || w == 1 || w == 2 || w == 4 || w == 8 || w == 16)) {
+      return static_cast<int>(w);
     }
-    return -1;
-  }();
-  return v;
+  }
+  return -1;
+}
+
+/// CMI_STEP_GRAPH=1 runs each step's copies and launches as one HIP graph
+/// (built per shape of step, its nodes' parameters set every step) instead
+/// of one by one. Measured on the bench's streams and batch, the graph was
+/// no faster: the 2,000 req/s stream of small QCQPs 395-415 against 423-441
+/// served solves/s, the 50 req/s stream's p50 94-99 against 60 ms (a graph
+/// launch with parameter updates costs more than a 4-operation step), the
+/// batch equal, the stream of up to 96 variables 3 % faster.
+bool step_graph()
+{
+  const char* e = std::getenv("CMI_STEP_GRAPH");
+  return e != nullptr && std::strcmp(e, "1") == 0;
 }
 
 std::size_t host_threads()
@@ -209,9 +219,21 @@ Runner::Runner(std::shared_ptr<hip::Context> ctx)
   }
 }
 
+void Runner::StepGraph::reset()
+{
+  if (exec != nullptr) {
+    (void)hipGraphExecDestroy(exec);
+  }
+  if (graph != nullptr) {
+    (void)hipGraphDestroy(graph);
+  }
+  *this = StepGraph {};
+}
+
 Runner::~Runner()
 {
   ctx_->activate();
+  graph_.reset();
   for (hipEvent_t e : ev_) {
     (void)hipEventDestroy(e);
   }
@@ -487,30 +509,109 @@ void Runner::run(std::span<const Request> reqs)
     tile_cursor += blocks;
   }
 
-  check(hipMemcpyAsync(d, h, staging_bytes, hipMemcpyHostToDevice, s), "hipMemcpyAsync(batch staging)");
   // the launches of every kTimedEvery-th run are bracketed by events
   bool const timed = runs_++ % kTimedEvery == 0;
-  if (timed) {
-    check(hipEventRecord(ev_[0], s), "hipEventRecord");
-  }
-  std::size_t first_item = 0, first_tile = 0;
-  for (int pass = 0; pass < 3; ++pass) {
-    dev::Mode const m = pass == 0 ? dev::Mode::Sample : pass == 1 ? dev::Mode::Enumerate : dev::Mode::Bound;
-    if (counts[pass] > 0) {
-      dev::launch_items(m, ext[pass], reinterpret_cast<const dev::Item*>(d + off_items) + first_item,
-                        reinterpret_cast<const std::uint32_t*>(d + off_tile_item) + first_tile,
-                        static_cast<std::uint32_t>(tiles[pass]), lds[pass], waves[pass], s);
-      check(hipGetLastError(), "launch(k_items batch)");
-      ++launches_;
+  dev::ItemsLaunch launch[3];
+  void* args[3][3];
+  const dev::Item* items_d[3];
+  const std::uint32_t* tile_item_d[3];
+  std::uint32_t slots_arg[3];
+  {
+    std::size_t first_item = 0, first_tile = 0;
+    for (int pass = 0; pass < 3; ++pass) {
+      dev::Mode const m = pass == 0 ? dev::Mode::Sample : pass == 1 ? dev::Mode::Enumerate : dev::Mode::Bound;
+      items_d[pass] = reinterpret_cast<const dev::Item*>(d + off_items) + first_item;
+      tile_item_d[pass] = reinterpret_cast<const std::uint32_t*>(d + off_tile_item) + first_tile;
+      slots_arg[pass] = lds[pass];
+      args[pass][0] = &items_d[pass];
+      args[pass][1] = &tile_item_d[pass];
+      args[pass][2] = &slots_arg[pass];
+      if (counts[pass] > 0) {
+        launch[pass] = dev::items_launch(m, ext[pass], static_cast<std::uint32_t>(tiles[pass]), lds[pass],
+                                         waves[pass]);
+        ++launches_;
+      }
+      first_item += counts[pass];
+      first_tile += tiles[pass];
     }
-    first_item += counts[pass];
-    first_tile += tiles[pass];
   }
-  if (timed) {
-    check(hipEventRecord(ev_[1], s), "hipEventRecord");
+  if (step_graph()) {
+    // one graph per shape of step (which kinds launch, and which of them run
+    // the extended op set): copy in, the launches between two event records,
+    // copy out; every step sets its nodes' parameters and launches it
+    std::uint32_t key = 0;
+    for (int pass = 0; pass < 3; ++pass) {
+      key |= (counts[pass] > 0 ? 1u : 0u) << pass;
+      key |= (counts[pass] > 0 && ext[pass] ? 1u : 0u) << (pass + 3);
+    }
+    if (!graph_.exec || graph_.key != key) {
+      graph_.reset();
+      check(hipGraphCreate(&graph_.graph, 0), "hipGraphCreate");
+      check(hipGraphAddMemcpyNode1D(&graph_.h2d, graph_.graph, nullptr, 0, d, h, staging_bytes,
+                                    hipMemcpyHostToDevice),
+            "hipGraphAddMemcpyNode1D");
+      check(hipGraphAddEventRecordNode(&graph_.ev0, graph_.graph, &graph_.h2d, 1, ev_[0]),
+            "hipGraphAddEventRecordNode");
+      hipGraphNode_t prev = graph_.ev0;
+      for (int pass = 0; pass < 3; ++pass) {
+        if (counts[pass] == 0) {
+          continue;
+        }
+        hipKernelNodeParams kp {};
+        kp.func = const_cast<void*>(launch[pass].func);
+        kp.gridDim = dim3(launch[pass].grid);
+        kp.blockDim = dim3(launch[pass].block);
+        kp.sharedMemBytes = static_cast<unsigned>(launch[pass].lds_bytes);
+        kp.kernelParams = args[pass];
+        check(hipGraphAddKernelNode(&graph_.kernel[pass], graph_.graph, &prev, 1, &kp), "hipGraphAddKernelNode");
+        prev = graph_.kernel[pass];
+      }
+      check(hipGraphAddEventRecordNode(&graph_.ev1, graph_.graph, &prev, 1, ev_[1]),
+            "hipGraphAddEventRecordNode");
+      check(hipGraphAddMemcpyNode1D(&graph_.d2h, graph_.graph, &graph_.ev1, 1, results_h_.data(), results_d_.get(),
+                                    results_bytes_, hipMemcpyDeviceToHost),
+            "hipGraphAddMemcpyNode1D");
+      check(hipGraphInstantiate(&graph_.exec, graph_.graph, nullptr, nullptr, 0), "hipGraphInstantiate");
+      graph_.key = key;
+    } else {
+      check(hipGraphExecMemcpyNodeSetParams1D(graph_.exec, graph_.h2d, d, h, staging_bytes, hipMemcpyHostToDevice),
+            "hipGraphExecMemcpyNodeSetParams1D");
+      for (int pass = 0; pass < 3; ++pass) {
+        if (counts[pass] == 0) {
+          continue;
+        }
+        hipKernelNodeParams kp {};
+        kp.func = const_cast<void*>(launch[pass].func);
+        kp.gridDim = dim3(launch[pass].grid);
+        kp.blockDim = dim3(launch[pass].block);
+        kp.sharedMemBytes = static_cast<unsigned>(launch[pass].lds_bytes);
+        kp.kernelParams = args[pass];
+        check(hipGraphExecKernelNodeSetParams(graph_.exec, graph_.kernel[pass], &kp),
+              "hipGraphExecKernelNodeSetParams");
+      }
+      check(hipGraphExecMemcpyNodeSetParams1D(graph_.exec, graph_.d2h, results_h_.data(), results_d_.get(),
+                                              results_bytes_, hipMemcpyDeviceToHost),
+            "hipGraphExecMemcpyNodeSetParams1D");
+    }
+    check(hipGraphLaunch(graph_.exec, s), "hipGraphLaunch");
+  } else {
+    check(hipMemcpyAsync(d, h, staging_bytes, hipMemcpyHostToDevice, s), "hipMemcpyAsync(batch staging)");
+    if (timed) {
+      check(hipEventRecord(ev_[0], s), "hipEventRecord");
+    }
+    for (int pass = 0; pass < 3; ++pass) {
+      if (counts[pass] > 0) {
+        check(hipLaunchKernel(launch[pass].func, dim3(launch[pass].grid), dim3(launch[pass].block), args[pass],
+                              launch[pass].lds_bytes, s),
+              "launch(k_items batch)");
+      }
+    }
+    if (timed) {
+      check(hipEventRecord(ev_[1], s), "hipEventRecord");
+    }
+    check(hipMemcpyAsync(results_h_.data(), results_d_.get(), results_bytes_, hipMemcpyDeviceToHost, s),
+          "hipMemcpyAsync(batch results)");
   }
-  check(hipMemcpyAsync(results_h_.data(), results_d_.get(), results_bytes_, hipMemcpyDeviceToHost, s),
-        "hipMemcpyAsync(batch results)");
   check(hipStreamSynchronize(s), "hipStreamSynchronize");
   if (timed) {
     float ms = 0.0f;

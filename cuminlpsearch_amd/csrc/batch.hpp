@@ -135,6 +135,18 @@ private:
   double fp64_ops_ = 0.0;
   double timed_seconds_ = 0.0, timed_ops_ = 0.0;  // the timed runs
   hipEvent_t ev_[2] = {nullptr, nullptr};  // around a run's launches
+  /// A step as one HIP graph (copy in, the launches, copy out), rebuilt when
+  /// the step's shape changes and otherwise given each step's parameters.
+  struct StepGraph
+  {
+    std::uint32_t key = 0;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    hipGraphNode_t h2d = nullptr, d2h = nullptr, ev0 = nullptr, ev1 = nullptr;
+    hipGraphNode_t kernel[3] = {nullptr, nullptr, nullptr};
+    void reset();
+  };
+  StepGraph graph_;
   std::unique_ptr<Workers> workers_;
 };
 

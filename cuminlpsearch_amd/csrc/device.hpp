@@ -95,6 +95,19 @@ enum class Mode : int
 void launch_items(Mode mode, bool extended, const Item* items, const std::uint32_t* tile_item, std::uint32_t n_tiles,
                   std::uint32_t lds_slots, std::uint32_t waves, hipStream_t stream);
 
+/// What launch_items launches, for a graph's kernel node: the kernel (its
+/// arguments are items, tile_item, lds_slots), grid, workgroup size and
+/// dynamic LDS bytes.
+struct ItemsLaunch
+{
+  const void* func = nullptr;
+  unsigned grid = 0;
+  unsigned block = 0;
+  std::size_t lds_bytes = 0;
+};
+ItemsLaunch items_launch(Mode mode, bool extended, std::uint32_t n_tiles, std::uint32_t lds_slots,
+                         std::uint32_t waves);
+
 /// LDS bytes of one decode-mode workgroup of `waves` waves over `lds_slots` slots.
 std::size_t items_lds_bytes(Mode mode, std::uint32_t lds_slots, std::uint32_t waves);
 

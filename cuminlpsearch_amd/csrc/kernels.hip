@@ -995,6 +995,26 @@ void launch_items_as(const Item* items, const std::uint32_t* tile_item, std::uin
                      lds_slots);
 }
 
+template<int M, bool Ext>
+const void* items_kernel_fn()
+{
+  static bool once = (allow_large_lds(k_items<M, Ext>), true);
+  (void)once;
+  return reinterpret_cast<const void*>(&k_items<M, Ext>);
+}
+
+template<bool Ext>
+const void* items_kernel_mode(Mode mode)
+{
+  switch (mode) {
+    case Mode::Bound: return items_kernel_fn<0, Ext>();
+    case Mode::Sample: return items_kernel_fn<1, Ext>();
+    case Mode::Enumerate: return items_kernel_fn<2, Ext>();
+    case Mode::Aggregate: return items_kernel_fn<3, Ext>();
+  }
+  return nullptr;
+}
+
 template<bool Ext>
 void launch_items_mode(Mode mode, const Item* items, const std::uint32_t* tile_item, std::uint32_t n_tiles,
                        std::size_t bytes, std::uint32_t lds_slots, std::uint32_t waves, hipStream_t stream)
@@ -1059,6 +1079,18 @@ void launch_items(Mode mode, bool extended, const Item* items, const std::uint32
   } else {
     launch_items_mode<false>(mode, items, tile_item, n_tiles, bytes, lds_slots, waves, stream);
   }
+}
+
+ItemsLaunch items_launch(Mode mode, bool extended, std::uint32_t n_tiles, std::uint32_t lds_slots,
+                         std::uint32_t waves)
+{
+  waves = std::clamp<std::uint32_t>(waves, 1, kMaxGroups);
+  ItemsLaunch l;
+  l.func = extended ? items_kernel_mode<true>(mode) : items_kernel_mode<false>(mode);
+  l.grid = n_tiles;
+  l.block = kTile * waves;
+  l.lds_bytes = items_lds_bytes(mode, lds_slots, waves);
+  return l;
 }
 
 void launch_boxes(const Instr* code, bool extended, const GroupTable& groups, std::uint32_t n_slots,

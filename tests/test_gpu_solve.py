@@ -92,6 +92,26 @@ def test_mixed_size_batch_equals_plain(ctx):
         same_outcome(d, s)
 
 
+def test_batch_stats_account_for_the_launches(ctx):
+    """The instrumented batch gives the plain batch's results, one step per
+    iteration of its longest solve, two launches (sampler and bounder) per
+    step with work, and positive device time and FP64 work; the served
+    stream's stats carry the same accounting."""
+    probs = [P.c1(s) for s in range(8)]
+    outs, _ = cmi.solve_batch(ctx, probs, **C1_OPTS)
+    outs_s, _, st = cmi.solve_batch_stats(ctx, probs, **C1_OPTS)
+    for a, b in zip(outs, outs_s):
+        same_outcome(a, b)
+    longest = max(int(o.iterations) for o in outs)
+    assert longest <= st.steps <= longest + 2
+    assert 0 < st.launches <= 2 * st.steps
+    assert st.kernel_seconds > 0 and st.kernel_seconds < st.wall_seconds
+    assert st.kernel_fp64_ops > 0
+    arrival = np.zeros(len(probs))
+    _, _, _, ss = cmi.serve_stream(ctx, probs, arrival, max_active=8, **C1_OPTS)
+    assert ss.kernel_seconds > 0 and ss.kernel_fp64_ops == st.kernel_fp64_ops
+
+
 def test_served_equals_plain(ctx):
     probs = [P.c1(s) for s in range(10)]
     arrival = np.linspace(0.0, 0.02, len(probs))
@@ -190,3 +210,22 @@ def test_batched_and_served_solves_stop_over_budget_like_plain(ctx):
             assert b.as_dict()["over_budget"] == want and s.as_dict()["over_budget"] == want
             assert want["role"] == "point sampler" and want["elements_per_region"] == 64
             assert want["composition"] == "CCC" and want["solve_samples_per_region"] == 64
+
+
+@pytest.mark.parametrize("env", [{"CMI_STEP_GRAPH": "1"}, {"CMI_ITEM_WAVES": "1"}, {"CMI_ITEM_WAVES": "4"},
+                                 {"CMI_ITEM_WAVES": "16"}, {"CMI_STEP_GRAPH": "1", "CMI_ITEM_WAVES": "2"}])
+def test_step_forms_equal_plain(ctx, monkeypatch, env):
+    """A batched step run as one HIP graph, or with its launches' function
+    groups sliced over workgroups of 1, 2, 4 or 16 waves, gives the plain
+    solves' results (long tapes included: n = 40 and 64 have a wide grouping)."""
+    for k, v in env.items():
+        monkeypatch.setenv(k, v)
+    probs = [cmi.EngineProblem.qcqp(90 + i, n=n, m=n // 2, cont_lb=-5.0, cont_ub=5.0, pair_density=0.2)
+             for i, n in enumerate([4, 40, 9, 64, 13])]
+    opts = dict(partition_num=2, enumerate_cap=2, max_cycle_size=4, sample_points=16, iter_limit=12)
+    outs, _ = cmi.solve_batch(ctx, probs, **opts)
+    for k in env:
+        monkeypatch.delenv(k)
+    for p, d in zip(probs, outs):
+        s, _ = cmi.solve(ctx, p, **opts)
+        same_outcome(d, s)
