@@ -77,18 +77,66 @@ constexpr std::size_t kParallelMembers = 256;
 
 /// Host threads of a runner's workers: CMI_BATCH_HOST_THREADS when it holds
 /// a count in [1, 64], else min(16, hardware threads).
-/// CMI_ITEM_WAVES: waves per workgroup of every batched launch (0: each
-/// item's groups in one workgroup; 1, 2, 4, 8 or 16), or -1 when unset.
-int item_waves_override()
+/// CMI_ITEM_WAVES: waves per workgroup of the batched launches (0: each
+/// item's groups in one workgroup; 1, 2, 4, 8 or 16), one value for every
+/// launch or three separated by commas (sampler, enumerator, bounder); -1
+/// where unset.
+void item_waves_override(int (&out)[3])
 {
-  if (const char* e = std::getenv("CMI_ITEM_WAVES")) {
+  out[0] = out[1] = out[2] = -1;
+  const char* e = std::getenv("CMI_ITEM_WAVES");
+  if (e == nullptr) {
+    return;
+  }
+  int vals[3];
+  int n = 0;
+  for (const char* c = e; n < 3;) {
     char* end = nullptr;
-    long const w = std::strtol(e, &end, 10);
-    if (end != e && *end == '\0' && (w == 0 || w == 1 || w == 2 || w == 4 || w == 8 || w == 16)) {
-      return static_cast<int>(w);
+    long const w = std::strtol(c, &end, 10);
+    if (end == c || !(w == 0 || w == 1 || w == 2 || w == 4 || w == 8 || w == 16)) {
+      return;
+    }
+    vals[n++] = static_cast<int>(w);
+    if (*end == '\0') {
+      break;
+    }
+    if (*end != ',') {
+      return;
+    }
+    c = end + 1;
+  }
+  if (n == 1) {
+    out[0] = out[1] = out[2] = vals[0];
+  } else if (n == 3) {
+    out[0] = vals[0];
+    out[1] = vals[1];
+    out[2] = vals[2];
+  }
+}
+
+/// CMI_POINTS_PER_LANE: points each lane of a sampler or enumerator launch
+/// evaluates (1, 2 or 4; else the default of points_per_lane), or 0 when
+/// unset.
+std::uint32_t points_per_lane_override()
+{
+  if (const char* e = std::getenv("CMI_POINTS_PER_LANE")) {
+    if (std::strcmp(e, "1") == 0 || std::strcmp(e, "2") == 0 || std::strcmp(e, "4") == 0) {
+      return static_cast<std::uint32_t>(e[0] - '0');
     }
   }
-  return -1;
+  return 0;
+}
+
+/// Points per lane of a sampler or enumerator launch of `workgroups`
+/// workgroups (at one point per lane) on n_cus compute units: 2 when they
+/// number at least 4 per unit, so that 2 per unit remain, else 1. Measured
+/// on 512 batched QCQPs of up to 48 variables (sampler ~2,048 workgroups:
+/// 2 points 14 % less kernel time per step than 1, 4 points 6 %) and on
+/// streams whose sampler launches have at most one workgroup per unit (2
+/// points: up to 8 % more kernel time).
+std::uint32_t points_per_lane(std::size_t workgroups, std::uint32_t n_cus)
+{
+  return workgroups >= std::size_t {4} * n_cus ? 2u : 1u;
 }
 
 /// CMI_STEP_GRAPH=1 runs each step's copies and launches as one HIP graph
@@ -304,8 +352,14 @@ void Runner::run(std::span<const Request> reqs)
           hip::DeviceTape::slice_waves(pass_groups[pass], pass_tiles[pass], ctx_->n_cus(), pass_words[pass]);
     }
   }
-  if (int const forced = item_waves_override(); forced >= 0) {
-    pass_waves[0] = pass_waves[1] = pass_waves[2] = static_cast<std::uint32_t>(forced);
+  {
+    int forced[3];
+    item_waves_override(forced);
+    for (int pass = 0; pass < 3; ++pass) {
+      if (forced[pass] >= 0) {
+        pass_waves[pass] = static_cast<std::uint32_t>(forced[pass]);
+      }
+    }
   }
 
   // Sizes.
@@ -381,6 +435,30 @@ void Runner::run(std::span<const Request> reqs)
       sliced_tiles += (slots_[order[k]].n_elems + dev::kTile - 1) / dev::kTile;
     }
   }
+  // Points per lane of the sampler and enumerator launches (points_per_lane):
+  // a launch whose workgroup still fits in LDS evaluates that many points per
+  // lane, its items' tiles then covering kTile * ppl elements each.
+  std::uint32_t ppl[3] = {1, 1, 1};
+  {
+    std::uint32_t const forced = points_per_lane_override();
+    for (int pass = 0; pass < 2; ++pass) {
+      dev::Mode const m = pass == 0 ? dev::Mode::Sample : dev::Mode::Enumerate;
+      // four points of the extended op set would spill registers
+      std::uint32_t const want
+          = std::min<std::uint32_t>(forced != 0 ? forced : points_per_lane(tiles[pass], ctx_->n_cus()), ext[pass] ? 2 : 4);
+      if (want <= 1 || counts[pass] == 0 || dev::items_lds_bytes(m, lds[pass], waves[pass], want) > ctx_->lds_bytes()) {
+        continue;
+      }
+      ppl[pass] = want;
+      tiles[pass] = 0;
+      for (std::size_t k = 0; k < n; ++k) {
+        int const kp = k < counts[0] ? 0 : k < counts[0] + counts[1] ? 1 : 2;
+        if (kp == pass) {
+          tiles[pass] += (slots_[order[k]].n_elems + dev::kTile * want - 1) / (dev::kTile * want) * plans_[k].slices;
+        }
+      }
+    }
+  }
   if (lds[2] > ctx_->lds_interval_slots() || std::max(lds[0], lds[1]) > ctx_->lds_point_slots()) {
     throw ResourceExhausted("batched request needs more LDS slots than a workgroup holds");
   }
@@ -415,9 +493,12 @@ void Runner::run(std::span<const Request> reqs)
     check(hipMemsetAsync(slice_d_.get(), 0, slice_d_.bytes(), s), "hipMemsetAsync(slice scratch)");
     slice_capacity_ = cap;
   }
+  // slice scratch: every tile's masks, then counts, then objectives
+  std::size_t const slice_masks_bytes = slice_capacity_ * dev::kSliceMasks * 8;
   auto* const slice_inf = slice_d_.as<std::uint64_t>();
-  auto* const slice_done = reinterpret_cast<std::uint32_t*>(slice_d_.as<unsigned char>() + slice_capacity_ * 8);
-  auto* const slice_obj = reinterpret_cast<double*>(slice_d_.as<unsigned char>() + align_up(slice_capacity_ * 12, 16));
+  auto* const slice_done = reinterpret_cast<std::uint32_t*>(slice_d_.as<unsigned char>() + slice_masks_bytes);
+  auto* const slice_obj = reinterpret_cast<double*>(slice_d_.as<unsigned char>()
+                                                    + align_up(slice_masks_bytes + slice_capacity_ * 4, 16));
   std::size_t slice_cursor = 0;
 
   unsigned char* const h = staging_h_.data();
@@ -466,15 +547,16 @@ void Runner::run(std::span<const Request> reqs)
       kind[j] = static_cast<std::uint8_t>(r.assignment->composition[j]);
     }
 
-    std::uint32_t const n_tiles = static_cast<std::uint32_t>((p.n_elems + dev::kTile - 1) / dev::kTile);
+    std::size_t const tile_elems = std::size_t {dev::kTile} * ppl[pass];
+    std::uint32_t const n_tiles = static_cast<std::uint32_t>((p.n_elems + tile_elems - 1) / tile_elems);
     hip::DeviceTape::ItemPlan const& plan = plans_[k];
     dev::Item& it = items[k];
     std::memset(&it, 0, sizeof(it));
     r.tape->item_code(it, plan);
     if (plan.slices > 1) {
-      it.tile_infeasible = slice_inf + slice_cursor;
+      it.tile_infeasible = slice_inf + slice_cursor * dev::kSliceMasks;
       it.tile_slices_done = slice_done + slice_cursor;
-      it.tile_obj = slice_obj + slice_cursor * dev::kTile * 2;
+      it.tile_obj = slice_obj + slice_cursor * dev::kSliceObjDoubles;
       slice_cursor += n_tiles;
     }
     it.n_vars = static_cast<std::uint32_t>(p.n_vars);
@@ -528,7 +610,7 @@ void Runner::run(std::span<const Request> reqs)
       args[pass][2] = &slots_arg[pass];
       if (counts[pass] > 0) {
         launch[pass] = dev::items_launch(m, ext[pass], static_cast<std::uint32_t>(tiles[pass]), lds[pass],
-                                         waves[pass]);
+                                         waves[pass], ppl[pass]);
         ++launches_;
       }
       first_item += counts[pass];
@@ -536,13 +618,14 @@ void Runner::run(std::span<const Request> reqs)
     }
   }
   if (step_graph()) {
-    // one graph per shape of step (which kinds launch, and which of them run
-    // the extended op set): copy in, the launches between two event records,
+    // one graph per shape of step (which kinds launch, which of them run the
+    // extended op set, and their points per lane): copy in, the launches between two event records,
     // copy out; every step sets its nodes' parameters and launches it
     std::uint32_t key = 0;
     for (int pass = 0; pass < 3; ++pass) {
       key |= (counts[pass] > 0 ? 1u : 0u) << pass;
       key |= (counts[pass] > 0 && ext[pass] ? 1u : 0u) << (pass + 3);
+      key |= (counts[pass] > 0 ? ppl[pass] : 0u) << (8 + 4 * pass);
     }
     if (!graph_.exec || graph_.key != key) {
       graph_.reset();

@@ -71,13 +71,20 @@ struct Item
   const std::uint32_t* slice_begin;  // slice_groups + 1 word offsets into code
   std::uint32_t slice_objective;     // the group that reads out the objective
   std::uint32_t slice_pad;
-  std::uint64_t* tile_infeasible;    // per tile: lanes some slice found infeasible; zero between launches
+  std::uint64_t* tile_infeasible;    // kSliceMasks per tile: lanes some slice found infeasible, per point of a
+                                     // lane; zero between launches
   std::uint32_t* tile_slices_done;   // per tile: slices finished; zero between launches
-  double* tile_obj;                  // per tile: the objective of each lane (intervals: (lb, ub) pairs)
+  double* tile_obj;                  // kSliceObjDoubles per tile: the objective of each lane and point
+                                     // (intervals: (lb, ub) pairs)
 };
 
+/// Most points per lane of a sampler or enumerator launch.
+inline constexpr unsigned kMaxPointsPerLane = 4;
+/// A sliced tile's masks (one per point of a lane) and objective doubles.
+inline constexpr std::size_t kSliceMasks = kMaxPointsPerLane;
+inline constexpr std::size_t kSliceObjDoubles = std::size_t {kTile} * kMaxPointsPerLane;
 /// Device scratch of one sliced tile (tile_infeasible, tile_slices_done, tile_obj).
-inline constexpr std::size_t kSliceTileBytes = 8 + 4 + kTile * 16;
+inline constexpr std::size_t kSliceTileBytes = kSliceMasks * 8 + 4 + kSliceObjDoubles * 8;
 
 enum class Mode : int
 {
@@ -97,7 +104,9 @@ void launch_items(Mode mode, bool extended, const Item* items, const std::uint32
 
 /// What launch_items launches, for a graph's kernel node: the kernel (its
 /// arguments are items, tile_item, lds_slots), grid, workgroup size and
-/// dynamic LDS bytes.
+/// dynamic LDS bytes. `points_per_lane` (sampler and enumerator: 1, 2 or 4;
+/// others 1): each lane evaluates that many points, an item's tiles are then
+/// kTile * points_per_lane elements each, and no item may be sliced.
 struct ItemsLaunch
 {
   const void* func = nullptr;
@@ -106,10 +115,12 @@ struct ItemsLaunch
   std::size_t lds_bytes = 0;
 };
 ItemsLaunch items_launch(Mode mode, bool extended, std::uint32_t n_tiles, std::uint32_t lds_slots,
-                         std::uint32_t waves);
+                         std::uint32_t waves, std::uint32_t points_per_lane = 1);
 
-/// LDS bytes of one decode-mode workgroup of `waves` waves over `lds_slots` slots.
-std::size_t items_lds_bytes(Mode mode, std::uint32_t lds_slots, std::uint32_t waves);
+/// LDS bytes of one decode-mode workgroup of `waves` waves over `lds_slots`
+/// slots, with `points_per_lane` points per lane (point modes).
+std::size_t items_lds_bytes(Mode mode, std::uint32_t lds_slots, std::uint32_t waves,
+                            std::uint32_t points_per_lane = 1);
 
 /// Explicit-box evaluation: box e's variable v is [lb[v*ld+e], ub[v*ld+e]].
 /// Writes feasible[e] and obj_lb[e]; obj_ub and roots (n_outputs intervals,

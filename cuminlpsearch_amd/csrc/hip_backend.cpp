@@ -256,7 +256,7 @@ std::uint32_t DeviceTape::slice_waves(std::uint32_t groups, std::size_t tiles, s
   if (groups <= kWidePeriod || words < kSliceMinWords) {
     return 0;
   }
-  if (tiles * 2 <= n_cus) {
+  if (tiles <= n_cus) {
     return kWidePeriod;
   }
   if (tiles <= std::size_t {2} * n_cus && groups > 8) {
@@ -892,9 +892,10 @@ void RoleImpl::run(std::span<const interval> box, std::span<const std::size_t> v
   tape_->item_code(*item, plan);
   if (plan.slices > 1) {
     auto* const sc = slice_d_.as<unsigned char>();
+    std::size_t const masks_bytes = std::size_t {n_tiles_} * dev::kSliceMasks * 8;
     item->tile_infeasible = reinterpret_cast<std::uint64_t*>(sc);
-    item->tile_slices_done = reinterpret_cast<std::uint32_t*>(sc + std::size_t {n_tiles_} * 8);
-    item->tile_obj = reinterpret_cast<double*>(sc + align_up(std::size_t {n_tiles_} * 12, 16));
+    item->tile_slices_done = reinterpret_cast<std::uint32_t*>(sc + masks_bytes);
+    item->tile_obj = reinterpret_cast<double*>(sc + align_up(masks_bytes + std::size_t {n_tiles_} * 4, 16));
   }
   check(hipMemcpyAsync(staging_d_.get(), h, staging_bytes_, hipMemcpyHostToDevice, s), "hipMemcpyAsync(staging)");
   if (compiled) {

@@ -309,13 +309,17 @@ public:
   /// Waves per workgroup of a decode-mode launch of `tiles` tiles whose
   /// items have at most `groups` function groups and `words` tape words, on
   /// n_cus compute units: 0 (every item's groups in one workgroup) unless
-  /// some tape has kSliceMinWords words; then one wave per SIMD (4) when the
-  /// tiles cover at most half the compute units, so that each tile's groups
-  /// spread over several of them, 8 when they cover at most two workgroups
-  /// per unit, else 0. Measured on 32 served QCQPs of up to 93 variables
-  /// (4: 1.7x faster than 0), 512 batched ones of up to 48 (8: 4 % faster
-  /// than 0; 4: 15 % slower) and streams of up to 16 variables (tapes of at
-  /// most 531 words: slicing cost 10 % of the served rate).
+  /// some tape has kSliceMinWords words; then one wave per SIMD (4, the
+  /// width the wide grouping's temporaries repeat with) when there are at
+  /// most as many tiles as compute units, so that each tile's groups spread
+  /// over several of them, 8 when the tiles cover at most two workgroups per
+  /// unit, else 0. Measured per launch kind on a stream of 64 QCQPs of up to
+  /// 96 variables at 2,000 req/s (sampler ~256 tiles and bounder ~64 tiles
+  /// both at 4: 0.59 ms of kernel time per step; sampler at 8: 1.05 ms; both
+  /// at 0: 2.03 ms) and 512 batched ones of up to 48 (bounder, 512 tiles: 8
+  /// 0.68 ms, 0 0.71 ms, 4 0.78 ms per step), and on streams of up to 16
+  /// variables (tapes of at most 531 words: slicing cost 10 % of the served
+  /// rate).
   static std::uint32_t slice_waves(std::uint32_t groups, std::size_t tiles, std::uint32_t n_cus, std::size_t words);
   static constexpr std::size_t kSliceMinWords = 2048;
   /// Fill the Item's code fields from a plan (slice scratch is the caller's).

@@ -350,10 +350,10 @@ struct PointResult
 
 // One word of the extended op set (round to nearest).
 __device__ __forceinline__ double ext_point(std::uint32_t op, double a, const double* P, const Word& ins,
-                                            unsigned lane)
+                                            unsigned lane, std::uint32_t stride = kTile)
 {
   switch (static_cast<TOp>(op)) {
-    case TOp::Div: return a / P[ins.b * kTile + lane];
+    case TOp::Div: return a / P[ins.b * stride + lane];
     case TOp::DivImm: return a / ins.imm;
     case TOp::ImmDiv: return ins.imm / a;
     case TOp::PowN: return pt_pown(a, static_cast<int>(ins.imm));
@@ -364,9 +364,9 @@ __device__ __forceinline__ double ext_point(std::uint32_t op, double a, const do
     case TOp::Cos: return pt_sincos(a, 1);
     case TOp::Tanh: return pt_tanh(a);
     case TOp::Abs: return __builtin_fabs(a);
-    case TOp::Min: return __builtin_fmin(a, P[ins.b * kTile + lane]);
+    case TOp::Min: return __builtin_fmin(a, P[ins.b * stride + lane]);
     case TOp::MinImm: return __builtin_fmin(a, ins.imm);
-    case TOp::Max: return __builtin_fmax(a, P[ins.b * kTile + lane]);
+    case TOp::Max: return __builtin_fmax(a, P[ins.b * stride + lane]);
     case TOp::MaxImm: return __builtin_fmax(a, ins.imm);
     default: return a;
   }
@@ -579,6 +579,113 @@ __global__ __launch_bounds__(kTile) void k_points(const Instr* __restrict__ code
 }
 
 
+// The round-to-nearest tape over N points per lane: point j of lane l is in
+// LDS at P[(slot * N + j) * kTile + l]. The word stream and its dispatch are
+// shared by the N points, so a sampler tile of N x 64 points costs one
+// dispatch per word instead of N; each point's arithmetic is that of
+// run_point.
+template<bool Ext, int N>
+struct PointsResult
+{
+  double obj[N];
+  bool feasible[N];
+};
+
+template<bool Ext, int N>
+__device__ __forceinline__ PointsResult<Ext, N> run_points(const Instr* code, std::uint32_t n_instr, double* P,
+                                                           unsigned lane)
+{
+  PointsResult<Ext, N> res;
+  double acc[N];
+#pragma unroll
+  for (int j = 0; j < N; ++j) {
+    res.obj[j] = 0.0;
+    res.feasible[j] = true;
+    acc[j] = 0.0;
+  }
+  auto at = [&](std::uint32_t slot, int j) -> double& { return P[(slot * N + j) * kTile + lane]; };
+  // each dispatch branch below runs its operation on every point, so a word
+  // is decoded and branched on once
+#define CMI_EACH(stmt)            \
+  _Pragma("unroll") for (int j = 0; j < N; ++j) { stmt; }
+  const u32x4* const words = reinterpret_cast<const u32x4*>(code);
+  u32x4 cur = tape_chunk(words, lane, n_instr);
+  for (std::uint32_t c0 = 0; c0 < n_instr; c0 += kTile) {
+    u32x4 const nxt = tape_chunk(words, c0 + kTile + lane, n_instr);
+    std::uint32_t const end = std::min<std::uint32_t>(n_instr - c0, kTile);
+    for (std::uint32_t slot = 0; slot < end; ++slot) {
+      Word const ins = decode_word(lane_word(cur, slot));
+      std::uint32_t const op = ins.op;
+      if (op >= tape::kTermBase) {
+        std::uint32_t const term = op & 7u;
+        std::uint32_t const mode = (op - tape::kTermBase) >> 3;
+        double x[N];
+        if (term == static_cast<std::uint32_t>(tape::Term::SSC)) {
+          CMI_EACH(x[j] = (at(ins.a, j) * at(ins.b, j)) * ins.imm)
+        } else if (term == static_cast<std::uint32_t>(tape::Term::QC)) {
+          CMI_EACH(double const a = at(ins.a, j); x[j] = (a * a) * ins.imm)
+        } else if (term == static_cast<std::uint32_t>(tape::Term::SC)) {
+          CMI_EACH(x[j] = at(ins.a, j) * ins.imm)
+        } else if (term == static_cast<std::uint32_t>(tape::Term::SS)) {
+          CMI_EACH(x[j] = at(ins.a, j) * at(ins.b, j))
+        } else if (term == static_cast<std::uint32_t>(tape::Term::Q)) {
+          CMI_EACH(double const a = at(ins.a, j); x[j] = a * a)
+        } else {
+          CMI_EACH(x[j] = at(ins.a, j))
+        }
+        if (mode == static_cast<std::uint32_t>(tape::AccMode::Add)) {
+          CMI_EACH(acc[j] = acc[j] + x[j])
+        } else if (mode == static_cast<std::uint32_t>(tape::AccMode::Set)) {
+          CMI_EACH(acc[j] = x[j])
+        } else {
+          CMI_EACH(acc[j] = acc[j] - x[j])
+        }
+        continue;
+      }
+      if (op >= static_cast<std::uint32_t>(TOp::AccStore) && op < tape::kExtBase) {
+        switch (static_cast<TOp>(op)) {
+          case TOp::AccStore: CMI_EACH(at(ins.dst, j) = acc[j]) break;
+          case TOp::AccAddImm: CMI_EACH(acc[j] = acc[j] + ins.imm) break;
+          case TOp::AccSubImm: CMI_EACH(acc[j] = acc[j] - ins.imm) break;
+          case TOp::AccImmSub: CMI_EACH(acc[j] = ins.imm - acc[j]) break;
+          case TOp::AccScale: CMI_EACH(acc[j] = acc[j] * ins.imm) break;
+          case TOp::AccNeg: CMI_EACH(acc[j] = -acc[j]) break;
+          case TOp::AccCheckLE: CMI_EACH(res.feasible[j] = res.feasible[j] && (acc[j] <= ins.imm)) break;
+          case TOp::AccCheckEQ: CMI_EACH(res.feasible[j] = res.feasible[j] && almost_equal(acc[j], ins.imm)) break;
+          case TOp::AccObjective: CMI_EACH(res.obj[j] = acc[j]) break;
+          default: break;
+        }
+        continue;
+      }
+      if constexpr (Ext) {
+        if (op >= tape::kExtBase && op < tape::kExtEnd) {
+          CMI_EACH(at(ins.dst, j) = ext_point(op, at(ins.a, j), P + j * kTile, ins, lane, N * kTile))
+          continue;
+        }
+      }
+      switch (static_cast<TOp>(op)) {
+        case TOp::LoadConst: CMI_EACH(at(ins.dst, j) = ins.imm) break;
+        case TOp::Add: CMI_EACH(at(ins.dst, j) = at(ins.a, j) + at(ins.b, j)) break;
+        case TOp::AddImm: CMI_EACH(at(ins.dst, j) = at(ins.a, j) + ins.imm) break;
+        case TOp::Sub: CMI_EACH(at(ins.dst, j) = at(ins.a, j) - at(ins.b, j)) break;
+        case TOp::SubImm: CMI_EACH(at(ins.dst, j) = at(ins.a, j) - ins.imm) break;
+        case TOp::ImmSub: CMI_EACH(at(ins.dst, j) = ins.imm - at(ins.a, j)) break;
+        case TOp::Mul: CMI_EACH(at(ins.dst, j) = at(ins.a, j) * at(ins.b, j)) break;
+        case TOp::MulImm: CMI_EACH(at(ins.dst, j) = at(ins.a, j) * ins.imm) break;
+        case TOp::Sqr: CMI_EACH(double const a = at(ins.a, j); at(ins.dst, j) = a * a) break;
+        case TOp::Neg: CMI_EACH(at(ins.dst, j) = -at(ins.a, j)) break;
+        case TOp::CheckLE: CMI_EACH(res.feasible[j] = res.feasible[j] && (at(ins.a, j) <= ins.imm)) break;
+        case TOp::CheckEQ: CMI_EACH(res.feasible[j] = res.feasible[j] && almost_equal(at(ins.a, j), ins.imm)) break;
+        case TOp::Objective: CMI_EACH(res.obj[j] = at(ins.a, j)) break;
+        default: break;
+      }
+    }
+    cur = nxt;
+  }
+#undef CMI_EACH
+  return res;
+}
+
 // ---------------------------------------------------------------------------
 // An item's parent box and slot data, fetched lane-parallel (one memory round
 // trip for the whole wave) and handed out by index with v_readlane: the
@@ -629,21 +736,31 @@ __device__ __forceinline__ double atomic_load_f64(const double* p)
 }
 
 /// Wave 0 of one slice of tile lt, after its groups' verdicts are combined
-/// into `feasible` (and the slice that holds the objective group has stored
-/// the tile's objective values): folds the infeasible lanes into the tile's
-/// mask and counts the slice. Returns true in the last slice of the tile to
+/// into `feasible` (N per lane: one per point, or the lane's one region) and
+/// the slice that holds the objective group has stored the tile's objective
+/// values: folds the infeasible lanes of each point j into the tile's mask j
+/// and counts the slice. Returns true in the last slice of the tile to
 /// finish, with `feasible` then the verdict of every group; that slice clears
-/// the tile's mask and count for the next launch. Nothing waits on another
+/// the tile's masks and count for the next launch. Nothing waits on another
 /// workgroup, so the slices may run in any order.
-__device__ __forceinline__ bool join_slices(const Item& it, std::uint32_t lt, std::uint32_t n_slices, bool& feasible,
+template<int N>
+__device__ __forceinline__ bool join_slices(const Item& it, std::uint32_t lt, std::uint32_t n_slices, bool* feasible,
                                             unsigned lane)
 {
-  std::uint64_t const bad = __ballot(!feasible);
+  std::uint64_t bad[N];
+#pragma unroll
+  for (int j = 0; j < N; ++j) {
+    bad[j] = __ballot(!feasible[j]);
+  }
+  std::uint64_t* const masks = it.tile_infeasible + static_cast<std::size_t>(lt) * kSliceMasks;
   std::uint32_t ticket = 0;
   __threadfence();
   if (lane == 0) {
-    if (bad != 0) {
-      atomicOr(reinterpret_cast<unsigned long long*>(it.tile_infeasible + lt), static_cast<unsigned long long>(bad));
+#pragma unroll
+    for (int j = 0; j < N; ++j) {
+      if (bad[j] != 0) {
+        atomicOr(reinterpret_cast<unsigned long long*>(masks + j), static_cast<unsigned long long>(bad[j]));
+      }
     }
     __threadfence();
     ticket = atomicAdd(it.tile_slices_done + lt, 1u);
@@ -653,11 +770,16 @@ __device__ __forceinline__ bool join_slices(const Item& it, std::uint32_t lt, st
     return false;
   }
   __threadfence();
-  std::uint64_t const mask =
-      __hip_atomic_load(it.tile_infeasible + lt, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  feasible = ((mask >> lane) & 1u) == 0u;
+#pragma unroll
+  for (int j = 0; j < N; ++j) {
+    std::uint64_t const mask = __hip_atomic_load(masks + j, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    feasible[j] = ((mask >> lane) & 1u) == 0u;
+  }
   if (lane == 0) {
-    it.tile_infeasible[lt] = 0;
+#pragma unroll
+    for (int j = 0; j < N; ++j) {
+      masks[j] = 0;
+    }
     it.tile_slices_done[lt] = 0;
   }
   return true;
@@ -671,7 +793,13 @@ __device__ __forceinline__ bool join_slices(const Item& it, std::uint32_t lt, st
 // verdicts and the objective meet in LDS after the lds_slots slots, and wave
 // 0 writes the results. Groups share only the (read-only) variable slots, so
 // the values are those of the whole tape run by one wave.
-template<int M, bool Ext>
+//
+// N (sampler and enumerator only): points per lane. A tile is then 64 x N
+// elements, point j of lane l being element tile * 64N + j * 64 + l, and
+// every word of the tape is fetched and dispatched once for N points (its
+// arithmetic per point is unchanged, so each point's values are those of
+// N = 1).
+template<int M, bool Ext, int N>
 __global__ __launch_bounds__(kTile * kMaxGroups) void k_items(const Item* __restrict__ items,
                                                                const std::uint32_t* __restrict__ tile_item,
                                                                std::uint32_t lds_slots)
@@ -686,12 +814,7 @@ __global__ __launch_bounds__(kTile * kMaxGroups) void k_items(const Item* __rest
   std::uint32_t const local = blockIdx.x - it.tile_begin;
   std::uint32_t const lt = n_slices == 1 ? local : local / n_slices;
   std::uint32_t const slice = local - lt * n_slices;
-  std::uint64_t const e = static_cast<std::uint64_t>(lt) * kTile + lane;
-  bool const active = e < it.n_elems;
-  std::uint32_t const ee = static_cast<std::uint32_t>(active ? e : it.n_elems - 1);
   std::uint32_t const S = it.samples;
-  std::uint32_t const region = (M == static_cast<int>(Mode::Sample)) ? ee / S : ee;
-  std::uint32_t const sample = (M == static_cast<int>(Mode::Sample)) ? ee - region * S : 0;
   // this wave's function group, its code range, and the objective's group
   std::uint32_t const group = slice * n_waves + wave;
   std::uint32_t g_begin = 0, g_len = 0, objective = 0;
@@ -711,11 +834,16 @@ __global__ __launch_bounds__(kTile * kMaxGroups) void k_items(const Item* __rest
   const Instr* const code = it.code + g_begin;
   std::uint32_t const n_vars = it.n_vars;
   constexpr bool kInterval = M == static_cast<int>(Mode::Bound) || M == static_cast<int>(Mode::Aggregate);
-  // per-group verdicts [kMaxGroups][64] and the objective [64], after the slots
+  // per-group verdicts [kMaxGroups][N][64] and the objective [N][64], after
+  // the slots (N points of 8 bytes, or one interval of 16, per lane and slot)
   auto* const feas_g = reinterpret_cast<std::uint32_t*>(reinterpret_cast<unsigned char*>(lds)
-                                                        + std::size_t {lds_slots} * kTile * (kInterval ? 16u : 8u));
+                                                        + std::size_t {lds_slots} * kTile * (kInterval ? 16u : 8u * N));
 
   if constexpr (kInterval) {
+    static_assert(N == 1, "interval items take one region per lane");
+    std::uint64_t const e = static_cast<std::uint64_t>(lt) * kTile + lane;
+    bool const active = e < it.n_elems;
+    std::uint32_t const region = static_cast<std::uint32_t>(active ? e : it.n_elems - 1);
     double2* const Sl = lds;
     if (wave == 0) {
       LaneSlots const ls = lane_slots(it, lane);
@@ -767,11 +895,11 @@ __global__ __launch_bounds__(kTile * kMaxGroups) void k_items(const Item* __rest
       obj = {o.x, o.y};
     }
     if (n_slices > 1) {
-      auto* const tobj = reinterpret_cast<double2*>(it.tile_obj) + static_cast<std::size_t>(lt) * kTile;
+      auto* const tobj = reinterpret_cast<double2*>(it.tile_obj + static_cast<std::size_t>(lt) * kSliceObjDoubles);
       if (objective / n_waves == slice) {
         tobj[lane] = to_d2(obj);
       }
-      if (!join_slices(it, lt, n_slices, res.feasible, lane)) {
+      if (!join_slices<1>(it, lt, n_slices, &res.feasible, lane)) {
         return;
       }
       obj = {atomic_load_f64(&tobj[lane].x), atomic_load_f64(&tobj[lane].y)};
@@ -812,6 +940,19 @@ __global__ __launch_bounds__(kTile * kMaxGroups) void k_items(const Item* __rest
   } else {
     double* const P = reinterpret_cast<double*>(lds);
     bool const sampling = M == static_cast<int>(Mode::Sample);
+    // point j of this lane: element e[j], its region and sample
+    std::uint64_t e[N];
+    bool active[N];
+    std::uint32_t region[N], sample[N];
+#pragma unroll
+    for (int j = 0; j < N; ++j) {
+      e[j] = (static_cast<std::uint64_t>(lt) * N + j) * kTile + lane;
+      active[j] = e[j] < it.n_elems;
+      std::uint32_t const ee = static_cast<std::uint32_t>(active[j] ? e[j] : it.n_elems - 1);
+      region[j] = sampling ? ee / S : ee;
+      sample[j] = sampling ? ee - region[j] * S : 0;
+    }
+    auto at = [&](std::uint32_t v, int j) -> double& { return P[(v * N + j) * kTile + lane]; };
     if (wave == 0) {
       LaneSlots const ls = lane_slots(it, lane);
       std::uint32_t skind = 0;  // lane j: kind of the variable of slot j
@@ -830,59 +971,106 @@ __global__ __launch_bounds__(kTile * kMaxGroups) void k_items(const Item* __rest
         for (std::uint32_t k = 0; k < cnt; ++k) {
           std::uint32_t const v = v0 + k;
           double const l = lane_f64(lo, k);
-          P[v * kTile + lane] =
-              sampling ? draw(lane_u32(vk, k) != 0, l, lane_f64(hi, k), v, sample, region, it.salt) : l;
+          double const h = lane_f64(hi, k);
+          bool const integral = lane_u32(vk, k) != 0;
+#pragma unroll
+          for (int j = 0; j < N; ++j) {
+            at(v, j) = sampling ? draw(integral, l, h, v, sample[j], region[j], it.salt) : l;
+          }
         }
       }
-      std::uint32_t idx = region;
-      for (std::uint32_t j = 0; j < it.slot_count; ++j) {
-        std::uint32_t const w = lane_u32(ls.width, j);
-        std::uint32_t const part = idx % w;
-        idx /= w;
-        std::uint32_t const vid = lane_u32(ls.var, j);
-        Bounds const b = slot_bounds(static_cast<std::uint8_t>(lane_u32(ls.kind, j)),
-                                     {lane_f64(ls.plo, j), lane_f64(ls.phi, j)}, part, w);
-        P[vid * kTile + lane] =
-            sampling ? draw(lane_u32(skind, j) != 0, b.lo, b.hi, vid, sample, region, it.salt) : b.lo;
+      std::uint32_t idx[N];
+#pragma unroll
+      for (int j = 0; j < N; ++j) {
+        idx[j] = region[j];
+      }
+      for (std::uint32_t k = 0; k < it.slot_count; ++k) {
+        std::uint32_t const w = lane_u32(ls.width, k);
+        std::uint32_t const vid = lane_u32(ls.var, k);
+        auto const kind = static_cast<std::uint8_t>(lane_u32(ls.kind, k));
+        bool const integral = lane_u32(skind, k) != 0;
+        Bounds const pb {lane_f64(ls.plo, k), lane_f64(ls.phi, k)};
+#pragma unroll
+        for (int j = 0; j < N; ++j) {
+          std::uint32_t const part = idx[j] % w;
+          idx[j] /= w;
+          Bounds const b = slot_bounds(kind, pb, part, w);
+          at(vid, j) = sampling ? draw(integral, b.lo, b.hi, vid, sample[j], region[j], it.salt) : b.lo;
+        }
       }
     }
     if (n_waves > 1) {
       __syncthreads();
     }
-    PointResult res = run_point<Ext>(code, g_len, P, lane, nullptr, 0, 0);
+    double obj[N];
+    bool feasible[N];
+    if constexpr (N == 1) {
+      PointResult const r = run_point<Ext>(code, g_len, P, lane, nullptr, 0, 0);
+      obj[0] = r.obj;
+      feasible[0] = r.feasible;
+    } else {
+      PointsResult<Ext, N> const r = run_points<Ext, N>(code, g_len, P, lane);
+#pragma unroll
+      for (int j = 0; j < N; ++j) {
+        obj[j] = r.obj[j];
+        feasible[j] = r.feasible[j];
+      }
+    }
     if (n_waves > 1) {
-      auto* const obj_s = reinterpret_cast<double*>(feas_g + kMaxGroups * kTile);
-      feas_g[wave * kTile + lane] = res.feasible ? 1u : 0u;
-      if (group == objective) {
-        obj_s[lane] = res.obj;
+      auto* const obj_s = reinterpret_cast<double*>(feas_g + kMaxGroups * kTile * N);
+#pragma unroll
+      for (int j = 0; j < N; ++j) {
+        feas_g[(wave * N + j) * kTile + lane] = feasible[j] ? 1u : 0u;
+        if (group == objective) {
+          obj_s[j * kTile + lane] = obj[j];
+        }
       }
       __syncthreads();
       if (wave != 0) {
         return;
       }
-      bool f = true;
-      for (unsigned g = 0; g < n_waves; ++g) {
-        f = f && feas_g[g * kTile + lane] != 0u;
+#pragma unroll
+      for (int j = 0; j < N; ++j) {
+        bool f = true;
+        for (unsigned g = 0; g < n_waves; ++g) {
+          f = f && feas_g[(g * N + j) * kTile + lane] != 0u;
+        }
+        feasible[j] = f;
+        obj[j] = obj_s[j * kTile + lane];
       }
-      res.feasible = f;
-      res.obj = obj_s[lane];
     }
     if (n_slices > 1) {
-      double* const tobj = it.tile_obj + static_cast<std::size_t>(lt) * kTile;
+      double* const tobj = it.tile_obj + static_cast<std::size_t>(lt) * kSliceObjDoubles;
       if (objective / n_waves == slice) {
-        tobj[lane] = res.obj;
+#pragma unroll
+        for (int j = 0; j < N; ++j) {
+          tobj[j * kTile + lane] = obj[j];
+        }
       }
-      if (!join_slices(it, lt, n_slices, res.feasible, lane)) {
+      if (!join_slices<N>(it, lt, n_slices, feasible, lane)) {
         return;
       }
-      res.obj = atomic_load_f64(&tobj[lane]);
+#pragma unroll
+      for (int j = 0; j < N; ++j) {
+        obj[j] = atomic_load_f64(&tobj[j * kTile + lane]);
+      }
     }
-    // a point needs a finite objective (log(0) = -inf is no incumbent)
-    double v = (active && res.feasible) ? res.obj : INFINITY;
-    if (v != v || v == -INFINITY) {
-      v = INFINITY;
+    // this lane's best point (lowest element on ties), then the tile's
+    double v = INFINITY;
+    std::int64_t i = INT64_MAX;
+#pragma unroll
+    for (int j = 0; j < N; ++j) {
+      // a point needs a finite objective (log(0) = -inf is no incumbent)
+      double vj = (active[j] && feasible[j]) ? obj[j] : INFINITY;
+      if (vj != vj || vj == -INFINITY) {
+        vj = INFINITY;
+      }
+      std::int64_t const ij = active[j] ? static_cast<std::int64_t>(e[j]) : INT64_MAX;
+      if (vj < v || (vj == v && ij < i)) {
+        v = vj;
+        i = ij;
+      }
     }
-    std::int64_t i = active ? static_cast<std::int64_t>(e) : INT64_MAX;
     wave_argmin(v, i);
     if (lane == 0) {
       it.tile_min[lt] = v;
@@ -989,28 +1177,38 @@ template<int M, bool Ext>
 void launch_items_as(const Item* items, const std::uint32_t* tile_item, std::uint32_t n_tiles, std::size_t bytes,
                      std::uint32_t lds_slots, std::uint32_t waves, hipStream_t stream)
 {
-  static bool once = (allow_large_lds(k_items<M, Ext>), true);
+  static bool once = (allow_large_lds(k_items<M, Ext, 1>), true);
   (void)once;
-  hipLaunchKernelGGL((k_items<M, Ext>), dim3(n_tiles), dim3(kTile * waves), bytes, stream, items, tile_item,
+  hipLaunchKernelGGL((k_items<M, Ext, 1>), dim3(n_tiles), dim3(kTile * waves), bytes, stream, items, tile_item,
                      lds_slots);
 }
 
-template<int M, bool Ext>
+template<int M, bool Ext, int N>
 const void* items_kernel_fn()
 {
-  static bool once = (allow_large_lds(k_items<M, Ext>), true);
+  static bool once = (allow_large_lds(k_items<M, Ext, N>), true);
   (void)once;
-  return reinterpret_cast<const void*>(&k_items<M, Ext>);
+  return reinterpret_cast<const void*>(&k_items<M, Ext, N>);
+}
+
+template<int M, bool Ext>
+const void* point_items_kernel(std::uint32_t points_per_lane)
+{
+  switch (points_per_lane) {
+    case 2: return items_kernel_fn<M, Ext, 2>();
+    case 4: return items_kernel_fn<M, Ext, 4>();
+    default: return items_kernel_fn<M, Ext, 1>();
+  }
 }
 
 template<bool Ext>
-const void* items_kernel_mode(Mode mode)
+const void* items_kernel_mode(Mode mode, std::uint32_t points_per_lane)
 {
   switch (mode) {
-    case Mode::Bound: return items_kernel_fn<0, Ext>();
-    case Mode::Sample: return items_kernel_fn<1, Ext>();
-    case Mode::Enumerate: return items_kernel_fn<2, Ext>();
-    case Mode::Aggregate: return items_kernel_fn<3, Ext>();
+    case Mode::Bound: return items_kernel_fn<0, Ext, 1>();
+    case Mode::Sample: return point_items_kernel<1, Ext>(points_per_lane);
+    case Mode::Enumerate: return point_items_kernel<2, Ext>(points_per_lane);
+    case Mode::Aggregate: return items_kernel_fn<3, Ext, 1>();
   }
   return nullptr;
 }
@@ -1059,11 +1257,12 @@ void launch_points_as(const Instr* code, std::uint32_t n_instr, std::uint32_t n_
 
 }  // namespace
 
-std::size_t items_lds_bytes(Mode mode, std::uint32_t lds_slots, std::uint32_t waves)
+std::size_t items_lds_bytes(Mode mode, std::uint32_t lds_slots, std::uint32_t waves, std::uint32_t points_per_lane)
 {
   bool const interval = mode == Mode::Bound || mode == Mode::Aggregate;
-  std::size_t const slots = static_cast<std::size_t>(lds_slots) * kTile * (interval ? 16 : 8);
-  return waves > 1 ? slots + kMaxGroups * kTile * 4 + kTile * 16 : slots;
+  std::size_t const n = interval ? 1 : std::max<std::uint32_t>(points_per_lane, 1);
+  std::size_t const slots = static_cast<std::size_t>(lds_slots) * kTile * (interval ? 16 : 8) * n;
+  return waves > 1 ? slots + (kMaxGroups * kTile * 4 + kTile * 16) * n : slots;
 }
 
 void launch_items(Mode mode, bool extended, const Item* items, const std::uint32_t* tile_item, std::uint32_t n_tiles,
@@ -1082,14 +1281,17 @@ void launch_items(Mode mode, bool extended, const Item* items, const std::uint32
 }
 
 ItemsLaunch items_launch(Mode mode, bool extended, std::uint32_t n_tiles, std::uint32_t lds_slots,
-                         std::uint32_t waves)
+                         std::uint32_t waves, std::uint32_t points_per_lane)
 {
   waves = std::clamp<std::uint32_t>(waves, 1, kMaxGroups);
+  if (points_per_lane != 2 && points_per_lane != 4) {
+    points_per_lane = 1;
+  }
   ItemsLaunch l;
-  l.func = extended ? items_kernel_mode<true>(mode) : items_kernel_mode<false>(mode);
+  l.func = extended ? items_kernel_mode<true>(mode, points_per_lane) : items_kernel_mode<false>(mode, points_per_lane);
   l.grid = n_tiles;
   l.block = kTile * waves;
-  l.lds_bytes = items_lds_bytes(mode, lds_slots, waves);
+  l.lds_bytes = items_lds_bytes(mode, lds_slots, waves, points_per_lane);
   return l;
 }
 

@@ -229,3 +229,32 @@ def test_step_forms_equal_plain(ctx, monkeypatch, env):
     for p, d in zip(probs, outs):
         s, _ = cmi.solve(ctx, p, **opts)
         same_outcome(d, s)
+
+
+@pytest.mark.parametrize("env", [{"CMI_POINTS_PER_LANE": "2"}, {"CMI_POINTS_PER_LANE": "4"},
+                                 {"CMI_POINTS_PER_LANE": "4", "CMI_ITEM_WAVES": "0"},
+                                 {"CMI_POINTS_PER_LANE": "2", "CMI_ITEM_WAVES": "4"},
+                                 {"CMI_POINTS_PER_LANE": "4", "CMI_ITEM_WAVES": "2,1,8"},
+                                 {"CMI_POINTS_PER_LANE": "2", "CMI_STEP_GRAPH": "1"}])
+def test_points_per_lane_equal_plain(ctx, monkeypatch, env):
+    """Sampler and enumerator launches with 2 or 4 points per lane give the
+    plain solves' results and the same incumbents: continuous QCQPs (sampled
+    regions in tiles of 64 x N elements, ragged last tiles), mixed-integer ones
+    (lattice enumeration), random DAGs of the extended op set, with the long
+    tapes' groups in one workgroup or sliced over several (per-launch waves),
+    alone and inside a step graph."""
+    for k, v in env.items():
+        monkeypatch.setenv(k, v)
+    probs = [cmi.EngineProblem.qcqp(70 + i, n=n, m=n // 2, n_int=n_int, int_lb=0.0, int_ub=4.0, cont_lb=-5.0,
+                                    cont_ub=5.0, pair_density=0.25)
+             for i, (n, n_int) in enumerate([(5, 0), (12, 3), (30, 0), (8, 8), (40, 6)])]
+    probs += [cmi.EngineProblem.from_problem(P.random_ext_dag(s, n_vars=3 + s % 3, n_ops=30, n_con=4))
+              for s in range(3)]
+    opts = dict(partition_num=3, enumerate_cap=3, max_cycle_size=3, sample_points=24, iter_limit=15)
+    outs, bests = cmi.solve_batch(ctx, probs, **opts)
+    for k in env:
+        monkeypatch.delenv(k)
+    for p, d, bd in zip(probs, outs, bests):
+        s, bs = cmi.solve(ctx, p, **opts)
+        same_outcome(d, s)
+        assert np.array_equal(bd.view(np.uint64), bs.view(np.uint64))
