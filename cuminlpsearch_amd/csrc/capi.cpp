@@ -555,7 +555,7 @@ cmi::tape::Tape grouped_for_boxes(const cmi::model::Problem<double>& p, size_t g
   std::uint32_t k = groups == 0 ? cmi::tape::auto_groups(whole) : static_cast<std::uint32_t>(std::min<size_t>(groups, 8));
   k = std::clamp<std::uint32_t>(k, 1, std::min<std::uint32_t>(cmi::dev::kMaxGroups, 1 + whole.n_constraints));
   for (; k > 1; --k) {
-    cmi::tape::Tape g = cmi::tape::compile_groups(p, k);
+    cmi::tape::Tape g = cmi::tape::compile_groups(p, k, {}, 0, &whole);
     if (cmi::dev::boxes_lds_bytes(g.n_slots) <= cmi::jit::kLdsBytes) {
       return g;
     }
@@ -653,7 +653,8 @@ cmi_status cmi_tape_compile_words(const cmi_problem* p, size_t k, void* words, s
     auto const whole = cmi::tape::compile(p->problem);
     auto const tp = k <= 1 ? whole
                            : cmi::tape::compile_groups(p->problem,
-                                                       static_cast<std::uint32_t>(std::min<size_t>(k, 1u << 16)));
+                                                       static_cast<std::uint32_t>(std::min<size_t>(k, 1u << 16)),
+                                                       {}, 0, &whole);
     out->n_instr = whole.code.size();
     out->n_slots = whole.n_slots;
     out->n_vars = whole.n_vars;

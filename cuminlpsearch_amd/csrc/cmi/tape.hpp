@@ -335,67 +335,97 @@ private:
   bool is_const(std::size_t id) const { return nodes_[id].op == Op::Const; }
   double cval(std::size_t id) const { return nodes_[id].payload.constant; }
 
+  /// Whether node `nd` (reachable, not a Const) is an op the device tape
+  /// evaluates; throws the error the compile reports for it otherwise.
+  static void check_op(const model::DAGNode<double>& nd, const std::vector<model::DAGNode<double>>& nodes)
+  {
+    switch (nd.op) {
+      case Op::Var:
+      case Op::Add:
+      case Op::Sub:
+      case Op::Mul:
+      case Op::Div:
+      case Op::Min:
+      case Op::Max:
+      case Op::Pow:
+        return;
+      case Op::Sqr:
+      case Op::Neg:
+      case Op::Sqrt:
+      case Op::Exp:
+      case Op::Log:
+      case Op::Sin:
+      case Op::Cos:
+      case Op::Tanh:
+      case Op::Abs:
+      case Op::PowN:
+        if (nodes[nd.in[0]].op == Op::Const) {
+          throw InvalidDAG("unary op applied directly to an Op::Const");
+        }
+        return;
+      default:
+        throw InvalidConfiguration(std::string("Op::") + model::op_name(nd.op)
+                                   + " is not implemented by the device tape evaluator");
+    }
+  }
+
+  /// Reachability and use counts of the selected functions' cone. The cone
+  /// is found from the roots down, so a function group's compile costs its
+  /// cone, not the whole DAG; the counts do not depend on the visiting
+  /// order. A DAG with an op the tape cannot evaluate reports the one with
+  /// the highest node index, as a scan from the top would.
   void analyse()
   {
     std::size_t const n = nodes_.size();
-    reach_.assign(n, false);
+    reach_.assign(n, 0);
     parents_.assign(n, 0);
+    leaf_.assign(n, 0);
+    cone_.clear();
     for (std::size_t k : selected_) {
-      reach_[roots_[k]] = true;
-      ++parents_[roots_[k]];
+      std::size_t const r = roots_[k];
+      ++parents_[r];
+      if (!reach_[r]) {
+        reach_[r] = 1;
+        cone_.push_back(r);
+      }
     }
-    for (std::size_t i = n; i-- > 0;) {
-      if (!reach_[i]) {
-        continue;
-      }
-      auto const& nd = nodes_[i];
-      for (std::size_t in : nd.in) {
-        reach_[in] = true;
+    for (std::size_t q = 0; q < cone_.size(); ++q) {
+      for (std::size_t in : nodes_[cone_[q]].in) {
         ++parents_[in];
+        if (!reach_[in]) {
+          reach_[in] = 1;
+          cone_.push_back(in);
+        }
       }
+    }
+    bool valid = true;
+    for (std::size_t i : cone_) {
+      auto const& nd = nodes_[i];
       if (nd.op == Op::Const) {
         continue;
       }
       ++reachable_;
-      switch (nd.op) {
-        case Op::Var:
-        case Op::Add:
-        case Op::Sub:
-        case Op::Mul:
-        case Op::Div:
-        case Op::Min:
-        case Op::Max:
-        case Op::Pow:
-          break;
-        case Op::Sqr:
-        case Op::Neg:
-        case Op::Sqrt:
-        case Op::Exp:
-        case Op::Log:
-        case Op::Sin:
-        case Op::Cos:
-        case Op::Tanh:
-        case Op::Abs:
-        case Op::PowN:
-          if (is_const(nd.in[0])) {
-            throw InvalidDAG("unary op applied directly to an Op::Const");
-          }
-          break;
-        default:
-          throw InvalidConfiguration(std::string("Op::") + model::op_name(nd.op)
-                                     + " is not implemented by the device tape evaluator");
+      try {
+        check_op(nd, nodes_);
+      } catch (...) {
+        valid = false;
       }
-    }
-    leaf_.assign(n, false);
-    for (std::size_t i = 0; i < n; ++i) {
-      if (!reach_[i] || is_var(i) || is_const(i)) {
+      if (nd.op == Op::Var) {
         continue;
       }
       bool all = true;
-      for (std::size_t in : nodes_[i].in) {
+      for (std::size_t in : nd.in) {
         all = all && (is_var(in) || is_const(in));
       }
-      leaf_[i] = all && opt_.rematerialise_leaves && cheap(nodes_[i].op);
+      leaf_[i] = all && opt_.rematerialise_leaves && cheap(nd.op);
+    }
+    if (!valid) {
+      std::sort(cone_.begin(), cone_.end(), std::greater<>());
+      for (std::size_t i : cone_) {
+        if (nodes_[i].op != Op::Const) {
+          check_op(nodes_[i], nodes_);
+        }
+      }
     }
   }
 
@@ -802,8 +832,9 @@ private:
   CompileOptions opt_;
   std::vector<std::size_t> roots_;
   std::vector<std::size_t> selected_;
-  std::vector<bool> reach_;
-  std::vector<bool> leaf_;
+  std::vector<std::uint8_t> reach_;
+  std::vector<std::uint8_t> leaf_;
+  std::vector<std::size_t> cone_;  // reachable nodes, highest index first
   std::vector<std::uint32_t> parents_;
   std::vector<std::int64_t> cache_;
   std::vector<VInstr> code_;
@@ -880,9 +911,10 @@ inline std::vector<std::size_t> root_work(const model::Problem<double>& p)
 /// so the waves of one workgroup share the variable slots and nothing else.
 /// slot_period (0 = n_groups): group g's temporaries take range g mod
 /// slot_period, for code whose groups run slot_period (or a divisor of it) to
-/// a workgroup.
+/// a workgroup. `whole`: the problem's tape compiled whole with the same
+/// options, when the caller has it (its counts are the grouped tape's).
 inline Tape compile_groups(const model::Problem<double>& p, std::uint32_t n_groups, const CompileOptions& opt = {},
-                           std::uint32_t slot_period = 0)
+                           std::uint32_t slot_period = 0, const Tape* whole_tape = nullptr)
 {
   std::size_t const n_roots = 1 + p.constraints.size();
   n_groups = std::max<std::uint32_t>(1, std::min<std::uint32_t>(n_groups, static_cast<std::uint32_t>(n_roots)));
@@ -926,8 +958,9 @@ inline Tape compile_groups(const model::Problem<double>& p, std::uint32_t n_grou
   for (auto const& q : parts) {
     t.temps_per_group = std::max(t.temps_per_group, q.n_slots - q.n_vars);
   }
-  Tape const whole = detail::Compiler(p, opt).run();
-  t.reachable_nodes = whole.reachable_nodes;
+  Tape const whole = whole_tape != nullptr ? Tape {} : detail::Compiler(p, opt).run();
+  Tape const& w = whole_tape != nullptr ? *whole_tape : whole;
+  t.reachable_nodes = w.reachable_nodes;
   std::uint32_t const period = slot_period == 0 ? n_groups : std::min(slot_period, n_groups);
   for (std::uint32_t g = 0; g < n_groups; ++g) {
     t.group_begin.push_back(static_cast<std::uint32_t>(t.code.size()));
@@ -951,8 +984,8 @@ inline Tape compile_groups(const model::Problem<double>& p, std::uint32_t n_grou
     }
     t.node_ops += parts[g].node_ops;
   }
-  t.flops = whole.flops;  // algorithmic work: every DAG operation once
-  t.extended = whole.extended;
+  t.flops = w.flops;  // algorithmic work: every DAG operation once
+  t.extended = w.extended;
   t.group_begin.push_back(static_cast<std::uint32_t>(t.code.size()));
   std::uint64_t const slots = std::uint64_t(t.n_vars) + std::uint64_t(period) * t.temps_per_group;
   if (slots > 65535) {

@@ -123,44 +123,88 @@ DeviceTape::DeviceTape(std::shared_ptr<Context> ctx, const model::Problem<double
     throw ResourceExhausted("tape needs " + std::to_string(tape_.n_slots) + " slots per lane; LDS holds "
                             + std::to_string(ctx_->lds_interval_slots()) + " interval slots per 64-lane workgroup");
   }
-  upload_code(tape_.code, code_);
-  std::vector<std::uint8_t> kinds(problem.var_kinds.size());
-  for (std::size_t i = 0; i < kinds.size(); ++i) {
-    kinds[i] = static_cast<std::uint8_t>(problem.var_kinds[i]);
-  }
-  kinds_.reset(std::max<std::size_t>(kinds.size(), 1));
-  check(hipMemcpy(kinds_.get(), kinds.data(), kinds.size(), hipMemcpyHostToDevice), "hipMemcpy(var kinds)");
-  set_box_groups(problem, box_groups);
+  tape::Tape grouped = grouping(problem, box_groups);
   std::uint32_t const n_roots = 1 + tape_.n_constraints;
   if (tape_.code.size() >= kWideMinWords && n_roots > dev::kMaxGroups) {
     auto const by_size = static_cast<std::uint32_t>((tape_.code.size() + kWideWordsPerGroup - 1) / kWideWordsPerGroup);
     std::uint32_t const k = std::min({n_roots, kMaxWideGroups, std::max(by_size, dev::kMaxGroups + 1)});
-    wide_ = tape::compile_groups(problem, k, opt_, kWidePeriod);
-    upload_code(wide_.code, wide_code_);
-    upload_words(wide_.group_begin, wide_begin_);
+    wide_ = tape::compile_groups(problem, k, opt_, kWidePeriod, &tape_);
   }
+  // Every array the kernels read, in one allocation filled by one copy (a
+  // batch builds hundreds of tapes at once, and each allocation and copy
+  // takes the runtime's locks). Codes carry one trailing Nop word.
+  std::size_t at = 0;
+  auto place = [&at](std::size_t bytes) {
+    std::size_t const off = (at + 255) / 256 * 256;
+    at = off + bytes;
+    return off;
+  };
+  auto code_bytes = [](const tape::Tape& t) { return (t.code.size() + 1) * sizeof(tape::Instr); };
+  bool const wide = wide_.n_groups > 1;
+  std::size_t const o_code = place(code_bytes(tape_));
+  std::size_t const o_grouped = place(code_bytes(grouped));
+  std::size_t const o_grouped_begin = place(grouped.group_begin.size() * sizeof(std::uint32_t));
+  std::size_t const o_wide = wide ? place(code_bytes(wide_)) : 0;
+  std::size_t const o_wide_begin = wide ? place(wide_.group_begin.size() * sizeof(std::uint32_t)) : 0;
+  std::size_t const o_kinds = place(std::max<std::size_t>(problem.var_kinds.size(), 1));
+  std::vector<unsigned char> host(at, 0);  // zero bytes are the Nop word
+  auto put = [&host](std::size_t off, const void* src, std::size_t bytes) {
+    if (bytes > 0) {
+      std::memcpy(host.data() + off, src, bytes);
+    }
+  };
+  put(o_code, tape_.code.data(), tape_.code.size() * sizeof(tape::Instr));
+  put(o_grouped, grouped.code.data(), grouped.code.size() * sizeof(tape::Instr));
+  put(o_grouped_begin, grouped.group_begin.data(), grouped.group_begin.size() * sizeof(std::uint32_t));
+  if (wide) {
+    put(o_wide, wide_.code.data(), wide_.code.size() * sizeof(tape::Instr));
+    put(o_wide_begin, wide_.group_begin.data(), wide_.group_begin.size() * sizeof(std::uint32_t));
+  }
+  for (std::size_t i = 0; i < problem.var_kinds.size(); ++i) {
+    host[o_kinds + i] = static_cast<std::uint8_t>(problem.var_kinds[i]);
+  }
+  image_.reset(at);
+  check(hipMemcpy(image_.get(), host.data(), at, hipMemcpyHostToDevice), "hipMemcpy(tape image)");
+  auto* const base = image_.as<unsigned char>();
+  code_d_ = reinterpret_cast<const tape::Instr*>(base + o_code);
+  kinds_d_ = base + o_kinds;
+  if (wide) {
+    wide_code_d_ = reinterpret_cast<const tape::Instr*>(base + o_wide);
+    wide_begin_d_ = reinterpret_cast<const std::uint32_t*>(base + o_wide_begin);
+  }
+  install_groups(std::move(grouped), reinterpret_cast<const tape::Instr*>(base + o_grouped),
+                 reinterpret_cast<const std::uint32_t*>(base + o_grouped_begin));
 }
 
-void DeviceTape::set_box_groups(const model::Problem<double>& problem, std::uint32_t k)
+tape::Tape DeviceTape::grouping(const model::Problem<double>& problem, std::uint32_t k) const
 {
   if (k == 0) {
     k = tape::auto_groups(tape_);
   }
   std::uint32_t const n_roots = 1 + tape_.n_constraints;
   k = std::clamp<std::uint32_t>(k, 1, std::min<std::uint32_t>(dev::kMaxGroups, n_roots));
-  tape::Tape g = tape_;
   for (; k > 1; --k) {
-    g = tape::compile_groups(problem, k, opt_);
+    tape::Tape g = tape::compile_groups(problem, k, opt_, 0, &tape_);
     if (dev::boxes_lds_bytes(g.n_slots) <= dev::lds_limit_bytes()) {
-      break;
+      return g;
     }
   }
-  if (k == 1) {
-    g = tape_;
-  }
+  return tape_;
+}
+
+void DeviceTape::set_box_groups(const model::Problem<double>& problem, std::uint32_t k)
+{
   ctx_->activate();
+  tape::Tape g = grouping(problem, k);
   upload_code(g.code, grouped_code_);
   upload_words(g.group_begin, grouped_begin_);
+  install_groups(std::move(g), grouped_code_.as<tape::Instr>(), grouped_begin_.as<std::uint32_t>());
+}
+
+void DeviceTape::install_groups(tape::Tape g, const tape::Instr* code, const std::uint32_t* begin)
+{
+  grouped_code_d_ = code;
+  grouped_begin_d_ = begin;
   grouped_ = std::move(g);
   drop_compiled();  // the compiled kernel follows the grouping; compile again when needed
   if (engine_ == BoxEngine::Compiled) {
@@ -181,11 +225,11 @@ void DeviceTape::item_code(dev::Item& it, bool grouped) const
   it.slice_begin = nullptr;
   it.slice_objective = 0;
   if (grouped) {
-    it.code = grouped_code_.as<tape::Instr>();
+    it.code = grouped_code_d_;
     it.groups = groups_;
     it.n_instr = static_cast<std::uint32_t>(grouped_.code.size());
   } else {
-    it.code = code_.as<tape::Instr>();
+    it.code = code_d_;
     it.groups = {};
     it.groups.n_groups = 1;
     it.groups.objective_group = 0;
@@ -201,8 +245,8 @@ DeviceTape::ItemPlan DeviceTape::item_plan(dev::Mode mode, std::uint32_t waves) 
   std::size_t const lds = ctx_->lds_bytes();
   if (waves == 0) {
     bool const grouped = items_grouped_fit(mode);
-    p.code = grouped ? grouped_code_.as<tape::Instr>() : code_.as<tape::Instr>();
-    p.begin = grouped ? grouped_begin_.as<std::uint32_t>() : nullptr;
+    p.code = grouped ? grouped_code_d_ : code_d_;
+    p.begin = grouped ? grouped_begin_d_ : nullptr;
     p.n_instr = static_cast<std::uint32_t>(grouped ? grouped_.code.size() : tape_.code.size());
     p.groups = grouped ? grouped_.n_groups : 1;
     p.objective = grouped ? grouped_.objective_group : 0;
@@ -214,8 +258,8 @@ DeviceTape::ItemPlan DeviceTape::item_plan(dev::Mode mode, std::uint32_t waves) 
   waves = std::clamp<std::uint32_t>(waves, 1, dev::kMaxGroups);
   if (wide_.n_groups > 1 && kWidePeriod % waves == 0
       && dev::items_lds_bytes(mode, wide_.n_slots, std::max<std::uint32_t>(waves, 2)) <= lds) {
-    p.code = wide_code_.as<tape::Instr>();
-    p.begin = wide_begin_.as<std::uint32_t>();
+    p.code = wide_code_d_;
+    p.begin = wide_begin_d_;
     p.n_instr = static_cast<std::uint32_t>(wide_.code.size());
     p.groups = wide_.n_groups;
     p.objective = wide_.objective_group;
@@ -224,15 +268,15 @@ DeviceTape::ItemPlan DeviceTape::item_plan(dev::Mode mode, std::uint32_t waves) 
   } else if (grouped_.n_groups > 1
              && dev::items_lds_bytes(mode, grouped_.n_slots, std::max<std::uint32_t>(std::min(waves, grouped_.n_groups), 2))
                  <= lds) {
-    p.code = grouped_code_.as<tape::Instr>();
-    p.begin = grouped_begin_.as<std::uint32_t>();
+    p.code = grouped_code_d_;
+    p.begin = grouped_begin_d_;
     p.n_instr = static_cast<std::uint32_t>(grouped_.code.size());
     p.groups = grouped_.n_groups;
     p.objective = grouped_.objective_group;
     p.slots = grouped_.n_slots;
     p.code_kind = 1;
   } else {
-    p.code = code_.as<tape::Instr>();
+    p.code = code_d_;
     p.n_instr = static_cast<std::uint32_t>(tape_.code.size());
     p.slots = tape_.n_slots;
     p.code_kind = 0;
@@ -245,7 +289,7 @@ DeviceTape::ItemPlan DeviceTape::item_plan(dev::Mode mode, std::uint32_t waves) 
 DeviceTape::ItemPlan DeviceTape::item_plan_whole() const
 {
   ItemPlan p;
-  p.code = code_.as<tape::Instr>();
+  p.code = code_d_;
   p.n_instr = static_cast<std::uint32_t>(tape_.code.size());
   p.slots = tape_.n_slots;
   return p;
@@ -473,7 +517,7 @@ void DeviceTape::eval_boxes_device(const double* lb, const double* ub, std::size
     c->launch(lb, ub, ld, n, feasible, obj_lb, obj_ub, roots, stream);
     return;
   }
-  dev::launch_boxes(grouped_code_.as<tape::Instr>(), grouped_.extended, groups_, grouped_.n_slots, n_vars(), lb, ub, ld, n, feasible,
+  dev::launch_boxes(grouped_code_d_, grouped_.extended, groups_, grouped_.n_slots, n_vars(), lb, ub, ld, n, feasible,
                     obj_lb, obj_ub, roots, stream);
   check(hipGetLastError(), "launch(k_boxes)");
 }

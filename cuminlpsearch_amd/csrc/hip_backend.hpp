@@ -259,8 +259,8 @@ public:
   std::vector<BoxCandidate> autotune(const model::Problem<double>& problem, std::size_t n);
 
   const tape::Tape& tape() const { return tape_; }
-  const tape::Instr* code() const { return code_.as<tape::Instr>(); }
-  const std::uint8_t* var_kinds() const { return kinds_.as<std::uint8_t>(); }
+  const tape::Instr* code() const { return code_d_; }
+  const std::uint8_t* var_kinds() const { return kinds_d_; }
   std::uint32_t n_vars() const { return tape_.n_vars; }
   std::uint32_t n_instr() const { return static_cast<std::uint32_t>(tape_.code.size()); }
   std::uint32_t n_slots() const { return tape_.n_slots; }
@@ -365,13 +365,23 @@ private:
 
   std::shared_ptr<const jit::CompiledBoxes> compile_now(bool roots) const;
   void drop_compiled() const;
-  DeviceBuffer code_;
-  DeviceBuffer grouped_code_;
-  DeviceBuffer grouped_begin_;  // grouped_.group_begin on the device
+  /// The tape in `k` explicit-box function groups (0: tape::auto_groups),
+  /// as set_box_groups describes.
+  tape::Tape grouping(const model::Problem<double>& problem, std::uint32_t k) const;
+  /// Make `g` (its code and group table already on the device) the grouped code.
+  void install_groups(tape::Tape g, const tape::Instr* code, const std::uint32_t* begin);
+  // The code, the first grouping, the wide grouping and the variable kinds,
+  // in one allocation (image_); a later regrouping has buffers of its own.
+  DeviceBuffer image_;
+  const tape::Instr* code_d_ = nullptr;
+  const tape::Instr* grouped_code_d_ = nullptr;
+  const std::uint32_t* grouped_begin_d_ = nullptr;  // grouped_.group_begin on the device
+  const tape::Instr* wide_code_d_ = nullptr;
+  const std::uint32_t* wide_begin_d_ = nullptr;
+  const std::uint8_t* kinds_d_ = nullptr;
+  DeviceBuffer grouped_code_;   // set_box_groups' code
+  DeviceBuffer grouped_begin_;
   tape::Tape wide_;             // n_groups 0: none
-  DeviceBuffer wide_code_;
-  DeviceBuffer wide_begin_;
-  DeviceBuffer kinds_;
 };
 
 /// Bytes a role of each kind needs for n_regions (affine in n_regions).

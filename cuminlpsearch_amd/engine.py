@@ -27,6 +27,7 @@ class EngineProblem:
 
     def __init__(self, handle: C.c_void_p):
         self._h = handle
+        self._info = None
         _c.track(self)
 
     @classmethod
@@ -48,9 +49,14 @@ class EngineProblem:
         return cls(h)
 
     def info(self) -> _c.ProblemInfo:
-        out = _c.ProblemInfo()
-        _c.check(_c.lib().cmi_problem_info_get(self._h, C.byref(out)))
-        return out
+        """Node, variable and constraint counts (a problem does not change
+        once created, so the first answer is kept: the reachable-node count
+        sweeps the DAG)."""
+        if self._info is None:
+            out = _c.ProblemInfo()
+            _c.check(_c.lib().cmi_problem_info_get(self._h, C.byref(out)))
+            self._info = out
+        return self._info
 
     def arrays(self) -> ProblemArrays:
         inf = self.info()
