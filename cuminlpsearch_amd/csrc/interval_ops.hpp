@@ -224,18 +224,45 @@ __device__ __forceinline__ Iv iv_const(double c, int t) { return {neg_plus0_s(c,
 
 // [x] * [y] from the four endpoint products (xl = -xnl, yl = -ynl):
 // -lb = max RU(-xi*yj), ub = max RU(xi*yj).
-__device__ __forceinline__ Iv iv_mul(Iv x, Iv y, int t)
+//
+// Each bound is one asm statement of 7 instructions (4 products, then the max
+// tree max(max(p0, p1), max(p2, p3))). The compiler's hazard check assumes that
+// any inline asm may write a 16-bit half of a VGPR and puts an s_nop between
+// two asm statements when the second reads what the first wrote; with one
+// statement per instruction that was one s_nop per 4 to 5 instructions of the
+// product chains. Inside a statement nothing is inserted, and none is needed:
+// a dependent full-width FP64 VALU instruction waits in hardware (the compiler
+// emits such pairs back to back in its own code). The bounds stay separate
+// statements so that a bound nothing reads is still deleted.
+__device__ __forceinline__ double mul_nl(Iv x, Iv y, int t)
 {
-  double const l0 = ru_mul_na(x.nl, y.nl, t);  // -(xl*yl)
-  double const l1 = ru_mul(x.nl, y.ub, t);     // -(xl*yu)
-  double const l2 = ru_mul(x.ub, y.nl, t);     // -(xu*yl)
-  double const l3 = ru_mul_na(x.ub, y.ub, t);  // -(xu*yu)
-  double const u0 = ru_mul(x.nl, y.nl, t);     // xl*yl
-  double const u1 = ru_mul_na(x.nl, y.ub, t);  // xl*yu
-  double const u2 = ru_mul_na(x.ub, y.nl, t);  // xu*yl
-  double const u3 = ru_mul(x.ub, y.ub, t);     // xu*yu
-  return {vmax(vmax(l0, l1, t), vmax(l2, l3, t), t), vmax(vmax(u0, u1, t), vmax(u2, u3, t), t)};
+  double r, p0, p1, p2, p3;
+  asm("v_mul_f64 %1, -%6, %7\n\t"  // -(xl*yl)
+      "v_mul_f64 %2, %6, %8\n\t"   // -(xl*yu)
+      "v_mul_f64 %3, %9, %7\n\t"   // -(xu*yl)
+      "v_mul_f64 %4, -%9, %8\n\t"  // -(xu*yu)
+      "v_max_f64 %1, %1, %2\n\t"
+      "v_max_f64 %3, %3, %4\n\t"
+      "v_max_f64 %0, %1, %3 ; %5"
+      : "=v"(r), "=&v"(p0), "=&v"(p1), "=&v"(p2), "=&v"(p3)
+      : "s"(t), "v"(x.nl), "v"(y.nl), "v"(y.ub), "v"(x.ub));
+  return r;
 }
+__device__ __forceinline__ double mul_ub(Iv x, Iv y, int t)
+{
+  double r, p0, p1, p2, p3;
+  asm("v_mul_f64 %1, %6, %7\n\t"   // xl*yl
+      "v_mul_f64 %2, -%6, %8\n\t"  // xl*yu
+      "v_mul_f64 %3, -%9, %7\n\t"  // xu*yl
+      "v_mul_f64 %4, %9, %8\n\t"   // xu*yu
+      "v_max_f64 %1, %1, %2\n\t"
+      "v_max_f64 %3, %3, %4\n\t"
+      "v_max_f64 %0, %1, %3 ; %5"
+      : "=v"(r), "=&v"(p0), "=&v"(p1), "=&v"(p2), "=&v"(p3)
+      : "s"(t), "v"(x.nl), "v"(y.nl), "v"(y.ub), "v"(x.ub));
+  return r;
+}
+__device__ __forceinline__ Iv iv_mul(Iv x, Iv y, int t) { return {mul_nl(x, y, t), mul_ub(x, y, t)}; }
 
 // [a] * c for c > 0: [RD(al*c), RU(au*c)]; -lb = RU(-al*c) = RU(anl*c)
 __device__ __forceinline__ Iv iv_mul_pos(Iv a, double c, int t) { return {ru_mul_s(a.nl, c, t), ru_mul_s(a.ub, c, t)}; }
@@ -260,11 +287,20 @@ __device__ __forceinline__ Iv iv_mul_imm(Iv a, double c, int t)
 
 // [a]^2: s = max(lb, min(ub, 0)) is the endpoint nearest zero, or 0 when the
 // interval straddles zero; -lb = RU(-s*s); ub = RU(m*m), m = max(|lb|, |ub|).
+// One asm statement per bound, as for the product.
 __device__ __forceinline__ Iv iv_sqr(Iv a, int t)
 {
-  double const s = vmax_na(a.nl, vmin0(a.ub, t), t);
-  double const m = vmax_abs(a.nl, a.ub, t);
-  return {ru_mul_na(s, s, t), ru_mul(m, m, t)};
+  double l, u, s, m;
+  asm("v_min_f64 %1, %4, 0\n\t"    // min(ub, 0)
+      "v_max_f64 %1, -%3, %1\n\t"  // max(lb, min(ub, 0))
+      "v_mul_f64 %0, -%1, %1 ; %2"
+      : "=v"(l), "=&v"(s)
+      : "s"(t), "v"(a.nl), "v"(a.ub));
+  asm("v_max_f64 %1, |%3|, |%4|\n\t"
+      "v_mul_f64 %0, %1, %1 ; %2"
+      : "=v"(u), "=&v"(m)
+      : "s"(t), "v"(a.nl), "v"(a.ub));
+  return {l, u};
 }
 
 /// [lb, ub] with a zero bound reported as +0 (x + 0 is exact otherwise). A
